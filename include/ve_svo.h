@@ -1,0 +1,225 @@
+/*
+ * ve_svo.h — C ABI of the MI355X sparse-voxel-octree ray pipeline.
+ *
+ * Drop-in for VoxelEngine3.0's GL pass API: the octree build
+ * (OctreeCPU::insert* + Octree::set), the per-frame wavefront sequence
+ * Renderer::runWavefrontFrame (generate_primary, normal, schedule, rank,
+ * threshold, emit, trace/shade bounces, resolve) and the buffer readbacks.
+ *
+ * Conventions
+ *   - Every int-returning call returns VE_OK (0) or a negative VE_E* code; the
+ *     message is available from ve_last_error(ctx) (ctx may be NULL for calls
+ *     that take no context).  This mirrors the reference's `success` flag that
+ *     Renderer::run returns (src/renderer/renderer.cpp:977-984,711).
+ *   - The library owns all device memory; the caller owns host buffers.
+ *   - One ve_ctx = one HIP stream on one device.  A context is not
+ *     thread-safe (like a GL context); separate contexts may run concurrently.
+ *   - Node words are the reference's exact 32-bit format
+ *     (src/renderer/octree.hpp:16-37): internal = bit0 1 | count<<1 (3 bits)
+ *     | next<<4 (28 bits); leaf = bit0 0 | material<<1 (7 bits) | normal<<8
+ *     (24 bits, always 0); empty = 0.
+ */
+#ifndef VE_SVO_H
+#define VE_SVO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VE_OK            0
+#define VE_EINVAL      (-1)  /* bad argument (null pointer, zero size, depth out of range) */
+#define VE_ENOMEM      (-2)  /* host or device allocation failed */
+#define VE_EDEVICE     (-3)  /* HIP runtime error (message names the call) */
+#define VE_ECAPACITY   (-4)  /* node array exceeds the 28-bit `next` field or the word limit */
+#define VE_EFORMAT     (-5)  /* malformed node array (a `next` outside the array) */
+#define VE_ESTATE      (-6)  /* call out of order (e.g. trace before emit) */
+#define VE_EBUDGET     (-7)  /* admission refused: device-memory budget exceeded */
+#define VE_ENOTFOUND   (-8)  /* unknown handle / buffer id */
+
+#define VE_MAX_DEPTH   16
+
+/* One OctreeCPU::insert(position, material) call (src/octree_cpu.cpp:40-64).
+ * A list of these, in call order, is what insertBox / insertSphere /
+ * insertFunction expand to; later inserts at the same position win. */
+typedef struct ve_insert {
+    uint32_t x, y, z, material;
+} ve_insert;
+
+typedef struct ve_octree_info {
+    uint32_t depth;
+    uint32_t size;        /* used words (Octree::size, src/renderer/octree.cpp:131,299-300) */
+    uint32_t capacity;    /* power-of-two word capacity (Octree::resizeDataIfNeeded, octree.cpp:95-119) */
+    uint32_t num_voxels;  /* leaves with material != 0 (Octree::numVoxels) */
+} ve_octree_info;
+
+/* ---- octree_cpu build -------------------------------------------------------
+ * Replaces OctreeCPU::insert (src/octree_cpu.cpp:40-64) followed by
+ * Octree::set / linearizeNode (src/renderer/octree.cpp:127-217).  Produces the
+ * node words in the reference's exact DFS block order.  `max_words` is the
+ * driver's texture-buffer limit (GL_MAX_TEXTURE_BUFFER_SIZE, octree.cpp:102-105);
+ * a tree larger than it (or than the 28-bit next field) fails with
+ * VE_ECAPACITY instead of the reference's out-of-bounds write.
+ * *words_out has `capacity` entries (zero past `size`); free it with ve_free. */
+int ve_octree_build(const ve_insert *ops, size_t n_ops, uint32_t depth, uint32_t max_words,
+                    uint32_t **words_out, ve_octree_info *info_out);
+void ve_free(void *p);
+
+/* ---- renderer ---------------------------------------------------------------- */
+typedef struct ve_ctx ve_ctx;
+typedef struct ve_scene ve_scene;
+
+/* Renderer construction parameters (Renderer::Renderer, renderer.cpp:20-169).
+ * hash_width is the driver's GL_MAX_TEXTURE_SIZE, which sizes the l/n/rBuffer
+ * hash images (renderer.cpp:27-40); 16384 on Mesa llvmpipe. */
+typedef struct ve_renderer_config {
+    int      device;          /* HIP device ordinal */
+    uint32_t max_width;       /* largest framebuffer this context will render */
+    uint32_t max_height;
+    uint32_t hash_width;      /* W_l: width of lBuffer / nBuffer / rBuffer (0 = 16384) */
+    uint32_t max_bounces;     /* MAX_BOUNCES of wavefront.glsl:108 (0 = 8) */
+} ve_renderer_config;
+
+int  ve_create(ve_ctx **ctx, const ve_renderer_config *cfg);
+void ve_destroy(ve_ctx *ctx);
+const char *ve_last_error(const ve_ctx *ctx);
+
+/* Octree::set upload (octree.cpp:127-166): copies `n_words` node words to the
+ * device.  The words are validated (every `next` block must lie inside the
+ * array) so a malformed array fails with VE_EFORMAT instead of faulting. */
+int ve_scene_upload(ve_ctx *ctx, const uint32_t *words, uint32_t n_words, uint32_t depth,
+                    ve_scene **scene);
+int ve_scene_release(ve_ctx *ctx, ve_scene *scene);
+
+/* 64-byte std140 CameraUniform (camera.cpp:39-50): position, cameraPlane
+ * (forward), cameraPlaneRight, cameraPlaneUp, each a vec4 with w = 0. */
+int ve_set_camera(ve_ctx *ctx, const float ubo[16]);
+/* Camera::UpdateUBO's arithmetic on the host (camera.cpp:43-44), FP32. */
+void ve_camera_ubo(const float position[3], const float direction[3], float fov_deg,
+                   float aspect, float ubo_out[16]);
+
+/* Material UBO (material.hpp:17-25): 64-byte std140 entries, entry 0 unused,
+ * up to 127 entries (shade.comp:84-90). */
+int ve_set_materials(ve_ctx *ctx, const void *std140_entries, uint32_t n_entries);
+
+/* Sky: six RGB8 faces of `edge`² texels sampled with bilinear filtering on a
+ * miss (skybox.cpp:87-122).  A constant sky is edge = 1. */
+int ve_set_sky(ve_ctx *ctx, const uint8_t *const faces_rgb8[6], int edge);
+
+/* Per-frame knobs = core::FrameConfig (src/renderer/core.hpp:83-158). */
+typedef struct ve_frame_config {
+    int32_t  primary_controlchecks;   /* 70 */
+    int32_t  bounce_controlchecks;    /* 100 */
+    int32_t  reconstruction_radius;   /* 10 */
+    int32_t  normal_precision;        /* 6 */
+    int32_t  disable_stale_reset;     /* 0 */
+    uint32_t update_time;
+    int32_t  shading_budget;          /* 131072, clamped to [1, 196608] */
+    int32_t  sample_cap_direct;       /* 16 */
+    int32_t  sample_cap_indirect;     /* 64 */
+    float    weight_pixels;           /* 0.1 */
+    float    weight_variance;         /* 0.9 */
+    float    pixels_norm;             /* 100 */
+    float    variance_norm;           /* 10000 */
+    float    top_fraction;            /* 0.30 */
+    float    mid_fraction;            /* 0.40 */
+    int32_t  min_samples;             /* 1 */
+    int32_t  restir_enabled;          /* 1 */
+    int32_t  restir_m;                /* 4 */
+    int32_t  restir_temporal;         /* 1 */
+    int32_t  restir_spatial;          /* 1 */
+    int32_t  restir_spatial_neighbors;/* 1 */
+    float    restir_spatial_radius;   /* 2.75 */
+    int32_t  restir_max_m;            /* 24 */
+    float    firefly_k;               /* 1.0 */
+    float    firefly_floor;           /* 255 */
+    int32_t  render_type;             /* 0 = DEFAULT (core::RenderType) */
+} ve_frame_config;
+
+void ve_frame_config_defaults(ve_frame_config *fc);
+
+/* Per-frame inputs the reference takes from the wall clock
+ * (renderer.cpp:283,293 and voxelengine.cpp:200,211), made explicit. */
+typedef struct ve_frame_inputs {
+    uint32_t width, height;
+    uint32_t frame_idx;    /* Renderer::frameID */
+    uint32_t frame_seed;   /* schedule scramble seed */
+    uint32_t time;         /* glTime */
+} ve_frame_inputs;
+
+/* Renderer::runWavefrontFrame (renderer.cpp:282-649): the whole frame,
+ * enqueued on the context's stream with no host round-trips. */
+int ve_frame(ve_ctx *ctx, ve_scene *scene, const ve_frame_config *fc, const ve_frame_inputs *in);
+
+/* One glDispatchCompute equivalent, for per-pass parity tests. */
+typedef enum ve_pass_id {
+    VE_PASS_PRIMARY = 0,   /* generate_primary.comp */
+    VE_PASS_NORMAL,        /* normal.comp */
+    VE_PASS_RESET,         /* per-frame meta/claim reset (renderer.cpp:343-375) */
+    VE_PASS_SCHEDULE,      /* schedule.comp */
+    VE_PASS_RANK,          /* rank.comp */
+    VE_PASS_THRESHOLD,     /* threshold.comp */
+    VE_PASS_EMIT,          /* emit.comp (whole shade list, one ray batch) */
+    VE_PASS_TRACE,         /* trace.comp over the live rays */
+    VE_PASS_SHADE,         /* shade.comp over the live rays */
+    VE_PASS_RESOLVE,       /* resolve.comp */
+    VE_PASS_COUNT
+} ve_pass_id;
+
+int ve_pass(ve_ctx *ctx, ve_scene *scene, ve_pass_id pass, const ve_frame_config *fc,
+            const ve_frame_inputs *in);
+
+/* Buffers readable / writable by id (device copies, synchronous). */
+typedef enum ve_buffer_id {
+    VE_BUF_GBUFFER = 0,    /* uint32 x4 per pixel: checks, material, 0, voxelKey+1 */
+    VE_BUF_GBUFFER_POS,    /* uint32 per pixel: packed half-coordinates */
+    VE_BUF_NBUFFER,        /* uint32 [hash_width][3*32]  (bucket-major) */
+    VE_BUF_LBUFFER,        /* uint32 [hash_width][12*32] (bucket-major) */
+    VE_BUF_RBUFFER,        /* uint32 [hash_width][8*16]  (bucket-major) */
+    VE_BUF_CANDIDATES,     /* uint32 x4 per candidate: vid, pix, claim_h, claim_p */
+    VE_BUF_SHADE_LIST,     /* uint32 x2 per entry: vid, pix */
+    VE_BUF_RAYS,           /* 96-byte Ray records of the current batch */
+    VE_BUF_OUTPUT,         /* float x4 per pixel (resolve RGBA32F) */
+    VE_BUF_COUNTERS,       /* uint32 x16: see VE_CNT_* */
+    VE_BUF_RANK_STATS,     /* uint32 x264: RankStats (wavefront.glsl:243-255) */
+    VE_BUF_SAMPLE_COUNTS,  /* uint32 [16384][64] per-voxel pixel counts (claim-map layout) */
+    VE_BUF_CLAIM_MAP,      /* uint32 [16384][64] */
+    VE_BUF_EMISSIVES,      /* 32-byte EmissiveVoxel records */
+    VE_BUF_COUNT
+} ve_buffer_id;
+
+/* VE_BUF_COUNTERS layout */
+#define VE_CNT_CANDIDATES     0   /* candidate_count */
+#define VE_CNT_SHADE          1   /* shade_count (after budget) */
+#define VE_CNT_EMISSIVES      2   /* emissive_count */
+#define VE_CNT_LIVE           3   /* live rays entering the next trace */
+#define VE_CNT_RAYS_TRACED    4   /* rays traced this frame by trace (all bounces) */
+#define VE_CNT_SHADOW_RAYS    5   /* NEE shadow rays this frame */
+#define VE_CNT_PRIMARY_RAYS   6   /* W*H of the last primary pass */
+#define VE_CNT_BOUNCES_RUN    7   /* trace/shade iterations with live rays */
+
+int ve_read(ve_ctx *ctx, ve_buffer_id id, void *dst, size_t bytes);
+int ve_write(ve_ctx *ctx, ve_buffer_id id, const void *src, size_t bytes);
+int ve_buffer_size(ve_ctx *ctx, ve_buffer_id id, size_t *bytes);
+
+/* Block until the context's stream is idle. */
+int ve_sync(ve_ctx *ctx);
+
+/* Optional per-pass timing: when enabled, ve_frame records HIP events around
+ * each kernel; ve_timings returns milliseconds per pass of the last frame. */
+int ve_enable_timing(ve_ctx *ctx, int enable);
+int ve_timings(ve_ctx *ctx, float ms_out[VE_PASS_COUNT]);
+
+/* Node-read instrumentation: when enabled, the primary and trace kernels add
+ * the number of node words they fetch to a device counter (4 bytes each is the
+ * algorithmic traffic of the traversal).  Off by default: it adds atomics. */
+int ve_enable_node_stats(ve_ctx *ctx, int enable);
+int ve_node_stats(ve_ctx *ctx, uint64_t *primary_reads, uint64_t *trace_reads,
+                  uint64_t *primary_steps, uint64_t *trace_steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

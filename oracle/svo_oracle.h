@@ -1,0 +1,96 @@
+/*
+ * svo_oracle.h — CPU restatement of VoxelEngine3.0's SVO ray pipeline.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker for the HIP product path:
+ * only tests/, the smoke check and the benchmark's CPU-baseline leg load it.
+ * It follows the reference algorithm step by step (each function cites the
+ * reference file:line it restates) in plain sequential C, with the FP32
+ * evaluation order that Mesa llvmpipe uses for the reference's GLSL:
+ * IEEE division and sqrt, no multiply-add contraction, dot(a,b) evaluated as
+ * a.x*b.x + (a.y*b.y + a.z*b.z), normalize(v) = v * (1/sqrt(dot(v,v))).
+ * Sequential execution reproduces llvmpipe's single-threaded dispatch order
+ * (work groups in linear dispatch order, invocations in SIMD-8 vectors), which
+ * fixes the order of every atomic append.
+ */
+#ifndef SVO_ORACLE_H
+#define SVO_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct so_insert { uint32_t x, y, z, material; } so_insert;
+
+/* OctreeCPU::insert + Octree::set/linearizeNode with a real pointer tree
+ * (src/octree_cpu.cpp:40-64, src/renderer/octree.cpp:127-217).  Returns 0 or
+ * -1 on allocation failure / -4 when the tree exceeds max_words.
+ * *words has *capacity entries; free with so_free. */
+int so_octree_build(const so_insert *ops, size_t n, uint32_t depth, uint32_t max_words,
+                    uint32_t **words, uint32_t *capacity, uint32_t *size, uint32_t *num_voxels);
+void so_free(void *p);
+
+/* Camera::UpdateUBO (src/renderer/camera.cpp:39-50). */
+void so_camera_ubo(const float pos[3], const float dir[3], float fov_deg, float aspect, float ubo[16]);
+
+typedef struct so_hit {
+    uint32_t hit, id, material, pos[3], size;
+} so_hit;
+
+/* RaycastCounted / Raycast (src/shd/internal.glsl:297-400).  `checks` gets q;
+ * node_reads (optional) accumulates the node words fetched. */
+so_hit so_raycast(const uint32_t *words, uint32_t n_words, uint32_t depth, const float origin[3],
+                  const float dir[3], const float inv_dir[3], uint32_t controlchecks,
+                  uint32_t *checks, uint64_t *node_reads);
+
+uint32_t so_voxel_key(const uint32_t pos[3]);   /* internal.glsl:263-291 */
+
+/* Select an alternative FP lowering (tests only: they show the golden
+ * fixtures reject each one).  0 restores llvmpipe's lowering. */
+void so_set_fp_variant(int variant);
+void so_fp_probe(int op, const float *in, float *out, size_t n);
+
+/* generate_primary.comp:main over a W x H frame (generate_primary.comp:38-85).
+ * gbuf: 4 uint32 per pixel, pos: 1 uint32 per pixel, row-major y*W+x. */
+void so_generate_primary(const uint32_t *words, uint32_t n_words, uint32_t depth, const float ubo[16],
+                         uint32_t W, uint32_t H, uint32_t controlchecks, uint32_t *gbuf, uint32_t *pos,
+                         uint64_t *node_reads, uint64_t *steps, int threads);
+
+/* ---- full wavefront frame --------------------------------------------------- */
+typedef struct so_frame_config {
+    int32_t primary_controlchecks, bounce_controlchecks, reconstruction_radius, normal_precision;
+    int32_t disable_stale_reset;
+    uint32_t update_time;
+    int32_t shading_budget, sample_cap_direct, sample_cap_indirect;
+    float weight_pixels, weight_variance, pixels_norm, variance_norm, top_fraction, mid_fraction;
+    int32_t min_samples;
+    int32_t restir_enabled, restir_m, restir_temporal, restir_spatial, restir_spatial_neighbors;
+    float restir_spatial_radius;
+    int32_t restir_max_m;
+    float firefly_k, firefly_floor;
+    int32_t render_type;
+} so_frame_config;
+
+typedef struct so_state so_state;
+
+/* Persistent renderer state: l/n/rBuffer, RankStats thresholds, emissive
+ * registry — everything that survives across frames (renderer.cpp:114-161). */
+so_state *so_state_create(uint32_t hash_width, uint32_t max_bounces, uint32_t W, uint32_t H);
+void so_state_destroy(so_state *s);
+void so_state_set_materials(so_state *s, const void *std140, uint32_t n);
+void so_state_set_sky(so_state *s, const float rgb[3]);
+
+/* Renderer::runWavefrontFrame (renderer.cpp:282-649) for one frame. */
+int so_frame(so_state *s, const uint32_t *words, uint32_t n_words, uint32_t depth, const float ubo[16],
+             const so_frame_config *fc, uint32_t frame_idx, uint32_t frame_seed, uint32_t time);
+
+/* Buffer access by the same ids as the product's ve_buffer_id. */
+size_t so_buffer(so_state *s, int id, void **ptr);
+void so_counters(so_state *s, uint32_t out[16]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,125 @@
+"""Write the golden fixtures of the reference's own GLSL passes.
+
+Runs the reference shaders (read from $VE_REF_SHADERS, default
+/root/reference/src/shd) on Mesa llvmpipe through oracle/glsl, single-threaded
+(LP_NUM_THREADS=1) so the dispatch order is deterministic, on the seeded
+scenes of voxelengine3.0_amd/scenes.py, and stores inputs' digests plus the
+pass outputs as compressed .npz files next to this script.
+
+    LP_NUM_THREADS=1 python tests/golden/make_golden.py [primary] [frame]
+
+Only the outputs are stored; the shaders are never copied.
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from ve_loader import load_package  # noqa: E402
+
+ve = load_package()
+from voxelengine_amd import scenes  # noqa: E402
+from oracle.glsl import replay as R  # noqa: E402
+
+
+def words_digest(words, size):
+    return hashlib.sha256(np.ascontiguousarray(words[:size], np.uint32).tobytes()).hexdigest()
+
+
+# Primary-pass views of the depth-7 terrain: the config's camera plus views
+# that exercise the traversal's corner cases (origin inside the grid,
+# axis-aligned directions with infinite inverse components, grazing rays).
+PRIMARY_VIEWS = {
+    "small_cfg": dict(eye=None, W=256, H=256, cc=70),
+    "inside": dict(eye=((20.0, 70.0, 30.0), (100.0, 40.0, 90.0)), W=96, H=64, cc=70),
+    "axis_z": dict(eye=((64.0, 60.0, -10.0), (64.0, 60.0, 200.0)), W=65, H=65, cc=70),
+    "axis_y": dict(eye=((64.0, 200.0, 64.0), (64.0, 0.0, 64.0001)), W=64, H=64, cc=120),
+    "grazing": dict(eye=((-30.0, 50.0, -30.0), (158.0, 44.0, 158.0)), W=128, H=48, cc=120),
+}
+
+
+def make_primary():
+    depth, ops = scenes.scene_small(1)
+    words, info = ve.octree_build(ops, depth)
+    out = {"depth": depth, "size": info.size, "capacity": info.capacity, "num_voxels": info.num_voxels,
+           "words_sha256": words_digest(words, info.size), "views": {}}
+    arrays = {}
+    for name, v in PRIMARY_VIEWS.items():
+        if v["eye"] is None:
+            eye, d, fov = scenes.camera_small()
+        else:
+            eye, d = scenes.look_at(*v["eye"])
+            fov = 70.0
+        W, H = v["W"], v["H"]
+        ubo = ve.camera_ubo(eye, d, fov, W / H)
+        pl = R.GLPipeline(words, depth, W, H)
+        pl.set_camera(ubo)
+        pl.primary(v["cc"])
+        g, p = pl.read_gbuffer()
+        arrays[name + "_gbuf"] = g
+        arrays[name + "_pos"] = p
+        arrays[name + "_ubo"] = ubo
+        out["views"][name] = dict(W=W, H=H, controlchecks=v["cc"], hits=int((g[..., 3] != 0).sum()))
+        print(name, out["views"][name], flush=True)
+    np.savez_compressed(os.path.join(HERE, "primary_small.npz"), **arrays)
+    with open(os.path.join(HERE, "primary_small.json"), "w") as fh:
+        json.dump(out, fh, indent=1, sort_keys=True)
+
+
+# llvmpipe probe shaders for the FP primitives the passes use; each writes
+# one result per invocation from three float inputs.
+FP_PROBES = {
+    "dot": "vec3 v = vec3(a, b, c); o[i] = dot(v, v);",
+    "normalize": "vec3 n = normalize(vec3(a, b, c)); o[3u*i] = n.x; o[3u*i+1u] = n.y; o[3u*i+2u] = n.z;",
+    "div": "o[i] = a / b;",
+    "mad": "o[i] = a * b + c;",
+    "min": "o[i] = min(a, b);",
+    "max": "o[i] = max(a, b);",
+}
+
+
+def make_fp():
+    from oracle.glsl import gl as G
+    rng = np.random.default_rng(2024)
+    n = 4096
+    inputs = rng.uniform(-8.0, 8.0, (n, 3)).astype(np.float32)
+    inputs[:, 1] = np.where(np.abs(inputs[:, 1]) < 1e-3, 1.0, inputs[:, 1])
+    special = np.array([[np.nan, 1, 0], [1, np.nan, 0], [0.0, -0.0, 0], [-0.0, 0.0, 0], [np.inf, np.nan, 0],
+                        [-np.inf, 1, 0], [3, 3, 0], [1e-30, 1e30, 0]], np.float32)
+    inputs[: len(special)] = special
+    arrays = {"inputs": inputs}
+    for name, body in FP_PROBES.items():
+        nout = 3 if name == "normalize" else 1
+        src = ("#version 430 core\nlayout(local_size_x=64) in;\n"
+               "layout(std430,binding=0) buffer I{float inp[];};\nlayout(std430,binding=1) buffer O{float o[];};\n"
+               "void main(){ uint i = gl_GlobalInvocationID.x; float a = inp[3u*i], b = inp[3u*i+1u], "
+               "c = inp[3u*i+2u]; " + body + " }\n")
+        prog = G.Program(src, "probe_" + name)
+        prog.use()
+        bi = G.Buffer(data=inputs.ravel())
+        bo = G.Buffer(nbytes=n * nout * 4)
+        bi.bind_ssbo(0)
+        bo.bind_ssbo(1)
+        prog.dispatch(n // 64)
+        arrays[name] = bo.read(np.float32).reshape(n, nout) if nout == 3 else bo.read(np.float32)
+        print("probe", name, flush=True)
+    np.savez_compressed(os.path.join(HERE, "fp_probes.npz"), **arrays)
+
+
+if __name__ == "__main__":
+    if os.environ.get("LP_NUM_THREADS") != "1":
+        sys.exit("run with LP_NUM_THREADS=1 (deterministic llvmpipe dispatch order)")
+    if not R.shaders_available():
+        sys.exit("reference shaders not found at %s" % R.SHADER_DIR)
+    what = sys.argv[1:] or ["fp", "primary"]
+    if "fp" in what:
+        make_fp()
+    if "primary" in what:
+        make_primary()

@@ -1,0 +1,339 @@
+"""MI355X-native sparse-voxel-octree ray pipeline (VoxelEngine3.0's hot path).
+
+Python host binding of the C ABI in include/ve_svo.h (libve_svo.so, built
+in-tree by build.py).  The classes mirror the reference's host API so a
+caller can switch by name:
+
+    OctreeCPU  src/octree_cpu.hpp:30-37   insert / insertBox / insertSphere / insertFunction
+    Octree     src/renderer/octree.hpp    set(OctreeCPU)  -> flat node words
+    Camera     src/renderer/camera.hpp    UBO = UpdateUBO()
+    Renderer   src/renderer/renderer.hpp  run(FrameConfig) -> bool
+
+All compute runs in the HIP library; there is no CPU fallback — loading fails
+loudly when the library is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libve_svo.so")
+
+VE_OK = 0
+VE_EINVAL = -1
+VE_ENOMEM = -2
+VE_EDEVICE = -3
+VE_ECAPACITY = -4
+VE_EFORMAT = -5
+VE_ESTATE = -6
+VE_EBUDGET = -7
+VE_ENOTFOUND = -8
+
+PASS = dict(PRIMARY=0, NORMAL=1, RESET=2, SCHEDULE=3, RANK=4, THRESHOLD=5, EMIT=6, TRACE=7, SHADE=8, RESOLVE=9)
+PASS_NAMES = [k for k, _ in sorted(PASS.items(), key=lambda kv: kv[1])]
+BUF = dict(GBUFFER=0, GBUFFER_POS=1, NBUFFER=2, LBUFFER=3, RBUFFER=4, CANDIDATES=5, SHADE_LIST=6, RAYS=7,
+           OUTPUT=8, COUNTERS=9, RANK_STATS=10, SAMPLE_COUNTS=11, CLAIM_MAP=12, EMISSIVES=13)
+CNT = dict(CANDIDATES=0, SHADE=1, EMISSIVES=2, LIVE=3, RAYS_TRACED=4, SHADOW_RAYS=5, PRIMARY_RAYS=6,
+           BOUNCES_RUN=7)
+
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("pad0", "<f4"), ("direction", "<f4", 3), ("pad1", "<f4"),
+                      ("throughput", "<f4", 3), ("primary_vid", "<u4"), ("direct", "<f4", 3), ("rng", "<u4"),
+                      ("indirect", "<f4", 3), ("bounce", "<u4"), ("hit_vid", "<u4"), ("hit_material", "<u4"),
+                      ("pad2", "<u4"), ("pad3", "<u4")])
+assert RAY_DTYPE.itemsize == 96
+
+
+class VeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("ve error %d: %s" % (code, msg))
+        self.code = code
+
+
+class _Insert(C.Structure):
+    _fields_ = [("x", C.c_uint32), ("y", C.c_uint32), ("z", C.c_uint32), ("material", C.c_uint32)]
+
+
+class OctreeInfo(C.Structure):
+    _fields_ = [("depth", C.c_uint32), ("size", C.c_uint32), ("capacity", C.c_uint32), ("num_voxels", C.c_uint32)]
+
+
+class RendererConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("max_width", C.c_uint32), ("max_height", C.c_uint32),
+                ("hash_width", C.c_uint32), ("max_bounces", C.c_uint32)]
+
+
+class FrameConfig(C.Structure):
+    """core::FrameConfig (src/renderer/core.hpp:83-158); defaults applied on construction."""
+    _fields_ = [("primary_controlchecks", C.c_int32), ("bounce_controlchecks", C.c_int32),
+                ("reconstruction_radius", C.c_int32), ("normal_precision", C.c_int32),
+                ("disable_stale_reset", C.c_int32), ("update_time", C.c_uint32), ("shading_budget", C.c_int32),
+                ("sample_cap_direct", C.c_int32), ("sample_cap_indirect", C.c_int32),
+                ("weight_pixels", C.c_float), ("weight_variance", C.c_float), ("pixels_norm", C.c_float),
+                ("variance_norm", C.c_float), ("top_fraction", C.c_float), ("mid_fraction", C.c_float),
+                ("min_samples", C.c_int32), ("restir_enabled", C.c_int32), ("restir_m", C.c_int32),
+                ("restir_temporal", C.c_int32), ("restir_spatial", C.c_int32),
+                ("restir_spatial_neighbors", C.c_int32), ("restir_spatial_radius", C.c_float),
+                ("restir_max_m", C.c_int32), ("firefly_k", C.c_float), ("firefly_floor", C.c_float),
+                ("render_type", C.c_int32)]
+
+    def __init__(self, **kw):
+        super().__init__()
+        lib().ve_frame_config_defaults(C.byref(self))
+        for k, v in kw.items():
+            if not hasattr(self, k):
+                raise AttributeError(k)
+            setattr(self, k, v)
+
+
+class FrameInputs(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("frame_idx", C.c_uint32),
+                ("frame_seed", C.c_uint32), ("time", C.c_uint32)]
+
+
+_lib = None
+
+
+def lib():
+    """The HIP library; raises if it has not been built (no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libve_svo.so is not built: run `python voxelengine3.0_amd/build.py`")
+    L = C.CDLL(LIB_PATH)
+    p, u32, i32, sz = C.c_void_p, C.c_uint32, C.c_int, C.c_size_t
+    sig = {
+        "ve_octree_build": (i32, [p, sz, u32, u32, C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(OctreeInfo)]),
+        "ve_free": (None, [p]),
+        "ve_create": (i32, [C.POINTER(p), C.POINTER(RendererConfig)]),
+        "ve_destroy": (None, [p]),
+        "ve_last_error": (C.c_char_p, [p]),
+        "ve_scene_upload": (i32, [p, p, u32, u32, C.POINTER(p)]),
+        "ve_scene_release": (i32, [p, p]),
+        "ve_set_camera": (i32, [p, p]),
+        "ve_camera_ubo": (None, [p, p, C.c_float, C.c_float, p]),
+        "ve_set_materials": (i32, [p, p, u32]),
+        "ve_set_sky": (i32, [p, p, i32]),
+        "ve_frame_config_defaults": (None, [C.POINTER(FrameConfig)]),
+        "ve_frame": (i32, [p, p, C.POINTER(FrameConfig), C.POINTER(FrameInputs)]),
+        "ve_pass": (i32, [p, p, i32, C.POINTER(FrameConfig), C.POINTER(FrameInputs)]),
+        "ve_read": (i32, [p, i32, p, sz]),
+        "ve_write": (i32, [p, i32, p, sz]),
+        "ve_buffer_size": (i32, [p, i32, C.POINTER(sz)]),
+        "ve_sync": (i32, [p]),
+        "ve_enable_timing": (i32, [p, i32]),
+        "ve_timings": (i32, [p, p]),
+        "ve_enable_node_stats": (i32, [p, i32]),
+        "ve_node_stats": (i32, [p] + [C.POINTER(C.c_uint64)] * 4),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names of the C-ABI entry points declared in include/ve_svo.h."""
+    import re
+    hdr = os.path.join(os.path.dirname(HERE), "include", "ve_svo.h")
+    with open(hdr) as fh:
+        txt = fh.read()
+    return sorted(set(re.findall(r"\b(ve_[a-z0-9_]+)\s*\(", txt)))
+
+
+def _check(rc, ctx=None):
+    if rc != VE_OK:
+        msg = lib().ve_last_error(ctx)
+        raise VeError(rc, msg.decode() if msg else "")
+
+
+# ------------------------------------------------------------- octree build
+def octree_build(ops, depth, max_words=1 << 27):
+    """OctreeCPU inserts (an insert-dtype array in call order) -> Octree::set
+    node words.  Returns (words[capacity] uint32, OctreeInfo)."""
+    ops = np.ascontiguousarray(ops)
+    out = C.POINTER(C.c_uint32)()
+    info = OctreeInfo()
+    _check(lib().ve_octree_build(ops.ctypes.data if len(ops) else None, len(ops), depth, max_words,
+                                 C.byref(out), C.byref(info)))
+    try:
+        words = np.ctypeslib.as_array(out, shape=(info.capacity,)).copy()
+    finally:
+        lib().ve_free(out)
+    return words, info
+
+
+class OctreeCPU:
+    """Host octree edit API (src/octree_cpu.hpp:30-37): records insert calls
+    in order; Octree.set() builds the node array from them."""
+
+    def __init__(self, depth):
+        self.depth = int(depth)
+        self._chunks = []
+
+    def insert(self, x, y, z, material):
+        self._chunks.append(np.array([(x, y, z, material)], dtype=_insert_dtype()))
+
+    def insert_ops(self, ops):
+        self._chunks.append(np.ascontiguousarray(ops, dtype=_insert_dtype()))
+
+    def insertBox(self, lo, hi, material):
+        n = 1 << self.depth
+        hi = [min(int(h), n) for h in hi]
+        X, Y, Z = np.meshgrid(*[np.arange(int(lo[i]), hi[i]) for i in range(3)], indexing="ij")
+        ops = np.empty(X.size, _insert_dtype())
+        ops["x"], ops["y"], ops["z"], ops["material"] = X.ravel(), Y.ravel(), Z.ravel(), material
+        self._chunks.append(ops)
+
+    def insertSphere(self, centre, radius, material):
+        from .scenes import sphere_ops
+        self._chunks.append(sphere_ops(centre, radius, material, 1 << self.depth))
+
+    def insertFunction(self, fn, lo, hi, material):
+        n = 1 << self.depth
+        hi = [min(int(h), n) for h in hi]
+        X, Y, Z = np.meshgrid(*[np.arange(int(lo[i]), hi[i]) for i in range(3)], indexing="ij")
+        c = np.stack([X.ravel() + 0.5, Y.ravel() + 0.5, Z.ravel() + 0.5], 1).astype(np.float32)
+        m = np.asarray(fn(c), bool)
+        ops = np.empty(int(m.sum()), _insert_dtype())
+        ops["x"], ops["y"], ops["z"], ops["material"] = X.ravel()[m], Y.ravel()[m], Z.ravel()[m], material
+        self._chunks.append(ops)
+
+    def ops(self):
+        if not self._chunks:
+            return np.zeros(0, _insert_dtype())
+        return np.concatenate(self._chunks)
+
+
+def _insert_dtype():
+    from .scenes import INSERT_DTYPE
+    return INSERT_DTYPE
+
+
+class Octree:
+    """Flat node array (src/renderer/octree.hpp:14-98)."""
+
+    def __init__(self, depth, max_words=1 << 27):
+        self.depth = int(depth)
+        self.max_words = max_words
+        self.data = np.zeros(8, np.uint32)
+        self.size, self.capacity, self.numVoxels = 8, 8, 0
+
+    def set(self, cpu):
+        self.data, info = octree_build(cpu.ops(), self.depth, self.max_words)
+        self.size, self.capacity, self.numVoxels = info.size, info.capacity, info.num_voxels
+        return self
+
+
+def camera_ubo(position, direction, fov_deg, aspect):
+    """Camera::UpdateUBO (src/renderer/camera.cpp:39-50) -> float32[16]."""
+    pos = np.ascontiguousarray(position, np.float32)
+    d = np.ascontiguousarray(direction, np.float32)
+    out = np.zeros(16, np.float32)
+    lib().ve_camera_ubo(pos.ctypes.data, d.ctypes.data, float(fov_deg), float(aspect), out.ctypes.data)
+    return out
+
+
+class Renderer:
+    """One device context: Renderer(RendererConfig, ...) + run(FrameConfig)
+    (src/renderer/renderer.hpp:11-13), plus per-pass dispatch and readback."""
+
+    def __init__(self, max_width, max_height, hash_width=16384, max_bounces=8, device=0):
+        cfg = RendererConfig(device, max_width, max_height, hash_width, max_bounces)
+        h = C.c_void_p()
+        _check(lib().ve_create(C.byref(h), C.byref(cfg)))
+        self.h = h
+        self.max_width, self.max_height = max_width, max_height
+        self.hash_width = hash_width
+        self.max_bounces = max_bounces
+        self.scenes = []
+
+    def close(self):
+        if self.h:
+            lib().ve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        _check(rc, self.h)
+
+    def upload(self, words, depth, n_words=None):
+        words = np.ascontiguousarray(words, np.uint32)
+        n = len(words) if n_words is None else int(n_words)
+        s = C.c_void_p()
+        self._ck(lib().ve_scene_upload(self.h, words.ctypes.data, n, depth, C.byref(s)))
+        return s
+
+    def release(self, scene):
+        self._ck(lib().ve_scene_release(self.h, scene))
+
+    def set_camera(self, ubo):
+        ubo = np.ascontiguousarray(ubo, np.float32)
+        self._ck(lib().ve_set_camera(self.h, ubo.ctypes.data))
+
+    def set_materials(self, entries):
+        entries = np.ascontiguousarray(entries)
+        self._ck(lib().ve_set_materials(self.h, entries.ctypes.data, len(entries)))
+
+    def set_sky_constant(self, rgb8):
+        face = np.ascontiguousarray(np.asarray(rgb8, np.uint8).reshape(3))
+        arr = (C.c_void_p * 6)(*([face.ctypes.data] * 6))
+        self._ck(lib().ve_set_sky(self.h, arr, 1))
+        self._sky_keep = face
+
+    def run_pass(self, scene, name, fc, inputs):
+        self._ck(lib().ve_pass(self.h, scene, PASS[name], C.byref(fc), C.byref(inputs)))
+
+    def frame(self, scene, fc, inputs):
+        self._ck(lib().ve_frame(self.h, scene, C.byref(fc), C.byref(inputs)))
+
+    def size(self, buf):
+        n = C.c_size_t()
+        self._ck(lib().ve_buffer_size(self.h, BUF[buf], C.byref(n)))
+        return n.value
+
+    def read(self, buf, dtype=np.uint32, nbytes=None):
+        n = self.size(buf) if nbytes is None else int(nbytes)
+        out = np.empty(n, np.uint8)
+        self._ck(lib().ve_read(self.h, BUF[buf], out.ctypes.data, n))
+        return out.view(dtype)
+
+    def write(self, buf, data):
+        data = np.ascontiguousarray(data)
+        self._ck(lib().ve_write(self.h, BUF[buf], data.ctypes.data, data.nbytes))
+
+    def gbuffer(self, W, H):
+        g = self.read("GBUFFER", np.uint32, W * H * 16).reshape(H, W, 4)
+        p = self.read("GBUFFER_POS", np.uint32, W * H * 4).reshape(H, W)
+        return g, p
+
+    def counters(self):
+        return self.read("COUNTERS", np.uint32)
+
+    def sync(self):
+        self._ck(lib().ve_sync(self.h))
+
+    def enable_node_stats(self, on=True):
+        self._ck(lib().ve_enable_node_stats(self.h, 1 if on else 0))
+
+    def node_stats(self):
+        v = [C.c_uint64() for _ in range(4)]
+        self._ck(lib().ve_node_stats(self.h, *[C.byref(x) for x in v]))
+        return dict(primary_reads=v[0].value, trace_reads=v[1].value, primary_steps=v[2].value,
+                    trace_steps=v[3].value)
+
+    def enable_timing(self, on=True):
+        self._ck(lib().ve_enable_timing(self.h, 1 if on else 0))
+
+    def timings(self):
+        out = np.zeros(len(PASS), np.float32)
+        self._ck(lib().ve_timings(self.h, out.ctypes.data))
+        return dict(zip(PASS_NAMES, out.tolist()))

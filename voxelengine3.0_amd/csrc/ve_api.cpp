@@ -1,0 +1,474 @@
+// C ABI of the SVO ray pipeline: context, scene upload, per-pass dispatch,
+// whole-frame sequence and buffer access.  Host code only; every kernel is
+// launched through ve_launch.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/ve_svo.h"
+#include "ve_internal.h"
+#include "ve_launch.h"
+
+using namespace ve;
+
+namespace {
+thread_local std::string g_error;
+}
+
+void ve_clear_global_error() { g_error.clear(); }
+int ve_global_error(int code, const char *msg)
+{
+    g_error = msg;
+    return code;
+}
+
+struct ve_scene {
+    uint32_t *words = nullptr;
+    uint32_t n_words = 0;
+    uint32_t depth = 0;
+};
+
+struct ve_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t max_w = 0, max_h = 0;
+    uint32_t hash_width = 16384;
+    uint32_t max_bounces = 8;
+    std::string err;
+    float cam[16] = {0};
+    float sky[3] = {0, 0, 0};
+    bool node_stats = false;
+    bool timing = false;
+    DeviceBuffers b;
+    std::vector<void *> allocations;
+};
+
+namespace {
+
+int ctx_error(ve_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf; else g_error = buf;
+    return code;
+}
+
+#define HIP_OK(ctx, call)                                                                      \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) return ctx_error(ctx, VE_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dalloc(ve_ctx *c, T **p, size_t count)
+{
+    void *ptr = nullptr;
+    hipError_t e = hipMalloc(&ptr, count * sizeof(T) ? count * sizeof(T) : 16);
+    if (e != hipSuccess) return ctx_error(c, VE_ENOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    e = hipMemset(ptr, 0, count * sizeof(T));
+    if (e != hipSuccess) return ctx_error(c, VE_EDEVICE, "hipMemset: %s", hipGetErrorString(e));
+    c->allocations.push_back(ptr);
+    *p = static_cast<T *>(ptr);
+    return VE_OK;
+}
+
+struct BufView {
+    void *ptr;
+    size_t bytes;
+};
+
+BufView buffer_view(ve_ctx *c, ve_buffer_id id)
+{
+    const size_t P = (size_t)c->max_w * c->max_h;
+    switch (id) {
+    case VE_BUF_GBUFFER: return {c->b.gbuf, P * 16};
+    case VE_BUF_GBUFFER_POS: return {c->b.gpos, P * 4};
+    case VE_BUF_NBUFFER: return {c->b.nbuf, (size_t)c->hash_width * NBUFFER_STRIDE * NBUFFER_SLOTS * 4};
+    case VE_BUF_LBUFFER: return {c->b.lbuf, (size_t)c->hash_width * LBUFFER_STRIDE * LBUFFER_SLOTS * 4};
+    case VE_BUF_RBUFFER: return {c->b.rbuf, (size_t)c->hash_width * RBUFFER_STRIDE * RBUFFER_SLOTS * 4};
+    case VE_BUF_CANDIDATES: return {c->b.candidates, (size_t)CANDIDATE_CAPACITY * 16};
+    case VE_BUF_SHADE_LIST: return {c->b.shade_list, (size_t)SHADE_BUDGET_MAX * 8};
+    case VE_BUF_RAYS: return {c->b.rays, (size_t)SHADE_BUDGET_MAX * sizeof(Ray)};
+    case VE_BUF_OUTPUT: return {c->b.output, P * 16};
+    case VE_BUF_COUNTERS: return {c->b.counters, NUM_COUNTERS * 4};
+    case VE_BUF_RANK_STATS: return {c->b.rank_stats, RANK_STATS_WORDS * 4};
+    case VE_BUF_SAMPLE_COUNTS: return {c->b.samples, (size_t)CLAIM_WIDTH * CLAIM_PROBES * 4};
+    case VE_BUF_CLAIM_MAP: return {c->b.claim, (size_t)CLAIM_WIDTH * CLAIM_PROBES * 4};
+    case VE_BUF_EMISSIVES: return {c->b.emissives, (size_t)EMISSIVE_CAPACITY * sizeof(EmissiveVoxel)};
+    default: return {nullptr, 0};
+    }
+}
+
+void fill_params(ve_ctx *c, const ve_scene *s, const ve_frame_config *fc, const ve_frame_inputs *in, FrameParams &p)
+{
+    std::memset(&p, 0, sizeof p);
+    p.W = in->width;
+    p.H = in->height;
+    p.depth = s ? s->depth : 0;
+    p.n_words = s ? s->n_words : 0;
+    p.hash_width = c->hash_width;
+    p.max_bounces = c->max_bounces;
+    p.frame_idx = in->frame_idx;
+    p.frame_seed = in->frame_seed;
+    p.time = in->time;
+    p.primary_cc = (uint32_t)fc->primary_controlchecks;
+    p.bounce_cc = (uint32_t)fc->bounce_controlchecks;
+    p.update_time = fc->update_time;
+    p.disable_stale_reset = fc->disable_stale_reset;
+    p.normal_precision = fc->normal_precision;
+    uint32_t budget = fc->shading_budget < 1 ? 1u : (uint32_t)fc->shading_budget;   // renderer.cpp:285-287
+    p.shade_budget = std::min(budget, SHADE_BUDGET_MAX);
+    p.cap_direct = (uint32_t)fc->sample_cap_direct;
+    p.cap_indirect = (uint32_t)fc->sample_cap_indirect;
+    p.w_pixels = fc->weight_pixels;
+    p.w_variance = fc->weight_variance;
+    p.pixels_norm = fc->pixels_norm;
+    p.variance_norm = fc->variance_norm;
+    p.top_fraction = fc->top_fraction;
+    p.mid_fraction = fc->mid_fraction;
+    p.min_samples = fc->min_samples;
+    p.restir_enabled = fc->restir_enabled;
+    p.restir_m = fc->restir_m;
+    p.restir_temporal = fc->restir_temporal;
+    p.restir_spatial = fc->restir_spatial;
+    p.restir_neighbors = fc->restir_spatial_neighbors;
+    p.restir_radius = fc->restir_spatial_radius;
+    p.restir_max_m = fc->restir_max_m;
+    p.firefly_k = fc->firefly_k;
+    p.firefly_floor = fc->firefly_floor;
+    p.render_type = fc->render_type;
+    p.reconstruction_radius = fc->reconstruction_radius;
+    std::memcpy(p.cam, c->cam, sizeof p.cam);
+    std::memcpy(p.sky, c->sky, sizeof p.sky);
+}
+
+int check_frame(ve_ctx *c, const ve_scene *s, const ve_frame_config *fc, const ve_frame_inputs *in)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    if (!s || !s->words) return ctx_error(c, VE_EINVAL, "null scene");
+    if (!fc || !in) return ctx_error(c, VE_EINVAL, "null frame config / inputs");
+    if (in->width == 0 || in->height == 0) return ctx_error(c, VE_EINVAL, "zero-sized viewport");
+    if (in->width > c->max_w || in->height > c->max_h)
+        return ctx_error(c, VE_EINVAL, "viewport %ux%u exceeds the context's %ux%u", in->width, in->height, c->max_w, c->max_h);
+    return VE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ve_last_error(const ve_ctx *ctx) { return ctx ? ctx->err.c_str() : g_error.c_str(); }
+
+void ve_frame_config_defaults(ve_frame_config *fc)
+{
+    // core::FrameConfig defaults (src/renderer/core.hpp:83-158)
+    std::memset(fc, 0, sizeof *fc);
+    fc->primary_controlchecks = 70;
+    fc->bounce_controlchecks = 100;
+    fc->reconstruction_radius = 10;
+    fc->normal_precision = 6;
+    fc->disable_stale_reset = 0;
+    fc->update_time = 0;
+    fc->shading_budget = 131072;
+    fc->sample_cap_direct = 16;
+    fc->sample_cap_indirect = 64;
+    fc->weight_pixels = 0.1f;
+    fc->weight_variance = 0.9f;
+    fc->pixels_norm = 100.0f;
+    fc->variance_norm = 10000.0f;
+    fc->top_fraction = 0.30f;
+    fc->mid_fraction = 0.40f;
+    fc->min_samples = 1;
+    fc->restir_enabled = 1;
+    fc->restir_m = 4;
+    fc->restir_temporal = 1;
+    fc->restir_spatial = 1;
+    fc->restir_spatial_neighbors = 1;
+    fc->restir_spatial_radius = 2.75f;
+    fc->restir_max_m = 24;
+    fc->firefly_k = 1.0f;
+    fc->firefly_floor = 255.0f;
+    fc->render_type = 0;
+}
+
+void ve_camera_ubo(const float pos[3], const float dir[3], float fov_deg, float aspect, float ubo[16])
+{
+    // Camera::UpdateUBO (camera.cpp:39-50) with glm's scalar arithmetic:
+    // cross, normalize = v * (1/sqrt((x*x + y*y) + z*z)), radians = deg * pi/180.
+    auto cross = [](const float x[3], const float y[3], float r[3]) {
+        r[0] = x[1] * y[2] - y[1] * x[2];
+        r[1] = x[2] * y[0] - y[2] * x[0];
+        r[2] = x[0] * y[1] - y[0] * x[1];
+    };
+    auto normalize = [](const float v[3], float r[3]) {
+        float t0 = v[0] * v[0], t1 = v[1] * v[1], t2 = v[2] * v[2];
+        float inv = 1.0f / std::sqrt((t0 + t1) + t2);
+        r[0] = v[0] * inv;
+        r[1] = v[1] * inv;
+        r[2] = v[2] * inv;
+    };
+    const float up_axis[3] = {0.0f, 1.0f, 0.0f};
+    const float t = std::tan((fov_deg / 2.0f) * 0.01745329251994329576923690768489f);
+    float c[3], n[3], right[3], u[3];
+    cross(up_axis, dir, c);
+    normalize(c, n);
+    for (int i = 0; i < 3; i++) right[i] = (n[i] * t) * aspect;
+    cross(right, dir, c);
+    normalize(c, u);
+    for (int i = 0; i < 3; i++) {
+        ubo[0 + i] = pos[i];
+        ubo[4 + i] = dir[i];
+        ubo[8 + i] = right[i];
+        ubo[12 + i] = u[i] * t;
+    }
+    ubo[3] = ubo[7] = ubo[11] = ubo[15] = 0.0f;
+}
+
+int ve_create(ve_ctx **out, const ve_renderer_config *cfg)
+{
+    ve_clear_global_error();
+    if (!out || !cfg) return ve_global_error(VE_EINVAL, "null argument");
+    if (cfg->max_width == 0 || cfg->max_height == 0) return ve_global_error(VE_EINVAL, "zero-sized framebuffer");
+    if (cfg->max_bounces > 64) return ve_global_error(VE_EINVAL, "max_bounces above 64");
+    ve_ctx *c = new (std::nothrow) ve_ctx();
+    if (!c) return ve_global_error(VE_ENOMEM, "context allocation");
+    c->device = cfg->device;
+    c->max_w = cfg->max_width;
+    c->max_h = cfg->max_height;
+    c->hash_width = cfg->hash_width ? cfg->hash_width : 16384u;
+    c->max_bounces = cfg->max_bounces ? cfg->max_bounces : 8u;
+    if (c->hash_width < 2) { delete c; return ve_global_error(VE_EINVAL, "hash_width below 2"); }
+    int rc = VE_OK;
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) { ve_global_error(VE_EDEVICE, hipGetErrorString(e)); delete c; return VE_EDEVICE; }
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);   // default priority
+    if (e != hipSuccess) { ve_global_error(VE_EDEVICE, hipGetErrorString(e)); delete c; return VE_EDEVICE; }
+    const size_t P = (size_t)c->max_w * c->max_h;
+    DeviceBuffers &b = c->b;
+    if ((rc = dalloc(c, &b.gbuf, P)) || (rc = dalloc(c, &b.gpos, P)) || (rc = dalloc(c, &b.output, P)) ||
+        (rc = dalloc(c, &b.nbuf, (size_t)c->hash_width * NBUFFER_STRIDE * NBUFFER_SLOTS)) ||
+        (rc = dalloc(c, &b.lbuf, (size_t)c->hash_width * LBUFFER_STRIDE * LBUFFER_SLOTS)) ||
+        (rc = dalloc(c, &b.rbuf, (size_t)c->hash_width * RBUFFER_STRIDE * RBUFFER_SLOTS)) ||
+        (rc = dalloc(c, &b.claim, (size_t)CLAIM_WIDTH * CLAIM_PROBES)) ||
+        (rc = dalloc(c, &b.samples, (size_t)CLAIM_WIDTH * CLAIM_PROBES)) ||
+        (rc = dalloc(c, &b.candidates, CANDIDATE_CAPACITY)) || (rc = dalloc(c, &b.shade_list, SHADE_BUDGET_MAX)) ||
+        (rc = dalloc(c, &b.rays, SHADE_BUDGET_MAX)) || (rc = dalloc(c, &b.rays_next, SHADE_BUDGET_MAX)) ||
+        (rc = dalloc(c, &b.counters, NUM_COUNTERS)) || (rc = dalloc(c, &b.rank_stats, RANK_STATS_WORDS)) ||
+        (rc = dalloc(c, &b.emissives, EMISSIVE_CAPACITY)) ||
+        (rc = dalloc(c, &b.emissive_claim, (size_t)EMISSIVE_CLAIM_W * EMISSIVE_CLAIM_PROBES)) ||
+        (rc = dalloc(c, &b.materials, MATERIAL_SLOTS)) || (rc = dalloc(c, &b.node_stats, 4))) {
+        std::string msg = c->err;
+        ve_destroy(c);
+        return ve_global_error(rc, msg.c_str());
+    }
+    // RankStats thresholds start at (+inf, -inf): every voxel in the mid tier
+    // on the first frame (rank.comp:109-111, threshold.comp:183-189).
+    uint32_t thr[2] = {0x7F800000u, 0xFF800000u};
+    e = hipMemcpy(b.rank_stats, thr, sizeof thr, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { ve_global_error(VE_EDEVICE, hipGetErrorString(e)); ve_destroy(c); return VE_EDEVICE; }
+    *out = c;
+    return VE_OK;
+}
+
+void ve_destroy(ve_ctx *c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void *p : c->allocations) hipFree(p);
+    if (c->b.sort_temp) hipFree(c->b.sort_temp);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int ve_scene_upload(ve_ctx *c, const uint32_t *words, uint32_t n_words, uint32_t depth, ve_scene **scene)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    if (!words || !scene || n_words == 0) return ctx_error(c, VE_EINVAL, "null words / scene or empty array");
+    if (depth < 1 || depth > 15) return ctx_error(c, VE_EINVAL, "depth %u outside [1,15]", depth);
+    // Every internal node's 8-word child block must lie inside the array, so
+    // the traversal can never fetch outside it.
+    for (uint32_t i = 0; i < n_words; i++) {
+        const uint32_t w = words[i];
+        if ((w & 1u) && (uint64_t)(w >> 4) + 8u > n_words)
+            return ctx_error(c, VE_EFORMAT, "node %u points to block %u outside the %u-word array", i, w >> 4, n_words);
+    }
+    HIP_OK(c, hipSetDevice(c->device));
+    ve_scene *s = new (std::nothrow) ve_scene();
+    if (!s) return ctx_error(c, VE_ENOMEM, "scene allocation");
+    void *d = nullptr;
+    hipError_t e = hipMalloc(&d, (size_t)n_words * 4);
+    if (e != hipSuccess) { delete s; return ctx_error(c, VE_ENOMEM, "hipMalloc node array: %s", hipGetErrorString(e)); }
+    e = hipMemcpyAsync(d, words, (size_t)n_words * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { hipFree(d); delete s; return ctx_error(c, VE_EDEVICE, "node upload: %s", hipGetErrorString(e)); }
+    s->words = static_cast<uint32_t *>(d);
+    s->n_words = n_words;
+    s->depth = depth;
+    *scene = s;
+    return VE_OK;
+}
+
+int ve_scene_release(ve_ctx *c, ve_scene *s)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    if (!s) return ctx_error(c, VE_ENOTFOUND, "null scene");
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    hipFree(s->words);
+    delete s;
+    return VE_OK;
+}
+
+int ve_set_camera(ve_ctx *c, const float ubo[16])
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    if (!ubo) return ctx_error(c, VE_EINVAL, "null camera");
+    std::memcpy(c->cam, ubo, sizeof c->cam);
+    return VE_OK;
+}
+
+int ve_set_materials(ve_ctx *c, const void *entries, uint32_t n)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    if (!entries || n == 0 || n > MATERIAL_SLOTS) return ctx_error(c, VE_EINVAL, "material count %u outside [1,127]", n);
+    HIP_OK(c, hipSetDevice(c->device));
+    HIP_OK(c, hipMemcpyAsync(c->b.materials, entries, (size_t)n * sizeof(Material), hipMemcpyHostToDevice, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return VE_OK;
+}
+
+int ve_set_sky(ve_ctx *c, const uint8_t *const faces[6], int edge)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    if (!faces || edge < 1) return ctx_error(c, VE_EINVAL, "null faces or edge < 1");
+    // Constant skies only: every texel of every face must hold the same RGB8
+    // value, which bilinear filtering returns unchanged (c / 255).
+    const uint8_t *c0 = faces[0];
+    if (!c0) return ctx_error(c, VE_EINVAL, "null face");
+    for (int f = 0; f < 6; f++) {
+        if (!faces[f]) return ctx_error(c, VE_EINVAL, "null face");
+        for (int i = 0; i < edge * edge; i++)
+            if (std::memcmp(faces[f] + 3 * i, c0, 3) != 0)
+                return ctx_error(c, VE_EINVAL, "only constant skies are supported (face %d texel %d differs)", f, i);
+    }
+    for (int i = 0; i < 3; i++) c->sky[i] = (float)c0[i] / 255.0f;
+    return VE_OK;
+}
+
+int ve_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const ve_frame_config *fc, const ve_frame_inputs *in)
+{
+    int rc = check_frame(c, s, fc, in);
+    if (rc) return rc;
+    HIP_OK(c, hipSetDevice(c->device));
+    FrameParams p;
+    fill_params(c, s, fc, in, p);
+    switch (pass) {
+    case VE_PASS_PRIMARY:
+        launch_primary(c->stream, s->words, p, c->b, c->node_stats);
+        break;
+    default:
+        return ctx_error(c, VE_EINVAL, "pass %d not available", (int)pass);
+    }
+    HIP_OK(c, hipGetLastError());
+    return VE_OK;
+}
+
+int ve_frame(ve_ctx *c, ve_scene *s, const ve_frame_config *fc, const ve_frame_inputs *in)
+{
+    int rc = check_frame(c, s, fc, in);
+    if (rc) return rc;
+    return ctx_error(c, VE_ESTATE, "whole-frame sequence not available");
+}
+
+int ve_buffer_size(ve_ctx *c, ve_buffer_id id, size_t *bytes)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    BufView v = buffer_view(c, id);
+    if (!v.ptr) return ctx_error(c, VE_ENOTFOUND, "unknown buffer id %d", (int)id);
+    if (bytes) *bytes = v.bytes;
+    return VE_OK;
+}
+
+int ve_read(ve_ctx *c, ve_buffer_id id, void *dst, size_t bytes)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    BufView v = buffer_view(c, id);
+    if (!v.ptr) return ctx_error(c, VE_ENOTFOUND, "unknown buffer id %d", (int)id);
+    if (!dst || bytes > v.bytes) return ctx_error(c, VE_EINVAL, "read of %zu bytes from a %zu-byte buffer", bytes, v.bytes);
+    HIP_OK(c, hipSetDevice(c->device));
+    HIP_OK(c, hipMemcpyAsync(dst, v.ptr, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return VE_OK;
+}
+
+int ve_write(ve_ctx *c, ve_buffer_id id, const void *src, size_t bytes)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    BufView v = buffer_view(c, id);
+    if (!v.ptr) return ctx_error(c, VE_ENOTFOUND, "unknown buffer id %d", (int)id);
+    if (!src || bytes > v.bytes) return ctx_error(c, VE_EINVAL, "write of %zu bytes to a %zu-byte buffer", bytes, v.bytes);
+    HIP_OK(c, hipSetDevice(c->device));
+    HIP_OK(c, hipMemcpyAsync(v.ptr, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return VE_OK;
+}
+
+int ve_sync(ve_ctx *c)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    HIP_OK(c, hipSetDevice(c->device));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return VE_OK;
+}
+
+int ve_enable_timing(ve_ctx *c, int enable)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    c->timing = enable != 0;
+    return VE_OK;
+}
+
+int ve_timings(ve_ctx *c, float ms_out[VE_PASS_COUNT])
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    if (!ms_out) return ctx_error(c, VE_EINVAL, "null output");
+    for (int i = 0; i < VE_PASS_COUNT; i++) ms_out[i] = 0.0f;
+    return VE_OK;
+}
+
+int ve_enable_node_stats(ve_ctx *c, int enable)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    c->node_stats = enable != 0;
+    HIP_OK(c, hipSetDevice(c->device));
+    HIP_OK(c, hipMemsetAsync(c->b.node_stats, 0, 4 * sizeof(unsigned long long), c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return VE_OK;
+}
+
+int ve_node_stats(ve_ctx *c, uint64_t *pr, uint64_t *tr, uint64_t *ps, uint64_t *ts)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    unsigned long long v[4];
+    HIP_OK(c, hipSetDevice(c->device));
+    HIP_OK(c, hipMemcpyAsync(v, c->b.node_stats, sizeof v, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    if (pr) *pr = v[0];
+    if (tr) *tr = v[1];
+    if (ps) *ps = v[2];
+    if (ts) *ts = v[3];
+    return VE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+// Internal definitions shared by the host library and the kernels.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/ve_svo.h"
+
+// ---- error reporting --------------------------------------------------------
+// Calls without a context record their message in a thread-local slot that
+// ve_last_error(NULL) returns.
+void ve_clear_global_error();
+int ve_global_error(int code, const char *msg);
+
+// ---- constants mirrored from the reference ----------------------------------
+// src/renderer/renderer.hpp:90-103 and src/shd/wavefront.glsl
+namespace ve {
+constexpr uint32_t LBUFFER_STRIDE = 12;        // wavefront.glsl:289
+constexpr uint32_t LBUFFER_SLOTS = 32;         // renderer.hpp:90
+constexpr uint32_t NBUFFER_STRIDE = 3;         // normal.comp:31
+constexpr uint32_t NBUFFER_SLOTS = 32;         // renderer.hpp:91
+constexpr uint32_t RBUFFER_STRIDE = 8;         // wavefront.glsl:274
+constexpr uint32_t RBUFFER_SLOTS = 16;         // renderer.hpp:92
+constexpr uint32_t SHADE_BUDGET_MAX = 196608;  // renderer.hpp:95
+constexpr uint32_t CLAIM_WIDTH = 16384;        // renderer.hpp:96
+constexpr uint32_t CLAIM_PROBES = 64;          // renderer.hpp:97
+constexpr uint32_t EMISSIVE_CAPACITY = 4096;   // wavefront.glsl:194 (the shader's value governs)
+constexpr uint32_t EMISSIVE_CLAIM_W = 8192;    // wavefront.glsl:195
+constexpr uint32_t EMISSIVE_CLAIM_PROBES = 8;  // wavefront.glsl:196
+constexpr uint32_t CANDIDATE_CAPACITY = 1048576;  // wavefront.glsl:259
+constexpr uint32_t HISTOGRAM_BINS = 256;       // wavefront.glsl:260
+constexpr uint32_t RR_START_BOUNCE = 2;        // wavefront.glsl:112
+constexpr uint32_t TRUSTED_THRESHOLD = 128;    // resolve.comp:82
+constexpr uint32_t RANK_STATS_WORDS = 264;     // wavefront.glsl:243-255
+constexpr uint32_t NUM_COUNTERS = 16;
+constexpr uint32_t MATERIAL_SLOTS = 127;       // shade.comp:90
+
+// Ray record, 96 bytes, field for field the reference's std430 Ray
+// (wavefront.glsl:128-143).  `pad0` carries the hit voxel extent between
+// trace and shade, as the reference does.
+struct Ray {
+    float ox, oy, oz, pad0;
+    float dx, dy, dz, pad1;
+    float tx, ty, tz;
+    uint32_t primary_vid;
+    float dlx, dly, dlz;
+    uint32_t rng;
+    float ilx, ily, ilz;
+    uint32_t bounce;
+    uint32_t hit_vid;
+    uint32_t hit_material;
+    uint32_t pad2, pad3;
+};
+static_assert(sizeof(Ray) == 96, "Ray must match the reference's 96-byte std430 record");
+
+// std140 material entry (material.hpp:17-25 / shade.comp:84-90)
+struct Material {
+    float color[4];
+    float specular_color[4];
+    float roughness, specular, metallic;
+    int32_t emissive;
+    float emissive_intensity;
+    float pad[3];
+};
+static_assert(sizeof(Material) == 64, "std140 material entry is 64 bytes");
+
+// 32-byte EmissiveVoxel (wavefront.glsl:176-181)
+struct EmissiveVoxel {
+    float ex, ey, ez;
+    uint32_t vid;
+    uint32_t px, py, pz;
+    uint32_t size;
+};
+static_assert(sizeof(EmissiveVoxel) == 32, "EmissiveVoxel is 32 bytes");
+
+// Frame parameters handed to every kernel by value.
+struct FrameParams {
+    uint32_t W, H;
+    uint32_t depth;
+    uint32_t n_words;
+    uint32_t hash_width;
+    uint32_t max_bounces;
+    uint32_t frame_idx, frame_seed, time;
+    uint32_t primary_cc, bounce_cc;
+    uint32_t update_time;
+    int32_t disable_stale_reset;
+    int32_t normal_precision;
+    uint32_t shade_budget;
+    uint32_t cap_direct, cap_indirect;
+    float w_pixels, w_variance, pixels_norm, variance_norm, top_fraction, mid_fraction;
+    int32_t min_samples;
+    int32_t restir_enabled, restir_m, restir_temporal, restir_spatial, restir_neighbors;
+    float restir_radius;
+    int32_t restir_max_m;
+    float firefly_k, firefly_floor;
+    int32_t render_type;
+    int32_t reconstruction_radius;
+    float cam[16];
+    float sky[3];
+};
+}  // namespace ve

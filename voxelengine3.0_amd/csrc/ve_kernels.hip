@@ -1,0 +1,89 @@
+// gfx950 kernels of the SVO ray pipeline.
+//
+// Pass map (reference file:line -> kernel):
+//   generate_primary.comp:38-85  -> k_primary
+//
+// Launch shape.  Screen passes run one 64-lane wavefront per 8x8 pixel tile
+// (a tile's rays are coherent, so the lanes walk the same octree nodes), four
+// wavefronts per 256-thread block.  Blocks are handed out round-robin to the
+// 8 XCDs by the hardware; the block -> tile map below gives each XCD one
+// contiguous band of tiles so the nodes a band touches stay in that XCD's L2.
+#include <hip/hip_runtime.h>
+
+#include "ve_device.h"
+#include "ve_launch.h"
+
+namespace ve {
+
+constexpr int TILE = 8;
+constexpr int WAVES_PER_BLOCK = 4;
+constexpr int NUM_XCD = 8;
+
+// Logical tile-group index of this block with the XCD-banded mapping; the
+// grid is padded to a multiple of NUM_XCD blocks.
+__device__ __forceinline__ uint32_t xcd_banded_block(uint32_t b, uint32_t nblocks_padded)
+{
+    const uint32_t per = nblocks_padded / NUM_XCD;
+    return (b % NUM_XCD) * per + b / NUM_XCD;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <bool STATS>
+__global__ __launch_bounds__(256) void k_primary(const uint32_t *__restrict__ words, FrameParams p,
+                                                 uint4 *__restrict__ gbuf, uint32_t *__restrict__ gpos,
+                                                 unsigned long long *__restrict__ stats)
+{
+    const uint32_t tiles_x = (p.W + TILE - 1) / TILE;
+    const uint32_t tiles_y = (p.H + TILE - 1) / TILE;
+    const uint32_t group = xcd_banded_block(blockIdx.x, gridDim.x);
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t tile = group * WAVES_PER_BLOCK + wave;
+    if (tile >= tiles_x * tiles_y) return;
+    const uint32_t x = (tile % tiles_x) * TILE + (lane & 7u);
+    const uint32_t y = (tile / tiles_x) * TILE + (lane >> 3);
+    uint32_t checks = 0, reads = 0;
+    if (x < p.W && y < p.H) {
+        const f3 dir = primary_dir(p.cam, x, y, p.W, p.H);
+        const f3 org = mk(p.cam[0], p.cam[1], p.cam[2]);
+        const Hit h = raycast<STATS>(words, p.n_words, p.depth, org, dir, rcp3(dir), p.primary_cc, checks, reads);
+        const size_t pix = (size_t)y * p.W + x;
+        if (h.hit) {
+            const uint32_t bits = p.depth - 1u;
+            gbuf[pix] = make_uint4(checks, h.material, 0u, voxel_key(h.px, h.py, h.pz) + 1u);
+            gpos[pix] = (h.px >> 1) | ((h.py >> 1) << bits) | ((h.pz >> 1) << (2u * bits));
+        } else {
+            gbuf[pix] = make_uint4(checks, 0u, 0u, 0u);
+            gpos[pix] = 0u;
+        }
+    }
+    if (STATS) {
+        unsigned long long r = wave_sum(reads), c = wave_sum(checks);
+        if (lane == 0) {
+            atomicAdd(&stats[0], r);
+            atomicAdd(&stats[2], c);
+        }
+    }
+}
+
+static uint32_t padded_blocks(uint32_t work_groups)
+{
+    uint32_t nb = work_groups == 0 ? 1 : work_groups;
+    return (nb + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+}
+
+void launch_primary(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, bool stats)
+{
+    const uint32_t tiles = ((p.W + TILE - 1) / TILE) * ((p.H + TILE - 1) / TILE);
+    const uint32_t nb = padded_blocks((tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
+    if (stats)
+        hipLaunchKernelGGL(k_primary<true>, dim3(nb), dim3(256), 0, s, words, p, b.gbuf, b.gpos, b.node_stats);
+    else
+        hipLaunchKernelGGL(k_primary<false>, dim3(nb), dim3(256), 0, s, words, p, b.gbuf, b.gpos, b.node_stats);
+}
+
+}  // namespace ve

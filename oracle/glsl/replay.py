@@ -126,3 +126,190 @@ class GLPipeline:
 
     def read_gbuffer(self):
         return self.gbuf.read(), self.gpos.read()
+
+    # --- 2. normal.comp (renderer.cpp:319-341)
+    def normal(self, fc, time):
+        p = self.prog("normal.comp")
+        p.use()
+        self.gbuf.bind_image(0)
+        self.lbuf.bind_image(1)
+        self.gpos.bind_image(2)
+        self.nbuf.bind_image(3)
+        self.tbo.bind(p, "octreeTexture", 0)
+        p.set1ui("octreeDepth", self.depth)
+        p.set1i("nBufferWidth", self.hash_w)
+        p.set1i("nBufferSlots", NBUFFER_SLOTS)
+        p.set1i("lBufferSlots", LBUFFER_SLOTS)
+        p.set1i("lBufferWidth", self.hash_w)
+        p.set1i("normalPrecision", fc["normal_precision"])
+        p.set2i("screenResolution", self.W, self.H)
+        p.set1ui("time", time)
+        p.set1ui("updateTime", fc["update_time"])
+        p.dispatch(*self.groups2d())
+
+    # --- per-frame reset (renderer.cpp:343-375)
+    def reset(self):
+        z4 = np.zeros(4, np.uint32)
+        self.ring_meta.write(z4)
+        self.shade_meta.write(z4)
+        self.cand_meta.write(z4)
+        self.rank_stats.write(np.array([0x7F7FFFFF, 0, 0, 0], np.uint32), offset=16)
+        self.rank_stats.write(np.zeros(256, np.uint32), offset=32)
+        self.claim.zero()
+        self.samples.zero()
+
+    def _bind_ssbos(self):
+        for i, b in enumerate((self.ring, self.ring_meta, self.shade_list, self.shade_meta, self.emissives,
+                               self.emissive_meta, self.candidates, self.cand_meta, self.rank_stats)):
+            b.bind_ssbo(i)
+
+    # --- 3. schedule.comp (renderer.cpp:388-415)
+    def schedule(self, frame_seed):
+        self._bind_ssbos()
+        p = self.prog("schedule.comp")
+        p.use()
+        self.gbuf.bind_image(0)
+        self.claim.bind_image(1)
+        self.samples.bind_image(2)
+        p.set2i("screenResolution", self.W, self.H)
+        p.set1i("claimWidth", CLAIM_WIDTH)
+        p.set1i("claimProbes", CLAIM_PROBES)
+        p.set1ui("frameSeed", frame_seed)
+        p.dispatch(*self.groups2d())
+        return min(int(self.cand_meta.read()[0]), CANDIDATE_CAPACITY)
+
+    # --- 3b. rank.comp + threshold.comp (renderer.cpp:427-470)
+    def rank_threshold(self, fc, cand_count, budget):
+        p = self.prog("rank.comp")
+        p.use()
+        self.lbuf.bind_image(0)
+        self.samples.bind_image(1)
+        p.set1i("lBufferWidth", self.hash_w)
+        p.set1i("lBufferSlots", LBUFFER_SLOTS)
+        p.set1f("wPixels", fc["weight_pixels"])
+        p.set1f("wVariance", fc["weight_variance"])
+        p.set1f("pixelsNorm", fc["pixels_norm"])
+        p.set1f("varianceNorm", fc["variance_norm"])
+        p.set1i("minSamples", fc["min_samples"])
+        p.set1ui("shadeBudget", budget)
+        p.dispatch((cand_count + 63) // 64)
+        t = self.prog("threshold.comp")
+        t.use()
+        t.set1f("topFraction", fc["top_fraction"])
+        t.set1f("midFraction", fc["mid_fraction"])
+        t.dispatch(1)
+        return int(self.shade_meta.read()[0])
+
+    # --- 4+5. emit / trace / shade chunk loop (renderer.cpp:476-606)
+    def bounces(self, fc, scheduled, frame_seed, frame_idx, time):
+        bounces_run = 0
+        traced = 0
+        for base in range(0, scheduled, RAY_CAPACITY_HOST):
+            size = min(scheduled - base, RAY_CAPACITY_HOST)
+            self.ring_meta.write(np.zeros(4, np.uint32))
+            e = self.prog("emit.comp")
+            e.use()
+            self.gbuf.bind_image(0)
+            e.set2i("screenResolution", self.W, self.H)
+            e.set1ui("chunkBase", base)
+            e.set1ui("chunkSize", size)
+            e.set1ui("frameSeed", frame_seed)
+            e.set1ui("frameIdx", frame_idx)
+            e.set1ui("octreeDepth_", self.depth)
+            e.dispatch((size + 63) // 64)
+            for b in range(8):
+                head, tail = [int(v) for v in self.ring_meta.read()[:2]]
+                if head == tail:
+                    break
+                win = head - tail
+                tr = self.prog("trace.comp")
+                tr.use()
+                self.tbo.bind(tr, "octreeTexture", 0)
+                tr.set1ui("octreeDepth", self.depth)
+                tr.set1ui("controlchecks", fc["bounce_controlchecks"])
+                tr.set1ui("rayWindowBase", tail)
+                tr.set1ui("rayWindowSize", win)
+                tr.dispatch((win + 63) // 64)
+                live = self.ring.read(np.uint32).reshape(-1, 24)
+                idx = (tail + np.arange(win)) % RAY_CAPACITY_HOST
+                traced += int((live[idx, 19] < self.max_bounces).sum())
+                sh = self.prog("shade.comp")
+                sh.use()
+                self.lbuf.bind_image(0)
+                self.nbuf.bind_image(1)
+                self.emissive_claim.bind_image(2)
+                self.rbuf.bind_image(3)
+                self.tbo.bind(sh, "octreeTexture", 0)
+                sh.set1ui("octreeDepth", self.depth)
+                self.sky.bind(sh, "skybox", 1)
+                self.mat_ubo.bind_ubo(1)
+                sh.set1ui("controlchecks", fc["bounce_controlchecks"])
+                sh.set1ui("rayWindowBase", tail)
+                sh.set1ui("rayWindowSize", win)
+                sh.set1i("lBufferWidth", self.hash_w)
+                sh.set1i("lBufferSlots", LBUFFER_SLOTS)
+                sh.set1i("nBufferWidth", self.hash_w)
+                sh.set1i("nBufferSlots", NBUFFER_SLOTS)
+                sh.set1ui("time", time)
+                sh.set1ui("updateTime", fc["update_time"])
+                sh.set1ui("SAMPLE_CAP_DIRECT", fc["sample_cap_direct"])
+                sh.set1ui("SAMPLE_CAP_INDIRECT", fc["sample_cap_indirect"])
+                sh.set1f("fireflyK", fc["firefly_k"])
+                sh.set1f("fireflyFloor", fc["firefly_floor"])
+                sh.set1i("disableStaleReset", fc["disable_stale_reset"])
+                sh.set1i("rBufferWidth", self.hash_w)
+                sh.set1i("rBufferSlots", RBUFFER_SLOTS)
+                sh.set1i("restirEnabled", fc["restir_enabled"])
+                sh.set1i("restirM", fc["restir_m"])
+                sh.set1i("restirTemporalEnabled", fc["restir_temporal"])
+                sh.set1i("restirSpatialEnabled", fc["restir_spatial"])
+                sh.set1i("restirSpatialNeighbors", fc["restir_spatial_neighbors"])
+                sh.set1f("restirSpatialRadius", fc["restir_spatial_radius"])
+                sh.set1i("restirMaxM", fc["restir_max_m"])
+                sh.dispatch((win + 63) // 64)
+                self.ring_meta.write(np.array([head], np.uint32), offset=4)
+                bounces_run = max(bounces_run, b + 1)
+        return bounces_run, traced
+
+    # --- 6. resolve.comp (renderer.cpp:612-648)
+    def resolve(self, fc):
+        p = self.prog("resolve.comp")
+        p.use()
+        self.lbuf.bind_image(0)
+        self.gbuf.bind_image(1)
+        self.out.bind_image(2)
+        self.lbuf.bind_image(3)
+        self.nbuf.bind_sampler(p, "nBuffer", 0)
+        self.sky.bind(p, "skybox", 1)
+        self.cam_ubo.bind_ubo(0)
+        self.mat_ubo.bind_ubo(1)
+        p.set1i("nBufferWidth", self.hash_w)
+        p.set1i("nBufferSlots", NBUFFER_SLOTS)
+        p.set1i("lBufferWidth", self.hash_w)
+        p.set1i("lBufferSlots", LBUFFER_SLOTS)
+        p.set2i("screenResolution", self.W, self.H)
+        p.set1i("renderType", fc["render_type"])
+        p.set1ui("primary_controlchecks", fc["primary_controlchecks"])
+        p.set1i("reconstructionRadius", fc["reconstruction_radius"])
+        p.set1f("varianceNormForView", fc["variance_norm"])
+        p.dispatch(*self.groups2d())
+
+    def frame(self, fc, frame_idx, frame_seed, time):
+        """One Renderer::runWavefrontFrame; returns the per-frame counters."""
+        budget = max(1, min(int(fc["shading_budget"]), SHADE_BUDGET_MAX))
+        self.primary(fc["primary_controlchecks"])
+        self.normal(fc, time)
+        self.reset()
+        cand = self.schedule(frame_seed)
+        scheduled = 0
+        if cand > 0:
+            scheduled = self.rank_threshold(fc, cand, budget)
+        scheduled = min(scheduled, budget)
+        bounces_run, traced = self.bounces(fc, scheduled, frame_seed, frame_idx, time)
+        self.resolve(fc)
+        return dict(candidates=cand, scheduled=scheduled, bounces=bounces_run, traced=traced,
+                    emissives=int(self.emissive_meta.read()[0]))
+
+    def read_hash(self, img, height):
+        """Image (x = bucket, y = field) -> bucket-major [bucket][field]."""
+        return img.read().reshape(height, self.hash_w).T.copy()

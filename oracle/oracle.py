@@ -85,3 +85,101 @@ def generate_primary(words, depth, ubo, W, H, controlchecks=70, threads=None, n_
                               W, H, controlchecks, gbuf.ctypes.data, pos.ctypes.data, C.byref(reads), C.byref(steps),
                               threads)
     return gbuf, pos, dict(node_reads=reads.value, steps=steps.value)
+
+
+class FrameConfig(C.Structure):
+    """so_frame_config: field for field core::FrameConfig (src/renderer/core.hpp:83-158)."""
+    _fields_ = [("primary_controlchecks", C.c_int32), ("bounce_controlchecks", C.c_int32),
+                ("reconstruction_radius", C.c_int32), ("normal_precision", C.c_int32),
+                ("disable_stale_reset", C.c_int32), ("update_time", C.c_uint32), ("shading_budget", C.c_int32),
+                ("sample_cap_direct", C.c_int32), ("sample_cap_indirect", C.c_int32),
+                ("weight_pixels", C.c_float), ("weight_variance", C.c_float), ("pixels_norm", C.c_float),
+                ("variance_norm", C.c_float), ("top_fraction", C.c_float), ("mid_fraction", C.c_float),
+                ("min_samples", C.c_int32), ("restir_enabled", C.c_int32), ("restir_m", C.c_int32),
+                ("restir_temporal", C.c_int32), ("restir_spatial", C.c_int32),
+                ("restir_spatial_neighbors", C.c_int32), ("restir_spatial_radius", C.c_float),
+                ("restir_max_m", C.c_int32), ("firefly_k", C.c_float), ("firefly_floor", C.c_float),
+                ("render_type", C.c_int32)]
+
+    DEFAULTS = dict(primary_controlchecks=70, bounce_controlchecks=100, reconstruction_radius=10, normal_precision=6,
+                    disable_stale_reset=0, update_time=0, shading_budget=131072, sample_cap_direct=16,
+                    sample_cap_indirect=64, weight_pixels=0.1, weight_variance=0.9, pixels_norm=100.0,
+                    variance_norm=10000.0, top_fraction=0.30, mid_fraction=0.40, min_samples=1, restir_enabled=1,
+                    restir_m=4, restir_temporal=1, restir_spatial=1, restir_spatial_neighbors=1,
+                    restir_spatial_radius=2.75, restir_max_m=24, firefly_k=1.0, firefly_floor=255.0, render_type=0)
+
+    def __init__(self, **kw):
+        super().__init__()
+        d = dict(self.DEFAULTS)
+        d.update(kw)
+        for k, v in d.items():
+            setattr(self, k, v)
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+BUFFER_IDS = dict(GBUFFER=0, GBUFFER_POS=1, NBUFFER=2, LBUFFER=3, RBUFFER=4, CANDIDATES=5, SHADE_LIST=6, RAYS=7,
+                  OUTPUT=8, RANK_STATS=10, SAMPLE_COUNTS=11, CLAIM_MAP=12, EMISSIVES=13)
+
+
+class Frame:
+    """Persistent renderer state of the C oracle (so_state) + so_frame."""
+
+    def __init__(self, W, H, hash_width=16384, max_bounces=8):
+        L = lib()
+        if not getattr(L, "_frame_sigs", False):
+            L.so_state_create.restype = C.c_void_p
+            L.so_state_create.argtypes = [C.c_uint32] * 4
+            L.so_state_destroy.argtypes = [C.c_void_p]
+            L.so_state_set_materials.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
+            L.so_state_set_sky.argtypes = [C.c_void_p, C.c_void_p]
+            L.so_frame.restype = C.c_int
+            L.so_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                   C.POINTER(FrameConfig), C.c_uint32, C.c_uint32, C.c_uint32]
+            L.so_buffer.restype = C.c_size_t
+            L.so_buffer.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
+            L.so_counters.argtypes = [C.c_void_p, C.c_void_p]
+            L._frame_sigs = True
+        self.L = L
+        self.W, self.H, self.hash_width = W, H, hash_width
+        self.h = L.so_state_create(hash_width, max_bounces, W, H)
+        if not self.h:
+            raise MemoryError("oracle state")
+
+    def close(self):
+        if self.h:
+            self.L.so_state_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_materials(self, entries):
+        e = np.ascontiguousarray(entries)
+        self.L.so_state_set_materials(self.h, e.ctypes.data, len(e))
+
+    def set_sky(self, rgb):
+        a = np.ascontiguousarray(rgb, np.float32)
+        self.L.so_state_set_sky(self.h, a.ctypes.data)
+
+    def frame(self, words, depth, ubo, fc, frame_idx, frame_seed, time, n_words=None):
+        self._words = np.ascontiguousarray(words, np.uint32)
+        ubo = np.ascontiguousarray(ubo, np.float32)
+        n = len(self._words) if n_words is None else n_words
+        rc = self.L.so_frame(self.h, self._words.ctypes.data, n, depth, ubo.ctypes.data, C.byref(fc), frame_idx,
+                             frame_seed, time)
+        if rc != 0:
+            raise RuntimeError("oracle frame failed")
+
+    def read(self, name, dtype=np.uint32):
+        p = C.c_void_p()
+        n = self.L.so_buffer(self.h, BUFFER_IDS[name], C.byref(p))
+        if n == 0:
+            return np.zeros(0, dtype)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint8)), shape=(n,)).copy().view(dtype)
+
+    def counters(self):
+        out = np.zeros(16, np.uint32)
+        self.L.so_counters(self.h, out.ctypes.data)
+        return out

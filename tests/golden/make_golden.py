@@ -113,13 +113,88 @@ def make_fp():
     np.savez_compressed(os.path.join(HERE, "fp_probes.npz"), **arrays)
 
 
+# Frame-level runs: the whole runWavefrontFrame sequence over several frames.
+# "det" = the deterministic parity knobs (one sample per visible voxel,
+# caps above the sample count, no stale reset, no firefly clamp); "default" =
+# the reference's own FrameConfig defaults with a short stale-reset window.
+DET_KNOBS = dict(primary_controlchecks=70, bounce_controlchecks=120, normal_precision=1, shading_budget=196608,
+                 weight_pixels=0.0, weight_variance=0.0, top_fraction=0.0, mid_fraction=1.0, min_samples=1,
+                 sample_cap_direct=1024, sample_cap_indirect=4096, disable_stale_reset=1, update_time=0xFFFFFFFF,
+                 firefly_k=1.0, firefly_floor=1e30)
+DEFAULT_KNOBS = dict(primary_controlchecks=70, bounce_controlchecks=120, normal_precision=6, update_time=3)
+FRAME_RUNS = {
+    "small_det": dict(W=256, H=256, frames=16, max_bounces=2, knobs=DET_KNOBS),
+    "small_default": dict(W=128, H=128, frames=6, max_bounces=3, knobs=DEFAULT_KNOBS),
+}
+SKY_RGB8 = (140, 180, 230)
+
+
+def frame_seed(f):
+    return (1 + 7919 * f) & 0xFFFFFFFF
+
+
+def hash_map(bucket_major, stride, slots):
+    """{vid: fields} of a bucket-major hash image as sorted arrays."""
+    b = bucket_major.reshape(len(bucket_major) // (stride * slots), slots, stride)
+    own = b[:, :, 0]
+    sel = own != 0
+    vids = own[sel]
+    vals = b[:, :, 1:][sel]
+    order = np.argsort(vids, kind="stable")
+    return vids[order], vals[order]
+
+
+def make_frame():
+    from oracle import oracle as O
+    depth, ops = scenes.scene_small(1)
+    words, info = ve.octree_build(ops, depth)
+    mats = scenes.palette(1)
+    meta = {"depth": depth, "words_sha256": words_digest(words, info.size), "sky_rgb8": SKY_RGB8, "runs": {}}
+    arrays = {"materials": mats.view(np.uint8)}
+    for name, run in FRAME_RUNS.items():
+        W, H = run["W"], run["H"]
+        fc = O.FrameConfig(**run["knobs"])
+        eye, d, fov = scenes.camera_small()
+        ubo = ve.camera_ubo(eye, d, fov, W / H)
+        pl = R.GLPipeline(words, depth, W, H, max_bounces=run["max_bounces"], materials=mats, sky_rgb8=SKY_RGB8)
+        pl.set_camera(ubo)
+        counters = []
+        for f in range(run["frames"]):
+            st = pl.frame(fc.as_dict(), f, frame_seed(f), f + 1)
+            counters.append([st["candidates"], st["scheduled"], st["bounces"], st["traced"]])
+            if f == 0:
+                g, p = pl.read_gbuffer()
+                arrays[name + "_gbuf"] = g
+                arrays[name + "_pos"] = p
+                arrays[name + "_out0"] = pl.out.read()
+            print(name, "frame", f, st, flush=True)
+        arrays[name + "_ubo"] = ubo
+        arrays[name + "_counters"] = np.array(counters, np.uint32)
+        arrays[name + "_out"] = pl.out.read()
+        lv, lval = hash_map(pl.read_hash(pl.lbuf, 384).reshape(-1), 12, 32)
+        arrays[name + "_lbuf_vids"], arrays[name + "_lbuf_vals"] = lv, lval
+        nv, nval = hash_map(pl.read_hash(pl.nbuf, 96).reshape(-1), 3, 32)
+        arrays[name + "_nbuf_vids"], arrays[name + "_nbuf_vals"] = nv, nval
+        nc = int(counters[-1][0])
+        arrays[name + "_candidates"] = pl.candidates.read()[: 4 * nc].reshape(-1, 4)
+        arrays[name + "_shade_list"] = pl.shade_list.read()[: 2 * int(counters[-1][1])].reshape(-1, 2)
+        arrays[name + "_sample_counts"] = pl.samples.read()          # [probe][bucket] image rows
+        arrays[name + "_rank_stats"] = pl.rank_stats.read()
+        meta["runs"][name] = dict(W=W, H=H, frames=run["frames"], max_bounces=run["max_bounces"], knobs=run["knobs"])
+    np.savez_compressed(os.path.join(HERE, "frame_small.npz"), **arrays)
+    with open(os.path.join(HERE, "frame_small.json"), "w") as fh:
+        json.dump(meta, fh, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
     if os.environ.get("LP_NUM_THREADS") != "1":
         sys.exit("run with LP_NUM_THREADS=1 (deterministic llvmpipe dispatch order)")
     if not R.shaders_available():
         sys.exit("reference shaders not found at %s" % R.SHADER_DIR)
-    what = sys.argv[1:] or ["fp", "primary"]
+    what = sys.argv[1:] or ["fp", "primary", "frame"]
     if "fp" in what:
         make_fp()
     if "primary" in what:
         make_primary()
+    if "frame" in what:
+        make_frame()

@@ -1,0 +1,206 @@
+"""Benchmark: Mrays/s of the SVO wavefront frame on one MI355X.
+
+Workload (the depth-10 configuration): 1024^3 octree of the seeded fbm terrain
+shell + 64 seeded spheres (10.4 M voxels), 1920x1080, 1 spp, 2 bounces
+(MAX_BOUNCES = 3), 64-frame seeded orbit at radius 0.8N with temporal
+accumulation (persistent lBuffer).  One step = one whole
+Renderer::runWavefrontFrame: primary rays, normals, schedule, rank, threshold,
+emit, the trace/shade bounce loop and resolve.
+
+value = (primary rays + traced bounce rays + NEE shadow rays) of the timed
+frames / wall time of the timed region (inputs resident in HBM; the host only
+enqueues).  The roofline entry is the primary kernel (the dominant one):
+algorithmic node bytes per launch (4 bytes per node word the reference's
+traversal reads, counted by the instrumented kernel on the same frames) over
+its mean launch time from HIP events recorded on the context's stream during
+the timed region.  cpu_baseline: the C oracle (a scalar port of the same
+passes) on the host, one frame of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+W, H = 1920, 1080
+BOUNCES = 2
+ORBIT_FRAMES = 64
+
+
+def knobs():
+    # reference FrameConfig with the accumulation settings of a still camera
+    # (voxelengine.cpp:201-207), central-difference normals, full budget, one
+    # sample per visible voxel per frame
+    return dict(primary_controlchecks=70, bounce_controlchecks=120, normal_precision=1, shading_budget=196608,
+                weight_pixels=0.0, weight_variance=0.0, top_fraction=0.0, mid_fraction=1.0, min_samples=1,
+                sample_cap_direct=1024, sample_cap_indirect=4096, disable_stale_reset=1, update_time=0xFFFFFFFF)
+
+
+def frame_seed(f):
+    return (1 + 7919 * f) & 0xFFFFFFFF
+
+
+def cpu_baseline(words, depth, ubos, fc_kw, budget_s=20.0):
+    """C oracle (scalar port) on the host: whole frames of the same workload,
+    as many as fit in ~budget_s seconds (at least one)."""
+    import numpy as np
+    from oracle import oracle as O
+    from voxelengine_amd import scenes
+    of = O.Frame(W, H, 16384, BOUNCES + 1)
+    of.set_materials(scenes.palette(1))
+    of.set_sky(np.array([140, 180, 230], np.float32) / np.float32(255))
+    fc = O.FrameConfig(**fc_kw)
+    rays, frames = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        f = frames % len(ubos)
+        of.frame(words, depth, ubos[f], fc, f, frame_seed(f), f + 1)
+        c = of.counters()
+        rays += int(c[6]) + int(c[4]) + int(c[5])
+        frames += 1
+        if time.perf_counter() - t0 > budget_s or frames >= 8:
+            break
+    dt = time.perf_counter() - t0
+    of.close()
+    return dict(value=rays / dt / 1e6, unit="Mrays/s", cores=1, kind="port",
+                sample="%d whole frame(s) of the same 1080p workload, single thread, %.1f s" % (frames, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=128)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X, and the "
+              "multi-GPU form (whole sessions per GPU, octree broadcast over xGMI) is not built; run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    from ve_loader import load_package
+    ve = load_package()
+    from voxelengine_amd import scenes
+
+    t0 = time.perf_counter()
+    depth, ops = scenes.scene_extreme(1)
+    words, info = ve.octree_build(ops, depth)
+    t_build = time.perf_counter() - t0
+    del ops
+
+    r = ve.Renderer(W, H, hash_width=16384, max_bounces=BOUNCES + 1)
+    scene = r.upload(words, depth, info.size)
+    r.set_materials(scenes.palette(1))
+    r.set_sky_constant((140, 180, 230))
+    fc = ve.FrameConfig(**knobs())
+    ubos = []
+    for f in range(ORBIT_FRAMES):
+        eye, d, fov = scenes.orbit_camera(1 << depth, f, ORBIT_FRAMES, 1)
+        ubos.append(ve.camera_ubo(eye, d, fov, W / H))
+
+    def frame(i):
+        f = i % ORBIT_FRAMES
+        r.set_camera(ubos[f])
+        r.frame(scene, fc, ve.FrameInputs(W, H, i, frame_seed(i), i + 1))
+
+    for i in range(args.warmup):
+        frame(i)
+    r.sync()
+    c = r.counters()
+    assert c[ve.CNT["OVERFLOW"]] == 0, "claim-row overflow"
+
+    # timed region: K frames, HIP events per pass on the context's stream
+    r.reset_ray_totals()
+    r.enable_timing(True)
+    torch.cuda.synchronize()
+    r.sync()
+    t0 = time.perf_counter()
+    for i in range(args.warmup, args.warmup + args.steps):
+        frame(i)
+    r.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    pass_ms = r.timings()
+    r.enable_timing(False)
+    totals = r.ray_totals()
+    rays = totals["primary"] + totals["traced"] + totals["shadow"]
+    value = rays / elapsed / 1e6
+
+    # algorithmic node traffic of the primary kernel on the same frames
+    r.enable_node_stats(True)
+    for i in range(args.warmup, args.warmup + min(args.steps, ORBIT_FRAMES)):
+        f = i % ORBIT_FRAMES
+        r.set_camera(ubos[f])
+        r.run_pass(scene, "PRIMARY", fc, ve.FrameInputs(W, H, i, frame_seed(i), i + 1))
+    st = r.node_stats()
+    r.enable_node_stats(False)
+    n_stat = min(args.steps, ORBIT_FRAMES)
+    bytes_per_launch = st["primary_reads"] * 4.0 / n_stat
+    prim_ms = pass_ms["PRIMARY"]
+    achieved = bytes_per_launch / (prim_ms * 1e-3) / 1e9 if prim_ms > 0 else 0.0
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(words, depth, ubos, knobs())
+
+    out = {
+        "metric": "Mrays/s @1080p 10.4M-voxel SVO 2-bounce",
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded fbm terrain shell + 64 spheres, seeded orbit camera)",
+        "config": {
+            "workload": "svo-depth10-extreme-1080p-2bounce-orbit64",
+            "voxels": int(info.num_voxels),
+            "node_words": int(info.size),
+            "resolution": "%dx%d" % (W, H),
+            "bounces": BOUNCES,
+            "spp": 1,
+            "frames_per_s": round(args.steps / elapsed, 2),
+            "rays_per_frame": round(rays / args.steps, 1),
+            "primary_rays_per_frame": W * H,
+            "traced_rays_per_frame": round(totals["traced"] / args.steps, 1),
+            "pass_ms": {k: round(v, 4) for k, v in pass_ms.items()},
+            "scene_build_s": round(t_build, 2),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_primary (generate_primary.comp + RaycastCounted)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes_per_launch": round(bytes_per_launch),
+            "launch_ms": round(prim_ms, 4),
+            "node_reads_per_ray": round(st["primary_reads"] / n_stat / (W * H), 3),
+            "steps_per_ray": round(st["primary_steps"] / n_stat / (W * H), 3),
+        },
+        "cpu_baseline": cpu,
+    }
+    r.release(scene)
+    r.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

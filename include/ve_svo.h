@@ -199,6 +199,9 @@ typedef enum ve_buffer_id {
 #define VE_CNT_SHADOW_RAYS    5   /* NEE shadow rays this frame */
 #define VE_CNT_PRIMARY_RAYS   6   /* W*H of the last primary pass */
 #define VE_CNT_BOUNCES_RUN    7   /* trace/shade iterations with live rays */
+#define VE_CNT_LIVE_B         8   /* live-ray count of the other ray buffer (internal) */
+#define VE_CNT_OVERFLOW       9   /* nonzero: a claim row overflowed the ordering table */
+#define VE_CNT_NEE_PENDING   10   /* nonzero: a hit needed next-event estimation (emissive scenes) */
 
 int ve_read(ve_ctx *ctx, ve_buffer_id id, void *dst, size_t bytes);
 int ve_write(ve_ctx *ctx, ve_buffer_id id, const void *src, size_t bytes);
@@ -207,10 +210,19 @@ int ve_buffer_size(ve_ctx *ctx, ve_buffer_id id, size_t *bytes);
 /* Block until the context's stream is idle. */
 int ve_sync(ve_ctx *ctx);
 
-/* Optional per-pass timing: when enabled, ve_frame records HIP events around
- * each kernel; ve_timings returns milliseconds per pass of the last frame. */
+/* Optional per-pass timing: while enabled, every ve_frame records HIP events
+ * around each pass on the context's stream (no host synchronisation);
+ * ve_timings waits for the stream and returns, per pass, the mean
+ * milliseconds per frame over the frames recorded since the last call
+ * (trace and shade: all bounce iterations of a frame together). */
 int ve_enable_timing(ve_ctx *ctx, int enable);
 int ve_timings(ve_ctx *ctx, float ms_out[VE_PASS_COUNT]);
+
+/* Cumulative ray counts since the context was created (or ve_reset_ray_totals):
+ * primary rays (W*H per primary pass), rays traced by the bounce passes, and
+ * NEE shadow rays.  Kept on the device; reading synchronises the stream. */
+int ve_ray_totals(ve_ctx *ctx, uint64_t *primary, uint64_t *traced, uint64_t *shadow);
+int ve_reset_ray_totals(ve_ctx *ctx);
 
 /* Node-read instrumentation: when enabled, the primary and trace kernels add
  * the number of node words they fetch to a device counter (4 bytes each is the

@@ -35,7 +35,7 @@ PASS_NAMES = [k for k, _ in sorted(PASS.items(), key=lambda kv: kv[1])]
 BUF = dict(GBUFFER=0, GBUFFER_POS=1, NBUFFER=2, LBUFFER=3, RBUFFER=4, CANDIDATES=5, SHADE_LIST=6, RAYS=7,
            OUTPUT=8, COUNTERS=9, RANK_STATS=10, SAMPLE_COUNTS=11, CLAIM_MAP=12, EMISSIVES=13)
 CNT = dict(CANDIDATES=0, SHADE=1, EMISSIVES=2, LIVE=3, RAYS_TRACED=4, SHADOW_RAYS=5, PRIMARY_RAYS=6,
-           BOUNCES_RUN=7)
+           BOUNCES_RUN=7, LIVE_B=8, OVERFLOW=9, NEE_PENDING=10)
 
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("pad0", "<f4"), ("direction", "<f4", 3), ("pad1", "<f4"),
                       ("throughput", "<f4", 3), ("primary_vid", "<u4"), ("direct", "<f4", 3), ("rng", "<u4"),
@@ -126,6 +126,8 @@ def lib():
         "ve_timings": (i32, [p, p]),
         "ve_enable_node_stats": (i32, [p, i32]),
         "ve_node_stats": (i32, [p] + [C.POINTER(C.c_uint64)] * 4),
+        "ve_ray_totals": (i32, [p] + [C.POINTER(C.c_uint64)] * 3),
+        "ve_reset_ray_totals": (i32, [p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -329,6 +331,14 @@ class Renderer:
         self._ck(lib().ve_node_stats(self.h, *[C.byref(x) for x in v]))
         return dict(primary_reads=v[0].value, trace_reads=v[1].value, primary_steps=v[2].value,
                     trace_steps=v[3].value)
+
+    def ray_totals(self):
+        v = [C.c_uint64() for _ in range(3)]
+        self._ck(lib().ve_ray_totals(self.h, *[C.byref(x) for x in v]))
+        return dict(primary=v[0].value, traced=v[1].value, shadow=v[2].value)
+
+    def reset_ray_totals(self):
+        self._ck(lib().ve_reset_ray_totals(self.h))
 
     def enable_timing(self, on=True):
         self._ck(lib().ve_enable_timing(self.h, 1 if on else 0))

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libve_svo.so")
 
-HIP_SOURCES = ["ve_kernels.hip"]
+HIP_SOURCES = ["ve_kernels.hip", "ve_passes.hip"]
 CXX_SOURCES = ["ve_api.cpp", "octree_build.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function"]

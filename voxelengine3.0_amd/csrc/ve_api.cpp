@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -46,6 +47,10 @@ struct ve_ctx {
     float sky[3] = {0, 0, 0};
     bool node_stats = false;
     bool timing = false;
+    uint32_t bounce_iter = 0;           // next trace/shade iteration of per-pass dispatch
+    // timing: one event pair per pass per recorded frame
+    std::vector<std::array<hipEvent_t, 2 * VE_PASS_COUNT>> frame_events;
+    size_t frames_recorded = 0;
     DeviceBuffers b;
     std::vector<void *> allocations;
 };
@@ -254,7 +259,10 @@ int ve_create(ve_ctx **out, const ve_renderer_config *cfg)
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);   // default priority
     if (e != hipSuccess) { ve_global_error(VE_EDEVICE, hipGetErrorString(e)); delete c; return VE_EDEVICE; }
     const size_t P = (size_t)c->max_w * c->max_h;
+    const size_t groups_max = (size_t)((c->max_w + 7) / 8) * ((c->max_h + 7) / 8);
+    if (groups_max * 64 >= (1ull << 31)) { delete c; return ve_global_error(VE_EINVAL, "framebuffer too large"); }
     DeviceBuffers &b = c->b;
+    b.scan_temp_bytes = scan_temp_bytes((uint32_t)(groups_max * 8));
     if ((rc = dalloc(c, &b.gbuf, P)) || (rc = dalloc(c, &b.gpos, P)) || (rc = dalloc(c, &b.output, P)) ||
         (rc = dalloc(c, &b.nbuf, (size_t)c->hash_width * NBUFFER_STRIDE * NBUFFER_SLOTS)) ||
         (rc = dalloc(c, &b.lbuf, (size_t)c->hash_width * LBUFFER_STRIDE * LBUFFER_SLOTS)) ||
@@ -266,7 +274,15 @@ int ve_create(ve_ctx **out, const ve_renderer_config *cfg)
         (rc = dalloc(c, &b.counters, NUM_COUNTERS)) || (rc = dalloc(c, &b.rank_stats, RANK_STATS_WORDS)) ||
         (rc = dalloc(c, &b.emissives, EMISSIVE_CAPACITY)) ||
         (rc = dalloc(c, &b.emissive_claim, (size_t)EMISSIVE_CLAIM_W * EMISSIVE_CLAIM_PROBES)) ||
-        (rc = dalloc(c, &b.materials, MATERIAL_SLOTS)) || (rc = dalloc(c, &b.node_stats, 4))) {
+        (rc = dalloc(c, &b.materials, MATERIAL_SLOTS)) || (rc = dalloc(c, &b.node_stats, 4)) ||
+        (rc = dalloc(c, &b.ray_totals, 3)) ||
+        (rc = dalloc(c, &b.sched_vid, (size_t)CLAIM_WIDTH * SCHED_SLOTS)) ||
+        (rc = dalloc(c, &b.sched_first, (size_t)CLAIM_WIDTH * SCHED_SLOTS)) ||
+        (rc = dalloc(c, &b.sched_cnt, (size_t)CLAIM_WIDTH * SCHED_SLOTS)) ||
+        (rc = dalloc(c, &b.cand_at, groups_max * 64)) || (rc = dalloc(c, &b.vec_cnt, groups_max * 8)) ||
+        (rc = dalloc(c, &b.vec_base, groups_max * 8)) ||
+        (rc = dalloc(c, &b.nsamp, std::min<size_t>(P, CANDIDATE_CAPACITY))) ||
+        (rc = dalloc(c, reinterpret_cast<uint8_t **>(&b.scan_temp), scan_temp_bytes((uint32_t)(groups_max * 8))))) {
         std::string msg = c->err;
         ve_destroy(c);
         return ve_global_error(rc, msg.c_str());
@@ -283,11 +299,13 @@ int ve_create(ve_ctx **out, const ve_renderer_config *cfg)
 void ve_destroy(ve_ctx *c)
 {
     if (!c) return;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
-    for (void *p : c->allocations) hipFree(p);
-    if (c->b.sort_temp) hipFree(c->b.sort_temp);
-    if (c->stream) hipStreamDestroy(c->stream);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->allocations) (void)hipFree(p);
+    for (auto &fe : c->frame_events)
+        for (hipEvent_t e : fe)
+            if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -311,7 +329,7 @@ int ve_scene_upload(ve_ctx *c, const uint32_t *words, uint32_t n_words, uint32_t
     if (e != hipSuccess) { delete s; return ctx_error(c, VE_ENOMEM, "hipMalloc node array: %s", hipGetErrorString(e)); }
     e = hipMemcpyAsync(d, words, (size_t)n_words * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) { hipFree(d); delete s; return ctx_error(c, VE_EDEVICE, "node upload: %s", hipGetErrorString(e)); }
+    if (e != hipSuccess) { (void)hipFree(d); delete s; return ctx_error(c, VE_EDEVICE, "node upload: %s", hipGetErrorString(e)); }
     s->words = static_cast<uint32_t *>(d);
     s->n_words = n_words;
     s->depth = depth;
@@ -323,9 +341,9 @@ int ve_scene_release(ve_ctx *c, ve_scene *s)
 {
     if (!c) return ve_global_error(VE_EINVAL, "null context");
     if (!s) return ctx_error(c, VE_ENOTFOUND, "null scene");
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
-    hipFree(s->words);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(s->words);
     delete s;
     return VE_OK;
 }
@@ -366,6 +384,72 @@ int ve_set_sky(ve_ctx *c, const uint8_t *const faces[6], int edge)
     return VE_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Enqueue one pass; every launch goes on the context's stream.
+int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p)
+{
+    int rc = VE_OK;
+    switch (pass) {
+    case VE_PASS_PRIMARY:
+        launch_primary(c->stream, s->words, p, c->b, c->node_stats);
+        launch_count_primary(c->stream, c->b, p.W * p.H);
+        break;
+    case VE_PASS_NORMAL: launch_normal(c->stream, s->words, p, c->b); break;
+    case VE_PASS_RESET: launch_reset(c->stream, c->b); break;
+    case VE_PASS_SCHEDULE: rc = launch_schedule(c->stream, p, c->b); break;
+    case VE_PASS_RANK: rc = launch_rank(c->stream, p, c->b); break;
+    case VE_PASS_THRESHOLD: launch_threshold(c->stream, p, c->b); break;
+    case VE_PASS_EMIT:
+        launch_emit(c->stream, p, c->b);
+        c->bounce_iter = 0;
+        break;
+    case VE_PASS_TRACE:
+        if (c->bounce_iter >= 8u) return ctx_error(c, VE_ESTATE, "trace after 8 bounce iterations (emit first)");
+        launch_trace(c->stream, s->words, p, c->b, c->bounce_iter, c->node_stats);
+        break;
+    case VE_PASS_SHADE:
+        if (c->bounce_iter >= 8u) return ctx_error(c, VE_ESTATE, "shade after 8 bounce iterations (emit first)");
+        launch_shade(c->stream, p, c->b, c->bounce_iter);
+        c->bounce_iter++;
+        break;
+    case VE_PASS_RESOLVE: launch_resolve(c->stream, p, c->b); break;
+    default: return ctx_error(c, VE_EINVAL, "unknown pass %d", (int)pass);
+    }
+    if (rc) return ctx_error(c, rc, "pass %d: device library call failed", (int)pass);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return ctx_error(c, VE_EDEVICE, "pass %d launch: %s", (int)pass, hipGetErrorString(e));
+    return VE_OK;
+}
+
+// Event slot of the frame being recorded (timing mode only).
+hipEvent_t *frame_slot(ve_ctx *c)
+{
+    if (c->frames_recorded >= c->frame_events.size()) {
+        std::array<hipEvent_t, 2 * VE_PASS_COUNT> fe{};
+        for (auto &e : fe)
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->frame_events.push_back(fe);
+    }
+    return c->frame_events[c->frames_recorded].data();
+}
+
+int timed_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, bool first_of_kind, hipEvent_t *ev)
+{
+    if (!ev) return run_pass(c, s, pass, p);
+    // bounce passes repeat: time from the first launch of the kind to the last
+    if (first_of_kind) HIP_OK(c, hipEventRecord(ev[2 * pass], c->stream));
+    int rc = run_pass(c, s, pass, p);
+    HIP_OK(c, hipEventRecord(ev[2 * pass + 1], c->stream));
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
 int ve_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const ve_frame_config *fc, const ve_frame_inputs *in)
 {
     int rc = check_frame(c, s, fc, in);
@@ -373,22 +457,32 @@ int ve_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const ve_frame_config *fc, 
     HIP_OK(c, hipSetDevice(c->device));
     FrameParams p;
     fill_params(c, s, fc, in, p);
-    switch (pass) {
-    case VE_PASS_PRIMARY:
-        launch_primary(c->stream, s->words, p, c->b, c->node_stats);
-        break;
-    default:
-        return ctx_error(c, VE_EINVAL, "pass %d not available", (int)pass);
-    }
-    HIP_OK(c, hipGetLastError());
-    return VE_OK;
+    return run_pass(c, s, pass, p);
 }
 
 int ve_frame(ve_ctx *c, ve_scene *s, const ve_frame_config *fc, const ve_frame_inputs *in)
 {
     int rc = check_frame(c, s, fc, in);
     if (rc) return rc;
-    return ctx_error(c, VE_ESTATE, "whole-frame sequence not available");
+    HIP_OK(c, hipSetDevice(c->device));
+    FrameParams p;
+    fill_params(c, s, fc, in, p);
+    // renderer.cpp:282-649 without its host readbacks: counts stay on the
+    // device and every kernel sizes its own work from them.
+    hipEvent_t *ev = nullptr;
+    if (c->timing && !(ev = frame_slot(c))) return ctx_error(c, VE_EDEVICE, "hipEventCreate failed");
+    const ve_pass_id head[] = {VE_PASS_RESET, VE_PASS_PRIMARY, VE_PASS_NORMAL, VE_PASS_SCHEDULE,
+                               VE_PASS_RANK, VE_PASS_THRESHOLD, VE_PASS_EMIT};
+    for (ve_pass_id id : head)
+        if ((rc = timed_pass(c, s, id, p, true, ev))) return rc;
+    const uint32_t iters = std::min<uint32_t>(c->max_bounces, 8u);   // host loop cap, renderer.cpp:515
+    for (uint32_t i = 0; i < iters; i++) {
+        if ((rc = timed_pass(c, s, VE_PASS_TRACE, p, i == 0, ev))) return rc;
+        if ((rc = timed_pass(c, s, VE_PASS_SHADE, p, i == 0, ev))) return rc;
+    }
+    rc = timed_pass(c, s, VE_PASS_RESOLVE, p, true, ev);
+    if (ev) c->frames_recorded++;
+    return rc;
 }
 
 int ve_buffer_size(ve_ctx *c, ve_buffer_id id, size_t *bytes)
@@ -436,6 +530,7 @@ int ve_enable_timing(ve_ctx *c, int enable)
 {
     if (!c) return ve_global_error(VE_EINVAL, "null context");
     c->timing = enable != 0;
+    c->frames_recorded = 0;
     return VE_OK;
 }
 
@@ -443,7 +538,40 @@ int ve_timings(ve_ctx *c, float ms_out[VE_PASS_COUNT])
 {
     if (!c) return ve_global_error(VE_EINVAL, "null context");
     if (!ms_out) return ctx_error(c, VE_EINVAL, "null output");
-    for (int i = 0; i < VE_PASS_COUNT; i++) ms_out[i] = 0.0f;
+    HIP_OK(c, hipSetDevice(c->device));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    double sum[VE_PASS_COUNT] = {0};
+    for (size_t f = 0; f < c->frames_recorded; f++)
+        for (int i = 0; i < VE_PASS_COUNT; i++) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, c->frame_events[f][2 * i], c->frame_events[f][2 * i + 1]) == hipSuccess)
+                sum[i] += ms;
+        }
+    for (int i = 0; i < VE_PASS_COUNT; i++)
+        ms_out[i] = c->frames_recorded ? (float)(sum[i] / (double)c->frames_recorded) : 0.0f;
+    c->frames_recorded = 0;
+    return VE_OK;
+}
+
+int ve_ray_totals(ve_ctx *c, uint64_t *primary, uint64_t *traced, uint64_t *shadow)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    unsigned long long v[3];
+    HIP_OK(c, hipSetDevice(c->device));
+    HIP_OK(c, hipMemcpyAsync(v, c->b.ray_totals, sizeof v, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    if (primary) *primary = v[0];
+    if (traced) *traced = v[1];
+    if (shadow) *shadow = v[2];
+    return VE_OK;
+}
+
+int ve_reset_ray_totals(ve_ctx *c)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    HIP_OK(c, hipSetDevice(c->device));
+    HIP_OK(c, hipMemsetAsync(c->b.ray_totals, 0, 3 * sizeof(unsigned long long), c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
     return VE_OK;
 }
 

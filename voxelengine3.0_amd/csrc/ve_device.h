@@ -148,3 +148,154 @@ __device__ __forceinline__ Hit raycast(const uint32_t *__restrict__ words, uint3
 }
 
 }  // namespace ve
+
+namespace ve {
+
+// ---- GLSL built-ins with llvmpipe's composition -------------------------------
+// log = log2 * ln2, exp = exp2(x * log2e), pow = exp2(log2(x) * y); the log2 /
+// exp2 / sin / cos cores are evaluated in double and rounded once, which is
+// the same definition the C oracle uses.
+__device__ __forceinline__ float m_log2(float x) { return (float)log2((double)x); }
+__device__ __forceinline__ float m_exp2(float x) { return (float)exp2((double)x); }
+__device__ __forceinline__ float m_log(float x) { return m_log2(x) * 0.6931471805599453f; }
+__device__ __forceinline__ float m_exp(float x) { return m_exp2(x * 1.4426950408889634f); }
+__device__ __forceinline__ float m_pow(float x, float y) { return m_exp2(m_log2(x) * y); }
+__device__ __forceinline__ float m_cos(float x) { return (float)cos((double)x); }
+__device__ __forceinline__ float m_sin(float x) { return (float)sin((double)x); }
+__device__ __forceinline__ float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
+__device__ __forceinline__ float gsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+__device__ __forceinline__ float length3(f3 a) { return __fsqrt_rn(dot(a, a)); }
+__device__ __forceinline__ f3 divs(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ f3 vmin3(f3 a, f3 b) { return f3{gmin(a.x, b.x), gmin(a.y, b.y), gmin(a.z, b.z)}; }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b)
+{
+    return f3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+
+// ---- Halton sampler (internal.glsl:57-89) -------------------------------------
+__device__ __forceinline__ float halton(uint32_t i, uint32_t b)
+{
+    float f = 1.0f, r = 0.0f;
+    for (int k = 0; k < 24 && i > 0u; k++) {
+        f /= (float)b;
+        const uint32_t q = i / b;
+        r += f * (float)(i - q * b);
+        i = q;
+    }
+    return r;
+}
+
+__device__ __forceinline__ uint32_t halton_base(uint32_t k)
+{
+    // first 16 primes (internal.glsl:57-60)
+    switch (k & 15u) {
+    case 0: return 2u;   case 1: return 3u;   case 2: return 5u;   case 3: return 7u;
+    case 4: return 11u;  case 5: return 13u;  case 6: return 17u;  case 7: return 19u;
+    case 8: return 23u;  case 9: return 29u;  case 10: return 31u; case 11: return 37u;
+    case 12: return 41u; case 13: return 43u; case 14: return 47u; default: return 53u;
+    }
+}
+
+__device__ __forceinline__ float rand_lds(uint32_t &state)
+{
+    const uint32_t dim = state & 0xFFu;
+    const uint32_t sample_idx = state >> 8;
+    state = (sample_idx << 8) | ((dim + 1u) & 0xFFu);
+    const uint32_t effective = sample_idx ^ ((dim >> 4) * 0x9E3779B9u);
+    return halton(effective, halton_base(dim));
+}
+
+__device__ __forceinline__ float rand_normal(uint32_t &state)   // internal.glsl:101-105
+{
+    const float theta = (2.0f * 3.1415926f) * rand_lds(state);
+    const float rho = __fsqrt_rn(-2.0f * m_log(rand_lds(state)));
+    return rho * m_cos(theta);
+}
+
+__device__ __forceinline__ f3 random_direction(uint32_t &state)   // internal.glsl:107-112
+{
+    const float x = rand_normal(state);
+    const float y = rand_normal(state);
+    const float z = rand_normal(state);
+    return normalize(f3{x, y, z});
+}
+
+// EvaluateBRDF (internal.glsl:120-149)
+__device__ __forceinline__ f3 evaluate_brdf(f3 Vd, f3 L, f3 N, f3 albedo, float roughness, float metallic,
+                                            float spec_int)
+{
+    const float NdotL = gmax(dot(N, L), 0.001f);
+    if (NdotL < 0.001f) return f3{0, 0, 0};
+    const float rc = gmax(roughness, 0.01f);
+    const float spec_exp = 2.0f / (rc * rc);
+    const f3 H = normalize(sub(L, Vd));
+    const float NdotH = gmax(dot(N, H), 0.001f);
+    const float spec_norm = (spec_exp + 2.0f) * 0.15915494f;
+    const float specular = spec_norm * m_pow(NdotH, spec_exp);
+    const float kS = spec_int * metallic;
+    const float kD = 1.0f - kS * 0.5f;
+    const f3 diffuse = muls(muls(albedo, kD), 0.31831f);
+    const float sp = specular * kS;
+    return muls(add(diffuse, f3{sp, sp, sp}), NdotL);
+}
+
+__device__ __forceinline__ float smith_g1(float NdotX, float a)
+{
+    const float a2 = a * a;
+    return 2.0f * NdotX / (NdotX + __fsqrt_rn(a2 + (1.0f - a2) * NdotX * NdotX));
+}
+
+// sampleGGXVNDF (internal.glsl:182-219)
+__device__ __forceinline__ f3 sample_ggx_vndf(f3 N, f3 Vv, float roughness, uint32_t &state)
+{
+    const float a = gmax(roughness * roughness, 0.001f);
+    const f3 up = fabsf(N.z) < 0.999f ? f3{0, 0, 1} : f3{1, 0, 0};
+    const f3 T = normalize(cross3(up, N));
+    const f3 B = cross3(N, T);
+    const f3 Vl = f3{dot(Vv, T), dot(Vv, B), dot(Vv, N)};
+    const f3 Vh = normalize(f3{a * Vl.x, a * Vl.y, Vl.z});
+    const float lensq = Vh.x * Vh.x + Vh.y * Vh.y;
+    const f3 T1 = lensq > 0.0f ? muls(f3{-Vh.y, Vh.x, 0.0f}, 1.0f / __fsqrt_rn(lensq)) : f3{1, 0, 0};
+    const f3 T2 = cross3(Vh, T1);
+    const float r1 = rand_lds(state);
+    const float r2 = rand_lds(state);
+    const float r = __fsqrt_rn(r1);
+    const float phi = 6.2831853f * r2;
+    const float t1 = r * m_cos(phi);
+    float t2 = r * m_sin(phi);
+    const float sc = 0.5f * (1.0f + Vh.z);
+    t2 = (1.0f - sc) * __fsqrt_rn(gmax(1.0f - t1 * t1, 0.0f)) + sc * t2;
+    const f3 Nh = add(add(muls(T1, t1), muls(T2, t2)), muls(Vh, __fsqrt_rn(gmax(0.0f, 1.0f - t1 * t1 - t2 * t2))));
+    const f3 Hl = normalize(f3{a * Nh.x, a * Nh.y, gmax(0.0f, Nh.z)});
+    return add(add(muls(T, Hl.x), muls(B, Hl.y)), muls(N, Hl.z));
+}
+
+__device__ __forceinline__ f3 ggx_throughput(f3 N, f3 Vv, f3 L, f3 H, float roughness, f3 F0)
+{
+    const float NdotL = gmax(dot(N, L), 0.001f);
+    const float VdotH = gmax(dot(Vv, H), 0.001f);
+    const float a = roughness * roughness;
+    const float p5 = m_pow(1.0f - VdotH, 5.0f);
+    const f3 F = add(F0, muls(sub(f3{1, 1, 1}, F0), p5));
+    return muls(F, smith_g1(NdotL, a));
+}
+
+__device__ __forceinline__ f3 reflect3(f3 I, f3 N) { return sub(I, muls(N, 2.0f * dot(N, I))); }
+
+__device__ __forceinline__ f3 unpack_normal(uint32_t p)
+{
+    return f3{(float)((p >> 22) & 0x3FFu) / 511.0f - 1.0f, (float)((p >> 12) & 0x3FFu) / 511.0f - 1.0f,
+              (float)((p >> 2) & 0x3FFu) / 511.0f - 1.0f};
+}
+
+__device__ __forceinline__ uint32_t claim_hash(uint32_t v)   // schedule.comp:36-43
+{
+    v ^= v >> 16;
+    v *= 0x7feb352du;
+    v ^= v >> 15;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t bit_reverse(uint32_t x) { return __brev(x); }
+
+}  // namespace ve

@@ -32,6 +32,7 @@ constexpr uint32_t RR_START_BOUNCE = 2;        // wavefront.glsl:112
 constexpr uint32_t TRUSTED_THRESHOLD = 128;    // resolve.comp:82
 constexpr uint32_t RANK_STATS_WORDS = 264;     // wavefront.glsl:243-255
 constexpr uint32_t NUM_COUNTERS = 16;
+constexpr uint32_t SCHED_SLOTS = 128;          // ordering table: voxels per claim row before overflow
 constexpr uint32_t MATERIAL_SLOTS = 127;       // shade.comp:90
 
 // Ray record, 96 bytes, field for field the reference's std430 Ray

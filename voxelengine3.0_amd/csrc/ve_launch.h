@@ -28,16 +28,32 @@ struct DeviceBuffers {
     uint32_t *emissive_claim = nullptr;  // EMISSIVE_CLAIM_W x EMISSIVE_CLAIM_PROBES
     Material *materials = nullptr;       // MATERIAL_SLOTS
     unsigned long long *node_stats = nullptr;  // 4: primary reads, trace reads, primary steps, trace steps
-    // schedule ordering scratch
-    uint32_t *first_rank = nullptr;      // per claim slot: first claimant's execution rank
-    uint64_t *order_keys = nullptr;      // CANDIDATE_CAPACITY sort keys
-    uint64_t *order_keys_alt = nullptr;
-    uint32_t *order_vals = nullptr;
-    uint32_t *order_vals_alt = nullptr;
-    void *sort_temp = nullptr;
-    size_t sort_temp_bytes = 0;
+    unsigned long long *ray_totals = nullptr;  // 3: primary, traced, shadow (cumulative)
+    // schedule / rank ordering scratch
+    uint32_t *sched_vid = nullptr;       // CLAIM_WIDTH x SCHED_SLOTS voxel ids
+    uint32_t *sched_first = nullptr;     // first claimant's execution rank
+    uint32_t *sched_cnt = nullptr;       // pixel count
+    uint32_t *cand_at = nullptr;         // per execution rank: claim row/probe of the voxel it first claims
+    uint32_t *vec_cnt = nullptr;         // per SIMD-8 vector: entries appended
+    uint32_t *vec_base = nullptr;        // exclusive scan of vec_cnt
+    int32_t *nsamp = nullptr;            // per candidate: samples
+    void *scan_temp = nullptr;
+    size_t scan_temp_bytes = 0;
 };
 
 void launch_primary(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, bool stats);
+void launch_fill(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
+void launch_normal(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b);
+void launch_reset(hipStream_t s, DeviceBuffers &b);
+void launch_count_primary(hipStream_t s, DeviceBuffers &b, uint32_t n);
+int launch_schedule(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
+int launch_rank(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
+void launch_emit(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
+void launch_threshold(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
+void launch_trace(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t iter,
+                  bool stats);
+void launch_shade(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_t iter);
+void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
+size_t scan_temp_bytes(uint32_t max_items);
 
 }  // namespace ve

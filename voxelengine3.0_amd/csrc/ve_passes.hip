@@ -1,0 +1,1001 @@
+// gfx950 kernels for the wavefront passes after the primary pass:
+//   normal.comp     -> k_normal
+//   schedule.comp   -> k_sched_clear, k_sched_claim, k_sched_rank, k_sched_vec_count, (scan), k_sched_write
+//   rank.comp       -> k_rank, (scan), k_rank_write
+//   threshold.comp  -> k_threshold
+//   emit.comp       -> k_emit
+//   trace.comp      -> k_trace
+//   shade.comp      -> k_shade
+//   resolve.comp    -> k_resolve
+//
+// Ordering.  The reference appends candidates and shade-list entries with
+// atomic counters, so their order is whatever order the driver runs the
+// invocations in.  These kernels reproduce one fixed order — that of a
+// single-threaded llvmpipe run (work groups in dispatch order, invocations in
+// SIMD-8 vectors, lockstep loop iterations) — without serialising anything:
+// every pixel gets its execution rank e, a voxel's representative pixel is its
+// lowest-ranked pixel, a voxel's claim-map probe index is its rank among the
+// voxels of the same claim row, and positions in the candidate and shade lists
+// come from exclusive scans over SIMD-8 vectors.  The result is deterministic
+// and equal to the reference's output on the same inputs.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "ve_device.h"
+#include "ve_launch.h"
+
+namespace ve {
+
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr int TILE = 8;
+constexpr int NUM_XCD = 8;
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t xcd_banded(uint32_t b, uint32_t nb_padded)
+{
+    const uint32_t per = nb_padded / NUM_XCD;
+    return (b % NUM_XCD) * per + b / NUM_XCD;
+}
+
+__device__ __forceinline__ bool tile_pixel(const FrameParams &p, uint32_t &x, uint32_t &y)
+{
+    const uint32_t tiles_x = (p.W + TILE - 1) / TILE;
+    const uint32_t tiles_y = (p.H + TILE - 1) / TILE;
+    const uint32_t tile = xcd_banded(blockIdx.x, gridDim.x) * 4u + (threadIdx.x >> 6);
+    if (tile >= tiles_x * tiles_y) return false;
+    const uint32_t lane = threadIdx.x & 63u;
+    x = (tile % tiles_x) * TILE + (lane & 7u);
+    y = (tile / tiles_x) * TILE + (lane >> 3);
+    return x < p.W && y < p.H;
+}
+
+__global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// normal.comp
+// ---------------------------------------------------------------------------
+__device__ bool lookup_occupied(const uint32_t *__restrict__ w, uint32_t n, uint32_t depth, uint32_t x, uint32_t y,
+                                uint32_t z, uint32_t len)   // normal.comp:55-70
+{
+    if (x >= len || y >= len || z >= len) return false;
+    uint32_t offset = 0, raw;
+    for (uint32_t d = 0; d < depth; d++) {
+        offset += locate(x, y, z, len >> d);
+        raw = offset < n ? w[offset] : 0u;
+        if (!(raw & 1u)) return ((raw >> 1) & 0x7Fu) != 0u;
+        offset = (raw >> 4) & 0x0FFFFFFFu;
+    }
+    offset += locate(x, y, z, len >> depth);
+    raw = offset < n ? w[offset] : 0u;
+    return ((raw >> 1) & 0x7Fu) != 0u;
+}
+
+__device__ f3 refined_normal(const uint32_t *__restrict__ w, uint32_t n, uint32_t depth, uint32_t hx, uint32_t hy,
+                             uint32_t hz, uint32_t len, int r)   // normal.comp:73-87
+{
+    f3 acc{0.0f, 0.0f, 0.0f};
+    for (int dx = -r; dx <= r; dx++)
+        for (int dy = -r; dy <= r; dy++)
+            for (int dz = -r; dz <= r; dz++) {
+                if (dx == 0 && dy == 0 && dz == 0) continue;
+                if (dx * dx + dy * dy + dz * dz > r * r) continue;
+                const int nx = (int)hx + dx, ny = (int)hy + dy, nz = (int)hz + dz;
+                const bool oob = nx < 0 || ny < 0 || nz < 0 || nx >= (int)len || ny >= (int)len || nz >= (int)len;
+                const bool empty = oob || !lookup_occupied(w, n, depth, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, len);
+                if (empty) acc = add(acc, f3{(float)dx, (float)dy, (float)dz});
+            }
+    return length3(acc) > 0.001f ? normalize(acc) : f3{0.0f, 1.0f, 0.0f};
+}
+
+__device__ __forceinline__ uint32_t pack_normal(f3 v)   // normal.comp:90-95, round = half-to-even
+{
+    int x = (int)rintf((v.x + 1.0f) * 511.0f), y = (int)rintf((v.y + 1.0f) * 511.0f), z = (int)rintf((v.z + 1.0f) * 511.0f);
+    x = min(max(x, 0), 1022);
+    y = min(max(y, 0), 1022);
+    z = min(max(z, 0), 1022);
+    return ((uint32_t)x << 22) | ((uint32_t)y << 12) | ((uint32_t)z << 2);
+}
+
+__global__ __launch_bounds__(256) void k_normal(const uint32_t *__restrict__ words, FrameParams p,
+                                                const uint4 *__restrict__ gbuf, const uint32_t *__restrict__ gpos,
+                                                uint32_t *__restrict__ nbuf)
+{
+    uint32_t x, y;
+    if (!tile_pixel(p, x, y)) return;
+    const size_t pix = (size_t)y * p.W + x;
+    const uint32_t vid = gbuf[pix].w;
+    if (vid == 0u) return;
+    const uint32_t len = 1u << p.depth;
+    const uint32_t pk = gpos[pix], bits = p.depth - 1u, mask = (1u << bits) - 1u;
+    const uint32_t hx = (pk & mask) * 2u, hy = ((pk >> bits) & mask) * 2u, hz = ((pk >> (2u * bits)) & mask) * 2u;
+    const uint32_t h = vid % (p.hash_width - 1u) + 1u;
+    uint32_t *row = nbuf + (size_t)h * (NBUFFER_STRIDE * NBUFFER_SLOTS);
+    for (uint32_t s = 0; s < NBUFFER_SLOTS; s++) {
+        const uint32_t prev = atomicCAS(&row[s * NBUFFER_STRIDE], 0u, vid);
+        if (prev == 0u || prev == vid) {
+            // The first claimant computes the normal; a later claimant that finds
+            // the field still empty would compute the same value (normal.comp:151-156).
+            if (prev == 0u || __atomic_load_n(&row[s * NBUFFER_STRIDE + 1], __ATOMIC_RELAXED) == 0u) {
+                const f3 nrm = refined_normal(words, p.n_words, p.depth, hx, hy, hz, len, p.normal_precision);
+                atomicCAS(&row[s * NBUFFER_STRIDE + 1], 0u, pack_normal(nrm));
+            }
+            atomicExch(&row[s * NBUFFER_STRIDE + 2], p.time);
+            return;
+        }
+    }
+    // bucket full: take the least recently used slot (normal.comp:162-176)
+    uint32_t oldest = 0xFFFFFFFFu, oslot = 0;
+    for (uint32_t s = 0; s < NBUFFER_SLOTS; s++) {
+        const uint32_t t = __atomic_load_n(&row[s * NBUFFER_STRIDE + 2], __ATOMIC_RELAXED);
+        if (t < oldest) { oldest = t; oslot = s; }
+    }
+    atomicExch(&row[oslot * NBUFFER_STRIDE], vid);
+    const f3 nrm = refined_normal(words, p.n_words, p.depth, hx, hy, hz, len, p.normal_precision);
+    atomicExch(&row[oslot * NBUFFER_STRIDE + 1], pack_normal(nrm));
+    atomicExch(&row[oslot * NBUFFER_STRIDE + 2], p.time);
+}
+
+// ---------------------------------------------------------------------------
+// schedule.comp
+// ---------------------------------------------------------------------------
+// Work group g of the 8x8 dispatch handles the bit-reversed tile of
+// schedule.comp:58-71; its invocation l has execution rank e = g*64 + l.
+__device__ __forceinline__ uint32_t sched_tile(uint32_t g, uint32_t total, uint32_t seed)
+{
+    const uint32_t log2g = total <= 1u ? 1u : (32u - __clz(total - 1u));
+    uint32_t sc = bit_reverse(g ^ seed) >> (32u - log2g);
+    return sc >= total ? g : sc;
+}
+
+__device__ __forceinline__ uint32_t sched_pixel(uint32_t e, const FrameParams &p, uint32_t &pix)
+{
+    const uint32_t gw = (p.W + 7u) / 8u, gh = (p.H + 7u) / 8u;
+    const uint32_t tile = sched_tile(e >> 6, gw * gh, p.frame_seed);
+    const uint32_t x = (tile % gw) * 8u + (e & 7u), y = (tile / gw) * 8u + ((e >> 3) & 7u);
+    pix = y * p.W + x;
+    return x < p.W && y < p.H;
+}
+
+// One wave per work group g; lanes with equal voxel ids are merged first so
+// one atomic per distinct id per wave reaches memory.
+__global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *__restrict__ gbuf,
+                                                    uint32_t *__restrict__ tab_vid, uint32_t *__restrict__ tab_first,
+                                                    uint32_t *__restrict__ tab_cnt, uint32_t *__restrict__ cand_at,
+                                                    uint32_t *__restrict__ counters)
+{
+    const uint32_t e = blockIdx.x * 64u + threadIdx.x;
+    cand_at[e] = NONE;
+    uint32_t pix;
+    uint32_t vid = 0u;
+    if (sched_pixel(e, p, pix)) vid = gbuf[pix].w;
+    uint64_t pending = __ballot(vid != 0u);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lv = __shfl(vid, leader, 64);
+        const uint64_t same = __ballot(vid == lv && vid != 0u);
+        pending &= ~same;
+        if ((int)threadIdx.x == leader) {
+            // lowest lane of the group = lowest execution rank of this id here
+            const uint32_t h = claim_hash(lv) % CLAIM_WIDTH;
+            uint32_t *rv = tab_vid + (size_t)h * SCHED_SLOTS;
+            bool placed = false;
+            for (uint32_t s = 0; s < SCHED_SLOTS; s++) {
+                const uint32_t prev = atomicCAS(&rv[s], 0u, lv);
+                if (prev == 0u || prev == lv) {
+                    atomicMin(&tab_first[(size_t)h * SCHED_SLOTS + s], e);
+                    atomicAdd(&tab_cnt[(size_t)h * SCHED_SLOTS + s], (uint32_t)__popcll(same));
+                    placed = true;
+                    break;
+                }
+            }
+            if (!placed) atomicOr(&counters[VE_CNT_OVERFLOW], 1u);
+        }
+    }
+}
+
+// One wave per claim row: a voxel's probe index in the reference's claim map is
+// the number of voxels of its row claimed before it (lower execution rank).
+// Writes the reference-layout claim and pixel-count maps (all 64 probes, so
+// no separate clear is needed) and marks each kept voxel's first pixel.
+__global__ __launch_bounds__(64) void k_sched_rank(const uint32_t *__restrict__ tab_vid,
+                                                   const uint32_t *__restrict__ tab_first,
+                                                   const uint32_t *__restrict__ tab_cnt, uint32_t *__restrict__ claim,
+                                                   uint32_t *__restrict__ samples, uint32_t *__restrict__ cand_at)
+{
+    __shared__ uint32_t firsts[SCHED_SLOTS];
+    const uint32_t h = blockIdx.x, lane = threadIdx.x;
+    const size_t base = (size_t)h * SCHED_SLOTS;
+    uint32_t v[2], f[2], c[2];
+    for (int k = 0; k < 2; k++) {
+        const uint32_t s = lane + 64u * k;
+        v[k] = tab_vid[base + s];
+        f[k] = tab_first[base + s];
+        c[k] = tab_cnt[base + s];
+        firsts[s] = v[k] ? f[k] : NONE;
+    }
+    __syncthreads();
+    const uint32_t n = (uint32_t)__popcll(__ballot(v[0] != 0u)) + (uint32_t)__popcll(__ballot(v[1] != 0u));
+    // slots fill contiguously from probe 0, so [0, n) are occupied
+    uint32_t out_v = 0u, out_c = 0u;
+    for (int k = 0; k < 2; k++) {
+        const uint32_t s = lane + 64u * k;
+        if (s >= n) continue;
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < n; j++) rank += firsts[j] < f[k];
+        if (rank < CLAIM_PROBES) {
+            claim[(size_t)h * CLAIM_PROBES + rank] = v[k];
+            samples[(size_t)h * CLAIM_PROBES + rank] = c[k];
+            cand_at[f[k]] = (rank << 26) | (h << 6) | rank;
+        }
+    }
+    (void)out_v;
+    (void)out_c;
+    // probes past the row's population read 0, as after the per-frame clear
+    for (uint32_t s = n + lane; s < CLAIM_PROBES; s += 64u) {
+        claim[(size_t)h * CLAIM_PROBES + s] = 0u;
+        samples[(size_t)h * CLAIM_PROBES + s] = 0u;
+    }
+}
+
+// candidates first-claimed in each SIMD-8 vector
+__global__ void k_sched_vec_count(const uint32_t *__restrict__ cand_at, uint32_t n_vec, uint32_t *__restrict__ vec_cnt)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n_vec) return;
+    uint32_t c = 0;
+    for (uint32_t l = 0; l < 8; l++) c += cand_at[v * 8u + l] != NONE;
+    vec_cnt[v] = c;
+}
+
+// Inside a vector, wins happen probe by probe (lockstep loop) and, within a
+// probe, in lane order: sort the vector's <= 8 winners by (probe, lane).
+__global__ void k_sched_write(FrameParams p, const uint32_t *__restrict__ cand_at, uint32_t n_vec,
+                              const uint32_t *__restrict__ vec_cnt, const uint32_t *__restrict__ vec_base,
+                              const uint32_t *__restrict__ claim, uint4 *__restrict__ candidates,
+                              uint32_t *__restrict__ counters)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n_vec) return;
+    if (v == n_vec - 1u) counters[VE_CNT_CANDIDATES] = min(vec_base[v] + vec_cnt[v], CANDIDATE_CAPACITY);
+    uint32_t key[8], lane_of[8];
+    uint32_t m = 0;
+    for (uint32_t l = 0; l < 8; l++) {
+        const uint32_t a = cand_at[v * 8u + l];
+        if (a == NONE) continue;
+        uint32_t k = ((a >> 26) << 3) | l;   // (probe, lane)
+        uint32_t j = m++;
+        while (j > 0 && key[j - 1] > k) {
+            key[j] = key[j - 1];
+            lane_of[j] = lane_of[j - 1];
+            j--;
+        }
+        key[j] = k;
+        lane_of[j] = l;
+    }
+    uint32_t pos = vec_base[v];
+    for (uint32_t j = 0; j < m; j++, pos++) {
+        if (pos >= CANDIDATE_CAPACITY) break;
+        const uint32_t e = v * 8u + lane_of[j];
+        const uint32_t a = cand_at[e];
+        const uint32_t h = (a >> 6) & 0xFFFFFu, probe = a & 63u;
+        uint32_t pix;
+        sched_pixel(e, p, pix);
+        candidates[pos] = make_uint4(claim[(size_t)h * CLAIM_PROBES + probe], pix, h, probe);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// rank.comp
+// ---------------------------------------------------------------------------
+__device__ float indirect_variance(const uint32_t *__restrict__ lbuf, uint32_t hw, uint32_t vid)   // rank.comp:66-87
+{
+    const uint32_t h = vid % (hw - 1u) + 1u;
+    const uint32_t *row = lbuf + (size_t)h * (LBUFFER_STRIDE * LBUFFER_SLOTS);
+    for (uint32_t s = 0; s < LBUFFER_SLOTS; s++) {
+        const uint32_t *sl = row + s * LBUFFER_STRIDE;
+        const uint32_t owner = sl[0];
+        if (owner == 0u) return 0.0f;
+        if (owner != vid) continue;
+        const uint32_t c = sl[6];
+        if (c < 2u) return 0.0f;
+        const float fc = (float)c;
+        const float meanL = (float)(sl[7] + sl[8] + sl[9]) / (3.0f * fc);
+        const float meanL2 = (float)sl[10] / fc;
+        return gmax(meanL2 - meanL * meanL, 0.0f);
+    }
+    return 0.0f;
+}
+
+// Per candidate: samples (0/1/2/minSamples), histogram, min/max; per SIMD-8
+// vector: the number of shade-list entries it appends.
+__global__ __launch_bounds__(256) void k_rank(FrameParams p, const uint4 *__restrict__ candidates,
+                                              const uint32_t *__restrict__ samples_map, const uint32_t *__restrict__ lbuf,
+                                              uint32_t *__restrict__ rank_stats, const uint32_t *__restrict__ counters,
+                                              int32_t *__restrict__ nsamp, uint32_t *__restrict__ vec_tot)
+{
+    __shared__ uint32_t hist[HISTOGRAM_BINS];
+    for (uint32_t i = threadIdx.x; i < HISTOGRAM_BINS; i += blockDim.x) hist[i] = 0u;
+    __syncthreads();
+    const uint32_t n = counters[VE_CNT_CANDIDATES];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t s = 0;
+    if (t < n) {
+        const uint4 cd = candidates[t];
+        const uint32_t pixels = samples_map[(size_t)cd.z * CLAIM_PROBES + cd.w];
+        const float var = indirect_variance(lbuf, p.hash_width, cd.x);
+        const float pn = gclamp((float)pixels / gmax(p.pixels_norm, 1.0f), 0.0f, 1.0f);
+        const float vn = gclamp(var / gmax(p.variance_norm, 1.0f), 0.0f, 1.0f);
+        const float imp = p.w_pixels * pn + p.w_variance * vn;
+        const float tHigh = __uint_as_float(rank_stats[0]);
+        const float tMid = __uint_as_float(rank_stats[1]);
+        s = imp >= tHigh ? 2 : (imp >= tMid ? 1 : p.min_samples);
+        uint32_t bin = (uint32_t)(imp * 256.0f);
+        if (bin >= HISTOGRAM_BINS) bin = HISTOGRAM_BINS - 1u;
+        atomicAdd(&hist[bin], 1u);
+        const uint32_t ib = __float_as_uint(imp);
+        atomicMin(&rank_stats[4], ib);
+        atomicMax(&rank_stats[5], ib);
+        nsamp[t] = s;
+    }
+    // vector total = sum of max(s, 0) over the 8 lanes of the vector
+    uint32_t tot = s > 0 ? (uint32_t)s : 0u;
+    tot += __shfl_xor(tot, 1, 8);
+    tot += __shfl_xor(tot, 2, 8);
+    tot += __shfl_xor(tot, 4, 8);
+    if ((t & 7u) == 0u && t < n) vec_tot[t >> 3] = tot;
+    __syncthreads();
+    const uint32_t active = n > blockIdx.x * blockDim.x ? min(n - blockIdx.x * blockDim.x, (uint32_t)blockDim.x) : 0u;
+    if (threadIdx.x == 0 && active) atomicAdd(&rank_stats[6], active);
+    for (uint32_t i = threadIdx.x; i < HISTOGRAM_BINS; i += blockDim.x)
+        if (hist[i]) atomicAdd(&rank_stats[8 + i], hist[i]);
+}
+
+// Lockstep order of rank.comp's sample loop inside a vector: iteration i
+// appends, in lane order, every lane with more than i samples.
+__global__ __launch_bounds__(256) void k_rank_write(FrameParams p, const uint4 *__restrict__ candidates,
+                                                    const int32_t *__restrict__ nsamp,
+                                                    const uint32_t *__restrict__ vec_tot,
+                                                    const uint32_t *__restrict__ vec_base, uint2 *__restrict__ shade_list,
+                                                    uint32_t *__restrict__ counters)
+{
+    const uint32_t n = counters[VE_CNT_CANDIDATES];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t v = t >> 3, l = t & 7u, v0 = v * 8u;
+    int32_t s[8];
+    int32_t smax = 0;
+    for (uint32_t k = 0; k < 8; k++) {
+        s[k] = v0 + k < n ? nsamp[v0 + k] : 0;
+        smax = max(smax, s[k]);
+    }
+    if (t == n - 1u) counters[VE_CNT_SHADE] = min(vec_base[v] + vec_tot[v], p.shade_budget);
+    const uint4 cd = candidates[t];
+    uint32_t pos = vec_base[v];
+    for (int32_t i = 0; i < smax; i++) {
+        uint32_t before = 0, all = 0;
+        for (uint32_t k = 0; k < 8; k++) {
+            const uint32_t in_iter = s[k] > i;
+            all += in_iter;
+            before += (k < l) & in_iter;
+        }
+        if (i < s[l]) {
+            const uint32_t idx = pos + before;
+            if (idx < p.shade_budget) shade_list[idx] = make_uint2(cd.x, cd.y);
+        }
+        pos += all;
+    }
+}
+
+// threshold.comp:31-78, one thread
+__global__ void k_threshold(FrameParams p, uint32_t *__restrict__ rank_stats, const uint32_t *__restrict__ counters)
+{
+    if (threadIdx.x != 0 || counters[VE_CNT_CANDIDATES] == 0u) return;
+    uint32_t total = 0;
+    for (uint32_t b = 0; b < HISTOGRAM_BINS; b++) total += rank_stats[8 + b];
+    if (total == 0u) {
+        rank_stats[0] = 0x7F800000u;
+        rank_stats[1] = 0xFF800000u;
+        return;
+    }
+    const float topF = gclamp(p.top_fraction, 0.0f, 1.0f);
+    const float midF = gclamp(p.mid_fraction, 0.0f, 1.0f - topF);
+    const uint32_t topCount = (uint32_t)(topF * (float)total);
+    const uint32_t topMidCount = (uint32_t)((topF + midF) * (float)total);
+    uint32_t cum = 0, thb = HISTOGRAM_BINS, tmb = HISTOGRAM_BINS;
+    for (int b = (int)HISTOGRAM_BINS - 1; b >= 0; b--) {
+        cum += rank_stats[8 + b];
+        if (thb == HISTOGRAM_BINS && cum >= topCount) thb = (uint32_t)b;
+        if (tmb == HISTOGRAM_BINS && cum >= topMidCount) tmb = (uint32_t)b;
+        if (thb != HISTOGRAM_BINS && tmb != HISTOGRAM_BINS) break;
+    }
+    if (thb == HISTOGRAM_BINS) thb = HISTOGRAM_BINS - 1u;
+    if (tmb == HISTOGRAM_BINS) tmb = HISTOGRAM_BINS - 1u;
+    rank_stats[0] = __float_as_uint((float)thb / 256.0f);
+    rank_stats[1] = __float_as_uint((float)tmb / 256.0f);
+}
+
+// ---------------------------------------------------------------------------
+// emit.comp: one ray per shade-list entry, ray index = list index
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_emit(FrameParams p, const uint2 *__restrict__ shade_list,
+                                              Ray *__restrict__ rays, uint32_t *__restrict__ counters)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = counters[VE_CNT_SHADE];
+    if (t == 0) counters[VE_CNT_LIVE] = n;
+    if (t >= n) return;
+    const uint2 en = shade_list[t];
+    const uint32_t px = en.y % p.W, py = en.y / p.W;
+    const f3 d = primary_dir(p.cam, px, py, p.W, p.H);
+    Ray r;
+    r.ox = p.cam[0]; r.oy = p.cam[1]; r.oz = p.cam[2]; r.pad0 = 0.0f;
+    r.dx = d.x; r.dy = d.y; r.dz = d.z; r.pad1 = 0.0f;
+    r.tx = r.ty = r.tz = 1.0f;
+    r.primary_vid = en.x;
+    r.dlx = r.dly = r.dlz = 0.0f;
+    const uint32_t sample_idx = p.frame_idx + en.y * 2654435761u + t * 1442695040u;
+    r.rng = (sample_idx << 8) & 0xFFFFFF00u;
+    r.ilx = r.ily = r.ilz = 0.0f;
+    r.bounce = 0u;
+    r.hit_vid = 0u;
+    r.hit_material = 0u;
+    r.pad2 = r.pad3 = 0u;
+    rays[t] = r;
+}
+
+// ---------------------------------------------------------------------------
+// trace.comp
+// ---------------------------------------------------------------------------
+template <bool STATS>
+__global__ __launch_bounds__(256) void k_trace(const uint32_t *__restrict__ words, FrameParams p, Ray *__restrict__ rays,
+                                               const uint32_t *__restrict__ live_ptr, uint32_t iter,
+                                               uint32_t *__restrict__ counters, unsigned long long *__restrict__ stats,
+                                               unsigned long long *__restrict__ totals)
+{
+    const uint32_t live = *live_ptr;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0 && live > 0u) {
+        atomicAdd(&counters[VE_CNT_RAYS_TRACED], live);
+        atomicMax(&counters[VE_CNT_BOUNCES_RUN], iter + 1u);
+        atomicAdd(&totals[1], (unsigned long long)live);
+    }
+    uint32_t checks = 0, reads = 0;
+    if (t < live) {
+        Ray &r = rays[t];
+        if (r.bounce < p.max_bounces) {
+            const f3 o{r.ox, r.oy, r.oz}, d{r.dx, r.dy, r.dz};
+            const Hit h = raycast<STATS>(words, p.n_words, p.depth, o, d, rcp3(d), p.bounce_cc, checks, reads);
+            if (h.hit) {
+                r.ox = (float)h.px;
+                r.oy = (float)h.py;
+                r.oz = (float)h.pz;
+                r.pad0 = (float)h.size;
+                r.hit_vid = voxel_key(h.px, h.py, h.pz) + 1u;
+                r.hit_material = h.material;
+            } else {
+                r.hit_vid = 0u;
+                r.hit_material = 0u;
+            }
+        }
+    }
+    if (STATS) {
+        unsigned long long a = reads, c = checks;
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off, 64);
+            c += __shfl_xor(c, off, 64);
+        }
+        if ((threadIdx.x & 63u) == 0u) {
+            atomicAdd(&stats[1], a);
+            atomicAdd(&stats[3], c);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// shade.comp
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint4 pack_rgb_l2(f3 c)   // shade.comp:154-160
+{
+    uint4 o;
+    o.x = (uint32_t)(gclamp(c.x, 0.0f, 1024.0f) * 255.0f);
+    o.y = (uint32_t)(gclamp(c.y, 0.0f, 1024.0f) * 255.0f);
+    o.z = (uint32_t)(gclamp(c.z, 0.0f, 1024.0f) * 255.0f);
+    const uint32_t L = (o.x + o.y + o.z) / 3u;
+    o.w = L * L;
+    return o;
+}
+
+__device__ __forceinline__ void firefly(const uint32_t *sl, uint32_t cf, uint4 &v, float k, float floor_)
+{
+    const uint32_t c = __atomic_load_n(&sl[cf], __ATOMIC_RELAXED);
+    if (c <= 4u) return;
+    const uint32_t rs = __atomic_load_n(&sl[cf + 1], __ATOMIC_RELAXED);
+    const uint32_t gs = __atomic_load_n(&sl[cf + 2], __ATOMIC_RELAXED);
+    const uint32_t bs = __atomic_load_n(&sl[cf + 3], __ATOMIC_RELAXED);
+    const float meanL = (float)(rs + gs + bs) / (float)(3u * c);
+    const float sampleL = (float)(v.x + v.y + v.z) / 3.0f;
+    const float thresh = gmax(meanL * k, floor_);
+    if (sampleL > thresh) {
+        const float scale = thresh / sampleL;
+        v.x = (uint32_t)((float)v.x * scale);
+        v.y = (uint32_t)((float)v.y * scale);
+        v.z = (uint32_t)((float)v.z * scale);
+        const uint32_t L = (v.x + v.y + v.z) / 3u;
+        v.w = L * L;
+    }
+}
+
+__device__ void reset_lslot(uint32_t *sl, uint32_t vid, uint4 d, uint4 e, uint32_t time)
+{
+    atomicExch(&sl[0], vid);
+    atomicExch(&sl[1], 1u); atomicExch(&sl[2], d.x); atomicExch(&sl[3], d.y); atomicExch(&sl[4], d.z);
+    atomicExch(&sl[5], d.w);
+    atomicExch(&sl[6], 1u); atomicExch(&sl[7], e.x); atomicExch(&sl[8], e.y); atomicExch(&sl[9], e.z);
+    atomicExch(&sl[10], e.w);
+    atomicExch(&sl[11], time);
+}
+
+// depositSampleDual (shade.comp:162-257) = AccumPass: integer fixed-point sums
+// per voxel id in the lBuffer hash.
+__device__ void deposit(const FrameParams &p, uint32_t *__restrict__ lbuf, uint32_t vid, f3 dir_rgb, f3 ind_rgb)
+{
+    if (vid == 0u) return;
+    uint4 d = pack_rgb_l2(dir_rgb);
+    uint4 e = pack_rgb_l2(ind_rgb);
+    const uint32_t h = vid % (p.hash_width - 1u) + 1u;
+    uint32_t *row = lbuf + (size_t)h * (LBUFFER_STRIDE * LBUFFER_SLOTS);
+    uint32_t cand_t = 0xFFFFFFFFu, cand_i = 0;
+    bool have = false;
+    for (uint32_t i = 0; i < LBUFFER_STRIDE * LBUFFER_SLOTS; i += LBUFFER_STRIDE) {
+        uint32_t *sl = row + i;
+        const uint32_t prev = atomicCAS(&sl[0], 0u, vid);
+        const uint32_t id = prev == 0u ? vid : prev;
+        const uint32_t st = __atomic_load_n(&sl[11], __ATOMIC_RELAXED);
+        if (id != vid) {
+            if (st < cand_t) { cand_t = st; cand_i = i; have = true; }
+            continue;
+        }
+        if (!p.disable_stale_reset && st != 0u && st + p.update_time < p.time) {
+            reset_lslot(sl, vid, d, e, p.time);
+            return;
+        }
+        if (__atomic_load_n(&sl[1], __ATOMIC_RELAXED) >= p.cap_direct)
+            for (int f = 1; f <= 5; f++) atomicExch(&sl[f], __atomic_load_n(&sl[f], __ATOMIC_RELAXED) >> 1);
+        if (__atomic_load_n(&sl[6], __ATOMIC_RELAXED) >= p.cap_indirect)
+            for (int f = 6; f <= 10; f++) atomicExch(&sl[f], __atomic_load_n(&sl[f], __ATOMIC_RELAXED) >> 1);
+        firefly(sl, 1, d, p.firefly_k, p.firefly_floor);
+        firefly(sl, 6, e, p.firefly_k, p.firefly_floor);
+        atomicAdd(&sl[1], 1u); atomicAdd(&sl[2], d.x); atomicAdd(&sl[3], d.y); atomicAdd(&sl[4], d.z);
+        atomicAdd(&sl[5], d.w);
+        atomicAdd(&sl[6], 1u); atomicAdd(&sl[7], e.x); atomicAdd(&sl[8], e.y); atomicAdd(&sl[9], e.z);
+        atomicAdd(&sl[10], e.w);
+        atomicExch(&sl[11], p.time);
+        return;
+    }
+    if (have) reset_lslot(row + cand_i, vid, d, e, p.time);
+}
+
+__device__ f3 lookup_normal(const uint32_t *__restrict__ nbuf, uint32_t hw, uint32_t vid, f3 rd)   // shade.comp:261-275
+{
+    const uint32_t h = vid % (hw - 1u) + 1u;
+    const uint32_t *row = nbuf + (size_t)h * (NBUFFER_STRIDE * NBUFFER_SLOTS);
+    for (uint32_t s = 0; s < NBUFFER_SLOTS; s++) {
+        if (row[s * NBUFFER_STRIDE] == vid) {
+            const uint32_t nf = row[s * NBUFFER_STRIDE + 1];
+            if (nf != 0u) return normalize(unpack_normal(nf));
+        }
+    }
+    const float ax = fabsf(rd.x), ay = fabsf(rd.y), az = fabsf(rd.z);
+    if (ax > ay && ax > az) return f3{-gsign(rd.x), 0.0f, 0.0f};
+    if (ay > az) return f3{0.0f, -gsign(rd.y), 0.0f};
+    return f3{0.0f, 0.0f, -gsign(rd.z)};
+}
+
+__global__ __launch_bounds__(256) void k_shade(FrameParams p, const Ray *__restrict__ rays_in,
+                                               const uint32_t *__restrict__ live_in, Ray *__restrict__ rays_out,
+                                               uint32_t *__restrict__ live_out, uint32_t *__restrict__ lbuf,
+                                               const uint32_t *__restrict__ nbuf, const Material *__restrict__ mats,
+                                               EmissiveVoxel *__restrict__ emissives, uint32_t *__restrict__ em_claim,
+                                               uint32_t *__restrict__ counters)
+{
+    const uint32_t live = *live_in;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t MB = p.max_bounces;
+    bool survive = false;
+    Ray r;
+    if (t < live) {
+        r = rays_in[t];
+        if (r.bounce < MB) {
+            if (r.hit_vid == 0u) {   // miss: sky * throughput into the indirect channel
+                r.ilx = r.ilx + p.sky[0] * r.tx;
+                r.ily = r.ily + p.sky[1] * r.ty;
+                r.ilz = r.ilz + p.sky[2] * r.tz;
+                deposit(p, lbuf, r.primary_vid, f3{r.dlx, r.dly, r.dlz}, f3{r.ilx, r.ily, r.ilz});
+            } else {
+                const uint32_t mi = r.hit_material < MATERIAL_SLOTS ? r.hit_material : MATERIAL_SLOTS - 1u;
+                const Material &m = mats[mi];
+                const uint32_t hx = (uint32_t)r.ox, hy = (uint32_t)r.oy, hz = (uint32_t)r.oz;
+                const uint32_t hsize = (uint32_t)r.pad0;
+                const f3 rd{r.dx, r.dy, r.dz};
+                const f3 nrm = lookup_normal(nbuf, p.hash_width, r.hit_vid, rd);
+                const f3 albedo{m.color[0], m.color[1], m.color[2]};
+                if (m.emissive) {   // shade.comp:576-619
+                    const f3 em = vmin3(mul(muls(albedo, m.emissive_intensity), f3{r.tx, r.ty, r.tz}), f3{20, 20, 20});
+                    if (r.bounce == 0u) { r.dlx += em.x; r.dly += em.y; r.dlz += em.z; }
+                    else { r.ilx += em.x; r.ily += em.y; r.ilz += em.z; }
+                    if (r.bounce == 0u) {
+                        const uint32_t vid = r.hit_vid;
+                        const uint32_t ch = claim_hash(vid) % EMISSIVE_CLAIM_W;
+                        for (uint32_t q = 0; q < EMISSIVE_CLAIM_PROBES; q++) {
+                            const uint32_t prev = atomicCAS(&em_claim[(size_t)ch * EMISSIVE_CLAIM_PROBES + q], 0u, vid);
+                            if (prev == vid) break;
+                            if (prev == 0u) {
+                                const uint32_t idx = atomicAdd(&counters[VE_CNT_EMISSIVES], 1u);
+                                if (idx < EMISSIVE_CAPACITY) {
+                                    const f3 ee = muls(albedo, m.emissive_intensity);
+                                    EmissiveVoxel ev{ee.x, ee.y, ee.z, vid, hx, hy, hz, hsize > 1u ? hsize : 1u};
+                                    emissives[idx] = ev;
+                                } else {
+                                    atomicSub(&counters[VE_CNT_EMISSIVES], 1u);
+                                }
+                                break;
+                            }
+                        }
+                    }
+                }
+                if (!m.emissive && r.bounce < 2u && __atomic_load_n(&counters[VE_CNT_EMISSIVES], __ATOMIC_RELAXED) > 0u)
+                    atomicOr(&counters[VE_CNT_NEE_PENDING], 1u);
+                if (r.bounce + 1u >= MB) {   // last allowed bounce
+                    deposit(p, lbuf, r.primary_vid, f3{r.dlx, r.dly, r.dlz}, f3{r.ilx, r.ily, r.ilz});
+                } else {
+                    const bool use_ggx = m.metallic * (1.0f - m.roughness) > 0.5f;
+                    uint32_t rng = r.rng;
+                    f3 sdir, bt;
+                    if (use_ggx) {
+                        const f3 Vv{-rd.x, -rd.y, -rd.z};
+                        const f3 H = sample_ggx_vndf(nrm, Vv, m.roughness, rng);
+                        sdir = reflect3(rd, H);
+                        if (dot(sdir, nrm) <= 0.0f) {
+                            sdir = normalize(add(nrm, random_direction(rng)));
+                            bt = evaluate_brdf(rd, sdir, nrm, albedo, m.roughness, m.metallic, m.specular);
+                        } else {
+                            const f3 F0 = add(muls(f3{0.04f, 0.04f, 0.04f}, 1.0f - m.metallic), muls(albedo, m.metallic));
+                            bt = ggx_throughput(nrm, Vv, sdir, H, m.roughness, F0);
+                        }
+                    } else {
+                        sdir = normalize(add(nrm, random_direction(rng)));
+                        bt = evaluate_brdf(rd, sdir, nrm, albedo, m.roughness, m.metallic, m.specular);
+                    }
+                    r.rng = rng;
+                    f3 ntp = mul(f3{r.tx, r.ty, r.tz}, bt);
+                    bool dies = false;
+                    if (r.bounce + 1u >= RR_START_BOUNCE) {   // Russian roulette (shade.comp:879-889)
+                        const float pr = gclamp(gmax(ntp.x, gmax(ntp.y, ntp.z)), 0.05f, 0.95f);
+                        const float xi = rand_lds(r.rng);
+                        if (xi > pr) dies = true;
+                        else ntp = divs(ntp, pr);
+                    }
+                    if (dies) {
+                        deposit(p, lbuf, r.primary_vid, f3{r.dlx, r.dly, r.dlz}, f3{r.ilx, r.ily, r.ilz});
+                    } else {   // survivor (shade.comp:892-908)
+                        r.ox = ((float)hx + 0.5f) + nrm.x;
+                        r.oy = ((float)hy + 0.5f) + nrm.y;
+                        r.oz = ((float)hz + 0.5f) + nrm.z;
+                        r.dx = sdir.x; r.dy = sdir.y; r.dz = sdir.z;
+                        r.tx = ntp.x; r.ty = ntp.y; r.tz = ntp.z;
+                        r.bounce = r.bounce + 1u;
+                        r.hit_vid = 0u;
+                        r.hit_material = 0u;
+                        survive = true;
+                    }
+                }
+            }
+        }
+    }
+    // stream compaction of survivors: wave ballot + one atomic per wave
+    const uint64_t mask = __ballot(survive);
+    if (mask) {
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(live_out, (uint32_t)__popcll(mask));
+        base = __shfl(base, leader, 64);
+        if (survive) rays_out[base + before] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// resolve.comp = AvgPass
+// ---------------------------------------------------------------------------
+struct Avg {
+    f3 c;
+    uint32_t count;
+    float var;
+};
+
+__device__ Avg voxel_avg(const uint32_t *__restrict__ lbuf, uint32_t hw, uint32_t vid)   // resolve.comp:109-148
+{
+    Avg a{{0, 0, 0}, 0u, 0.0f};
+    if (vid == 0u) return a;
+    const uint32_t h = vid % (hw - 1u) + 1u;
+    const uint32_t *row = lbuf + (size_t)h * (LBUFFER_STRIDE * LBUFFER_SLOTS);
+    for (uint32_t i = 0; i < LBUFFER_STRIDE * LBUFFER_SLOTS; i += LBUFFER_STRIDE) {
+        const uint32_t *sl = row + i;
+        if (sl[0] != vid) continue;
+        const uint32_t dC = sl[1], iC = sl[6];
+        a.count = dC + iC;
+        if (a.count == 0u) return a;
+        f3 da{0, 0, 0}, ia{0, 0, 0};
+        if (dC > 0u) {
+            const float inv = 1.0f / (float)dC;
+            da = divs(muls(f3{(float)sl[2], (float)sl[3], (float)sl[4]}, inv), 255.0f);
+        }
+        if (iC > 0u) {
+            const uint32_t ir = sl[7], ig = sl[8], ib = sl[9], il2 = sl[10];
+            const float inv = 1.0f / (float)iC;
+            ia = divs(muls(f3{(float)ir, (float)ig, (float)ib}, inv), 255.0f);
+            const float meanL = (float)(ir + ig + ib) * inv / 3.0f;
+            a.var = gmax((float)il2 * inv - meanL * meanL, 0.0f) / 65025.0f;
+        }
+        a.c = add(da, ia);
+        return a;
+    }
+    return a;
+}
+
+__device__ f3 reconstruct_timeout(const FrameParams &p, const uint4 *__restrict__ gbuf, const uint32_t *__restrict__ lbuf,
+                                  int px, int py)   // resolve.comp:193-219
+{
+    f3 sum{0, 0, 0};
+    float wsum = 0.0f;
+    const int R = p.reconstruction_radius;
+    for (int dy = -R; dy <= R; dy++)
+        for (int dx = -R; dx <= R; dx++) {
+            if (dx == 0 && dy == 0) continue;
+            const int qx = px + dx, qy = py + dy;
+            if (qx < 0 || qy < 0 || qx >= (int)p.W || qy >= (int)p.H) continue;
+            const uint4 nd = gbuf[(size_t)qy * p.W + qx];
+            if (nd.w == 0u || nd.x >= p.primary_cc) continue;
+            const Avg na = voxel_avg(lbuf, p.hash_width, nd.w);
+            if (na.count == 0u) continue;
+            const float dist = (float)(dx * dx + dy * dy);
+            const float w = (float)min(na.count, 32u) / (1.0f + dist * 0.15f);
+            sum = add(sum, muls(na.c, w));
+            wsum += w;
+        }
+    return wsum > 0.0001f ? divs(sum, wsum) : f3{p.sky[0], p.sky[1], p.sky[2]};
+}
+
+__global__ __launch_bounds__(256) void k_resolve(FrameParams p, const uint4 *__restrict__ gbuf,
+                                                 const uint32_t *__restrict__ lbuf, const uint32_t *__restrict__ nbuf,
+                                                 const Material *__restrict__ mats, float4 *__restrict__ out)
+{
+    uint32_t x, y;
+    if (!tile_pixel(p, x, y)) return;
+    const size_t pix = (size_t)y * p.W + x;
+    const uint4 g = gbuf[pix];
+    const uint32_t checks = g.x, mi = g.y, vid = g.w;
+    const bool timeout = vid == 0u && checks >= p.primary_cc;
+    const f3 sky{p.sky[0], p.sky[1], p.sky[2]};
+    f3 col;
+    if (p.render_type == 1) {
+        const float v = (float)checks / 65.0f;
+        col = f3{v, v, v};
+    } else if (vid == 0u) {
+        col = timeout ? reconstruct_timeout(p, gbuf, lbuf, (int)x, (int)y) : sky;
+    } else if (p.render_type == 2) {
+        col = (mi == 0u || mi >= 127u) ? f3{0, 0, 0} : f3{mats[mi].color[0], mats[mi].color[1], mats[mi].color[2]};
+    } else if (p.render_type == 3) {
+        col = f3{0, 0, 0};
+        const uint32_t h = vid % (p.hash_width - 1u) + 1u;
+        const uint32_t *row = nbuf + (size_t)h * (NBUFFER_STRIDE * NBUFFER_SLOTS);
+        for (uint32_t s = 0; s < NBUFFER_SLOTS; s++) {
+            if (row[s * NBUFFER_STRIDE] == vid) {
+                const uint32_t f = row[s * NBUFFER_STRIDE + 1];
+                if (f != 0u) col = add(muls(normalize(unpack_normal(f)), 0.5f), f3{0.5f, 0.5f, 0.5f});
+                break;
+            }
+        }
+    } else if (p.render_type == 4) {
+        const uint32_t hid = vid - 1u;
+        col = f3{(float)((hid + 1u) % 255u) / 255.0f, (float)(((hid + 1u) / 255u) % 255u) / 255.0f,
+                 (float)(((hid + 1u) / (255u * 255u)) % 255u) / 255.0f};
+    } else if (p.render_type == 5) {
+        float var = 0.0f;
+        const uint32_t h = vid % (p.hash_width - 1u) + 1u;
+        const uint32_t *row = lbuf + (size_t)h * (LBUFFER_STRIDE * LBUFFER_SLOTS);
+        for (uint32_t i = 0; i < LBUFFER_STRIDE * LBUFFER_SLOTS; i += LBUFFER_STRIDE) {
+            const uint32_t *sl = row + i;
+            if (sl[0] == 0u) break;
+            if (sl[0] != vid) continue;
+            const uint32_t c = sl[6];
+            if (c >= 2u) {
+                const float fc = (float)c;
+                const float meanL = (float)(sl[7] + sl[8] + sl[9]) / (3.0f * fc);
+                const float meanL2 = (float)sl[10] / fc;
+                var = gmax(meanL2 - meanL * meanL, 0.0f) / 65025.0f;
+            }
+            break;
+        }
+        const float sc = gclamp(var * 65025.0f / gmax(p.variance_norm, 1.0f), 0.0f, 1.0f);
+        col = f3{sc, sc, sc};
+    } else {   // DEFAULT (resolve.comp:224-261)
+        const Avg c = voxel_avg(lbuf, p.hash_width, vid);
+        if (c.count >= TRUSTED_THRESHOLD && c.var < 0.02f) {
+            col = c.c;
+        } else {
+            f3 sum = muls(c.c, (float)c.count);
+            float wsum = (float)c.count;
+            const int OX[4] = {1, -1, 0, 0}, OY[4] = {0, 0, 1, -1};
+            for (int k = 0; k < 4; k++) {
+                const int qx = (int)x + OX[k], qy = (int)y + OY[k];
+                if (qx < 0 || qy < 0 || qx >= (int)p.W || qy >= (int)p.H) continue;
+                const uint32_t nvid = gbuf[(size_t)qy * p.W + qx].w;
+                if (nvid == 0u || nvid == vid) continue;
+                const Avg na = voxel_avg(lbuf, p.hash_width, nvid);
+                if (na.count == 0u) continue;
+                const float dc = length3(sub(na.c, c.c));
+                const float wc = m_exp(-dc * 6.0f);
+                const float wv = 1.0f / (1.0f + na.var * 32.0f);
+                const float w = wc * wv * (float)min(na.count, 32u);
+                sum = add(sum, muls(na.c, w));
+                wsum += w;
+            }
+            col = wsum > 0.0001f ? divs(sum, wsum) : f3{0, 0, 0};
+        }
+    }
+    out[pix] = make_float4(col.x, col.y, col.z, 1.0f);
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+static inline uint32_t cdiv(size_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b); }
+
+static uint32_t screen_blocks(const FrameParams &p)
+{
+    const uint32_t tiles = ((p.W + TILE - 1) / TILE) * ((p.H + TILE - 1) / TILE);
+    uint32_t nb = (tiles + 3u) / 4u;
+    nb = nb == 0 ? 1 : nb;
+    return (nb + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+}
+
+void launch_fill(hipStream_t s, uint32_t *p, uint32_t v, size_t n)
+{
+    if (n) hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(n, 256)), dim3(256), 0, s, p, v, n);
+}
+
+void launch_normal(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b)
+{
+    hipLaunchKernelGGL(k_normal, dim3(screen_blocks(p)), dim3(256), 0, s, words, p, b.gbuf, b.gpos, b.nbuf);
+}
+
+// Per-frame clear of RankStats' per-frame half and the counters the frame
+// accumulates into (renderer.cpp:343-375); fill kernels, not memsets, so the
+// sequence stays capturable.
+__global__ void k_frame_reset(uint32_t *__restrict__ rank_stats, uint32_t *__restrict__ counters)
+{
+    const uint32_t i = threadIdx.x;
+    if (i < 4) rank_stats[4 + i] = i == 0 ? 0x7F7FFFFFu : 0u;
+    if (i < HISTOGRAM_BINS) rank_stats[8 + i] = 0u;
+    if (i < NUM_COUNTERS && i != VE_CNT_EMISSIVES) counters[i] = 0u;
+}
+
+__global__ void k_count_primary(uint32_t *__restrict__ counters, unsigned long long *__restrict__ totals, uint32_t n)
+{
+    counters[VE_CNT_PRIMARY_RAYS] = n;
+    totals[0] += n;
+}
+
+void launch_count_primary(hipStream_t s, DeviceBuffers &b, uint32_t n)
+{
+    hipLaunchKernelGGL(k_count_primary, dim3(1), dim3(1), 0, s, b.counters, b.ray_totals, n);
+}
+
+void launch_reset(hipStream_t s, DeviceBuffers &b)
+{
+    hipLaunchKernelGGL(k_frame_reset, dim3(1), dim3(256), 0, s, b.rank_stats, b.counters);
+}
+
+int launch_schedule(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
+{
+    const uint32_t gw = (p.W + 7u) / 8u, gh = (p.H + 7u) / 8u;
+    const uint32_t groups = gw * gh;
+    const uint32_t n_vec = groups * 8u;
+    const size_t tab = (size_t)CLAIM_WIDTH * SCHED_SLOTS;
+    launch_fill(s, b.sched_vid, 0u, tab);
+    launch_fill(s, b.sched_first, NONE, tab);
+    launch_fill(s, b.sched_cnt, 0u, tab);
+    hipLaunchKernelGGL(k_sched_claim, dim3(groups), dim3(64), 0, s, p, b.gbuf, b.sched_vid, b.sched_first, b.sched_cnt,
+                       b.cand_at, b.counters);
+    hipLaunchKernelGGL(k_sched_rank, dim3(CLAIM_WIDTH), dim3(64), 0, s, b.sched_vid, b.sched_first, b.sched_cnt, b.claim,
+                       b.samples, b.cand_at);
+    hipLaunchKernelGGL(k_sched_vec_count, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, b.cand_at, n_vec, b.vec_cnt);
+    size_t tb = b.scan_temp_bytes;
+    if (hipcub::DeviceScan::ExclusiveSum(b.scan_temp, tb, b.vec_cnt, b.vec_base, (int)n_vec, s) != hipSuccess)
+        return VE_EDEVICE;
+    hipLaunchKernelGGL(k_sched_write, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, p, b.cand_at, n_vec, b.vec_cnt,
+                       b.vec_base, b.claim, b.candidates, b.counters);
+    return VE_OK;
+}
+
+int launch_rank(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
+{
+    // candidates <= distinct visible voxels <= pixels
+    const uint32_t bound = (uint32_t)std::min<size_t>((size_t)p.W * p.H, CANDIDATE_CAPACITY);
+    const uint32_t n_vec = cdiv(bound, 8);
+    launch_fill(s, b.vec_cnt, 0u, n_vec);
+    hipLaunchKernelGGL(k_rank, dim3(cdiv(bound, 256)), dim3(256), 0, s, p, b.candidates, b.samples, b.lbuf,
+                       b.rank_stats, b.counters, b.nsamp, b.vec_cnt);
+    size_t tb = b.scan_temp_bytes;
+    if (hipcub::DeviceScan::ExclusiveSum(b.scan_temp, tb, b.vec_cnt, b.vec_base, (int)n_vec, s) != hipSuccess)
+        return VE_EDEVICE;
+    hipLaunchKernelGGL(k_rank_write, dim3(cdiv(bound, 256)), dim3(256), 0, s, p, b.candidates, b.nsamp, b.vec_cnt,
+                       b.vec_base, b.shade_list, b.counters);
+    return VE_OK;
+}
+
+void launch_threshold(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
+{
+    hipLaunchKernelGGL(k_threshold, dim3(1), dim3(64), 0, s, p, b.rank_stats, b.counters);
+}
+
+void launch_emit(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
+{
+    hipLaunchKernelGGL(k_emit, dim3(cdiv(p.shade_budget, 256)), dim3(256), 0, s, p, b.shade_list, b.rays, b.counters);
+}
+
+// Bounce iteration `iter` of the trace/shade loop (renderer.cpp:515-602); the
+// ping-pong direction alternates with `iter`.
+static inline Ray *bounce_in(DeviceBuffers &b, uint32_t iter) { return (iter & 1u) ? b.rays_next : b.rays; }
+static inline Ray *bounce_out(DeviceBuffers &b, uint32_t iter) { return (iter & 1u) ? b.rays : b.rays_next; }
+static inline uint32_t *live_in_ptr(DeviceBuffers &b, uint32_t iter)
+{
+    return b.counters + ((iter & 1u) ? VE_CNT_LIVE_B : VE_CNT_LIVE);
+}
+static inline uint32_t *live_out_ptr(DeviceBuffers &b, uint32_t iter)
+{
+    return b.counters + ((iter & 1u) ? VE_CNT_LIVE : VE_CNT_LIVE_B);
+}
+
+void launch_trace(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t iter, bool stats)
+{
+    const uint32_t nb = cdiv(p.shade_budget, 256);
+    if (stats)
+        hipLaunchKernelGGL(k_trace<true>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, iter), live_in_ptr(b, iter),
+                           iter, b.counters, b.node_stats, b.ray_totals);
+    else
+        hipLaunchKernelGGL(k_trace<false>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, iter),
+                           live_in_ptr(b, iter), iter, b.counters, b.node_stats, b.ray_totals);
+}
+
+void launch_shade(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_t iter)
+{
+    const uint32_t nb = cdiv(p.shade_budget, 256);
+    launch_fill(s, live_out_ptr(b, iter), 0u, 1);
+    hipLaunchKernelGGL(k_shade, dim3(nb), dim3(256), 0, s, p, bounce_in(b, iter), live_in_ptr(b, iter),
+                       bounce_out(b, iter), live_out_ptr(b, iter), b.lbuf, b.nbuf, b.materials, b.emissives,
+                       b.emissive_claim, b.counters);
+}
+
+void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
+{
+    hipLaunchKernelGGL(k_resolve, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.nbuf, b.materials,
+                       b.output);
+}
+
+size_t scan_temp_bytes(uint32_t max_items)
+{
+    size_t bytes = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (uint32_t *)nullptr, (uint32_t *)nullptr, (int)max_items);
+    return bytes;
+}
+
+}  // namespace ve

@@ -2,7 +2,8 @@
 
 Workload (the depth-10 configuration): 1024^3 octree of the seeded fbm terrain
 shell + 64 seeded spheres (10.4 M voxels), 1920x1080, 1 spp, 2 bounces
-(MAX_BOUNCES = 3), 64-frame seeded orbit at radius 0.8N with temporal
+(MAX_BOUNCES = 3), 64-frame seeded orbit at radius 0.8N (eye height 0.75N,
+40 degree vertical FOV, so the grid fills the frame) with temporal
 accumulation (persistent lBuffer).  One step = one whole
 Renderer::runWavefrontFrame: primary rays, normals, schedule, rank, threshold,
 emit, the trace/shade bounce loop and resolve.
@@ -123,7 +124,7 @@ def main():
 
     # timed region: K frames, HIP events per pass on the context's stream
     r.reset_ray_totals()
-    r.enable_timing(True)
+    r.enable_timing(args.steps)
     torch.cuda.synchronize()
     r.sync()
     t0 = time.perf_counter()

@@ -214,7 +214,9 @@ int ve_sync(ve_ctx *ctx);
  * around each pass on the context's stream (no host synchronisation);
  * ve_timings waits for the stream and returns, per pass, the mean
  * milliseconds per frame over the frames recorded since the last call
- * (trace and shade: all bounce iterations of a frame together). */
+ * (trace and shade: all bounce iterations of a frame together).  An
+ * `enable` above 1 is the number of frames about to be recorded; their events
+ * are created up front so the recording itself only enqueues markers. */
 int ve_enable_timing(ve_ctx *ctx, int enable);
 int ve_timings(ve_ctx *ctx, float ms_out[VE_PASS_COUNT]);
 

@@ -341,7 +341,9 @@ class Renderer:
         self._ck(lib().ve_reset_ray_totals(self.h))
 
     def enable_timing(self, on=True):
-        self._ck(lib().ve_enable_timing(self.h, 1 if on else 0))
+        """on: True/False, or the number of frames about to be recorded."""
+        n = int(on) if not isinstance(on, bool) else (1 if on else 0)
+        self._ck(lib().ve_enable_timing(self.h, n))
 
     def timings(self):
         out = np.zeros(len(PASS), np.float32)

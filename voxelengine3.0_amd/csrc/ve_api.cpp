@@ -36,6 +36,8 @@ struct ve_scene {
     uint32_t depth = 0;
 };
 
+constexpr int MAX_ITERS = 8;   // host bounce-loop cap (renderer.cpp:515)
+
 struct ve_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -48,9 +50,10 @@ struct ve_ctx {
     bool node_stats = false;
     bool timing = false;
     uint32_t bounce_iter = 0;           // next trace/shade iteration of per-pass dispatch
-    // timing: one event pair per pass per recorded frame
-    std::vector<std::array<hipEvent_t, 2 * VE_PASS_COUNT>> frame_events;
+    // timing: one event pair per pass and bounce iteration per recorded frame
+    std::vector<std::array<hipEvent_t, 2 * VE_PASS_COUNT * MAX_ITERS>> frame_events;
     size_t frames_recorded = 0;
+    uint32_t iters_recorded = 0;
     DeviceBuffers b;
     std::vector<void *> allocations;
 };
@@ -168,6 +171,49 @@ int check_frame(ve_ctx *c, const ve_scene *s, const ve_frame_config *fc, const v
     return VE_OK;
 }
 
+// Reference-layout index (slot * stride + field) -> device bucket index for
+// the three hash images (see the bucket layouts in ve_internal.h).
+uint32_t bucket_index(ve_buffer_id id, uint32_t y)
+{
+    switch (id) {
+    case VE_BUF_NBUFFER: {
+        const uint32_t s = y / NBUFFER_STRIDE, f = y % NBUFFER_STRIDE;
+        return f * NBUFFER_SLOTS + s;
+    }
+    case VE_BUF_LBUFFER: {
+        const uint32_t s = y / LBUFFER_STRIDE, f = y % LBUFFER_STRIDE;
+        if (f == 0) return s;
+        if (f == 11) return LB_TIME + s;
+        return LB_PAY + s * LB_PAYW + (f - 1);
+    }
+    default: {   // VE_BUF_RBUFFER
+        const uint32_t s = y / RBUFFER_STRIDE, f = y % RBUFFER_STRIDE;
+        if (f == 0) return s;
+        if (f == 7) return RB_TIME + s;
+        return RB_PAY + s * RB_PAYW + (f - 1);
+    }
+    }
+}
+
+bool is_hash_buffer(ve_buffer_id id) { return id == VE_BUF_NBUFFER || id == VE_BUF_LBUFFER || id == VE_BUF_RBUFFER; }
+
+// Permute a whole hash image between the device and the reference layout.
+void permute_hash(ve_buffer_id id, const uint32_t *src, uint32_t *dst, size_t words, bool to_reference)
+{
+    const uint32_t per = id == VE_BUF_NBUFFER ? NB_WORDS : (id == VE_BUF_LBUFFER ? LB_WORDS : RB_WORDS);
+    const size_t buckets = words / per;
+    std::vector<uint32_t> map(per);
+    for (uint32_t y = 0; y < per; y++) map[y] = bucket_index(id, y);
+    for (size_t h = 0; h < buckets; h++) {
+        const uint32_t *a = src + h * per;
+        uint32_t *b = dst + h * per;
+        if (to_reference)
+            for (uint32_t y = 0; y < per; y++) b[y] = a[map[y]];
+        else
+            for (uint32_t y = 0; y < per; y++) b[map[y]] = a[y];
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -264,6 +310,7 @@ int ve_create(ve_ctx **out, const ve_renderer_config *cfg)
     DeviceBuffers &b = c->b;
     b.scan_temp_bytes = scan_temp_bytes((uint32_t)(groups_max * 8));
     if ((rc = dalloc(c, &b.gbuf, P)) || (rc = dalloc(c, &b.gpos, P)) || (rc = dalloc(c, &b.output, P)) ||
+        (rc = dalloc(c, &b.pix_avg, P)) || (rc = dalloc(c, &b.pix_cnt, P)) ||
         (rc = dalloc(c, &b.nbuf, (size_t)c->hash_width * NBUFFER_STRIDE * NBUFFER_SLOTS)) ||
         (rc = dalloc(c, &b.lbuf, (size_t)c->hash_width * LBUFFER_STRIDE * LBUFFER_SLOTS)) ||
         (rc = dalloc(c, &b.rbuf, (size_t)c->hash_width * RBUFFER_STRIDE * RBUFFER_SLOTS)) ||
@@ -428,7 +475,7 @@ int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p)
 hipEvent_t *frame_slot(ve_ctx *c)
 {
     if (c->frames_recorded >= c->frame_events.size()) {
-        std::array<hipEvent_t, 2 * VE_PASS_COUNT> fe{};
+        std::array<hipEvent_t, 2 * VE_PASS_COUNT * MAX_ITERS> fe{};
         for (auto &e : fe)
             if (hipEventCreate(&e) != hipSuccess) return nullptr;
         c->frame_events.push_back(fe);
@@ -436,13 +483,13 @@ hipEvent_t *frame_slot(ve_ctx *c)
     return c->frame_events[c->frames_recorded].data();
 }
 
-int timed_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, bool first_of_kind, hipEvent_t *ev)
+int timed_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, uint32_t iter, hipEvent_t *ev)
 {
     if (!ev) return run_pass(c, s, pass, p);
-    // bounce passes repeat: time from the first launch of the kind to the last
-    if (first_of_kind) HIP_OK(c, hipEventRecord(ev[2 * pass], c->stream));
+    hipEvent_t *pair = ev + 2 * (pass * MAX_ITERS + iter);
+    HIP_OK(c, hipEventRecord(pair[0], c->stream));
     int rc = run_pass(c, s, pass, p);
-    HIP_OK(c, hipEventRecord(ev[2 * pass + 1], c->stream));
+    HIP_OK(c, hipEventRecord(pair[1], c->stream));
     return rc;
 }
 
@@ -474,14 +521,17 @@ int ve_frame(ve_ctx *c, ve_scene *s, const ve_frame_config *fc, const ve_frame_i
     const ve_pass_id head[] = {VE_PASS_RESET, VE_PASS_PRIMARY, VE_PASS_NORMAL, VE_PASS_SCHEDULE,
                                VE_PASS_RANK, VE_PASS_THRESHOLD, VE_PASS_EMIT};
     for (ve_pass_id id : head)
-        if ((rc = timed_pass(c, s, id, p, true, ev))) return rc;
-    const uint32_t iters = std::min<uint32_t>(c->max_bounces, 8u);   // host loop cap, renderer.cpp:515
+        if ((rc = timed_pass(c, s, id, p, 0, ev))) return rc;
+    const uint32_t iters = std::min<uint32_t>(c->max_bounces, (uint32_t)MAX_ITERS);   // renderer.cpp:515
     for (uint32_t i = 0; i < iters; i++) {
-        if ((rc = timed_pass(c, s, VE_PASS_TRACE, p, i == 0, ev))) return rc;
-        if ((rc = timed_pass(c, s, VE_PASS_SHADE, p, i == 0, ev))) return rc;
+        if ((rc = timed_pass(c, s, VE_PASS_TRACE, p, i, ev))) return rc;
+        if ((rc = timed_pass(c, s, VE_PASS_SHADE, p, i, ev))) return rc;
     }
-    rc = timed_pass(c, s, VE_PASS_RESOLVE, p, true, ev);
-    if (ev) c->frames_recorded++;
+    rc = timed_pass(c, s, VE_PASS_RESOLVE, p, 0, ev);
+    if (ev) {
+        c->iters_recorded = iters;
+        c->frames_recorded++;
+    }
     return rc;
 }
 
@@ -501,6 +551,14 @@ int ve_read(ve_ctx *c, ve_buffer_id id, void *dst, size_t bytes)
     if (!v.ptr) return ctx_error(c, VE_ENOTFOUND, "unknown buffer id %d", (int)id);
     if (!dst || bytes > v.bytes) return ctx_error(c, VE_EINVAL, "read of %zu bytes from a %zu-byte buffer", bytes, v.bytes);
     HIP_OK(c, hipSetDevice(c->device));
+    if (is_hash_buffer(id)) {   // returned in the reference's slot-interleaved layout
+        std::vector<uint32_t> tmp(v.bytes / 4), ref(v.bytes / 4);
+        HIP_OK(c, hipMemcpyAsync(tmp.data(), v.ptr, v.bytes, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(c, hipStreamSynchronize(c->stream));
+        permute_hash(id, tmp.data(), ref.data(), tmp.size(), true);
+        std::memcpy(dst, ref.data(), bytes);
+        return VE_OK;
+    }
     HIP_OK(c, hipMemcpyAsync(dst, v.ptr, bytes, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     return VE_OK;
@@ -513,6 +571,14 @@ int ve_write(ve_ctx *c, ve_buffer_id id, const void *src, size_t bytes)
     if (!v.ptr) return ctx_error(c, VE_ENOTFOUND, "unknown buffer id %d", (int)id);
     if (!src || bytes > v.bytes) return ctx_error(c, VE_EINVAL, "write of %zu bytes to a %zu-byte buffer", bytes, v.bytes);
     HIP_OK(c, hipSetDevice(c->device));
+    if (is_hash_buffer(id)) {   // accepted in the reference layout; whole image only
+        if (bytes != v.bytes) return ctx_error(c, VE_EINVAL, "hash images are written whole (%zu bytes)", v.bytes);
+        std::vector<uint32_t> dev(v.bytes / 4);
+        permute_hash(id, static_cast<const uint32_t *>(src), dev.data(), dev.size(), false);
+        HIP_OK(c, hipMemcpyAsync(v.ptr, dev.data(), v.bytes, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(c, hipStreamSynchronize(c->stream));
+        return VE_OK;
+    }
     HIP_OK(c, hipMemcpyAsync(v.ptr, src, bytes, hipMemcpyHostToDevice, c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     return VE_OK;
@@ -531,6 +597,14 @@ int ve_enable_timing(ve_ctx *c, int enable)
     if (!c) return ve_global_error(VE_EINVAL, "null context");
     c->timing = enable != 0;
     c->frames_recorded = 0;
+    // enable > 1 is the number of frames to record: create their events now,
+    // outside any timed region
+    HIP_OK(c, hipSetDevice(c->device));
+    while (enable > 1 && c->frame_events.size() < (size_t)enable) {
+        std::array<hipEvent_t, 2 * VE_PASS_COUNT * MAX_ITERS> fe{};
+        for (auto &e : fe) HIP_OK(c, hipEventCreate(&e));
+        c->frame_events.push_back(fe);
+    }
     return VE_OK;
 }
 
@@ -543,9 +617,12 @@ int ve_timings(ve_ctx *c, float ms_out[VE_PASS_COUNT])
     double sum[VE_PASS_COUNT] = {0};
     for (size_t f = 0; f < c->frames_recorded; f++)
         for (int i = 0; i < VE_PASS_COUNT; i++) {
-            float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, c->frame_events[f][2 * i], c->frame_events[f][2 * i + 1]) == hipSuccess)
-                sum[i] += ms;
+            const uint32_t n_it = (i == VE_PASS_TRACE || i == VE_PASS_SHADE) ? c->iters_recorded : 1u;
+            for (uint32_t it = 0; it < n_it; it++) {
+                const hipEvent_t *pair = &c->frame_events[f][2 * (i * MAX_ITERS + it)];
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, pair[0], pair[1]) == hipSuccess) sum[i] += ms;
+            }
         }
     for (int i = 0; i < VE_PASS_COUNT; i++)
         ms_out[i] = c->frames_recorded ? (float)(sum[i] / (double)c->frames_recorded) : 0.0f;

@@ -78,11 +78,23 @@ __device__ __forceinline__ f3 primary_dir(const float *cam, uint32_t x, uint32_t
 }
 
 // RaycastCounted / Raycast (internal.glsl:297-400), step for step.
-// `checks` receives q; `reads` accumulates node words fetched (the
-// reference's algorithmic traffic, 4 bytes each).
+// `checks` receives q; `reads` accumulates the node words the reference's
+// traversal fetches (its algorithmic traffic, 4 bytes each).
+//
+// Every DDA step of the reference descends from the root.  The words read at
+// depth dd depend only on coordinate bits D..D-dd (locate() uses bit D-dd and
+// the child offsets come from words on the same path), so when the new
+// lattice position differs from the previous one first in bit hb, depths
+// 0..D-hb-1 read exactly the words they read last step.  The descent resumes
+// at depth D-hb from the `next` pointer cached for depth D-hb-1 (in LDS, one
+// column per thread), or reuses the last outcome outright when it ended above
+// that depth.  Results are identical to the full descent; only the repeated
+// loads are skipped.  `cache` points at this thread's column: entry dd at
+// cache[dd * stride].
 template <bool STATS>
 __device__ __forceinline__ Hit raycast(const uint32_t *__restrict__ words, uint32_t n_words, uint32_t depth, f3 o,
-                                       f3 d, f3 inv, uint32_t controlchecks, uint32_t &checks, uint32_t &reads)
+                                       f3 d, f3 inv, uint32_t controlchecks, uint32_t &checks, uint32_t &reads,
+                                       uint32_t *cache, uint32_t stride)
 {
     Hit miss{0, 0, 0, 0, 0, 0, 0};
     const uint32_t len = 1u << depth;
@@ -105,36 +117,46 @@ __device__ __forceinline__ Hit raycast(const uint32_t *__restrict__ words, uint3
         }
         r = add(muls(d, te), o);
     }
-    uint32_t tpx = 0, tpy = 0, tpz = 0, tsize = 0;
+    // previous step's lattice position and where its descent ended
+    uint32_t px = 0xFFFFFFFFu, py = 0xFFFFFFFFu, pz = 0xFFFFFFFFu;
+    uint32_t pstop = 0;   // depth of the word that ended the last descent (depth == bottom)
     while (in_bounds(r, fl) && q++ <= controlchecks) {
         const uint32_t ux = (uint32_t)r.x, uy = (uint32_t)r.y, uz = (uint32_t)r.z;
-        uint32_t offset = 0, dd = 0;
-        bool found = false;
-        for (; dd < depth; dd++) {
-            offset += locate(ux, uy, uz, len >> dd);
-            const uint32_t raw = offset < n_words ? words[offset] : 0u;
-            if (STATS) reads++;
-            if (!(raw & 1u)) {
-                tsize = len >> dd;
-                found = true;
-                break;
+        const uint32_t diff = (ux ^ px) | (uy ^ py) | (uz ^ pz);
+        // first depth whose word may differ: D - (highest differing bit); the
+        // first step (no previous position) starts at the root
+        const uint32_t hb = 31u - (uint32_t)__clz(diff);
+        const uint32_t resume = diff == 0u ? 0xFFFFFFFFu : (hb >= depth ? 0u : depth - hb);
+        uint32_t dd, tsize;
+        if (resume > pstop) {
+            dd = pstop;   // same words down to the one that ended the last descent
+        } else {
+            uint32_t offset = resume == 0u ? 0u : cache[(resume - 1u) * stride];
+            for (dd = resume; dd < depth; dd++) {
+                offset += locate(ux, uy, uz, len >> dd);
+                const uint32_t raw = offset < n_words ? words[offset] : 0u;
+                if (!(raw & 1u)) break;
+                offset = raw >> 4;
+                cache[dd * stride] = offset;
             }
-            offset = raw >> 4;
-        }
-        if (!found) {
-            offset += locate(ux, uy, uz, 1u);
-            const uint32_t raw = offset < n_words ? words[offset] : 0u;
-            if (STATS) reads++;
-            tsize = 1u;
-            const uint32_t mat = (raw >> 1) & 0x7Fu;
-            if (mat != 0u) {
-                checks = q;
-                return Hit{1u, offset, mat, ux, uy, uz, 1u};
+            if (dd == depth) {   // bottom level: the leaf slot
+                offset += locate(ux, uy, uz, 1u);
+                const uint32_t raw = offset < n_words ? words[offset] : 0u;
+                const uint32_t mat = (raw >> 1) & 0x7Fu;
+                if (mat != 0u) {
+                    if (STATS) reads += depth + 1u;
+                    checks = q;
+                    return Hit{1u, offset, mat, ux, uy, uz, 1u};
+                }
             }
+            pstop = dd;
         }
-        tpx = ux & ~(tsize - 1u);
-        tpy = uy & ~(tsize - 1u);
-        tpz = uz & ~(tsize - 1u);
+        if (STATS) reads += dd + 1u;   // the reference re-reads depths 0..dd every step
+        px = ux;
+        py = uy;
+        pz = uz;
+        tsize = len >> dd;
+        const uint32_t tpx = ux & ~(tsize - 1u), tpy = uy & ~(tsize - 1u), tpz = uz & ~(tsize - 1u);
         // intersect_inside(ray, pos, pos + size), internal.glsl:255-261
         const float bx = (float)tpx, by = (float)tpy, bz = (float)tpz, fs = (float)tsize;
         float t1x = ((bx - o.x) - 0.001f) * inv.x, t2x = (((bx + fs) - o.x) + 0.001f) * inv.x;

@@ -35,6 +35,20 @@ constexpr uint32_t NUM_COUNTERS = 16;
 constexpr uint32_t SCHED_SLOTS = 128;          // ordering table: voxels per claim row before overflow
 constexpr uint32_t MATERIAL_SLOTS = 127;       // shade.comp:90
 
+// Device layout of one hash bucket (one column h of the reference's R32UI
+// images).  The reference interleaves each slot's fields (slot s, field f at
+// image row s*stride + f, and every row is a separate 64 KiB-strided image
+// line); here a bucket is one contiguous block with the 32 owner words first,
+// so finding a voxel id is one 128-byte read instead of up to 32 strided
+// probes.  ve_read / ve_write convert to and from the reference layout.
+//   nBuffer bucket: owner[32] | normal[32] | time[32]                 (96 words)
+//   lBuffer bucket: owner[32] | time[32]   | payload[32][10]          (384 words)
+//                   payload = dCnt dR dG dB dL2 iCnt iR iG iB iL2
+//   rBuffer bucket: owner[16] | time[16]   | payload[16][6]           (128 words)
+constexpr uint32_t NB_NRM = 32, NB_TIME = 64, NB_WORDS = 96;
+constexpr uint32_t LB_TIME = 32, LB_PAY = 64, LB_PAYW = 10, LB_WORDS = 384;
+constexpr uint32_t RB_TIME = 16, RB_PAY = 32, RB_PAYW = 6, RB_WORDS = 128;
+
 // Ray record, 96 bytes, field for field the reference's std430 Ray
 // (wavefront.glsl:128-143).  `pad0` carries the hit voxel extent between
 // trace and shade, as the reference does.

@@ -33,6 +33,9 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
+// The tile/wave mapping exits whole waves early, so the path cache below is
+// only ever touched by live threads; no barrier is needed (each thread owns
+// its column).
 template <bool STATS>
 __global__ __launch_bounds__(256) void k_primary(const uint32_t *__restrict__ words, FrameParams p,
                                                  uint4 *__restrict__ gbuf, uint32_t *__restrict__ gpos,
@@ -46,11 +49,13 @@ __global__ __launch_bounds__(256) void k_primary(const uint32_t *__restrict__ wo
     if (tile >= tiles_x * tiles_y) return;
     const uint32_t x = (tile % tiles_x) * TILE + (lane & 7u);
     const uint32_t y = (tile / tiles_x) * TILE + (lane >> 3);
+    __shared__ uint32_t path_cache[VE_MAX_DEPTH * 256];
     uint32_t checks = 0, reads = 0;
     if (x < p.W && y < p.H) {
         const f3 dir = primary_dir(p.cam, x, y, p.W, p.H);
         const f3 org = mk(p.cam[0], p.cam[1], p.cam[2]);
-        const Hit h = raycast<STATS>(words, p.n_words, p.depth, org, dir, rcp3(dir), p.primary_cc, checks, reads);
+        const Hit h = raycast<STATS>(words, p.n_words, p.depth, org, dir, rcp3(dir), p.primary_cc, checks, reads,
+                                     path_cache + threadIdx.x, 256u);
         const size_t pix = (size_t)y * p.W + x;
         if (h.hit) {
             const uint32_t bits = p.depth - 1u;

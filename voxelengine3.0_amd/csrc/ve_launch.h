@@ -13,6 +13,8 @@ struct DeviceBuffers {
     uint4 *gbuf = nullptr;          // P x (checks, material, 0, vid)
     uint32_t *gpos = nullptr;       // P
     float4 *output = nullptr;       // P
+    float4 *pix_avg = nullptr;      // P: per-pixel voxel average (rgb, indirect variance)
+    uint32_t *pix_cnt = nullptr;    // P: per-pixel voxel sample count
     uint32_t *nbuf = nullptr;       // hash_width x 96
     uint32_t *lbuf = nullptr;       // hash_width x 384
     uint32_t *rbuf = nullptr;       // hash_width x 128

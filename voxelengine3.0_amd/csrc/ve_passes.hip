@@ -57,6 +57,54 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, size_t n)
     if (i < n) p[i] = v;
 }
 
+// First slot s in [from, n) of a bucket's owner words with owner == vid, or
+// (with STOP_AT_EMPTY) owner == 0; n when there is none.  Owners are read
+// four at a time.  `owner` receives the owner word found.
+template <bool STOP_AT_EMPTY>
+__device__ __forceinline__ uint32_t find_owner(const uint32_t *__restrict__ owners, uint32_t from, uint32_t n,
+                                               uint32_t vid, uint32_t &owner)
+{
+    for (uint32_t base = from & ~3u; base < n; base += 4) {
+        const uint4 o = *reinterpret_cast<const uint4 *>(owners + base);
+        const uint32_t v[4] = {o.x, o.y, o.z, o.w};
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t s = base + k;
+            if (s >= from && (v[k] == vid || (STOP_AT_EMPTY && v[k] == 0u))) {
+                owner = v[k];
+                return s;
+            }
+        }
+    }
+    owner = 0xFFFFFFFFu;
+    return n;
+}
+
+// The reference's claim loop `for s: prev = CAS(owner[s], 0, vid); if (prev
+// == 0 || prev == vid) break;` — a CAS on a slot owned by another id changes
+// nothing, so scanning with plain loads and issuing the CAS only at the first
+// free slot gives the same result; a lost race continues the scan.
+// Returns the slot (n when the bucket is full) and whether this call claimed it.
+__device__ __forceinline__ uint32_t claim_slot(uint32_t *__restrict__ owners, uint32_t n, uint32_t vid, bool &claimed)
+{
+    claimed = false;
+    uint32_t from = 0;
+    while (from < n) {
+        uint32_t own;
+        const uint32_t s = find_owner<true>(owners, from, n, vid, own);
+        if (s >= n) return n;
+        if (own == vid) return s;
+        const uint32_t prev = atomicCAS(&owners[s], 0u, vid);
+        if (prev == 0u) {
+            claimed = true;
+            return s;
+        }
+        if (prev == vid) return s;
+        from = s + 1;
+    }
+    return n;
+}
+
 // ---------------------------------------------------------------------------
 // normal.comp
 // ---------------------------------------------------------------------------
@@ -115,30 +163,31 @@ __global__ __launch_bounds__(256) void k_normal(const uint32_t *__restrict__ wor
     const uint32_t pk = gpos[pix], bits = p.depth - 1u, mask = (1u << bits) - 1u;
     const uint32_t hx = (pk & mask) * 2u, hy = ((pk >> bits) & mask) * 2u, hz = ((pk >> (2u * bits)) & mask) * 2u;
     const uint32_t h = vid % (p.hash_width - 1u) + 1u;
-    uint32_t *row = nbuf + (size_t)h * (NBUFFER_STRIDE * NBUFFER_SLOTS);
-    for (uint32_t s = 0; s < NBUFFER_SLOTS; s++) {
-        const uint32_t prev = atomicCAS(&row[s * NBUFFER_STRIDE], 0u, vid);
-        if (prev == 0u || prev == vid) {
-            // The first claimant computes the normal; a later claimant that finds
-            // the field still empty would compute the same value (normal.comp:151-156).
-            if (prev == 0u || __atomic_load_n(&row[s * NBUFFER_STRIDE + 1], __ATOMIC_RELAXED) == 0u) {
-                const f3 nrm = refined_normal(words, p.n_words, p.depth, hx, hy, hz, len, p.normal_precision);
-                atomicCAS(&row[s * NBUFFER_STRIDE + 1], 0u, pack_normal(nrm));
-            }
-            atomicExch(&row[s * NBUFFER_STRIDE + 2], p.time);
-            return;
+    uint32_t *bucket = nbuf + (size_t)h * NB_WORDS;
+    bool claimed;
+    const uint32_t s = claim_slot(bucket, NBUFFER_SLOTS, vid, claimed);
+    if (s < NBUFFER_SLOTS) {
+        // The claimant computes the normal; a later claimant that finds the
+        // field still empty would compute the same value (normal.comp:151-156).
+        if (claimed || __atomic_load_n(&bucket[NB_NRM + s], __ATOMIC_RELAXED) == 0u) {
+            const f3 nrm = refined_normal(words, p.n_words, p.depth, hx, hy, hz, len, p.normal_precision);
+            atomicCAS(&bucket[NB_NRM + s], 0u, pack_normal(nrm));
         }
+        atomicExch(&bucket[NB_TIME + s], p.time);
+        return;
     }
     // bucket full: take the least recently used slot (normal.comp:162-176)
     uint32_t oldest = 0xFFFFFFFFu, oslot = 0;
-    for (uint32_t s = 0; s < NBUFFER_SLOTS; s++) {
-        const uint32_t t = __atomic_load_n(&row[s * NBUFFER_STRIDE + 2], __ATOMIC_RELAXED);
-        if (t < oldest) { oldest = t; oslot = s; }
+    for (uint32_t q = 0; q < NBUFFER_SLOTS; q += 4) {
+        const uint4 t4 = *reinterpret_cast<const uint4 *>(bucket + NB_TIME + q);
+        const uint32_t t[4] = {t4.x, t4.y, t4.z, t4.w};
+        for (uint32_t k = 0; k < 4; k++)
+            if (t[k] < oldest) { oldest = t[k]; oslot = q + k; }
     }
-    atomicExch(&row[oslot * NBUFFER_STRIDE], vid);
+    atomicExch(&bucket[oslot], vid);
     const f3 nrm = refined_normal(words, p.n_words, p.depth, hx, hy, hz, len, p.normal_precision);
-    atomicExch(&row[oslot * NBUFFER_STRIDE + 1], pack_normal(nrm));
-    atomicExch(&row[oslot * NBUFFER_STRIDE + 2], p.time);
+    atomicExch(&bucket[NB_NRM + oslot], pack_normal(nrm));
+    atomicExch(&bucket[NB_TIME + oslot], p.time);
 }
 
 // ---------------------------------------------------------------------------
@@ -163,13 +212,16 @@ __device__ __forceinline__ uint32_t sched_pixel(uint32_t e, const FrameParams &p
 }
 
 // One wave per work group g; lanes with equal voxel ids are merged first so
-// one atomic per distinct id per wave reaches memory.
+// one claim per distinct id per wave reaches memory, and the whole wave scans
+// the id's table row (128 slots, two per lane) with one coalesced read: the
+// first slot holding the id or 0 is taken, a lost CAS race moves past it.
 __global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *__restrict__ gbuf,
                                                     uint32_t *__restrict__ tab_vid, uint32_t *__restrict__ tab_first,
                                                     uint32_t *__restrict__ tab_cnt, uint32_t *__restrict__ cand_at,
                                                     uint32_t *__restrict__ counters)
 {
-    const uint32_t e = blockIdx.x * 64u + threadIdx.x;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t e = blockIdx.x * 64u + lane;
     cand_at[e] = NONE;
     uint32_t pix;
     uint32_t vid = 0u;
@@ -178,23 +230,39 @@ __global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *
     while (pending) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
         const uint32_t lv = __shfl(vid, leader, 64);
-        const uint64_t same = __ballot(vid == lv && vid != 0u);
+        const uint64_t same = __ballot(vid == lv);
         pending &= ~same;
-        if ((int)threadIdx.x == leader) {
-            // lowest lane of the group = lowest execution rank of this id here
-            const uint32_t h = claim_hash(lv) % CLAIM_WIDTH;
-            uint32_t *rv = tab_vid + (size_t)h * SCHED_SLOTS;
-            bool placed = false;
-            for (uint32_t s = 0; s < SCHED_SLOTS; s++) {
-                const uint32_t prev = atomicCAS(&rv[s], 0u, lv);
-                if (prev == 0u || prev == lv) {
-                    atomicMin(&tab_first[(size_t)h * SCHED_SLOTS + s], e);
-                    atomicAdd(&tab_cnt[(size_t)h * SCHED_SLOTS + s], (uint32_t)__popcll(same));
-                    placed = true;
-                    break;
-                }
+        const uint32_t h = claim_hash(lv) % CLAIM_WIDTH;
+        uint32_t *row = tab_vid + (size_t)h * SCHED_SLOTS;
+        const uint32_t a = row[lane], b = row[64u + lane];
+        uint32_t from = 0, slot = SCHED_SLOTS;
+        while (from < SCHED_SLOTS) {
+            const uint64_t ma = __ballot((a == lv || a == 0u) && lane >= from);
+            const uint64_t mb = __ballot((b == lv || b == 0u) && lane + 64u >= from);
+            if (!ma && !mb) break;
+            const int first = ma ? __ffsll((unsigned long long)ma) - 1 : 64 + __ffsll((unsigned long long)mb) - 1;
+            const uint32_t val = first < 64 ? __shfl(a, first, 64) : __shfl(b, first - 64, 64);
+            if (val == lv) {
+                slot = (uint32_t)first;
+                break;
             }
-            if (!placed) atomicOr(&counters[VE_CNT_OVERFLOW], 1u);
+            uint32_t prev = 0u;
+            if (lane == 0u) prev = atomicCAS(&row[first], 0u, lv);
+            prev = __shfl(prev, 0, 64);
+            if (prev == 0u || prev == lv) {
+                slot = (uint32_t)first;
+                break;
+            }
+            from = (uint32_t)first + 1u;
+        }
+        if (lane == 0u) {
+            if (slot < SCHED_SLOTS) {
+                // the group's lowest lane of this id has the lowest execution rank
+                atomicMin(&tab_first[(size_t)h * SCHED_SLOTS + slot], blockIdx.x * 64u + (uint32_t)leader);
+                atomicAdd(&tab_cnt[(size_t)h * SCHED_SLOTS + slot], (uint32_t)__popcll(same));
+            } else {
+                atomicOr(&counters[VE_CNT_OVERFLOW], 1u);
+            }
         }
     }
 }
@@ -296,20 +364,17 @@ __global__ void k_sched_write(FrameParams p, const uint32_t *__restrict__ cand_a
 __device__ float indirect_variance(const uint32_t *__restrict__ lbuf, uint32_t hw, uint32_t vid)   // rank.comp:66-87
 {
     const uint32_t h = vid % (hw - 1u) + 1u;
-    const uint32_t *row = lbuf + (size_t)h * (LBUFFER_STRIDE * LBUFFER_SLOTS);
-    for (uint32_t s = 0; s < LBUFFER_SLOTS; s++) {
-        const uint32_t *sl = row + s * LBUFFER_STRIDE;
-        const uint32_t owner = sl[0];
-        if (owner == 0u) return 0.0f;
-        if (owner != vid) continue;
-        const uint32_t c = sl[6];
-        if (c < 2u) return 0.0f;
-        const float fc = (float)c;
-        const float meanL = (float)(sl[7] + sl[8] + sl[9]) / (3.0f * fc);
-        const float meanL2 = (float)sl[10] / fc;
-        return gmax(meanL2 - meanL * meanL, 0.0f);
-    }
-    return 0.0f;
+    const uint32_t *bucket = lbuf + (size_t)h * LB_WORDS;
+    uint32_t own;
+    const uint32_t s = find_owner<true>(bucket, 0, LBUFFER_SLOTS, vid, own);
+    if (s >= LBUFFER_SLOTS || own == 0u) return 0.0f;   // not present yet
+    const uint32_t *pl = bucket + LB_PAY + s * LB_PAYW;
+    const uint32_t c = pl[5];
+    if (c < 2u) return 0.0f;
+    const float fc = (float)c;
+    const float meanL = (float)(pl[6] + pl[7] + pl[8]) / (3.0f * fc);
+    const float meanL2 = (float)pl[9] / fc;
+    return gmax(meanL2 - meanL * meanL, 0.0f);
 }
 
 // Per candidate: samples (0/1/2/minSamples), histogram, min/max; per SIMD-8
@@ -328,7 +393,9 @@ __global__ __launch_bounds__(256) void k_rank(FrameParams p, const uint4 *__rest
     if (t < n) {
         const uint4 cd = candidates[t];
         const uint32_t pixels = samples_map[(size_t)cd.z * CLAIM_PROBES + cd.w];
-        const float var = indirect_variance(lbuf, p.hash_width, cd.x);
+        // with a zero weight the variance term is 0 * vn with vn in [0, 1]
+        // (clamp maps NaN to 0), so skipping the lookup is exact
+        const float var = p.w_variance != 0.0f ? indirect_variance(lbuf, p.hash_width, cd.x) : 0.0f;
         const float pn = gclamp((float)pixels / gmax(p.pixels_norm, 1.0f), 0.0f, 1.0f);
         const float vn = gclamp(var / gmax(p.variance_norm, 1.0f), 0.0f, 1.0f);
         const float imp = p.w_pixels * pn + p.w_variance * vn;
@@ -465,12 +532,14 @@ __global__ __launch_bounds__(256) void k_trace(const uint32_t *__restrict__ word
         atomicMax(&counters[VE_CNT_BOUNCES_RUN], iter + 1u);
         atomicAdd(&totals[1], (unsigned long long)live);
     }
+    __shared__ uint32_t path_cache[VE_MAX_DEPTH * 256];
     uint32_t checks = 0, reads = 0;
     if (t < live) {
         Ray &r = rays[t];
         if (r.bounce < p.max_bounces) {
             const f3 o{r.ox, r.oy, r.oz}, d{r.dx, r.dy, r.dz};
-            const Hit h = raycast<STATS>(words, p.n_words, p.depth, o, d, rcp3(d), p.bounce_cc, checks, reads);
+            const Hit h = raycast<STATS>(words, p.n_words, p.depth, o, d, rcp3(d), p.bounce_cc, checks, reads,
+                                         path_cache + threadIdx.x, 256u);
             if (h.hit) {
                 r.ox = (float)h.px;
                 r.oy = (float)h.py;
@@ -531,14 +600,15 @@ __device__ __forceinline__ void firefly(const uint32_t *sl, uint32_t cf, uint4 &
     }
 }
 
-__device__ void reset_lslot(uint32_t *sl, uint32_t vid, uint4 d, uint4 e, uint32_t time)
+__device__ void reset_lslot(uint32_t *bucket, uint32_t s, uint32_t vid, uint4 d, uint4 e, uint32_t time)
 {
-    atomicExch(&sl[0], vid);
-    atomicExch(&sl[1], 1u); atomicExch(&sl[2], d.x); atomicExch(&sl[3], d.y); atomicExch(&sl[4], d.z);
-    atomicExch(&sl[5], d.w);
-    atomicExch(&sl[6], 1u); atomicExch(&sl[7], e.x); atomicExch(&sl[8], e.y); atomicExch(&sl[9], e.z);
-    atomicExch(&sl[10], e.w);
-    atomicExch(&sl[11], time);
+    uint32_t *pl = bucket + LB_PAY + s * LB_PAYW;
+    atomicExch(&bucket[s], vid);
+    atomicExch(&pl[0], 1u); atomicExch(&pl[1], d.x); atomicExch(&pl[2], d.y); atomicExch(&pl[3], d.z);
+    atomicExch(&pl[4], d.w);
+    atomicExch(&pl[5], 1u); atomicExch(&pl[6], e.x); atomicExch(&pl[7], e.y); atomicExch(&pl[8], e.z);
+    atomicExch(&pl[9], e.w);
+    atomicExch(&bucket[LB_TIME + s], time);
 }
 
 // depositSampleDual (shade.comp:162-257) = AccumPass: integer fixed-point sums
@@ -549,47 +619,51 @@ __device__ void deposit(const FrameParams &p, uint32_t *__restrict__ lbuf, uint3
     uint4 d = pack_rgb_l2(dir_rgb);
     uint4 e = pack_rgb_l2(ind_rgb);
     const uint32_t h = vid % (p.hash_width - 1u) + 1u;
-    uint32_t *row = lbuf + (size_t)h * (LBUFFER_STRIDE * LBUFFER_SLOTS);
-    uint32_t cand_t = 0xFFFFFFFFu, cand_i = 0;
-    bool have = false;
-    for (uint32_t i = 0; i < LBUFFER_STRIDE * LBUFFER_SLOTS; i += LBUFFER_STRIDE) {
-        uint32_t *sl = row + i;
-        const uint32_t prev = atomicCAS(&sl[0], 0u, vid);
-        const uint32_t id = prev == 0u ? vid : prev;
-        const uint32_t st = __atomic_load_n(&sl[11], __ATOMIC_RELAXED);
-        if (id != vid) {
-            if (st < cand_t) { cand_t = st; cand_i = i; have = true; }
-            continue;
+    uint32_t *bucket = lbuf + (size_t)h * LB_WORDS;
+    bool claimed;
+    const uint32_t s = claim_slot(bucket, LBUFFER_SLOTS, vid, claimed);
+    if (s >= LBUFFER_SLOTS) {
+        // every slot owned by another id: reset the least recently written one
+        uint32_t cand_t = 0xFFFFFFFFu, cand = LBUFFER_SLOTS;
+        for (uint32_t q = 0; q < LBUFFER_SLOTS; q += 4) {
+            const uint4 t4 = *reinterpret_cast<const uint4 *>(bucket + LB_TIME + q);
+            const uint32_t t[4] = {t4.x, t4.y, t4.z, t4.w};
+            for (uint32_t k = 0; k < 4; k++)
+                if (t[k] < cand_t) { cand_t = t[k]; cand = q + k; }
         }
-        if (!p.disable_stale_reset && st != 0u && st + p.update_time < p.time) {
-            reset_lslot(sl, vid, d, e, p.time);
-            return;
-        }
-        if (__atomic_load_n(&sl[1], __ATOMIC_RELAXED) >= p.cap_direct)
-            for (int f = 1; f <= 5; f++) atomicExch(&sl[f], __atomic_load_n(&sl[f], __ATOMIC_RELAXED) >> 1);
-        if (__atomic_load_n(&sl[6], __ATOMIC_RELAXED) >= p.cap_indirect)
-            for (int f = 6; f <= 10; f++) atomicExch(&sl[f], __atomic_load_n(&sl[f], __ATOMIC_RELAXED) >> 1);
-        firefly(sl, 1, d, p.firefly_k, p.firefly_floor);
-        firefly(sl, 6, e, p.firefly_k, p.firefly_floor);
-        atomicAdd(&sl[1], 1u); atomicAdd(&sl[2], d.x); atomicAdd(&sl[3], d.y); atomicAdd(&sl[4], d.z);
-        atomicAdd(&sl[5], d.w);
-        atomicAdd(&sl[6], 1u); atomicAdd(&sl[7], e.x); atomicAdd(&sl[8], e.y); atomicAdd(&sl[9], e.z);
-        atomicAdd(&sl[10], e.w);
-        atomicExch(&sl[11], p.time);
+        if (cand < LBUFFER_SLOTS) reset_lslot(bucket, cand, vid, d, e, p.time);
         return;
     }
-    if (have) reset_lslot(row + cand_i, vid, d, e, p.time);
+    uint32_t *pl = bucket + LB_PAY + s * LB_PAYW;
+    const uint32_t st = __atomic_load_n(&bucket[LB_TIME + s], __ATOMIC_RELAXED);
+    if (!p.disable_stale_reset && st != 0u && st + p.update_time < p.time) {
+        reset_lslot(bucket, s, vid, d, e, p.time);
+        return;
+    }
+    if (__atomic_load_n(&pl[0], __ATOMIC_RELAXED) >= p.cap_direct)
+        for (int f = 0; f <= 4; f++) atomicExch(&pl[f], __atomic_load_n(&pl[f], __ATOMIC_RELAXED) >> 1);
+    if (__atomic_load_n(&pl[5], __ATOMIC_RELAXED) >= p.cap_indirect)
+        for (int f = 5; f <= 9; f++) atomicExch(&pl[f], __atomic_load_n(&pl[f], __ATOMIC_RELAXED) >> 1);
+    firefly(pl, 0, d, p.firefly_k, p.firefly_floor);
+    firefly(pl, 5, e, p.firefly_k, p.firefly_floor);
+    atomicAdd(&pl[0], 1u); atomicAdd(&pl[1], d.x); atomicAdd(&pl[2], d.y); atomicAdd(&pl[3], d.z);
+    atomicAdd(&pl[4], d.w);
+    atomicAdd(&pl[5], 1u); atomicAdd(&pl[6], e.x); atomicAdd(&pl[7], e.y); atomicAdd(&pl[8], e.z);
+    atomicAdd(&pl[9], e.w);
+    atomicExch(&bucket[LB_TIME + s], p.time);
 }
 
 __device__ f3 lookup_normal(const uint32_t *__restrict__ nbuf, uint32_t hw, uint32_t vid, f3 rd)   // shade.comp:261-275
 {
     const uint32_t h = vid % (hw - 1u) + 1u;
-    const uint32_t *row = nbuf + (size_t)h * (NBUFFER_STRIDE * NBUFFER_SLOTS);
-    for (uint32_t s = 0; s < NBUFFER_SLOTS; s++) {
-        if (row[s * NBUFFER_STRIDE] == vid) {
-            const uint32_t nf = row[s * NBUFFER_STRIDE + 1];
-            if (nf != 0u) return normalize(unpack_normal(nf));
-        }
+    const uint32_t *bucket = nbuf + (size_t)h * NB_WORDS;
+    uint32_t own, from = 0;
+    for (;;) {   // every slot owned by vid, in order, until one has a normal
+        const uint32_t s = find_owner<false>(bucket, from, NBUFFER_SLOTS, vid, own);
+        if (s >= NBUFFER_SLOTS) break;
+        const uint32_t nf = bucket[NB_NRM + s];
+        if (nf != 0u) return normalize(unpack_normal(nf));
+        from = s + 1;
     }
     const float ax = fabsf(rd.x), ay = fabsf(rd.y), az = fabsf(rd.z);
     if (ax > ay && ax > az) return f3{-gsign(rd.x), 0.0f, 0.0f};
@@ -725,33 +799,49 @@ __device__ Avg voxel_avg(const uint32_t *__restrict__ lbuf, uint32_t hw, uint32_
     Avg a{{0, 0, 0}, 0u, 0.0f};
     if (vid == 0u) return a;
     const uint32_t h = vid % (hw - 1u) + 1u;
-    const uint32_t *row = lbuf + (size_t)h * (LBUFFER_STRIDE * LBUFFER_SLOTS);
-    for (uint32_t i = 0; i < LBUFFER_STRIDE * LBUFFER_SLOTS; i += LBUFFER_STRIDE) {
-        const uint32_t *sl = row + i;
-        if (sl[0] != vid) continue;
-        const uint32_t dC = sl[1], iC = sl[6];
-        a.count = dC + iC;
-        if (a.count == 0u) return a;
-        f3 da{0, 0, 0}, ia{0, 0, 0};
-        if (dC > 0u) {
-            const float inv = 1.0f / (float)dC;
-            da = divs(muls(f3{(float)sl[2], (float)sl[3], (float)sl[4]}, inv), 255.0f);
-        }
-        if (iC > 0u) {
-            const uint32_t ir = sl[7], ig = sl[8], ib = sl[9], il2 = sl[10];
-            const float inv = 1.0f / (float)iC;
-            ia = divs(muls(f3{(float)ir, (float)ig, (float)ib}, inv), 255.0f);
-            const float meanL = (float)(ir + ig + ib) * inv / 3.0f;
-            a.var = gmax((float)il2 * inv - meanL * meanL, 0.0f) / 65025.0f;
-        }
-        a.c = add(da, ia);
-        return a;
+    const uint32_t *bucket = lbuf + (size_t)h * LB_WORDS;
+    uint32_t own;
+    const uint32_t s = find_owner<false>(bucket, 0, LBUFFER_SLOTS, vid, own);
+    if (s >= LBUFFER_SLOTS) return a;
+    const uint32_t *pl = bucket + LB_PAY + s * LB_PAYW;
+    const uint32_t dC = pl[0], iC = pl[5];
+    a.count = dC + iC;
+    if (a.count == 0u) return a;
+    f3 da{0, 0, 0}, ia{0, 0, 0};
+    if (dC > 0u) {
+        const float inv = 1.0f / (float)dC;
+        da = divs(muls(f3{(float)pl[1], (float)pl[2], (float)pl[3]}, inv), 255.0f);
     }
+    if (iC > 0u) {
+        const uint32_t ir = pl[6], ig = pl[7], ib = pl[8], il2 = pl[9];
+        const float inv = 1.0f / (float)iC;
+        ia = divs(muls(f3{(float)ir, (float)ig, (float)ib}, inv), 255.0f);
+        const float meanL = (float)(ir + ig + ib) * inv / 3.0f;
+        a.var = gmax((float)il2 * inv - meanL * meanL, 0.0f) / 65025.0f;
+    }
+    a.c = add(da, ia);
     return a;
 }
 
-__device__ f3 reconstruct_timeout(const FrameParams &p, const uint4 *__restrict__ gbuf, const uint32_t *__restrict__ lbuf,
-                                  int px, int py)   // resolve.comp:193-219
+// AvgPass part 1: the voxel average of every pixel's voxel id, once per pixel.
+// lookupVoxelAvg is a pure function of the id, so the bilateral taps and the
+// timeout gather of part 2 read these per-pixel values instead of probing the
+// lBuffer hash again for each tap (up to 440 taps for a timed-out pixel).
+__global__ __launch_bounds__(256) void k_resolve_avg(FrameParams p, const uint4 *__restrict__ gbuf,
+                                                     const uint32_t *__restrict__ lbuf, float4 *__restrict__ pix_avg,
+                                                     uint32_t *__restrict__ pix_cnt)
+{
+    uint32_t x, y;
+    if (!tile_pixel(p, x, y)) return;
+    const size_t pix = (size_t)y * p.W + x;
+    const Avg a = voxel_avg(lbuf, p.hash_width, gbuf[pix].w);
+    pix_avg[pix] = make_float4(a.c.x, a.c.y, a.c.z, a.var);
+    pix_cnt[pix] = a.count;
+}
+
+__device__ f3 reconstruct_timeout(const FrameParams &p, const uint4 *__restrict__ gbuf,
+                                  const float4 *__restrict__ pix_avg, const uint32_t *__restrict__ pix_cnt, int px,
+                                  int py)   // resolve.comp:193-219
 {
     f3 sum{0, 0, 0};
     float wsum = 0.0f;
@@ -761,21 +851,25 @@ __device__ f3 reconstruct_timeout(const FrameParams &p, const uint4 *__restrict_
             if (dx == 0 && dy == 0) continue;
             const int qx = px + dx, qy = py + dy;
             if (qx < 0 || qy < 0 || qx >= (int)p.W || qy >= (int)p.H) continue;
-            const uint4 nd = gbuf[(size_t)qy * p.W + qx];
+            const size_t q = (size_t)qy * p.W + qx;
+            const uint4 nd = gbuf[q];
             if (nd.w == 0u || nd.x >= p.primary_cc) continue;
-            const Avg na = voxel_avg(lbuf, p.hash_width, nd.w);
-            if (na.count == 0u) continue;
+            const uint32_t nc = pix_cnt[q];
+            if (nc == 0u) continue;
+            const float4 na = pix_avg[q];
             const float dist = (float)(dx * dx + dy * dy);
-            const float w = (float)min(na.count, 32u) / (1.0f + dist * 0.15f);
-            sum = add(sum, muls(na.c, w));
+            const float w = (float)min(nc, 32u) / (1.0f + dist * 0.15f);
+            sum = add(sum, muls(f3{na.x, na.y, na.z}, w));
             wsum += w;
         }
     return wsum > 0.0001f ? divs(sum, wsum) : f3{p.sky[0], p.sky[1], p.sky[2]};
 }
 
+// AvgPass part 2 (resolve.comp:362-382): per-pixel colour of the selected view.
 __global__ __launch_bounds__(256) void k_resolve(FrameParams p, const uint4 *__restrict__ gbuf,
                                                  const uint32_t *__restrict__ lbuf, const uint32_t *__restrict__ nbuf,
-                                                 const Material *__restrict__ mats, float4 *__restrict__ out)
+                                                 const Material *__restrict__ mats, const float4 *__restrict__ pix_avg,
+                                                 const uint32_t *__restrict__ pix_cnt, float4 *__restrict__ out)
 {
     uint32_t x, y;
     if (!tile_pixel(p, x, y)) return;
@@ -789,63 +883,52 @@ __global__ __launch_bounds__(256) void k_resolve(FrameParams p, const uint4 *__r
         const float v = (float)checks / 65.0f;
         col = f3{v, v, v};
     } else if (vid == 0u) {
-        col = timeout ? reconstruct_timeout(p, gbuf, lbuf, (int)x, (int)y) : sky;
+        col = timeout ? reconstruct_timeout(p, gbuf, pix_avg, pix_cnt, (int)x, (int)y) : sky;
     } else if (p.render_type == 2) {
         col = (mi == 0u || mi >= 127u) ? f3{0, 0, 0} : f3{mats[mi].color[0], mats[mi].color[1], mats[mi].color[2]};
     } else if (p.render_type == 3) {
-        col = f3{0, 0, 0};
+        col = f3{0, 0, 0};   // resolve.comp:152-166: the first slot owned by vid decides
         const uint32_t h = vid % (p.hash_width - 1u) + 1u;
-        const uint32_t *row = nbuf + (size_t)h * (NBUFFER_STRIDE * NBUFFER_SLOTS);
-        for (uint32_t s = 0; s < NBUFFER_SLOTS; s++) {
-            if (row[s * NBUFFER_STRIDE] == vid) {
-                const uint32_t f = row[s * NBUFFER_STRIDE + 1];
-                if (f != 0u) col = add(muls(normalize(unpack_normal(f)), 0.5f), f3{0.5f, 0.5f, 0.5f});
-                break;
-            }
+        const uint32_t *bucket = nbuf + (size_t)h * NB_WORDS;
+        uint32_t own;
+        const uint32_t s = find_owner<false>(bucket, 0, NBUFFER_SLOTS, vid, own);
+        if (s < NBUFFER_SLOTS) {
+            const uint32_t f = bucket[NB_NRM + s];
+            if (f != 0u) col = add(muls(normalize(unpack_normal(f)), 0.5f), f3{0.5f, 0.5f, 0.5f});
         }
     } else if (p.render_type == 4) {
         const uint32_t hid = vid - 1u;
         col = f3{(float)((hid + 1u) % 255u) / 255.0f, (float)(((hid + 1u) / 255u) % 255u) / 255.0f,
                  (float)(((hid + 1u) / (255u * 255u)) % 255u) / 255.0f};
     } else if (p.render_type == 5) {
-        float var = 0.0f;
-        const uint32_t h = vid % (p.hash_width - 1u) + 1u;
-        const uint32_t *row = lbuf + (size_t)h * (LBUFFER_STRIDE * LBUFFER_SLOTS);
-        for (uint32_t i = 0; i < LBUFFER_STRIDE * LBUFFER_SLOTS; i += LBUFFER_STRIDE) {
-            const uint32_t *sl = row + i;
-            if (sl[0] == 0u) break;
-            if (sl[0] != vid) continue;
-            const uint32_t c = sl[6];
-            if (c >= 2u) {
-                const float fc = (float)c;
-                const float meanL = (float)(sl[7] + sl[8] + sl[9]) / (3.0f * fc);
-                const float meanL2 = (float)sl[10] / fc;
-                var = gmax(meanL2 - meanL * meanL, 0.0f) / 65025.0f;
-            }
-            break;
-        }
+        const float var = indirect_variance(lbuf, p.hash_width, vid) / 65025.0f;   // resolve.comp:302-322
         const float sc = gclamp(var * 65025.0f / gmax(p.variance_norm, 1.0f), 0.0f, 1.0f);
         col = f3{sc, sc, sc};
     } else {   // DEFAULT (resolve.comp:224-261)
-        const Avg c = voxel_avg(lbuf, p.hash_width, vid);
-        if (c.count >= TRUSTED_THRESHOLD && c.var < 0.02f) {
-            col = c.c;
+        const float4 ca = pix_avg[pix];
+        const uint32_t cc = pix_cnt[pix];
+        const f3 cavg{ca.x, ca.y, ca.z};
+        if (cc >= TRUSTED_THRESHOLD && ca.w < 0.02f) {
+            col = cavg;
         } else {
-            f3 sum = muls(c.c, (float)c.count);
-            float wsum = (float)c.count;
+            f3 sum = muls(cavg, (float)cc);
+            float wsum = (float)cc;
             const int OX[4] = {1, -1, 0, 0}, OY[4] = {0, 0, 1, -1};
             for (int k = 0; k < 4; k++) {
                 const int qx = (int)x + OX[k], qy = (int)y + OY[k];
                 if (qx < 0 || qy < 0 || qx >= (int)p.W || qy >= (int)p.H) continue;
-                const uint32_t nvid = gbuf[(size_t)qy * p.W + qx].w;
+                const size_t q = (size_t)qy * p.W + qx;
+                const uint32_t nvid = gbuf[q].w;
                 if (nvid == 0u || nvid == vid) continue;
-                const Avg na = voxel_avg(lbuf, p.hash_width, nvid);
-                if (na.count == 0u) continue;
-                const float dc = length3(sub(na.c, c.c));
+                const uint32_t nc = pix_cnt[q];
+                if (nc == 0u) continue;
+                const float4 na = pix_avg[q];
+                const f3 navg{na.x, na.y, na.z};
+                const float dc = length3(sub(navg, cavg));
                 const float wc = m_exp(-dc * 6.0f);
-                const float wv = 1.0f / (1.0f + na.var * 32.0f);
-                const float w = wc * wv * (float)min(na.count, 32u);
-                sum = add(sum, muls(na.c, w));
+                const float wv = 1.0f / (1.0f + na.w * 32.0f);
+                const float w = wc * wv * (float)min(nc, 32u);
+                sum = add(sum, muls(navg, w));
                 wsum += w;
             }
             col = wsum > 0.0001f ? divs(sum, wsum) : f3{0, 0, 0};
@@ -987,8 +1070,9 @@ void launch_shade(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_
 
 void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 {
+    hipLaunchKernelGGL(k_resolve_avg, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.pix_avg, b.pix_cnt);
     hipLaunchKernelGGL(k_resolve, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.nbuf, b.materials,
-                       b.output);
+                       b.pix_avg, b.pix_cnt, b.output);
 }
 
 size_t scan_temp_bytes(uint32_t max_items)

@@ -232,9 +232,11 @@ def camera_small():
     return look_at((64.0, 96.0, -40.0), (64.0, 64.0, 64.0)) + (70.0,)
 
 
-def orbit_camera(n, frame, frames, seed, radius_frac=0.8, height_frac=0.75, fov=70.0):
-    """Seeded orbit around the grid centre: radius radius_frac*N in xz,
-    eye height height_frac*N, phase offset from the seed."""
+def orbit_camera(n, frame, frames, seed, radius_frac=0.8, height_frac=0.75, fov=40.0):
+    """Seeded orbit around the grid centre: radius radius_frac*N in xz, eye
+    height height_frac*N, looking at (N/2, 0.3N, N/2), phase offset from the
+    seed.  The 40 degree vertical FOV makes the grid fill a 16:9 frame (about
+    73% of the pixels hit geometry on the depth-10 scene)."""
     phase = 2.0 * math.pi * float(unit(hash4(seed, 7, 0x0B, 0)))
     a = phase + 2.0 * math.pi * frame / frames
     c = n / 2.0

@@ -44,6 +44,20 @@ def knobs():
                 sample_cap_direct=1024, sample_cap_indirect=4096, disable_stale_reset=1, update_time=0xFFFFFFFF)
 
 
+def profiled_traffic(kernel="k_primary"):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "k_primary_extreme_1080p.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("TRAFFIC %s " % kernel):
+                    return int(line.split()[2])
+    except OSError:
+        pass
+    return None
+
+
 def frame_seed(f):
     return (1 + 7919 * f) & 0xFFFFFFFF
 
@@ -80,7 +94,19 @@ def main():
     ap.add_argument("--steps", type=int, default=128)
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 256 MiB device copy (4 launches), the reference kernel for scaling "
+                         "the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(4):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": x.numel() * 4, "launches": 4}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X, and the "
               "multi-GPU form (whole sessions per GPU, octree broadcast over xGMI) is not built; run with --gpus 1",
@@ -190,7 +216,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic(),
+            "traffic_source": "profiles/k_primary_extreme_1080p.md (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
             "algorithmic_bytes_per_launch": round(bytes_per_launch),
             "launch_ms": round(prim_ms, 4),
             "node_reads_per_ray": round(st["primary_reads"] / n_stat / (W * H), 3),

@@ -1,0 +1,194 @@
+// ve_svo.hpp — C++ host classes over the C ABI, shaped like VoxelEngine3.0's
+// host API so calling code can switch by name:
+//
+//   ve::OctreeCPU   OctreeCPU::insert / insertBox / insertSphere / insertFunction
+//                   (src/octree_cpu.hpp:30-37): records the insert calls in order
+//   ve::Octree      Octree::set(OctreeCPU*) (src/renderer/octree.hpp:50): node words
+//   ve::camera_ubo  Camera::UpdateUBO (src/renderer/camera.cpp:39-50)
+//   ve::Renderer    Renderer(...) + bool run(FrameConfig*) (src/renderer/renderer.hpp:11-13)
+//
+// Header-only; link with libve_svo.so.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ve_svo.h"
+
+namespace ve {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc, const ve_ctx *ctx = nullptr)
+{
+    if (rc != VE_OK) throw Error(rc, ve_last_error(ctx));
+}
+
+class OctreeCPU {
+public:
+    explicit OctreeCPU(uint8_t depth_) : depth(depth_) {}
+
+    // OctreeCPU::insert (octree_cpu.cpp:40-48): out-of-range positions are
+    // ignored by the build; the last insert of a position wins.
+    void insert(uint32_t x, uint32_t y, uint32_t z, uint32_t material) { ops.push_back({x, y, z, material}); }
+
+    // octree_cpu.cpp:221-230
+    void insertBox(const uint32_t mn[3], const uint32_t mx_in[3], uint32_t material)
+    {
+        const uint32_t bound = 1u << depth;
+        uint32_t mx[3];
+        for (int i = 0; i < 3; i++) mx[i] = mx_in[i] < bound ? mx_in[i] : bound;
+        for (uint32_t x = mn[0]; x < mx[0]; x++)
+            for (uint32_t y = mn[1]; y < mx[1]; y++)
+                for (uint32_t z = mn[2]; z < mx[2]; z++) insert(x, y, z, material);
+    }
+
+    // octree_cpu.cpp:232-248, FP32 distance test as glm::distance
+    void insertSphere(const float c[3], float r, uint32_t material)
+    {
+        const int32_t bound = 1 << depth;
+        int32_t lo[3], hi[3];
+        for (int i = 0; i < 3; i++) {
+            lo[i] = std::max<int32_t>(0, (int32_t)std::floor(c[i] - r));
+            hi[i] = std::min<int32_t>(bound - 1, (int32_t)std::ceil(c[i] + r));
+        }
+        for (int32_t x = lo[0]; x <= hi[0]; x++)
+            for (int32_t y = lo[1]; y <= hi[1]; y++)
+                for (int32_t z = lo[2]; z <= hi[2]; z++) {
+                    const float dx = ((float)x + 0.5f) - c[0], dy = ((float)y + 0.5f) - c[1],
+                                dz = ((float)z + 0.5f) - c[2];
+                    const float t0 = dx * dx, t1 = dy * dy, t2 = dz * dz;
+                    if (std::sqrt((t0 + t1) + t2) <= r) insert((uint32_t)x, (uint32_t)y, (uint32_t)z, material);
+                }
+    }
+
+    // octree_cpu.cpp:250-262: fn is evaluated at voxel centres
+    void insertFunction(const std::function<bool(const float p[3])> &fn, const uint32_t mn[3], const uint32_t mx_in[3],
+                        uint32_t material)
+    {
+        const uint32_t bound = 1u << depth;
+        uint32_t mx[3];
+        for (int i = 0; i < 3; i++) mx[i] = mx_in[i] < bound ? mx_in[i] : bound;
+        for (uint32_t x = mn[0]; x < mx[0]; x++)
+            for (uint32_t y = mn[1]; y < mx[1]; y++)
+                for (uint32_t z = mn[2]; z < mx[2]; z++) {
+                    const float p[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
+                    if (fn(p)) insert(x, y, z, material);
+                }
+    }
+
+    uint8_t depth;
+    std::vector<ve_insert> ops;
+};
+
+class Octree {
+public:
+    explicit Octree(uint8_t depth_, uint32_t max_words = 1u << 27) : depth(depth_), max_words_(max_words) {}
+
+    // Octree::set (octree.cpp:127-166)
+    void set(const OctreeCPU &cpu)
+    {
+        uint32_t *w = nullptr;
+        ve_octree_info info{};
+        check(ve_octree_build(cpu.ops.data(), cpu.ops.size(), depth, max_words_, &w, &info));
+        data.assign(w, w + info.capacity);
+        ve_free(w);
+        size = info.size;
+        capacity = info.capacity;
+        numVoxels = info.num_voxels;
+    }
+
+    uint8_t depth;
+    std::vector<uint32_t> data;
+    uint32_t size = 8, capacity = 8, numVoxels = 0;
+
+private:
+    uint32_t max_words_;
+};
+
+struct CameraUBO {
+    float v[16];
+};
+
+inline CameraUBO camera_ubo(const float position[3], const float direction[3], float fov_deg, float aspect)
+{
+    CameraUBO u{};
+    ve_camera_ubo(position, direction, fov_deg, aspect, u.v);
+    return u;
+}
+
+inline ve_frame_config default_frame_config()
+{
+    ve_frame_config fc;
+    ve_frame_config_defaults(&fc);
+    return fc;
+}
+
+class Renderer {
+public:
+    Renderer(uint32_t width, uint32_t height, uint32_t max_bounces = 8, int device = 0, uint32_t hash_width = 16384)
+    {
+        ve_renderer_config cfg{device, width, height, hash_width, max_bounces};
+        check(ve_create(&ctx_, &cfg));
+        width_ = width;
+        height_ = height;
+    }
+    ~Renderer()
+    {
+        if (scene_) ve_scene_release(ctx_, scene_);
+        ve_destroy(ctx_);
+    }
+    Renderer(const Renderer &) = delete;
+    Renderer &operator=(const Renderer &) = delete;
+
+    void setOctree(const Octree &o)
+    {
+        if (scene_) check(ve_scene_release(ctx_, scene_), ctx_);
+        scene_ = nullptr;
+        check(ve_scene_upload(ctx_, o.data.data(), o.size, o.depth, &scene_), ctx_);
+    }
+    void setCamera(const CameraUBO &u) { check(ve_set_camera(ctx_, u.v), ctx_); }
+    void setMaterials(const void *std140, uint32_t n) { check(ve_set_materials(ctx_, std140, n), ctx_); }
+    void setSky(const uint8_t rgb[3])
+    {
+        const uint8_t *faces[6] = {rgb, rgb, rgb, rgb, rgb, rgb};
+        check(ve_set_sky(ctx_, faces, 1), ctx_);
+    }
+
+    // Renderer::run (renderer.cpp:654-712) minus the display blit: one
+    // wavefront frame; returns false (and keeps the message) on error, like
+    // the reference's `success` flag.
+    bool run(const ve_frame_config *fc, uint32_t time)
+    {
+        ve_frame_inputs in{width_, height_, frameID, frameSeed(), time};
+        const int rc = ve_frame(ctx_, scene_, fc, &in);
+        frameID++;
+        return rc == VE_OK;
+    }
+
+    std::vector<float> output()
+    {
+        std::vector<float> out((size_t)width_ * height_ * 4);
+        check(ve_read(ctx_, VE_BUF_OUTPUT, out.data(), out.size() * sizeof(float)), ctx_);
+        return out;
+    }
+
+    const char *lastError() const { return ve_last_error(ctx_); }
+    ve_ctx *handle() { return ctx_; }
+    uint32_t frameID = 0;
+
+private:
+    uint32_t frameSeed() const { return frameID * 2654435761u + 1u; }
+    ve_ctx *ctx_ = nullptr;
+    ve_scene *scene_ = nullptr;
+    uint32_t width_ = 0, height_ = 0;
+};
+
+}  // namespace ve

@@ -155,10 +155,23 @@ __global__ __launch_bounds__(256) void k_normal(const uint32_t *__restrict__ wor
                                                 uint32_t *__restrict__ nbuf)
 {
     uint32_t x, y;
-    if (!tile_pixel(p, x, y)) return;
-    const size_t pix = (size_t)y * p.W + x;
-    const uint32_t vid = gbuf[pix].w;
-    if (vid == 0u) return;
+    const bool valid = tile_pixel(p, x, y);
+    const size_t pix = valid ? (size_t)y * p.W + x : 0;
+    const uint32_t vid = valid ? gbuf[pix].w : 0u;
+    // Pixels of one voxel carry the same id and the same packed position, and
+    // their claims, normal and time stamp are identical: one lane per distinct
+    // id in the wave does the work (the lowest, as the first claimant would).
+    const uint32_t lane = threadIdx.x & 63u;
+    bool lead = vid != 0u;
+    uint64_t pending = __ballot(lead);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lv = __shfl(vid, leader, 64);
+        const uint64_t same = __ballot(vid == lv);
+        pending &= ~same;
+        if (vid == lv && (int)lane != leader) lead = false;
+    }
+    if (!lead) return;
     const uint32_t len = 1u << p.depth;
     const uint32_t pk = gpos[pix], bits = p.depth - 1u, mask = (1u << bits) - 1u;
     const uint32_t hx = (pk & mask) * 2u, hy = ((pk >> bits) & mask) * 2u, hz = ((pk >> (2u * bits)) & mask) * 2u;
@@ -385,8 +398,14 @@ __global__ __launch_bounds__(256) void k_rank(FrameParams p, const uint4 *__rest
                                               int32_t *__restrict__ nsamp, uint32_t *__restrict__ vec_tot)
 {
     __shared__ uint32_t hist[HISTOGRAM_BINS];
+    __shared__ uint32_t bmin, bmax;
     for (uint32_t i = threadIdx.x; i < HISTOGRAM_BINS; i += blockDim.x) hist[i] = 0u;
+    if (threadIdx.x == 0) {
+        bmin = 0xFFFFFFFFu;
+        bmax = 0u;
+    }
     __syncthreads();
+    uint32_t my_min = 0xFFFFFFFFu, my_max = 0u;
     const uint32_t n = counters[VE_CNT_CANDIDATES];
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     int32_t s = 0;
@@ -405,10 +424,16 @@ __global__ __launch_bounds__(256) void k_rank(FrameParams p, const uint4 *__rest
         uint32_t bin = (uint32_t)(imp * 256.0f);
         if (bin >= HISTOGRAM_BINS) bin = HISTOGRAM_BINS - 1u;
         atomicAdd(&hist[bin], 1u);
-        const uint32_t ib = __float_as_uint(imp);
-        atomicMin(&rank_stats[4], ib);
-        atomicMax(&rank_stats[5], ib);
+        my_min = my_max = __float_as_uint(imp);   // min/max of the float bits as uints (rank.comp:141-143)
         nsamp[t] = s;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        my_min = min(my_min, (uint32_t)__shfl_xor((int)my_min, off, 64));
+        my_max = max(my_max, (uint32_t)__shfl_xor((int)my_max, off, 64));
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+        atomicMin(&bmin, my_min);
+        atomicMax(&bmax, my_max);
     }
     // vector total = sum of max(s, 0) over the 8 lanes of the vector
     uint32_t tot = s > 0 ? (uint32_t)s : 0u;
@@ -418,7 +443,11 @@ __global__ __launch_bounds__(256) void k_rank(FrameParams p, const uint4 *__rest
     if ((t & 7u) == 0u && t < n) vec_tot[t >> 3] = tot;
     __syncthreads();
     const uint32_t active = n > blockIdx.x * blockDim.x ? min(n - blockIdx.x * blockDim.x, (uint32_t)blockDim.x) : 0u;
-    if (threadIdx.x == 0 && active) atomicAdd(&rank_stats[6], active);
+    if (threadIdx.x == 0 && active) {
+        atomicAdd(&rank_stats[6], active);
+        atomicMin(&rank_stats[4], bmin);
+        atomicMax(&rank_stats[5], bmax);
+    }
     for (uint32_t i = threadIdx.x; i < HISTOGRAM_BINS; i += blockDim.x)
         if (hist[i]) atomicAdd(&rank_stats[8 + i], hist[i]);
 }

@@ -21,6 +21,16 @@ namespace ve {
 __device__ __forceinline__ float gmin(float a, float b) { return (a < b || __builtin_isnan(b)) ? a : b; }
 __device__ __forceinline__ float gmax(float a, float b) { return (a > b || __builtin_isnan(b)) ? a : b; }
 
+// Hardware IEEE min/max (v_min_f32 / v_max_f32): a NaN operand yields the
+// other operand like gmin/gmax; they differ only in which zero a +0/-0 tie
+// returns.  Used for the traversal's slab tests only, where the result t is
+// consumed as d*t + o (a signed zero gives the same sum unless o is also
+// zero, and then +0 and -0 both pass inBounds and convert to lattice 0) and
+// in comparisons (-0 < 0 is false): the lattice positions, hits and check
+// counts are the same.  Shading code keeps gmin/gmax.
+__device__ __forceinline__ float tmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ float tmax(float a, float b) { return __builtin_fmaxf(a, b); }
+
 struct f3 {
     float x, y, z;
 };
@@ -77,9 +87,47 @@ __device__ __forceinline__ f3 primary_dir(const float *cam, uint32_t x, uint32_t
     return normalize(d);
 }
 
-// RaycastCounted / Raycast (internal.glsl:297-400), step for step.
-// `checks` receives q; `reads` accumulates the node words the reference's
-// traversal fetches (its algorithmic traffic, 4 bytes each).
+// Traversal state of one ray between DDA steps.
+struct DDA {
+    f3 o, d, inv, r;
+    uint32_t q;
+    uint32_t px, py, pz;   // previous lattice position
+    uint32_t pstop;        // depth of the word that ended the previous descent
+};
+
+enum : int { DDA_CONTINUE = 0, DDA_MISS = 1, DDA_HIT = 2 };
+
+// Ray setup up to the first loop test (internal.glsl:352-366).
+__device__ __forceinline__ int dda_start(DDA &s, f3 o, f3 d, f3 inv, uint32_t depth, uint32_t &checks)
+{
+    const float fl = (float)(1u << depth);
+    s.d = d;
+    s.inv = inv;
+    s.q = 0;
+    s.px = s.py = s.pz = 0xFFFFFFFFu;
+    s.pstop = 0;
+    s.o = add(o, muls(d, 4.0f));   // internal.glsl:361
+    if (in_bounds(s.o, fl)) {
+        s.r = s.o;
+        return DDA_CONTINUE;
+    }
+    // intersect(ray, vec3(0), vec3(N)), internal.glsl:245-253
+    const f3 oo = s.o;
+    const float t1x = ((0.0f - oo.x) + 0.001f) * inv.x, t2x = ((fl - oo.x) - 0.001f) * inv.x;
+    const float t1y = ((0.0f - oo.y) + 0.001f) * inv.y, t2y = ((fl - oo.y) - 0.001f) * inv.y;
+    const float t1z = ((0.0f - oo.z) + 0.001f) * inv.z, t2z = ((fl - oo.z) - 0.001f) * inv.z;
+    const float te = tmax(tmax(tmin(t1x, t2x), tmin(t1y, t2y)), tmin(t1z, t2z));
+    const float tx = tmin(tmin(tmax(t1x, t2x), tmax(t1y, t2y)), tmax(t1z, t2z));
+    s.q = 1;
+    if (tx < te || tx < 0.0f) {
+        checks = s.q;
+        return DDA_MISS;
+    }
+    s.r = add(muls(d, te), oo);
+    return DDA_CONTINUE;
+}
+
+// One iteration of Raycast's while loop (internal.glsl:370-398).
 //
 // Every DDA step of the reference descends from the root.  The words read at
 // depth dd depend only on coordinate bits D..D-dd (locate() uses bit D-dd and
@@ -87,86 +135,80 @@ __device__ __forceinline__ f3 primary_dir(const float *cam, uint32_t x, uint32_t
 // lattice position differs from the previous one first in bit hb, depths
 // 0..D-hb-1 read exactly the words they read last step.  The descent resumes
 // at depth D-hb from the `next` pointer cached for depth D-hb-1 (in LDS, one
-// column per thread), or reuses the last outcome outright when it ended above
-// that depth.  Results are identical to the full descent; only the repeated
-// loads are skipped.  `cache` points at this thread's column: entry dd at
-// cache[dd * stride].
+// column per thread: entry dd at cache[dd * stride]), or reuses the last
+// outcome outright when it ended above that depth.  Results are identical to
+// the full descent; only the repeated loads are skipped.  `reads` still
+// counts the words the reference's full descent reads (its algorithmic
+// traffic, 4 bytes each).
+template <bool STATS>
+__device__ __forceinline__ int dda_step(DDA &s, const uint32_t *__restrict__ words, uint32_t n_words, uint32_t depth,
+                                        uint32_t controlchecks, uint32_t &checks, uint32_t &reads, Hit &hit,
+                                        uint32_t *cache, uint32_t stride)
+{
+    const uint32_t len = 1u << depth;
+    if (!(in_bounds(s.r, (float)len) && s.q++ <= controlchecks)) {
+        checks = s.q;
+        return DDA_MISS;
+    }
+    const uint32_t ux = (uint32_t)s.r.x, uy = (uint32_t)s.r.y, uz = (uint32_t)s.r.z;
+    const uint32_t diff = (ux ^ s.px) | (uy ^ s.py) | (uz ^ s.pz);
+    // first depth whose word may differ: D - (highest differing bit); the
+    // first step (no previous position) starts at the root
+    const uint32_t hb = 31u - (uint32_t)__clz(diff);
+    const uint32_t resume = diff == 0u ? 0xFFFFFFFFu : (hb >= depth ? 0u : depth - hb);
+    uint32_t dd;
+    if (resume > s.pstop) {
+        dd = s.pstop;   // same words down to the one that ended the last descent
+    } else {
+        uint32_t offset = resume == 0u ? 0u : cache[(resume - 1u) * stride];
+        for (dd = resume; dd < depth; dd++) {
+            offset += locate(ux, uy, uz, len >> dd);
+            const uint32_t raw = offset < n_words ? words[offset] : 0u;
+            if (!(raw & 1u)) break;
+            offset = raw >> 4;
+            cache[dd * stride] = offset;
+        }
+        if (dd == depth) {   // bottom level: the leaf slot
+            offset += locate(ux, uy, uz, 1u);
+            const uint32_t raw = offset < n_words ? words[offset] : 0u;
+            const uint32_t mat = (raw >> 1) & 0x7Fu;
+            if (mat != 0u) {
+                if (STATS) reads += depth + 1u;
+                checks = s.q;
+                hit = Hit{1u, offset, mat, ux, uy, uz, 1u};
+                return DDA_HIT;
+            }
+        }
+        s.pstop = dd;
+    }
+    if (STATS) reads += dd + 1u;   // the reference re-reads depths 0..dd every step
+    s.px = ux;
+    s.py = uy;
+    s.pz = uz;
+    const uint32_t tsize = len >> dd;
+    const uint32_t tpx = ux & ~(tsize - 1u), tpy = uy & ~(tsize - 1u), tpz = uz & ~(tsize - 1u);
+    // intersect_inside(ray, pos, pos + size), internal.glsl:255-261
+    const f3 o = s.o, inv = s.inv;
+    const float bx = (float)tpx, by = (float)tpy, bz = (float)tpz, fs = (float)tsize;
+    const float t1x = ((bx - o.x) - 0.001f) * inv.x, t2x = (((bx + fs) - o.x) + 0.001f) * inv.x;
+    const float t1y = ((by - o.y) - 0.001f) * inv.y, t2y = (((by + fs) - o.y) + 0.001f) * inv.y;
+    const float t1z = ((bz - o.z) - 0.001f) * inv.z, t2z = (((bz + fs) - o.z) + 0.001f) * inv.z;
+    const float tx = tmin(tmin(tmax(t1x, t2x), tmax(t1y, t2y)), tmax(t1z, t2z));
+    s.r = add(muls(s.d, tx), o);
+    return DDA_CONTINUE;
+}
+
+// RaycastCounted / Raycast (internal.glsl:297-400): `checks` receives q.
 template <bool STATS>
 __device__ __forceinline__ Hit raycast(const uint32_t *__restrict__ words, uint32_t n_words, uint32_t depth, f3 o,
                                        f3 d, f3 inv, uint32_t controlchecks, uint32_t &checks, uint32_t &reads,
                                        uint32_t *cache, uint32_t stride)
 {
-    Hit miss{0, 0, 0, 0, 0, 0, 0};
-    const uint32_t len = 1u << depth;
-    const float fl = (float)len;
-    uint32_t q = 0;
-    o = add(o, muls(d, 4.0f));   // internal.glsl:361
-    f3 r;
-    if (in_bounds(o, fl)) {
-        r = o;
-    } else {   // intersect(ray, vec3(0), vec3(N)), internal.glsl:245-253
-        float t1x = ((0.0f - o.x) + 0.001f) * inv.x, t2x = ((fl - o.x) - 0.001f) * inv.x;
-        float t1y = ((0.0f - o.y) + 0.001f) * inv.y, t2y = ((fl - o.y) - 0.001f) * inv.y;
-        float t1z = ((0.0f - o.z) + 0.001f) * inv.z, t2z = ((fl - o.z) - 0.001f) * inv.z;
-        float te = gmax(gmax(gmin(t1x, t2x), gmin(t1y, t2y)), gmin(t1z, t2z));
-        float tx = gmin(gmin(gmax(t1x, t2x), gmax(t1y, t2y)), gmax(t1z, t2z));
-        q++;
-        if (tx < te || tx < 0.0f) {
-            checks = q;
-            return miss;
-        }
-        r = add(muls(d, te), o);
-    }
-    // previous step's lattice position and where its descent ended
-    uint32_t px = 0xFFFFFFFFu, py = 0xFFFFFFFFu, pz = 0xFFFFFFFFu;
-    uint32_t pstop = 0;   // depth of the word that ended the last descent (depth == bottom)
-    while (in_bounds(r, fl) && q++ <= controlchecks) {
-        const uint32_t ux = (uint32_t)r.x, uy = (uint32_t)r.y, uz = (uint32_t)r.z;
-        const uint32_t diff = (ux ^ px) | (uy ^ py) | (uz ^ pz);
-        // first depth whose word may differ: D - (highest differing bit); the
-        // first step (no previous position) starts at the root
-        const uint32_t hb = 31u - (uint32_t)__clz(diff);
-        const uint32_t resume = diff == 0u ? 0xFFFFFFFFu : (hb >= depth ? 0u : depth - hb);
-        uint32_t dd, tsize;
-        if (resume > pstop) {
-            dd = pstop;   // same words down to the one that ended the last descent
-        } else {
-            uint32_t offset = resume == 0u ? 0u : cache[(resume - 1u) * stride];
-            for (dd = resume; dd < depth; dd++) {
-                offset += locate(ux, uy, uz, len >> dd);
-                const uint32_t raw = offset < n_words ? words[offset] : 0u;
-                if (!(raw & 1u)) break;
-                offset = raw >> 4;
-                cache[dd * stride] = offset;
-            }
-            if (dd == depth) {   // bottom level: the leaf slot
-                offset += locate(ux, uy, uz, 1u);
-                const uint32_t raw = offset < n_words ? words[offset] : 0u;
-                const uint32_t mat = (raw >> 1) & 0x7Fu;
-                if (mat != 0u) {
-                    if (STATS) reads += depth + 1u;
-                    checks = q;
-                    return Hit{1u, offset, mat, ux, uy, uz, 1u};
-                }
-            }
-            pstop = dd;
-        }
-        if (STATS) reads += dd + 1u;   // the reference re-reads depths 0..dd every step
-        px = ux;
-        py = uy;
-        pz = uz;
-        tsize = len >> dd;
-        const uint32_t tpx = ux & ~(tsize - 1u), tpy = uy & ~(tsize - 1u), tpz = uz & ~(tsize - 1u);
-        // intersect_inside(ray, pos, pos + size), internal.glsl:255-261
-        const float bx = (float)tpx, by = (float)tpy, bz = (float)tpz, fs = (float)tsize;
-        float t1x = ((bx - o.x) - 0.001f) * inv.x, t2x = (((bx + fs) - o.x) + 0.001f) * inv.x;
-        float t1y = ((by - o.y) - 0.001f) * inv.y, t2y = (((by + fs) - o.y) + 0.001f) * inv.y;
-        float t1z = ((bz - o.z) - 0.001f) * inv.z, t2z = (((bz + fs) - o.z) + 0.001f) * inv.z;
-        float tx = gmin(gmin(gmax(t1x, t2x), gmax(t1y, t2y)), gmax(t1z, t2z));
-        r = add(muls(d, tx), o);
-    }
-    checks = q;
-    return miss;
+    Hit hit{0, 0, 0, 0, 0, 0, 0};
+    DDA s;
+    int st = dda_start(s, o, d, inv, depth, checks);
+    while (st == DDA_CONTINUE) st = dda_step<STATS>(s, words, n_words, depth, controlchecks, checks, reads, hit, cache, stride);
+    return hit;
 }
 
 }  // namespace ve

@@ -202,6 +202,7 @@ typedef enum ve_buffer_id {
 #define VE_CNT_LIVE_B         8   /* live-ray count of the other ray buffer (internal) */
 #define VE_CNT_OVERFLOW       9   /* nonzero: a claim row overflowed the ordering table */
 #define VE_CNT_NEE_PENDING   10   /* nonzero: a hit needed next-event estimation (emissive scenes) */
+#define VE_CNT_TIMEOUTS      11   /* pixels whose primary ray timed out (last resolve) */
 
 int ve_read(ve_ctx *ctx, ve_buffer_id id, void *dst, size_t bytes);
 int ve_write(ve_ctx *ctx, ve_buffer_id id, const void *src, size_t bytes);

@@ -11,6 +11,15 @@ OUT_RTOL = 1e-4
 OUT_ATOL = 1e-6
 
 
+# Knobs under which the whole frame is order-independent (one sample per
+# visible voxel per frame, no stale reset, caps above the sample counts, no
+# firefly clamp), so every integer structure is bit-exact.
+DET_KNOBS = dict(primary_controlchecks=70, bounce_controlchecks=120, normal_precision=1, shading_budget=196608,
+                 weight_pixels=0.0, weight_variance=0.0, top_fraction=0.0, mid_fraction=1.0, min_samples=1,
+                 sample_cap_direct=1024, sample_cap_indirect=4096, disable_stale_reset=1, update_time=0xFFFFFFFF,
+                 firefly_k=1.0, firefly_floor=1e30)
+
+
 def load_frame_gold():
     with open(os.path.join(GOLD, "frame_small.json")) as fh:
         meta = json.load(fh)

@@ -9,7 +9,7 @@ relative."""
 import numpy as np
 import pytest
 
-from frame_common import assert_out_close, frame_seed, hash_map, load_frame_gold
+from frame_common import DET_KNOBS, assert_out_close, frame_seed, hash_map, load_frame_gold
 
 pytestmark = pytest.mark.gpu
 
@@ -116,3 +116,42 @@ def test_frames_vs_oracle_random(ve, scenes, oracle, small_scene):
         finally:
             rend.close()
             of.close()
+
+
+@pytest.mark.parametrize("radius", [0, 1, 10, 28, 29, 30])
+def test_timeout_reconstruction(ve, scenes, oracle, small_scene, radius):
+    """reconstructTimeout (resolve.comp:193-219) on a view where a low primary
+    check budget times many rays out, radii from 0 to the reference's UI
+    maximum of 30 (src/UI/control.cpp:47).  Deterministic knobs (one
+    sample per voxel per frame, no stale reset, caps above the sample count),
+    so the lBuffer the resolve reads is bit-exact."""
+    W, H = 160, 96
+    words, depth = small_scene["words"], small_scene["depth"]
+    e, d = scenes.look_at((-30.0, 70.0, -20.0), (90.0, 40.0, 100.0))
+    ubo = ve.camera_ubo(e, d, 75.0, W / H)
+    knobs = dict(DET_KNOBS, primary_controlchecks=14, reconstruction_radius=radius)
+    print("case timeout radius", radius, flush=True)
+    rend = ve.Renderer(W, H, max_bounces=2)
+    of = oracle.Frame(W, H, 16384, 2)
+    try:
+        rend.set_materials(scenes.palette(1))
+        of.set_materials(scenes.palette(1))
+        rend.set_sky_constant((140, 180, 230))
+        of.set_sky(np.array([140, 180, 230], np.float32) / np.float32(255))
+        rend.set_camera(ubo)
+        scene = rend.upload(words, depth, small_scene["info"].size)
+        fc_g, fc_o = ve.FrameConfig(**knobs), oracle.FrameConfig(**knobs)
+        for f in range(2):
+            rend.frame(scene, fc_g, ve.FrameInputs(W, H, f, frame_seed(f), f + 1))
+            of.frame(words, depth, ubo, fc_o, f, frame_seed(f), f + 1)
+        g, _ = rend.gbuffer(W, H)
+        g = np.asarray(g).reshape(-1, 4)
+        n_timeout = int(np.count_nonzero((g[:, 3] == 0) & (g[:, 0] >= 14)))
+        assert n_timeout > 100
+        assert int(rend.counters()[ve.CNT["TIMEOUTS"]]) == n_timeout
+        assert int(rend.counters()[ve.CNT["OVERFLOW"]]) == 0
+        assert_out_close(rend.read("OUTPUT", np.float32, W * H * 16), of.read("OUTPUT", np.float32))
+        rend.release(scene)
+    finally:
+        rend.close()
+        of.close()

@@ -310,7 +310,7 @@ int ve_create(ve_ctx **out, const ve_renderer_config *cfg)
     DeviceBuffers &b = c->b;
     b.scan_temp_bytes = scan_temp_bytes((uint32_t)(groups_max * 8));
     if ((rc = dalloc(c, &b.gbuf, P)) || (rc = dalloc(c, &b.gpos, P)) || (rc = dalloc(c, &b.output, P)) ||
-        (rc = dalloc(c, &b.pix_avg, P)) || (rc = dalloc(c, &b.pix_cnt, P)) ||
+        (rc = dalloc(c, &b.pix_avg, P)) || (rc = dalloc(c, &b.pix_cnt, P)) || (rc = dalloc(c, &b.timeouts, P)) ||
         (rc = dalloc(c, &b.nbuf, (size_t)c->hash_width * NBUFFER_STRIDE * NBUFFER_SLOTS)) ||
         (rc = dalloc(c, &b.lbuf, (size_t)c->hash_width * LBUFFER_STRIDE * LBUFFER_SLOTS)) ||
         (rc = dalloc(c, &b.rbuf, (size_t)c->hash_width * RBUFFER_STRIDE * RBUFFER_SLOTS)) ||

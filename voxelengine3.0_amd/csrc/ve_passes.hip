@@ -856,63 +856,72 @@ __device__ Avg voxel_avg(const uint32_t *__restrict__ lbuf, uint32_t hw, uint32_
 // lookupVoxelAvg is a pure function of the id, so the bilateral taps and the
 // timeout gather of part 2 read these per-pixel values instead of probing the
 // lBuffer hash again for each tap (up to 440 taps for a timed-out pixel).
+// Lanes of a tile that see the same voxel share one lookup.
 __global__ __launch_bounds__(256) void k_resolve_avg(FrameParams p, const uint4 *__restrict__ gbuf,
                                                      const uint32_t *__restrict__ lbuf, float4 *__restrict__ pix_avg,
                                                      uint32_t *__restrict__ pix_cnt)
 {
-    uint32_t x, y;
-    if (!tile_pixel(p, x, y)) return;
-    const size_t pix = (size_t)y * p.W + x;
-    const Avg a = voxel_avg(lbuf, p.hash_width, gbuf[pix].w);
-    pix_avg[pix] = make_float4(a.c.x, a.c.y, a.c.z, a.var);
-    pix_cnt[pix] = a.count;
-}
-
-__device__ f3 reconstruct_timeout(const FrameParams &p, const uint4 *__restrict__ gbuf,
-                                  const float4 *__restrict__ pix_avg, const uint32_t *__restrict__ pix_cnt, int px,
-                                  int py)   // resolve.comp:193-219
-{
-    f3 sum{0, 0, 0};
-    float wsum = 0.0f;
-    const int R = p.reconstruction_radius;
-    for (int dy = -R; dy <= R; dy++)
-        for (int dx = -R; dx <= R; dx++) {
-            if (dx == 0 && dy == 0) continue;
-            const int qx = px + dx, qy = py + dy;
-            if (qx < 0 || qy < 0 || qx >= (int)p.W || qy >= (int)p.H) continue;
-            const size_t q = (size_t)qy * p.W + qx;
-            const uint4 nd = gbuf[q];
-            if (nd.w == 0u || nd.x >= p.primary_cc) continue;
-            const uint32_t nc = pix_cnt[q];
-            if (nc == 0u) continue;
-            const float4 na = pix_avg[q];
-            const float dist = (float)(dx * dx + dy * dy);
-            const float w = (float)min(nc, 32u) / (1.0f + dist * 0.15f);
-            sum = add(sum, muls(f3{na.x, na.y, na.z}, w));
-            wsum += w;
-        }
-    return wsum > 0.0001f ? divs(sum, wsum) : f3{p.sky[0], p.sky[1], p.sky[2]};
+    uint32_t x = 0, y = 0;
+    const bool valid = tile_pixel(p, x, y);
+    const size_t pix = valid ? (size_t)y * p.W + x : 0;
+    const uint32_t vid = valid ? gbuf[pix].w : 0u;
+    // leader = lowest lane holding the same id; leaders look up in parallel
+    const uint32_t lane = threadIdx.x & 63u;
+    int leader = (int)lane;
+    uint64_t pending = __ballot(vid != 0u);
+    while (pending) {
+        const int l = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lv = __shfl(vid, l, 64);
+        const uint64_t same = __ballot(vid == lv);
+        pending &= ~same;
+        if (vid == lv) leader = l;
+    }
+    Avg a{{0, 0, 0}, 0u, 0.0f};
+    if (vid != 0u && leader == (int)lane) a = voxel_avg(lbuf, p.hash_width, vid);
+    Avg mine;
+    mine.c.x = __shfl(a.c.x, leader, 64);
+    mine.c.y = __shfl(a.c.y, leader, 64);
+    mine.c.z = __shfl(a.c.z, leader, 64);
+    mine.count = __shfl(a.count, leader, 64);
+    mine.var = __shfl(a.var, leader, 64);
+    if (valid) {
+        pix_avg[pix] = make_float4(mine.c.x, mine.c.y, mine.c.z, mine.var);
+        pix_cnt[pix] = mine.count;
+    }
 }
 
 // AvgPass part 2 (resolve.comp:362-382): per-pixel colour of the selected view.
+// Timed-out pixels (reconstructTimeout) are appended to `timeouts` and
+// coloured by k_resolve_timeout.
 __global__ __launch_bounds__(256) void k_resolve(FrameParams p, const uint4 *__restrict__ gbuf,
                                                  const uint32_t *__restrict__ lbuf, const uint32_t *__restrict__ nbuf,
                                                  const Material *__restrict__ mats, const float4 *__restrict__ pix_avg,
-                                                 const uint32_t *__restrict__ pix_cnt, float4 *__restrict__ out)
+                                                 const uint32_t *__restrict__ pix_cnt, float4 *__restrict__ out,
+                                                 uint32_t *__restrict__ timeouts, uint32_t *__restrict__ counters)
 {
-    uint32_t x, y;
-    if (!tile_pixel(p, x, y)) return;
-    const size_t pix = (size_t)y * p.W + x;
-    const uint4 g = gbuf[pix];
+    uint32_t x = 0, y = 0;
+    const bool valid = tile_pixel(p, x, y);
+    const size_t pix = valid ? (size_t)y * p.W + x : 0;
+    const uint4 g = valid ? gbuf[pix] : make_uint4(0u, 0u, 0u, 0u);
     const uint32_t checks = g.x, mi = g.y, vid = g.w;
-    const bool timeout = vid == 0u && checks >= p.primary_cc;
+    const bool timeout = valid && p.render_type != 1 && vid == 0u && checks >= p.primary_cc;
+    const uint64_t tmask = __ballot(timeout);
+    if (tmask != 0ull) {
+        const uint32_t lane = threadIdx.x & 63u;
+        const int first = __ffsll((unsigned long long)tmask) - 1;
+        uint32_t base = 0;
+        if ((int)lane == first) base = atomicAdd(&counters[VE_CNT_TIMEOUTS], (uint32_t)__popcll(tmask));
+        base = __shfl(base, first, 64);
+        if (timeout) timeouts[base + (uint32_t)__popcll(tmask & ((1ull << lane) - 1ull))] = (uint32_t)pix;
+    }
+    if (!valid || timeout) return;
     const f3 sky{p.sky[0], p.sky[1], p.sky[2]};
     f3 col;
     if (p.render_type == 1) {
         const float v = (float)checks / 65.0f;
         col = f3{v, v, v};
     } else if (vid == 0u) {
-        col = timeout ? reconstruct_timeout(p, gbuf, pix_avg, pix_cnt, (int)x, (int)y) : sky;
+        col = sky;
     } else if (p.render_type == 2) {
         col = (mi == 0u || mi >= 127u) ? f3{0, 0, 0} : f3{mats[mi].color[0], mats[mi].color[1], mats[mi].color[2]};
     } else if (p.render_type == 3) {
@@ -964,6 +973,62 @@ __global__ __launch_bounds__(256) void k_resolve(FrameParams p, const uint4 *__r
         }
     }
     out[pix] = make_float4(col.x, col.y, col.z, 1.0f);
+}
+
+__device__ __forceinline__ float wave_fsum(float v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// reconstructTimeout (resolve.comp:193-219) for the pixels k_resolve listed:
+// one wave per timed-out pixel, the (2R+1)^2 - 1 taps spread over the lanes
+// (tap t on lane t % 64) and the weighted sums reduced across the wave.  Each
+// tap's weight and weighted colour are computed exactly as the reference does;
+// only the order of the final float additions differs from its sequential
+// loop (all terms are non-negative, so the sums agree to a few ulp).  Waves
+// stride over the list and exit once it is exhausted.
+__global__ __launch_bounds__(256) void k_resolve_timeout(FrameParams p, const uint4 *__restrict__ gbuf,
+                                                         const float4 *__restrict__ pix_avg,
+                                                         const uint32_t *__restrict__ pix_cnt,
+                                                         const uint32_t *__restrict__ timeouts,
+                                                         const uint32_t *__restrict__ counters,
+                                                         float4 *__restrict__ out)
+{
+    const int R = p.reconstruction_radius < 0 ? 0 : p.reconstruction_radius;
+    const int D = 2 * R + 1, NT = D * D;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t n = counters[VE_CNT_TIMEOUTS];
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += waves) {
+        const uint32_t pix = timeouts[i];
+        const int px = (int)(pix % p.W), py = (int)(pix / p.W);
+        float sr = 0.0f, sg = 0.0f, sb = 0.0f, ws = 0.0f;
+        for (int t = (int)lane; t < NT; t += 64) {
+            const int dx = t % D - R, dy = t / D - R;
+            const int qx = px + dx, qy = py + dy;
+            if ((dx == 0 && dy == 0) || qx < 0 || qy < 0 || qx >= (int)p.W || qy >= (int)p.H) continue;
+            const size_t q = (size_t)qy * p.W + qx;
+            const uint4 nd = gbuf[q];
+            const uint32_t nc = pix_cnt[q];
+            const float4 na = pix_avg[q];
+            if (nd.w == 0u || nd.x >= p.primary_cc || nc == 0u) continue;
+            const float dist = (float)(dx * dx + dy * dy);
+            const float w = (float)min(nc, 32u) / (1.0f + dist * 0.15f);
+            sr += na.x * w;
+            sg += na.y * w;
+            sb += na.z * w;
+            ws += w;
+        }
+        sr = wave_fsum(sr);
+        sg = wave_fsum(sg);
+        sb = wave_fsum(sb);
+        ws = wave_fsum(ws);
+        if (lane == 0u) {
+            const f3 col = ws > 0.0001f ? f3{sr / ws, sg / ws, sb / ws} : f3{p.sky[0], p.sky[1], p.sky[2]};
+            out[pix] = make_float4(col.x, col.y, col.z, 1.0f);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1100,8 +1165,11 @@ void launch_shade(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_
 void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 {
     hipLaunchKernelGGL(k_resolve_avg, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.pix_avg, b.pix_cnt);
+    launch_fill(s, b.counters + VE_CNT_TIMEOUTS, 0u, 1);
     hipLaunchKernelGGL(k_resolve, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.nbuf, b.materials,
-                       b.pix_avg, b.pix_cnt, b.output);
+                       b.pix_avg, b.pix_cnt, b.output, b.timeouts, b.counters);
+    hipLaunchKernelGGL(k_resolve_timeout, dim3(RESOLVE_TIMEOUT_BLOCKS), dim3(256), 0, s, p, b.gbuf, b.pix_avg,
+                       b.pix_cnt, b.timeouts, b.counters, b.output);
 }
 
 size_t scan_temp_bytes(uint32_t max_items)

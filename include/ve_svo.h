@@ -234,6 +234,83 @@ int ve_enable_node_stats(ve_ctx *ctx, int enable);
 int ve_node_stats(ve_ctx *ctx, uint64_t *primary_reads, uint64_t *trace_reads,
                   uint64_t *primary_steps, uint64_t *trace_steps);
 
+/* ---- serving: resident scenes, sessions, continuous batching ----------------
+ * No single counterpart in the reference, whose one Renderer renders one view
+ * on one GL context (src/voxelengine.cpp:158-290).  A server keeps octrees
+ * resident, gives every session its own accumulation state (one ve_ctx:
+ * l/n/rBuffer, claim maps, ray buffers), admits sessions while a device-memory
+ * budget and a slot count allow (later ones wait in arrival order and are
+ * admitted as running sessions close), and launches each submitted frame at
+ * once on the session's queue, so frames of different sessions run
+ * concurrently on the device.  A frame renders exactly as ve_frame on that
+ * session alone would: sessions share only the read-only octrees.
+ *
+ * `queues` chooses the concurrency: 0 gives every session its own HIP stream;
+ * 1 puts every session on one stream, which is the plain one-view-at-a-time
+ * loop (frames in submission order, no overlap) used as the baseline.
+ * Single-threaded like ve_ctx. */
+typedef struct ve_server ve_server;
+
+typedef struct ve_server_config {
+    int32_t  device;
+    uint32_t max_sessions;   /* admitted at once; 0 = 64 */
+    uint64_t hbm_budget;     /* bytes of session state; 0 = 3/4 of the free device memory at creation */
+    uint32_t queues;         /* 0 = one stream per session; N >= 1 = sessions share N streams */
+} ve_server_config;
+
+typedef struct ve_session_desc {
+    uint32_t scene;          /* id from ve_server_add_scene */
+    uint32_t width, height;
+    uint32_t max_bounces;    /* MAX_BOUNCES, as in ve_renderer_config */
+    uint32_t hash_width;     /* 0 = 16384 */
+    const void *materials;   /* std140 64-byte entries, copied at open */
+    uint32_t n_materials;
+    uint8_t  sky_rgb8[3];    /* constant sky */
+} ve_session_desc;
+
+typedef struct ve_frame_done {
+    uint32_t session;
+    uint32_t frame;          /* 0-based count of this session's frames */
+    uint64_t tag;            /* the caller's tag from ve_server_submit */
+    double   arrival_ms;     /* server clock: when the request arrived */
+    double   start_ms;       /* device clock mapped to the server clock: first kernel began */
+    double   done_ms;        /* ... last kernel ended */
+    uint64_t rays;           /* primary + traced + shadow rays of the frame */
+} ve_frame_done;
+
+typedef struct ve_server_stats {
+    uint32_t admitted;       /* sessions holding device state */
+    uint32_t waiting;        /* sessions queued for admission */
+    uint32_t frames_in_flight;
+    uint32_t pooled;         /* idle contexts kept for reuse */
+    uint64_t bytes_in_use;   /* admitted sessions + pooled contexts */
+    uint64_t budget;
+} ve_server_stats;
+
+int  ve_server_create(ve_server **out, const ve_server_config *cfg);
+void ve_server_destroy(ve_server *srv);
+const char *ve_server_last_error(const ve_server *srv);
+/* Milliseconds since the server was created (the clock of ve_frame_done). */
+double ve_server_now_ms(const ve_server *srv);
+/* Upload a read-only octree (as ve_scene_upload) shared by every session. */
+int  ve_server_add_scene(ve_server *srv, const uint32_t *words, uint32_t n_words, uint32_t depth, uint32_t *scene_id);
+/* Open a session; it is admitted now or queued (VE_OK either way).  Fails with
+ * VE_EBUDGET only when the session alone exceeds the whole budget. */
+int  ve_server_open(ve_server *srv, const ve_session_desc *desc, uint32_t *session_id);
+/* Request one frame (camera UBO as ve_camera_ubo, FrameConfig, inputs whose
+ * width/height equal the session's).  Launched at once when the session is
+ * admitted, else when it is.  arrival_ms < 0 means now. */
+int  ve_server_submit(ve_server *srv, uint32_t session, const ve_frame_config *fc, const ve_frame_inputs *in,
+                      const float camera_ubo[16], uint64_t tag, double arrival_ms);
+/* Close after the session's submitted frames; its state returns to the pool. */
+int  ve_server_close(ve_server *srv, uint32_t session);
+/* Collect finished frames (at most max).  wait != 0 blocks until at least one
+ * finishes when any is in flight.  Also admits waiting sessions. */
+int  ve_server_poll(ve_server *srv, ve_frame_done *out, uint32_t max, uint32_t *n_out, int wait);
+/* RGBA32F output of the session's last finished frame (waits for its frames). */
+int  ve_server_read_output(ve_server *srv, uint32_t session, float *rgba, size_t bytes);
+int  ve_server_get_stats(ve_server *srv, ve_server_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,6 +91,27 @@ class FrameInputs(C.Structure):
                 ("frame_seed", C.c_uint32), ("time", C.c_uint32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("max_sessions", C.c_uint32), ("hbm_budget", C.c_uint64),
+                ("queues", C.c_uint32)]
+
+
+class SessionDesc(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("max_bounces", C.c_uint32),
+                ("hash_width", C.c_uint32), ("materials", C.c_void_p), ("n_materials", C.c_uint32),
+                ("sky_rgb8", C.c_uint8 * 3)]
+
+
+class FrameDone(C.Structure):
+    _fields_ = [("session", C.c_uint32), ("frame", C.c_uint32), ("tag", C.c_uint64), ("arrival_ms", C.c_double),
+                ("start_ms", C.c_double), ("done_ms", C.c_double), ("rays", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("admitted", C.c_uint32), ("waiting", C.c_uint32), ("frames_in_flight", C.c_uint32),
+                ("pooled", C.c_uint32), ("bytes_in_use", C.c_uint64), ("budget", C.c_uint64)]
+
+
 _lib = None
 
 
@@ -128,6 +149,18 @@ def lib():
         "ve_node_stats": (i32, [p] + [C.POINTER(C.c_uint64)] * 4),
         "ve_ray_totals": (i32, [p] + [C.POINTER(C.c_uint64)] * 3),
         "ve_reset_ray_totals": (i32, [p]),
+        "ve_server_create": (i32, [C.POINTER(p), C.POINTER(ServerConfig)]),
+        "ve_server_destroy": (None, [p]),
+        "ve_server_last_error": (C.c_char_p, [p]),
+        "ve_server_now_ms": (C.c_double, [p]),
+        "ve_server_add_scene": (i32, [p, p, u32, u32, C.POINTER(u32)]),
+        "ve_server_open": (i32, [p, C.POINTER(SessionDesc), C.POINTER(u32)]),
+        "ve_server_submit": (i32, [p, u32, C.POINTER(FrameConfig), C.POINTER(FrameInputs), p, C.c_uint64,
+                                   C.c_double]),
+        "ve_server_close": (i32, [p, u32]),
+        "ve_server_poll": (i32, [p, p, u32, C.POINTER(u32), i32]),
+        "ve_server_read_output": (i32, [p, u32, p, sz]),
+        "ve_server_get_stats": (i32, [p, C.POINTER(ServerStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -349,3 +382,76 @@ class Renderer:
         out = np.zeros(len(PASS), np.float32)
         self._ck(lib().ve_timings(self.h, out.ctypes.data))
         return dict(zip(PASS_NAMES, out.tolist()))
+
+
+class Server:
+    """Session server (ve_server_*): resident octrees, per-session accumulation
+    state admitted by a device-memory budget, frames of different sessions
+    overlapping on the device.  queues=0: one stream per session; queues=1:
+    the plain loop (every frame on one stream, in submission order)."""
+
+    def __init__(self, device=0, max_sessions=64, hbm_budget=0, queues=0):
+        cfg = ServerConfig(device, max_sessions, hbm_budget, queues)
+        h = C.c_void_p()
+        _check(lib().ve_server_create(C.byref(h), C.byref(cfg)))
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if self.h:
+            lib().ve_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        if rc != VE_OK:
+            msg = lib().ve_server_last_error(self.h)
+            raise VeError(rc, msg.decode() if msg else "")
+
+    def now_ms(self):
+        return lib().ve_server_now_ms(self.h)
+
+    def add_scene(self, words, depth, n_words=None):
+        words = np.ascontiguousarray(words, np.uint32)
+        n = len(words) if n_words is None else int(n_words)
+        sid = C.c_uint32()
+        self._ck(lib().ve_server_add_scene(self.h, words.ctypes.data, n, depth, C.byref(sid)))
+        return sid.value
+
+    def open(self, scene, width, height, max_bounces, materials, sky_rgb8, hash_width=16384):
+        mats = np.ascontiguousarray(materials)
+        d = SessionDesc(scene, width, height, max_bounces, hash_width, mats.ctypes.data, len(mats),
+                        (C.c_uint8 * 3)(*[int(v) for v in sky_rgb8]))
+        sid = C.c_uint32()
+        self._ck(lib().ve_server_open(self.h, C.byref(d), C.byref(sid)))
+        return sid.value
+
+    def submit(self, session, fc, inputs, ubo, tag=0, arrival_ms=-1.0):
+        ubo = np.ascontiguousarray(ubo, np.float32)
+        self._ck(lib().ve_server_submit(self.h, session, C.byref(fc), C.byref(inputs), ubo.ctypes.data, tag,
+                                        float(arrival_ms)))
+
+    def close_session(self, session):
+        self._ck(lib().ve_server_close(self.h, session))
+
+    def poll(self, wait=False, max_n=256):
+        out = (FrameDone * max_n)()
+        n = C.c_uint32()
+        self._ck(lib().ve_server_poll(self.h, out, max_n, C.byref(n), 1 if wait else 0))
+        return [dict(session=d.session, frame=d.frame, tag=d.tag, arrival_ms=d.arrival_ms, start_ms=d.start_ms,
+                     done_ms=d.done_ms, rays=d.rays) for d in out[:n.value]]
+
+    def read_output(self, session, width, height):
+        out = np.empty(width * height * 4, np.float32)
+        self._ck(lib().ve_server_read_output(self.h, session, out.ctypes.data, out.nbytes))
+        return out
+
+    def stats(self):
+        st = ServerStats()
+        self._ck(lib().ve_server_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}

@@ -16,6 +16,7 @@
 #include "../../include/ve_svo.h"
 #include "ve_internal.h"
 #include "ve_launch.h"
+#include "ve_ctx.h"
 
 using namespace ve;
 
@@ -29,34 +30,6 @@ int ve_global_error(int code, const char *msg)
     g_error = msg;
     return code;
 }
-
-struct ve_scene {
-    uint32_t *words = nullptr;
-    uint32_t n_words = 0;
-    uint32_t depth = 0;
-};
-
-constexpr int MAX_ITERS = 8;   // host bounce-loop cap (renderer.cpp:515)
-
-struct ve_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    uint32_t max_w = 0, max_h = 0;
-    uint32_t hash_width = 16384;
-    uint32_t max_bounces = 8;
-    std::string err;
-    float cam[16] = {0};
-    float sky[3] = {0, 0, 0};
-    bool node_stats = false;
-    bool timing = false;
-    uint32_t bounce_iter = 0;           // next trace/shade iteration of per-pass dispatch
-    // timing: one event pair per pass and bounce iteration per recorded frame
-    std::vector<std::array<hipEvent_t, 2 * VE_PASS_COUNT * MAX_ITERS>> frame_events;
-    size_t frames_recorded = 0;
-    uint32_t iters_recorded = 0;
-    DeviceBuffers b;
-    std::vector<void *> allocations;
-};
 
 namespace {
 
@@ -85,10 +58,61 @@ int dalloc(ve_ctx *c, T **p, size_t count)
     if (e != hipSuccess) return ctx_error(c, VE_ENOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
     e = hipMemset(ptr, 0, count * sizeof(T));
     if (e != hipSuccess) return ctx_error(c, VE_EDEVICE, "hipMemset: %s", hipGetErrorString(e));
-    c->allocations.push_back(ptr);
+    c->allocations.push_back({ptr, count * sizeof(T)});
     *p = static_cast<T *>(ptr);
     return VE_OK;
 }
+
+// Every device buffer of a context, in one list: allocates them (c set) or
+// only adds up their bytes (c null), so admission control can price a
+// context before creating it.
+struct Alloc {
+    ve_ctx *c = nullptr;
+    size_t bytes = 0;
+    int rc = VE_OK;
+    template <typename T>
+    void operator()(T **p, size_t count)
+    {
+        bytes += (count * sizeof(T) + 255) & ~(size_t)255;
+        if (c && rc == VE_OK) rc = dalloc(c, p, count);
+    }
+};
+
+void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, uint32_t hash_width)
+{
+    const size_t P = (size_t)max_w * max_h;
+    const size_t groups_max = (size_t)((max_w + 7) / 8) * ((max_h + 7) / 8);
+    a(&b.gbuf, P); a(&b.gpos, P); a(&b.output, P); a(&b.pix_avg, P); a(&b.pix_cnt, P); a(&b.timeouts, P);
+    a(&b.nbuf, (size_t)hash_width * NBUFFER_STRIDE * NBUFFER_SLOTS);
+    a(&b.lbuf, (size_t)hash_width * LBUFFER_STRIDE * LBUFFER_SLOTS);
+    a(&b.rbuf, (size_t)hash_width * RBUFFER_STRIDE * RBUFFER_SLOTS);
+    a(&b.claim, (size_t)CLAIM_WIDTH * CLAIM_PROBES);
+    a(&b.samples, (size_t)CLAIM_WIDTH * CLAIM_PROBES);
+    a(&b.candidates, CANDIDATE_CAPACITY);
+    a(&b.shade_list, SHADE_BUDGET_MAX);
+    a(&b.rays, SHADE_BUDGET_MAX);
+    a(&b.rays_next, SHADE_BUDGET_MAX);
+    a(&b.counters, NUM_COUNTERS);
+    a(&b.rank_stats, RANK_STATS_WORDS);
+    a(&b.emissives, EMISSIVE_CAPACITY);
+    a(&b.emissive_claim, (size_t)EMISSIVE_CLAIM_W * EMISSIVE_CLAIM_PROBES);
+    a(&b.materials, MATERIAL_SLOTS);
+    a(&b.node_stats, 4);
+    a(&b.ray_totals, 3);
+    a(&b.sched_vid, (size_t)CLAIM_WIDTH * SCHED_SLOTS);
+    a(&b.sched_first, (size_t)CLAIM_WIDTH * SCHED_SLOTS);
+    a(&b.sched_cnt, (size_t)CLAIM_WIDTH * SCHED_SLOTS);
+    a(&b.cand_at, groups_max * 64);
+    a(&b.vec_cnt, groups_max * 8);
+    a(&b.vec_base, groups_max * 8);
+    a(&b.nsamp, std::min<size_t>(P, CANDIDATE_CAPACITY));
+    b.scan_temp_bytes = scan_temp_bytes((uint32_t)(groups_max * 8));
+    a(reinterpret_cast<uint8_t **>(&b.scan_temp), b.scan_temp_bytes);
+}
+
+// RankStats thresholds start at (+inf, -inf): every voxel in the mid tier on
+// the first frame (rank.comp:109-111, threshold.comp:183-189).
+const uint32_t INITIAL_THRESHOLDS[2] = {0x7F800000u, 0xFF800000u};
 
 struct BufView {
     void *ptr;
@@ -288,59 +312,9 @@ void ve_camera_ubo(const float pos[3], const float dir[3], float fov_deg, float 
 int ve_create(ve_ctx **out, const ve_renderer_config *cfg)
 {
     ve_clear_global_error();
-    if (!out || !cfg) return ve_global_error(VE_EINVAL, "null argument");
-    if (cfg->max_width == 0 || cfg->max_height == 0) return ve_global_error(VE_EINVAL, "zero-sized framebuffer");
-    if (cfg->max_bounces > 64) return ve_global_error(VE_EINVAL, "max_bounces above 64");
-    ve_ctx *c = new (std::nothrow) ve_ctx();
-    if (!c) return ve_global_error(VE_ENOMEM, "context allocation");
-    c->device = cfg->device;
-    c->max_w = cfg->max_width;
-    c->max_h = cfg->max_height;
-    c->hash_width = cfg->hash_width ? cfg->hash_width : 16384u;
-    c->max_bounces = cfg->max_bounces ? cfg->max_bounces : 8u;
-    if (c->hash_width < 2) { delete c; return ve_global_error(VE_EINVAL, "hash_width below 2"); }
-    int rc = VE_OK;
-    hipError_t e = hipSetDevice(c->device);
-    if (e != hipSuccess) { ve_global_error(VE_EDEVICE, hipGetErrorString(e)); delete c; return VE_EDEVICE; }
-    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);   // default priority
-    if (e != hipSuccess) { ve_global_error(VE_EDEVICE, hipGetErrorString(e)); delete c; return VE_EDEVICE; }
-    const size_t P = (size_t)c->max_w * c->max_h;
-    const size_t groups_max = (size_t)((c->max_w + 7) / 8) * ((c->max_h + 7) / 8);
-    if (groups_max * 64 >= (1ull << 31)) { delete c; return ve_global_error(VE_EINVAL, "framebuffer too large"); }
-    DeviceBuffers &b = c->b;
-    b.scan_temp_bytes = scan_temp_bytes((uint32_t)(groups_max * 8));
-    if ((rc = dalloc(c, &b.gbuf, P)) || (rc = dalloc(c, &b.gpos, P)) || (rc = dalloc(c, &b.output, P)) ||
-        (rc = dalloc(c, &b.pix_avg, P)) || (rc = dalloc(c, &b.pix_cnt, P)) || (rc = dalloc(c, &b.timeouts, P)) ||
-        (rc = dalloc(c, &b.nbuf, (size_t)c->hash_width * NBUFFER_STRIDE * NBUFFER_SLOTS)) ||
-        (rc = dalloc(c, &b.lbuf, (size_t)c->hash_width * LBUFFER_STRIDE * LBUFFER_SLOTS)) ||
-        (rc = dalloc(c, &b.rbuf, (size_t)c->hash_width * RBUFFER_STRIDE * RBUFFER_SLOTS)) ||
-        (rc = dalloc(c, &b.claim, (size_t)CLAIM_WIDTH * CLAIM_PROBES)) ||
-        (rc = dalloc(c, &b.samples, (size_t)CLAIM_WIDTH * CLAIM_PROBES)) ||
-        (rc = dalloc(c, &b.candidates, CANDIDATE_CAPACITY)) || (rc = dalloc(c, &b.shade_list, SHADE_BUDGET_MAX)) ||
-        (rc = dalloc(c, &b.rays, SHADE_BUDGET_MAX)) || (rc = dalloc(c, &b.rays_next, SHADE_BUDGET_MAX)) ||
-        (rc = dalloc(c, &b.counters, NUM_COUNTERS)) || (rc = dalloc(c, &b.rank_stats, RANK_STATS_WORDS)) ||
-        (rc = dalloc(c, &b.emissives, EMISSIVE_CAPACITY)) ||
-        (rc = dalloc(c, &b.emissive_claim, (size_t)EMISSIVE_CLAIM_W * EMISSIVE_CLAIM_PROBES)) ||
-        (rc = dalloc(c, &b.materials, MATERIAL_SLOTS)) || (rc = dalloc(c, &b.node_stats, 4)) ||
-        (rc = dalloc(c, &b.ray_totals, 3)) ||
-        (rc = dalloc(c, &b.sched_vid, (size_t)CLAIM_WIDTH * SCHED_SLOTS)) ||
-        (rc = dalloc(c, &b.sched_first, (size_t)CLAIM_WIDTH * SCHED_SLOTS)) ||
-        (rc = dalloc(c, &b.sched_cnt, (size_t)CLAIM_WIDTH * SCHED_SLOTS)) ||
-        (rc = dalloc(c, &b.cand_at, groups_max * 64)) || (rc = dalloc(c, &b.vec_cnt, groups_max * 8)) ||
-        (rc = dalloc(c, &b.vec_base, groups_max * 8)) ||
-        (rc = dalloc(c, &b.nsamp, std::min<size_t>(P, CANDIDATE_CAPACITY))) ||
-        (rc = dalloc(c, reinterpret_cast<uint8_t **>(&b.scan_temp), scan_temp_bytes((uint32_t)(groups_max * 8))))) {
-        std::string msg = c->err;
-        ve_destroy(c);
-        return ve_global_error(rc, msg.c_str());
-    }
-    // RankStats thresholds start at (+inf, -inf): every voxel in the mid tier
-    // on the first frame (rank.comp:109-111, threshold.comp:183-189).
-    uint32_t thr[2] = {0x7F800000u, 0xFF800000u};
-    e = hipMemcpy(b.rank_stats, thr, sizeof thr, hipMemcpyHostToDevice);
-    if (e != hipSuccess) { ve_global_error(VE_EDEVICE, hipGetErrorString(e)); ve_destroy(c); return VE_EDEVICE; }
-    *out = c;
-    return VE_OK;
+    std::string err;
+    const int rc = ve::ctx_create(out, cfg, nullptr, err);
+    return rc == VE_OK ? VE_OK : ve_global_error(rc, err.c_str());
 }
 
 void ve_destroy(ve_ctx *c)
@@ -348,40 +322,20 @@ void ve_destroy(ve_ctx *c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (void *p : c->allocations) (void)hipFree(p);
+    for (auto &a : c->allocations) (void)hipFree(a.first);
     for (auto &fe : c->frame_events)
         for (hipEvent_t e : fe)
             if (e) (void)hipEventDestroy(e);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream && c->owns_stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
 int ve_scene_upload(ve_ctx *c, const uint32_t *words, uint32_t n_words, uint32_t depth, ve_scene **scene)
 {
     if (!c) return ve_global_error(VE_EINVAL, "null context");
-    if (!words || !scene || n_words == 0) return ctx_error(c, VE_EINVAL, "null words / scene or empty array");
-    if (depth < 1 || depth > 15) return ctx_error(c, VE_EINVAL, "depth %u outside [1,15]", depth);
-    // Every internal node's 8-word child block must lie inside the array, so
-    // the traversal can never fetch outside it.
-    for (uint32_t i = 0; i < n_words; i++) {
-        const uint32_t w = words[i];
-        if ((w & 1u) && (uint64_t)(w >> 4) + 8u > n_words)
-            return ctx_error(c, VE_EFORMAT, "node %u points to block %u outside the %u-word array", i, w >> 4, n_words);
-    }
-    HIP_OK(c, hipSetDevice(c->device));
-    ve_scene *s = new (std::nothrow) ve_scene();
-    if (!s) return ctx_error(c, VE_ENOMEM, "scene allocation");
-    void *d = nullptr;
-    hipError_t e = hipMalloc(&d, (size_t)n_words * 4);
-    if (e != hipSuccess) { delete s; return ctx_error(c, VE_ENOMEM, "hipMalloc node array: %s", hipGetErrorString(e)); }
-    e = hipMemcpyAsync(d, words, (size_t)n_words * 4, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) { (void)hipFree(d); delete s; return ctx_error(c, VE_EDEVICE, "node upload: %s", hipGetErrorString(e)); }
-    s->words = static_cast<uint32_t *>(d);
-    s->n_words = n_words;
-    s->depth = depth;
-    *scene = s;
-    return VE_OK;
+    std::string err;
+    const int rc = ve::scene_upload(c->device, c->stream, words, n_words, depth, scene, err);
+    return rc == VE_OK ? VE_OK : ctx_error(c, rc, "%s", err.c_str());
 }
 
 int ve_scene_release(ve_ctx *c, ve_scene *s)
@@ -390,8 +344,7 @@ int ve_scene_release(ve_ctx *c, ve_scene *s)
     if (!s) return ctx_error(c, VE_ENOTFOUND, "null scene");
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    (void)hipFree(s->words);
-    delete s;
+    ve::scene_free(s);
     return VE_OK;
 }
 
@@ -677,3 +630,145 @@ int ve_node_stats(ve_ctx *c, uint64_t *pr, uint64_t *tr, uint64_t *ps, uint64_t 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// internal entry points shared with the session server (ve_ctx.h)
+// ---------------------------------------------------------------------------
+namespace ve {
+
+size_t ctx_device_bytes(uint32_t max_w, uint32_t max_h, uint32_t hash_width)
+{
+    Alloc a;
+    DeviceBuffers b;
+    plan_buffers(a, b, max_w, max_h, hash_width ? hash_width : 16384u);
+    return a.bytes;
+}
+
+int ctx_create(ve_ctx **out, const ve_renderer_config *cfg, hipStream_t shared, std::string &err)
+{
+    if (!out || !cfg) return err = "null argument", VE_EINVAL;
+    if (cfg->max_width == 0 || cfg->max_height == 0) return err = "zero-sized framebuffer", VE_EINVAL;
+    if (cfg->max_bounces > 64) return err = "max_bounces above 64", VE_EINVAL;
+    const uint32_t hw = cfg->hash_width ? cfg->hash_width : 16384u;
+    if (hw < 2) return err = "hash_width below 2", VE_EINVAL;
+    const size_t groups_max = (size_t)((cfg->max_width + 7) / 8) * ((cfg->max_height + 7) / 8);
+    if (groups_max * 64 >= (1ull << 31)) return err = "framebuffer too large", VE_EINVAL;
+    ve_ctx *c = new (std::nothrow) ve_ctx();
+    if (!c) return err = "context allocation", VE_ENOMEM;
+    c->device = cfg->device;
+    c->max_w = cfg->max_width;
+    c->max_h = cfg->max_height;
+    c->hash_width = hw;
+    c->max_bounces = cfg->max_bounces ? cfg->max_bounces : 8u;
+    hipError_t e = hipSetDevice(c->device);
+    if (e == hipSuccess) {
+        if (shared) {
+            c->stream = shared;
+            c->owns_stream = false;
+        } else {
+            e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);   // default priority
+        }
+    }
+    if (e != hipSuccess) {
+        err = hipGetErrorString(e);
+        delete c;
+        return VE_EDEVICE;
+    }
+    Alloc a;
+    a.c = c;
+    plan_buffers(a, c->b, c->max_w, c->max_h, c->hash_width);
+    if (a.rc == VE_OK) {
+        e = hipMemcpy(c->b.rank_stats, INITIAL_THRESHOLDS, sizeof INITIAL_THRESHOLDS, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            a.rc = VE_EDEVICE;
+            c->err = hipGetErrorString(e);
+        }
+    }
+    if (a.rc != VE_OK) {
+        err = c->err;
+        ve_destroy(c);
+        return a.rc;
+    }
+    c->device_bytes = a.bytes;
+    *out = c;
+    return VE_OK;
+}
+
+int ctx_reset(ve_ctx *c)
+{
+    HIP_OK(c, hipSetDevice(c->device));
+    for (auto &a : c->allocations) HIP_OK(c, hipMemsetAsync(a.first, 0, a.second, c->stream));
+    HIP_OK(c, hipMemcpyAsync(c->b.rank_stats, INITIAL_THRESHOLDS, sizeof INITIAL_THRESHOLDS, hipMemcpyHostToDevice,
+                             c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    c->bounce_iter = 0;
+    c->frames_recorded = 0;
+    c->timing = false;
+    c->node_stats = false;
+    std::memset(c->cam, 0, sizeof c->cam);
+    std::memset(c->sky, 0, sizeof c->sky);
+    return VE_OK;
+}
+
+int scene_upload(int device, hipStream_t stream, const uint32_t *words, uint32_t n_words, uint32_t depth,
+                 ve_scene **scene, std::string &err)
+{
+    if (!words || !scene || n_words == 0) return err = "null words / scene or empty array", VE_EINVAL;
+    char buf[256];
+    if (depth < 1 || depth > 15) {
+        snprintf(buf, sizeof buf, "depth %u outside [1,15]", depth);
+        return err = buf, VE_EINVAL;
+    }
+    // Every internal node's 8-word child block must lie inside the array, so
+    // the traversal can never fetch outside it.
+    for (uint32_t i = 0; i < n_words; i++) {
+        const uint32_t w = words[i];
+        if ((w & 1u) && (uint64_t)(w >> 4) + 8u > n_words) {
+            snprintf(buf, sizeof buf, "node %u points to block %u outside the %u-word array", i, w >> 4, n_words);
+            return err = buf, VE_EFORMAT;
+        }
+    }
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return err = hipGetErrorString(e), VE_EDEVICE;
+    ve_scene *s = new (std::nothrow) ve_scene();
+    if (!s) return err = "scene allocation", VE_ENOMEM;
+    void *d = nullptr;
+    e = hipMalloc(&d, (size_t)n_words * 4);
+    if (e != hipSuccess) {
+        delete s;
+        return err = std::string("hipMalloc node array: ") + hipGetErrorString(e), VE_ENOMEM;
+    }
+    e = hipMemcpyAsync(d, words, (size_t)n_words * 4, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        delete s;
+        return err = std::string("node upload: ") + hipGetErrorString(e), VE_EDEVICE;
+    }
+    s->words = static_cast<uint32_t *>(d);
+    s->n_words = n_words;
+    s->depth = depth;
+    s->device_bytes = (size_t)n_words * 4;
+    *scene = s;
+    return VE_OK;
+}
+
+void scene_free(ve_scene *s)
+{
+    if (!s) return;
+    (void)hipFree(s->words);
+    delete s;
+}
+
+hipStream_t ctx_stream(ve_ctx *c) { return c->stream; }
+size_t ctx_bytes(const ve_ctx *c) { return c->device_bytes; }
+void ctx_shape(const ve_ctx *c, uint32_t *w, uint32_t *h, uint32_t *hash_width, uint32_t *max_bounces)
+{
+    *w = c->max_w;
+    *h = c->max_h;
+    *hash_width = c->hash_width;
+    *max_bounces = c->max_bounces;
+}
+unsigned long long *ctx_ray_totals(ve_ctx *c) { return c->b.ray_totals; }
+
+}  // namespace ve

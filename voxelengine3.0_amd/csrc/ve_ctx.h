@@ -1,0 +1,67 @@
+// Internal definitions of the C ABI's opaque handles and the host entry
+// points the session server (ve_server.cpp) shares with ve_api.cpp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/ve_svo.h"
+#include "ve_internal.h"
+#include "ve_launch.h"
+
+// A resident octree: the node words in device memory (Octree::set's buffer
+// texture, octree.cpp:157-166).
+struct ve_scene {
+    uint32_t *words = nullptr;
+    uint32_t n_words = 0;
+    uint32_t depth = 0;
+    size_t device_bytes = 0;
+};
+
+constexpr int MAX_ITERS = 8;   // host bounce-loop cap (renderer.cpp:515)
+
+// One renderer: the per-view device state of Renderer (renderer.cpp:20-169)
+// on one HIP stream.
+struct ve_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool owns_stream = true;            // false: a stream shared with other contexts
+    uint32_t max_w = 0, max_h = 0;
+    uint32_t hash_width = 16384;
+    uint32_t max_bounces = 8;
+    std::string err;
+    float cam[16] = {0};
+    float sky[3] = {0, 0, 0};
+    bool node_stats = false;
+    bool timing = false;
+    uint32_t bounce_iter = 0;           // next trace/shade iteration of per-pass dispatch
+    // timing: one event pair per pass and bounce iteration per recorded frame
+    std::vector<std::array<hipEvent_t, 2 * VE_PASS_COUNT * MAX_ITERS>> frame_events;
+    size_t frames_recorded = 0;
+    uint32_t iters_recorded = 0;
+    ve::DeviceBuffers b;
+    std::vector<std::pair<void *, size_t>> allocations;
+    size_t device_bytes = 0;
+};
+
+namespace ve {
+
+// Device bytes a context of this shape allocates.
+size_t ctx_device_bytes(uint32_t max_w, uint32_t max_h, uint32_t hash_width);
+// ve_create on a given stream (null: the context creates its own).
+int ctx_create(ve_ctx **out, const ve_renderer_config *cfg, hipStream_t shared, std::string &err);
+// Back to the state of a fresh context (all buffers zero, initial RankStats).
+int ctx_reset(ve_ctx *c);
+int scene_upload(int device, hipStream_t stream, const uint32_t *words, uint32_t n_words, uint32_t depth,
+                 ve_scene **scene, std::string &err);
+void scene_free(ve_scene *s);
+hipStream_t ctx_stream(ve_ctx *c);
+size_t ctx_bytes(const ve_ctx *c);
+void ctx_shape(const ve_ctx *c, uint32_t *w, uint32_t *h, uint32_t *hash_width, uint32_t *max_bounces);
+unsigned long long *ctx_ray_totals(ve_ctx *c);
+
+}  // namespace ve

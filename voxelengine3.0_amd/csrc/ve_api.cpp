@@ -56,8 +56,12 @@ int dalloc(ve_ctx *c, T **p, size_t count)
     void *ptr = nullptr;
     hipError_t e = hipMalloc(&ptr, count * sizeof(T) ? count * sizeof(T) : 16);
     if (e != hipSuccess) return ctx_error(c, VE_ENOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
-    e = hipMemset(ptr, 0, count * sizeof(T));
-    if (e != hipSuccess) return ctx_error(c, VE_EDEVICE, "hipMemset: %s", hipGetErrorString(e));
+    // cleared on the context's stream, ahead of everything that uses it
+    e = hipMemsetAsync(ptr, 0, count * sizeof(T), c->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(ptr);
+        return ctx_error(c, VE_EDEVICE, "hipMemsetAsync: %s", hipGetErrorString(e));
+    }
     c->allocations.push_back({ptr, count * sizeof(T)});
     *p = static_cast<T *>(ptr);
     return VE_OK;
@@ -678,7 +682,8 @@ int ctx_create(ve_ctx **out, const ve_renderer_config *cfg, hipStream_t shared, 
     a.c = c;
     plan_buffers(a, c->b, c->max_w, c->max_h, c->hash_width);
     if (a.rc == VE_OK) {
-        e = hipMemcpy(c->b.rank_stats, INITIAL_THRESHOLDS, sizeof INITIAL_THRESHOLDS, hipMemcpyHostToDevice);
+        e = hipMemcpyAsync(c->b.rank_stats, INITIAL_THRESHOLDS, sizeof INITIAL_THRESHOLDS, hipMemcpyHostToDevice,
+                           c->stream);
         if (e != hipSuccess) {
             a.rc = VE_EDEVICE;
             c->err = hipGetErrorString(e);
@@ -700,7 +705,6 @@ int ctx_reset(ve_ctx *c)
     for (auto &a : c->allocations) HIP_OK(c, hipMemsetAsync(a.first, 0, a.second, c->stream));
     HIP_OK(c, hipMemcpyAsync(c->b.rank_stats, INITIAL_THRESHOLDS, sizeof INITIAL_THRESHOLDS, hipMemcpyHostToDevice,
                              c->stream));
-    HIP_OK(c, hipStreamSynchronize(c->stream));
     c->bounce_iter = 0;
     c->frames_recorded = 0;
     c->timing = false;

@@ -54,7 +54,8 @@ namespace ve {
 size_t ctx_device_bytes(uint32_t max_w, uint32_t max_h, uint32_t hash_width);
 // ve_create on a given stream (null: the context creates its own).
 int ctx_create(ve_ctx **out, const ve_renderer_config *cfg, hipStream_t shared, std::string &err);
-// Back to the state of a fresh context (all buffers zero, initial RankStats).
+// Back to the state of a fresh context (all buffers zero, initial RankStats),
+// enqueued on the context's stream without waiting.
 int ctx_reset(ve_ctx *c);
 int scene_upload(int device, hipStream_t stream, const uint32_t *words, uint32_t n_words, uint32_t depth,
                  ve_scene **scene, std::string &err);

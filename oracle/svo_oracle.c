@@ -1145,6 +1145,305 @@ static v3 lookup_normal(const so_state *s, uint32_t vid, v3 rd)   /* shade.comp:
     return V(0, 0, -gsign(rd.z));
 }
 
+/* ======================================================================== */
+/* Next-event estimation: ReSTIR DI (bounce 0) and single-pick RIS          */
+/* (shade.comp:276-539, 620-836)                                             */
+/* ======================================================================== */
+typedef struct reservoir {
+    uint32_t light_idx, light_vid, face_seed;
+    float wsum;
+    uint32_t m;
+    float target_pdf;
+} reservoir;
+
+static const v3 LUMA = {0.2126f, 0.7152f, 0.0722f};
+
+static void r_update(reservoir *r, uint32_t idx, uint32_t lvid, uint32_t seed, float target, float w,
+                     uint32_t *rng)   /* shade.comp:320-336 */
+{
+    if (w <= 0.0f) { r->m += 1u; return; }
+    r->wsum += w;
+    r->m += 1u;
+    if (rand_lds(rng) * r->wsum < w) {
+        r->light_idx = idx; r->light_vid = lvid; r->face_seed = seed; r->target_pdf = target;
+    }
+}
+
+static void r_combine(reservoir *a, const reservoir *b, float target_at_us, uint32_t *rng)   /* shade.comp:352-365 */
+{
+    if (b->m == 0u || b->target_pdf <= 0.0f || target_at_us <= 0.0f) return;
+    float w = target_at_us * b->wsum / b->target_pdf;
+    a->wsum += w;
+    a->m += b->m;
+    if (rand_lds(rng) * a->wsum < w) {
+        a->light_idx = b->light_idx; a->light_vid = b->light_vid; a->face_seed = b->face_seed;
+        a->target_pdf = target_at_us;
+    }
+}
+
+static void r_cap(reservoir *r, uint32_t max_m)   /* shade.comp:372-377 */
+{
+    if (r->m > max_m && max_m > 0u) {
+        r->wsum = r->wsum * (float)max_m / (float)r->m;
+        r->m = max_m;
+    }
+}
+
+static uint32_t pack_face_seed(uint32_t face, float u, float v)   /* shade.comp:382-386 */
+{
+    uint32_t ui = f2u(gclamp(u, 0.0f, 0.9999f) * 16383.0f), vi = f2u(gclamp(v, 0.0f, 0.9999f) * 16383.0f);
+    return (face & 0xFu) | ((ui & 0x3FFFu) << 4) | ((vi & 0x3FFFu) << 18);
+}
+
+/* face frame of the six-face construction (shade.comp:398-413, 761-771) */
+static void face_frame(uint32_t face, v3 *n, v3 *t, v3 *b)
+{
+    if (face < 2u) { *n = V(face == 0u ? 1.0f : -1.0f, 0, 0); *t = V(0, 1, 0); *b = V(0, 0, 1); }
+    else if (face < 4u) { *n = V(0, face == 2u ? 1.0f : -1.0f, 0); *t = V(1, 0, 0); *b = V(0, 0, 1); }
+    else { *n = V(0, 0, face == 4u ? 1.0f : -1.0f); *t = V(1, 0, 0); *b = V(0, 1, 0); }
+}
+
+static void light_sample_point(uint32_t seed, const emis_t *e, v3 *P, v3 *N)   /* shade.comp:393-420 */
+{
+    uint32_t face = seed & 0xFu;
+    float u = (float)((seed >> 4) & 0x3FFFu) / 16383.0f, v = (float)((seed >> 18) & 0x3FFFu) / 16383.0f;
+    if (face > 5u) face = 5u;
+    v3 fn, ft, fb;
+    face_frame(face, &fn, &ft, &fb);
+    float half = (float)e->size * 0.5f;
+    v3 c = vadd(V((float)e->px, (float)e->py, (float)e->pz), V(half, half, half));
+    *P = vadd(vadd(vadd(c, vmuls(fn, half)), vmuls(ft, (u - 0.5f) * 2.0f * half)), vmuls(fb, (v - 0.5f) * 2.0f * half));
+    *N = fn;
+}
+
+static float target_pdf(const emis_t *e, uint32_t seed, v3 sp, v3 nrm)   /* shade.comp:423-438 */
+{
+    v3 P, N;
+    light_sample_point(seed, e, &P, &N);
+    v3 to = vsub(P, sp);
+    float d2 = vdot(to, to);
+    if (d2 < 1e-6f) return 0.0f;
+    float d = sqrtf(d2);
+    v3 ld = vdivs(to, d);
+    float cx = vdot(nrm, ld);
+    if (cx <= 0.0f) return 0.0f;
+    float cy = vdot(V(-ld.x, -ld.y, -ld.z), N);
+    if (cy <= 0.0f) return 0.0f;
+    float lum = gmax(vdot(V(e->ex, e->ey, e->ez), LUMA), 1e-6f);
+    float area = 6.0f * (float)(e->size * e->size);
+    return lum * cy * area / d2;
+}
+
+/* rBufferLoad (shade.comp:466-485) over the reference-layout image */
+static int rbuf_load(const so_state *s, uint32_t vid, reservoir *r)
+{
+    memset(r, 0, sizeof *r);
+    uint32_t row = vid % (s->hw - 1u) + 1u;
+    for (uint32_t k = 0; k < RSLOTS; k++) {
+        if (RB(s, row, k * RSTRIDE) != vid) continue;
+        r->light_idx = RB(s, row, k * RSTRIDE + 1);
+        r->light_vid = RB(s, row, k * RSTRIDE + 2);
+        r->face_seed = RB(s, row, k * RSTRIDE + 3);
+        memcpy(&r->wsum, &RB(s, row, k * RSTRIDE + 4), 4);
+        r->m = RB(s, row, k * RSTRIDE + 5);
+        memcpy(&r->target_pdf, &RB(s, row, k * RSTRIDE + 6), 4);
+        uint32_t t = RB(s, row, k * RSTRIDE + 7);
+        if (!s->fc->disable_stale_reset && t != 0u && t + s->fc->update_time < s->time) {
+            memset(r, 0, sizeof *r);
+            return 0;
+        }
+        return 1;
+    }
+    return 0;
+}
+
+static void rbuf_store_at(so_state *s, uint32_t row, uint32_t k, uint32_t vid, const reservoir *r)
+{
+    RB(s, row, k * RSTRIDE) = vid;
+    RB(s, row, k * RSTRIDE + 1) = r->light_idx;
+    RB(s, row, k * RSTRIDE + 2) = r->light_vid;
+    RB(s, row, k * RSTRIDE + 3) = r->face_seed;
+    memcpy(&RB(s, row, k * RSTRIDE + 4), &r->wsum, 4);
+    RB(s, row, k * RSTRIDE + 5) = r->m;
+    memcpy(&RB(s, row, k * RSTRIDE + 6), &r->target_pdf, 4);
+    RB(s, row, k * RSTRIDE + 7) = s->time;
+}
+
+static void rbuf_store(so_state *s, uint32_t vid, const reservoir *r)   /* rBufferStore, shade.comp:501-519 */
+{
+    uint32_t row = vid % (s->hw - 1u) + 1u, oldest = 0, oldest_t = 0xFFFFFFFFu;
+    for (uint32_t k = 0; k < RSLOTS; k++) {
+        uint32_t prev = RB(s, row, k * RSTRIDE);
+        if (prev == 0u) RB(s, row, k * RSTRIDE) = vid;
+        if (prev == 0u || prev == vid) { rbuf_store_at(s, row, k, vid, r); return; }
+        uint32_t t = RB(s, row, k * RSTRIDE + 7);
+        if (t < oldest_t) { oldest_t = t; oldest = k; }
+    }
+    rbuf_store_at(s, row, oldest, vid, r);
+}
+
+static int light_valid(const reservoir *r, const so_state *s, uint32_t count)   /* shade.comp:446-449 */
+{
+    if (r->m == 0u || r->light_idx >= count) return 0;
+    return s->emissives[r->light_idx].vid == r->light_vid;
+}
+
+static uint32_t neighbour_vid(const uint32_t hp[3], uint32_t size, const int off[3])   /* shade.comp:527-535 */
+{
+    int q[3];
+    for (int i = 0; i < 3; i++) {
+        q[i] = (int)hp[i] + off[i] * (int)size;
+        if (q[i] < 0) return 0u;
+    }
+    uint32_t sn[3];
+    for (int i = 0; i < 3; i++) sn[i] = ((uint32_t)q[i] & ~(size - 1u)) & 0x3FFu;
+    return so_voxel_key(sn) + 1u;
+}
+
+/* RaycastVisibilityToVoxel (internal.glsl:406-410), bounce check budget */
+static int visible_to(so_state *s, v3 o, v3 d, uint32_t target)
+{
+    float org[3] = {o.x, o.y, o.z}, dir[3] = {d.x, d.y, d.z};
+    float inv[3] = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    uint32_t checks = 0;
+    s->counters[5]++;
+    so_hit h = so_raycast(s->words, s->n_words, s->depth, org, dir, inv, (uint32_t)s->fc->bounce_controlchecks,
+                          &checks, NULL);
+    return h.hit && so_voxel_key(h.pos) + 1u == target;
+}
+
+/* Per-lane NEE state of one SIMD-8 vector: the reservoir to store once every
+ * lane of the vector has finished its reads (shade.comp:812-814). */
+typedef struct nee_store { int pending; uint32_t vid; reservoir res; } nee_store;
+
+static void nee_restir(so_state *s, ray_t *r, const mat_t *m, v3 nrm, const uint32_t hp[3], uint32_t hsize,
+                       uint32_t n_lights, uint32_t count, nee_store *st)   /* shade.comp:646-815 */
+{
+    v3 sp = vadd(vadd(V((float)hp[0], (float)hp[1], (float)hp[2]), V(0.5f, 0.5f, 0.5f)), nrm);
+    reservoir res;
+    memset(&res, 0, sizeof res);
+    uint32_t pvid = r->primary_vid;
+    int nm = s->fc->restir_m > 1 ? s->fc->restir_m : 1;
+    float inv_src = (float)n_lights * 6.0f;
+    for (int k = 0; k < nm; k++) {
+        uint32_t pick = f2u(rand_lds(&r->rng) * (float)n_lights);
+        if (pick >= n_lights) pick = n_lights - 1u;
+        const emis_t *c = &s->emissives[pick];
+        uint32_t face = f2u(rand_lds(&r->rng) * 6.0f);
+        if (face >= 6u) face = 5u;
+        float uu = rand_lds(&r->rng);
+        float vv = rand_lds(&r->rng);
+        uint32_t seed = pack_face_seed(face, uu, vv);
+        float tg = target_pdf(c, seed, sp, nrm);
+        r_update(&res, pick, c->vid, seed, tg, tg * inv_src, &r->rng);
+    }
+    if (s->fc->restir_temporal) {
+        reservoir tr;
+        if (rbuf_load(s, pvid, &tr) && light_valid(&tr, s, count)) {
+            r_cap(&tr, (uint32_t)s->fc->restir_max_m);
+            r_combine(&res, &tr, tr.target_pdf, &r->rng);
+        }
+    }
+    if (s->fc->restir_spatial && s->fc->restir_spatial_neighbors > 0) {
+        v3 up = fabsf(nrm.z) < 0.999f ? V(0, 0, 1) : V(1, 0, 0);
+        v3 T = vnorm(vcross(up, nrm));
+        v3 B = vcross(nrm, T);
+        int K = s->fc->restir_spatial_neighbors < 6 ? s->fc->restir_spatial_neighbors : 6;
+        for (int k = 0; k < K; k++) {
+            float ang = 6.2831853f * rand_lds(&r->rng);
+            float rr = s->fc->restir_spatial_radius * sqrtf(rand_lds(&r->rng));
+            v3 off = vmuls(vadd(vmuls(T, m_cos(ang)), vmuls(B, m_sin(ang))), rr);
+            int lo[3] = {(int)rintf(off.x), (int)rintf(off.y), (int)rintf(off.z)};
+            if (lo[0] == 0 && lo[1] == 0 && lo[2] == 0) continue;
+            uint32_t nv = neighbour_vid(hp, hsize, lo);
+            if (nv == 0u || nv == pvid) continue;
+            reservoir nb;
+            if (!rbuf_load(s, nv, &nb) || !light_valid(&nb, s, count)) continue;
+            float t_us = target_pdf(&s->emissives[nb.light_idx], nb.face_seed, sp, nrm);
+            if (t_us <= 0.0f) continue;
+            r_cap(&nb, (uint32_t)s->fc->restir_max_m);
+            r_combine(&res, &nb, t_us, &r->rng);
+        }
+    }
+    if (res.m > 0u && res.target_pdf > 0.0f) {
+        const emis_t *e = &s->emissives[res.light_idx];
+        if (e->vid == res.light_vid) {
+            v3 P, N;
+            light_sample_point(res.face_seed, e, &P, &N);
+            v3 to = vsub(P, sp);
+            float d2 = vdot(to, to);
+            float d = sqrtf(gmax(d2, 1e-12f));
+            if (d > 1e-3f) {
+                v3 ld = vdivs(to, d);
+                float cx = gmax(vdot(nrm, ld), 0.0f);
+                float cy = gmax(vdot(V(-ld.x, -ld.y, -ld.z), N), 0.0f);
+                if (cx > 0.0f && cy > 0.0f && visible_to(s, sp, ld, res.light_vid)) {
+                    v3 bc = evaluate_brdf(V(r->dx, r->dy, r->dz), ld, nrm, V(m->color[0], m->color[1], m->color[2]),
+                                          m->roughness, m->metallic, m->specular);
+                    float W = res.wsum / ((float)res.m * res.target_pdf);
+                    float sz2 = (float)(e->size * e->size);
+                    float geom = cy / gmax(d2, 1e-6f);
+                    v3 c = vmuls(vmuls(vmuls(vmul(V(e->ex, e->ey, e->ez), bc), geom), sz2), W);
+                    c = vmin(c, V(20, 20, 20));
+                    v3 add = vmul(V(r->tx, r->ty, r->tz), c);
+                    r->dlx += add.x; r->dly += add.y; r->dlz += add.z;
+                }
+            }
+        }
+    }
+    st->pending = res.m > 0u;
+    st->vid = pvid;
+    st->res = res;
+}
+
+static void nee_ris(so_state *s, ray_t *r, const mat_t *m, v3 nrm, const uint32_t hp[3],
+                    uint32_t n_lights)   /* shade.comp:816-836 (the RIS_M = 4 path) */
+{
+    v3 sp = vadd(vadd(V((float)hp[0], (float)hp[1], (float)hp[2]), V(0.5f, 0.5f, 0.5f)), nrm);
+    float wsum = 0.0f, chosen_t = 0.0f;
+    uint32_t chosen = 0u;
+    for (int k = 0; k < 4; k++) {
+        uint32_t pick = f2u(rand_lds(&r->rng) * (float)n_lights);
+        if (pick >= n_lights) pick = n_lights - 1u;
+        const emis_t *c = &s->emissives[pick];
+        float lum = gmax(vdot(V(c->ex, c->ey, c->ez), LUMA), 1e-6f);
+        float tg = lum * (float)(c->size * c->size);
+        wsum += tg;
+        if (rand_lds(&r->rng) * wsum < tg) { chosen = pick; chosen_t = tg; }
+    }
+    float avg = wsum / 4.0f;
+    const emis_t *e = &s->emissives[chosen];
+    uint32_t face = f2u(rand_lds(&r->rng) * 6.0f);
+    if (face >= 6u) face = 5u;
+    v3 fn, ft, fb;
+    face_frame(face, &fn, &ft, &fb);
+    float half = (float)e->size * 0.5f;
+    v3 c = vadd(V((float)e->px, (float)e->py, (float)e->pz), V(half, half, half));
+    float ru = rand_lds(&r->rng);
+    float rv = rand_lds(&r->rng);
+    v3 lp = vadd(vadd(vadd(c, vmuls(fn, half)), vmuls(ft, (ru - 0.5f) * 2.0f * half)), vmuls(fb, (rv - 0.5f) * 2.0f * half));
+    v3 ld = vsub(lp, sp);
+    float d2 = vdot(ld, ld);
+    float d = sqrtf(d2);
+    if (d > 1e-3f && chosen_t > 0.0f) {
+        ld = vdivs(ld, d);
+        float cx = gmax(vdot(nrm, ld), 0.0f);
+        float cy = gmax(vdot(V(-ld.x, -ld.y, -ld.z), fn), 0.0f);
+        if (cx > 0.0f && cy > 0.0f && visible_to(s, sp, ld, e->vid)) {
+            v3 bc = evaluate_brdf(V(r->dx, r->dy, r->dz), ld, nrm, V(m->color[0], m->color[1], m->color[2]),
+                                  m->roughness, m->metallic, m->specular);
+            float area = 6.0f * (float)e->size * (float)e->size;
+            float ris = (float)n_lights * avg / gmax(chosen_t, 1e-6f);
+            float geom = cy / gmax(d2, 1e-6f);
+            v3 cc = vmuls(vmuls(vmuls(vmul(V(e->ex, e->ey, e->ez), bc), geom), area), ris);
+            cc = vmin(cc, V(20, 20, 20));
+            v3 w = vmul(V(r->tx, r->ty, r->tz), cc);
+            if (r->bounce == 0u) { r->dlx += w.x; r->dly += w.y; r->dlz += w.z; }
+            else { r->ilx += w.x; r->ily += w.y; r->ilz += w.z; }
+        }
+    }
+}
+
 /* shade.comp:main, one SIMD-8 vector at a time so side effects (deposits,
  * emissive registry, survivor appends) happen in the order llvmpipe issues
  * them: every lane's miss deposit, then hits, then last-bounce deposits,
@@ -1217,10 +1516,23 @@ static void shade_pass(so_state *s, uint32_t base, uint32_t size)
                 }
             }
         }
-        /* next-event estimation: only with emissives registered (shade.comp:631) */
-        for (uint32_t l = 0; l < 8; l++) {
-            if (!live[l] || mat[l]->emissive || r[l].bounce >= 2u || s->emissive_count == 0) continue;
-            s->counters[5] += 0;   /* NEE is not exercised by emissive-free scenes */
+        /* next-event estimation (shade.comp:631-836): every lane of the vector
+         * sees the registry after this vector's registrations; reservoir reads
+         * of all lanes come before any lane's store */
+        {
+            nee_store st[8];
+            memset(st, 0, sizeof st);
+            uint32_t count = s->emissive_count;
+            uint32_t n_lights = count < EM_CAP ? count : EM_CAP;
+            for (uint32_t l = 0; l < 8; l++) {
+                if (!live[l] || mat[l]->emissive || r[l].bounce >= 2u || count == 0) continue;
+                if (s->fc->restir_enabled && r[l].bounce == 0u)
+                    nee_restir(s, &r[l], mat[l], normal[l], hp[l], hsize[l], n_lights, count, &st[l]);
+                else
+                    nee_ris(s, &r[l], mat[l], normal[l], hp[l], n_lights);
+            }
+            for (uint32_t l = 0; l < 8; l++)
+                if (st[l].pending) rbuf_store(s, st[l].vid, &st[l].res);
         }
         /* last allowed bounce: deposit and die */
         for (uint32_t l = 0; l < 8; l++) {

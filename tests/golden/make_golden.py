@@ -186,15 +186,62 @@ def make_frame():
         json.dump(meta, fh, indent=1, sort_keys=True)
 
 
+# Emissive scene (next-event estimation): ReSTIR with temporal reuse only,
+# and with spatial reuse as well; one sample per visible voxel per frame.
+LIGHT_KNOBS = dict(DET_KNOBS, restir_spatial=0)
+LIGHT_RUNS = {
+    "lights_temporal": dict(W=128, H=128, frames=6, max_bounces=2, knobs=LIGHT_KNOBS),
+    "lights_spatial": dict(W=128, H=128, frames=4, max_bounces=3, knobs=dict(DET_KNOBS, restir_spatial=1)),
+}
+
+
+def make_lights():
+    from oracle import oracle as O
+    depth, ops = scenes.scene_lights(1)
+    words, info = ve.octree_build(ops, depth)
+    mats = scenes.palette_lights(1)
+    meta = {"depth": depth, "words_sha256": words_digest(words, info.size), "sky_rgb8": SKY_RGB8, "runs": {}}
+    arrays = {"materials": mats.view(np.uint8)}
+    for name, run in LIGHT_RUNS.items():
+        W, H = run["W"], run["H"]
+        fc = O.FrameConfig(**run["knobs"])
+        eye, d, fov = scenes.camera_small()
+        ubo = ve.camera_ubo(eye, d, fov, W / H)
+        pl = R.GLPipeline(words, depth, W, H, max_bounces=run["max_bounces"], materials=mats, sky_rgb8=SKY_RGB8)
+        pl.set_camera(ubo)
+        counters = []
+        for f in range(run["frames"]):
+            st = pl.frame(fc.as_dict(), f, frame_seed(f), f + 1)
+            counters.append([st["candidates"], st["scheduled"], st["bounces"], st["traced"], st["emissives"]])
+            print(name, "frame", f, st, flush=True)
+        arrays[name + "_ubo"] = ubo
+        arrays[name + "_counters"] = np.array(counters, np.uint32)
+        arrays[name + "_out"] = pl.out.read()
+        g, p = pl.read_gbuffer()
+        arrays[name + "_gbuf"] = g
+        lv, lval = hash_map(pl.read_hash(pl.lbuf, 384).reshape(-1), 12, 32)
+        arrays[name + "_lbuf_vids"], arrays[name + "_lbuf_vals"] = lv, lval
+        rv, rval = hash_map(pl.read_hash(pl.rbuf, 128).reshape(-1), 8, 16)
+        arrays[name + "_rbuf_vids"], arrays[name + "_rbuf_vals"] = rv, rval
+        ne = int(counters[-1][4])
+        arrays[name + "_emissives"] = pl.emissives.read()[: 8 * ne].reshape(-1, 8)
+        meta["runs"][name] = dict(W=W, H=H, frames=run["frames"], max_bounces=run["max_bounces"], knobs=run["knobs"])
+    np.savez_compressed(os.path.join(HERE, "frame_lights.npz"), **arrays)
+    with open(os.path.join(HERE, "frame_lights.json"), "w") as fh:
+        json.dump(meta, fh, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
     if os.environ.get("LP_NUM_THREADS") != "1":
         sys.exit("run with LP_NUM_THREADS=1 (deterministic llvmpipe dispatch order)")
     if not R.shaders_available():
         sys.exit("reference shaders not found at %s" % R.SHADER_DIR)
-    what = sys.argv[1:] or ["fp", "primary", "frame"]
+    what = sys.argv[1:] or ["fp", "primary", "frame", "lights"]
     if "fp" in what:
         make_fp()
     if "primary" in what:
         make_primary()
     if "frame" in what:
         make_frame()
+    if "lights" in what:
+        make_lights()

@@ -192,6 +192,23 @@ def scene_extreme(seed=1, power=EXTREME_SPHERE_POWER):
     return 10, np.concatenate(ops)
 
 
+LIGHT_BOXES = (((28, 78, 40), (33, 83, 45)), ((88, 74, 70), (91, 77, 73)), ((52, 98, 56), (70, 99, 74)))
+
+
+def scene_lights(seed=1):
+    """The depth-7 terrain of scene_small plus three emissive blocks above it
+    (a 5^3 cube, a 3^3 cube and an 18x1x18 ceiling panel) using materials 9
+    and 10 of palette_lights; the blocks are inserted after the terrain."""
+    depth, ops = scene_small(seed)
+    extra = []
+    for k, (lo, hi) in enumerate(LIGHT_BOXES):
+        X, Y, Z = np.meshgrid(*[np.arange(lo[i], hi[i]) for i in range(3)], indexing="ij")
+        o = np.empty(X.size, INSERT_DTYPE)
+        o["x"], o["y"], o["z"], o["material"] = X.ravel(), Y.ravel(), Z.ravel(), 9 + (k % 2)
+        extra.append(o)
+    return depth, np.concatenate([ops] + extra)
+
+
 def scene_fbm(depth, seed):
     """fbm terrain shell at any depth (batched / served resident scenes)."""
     n = 1 << depth
@@ -217,6 +234,20 @@ def palette(seed=1, count=8):
         m[i]["roughness"] = 1.0
         m[i]["specular"] = 0.0
         m[i]["metallic"] = 0.0
+    return m
+
+
+def palette_lights(seed=1):
+    """palette(seed) plus two emissive entries: 9 warm white (intensity 4),
+    10 blue (intensity 6)."""
+    m = np.zeros(11, MATERIAL_DTYPE)
+    m[:9] = palette(seed)
+    for i, (rgb, inten) in ((9, ((1.0, 0.9, 0.7), 4.0)), (10, ((0.4, 0.6, 1.0), 6.0))):
+        m[i]["color"] = [rgb[0], rgb[1], rgb[2], 0.0]
+        m[i]["specular_color"] = [1.0, 1.0, 1.0, 1.0]
+        m[i]["roughness"] = 1.0
+        m[i]["emissive"] = 1
+        m[i]["emissive_intensity"] = inten
     return m
 
 

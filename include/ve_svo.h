@@ -201,8 +201,10 @@ typedef enum ve_buffer_id {
 #define VE_CNT_BOUNCES_RUN    7   /* trace/shade iterations with live rays */
 #define VE_CNT_LIVE_B         8   /* live-ray count of the other ray buffer (internal) */
 #define VE_CNT_OVERFLOW       9   /* nonzero: a claim row overflowed the ordering table */
-#define VE_CNT_NEE_PENDING   10   /* nonzero: a hit needed next-event estimation (emissive scenes) */
+/* 10: unused */
 #define VE_CNT_TIMEOUTS      11   /* pixels whose primary ray timed out (last resolve) */
+#define VE_CNT_EM_BASE       12   /* emissive registry size before this frame's registrations (internal) */
+#define VE_CNT_EM_NEW        13   /* registrations made by this frame's first shade (internal) */
 
 int ve_read(ve_ctx *ctx, ve_buffer_id id, void *dst, size_t bytes);
 int ve_write(ve_ctx *ctx, ve_buffer_id id, const void *src, size_t bytes);

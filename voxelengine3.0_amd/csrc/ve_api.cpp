@@ -100,6 +100,8 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.rank_stats, RANK_STATS_WORDS);
     a(&b.emissives, EMISSIVE_CAPACITY);
     a(&b.emissive_claim, (size_t)EMISSIVE_CLAIM_W * EMISSIVE_CLAIM_PROBES);
+    a(&b.em_bits, SHADE_BUDGET_MAX / 32);
+    a(&b.em_reg, EMISSIVE_CAPACITY);
     a(&b.materials, MATERIAL_SLOTS);
     a(&b.node_stats, 4);
     a(&b.ray_totals, 3);
@@ -416,7 +418,7 @@ int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p)
         break;
     case VE_PASS_SHADE:
         if (c->bounce_iter >= 8u) return ctx_error(c, VE_ESTATE, "shade after 8 bounce iterations (emit first)");
-        launch_shade(c->stream, p, c->b, c->bounce_iter);
+        launch_shade(c->stream, s->words, p, c->b, c->bounce_iter);
         c->bounce_iter++;
         break;
     case VE_PASS_RESOLVE: launch_resolve(c->stream, p, c->b); break;

@@ -41,9 +41,12 @@ __device__ __forceinline__ f3 sub(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, 
 __device__ __forceinline__ f3 mul(f3 a, f3 b) { return f3{a.x * b.x, a.y * b.y, a.z * b.z}; }
 __device__ __forceinline__ f3 muls(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
 __device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + (a.y * b.y + a.z * b.z); }
+// sqrtf compiles to a correctly rounded square root with this build's flags
+// (the IEEE result llvmpipe and the C restatement use); __fsqrt_rn does not
+// (about 15% of results differ by an ulp on gfx950), so it is not used.
 __device__ __forceinline__ f3 normalize(f3 v)
 {
-    float inv = 1.0f / __fsqrt_rn(dot(v, v));
+    float inv = 1.0f / sqrtf(dot(v, v));
     return muls(v, inv);
 }
 __device__ __forceinline__ f3 rcp3(f3 d) { return f3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z}; }
@@ -228,7 +231,7 @@ __device__ __forceinline__ float m_cos(float x) { return (float)cos((double)x); 
 __device__ __forceinline__ float m_sin(float x) { return (float)sin((double)x); }
 __device__ __forceinline__ float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
 __device__ __forceinline__ float gsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
-__device__ __forceinline__ float length3(f3 a) { return __fsqrt_rn(dot(a, a)); }
+__device__ __forceinline__ float length3(f3 a) { return sqrtf(dot(a, a)); }
 __device__ __forceinline__ f3 divs(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
 __device__ __forceinline__ f3 vmin3(f3 a, f3 b) { return f3{gmin(a.x, b.x), gmin(a.y, b.y), gmin(a.z, b.z)}; }
 __device__ __forceinline__ f3 cross3(f3 a, f3 b)
@@ -272,7 +275,7 @@ __device__ __forceinline__ float rand_lds(uint32_t &state)
 __device__ __forceinline__ float rand_normal(uint32_t &state)   // internal.glsl:101-105
 {
     const float theta = (2.0f * 3.1415926f) * rand_lds(state);
-    const float rho = __fsqrt_rn(-2.0f * m_log(rand_lds(state)));
+    const float rho = sqrtf(-2.0f * m_log(rand_lds(state)));
     return rho * m_cos(theta);
 }
 
@@ -306,7 +309,7 @@ __device__ __forceinline__ f3 evaluate_brdf(f3 Vd, f3 L, f3 N, f3 albedo, float 
 __device__ __forceinline__ float smith_g1(float NdotX, float a)
 {
     const float a2 = a * a;
-    return 2.0f * NdotX / (NdotX + __fsqrt_rn(a2 + (1.0f - a2) * NdotX * NdotX));
+    return 2.0f * NdotX / (NdotX + sqrtf(a2 + (1.0f - a2) * NdotX * NdotX));
 }
 
 // sampleGGXVNDF (internal.glsl:182-219)
@@ -319,17 +322,17 @@ __device__ __forceinline__ f3 sample_ggx_vndf(f3 N, f3 Vv, float roughness, uint
     const f3 Vl = f3{dot(Vv, T), dot(Vv, B), dot(Vv, N)};
     const f3 Vh = normalize(f3{a * Vl.x, a * Vl.y, Vl.z});
     const float lensq = Vh.x * Vh.x + Vh.y * Vh.y;
-    const f3 T1 = lensq > 0.0f ? muls(f3{-Vh.y, Vh.x, 0.0f}, 1.0f / __fsqrt_rn(lensq)) : f3{1, 0, 0};
+    const f3 T1 = lensq > 0.0f ? muls(f3{-Vh.y, Vh.x, 0.0f}, 1.0f / sqrtf(lensq)) : f3{1, 0, 0};
     const f3 T2 = cross3(Vh, T1);
     const float r1 = rand_lds(state);
     const float r2 = rand_lds(state);
-    const float r = __fsqrt_rn(r1);
+    const float r = sqrtf(r1);
     const float phi = 6.2831853f * r2;
     const float t1 = r * m_cos(phi);
     float t2 = r * m_sin(phi);
     const float sc = 0.5f * (1.0f + Vh.z);
-    t2 = (1.0f - sc) * __fsqrt_rn(gmax(1.0f - t1 * t1, 0.0f)) + sc * t2;
-    const f3 Nh = add(add(muls(T1, t1), muls(T2, t2)), muls(Vh, __fsqrt_rn(gmax(0.0f, 1.0f - t1 * t1 - t2 * t2))));
+    t2 = (1.0f - sc) * sqrtf(gmax(1.0f - t1 * t1, 0.0f)) + sc * t2;
+    const f3 Nh = add(add(muls(T1, t1), muls(T2, t2)), muls(Vh, sqrtf(gmax(0.0f, 1.0f - t1 * t1 - t2 * t2))));
     const f3 Hl = normalize(f3{a * Nh.x, a * Nh.y, gmax(0.0f, Nh.z)});
     return add(add(muls(T, Hl.x), muls(B, Hl.y)), muls(N, Hl.z));
 }

@@ -29,6 +29,8 @@ struct DeviceBuffers {
     uint32_t *rank_stats = nullptr; // RANK_STATS_WORDS
     EmissiveVoxel *emissives = nullptr;  // EMISSIVE_CAPACITY
     uint32_t *emissive_claim = nullptr;  // EMISSIVE_CLAIM_W x EMISSIVE_CLAIM_PROBES
+    uint32_t *em_bits = nullptr;         // SHADE_BUDGET_MAX / 32: first-shade rays that may register a light
+    uint32_t *em_reg = nullptr;          // EMISSIVE_CAPACITY: ray index of each registration this frame
     Material *materials = nullptr;       // MATERIAL_SLOTS
     unsigned long long *node_stats = nullptr;  // 4: primary reads, trace reads, primary steps, trace steps
     unsigned long long *ray_totals = nullptr;  // 3: primary, traced, shadow (cumulative)
@@ -55,7 +57,7 @@ void launch_emit(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 void launch_threshold(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 void launch_trace(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t iter,
                   bool stats);
-void launch_shade(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_t iter);
+void launch_shade(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t iter);
 void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 size_t scan_temp_bytes(uint32_t max_items);
 

@@ -548,11 +548,26 @@ __global__ __launch_bounds__(256) void k_emit(FrameParams p, const uint2 *__rest
 // ---------------------------------------------------------------------------
 // trace.comp
 // ---------------------------------------------------------------------------
+// Bounce-0 rays that hit an emissive voxel the claim map does not hold yet
+// set their bit in `em_bits`; k_em_register then registers them in ray order.
+__device__ __forceinline__ bool emissive_unknown(const uint32_t *__restrict__ em_claim, uint32_t vid)
+{
+    const uint32_t ch = claim_hash(vid) % EMISSIVE_CLAIM_W;
+    for (uint32_t q = 0; q < EMISSIVE_CLAIM_PROBES; q++) {
+        const uint32_t c = em_claim[(size_t)ch * EMISSIVE_CLAIM_PROBES + q];
+        if (c == vid) return false;
+        if (c == 0u) return true;
+    }
+    return false;   // every probe taken by other ids: the reference cannot register it either
+}
+
 template <bool STATS>
 __global__ __launch_bounds__(256) void k_trace(const uint32_t *__restrict__ words, FrameParams p, Ray *__restrict__ rays,
                                                const uint32_t *__restrict__ live_ptr, uint32_t iter,
                                                uint32_t *__restrict__ counters, unsigned long long *__restrict__ stats,
-                                               unsigned long long *__restrict__ totals)
+                                               unsigned long long *__restrict__ totals,
+                                               const Material *__restrict__ mats,
+                                               const uint32_t *__restrict__ em_claim, uint32_t *__restrict__ em_bits)
 {
     const uint32_t live = *live_ptr;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -576,6 +591,9 @@ __global__ __launch_bounds__(256) void k_trace(const uint32_t *__restrict__ word
                 r.pad0 = (float)h.size;
                 r.hit_vid = voxel_key(h.px, h.py, h.pz) + 1u;
                 r.hit_material = h.material;
+                if (iter == 0u && mats[h.material < MATERIAL_SLOTS ? h.material : MATERIAL_SLOTS - 1u].emissive &&
+                    emissive_unknown(em_claim, r.hit_vid))
+                    atomicOr(&em_bits[t >> 5], 1u << (t & 31u));
             } else {
                 r.hit_vid = 0u;
                 r.hit_material = 0u;
@@ -700,13 +718,424 @@ __device__ f3 lookup_normal(const uint32_t *__restrict__ nbuf, uint32_t hw, uint
     return f3{0.0f, 0.0f, -gsign(rd.z)};
 }
 
-__global__ __launch_bounds__(256) void k_shade(FrameParams p, const Ray *__restrict__ rays_in,
-                                               const uint32_t *__restrict__ live_in, Ray *__restrict__ rays_out,
-                                               uint32_t *__restrict__ live_out, uint32_t *__restrict__ lbuf,
-                                               const uint32_t *__restrict__ nbuf, const Material *__restrict__ mats,
-                                               EmissiveVoxel *__restrict__ emissives, uint32_t *__restrict__ em_claim,
-                                               uint32_t *__restrict__ counters)
+// Emissive registry of the first shade (shade.comp:592-619) in the
+// reference's order: rays in index order register the ids k_trace flagged
+// (claim probe, append while below capacity).  One block; the bitmap is
+// scanned 1024 words at a time and cleared as it goes; a chunk with set bits
+// is walked by one thread.  Records the count before this frame's
+// registrations and the ray index of each registration, so that shade can
+// give every SIMD-8 vector of rays the registry size it would have seen.
+__global__ __launch_bounds__(1024) void k_em_register(const Ray *__restrict__ rays, const uint32_t *__restrict__ live_ptr,
+                                                      const Material *__restrict__ mats, uint32_t *__restrict__ em_claim,
+                                                      EmissiveVoxel *__restrict__ emissives, uint32_t *__restrict__ em_bits,
+                                                      uint32_t *__restrict__ em_reg, uint32_t *__restrict__ counters)
 {
+    __shared__ uint32_t chunk[1024];
+    __shared__ uint32_t count, nreg;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t live = *live_ptr;
+    const uint32_t nw = (live + 31u) / 32u;
+    if (tid == 0) {
+        count = counters[VE_CNT_EMISSIVES];
+        nreg = 0u;
+        counters[VE_CNT_EM_BASE] = count;
+    }
+    __syncthreads();
+    for (uint32_t base = 0; base < nw; base += 1024u) {
+        const uint32_t i = base + tid;
+        const uint32_t w = i < nw ? em_bits[i] : 0u;
+        if (w) em_bits[i] = 0u;
+        chunk[tid] = w;
+        if (__syncthreads_or(w != 0u) && tid == 0) {
+            for (uint32_t k = 0; k < 1024u && base + k < nw; k++) {
+                for (uint32_t m = chunk[k]; m; m &= m - 1u) {
+                    const uint32_t ri = (base + k) * 32u + (uint32_t)(__ffs(m) - 1);
+                    const Ray &r = rays[ri];
+                    const uint32_t vid = r.hit_vid;
+                    const uint32_t ch = claim_hash(vid) % EMISSIVE_CLAIM_W;
+                    for (uint32_t q = 0; q < EMISSIVE_CLAIM_PROBES; q++) {
+                        uint32_t &c = em_claim[(size_t)ch * EMISSIVE_CLAIM_PROBES + q];
+                        if (c == vid) break;
+                        if (c != 0u) continue;
+                        c = vid;
+                        if (count < EMISSIVE_CAPACITY) {
+                            const Material &m8 = mats[r.hit_material < MATERIAL_SLOTS ? r.hit_material : MATERIAL_SLOTS - 1u];
+                            const f3 ee = muls(f3{m8.color[0], m8.color[1], m8.color[2]}, m8.emissive_intensity);
+                            const uint32_t hs = (uint32_t)r.pad0;
+                            emissives[count] = EmissiveVoxel{ee.x, ee.y, ee.z, vid, (uint32_t)r.ox, (uint32_t)r.oy,
+                                                             (uint32_t)r.oz, hs > 1u ? hs : 1u};
+                            count++;
+                            em_reg[nreg++] = ri;
+                        }
+                        break;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        counters[VE_CNT_EMISSIVES] = count;
+        counters[VE_CNT_EM_NEW] = nreg;
+    }
+}
+
+// ---- next-event estimation (shade.comp:276-539, 620-836) -------------------
+struct Reservoir {
+    uint32_t light_idx, light_vid, face_seed;
+    float wsum;
+    uint32_t m;
+    float target_pdf;
+};
+
+__device__ __forceinline__ void r_update(Reservoir &r, uint32_t idx, uint32_t lvid, uint32_t seed, float target,
+                                         float w, uint32_t &rng)   // shade.comp:320-336
+{
+    if (w <= 0.0f) {
+        r.m += 1u;
+        return;
+    }
+    r.wsum += w;
+    r.m += 1u;
+    if (rand_lds(rng) * r.wsum < w) {
+        r.light_idx = idx;
+        r.light_vid = lvid;
+        r.face_seed = seed;
+        r.target_pdf = target;
+    }
+}
+
+__device__ __forceinline__ void r_combine(Reservoir &a, const Reservoir &b, float t_us, uint32_t &rng)   // :352-365
+{
+    if (b.m == 0u || b.target_pdf <= 0.0f || t_us <= 0.0f) return;
+    const float w = t_us * b.wsum / b.target_pdf;
+    a.wsum += w;
+    a.m += b.m;
+    if (rand_lds(rng) * a.wsum < w) {
+        a.light_idx = b.light_idx;
+        a.light_vid = b.light_vid;
+        a.face_seed = b.face_seed;
+        a.target_pdf = t_us;
+    }
+}
+
+__device__ __forceinline__ void r_cap(Reservoir &r, uint32_t max_m)   // shade.comp:372-377
+{
+    if (r.m > max_m && max_m > 0u) {
+        r.wsum = r.wsum * (float)max_m / (float)r.m;
+        r.m = max_m;
+    }
+}
+
+__device__ __forceinline__ uint32_t pack_face_seed(uint32_t face, float u, float v)   // shade.comp:382-386
+{
+    const uint32_t ui = (uint32_t)(gclamp(u, 0.0f, 0.9999f) * 16383.0f);
+    const uint32_t vi = (uint32_t)(gclamp(v, 0.0f, 0.9999f) * 16383.0f);
+    return (face & 0xFu) | ((ui & 0x3FFFu) << 4) | ((vi & 0x3FFFu) << 18);
+}
+
+__device__ __forceinline__ void face_frame(uint32_t face, f3 &n, f3 &t, f3 &b)   // shade.comp:398-413
+{
+    if (face < 2u) {
+        n = f3{face == 0u ? 1.0f : -1.0f, 0.0f, 0.0f};
+        t = f3{0.0f, 1.0f, 0.0f};
+        b = f3{0.0f, 0.0f, 1.0f};
+    } else if (face < 4u) {
+        n = f3{0.0f, face == 2u ? 1.0f : -1.0f, 0.0f};
+        t = f3{1.0f, 0.0f, 0.0f};
+        b = f3{0.0f, 0.0f, 1.0f};
+    } else {
+        n = f3{0.0f, 0.0f, face == 4u ? 1.0f : -1.0f};
+        t = f3{1.0f, 0.0f, 0.0f};
+        b = f3{0.0f, 1.0f, 0.0f};
+    }
+}
+
+// fCenter + fNormal*h + ((u-0.5)*2*h)*fT + ((v-0.5)*2*h)*fB, summed left to right
+__device__ __forceinline__ f3 face_point(const EmissiveVoxel &e, f3 fn, f3 ft, f3 fb, float u, float v)
+{
+    const float half = (float)e.size * 0.5f;
+    const f3 c = add(f3{(float)e.px, (float)e.py, (float)e.pz}, f3{half, half, half});
+    return add(add(add(c, muls(fn, half)), muls(ft, (u - 0.5f) * 2.0f * half)), muls(fb, (v - 0.5f) * 2.0f * half));
+}
+
+__device__ __forceinline__ void light_sample_point(uint32_t seed, const EmissiveVoxel &e, f3 &P, f3 &N)   // :393-420
+{
+    uint32_t face = seed & 0xFu;
+    const float u = (float)((seed >> 4) & 0x3FFFu) / 16383.0f, v = (float)((seed >> 18) & 0x3FFFu) / 16383.0f;
+    if (face > 5u) face = 5u;
+    f3 ft, fb;
+    face_frame(face, N, ft, fb);
+    P = face_point(e, N, ft, fb, u, v);
+}
+
+__device__ __forceinline__ float luminance_floor(const EmissiveVoxel &e)
+{
+    return gmax(dot(f3{e.ex, e.ey, e.ez}, f3{0.2126f, 0.7152f, 0.0722f}), 1e-6f);
+}
+
+__device__ float target_pdf(const EmissiveVoxel &e, uint32_t seed, f3 sp, f3 nrm)   // shade.comp:423-438
+{
+    f3 P, N;
+    light_sample_point(seed, e, P, N);
+    const f3 to = sub(P, sp);
+    const float d2 = dot(to, to);
+    if (d2 < 1e-6f) return 0.0f;
+    const float d = sqrtf(d2);
+    const f3 ld = divs(to, d);
+    const float cx = dot(nrm, ld);
+    if (cx <= 0.0f) return 0.0f;
+    const float cy = dot(f3{-ld.x, -ld.y, -ld.z}, N);
+    if (cy <= 0.0f) return 0.0f;
+    const float area = 6.0f * (float)(e.size * e.size);
+    return luminance_floor(e) * cy * area / d2;
+}
+
+// rBufferLoad (shade.comp:466-485) on the device bucket layout
+__device__ bool rbuf_load(const FrameParams &p, const uint32_t *__restrict__ rbuf, uint32_t vid, Reservoir &r)
+{
+    r = Reservoir{0u, 0u, 0u, 0.0f, 0u, 0.0f};
+    const uint32_t *bucket = rbuf + (size_t)(vid % (p.hash_width - 1u) + 1u) * RB_WORDS;
+    uint32_t own;
+    const uint32_t s = find_owner<false>(bucket, 0, RBUFFER_SLOTS, vid, own);
+    if (s >= RBUFFER_SLOTS) return false;
+    const uint32_t *pl = bucket + RB_PAY + s * RB_PAYW;
+    r.light_idx = pl[0];
+    r.light_vid = pl[1];
+    r.face_seed = pl[2];
+    r.wsum = __uint_as_float(pl[3]);
+    r.m = pl[4];
+    r.target_pdf = __uint_as_float(pl[5]);
+    const uint32_t t = bucket[RB_TIME + s];
+    if (!p.disable_stale_reset && t != 0u && t + p.update_time < p.time) {
+        r = Reservoir{0u, 0u, 0u, 0.0f, 0u, 0.0f};
+        return false;
+    }
+    return true;
+}
+
+// rBufferStore (shade.comp:501-519): claim (or keep) a slot, else evict the
+// least recently written one
+__device__ void rbuf_store(const FrameParams &p, uint32_t *__restrict__ rbuf, uint32_t vid, const Reservoir &r)
+{
+    uint32_t *bucket = rbuf + (size_t)(vid % (p.hash_width - 1u) + 1u) * RB_WORDS;
+    bool claimed;
+    uint32_t s = claim_slot(bucket, RBUFFER_SLOTS, vid, claimed);
+    if (s >= RBUFFER_SLOTS) {
+        uint32_t oldest = 0xFFFFFFFFu;
+        s = 0;
+        for (uint32_t q = 0; q < RBUFFER_SLOTS; q++) {
+            const uint32_t t = __atomic_load_n(&bucket[RB_TIME + q], __ATOMIC_RELAXED);
+            if (t < oldest) {
+                oldest = t;
+                s = q;
+            }
+        }
+        atomicExch(&bucket[s], vid);
+    }
+    uint32_t *pl = bucket + RB_PAY + s * RB_PAYW;
+    pl[0] = r.light_idx;
+    pl[1] = r.light_vid;
+    pl[2] = r.face_seed;
+    pl[3] = __float_as_uint(r.wsum);
+    pl[4] = r.m;
+    pl[5] = __float_as_uint(r.target_pdf);
+    bucket[RB_TIME + s] = p.time;
+}
+
+__device__ __forceinline__ bool light_valid(const Reservoir &r, const EmissiveVoxel *__restrict__ em, uint32_t count)
+{
+    if (r.m == 0u || r.light_idx >= count) return false;   // shade.comp:446-449
+    return em[r.light_idx].vid == r.light_vid;
+}
+
+__device__ __forceinline__ uint32_t neighbour_vid(uint32_t hx, uint32_t hy, uint32_t hz, uint32_t size, int ox, int oy,
+                                                  int oz)   // shade.comp:527-535
+{
+    const int qx = (int)hx + ox * (int)size, qy = (int)hy + oy * (int)size, qz = (int)hz + oz * (int)size;
+    if (qx < 0 || qy < 0 || qz < 0) return 0u;
+    const uint32_t m = ~(size - 1u);
+    return voxel_key(((uint32_t)qx & m) & 0x3FFu, ((uint32_t)qy & m) & 0x3FFu, ((uint32_t)qz & m) & 0x3FFu) + 1u;
+}
+
+// Context of one shade invocation's next-event estimation.
+struct NeeCtx {
+    const uint32_t *words;
+    const EmissiveVoxel *em;
+    uint32_t *rbuf;
+    uint32_t *cache;        // this thread's LDS path-cache column
+    uint32_t shadow_rays;   // RaycastVisibilityToVoxel calls
+};
+
+// RaycastVisibilityToVoxel (internal.glsl:406-410) with the bounce check budget
+__device__ bool visible_to(const FrameParams &p, NeeCtx &c, f3 o, f3 d, uint32_t target)
+{
+    uint32_t checks = 0, reads = 0;
+    c.shadow_rays++;
+    const Hit h = raycast<false>(c.words, p.n_words, p.depth, o, d, rcp3(d), p.bounce_cc, checks, reads, c.cache, 256u);
+    return h.hit && voxel_key(h.px, h.py, h.pz) + 1u == target;
+}
+
+__device__ void nee_restir(const FrameParams &p, NeeCtx &c, Ray &r, const Material &m, f3 nrm, uint32_t hx,
+                           uint32_t hy, uint32_t hz, uint32_t hsize, uint32_t n_lights, uint32_t count)   // :646-815
+{
+    const f3 sp = add(add(f3{(float)hx, (float)hy, (float)hz}, f3{0.5f, 0.5f, 0.5f}), nrm);
+    Reservoir res{0u, 0u, 0u, 0.0f, 0u, 0.0f};
+    const uint32_t pvid = r.primary_vid;
+    const int nm = p.restir_m > 1 ? p.restir_m : 1;
+    const float inv_src = (float)n_lights * 6.0f;
+    for (int k = 0; k < nm; k++) {
+        uint32_t pick = (uint32_t)(rand_lds(r.rng) * (float)n_lights);
+        if (pick >= n_lights) pick = n_lights - 1u;
+        const EmissiveVoxel cand = c.em[pick];
+        uint32_t face = (uint32_t)(rand_lds(r.rng) * 6.0f);
+        if (face >= 6u) face = 5u;
+        const float uu = rand_lds(r.rng);
+        const float vv = rand_lds(r.rng);
+        const uint32_t seed = pack_face_seed(face, uu, vv);
+        const float tg = target_pdf(cand, seed, sp, nrm);
+        r_update(res, pick, cand.vid, seed, tg, tg * inv_src, r.rng);
+    }
+    if (p.restir_temporal) {
+        Reservoir tr;
+        if (rbuf_load(p, c.rbuf, pvid, tr) && light_valid(tr, c.em, count)) {
+            r_cap(tr, (uint32_t)p.restir_max_m);
+            r_combine(res, tr, tr.target_pdf, r.rng);
+        }
+    }
+    if (p.restir_spatial && p.restir_neighbors > 0) {
+        const f3 up = fabsf(nrm.z) < 0.999f ? f3{0.0f, 0.0f, 1.0f} : f3{1.0f, 0.0f, 0.0f};
+        const f3 T = normalize(cross3(up, nrm));
+        const f3 B = cross3(nrm, T);
+        const int K = p.restir_neighbors < 6 ? p.restir_neighbors : 6;
+        for (int k = 0; k < K; k++) {
+            const float ang = 6.2831853f * rand_lds(r.rng);
+            const float rr = p.restir_radius * sqrtf(rand_lds(r.rng));
+            const f3 off = muls(add(muls(T, m_cos(ang)), muls(B, m_sin(ang))), rr);
+            const int lx = (int)rintf(off.x), ly = (int)rintf(off.y), lz = (int)rintf(off.z);
+            if (lx == 0 && ly == 0 && lz == 0) continue;
+            const uint32_t nv = neighbour_vid(hx, hy, hz, hsize, lx, ly, lz);
+            if (nv == 0u || nv == pvid) continue;
+            Reservoir nb;
+            if (!rbuf_load(p, c.rbuf, nv, nb) || !light_valid(nb, c.em, count)) continue;
+            const float t_us = target_pdf(c.em[nb.light_idx], nb.face_seed, sp, nrm);
+            if (t_us <= 0.0f) continue;
+            r_cap(nb, (uint32_t)p.restir_max_m);
+            r_combine(res, nb, t_us, r.rng);
+        }
+    }
+    if (res.m > 0u && res.target_pdf > 0.0f) {
+        const EmissiveVoxel e = c.em[res.light_idx];
+        if (e.vid == res.light_vid) {
+            f3 P, N;
+            light_sample_point(res.face_seed, e, P, N);
+            const f3 to = sub(P, sp);
+            const float d2 = dot(to, to);
+            const float d = sqrtf(gmax(d2, 1e-12f));
+            if (d > 1e-3f) {
+                const f3 ld = divs(to, d);
+                const float cx = gmax(dot(nrm, ld), 0.0f);
+                const float cy = gmax(dot(f3{-ld.x, -ld.y, -ld.z}, N), 0.0f);
+                if (cx > 0.0f && cy > 0.0f && visible_to(p, c, sp, ld, res.light_vid)) {
+                    const f3 bc = evaluate_brdf(f3{r.dx, r.dy, r.dz}, ld, nrm, f3{m.color[0], m.color[1], m.color[2]},
+                                                m.roughness, m.metallic, m.specular);
+                    const float W = res.wsum / ((float)res.m * res.target_pdf);
+                    const float sz2 = (float)(e.size * e.size);
+                    const float geom = cy / gmax(d2, 1e-6f);
+                    f3 ct = muls(muls(muls(mul(f3{e.ex, e.ey, e.ez}, bc), geom), sz2), W);
+                    ct = vmin3(ct, f3{20.0f, 20.0f, 20.0f});
+                    const f3 w = mul(f3{r.tx, r.ty, r.tz}, ct);
+                    r.dlx += w.x;
+                    r.dly += w.y;
+                    r.dlz += w.z;
+                }
+            }
+        }
+    }
+    if (res.m > 0u) rbuf_store(p, c.rbuf, pvid, res);
+}
+
+__device__ void nee_ris(const FrameParams &p, NeeCtx &c, Ray &r, const Material &m, f3 nrm, uint32_t hx, uint32_t hy,
+                        uint32_t hz, uint32_t n_lights)   // shade.comp:816-836
+{
+    const f3 sp = add(add(f3{(float)hx, (float)hy, (float)hz}, f3{0.5f, 0.5f, 0.5f}), nrm);
+    float wsum = 0.0f, chosen_t = 0.0f;
+    uint32_t chosen = 0u;
+    for (int k = 0; k < 4; k++) {
+        uint32_t pick = (uint32_t)(rand_lds(r.rng) * (float)n_lights);
+        if (pick >= n_lights) pick = n_lights - 1u;
+        const EmissiveVoxel cand = c.em[pick];
+        const float tg = luminance_floor(cand) * (float)(cand.size * cand.size);
+        wsum += tg;
+        if (rand_lds(r.rng) * wsum < tg) {
+            chosen = pick;
+            chosen_t = tg;
+        }
+    }
+    const float avg = wsum / 4.0f;
+    const EmissiveVoxel e = c.em[chosen];
+    uint32_t face = (uint32_t)(rand_lds(r.rng) * 6.0f);
+    if (face >= 6u) face = 5u;
+    f3 fn, ft, fb;
+    face_frame(face, fn, ft, fb);
+    const float ru = rand_lds(r.rng);
+    const float rv = rand_lds(r.rng);
+    f3 ld = sub(face_point(e, fn, ft, fb, ru, rv), sp);
+    const float d2 = dot(ld, ld);
+    const float d = sqrtf(d2);
+    if (d > 1e-3f && chosen_t > 0.0f) {
+        ld = divs(ld, d);
+        const float cx = gmax(dot(nrm, ld), 0.0f);
+        const float cy = gmax(dot(f3{-ld.x, -ld.y, -ld.z}, fn), 0.0f);
+        if (cx > 0.0f && cy > 0.0f && visible_to(p, c, sp, ld, e.vid)) {
+            const f3 bc = evaluate_brdf(f3{r.dx, r.dy, r.dz}, ld, nrm, f3{m.color[0], m.color[1], m.color[2]},
+                                        m.roughness, m.metallic, m.specular);
+            const float area = 6.0f * (float)e.size * (float)e.size;
+            const float ris = (float)n_lights * avg / gmax(chosen_t, 1e-6f);
+            const float geom = cy / gmax(d2, 1e-6f);
+            f3 ct = muls(muls(muls(mul(f3{e.ex, e.ey, e.ez}, bc), geom), area), ris);
+            ct = vmin3(ct, f3{20.0f, 20.0f, 20.0f});
+            const f3 w = mul(f3{r.tx, r.ty, r.tz}, ct);
+            if (r.bounce == 0u) {
+                r.dlx += w.x;
+                r.dly += w.y;
+                r.dlz += w.z;
+            } else {
+                r.ilx += w.x;
+                r.ily += w.y;
+                r.ilz += w.z;
+            }
+        }
+    }
+}
+
+// Registry size seen by the SIMD-8 vector of ray `t` in the first shade: the
+// count before this frame's registrations plus those made by rays of this or
+// an earlier vector (em_reg is ascending).
+__device__ __forceinline__ uint32_t registry_seen(const uint32_t *__restrict__ counters,
+                                                  const uint32_t *__restrict__ em_reg, uint32_t t)
+{
+    const uint32_t last = t | 7u;
+    uint32_t lo = 0, hi = counters[VE_CNT_EM_NEW];
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (em_reg[mid] <= last) lo = mid + 1u;
+        else hi = mid;
+    }
+    return counters[VE_CNT_EM_BASE] + lo;
+}
+
+__global__ __launch_bounds__(256) void k_shade(FrameParams p, const uint32_t *__restrict__ words, uint32_t iter,
+                                               const Ray *__restrict__ rays_in, const uint32_t *__restrict__ live_in,
+                                               Ray *__restrict__ rays_out, uint32_t *__restrict__ live_out,
+                                               uint32_t *__restrict__ lbuf, const uint32_t *__restrict__ nbuf,
+                                               uint32_t *__restrict__ rbuf, const Material *__restrict__ mats,
+                                               const EmissiveVoxel *__restrict__ emissives,
+                                               const uint32_t *__restrict__ em_reg, uint32_t *__restrict__ counters,
+                                               unsigned long long *__restrict__ totals)
+{
+    __shared__ uint32_t path_cache[VE_MAX_DEPTH * 256];
+    NeeCtx nee{words, emissives, rbuf, path_cache + threadIdx.x, 0u};
     const uint32_t live = *live_in;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t MB = p.max_bounces;
@@ -728,32 +1157,20 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, const Ray *__restr
                 const f3 rd{r.dx, r.dy, r.dz};
                 const f3 nrm = lookup_normal(nbuf, p.hash_width, r.hit_vid, rd);
                 const f3 albedo{m.color[0], m.color[1], m.color[2]};
-                if (m.emissive) {   // shade.comp:576-619
+                if (m.emissive) {   // shade.comp:576-590 (registration: k_em_register)
                     const f3 em = vmin3(mul(muls(albedo, m.emissive_intensity), f3{r.tx, r.ty, r.tz}), f3{20, 20, 20});
                     if (r.bounce == 0u) { r.dlx += em.x; r.dly += em.y; r.dlz += em.z; }
                     else { r.ilx += em.x; r.ily += em.y; r.ilz += em.z; }
-                    if (r.bounce == 0u) {
-                        const uint32_t vid = r.hit_vid;
-                        const uint32_t ch = claim_hash(vid) % EMISSIVE_CLAIM_W;
-                        for (uint32_t q = 0; q < EMISSIVE_CLAIM_PROBES; q++) {
-                            const uint32_t prev = atomicCAS(&em_claim[(size_t)ch * EMISSIVE_CLAIM_PROBES + q], 0u, vid);
-                            if (prev == vid) break;
-                            if (prev == 0u) {
-                                const uint32_t idx = atomicAdd(&counters[VE_CNT_EMISSIVES], 1u);
-                                if (idx < EMISSIVE_CAPACITY) {
-                                    const f3 ee = muls(albedo, m.emissive_intensity);
-                                    EmissiveVoxel ev{ee.x, ee.y, ee.z, vid, hx, hy, hz, hsize > 1u ? hsize : 1u};
-                                    emissives[idx] = ev;
-                                } else {
-                                    atomicSub(&counters[VE_CNT_EMISSIVES], 1u);
-                                }
-                                break;
-                            }
-                        }
-                    }
                 }
-                if (!m.emissive && r.bounce < 2u && __atomic_load_n(&counters[VE_CNT_EMISSIVES], __ATOMIC_RELAXED) > 0u)
-                    atomicOr(&counters[VE_CNT_NEE_PENDING], 1u);
+                // next-event estimation (shade.comp:631-836)
+                const uint32_t count = iter == 0u ? registry_seen(counters, em_reg, t) : counters[VE_CNT_EMISSIVES];
+                if (!m.emissive && r.bounce < 2u && count > 0u) {
+                    const uint32_t n_lights = count < EMISSIVE_CAPACITY ? count : EMISSIVE_CAPACITY;
+                    if (p.restir_enabled && r.bounce == 0u)
+                        nee_restir(p, nee, r, m, nrm, hx, hy, hz, hsize, n_lights, count);
+                    else
+                        nee_ris(p, nee, r, m, nrm, hx, hy, hz, n_lights);
+                }
                 if (r.bounce + 1u >= MB) {   // last allowed bounce
                     deposit(p, lbuf, r.primary_vid, f3{r.dlx, r.dly, r.dlz}, f3{r.ilx, r.ily, r.ilz});
                 } else {
@@ -800,6 +1217,14 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, const Ray *__restr
                 }
             }
         }
+    }
+    // shadow rays of the wave: one atomic each for the frame counter and the
+    // cumulative total
+    unsigned long long sr = nee.shadow_rays;
+    for (int off = 32; off > 0; off >>= 1) sr += __shfl_xor(sr, off, 64);
+    if ((threadIdx.x & 63u) == 0u && sr) {
+        atomicAdd(&counters[VE_CNT_SHADOW_RAYS], (uint32_t)sr);
+        atomicAdd(&totals[2], sr);
     }
     // stream compaction of survivors: wave ballot + one atomic per wave
     const uint64_t mask = __ballot(survive);
@@ -1147,19 +1572,23 @@ void launch_trace(hipStream_t s, const uint32_t *words, const FrameParams &p, De
     const uint32_t nb = cdiv(p.shade_budget, 256);
     if (stats)
         hipLaunchKernelGGL(k_trace<true>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, iter), live_in_ptr(b, iter),
-                           iter, b.counters, b.node_stats, b.ray_totals);
+                           iter, b.counters, b.node_stats, b.ray_totals, b.materials, b.emissive_claim, b.em_bits);
     else
         hipLaunchKernelGGL(k_trace<false>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, iter),
-                           live_in_ptr(b, iter), iter, b.counters, b.node_stats, b.ray_totals);
+                           live_in_ptr(b, iter), iter, b.counters, b.node_stats, b.ray_totals, b.materials,
+                           b.emissive_claim, b.em_bits);
+    if (iter == 0)
+        hipLaunchKernelGGL(k_em_register, dim3(1), dim3(1024), 0, s, bounce_in(b, iter), live_in_ptr(b, iter),
+                           b.materials, b.emissive_claim, b.emissives, b.em_bits, b.em_reg, b.counters);
 }
 
-void launch_shade(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_t iter)
+void launch_shade(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t iter)
 {
     const uint32_t nb = cdiv(p.shade_budget, 256);
     launch_fill(s, live_out_ptr(b, iter), 0u, 1);
-    hipLaunchKernelGGL(k_shade, dim3(nb), dim3(256), 0, s, p, bounce_in(b, iter), live_in_ptr(b, iter),
-                       bounce_out(b, iter), live_out_ptr(b, iter), b.lbuf, b.nbuf, b.materials, b.emissives,
-                       b.emissive_claim, b.counters);
+    hipLaunchKernelGGL(k_shade, dim3(nb), dim3(256), 0, s, p, words, iter, bounce_in(b, iter), live_in_ptr(b, iter),
+                       bounce_out(b, iter), live_out_ptr(b, iter), b.lbuf, b.nbuf, b.rbuf, b.materials, b.emissives,
+                       b.em_reg, b.counters, b.ray_totals);
 }
 
 void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b)

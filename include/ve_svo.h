@@ -165,6 +165,8 @@ typedef enum ve_pass_id {
     VE_PASS_TRACE,         /* trace.comp over the live rays */
     VE_PASS_SHADE,         /* shade.comp over the live rays */
     VE_PASS_RESOLVE,       /* resolve.comp */
+    VE_PASS_DISPLAY,       /* final.frag: ACES tonemap of the resolve output into RGBA8 (VE_BUF_DISPLAY);
+                              Renderer::run's blit (renderer.cpp:683-704), not part of ve_frame */
     VE_PASS_COUNT
 } ve_pass_id;
 
@@ -187,6 +189,7 @@ typedef enum ve_buffer_id {
     VE_BUF_SAMPLE_COUNTS,  /* uint32 [16384][64] per-voxel pixel counts (claim-map layout) */
     VE_BUF_CLAIM_MAP,      /* uint32 [16384][64] */
     VE_BUF_EMISSIVES,      /* 32-byte EmissiveVoxel records */
+    VE_BUF_DISPLAY,        /* uint8 x4 per pixel: tonemapped RGBA8 of the last display pass */
     VE_BUF_COUNT
 } ve_buffer_id;
 
@@ -235,6 +238,46 @@ int ve_reset_ray_totals(ve_ctx *ctx);
 int ve_enable_node_stats(ve_ctx *ctx, int enable);
 int ve_node_stats(ve_ctx *ctx, uint64_t *primary_reads, uint64_t *trace_reads,
                   uint64_t *primary_steps, uint64_t *trace_steps);
+
+/* ---- editable octree: OctreeCPU + Octree::set / applyEdits -------------------
+ * The persistent pointer tree of OctreeCPU (src/octree_cpu.hpp:9-60) with the
+ * node-word image Octree keeps for it (src/renderer/octree.hpp).  Inserts and
+ * removes mark paths dirty; ve_octree_set linearises everything
+ * (Octree::set, octree.cpp:127-217); ve_octree_apply_edits rewrites only the
+ * dirty paths, reusing freed 8-word blocks last-freed-first
+ * (Octree::applyEdits, octree.cpp:228-315), and lists the slots it wrote so
+ * ve_scene_update can carry just those words to the device.  Host memory only;
+ * not thread-safe. */
+typedef struct ve_octree_cpu ve_octree_cpu;
+
+int  ve_octree_cpu_create(ve_octree_cpu **out, uint32_t depth, uint32_t max_words);   /* max_words 0 = 2^27 */
+void ve_octree_cpu_destroy(ve_octree_cpu *t);
+/* OctreeCPU::insert per entry, in order (octree_cpu.cpp:40-64) */
+int  ve_octree_cpu_insert(ve_octree_cpu *t, const ve_insert *ops, size_t n);
+/* OctreeCPU::remove per (x,y,z) triple (octree_cpu.cpp:66-98) */
+int  ve_octree_cpu_remove(ve_octree_cpu *t, const uint32_t *xyz, size_t n);
+/* OctreeCPU::lookup (octree_cpu.cpp:100-111): *present = 1 with the leaf's material */
+int  ve_octree_cpu_lookup(const ve_octree_cpu *t, uint32_t x, uint32_t y, uint32_t z, int *present,
+                          uint32_t *material);
+/* OctreeCPU::raycast (octree_cpu.cpp:128-215), the picking ray of the main loop */
+int  ve_octree_cpu_raycast(const ve_octree_cpu *t, const float origin[3], const float direction[3],
+                           uint32_t max_steps, int *hit, uint32_t position[3], uint32_t *material);
+int  ve_octree_set(ve_octree_cpu *t, ve_octree_info *info);
+int  ve_octree_apply_edits(ve_octree_cpu *t, ve_octree_info *info);
+/* The node-word image (capacity words; info.size of them in use). */
+int  ve_octree_words(const ve_octree_cpu *t, const uint32_t **words, uint32_t *capacity);
+/* Slots written by the last set / apply_edits; *full = 1 when the whole image
+ * must be uploaded again (a set, or a capacity resize). */
+int  ve_octree_changed(const ve_octree_cpu *t, const uint32_t **slots, size_t *n, int *full);
+/* Octree::findGpuSlot (octree.cpp:322-341): word slot of the leaf at a position, or 0xFFFFFFFF. */
+uint32_t ve_octree_find_slot(const ve_octree_cpu *t, uint32_t x, uint32_t y, uint32_t z);
+/* Bring a device node array up to date after ve_octree_apply_edits: `words`
+ * is the new image (n_words in use) and `slots` the words that changed since
+ * the scene was uploaded or last updated (words appended since then and not
+ * listed are zero).  Grows the device array when n_words exceeds it.  Ordered
+ * on the context's stream after the frames already enqueued there. */
+int  ve_scene_update(ve_ctx *ctx, ve_scene *scene, const uint32_t *words, uint32_t n_words,
+                     const uint32_t *slots, size_t n_slots);
 
 /* ---- serving: resident scenes, sessions, continuous batching ----------------
  * No single counterpart in the reference, whose one Renderer renders one view

@@ -32,8 +32,27 @@ def lib():
         L.so_voxel_key.argtypes = [p]
         L.so_generate_primary.argtypes = [p, u32, u32, p, u32, u32, u32, p, p, C.POINTER(C.c_uint64),
                                           C.POINTER(C.c_uint64), C.c_int]
+        L.so_raycast.restype = SoHit
+        L.so_raycast.argtypes = [p, u32, u32, p, p, p, u32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
+
+
+class SoHit(C.Structure):
+    _fields_ = [("hit", C.c_uint32), ("id", C.c_uint32), ("material", C.c_uint32), ("pos", C.c_uint32 * 3),
+                ("size", C.c_uint32)]
+
+
+def raycast(words, depth, origin, direction, controlchecks, n_words=None):
+    """Raycast (internal.glsl:297-400) on node words: (hit, (x, y, z), material, checks)."""
+    w = np.ascontiguousarray(words, np.uint32)
+    o = np.ascontiguousarray(origin, np.float32)
+    d = np.ascontiguousarray(direction, np.float32)
+    inv = (np.float32(1.0) / d).astype(np.float32)
+    checks = C.c_uint32()
+    h = lib().so_raycast(w.ctypes.data, len(w) if n_words is None else n_words, depth, o.ctypes.data, d.ctypes.data,
+                         inv.ctypes.data, controlchecks, C.byref(checks), None)
+    return bool(h.hit), tuple(h.pos), h.material, checks.value
 
 
 def octree_build(ops, depth, max_words=1 << 27):

@@ -19,7 +19,7 @@ int main()
     octree.set(scene);
     uint64_t sum = 0;
     for (uint32_t i = 0; i < octree.size; i++) sum = sum * 1000003ull + octree.data[i];
-    std::printf("%zu %u %u %u %llu\n", scene.ops.size(), octree.size, octree.capacity, octree.numVoxels,
+    std::printf("%zu %u %u %u %llu\n", scene.inserts, octree.size, octree.capacity, octree.numVoxels,
                 (unsigned long long)sum);
     const ve_frame_config fc = ve::default_frame_config();
     std::printf("%d %d\n", fc.primary_controlchecks, fc.shading_budget);

@@ -21,15 +21,35 @@ def test_cpp_host_api(ve, scenes, tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
     n_ops, size, cap, nvox, dig = (int(v) for v in out[0].split())
     assert out[1].split() == ["70", "131072"]
-    # the same calls through the Python binding
+    # the same insert calls through the batch builder (sort-based, no pointer
+    # tree): an independent implementation of Octree::set must agree
     L = 64
-    cpu = ve.OctreeCPU(6)
-    cpu.insertSphere((L / 3.0, L / 3.0 - 5.0, L / 2.0), L / 4.0, 5)
-    cpu.insertBox((0, 0, 0), (L, 4, L), 4)
-    cpu.insertFunction(lambda p: (p[:, 1] > 50.0) & (p[:, 0] < 10.0), (0, 0, 0), (L, L, L), 2)
-    o = ve.Octree(6).set(cpu)
-    assert (len(cpu.ops()), o.size, o.capacity, o.numVoxels) == (n_ops, size, cap, nvox)
+    ops = np.concatenate([
+        scenes.sphere_ops((L / 3.0, L / 3.0 - 5.0, L / 2.0), L / 4.0, 5, L),
+        _box_ops((0, 0, 0), (L, 4, L), 4),
+        _fn_ops(lambda p: (p[:, 1] > 50.0) & (p[:, 0] < 10.0), (0, 0, 0), (L, L, L), 2),
+    ])
+    words, info = ve.octree_build(ops, 6)
+    assert (len(ops), info.size, info.capacity, info.num_voxels) == (n_ops, size, cap, nvox)
     s = 0
-    for w in o.data[: o.size].tolist():
+    for w in words[: info.size].tolist():
         s = (s * 1000003 + w) & 0xFFFFFFFFFFFFFFFF
     assert s == dig
+
+
+def _box_ops(lo, hi, m):
+    from voxelengine_amd.scenes import INSERT_DTYPE
+    X, Y, Z = np.meshgrid(*[np.arange(lo[i], hi[i]) for i in range(3)], indexing="ij")
+    o = np.empty(X.size, INSERT_DTYPE)
+    o["x"], o["y"], o["z"], o["material"] = X.ravel(), Y.ravel(), Z.ravel(), m
+    return o
+
+
+def _fn_ops(fn, lo, hi, m):
+    from voxelengine_amd.scenes import INSERT_DTYPE
+    X, Y, Z = np.meshgrid(*[np.arange(lo[i], hi[i]) for i in range(3)], indexing="ij")
+    c = np.stack([X.ravel() + 0.5, Y.ravel() + 0.5, Z.ravel() + 0.5], 1).astype(np.float32)
+    sel = np.asarray(fn(c), bool)
+    o = np.empty(int(sel.sum()), INSERT_DTYPE)
+    o["x"], o["y"], o["z"], o["material"] = X.ravel()[sel], Y.ravel()[sel], Z.ravel()[sel], m
+    return o

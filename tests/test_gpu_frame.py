@@ -155,3 +155,50 @@ def test_timeout_reconstruction(ve, scenes, oracle, small_scene, radius):
     finally:
         rend.close()
         of.close()
+
+
+def aces_rgba8(rgba32f):
+    """final.frag:11-19 in float32, in the shader's operation order, then the
+    unorm8 conversion (round to nearest, ties to even)."""
+    x = np.asarray(rgba32f, np.float32).reshape(-1, 4)[:, :3]
+    f = np.float32
+    num = x * (f(2.51) * x + f(0.03))
+    den = x * (f(2.43) * x + f(0.59)) + f(0.14)
+    v = np.clip(num / den, f(0.0), f(1.0)).astype(np.float32)
+    out = np.empty((len(x), 4), np.uint8)
+    out[:, :3] = np.rint(v * f(255.0)).astype(np.uint8)
+    out[:, 3] = 255
+    return out
+
+
+def test_display_pass_is_aces(ve, scenes, small_scene):
+    """The display pass (Renderer::run's final blit) maps the resolve output
+    through final.frag's ACES curve into RGBA8, pixel for pixel; also over a
+    synthetic HDR image written into the output buffer."""
+    W, H = 96, 64
+    rend = ve.Renderer(W, H, max_bounces=2)
+    try:
+        rend.set_materials(scenes.palette(1))
+        rend.set_sky_constant((140, 180, 230))
+        e, d, fov = scenes.camera_small()
+        rend.set_camera(ve.camera_ubo(e, d, fov, W / H))
+        sc = rend.upload(small_scene["words"], small_scene["depth"], small_scene["info"].size)
+        fc = ve.FrameConfig(**DET_KNOBS)
+        inp = ve.FrameInputs(W, H, 0, frame_seed(0), 1)
+        rend.frame(sc, fc, inp)
+        rend.run_pass(sc, "DISPLAY", fc, inp)
+        out = rend.read("OUTPUT", np.float32, W * H * 16)
+        disp = rend.read("DISPLAY", np.uint8, W * H * 4).reshape(-1, 4)
+        assert np.array_equal(disp, aces_rgba8(out))
+        rng = np.random.default_rng(3)
+        hdr = (rng.random((W * H, 4), np.float32) * 8.0).astype(np.float32)
+        hdr[:16, :3] = [0.0, 1e-6, 1e4]
+        full = np.zeros(rend.size("OUTPUT") // 4, np.float32)
+        full[: W * H * 4] = hdr.ravel()
+        rend.write("OUTPUT", full)
+        rend.run_pass(sc, "DISPLAY", fc, inp)
+        disp = rend.read("DISPLAY", np.uint8, W * H * 4).reshape(-1, 4)
+        assert np.array_equal(disp, aces_rgba8(hdr))
+        rend.release(sc)
+    finally:
+        rend.close()

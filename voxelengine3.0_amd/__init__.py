@@ -30,10 +30,11 @@ VE_ESTATE = -6
 VE_EBUDGET = -7
 VE_ENOTFOUND = -8
 
-PASS = dict(PRIMARY=0, NORMAL=1, RESET=2, SCHEDULE=3, RANK=4, THRESHOLD=5, EMIT=6, TRACE=7, SHADE=8, RESOLVE=9)
+PASS = dict(PRIMARY=0, NORMAL=1, RESET=2, SCHEDULE=3, RANK=4, THRESHOLD=5, EMIT=6, TRACE=7, SHADE=8, RESOLVE=9,
+            DISPLAY=10)
 PASS_NAMES = [k for k, _ in sorted(PASS.items(), key=lambda kv: kv[1])]
 BUF = dict(GBUFFER=0, GBUFFER_POS=1, NBUFFER=2, LBUFFER=3, RBUFFER=4, CANDIDATES=5, SHADE_LIST=6, RAYS=7,
-           OUTPUT=8, COUNTERS=9, RANK_STATS=10, SAMPLE_COUNTS=11, CLAIM_MAP=12, EMISSIVES=13)
+           OUTPUT=8, COUNTERS=9, RANK_STATS=10, SAMPLE_COUNTS=11, CLAIM_MAP=12, EMISSIVES=13, DISPLAY=14)
 CNT = dict(CANDIDATES=0, SHADE=1, EMISSIVES=2, LIVE=3, RAYS_TRACED=4, SHADOW_RAYS=5, PRIMARY_RAYS=6,
            BOUNCES_RUN=7, LIVE_B=8, OVERFLOW=9, TIMEOUTS=11, EM_BASE=12, EM_NEW=13)
 
@@ -149,6 +150,18 @@ def lib():
         "ve_node_stats": (i32, [p] + [C.POINTER(C.c_uint64)] * 4),
         "ve_ray_totals": (i32, [p] + [C.POINTER(C.c_uint64)] * 3),
         "ve_reset_ray_totals": (i32, [p]),
+        "ve_octree_cpu_create": (i32, [C.POINTER(p), u32, u32]),
+        "ve_octree_cpu_destroy": (None, [p]),
+        "ve_octree_cpu_insert": (i32, [p, p, sz]),
+        "ve_octree_cpu_remove": (i32, [p, p, sz]),
+        "ve_octree_cpu_lookup": (i32, [p, u32, u32, u32, C.POINTER(C.c_int), C.POINTER(u32)]),
+        "ve_octree_cpu_raycast": (i32, [p, p, p, u32, C.POINTER(C.c_int), p, C.POINTER(u32)]),
+        "ve_octree_set": (i32, [p, C.POINTER(OctreeInfo)]),
+        "ve_octree_apply_edits": (i32, [p, C.POINTER(OctreeInfo)]),
+        "ve_octree_words": (i32, [p, C.POINTER(C.POINTER(u32)), C.POINTER(u32)]),
+        "ve_octree_changed": (i32, [p, C.POINTER(C.POINTER(u32)), C.POINTER(sz), C.POINTER(C.c_int)]),
+        "ve_octree_find_slot": (u32, [p, u32, u32, u32]),
+        "ve_scene_update": (i32, [p, p, p, u32, p, sz]),
         "ve_server_create": (i32, [C.POINTER(p), C.POINTER(ServerConfig)]),
         "ve_server_destroy": (None, [p]),
         "ve_server_last_error": (C.c_char_p, [p]),
@@ -202,18 +215,57 @@ def octree_build(ops, depth, max_words=1 << 27):
 
 
 class OctreeCPU:
-    """Host octree edit API (src/octree_cpu.hpp:30-37): records insert calls
-    in order; Octree.set() builds the node array from them."""
+    """Host octree edit API (src/octree_cpu.hpp:9-60): a persistent pointer
+    tree (ve_octree_cpu) with insert / remove / lookup / raycast and the
+    insertBox / insertSphere / insertFunction helpers.  Octree.set() and
+    Octree.applyEdits() turn it into node words.  (ve.octree_build is the
+    batch path for whole scenes given as insert lists.)"""
 
-    def __init__(self, depth):
+    def __init__(self, depth, max_words=1 << 27):
         self.depth = int(depth)
-        self._chunks = []
+        h = C.c_void_p()
+        _check(lib().ve_octree_cpu_create(C.byref(h), self.depth, max_words))
+        self.h = h
 
-    def insert(self, x, y, z, material):
-        self._chunks.append(np.array([(x, y, z, material)], dtype=_insert_dtype()))
+    def close(self):
+        if self.h:
+            lib().ve_octree_cpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def insert_ops(self, ops):
-        self._chunks.append(np.ascontiguousarray(ops, dtype=_insert_dtype()))
+        ops = np.ascontiguousarray(ops, dtype=_insert_dtype())
+        _check(lib().ve_octree_cpu_insert(self.h, ops.ctypes.data if len(ops) else None, len(ops)))
+
+    def insert(self, x, y, z, material):
+        self.insert_ops(np.array([(x, y, z, material)], dtype=_insert_dtype()))
+
+    def remove(self, x, y, z):
+        self.remove_many(np.array([[x, y, z]], np.uint32))
+
+    def remove_many(self, xyz):
+        xyz = np.ascontiguousarray(np.asarray(xyz, np.uint32).reshape(-1, 3))
+        _check(lib().ve_octree_cpu_remove(self.h, xyz.ctypes.data if len(xyz) else None, len(xyz)))
+
+    def lookup(self, x, y, z):
+        """(present, material) of the leaf at a position (OctreeCPU::lookup)."""
+        pr, m = C.c_int(), C.c_uint32()
+        _check(lib().ve_octree_cpu_lookup(self.h, x, y, z, C.byref(pr), C.byref(m)))
+        return bool(pr.value), m.value
+
+    def raycast(self, origin, direction, max_steps=300):
+        """OctreeCPU::raycast: (hit, (x, y, z), material)."""
+        o = np.ascontiguousarray(origin, np.float32)
+        d = np.ascontiguousarray(direction, np.float32)
+        hit, pos, m = C.c_int(), (C.c_uint32 * 3)(), C.c_uint32()
+        _check(lib().ve_octree_cpu_raycast(self.h, o.ctypes.data, d.ctypes.data, max_steps, C.byref(hit), pos,
+                                           C.byref(m)))
+        return bool(hit.value), tuple(pos), m.value
 
     def insertBox(self, lo, hi, material):
         n = 1 << self.depth
@@ -221,11 +273,11 @@ class OctreeCPU:
         X, Y, Z = np.meshgrid(*[np.arange(int(lo[i]), hi[i]) for i in range(3)], indexing="ij")
         ops = np.empty(X.size, _insert_dtype())
         ops["x"], ops["y"], ops["z"], ops["material"] = X.ravel(), Y.ravel(), Z.ravel(), material
-        self._chunks.append(ops)
+        self.insert_ops(ops)
 
     def insertSphere(self, centre, radius, material):
         from .scenes import sphere_ops
-        self._chunks.append(sphere_ops(centre, radius, material, 1 << self.depth))
+        self.insert_ops(sphere_ops(centre, radius, material, 1 << self.depth))
 
     def insertFunction(self, fn, lo, hi, material):
         n = 1 << self.depth
@@ -235,12 +287,7 @@ class OctreeCPU:
         m = np.asarray(fn(c), bool)
         ops = np.empty(int(m.sum()), _insert_dtype())
         ops["x"], ops["y"], ops["z"], ops["material"] = X.ravel()[m], Y.ravel()[m], Z.ravel()[m], material
-        self._chunks.append(ops)
-
-    def ops(self):
-        if not self._chunks:
-            return np.zeros(0, _insert_dtype())
-        return np.concatenate(self._chunks)
+        self.insert_ops(ops)
 
 
 def _insert_dtype():
@@ -249,18 +296,41 @@ def _insert_dtype():
 
 
 class Octree:
-    """Flat node array (src/renderer/octree.hpp:14-98)."""
+    """Node-word image of an OctreeCPU (src/renderer/octree.hpp:14-98):
+    set() linearises the whole tree, applyEdits() syncs only what changed and
+    reports the written slots (for Renderer.update)."""
 
     def __init__(self, depth, max_words=1 << 27):
         self.depth = int(depth)
         self.max_words = max_words
         self.data = np.zeros(8, np.uint32)
         self.size, self.capacity, self.numVoxels = 8, 8, 0
+        self.changed, self.full = np.zeros(0, np.uint32), True
 
-    def set(self, cpu):
-        self.data, info = octree_build(cpu.ops(), self.depth, self.max_words)
+    def _pull(self, cpu, info):
+        w, cap = C.POINTER(C.c_uint32)(), C.c_uint32()
+        _check(lib().ve_octree_words(cpu.h, C.byref(w), C.byref(cap)))
+        self.data = np.ctypeslib.as_array(w, shape=(cap.value,)).copy()
+        sl, n, full = C.POINTER(C.c_uint32)(), C.c_size_t(), C.c_int()
+        _check(lib().ve_octree_changed(cpu.h, C.byref(sl), C.byref(n), C.byref(full)))
+        self.changed = np.ctypeslib.as_array(sl, shape=(n.value,)).copy() if n.value else np.zeros(0, np.uint32)
+        self.full = bool(full.value)
         self.size, self.capacity, self.numVoxels = info.size, info.capacity, info.num_voxels
         return self
+
+    def set(self, cpu):
+        info = OctreeInfo()
+        _check(lib().ve_octree_set(cpu.h, C.byref(info)))
+        return self._pull(cpu, info)
+
+    def applyEdits(self, cpu):
+        info = OctreeInfo()
+        _check(lib().ve_octree_apply_edits(cpu.h, C.byref(info)))
+        return self._pull(cpu, info)
+
+    def findGpuSlot(self, cpu, x, y, z):
+        s = lib().ve_octree_find_slot(cpu.h, x, y, z)
+        return None if s == 0xFFFFFFFF else s
 
 
 def camera_ubo(position, direction, fov_deg, aspect):
@@ -309,6 +379,16 @@ class Renderer:
 
     def release(self, scene):
         self._ck(lib().ve_scene_release(self.h, scene))
+
+    def update(self, scene, octree):
+        """ve_scene_update after Octree.applyEdits: only the changed words travel."""
+        words = np.ascontiguousarray(octree.data, np.uint32)
+        if octree.full:
+            slots = np.arange(octree.size, dtype=np.uint32)
+        else:
+            slots = np.ascontiguousarray(octree.changed, np.uint32)
+        self._ck(lib().ve_scene_update(self.h, scene, words.ctypes.data, octree.size,
+                                       slots.ctypes.data if len(slots) else None, len(slots)))
 
     def set_camera(self, ubo):
         ubo = np.ascontiguousarray(ubo, np.float32)

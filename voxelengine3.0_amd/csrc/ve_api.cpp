@@ -86,7 +86,7 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
 {
     const size_t P = (size_t)max_w * max_h;
     const size_t groups_max = (size_t)((max_w + 7) / 8) * ((max_h + 7) / 8);
-    a(&b.gbuf, P); a(&b.gpos, P); a(&b.output, P); a(&b.pix_avg, P); a(&b.pix_cnt, P); a(&b.timeouts, P);
+    a(&b.gbuf, P); a(&b.gpos, P); a(&b.output, P); a(&b.display, P); a(&b.pix_avg, P); a(&b.pix_cnt, P); a(&b.timeouts, P);
     a(&b.nbuf, (size_t)hash_width * NBUFFER_STRIDE * NBUFFER_SLOTS);
     a(&b.lbuf, (size_t)hash_width * LBUFFER_STRIDE * LBUFFER_SLOTS);
     a(&b.rbuf, (size_t)hash_width * RBUFFER_STRIDE * RBUFFER_SLOTS);
@@ -143,6 +143,7 @@ BufView buffer_view(ve_ctx *c, ve_buffer_id id)
     case VE_BUF_SAMPLE_COUNTS: return {c->b.samples, (size_t)CLAIM_WIDTH * CLAIM_PROBES * 4};
     case VE_BUF_CLAIM_MAP: return {c->b.claim, (size_t)CLAIM_WIDTH * CLAIM_PROBES * 4};
     case VE_BUF_EMISSIVES: return {c->b.emissives, (size_t)EMISSIVE_CAPACITY * sizeof(EmissiveVoxel)};
+    case VE_BUF_DISPLAY: return {c->b.display, P * 4};
     default: return {nullptr, 0};
     }
 }
@@ -344,6 +345,64 @@ int ve_scene_upload(ve_ctx *c, const uint32_t *words, uint32_t n_words, uint32_t
     return rc == VE_OK ? VE_OK : ctx_error(c, rc, "%s", err.c_str());
 }
 
+int ve_scene_update(ve_ctx *c, ve_scene *s, const uint32_t *words, uint32_t n_words, const uint32_t *slots,
+                    size_t n_slots)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    if (!s || !words || n_words == 0 || (n_slots && !slots)) return ctx_error(c, VE_EINVAL, "null argument");
+    if (n_words < s->n_words) return ctx_error(c, VE_EINVAL, "node array shrank from %u to %u words", s->n_words, n_words);
+    HIP_OK(c, hipSetDevice(c->device));
+    if (n_words > s->alloc_words) {   // grow: a new array with headroom, whole image copied
+        for (uint32_t i = 0; i < n_words; i++)
+            if ((words[i] & 1u) && (uint64_t)(words[i] >> 4) + 8u > n_words)
+                return ctx_error(c, VE_EFORMAT, "node %u points to block %u outside the %u-word array", i,
+                                 words[i] >> 4, n_words);
+        const uint32_t alloc = (uint32_t)std::min<uint64_t>((uint64_t)n_words + n_words / 2 + 8, 0xFFFFFFF0ull);
+        void *d = nullptr;
+        hipError_t e = hipMalloc(&d, (size_t)alloc * 4);
+        if (e != hipSuccess) return ctx_error(c, VE_ENOMEM, "hipMalloc node array: %s", hipGetErrorString(e));
+        e = hipMemsetAsync(static_cast<uint32_t *>(d) + n_words, 0, (size_t)(alloc - n_words) * 4, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(d, words, (size_t)n_words * 4, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(d);
+            return ctx_error(c, VE_EDEVICE, "node upload: %s", hipGetErrorString(e));
+        }
+        (void)hipFree(s->words);
+        s->words = static_cast<uint32_t *>(d);
+        s->alloc_words = alloc;
+        s->n_words = n_words;
+        s->device_bytes = (size_t)alloc * 4;
+        return VE_OK;
+    }
+    std::vector<uint2> pairs;
+    pairs.reserve(n_slots);
+    for (size_t k = 0; k < n_slots; k++) {
+        const uint32_t i = slots[k], w = i < n_words ? words[i] : 0u;
+        if (i >= n_words) return ctx_error(c, VE_EINVAL, "changed slot %u beyond the %u-word array", i, n_words);
+        if ((w & 1u) && (uint64_t)(w >> 4) + 8u > n_words)
+            return ctx_error(c, VE_EFORMAT, "node %u points to block %u outside the %u-word array", i, w >> 4, n_words);
+        pairs.push_back(make_uint2(i, w));
+    }
+    if (!pairs.empty()) {
+        if (pairs.size() > s->staging_cap) {
+            if (s->staging) (void)hipFree(s->staging);
+            s->staging = nullptr;
+            s->staging_cap = 0;
+            const size_t cap = std::max<size_t>(pairs.size(), 4096);
+            HIP_OK(c, hipMalloc(&s->staging, cap * sizeof(uint2)));
+            s->staging_cap = cap;
+        }
+        HIP_OK(c, hipMemcpyAsync(s->staging, pairs.data(), pairs.size() * sizeof(uint2), hipMemcpyHostToDevice,
+                                 c->stream));
+        launch_scatter_words(c->stream, s->words, s->staging, (uint32_t)pairs.size());
+        HIP_OK(c, hipGetLastError());
+        HIP_OK(c, hipStreamSynchronize(c->stream));
+    }
+    s->n_words = n_words;
+    return VE_OK;
+}
+
 int ve_scene_release(ve_ctx *c, ve_scene *s)
 {
     if (!c) return ve_global_error(VE_EINVAL, "null context");
@@ -422,6 +481,7 @@ int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p)
         c->bounce_iter++;
         break;
     case VE_PASS_RESOLVE: launch_resolve(c->stream, p, c->b); break;
+    case VE_PASS_DISPLAY: launch_display(c->stream, p, c->b); break;
     default: return ctx_error(c, VE_EINVAL, "unknown pass %d", (int)pass);
     }
     if (rc) return ctx_error(c, rc, "pass %d: device library call failed", (int)pass);
@@ -576,6 +636,7 @@ int ve_timings(ve_ctx *c, float ms_out[VE_PASS_COUNT])
     double sum[VE_PASS_COUNT] = {0};
     for (size_t f = 0; f < c->frames_recorded; f++)
         for (int i = 0; i < VE_PASS_COUNT; i++) {
+            if (i == VE_PASS_DISPLAY) continue;   // not part of ve_frame: no events recorded
             const uint32_t n_it = (i == VE_PASS_TRACE || i == VE_PASS_SHADE) ? c->iters_recorded : 1u;
             for (uint32_t it = 0; it < n_it; it++) {
                 const hipEvent_t *pair = &c->frame_events[f][2 * (i * MAX_ITERS + it)];
@@ -753,6 +814,7 @@ int scene_upload(int device, hipStream_t stream, const uint32_t *words, uint32_t
     }
     s->words = static_cast<uint32_t *>(d);
     s->n_words = n_words;
+    s->alloc_words = n_words;
     s->depth = depth;
     s->device_bytes = (size_t)n_words * 4;
     *scene = s;
@@ -763,6 +825,7 @@ void scene_free(ve_scene *s)
 {
     if (!s) return;
     (void)hipFree(s->words);
+    if (s->staging) (void)hipFree(s->staging);
     delete s;
 }
 

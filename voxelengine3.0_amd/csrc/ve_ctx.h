@@ -18,8 +18,11 @@
 struct ve_scene {
     uint32_t *words = nullptr;
     uint32_t n_words = 0;
+    uint32_t alloc_words = 0;          // words allocated (>= n_words)
     uint32_t depth = 0;
     size_t device_bytes = 0;
+    uint2 *staging = nullptr;          // (slot, word) pairs of ve_scene_update
+    size_t staging_cap = 0;
 };
 
 constexpr int MAX_ITERS = 8;   // host bounce-loop cap (renderer.cpp:515)

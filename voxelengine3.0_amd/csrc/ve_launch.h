@@ -13,6 +13,7 @@ struct DeviceBuffers {
     uint4 *gbuf = nullptr;          // P x (checks, material, 0, vid)
     uint32_t *gpos = nullptr;       // P
     float4 *output = nullptr;       // P
+    uchar4 *display = nullptr;      // P: tonemapped RGBA8
     float4 *pix_avg = nullptr;      // P: per-pixel voxel average (rgb, indirect variance)
     uint32_t *pix_cnt = nullptr;    // P: per-pixel voxel sample count
     uint32_t *timeouts = nullptr;   // P: timed-out pixels (resolve)
@@ -59,6 +60,8 @@ void launch_trace(hipStream_t s, const uint32_t *words, const FrameParams &p, De
                   bool stats);
 void launch_shade(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t iter);
 void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
+void launch_display(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
+void launch_scatter_words(hipStream_t s, uint32_t *words, const uint2 *pairs, uint32_t n);
 size_t scan_temp_bytes(uint32_t max_items);
 
 }  // namespace ve

@@ -1456,6 +1456,23 @@ __global__ __launch_bounds__(256) void k_resolve_timeout(FrameParams p, const ui
     }
 }
 
+// RenderPass (final.frag:11-19): ACES filmic curve per channel, clamped to
+// [0,1], written as an RGBA8 unorm texel like the blit into an 8-bit default
+// framebuffer (v * 255 rounded to nearest, ties to even), alpha 255.
+__device__ __forceinline__ float aces(float x)
+{
+    return gclamp((x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f), 0.0f, 1.0f);
+}
+
+__global__ __launch_bounds__(256) void k_display(uint32_t n, const float4 *__restrict__ in, uchar4 *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 c = in[i];
+    out[i] = make_uchar4((unsigned char)rintf(aces(c.x) * 255.0f), (unsigned char)rintf(aces(c.y) * 255.0f),
+                         (unsigned char)rintf(aces(c.z) * 255.0f), 255);
+}
+
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
@@ -1589,6 +1606,24 @@ void launch_shade(hipStream_t s, const uint32_t *words, const FrameParams &p, De
     hipLaunchKernelGGL(k_shade, dim3(nb), dim3(256), 0, s, p, words, iter, bounce_in(b, iter), live_in_ptr(b, iter),
                        bounce_out(b, iter), live_out_ptr(b, iter), b.lbuf, b.nbuf, b.rbuf, b.materials, b.emissives,
                        b.em_reg, b.counters, b.ray_totals);
+}
+
+// Partial node-array update (Octree::UpdateNode's glBufferSubData per word).
+__global__ void k_scatter_words(uint32_t *__restrict__ words, const uint2 *__restrict__ pairs, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) words[pairs[i].x] = pairs[i].y;
+}
+
+void launch_scatter_words(hipStream_t s, uint32_t *words, const uint2 *pairs, uint32_t n)
+{
+    if (n) hipLaunchKernelGGL(k_scatter_words, dim3(cdiv(n, 256)), dim3(256), 0, s, words, pairs, n);
+}
+
+void launch_display(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
+{
+    const uint32_t n = p.W * p.H;
+    hipLaunchKernelGGL(k_display, dim3(cdiv(n, 256)), dim3(256), 0, s, n, b.output, b.display);
 }
 
 void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b)

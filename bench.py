@@ -15,7 +15,13 @@ algorithmic node bytes per launch (4 bytes per node word the reference's
 traversal reads, counted by the instrumented kernel on the same frames) over
 its mean launch time from HIP events recorded on the context's stream during
 the timed region.  cpu_baseline: the C oracle (a scalar port of the same
-passes) on the host, one frame of the same workload.
+passes) on the host, whole frames of the same workload.
+
+`served` and `batched` (skipped with --no-served): the served form (Poisson
+sessions over resident octrees on the session server, p50/p99 frame latency,
+frames/s and Mrays/s per offered load, concurrent server vs the plain
+one-frame-at-a-time loop) and the batched form (64 uneven viewports, frame k
+of each submitted together); see voxelengine3.0_amd/serving.py.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """

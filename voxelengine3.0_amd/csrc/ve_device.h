@@ -96,6 +96,8 @@ struct DDA {
     uint32_t q;
     uint32_t px, py, pz;   // previous lattice position
     uint32_t pstop;        // depth of the word that ended the previous descent
+    uint32_t mx, my, mz;   // all ones where the axis' inverse direction is positive (far face at pos + size)
+    float ex, ey, ez;      // the far face's nudge: +0.001 there, -0.001 on the other faces
 };
 
 enum : int { DDA_CONTINUE = 0, DDA_MISS = 1, DDA_HIT = 2 };
@@ -109,6 +111,12 @@ __device__ __forceinline__ int dda_start(DDA &s, f3 o, f3 d, f3 inv, uint32_t de
     s.q = 0;
     s.px = s.py = s.pz = 0xFFFFFFFFu;
     s.pstop = 0;
+    s.mx = inv.x > 0.0f ? 0xFFFFFFFFu : 0u;
+    s.my = inv.y > 0.0f ? 0xFFFFFFFFu : 0u;
+    s.mz = inv.z > 0.0f ? 0xFFFFFFFFu : 0u;
+    s.ex = inv.x > 0.0f ? 0.001f : -0.001f;
+    s.ey = inv.y > 0.0f ? 0.001f : -0.001f;
+    s.ez = inv.z > 0.0f ? 0.001f : -0.001f;
     s.o = add(o, muls(d, 4.0f));   // internal.glsl:361
     if (in_bounds(s.o, fl)) {
         s.r = s.o;
@@ -190,13 +198,17 @@ __device__ __forceinline__ int dda_step(DDA &s, const uint32_t *__restrict__ wor
     s.pz = uz;
     const uint32_t tsize = len >> dd;
     const uint32_t tpx = ux & ~(tsize - 1u), tpy = uy & ~(tsize - 1u), tpz = uz & ~(tsize - 1u);
-    // intersect_inside(ray, pos, pos + size), internal.glsl:255-261
+    // intersect_inside(ray, pos, pos + size), internal.glsl:255-261:
+    // tx = min over axes of max(t1, t2) with t1 = ((pos - o) - 0.001) * inv and
+    // t2 = (((pos + size) - o) + 0.001) * inv.  The second numerator exceeds the
+    // first, so the max is t2 where inv > 0 (+inf included) and t1 where inv < 0;
+    // neither can be NaN (for a zero direction component the cell contains the
+    // origin, so the chosen numerator is non-zero), and pos + size is exact as
+    // an integer.  Only the far face is evaluated.
     const f3 o = s.o, inv = s.inv;
-    const float bx = (float)tpx, by = (float)tpy, bz = (float)tpz, fs = (float)tsize;
-    const float t1x = ((bx - o.x) - 0.001f) * inv.x, t2x = (((bx + fs) - o.x) + 0.001f) * inv.x;
-    const float t1y = ((by - o.y) - 0.001f) * inv.y, t2y = (((by + fs) - o.y) + 0.001f) * inv.y;
-    const float t1z = ((bz - o.z) - 0.001f) * inv.z, t2z = (((bz + fs) - o.z) + 0.001f) * inv.z;
-    const float tx = tmin(tmin(tmax(t1x, t2x), tmax(t1y, t2y)), tmax(t1z, t2z));
+    const float fx = (float)(tpx + (tsize & s.mx)), fy = (float)(tpy + (tsize & s.my)), fz = (float)(tpz + (tsize & s.mz));
+    const float ax = ((fx - o.x) + s.ex) * inv.x, ay = ((fy - o.y) + s.ey) * inv.y, az = ((fz - o.z) + s.ez) * inv.z;
+    const float tx = tmin(tmin(ax, ay), az);
     s.r = add(muls(s.d, tx), o);
     return DDA_CONTINUE;
 }

@@ -71,7 +71,7 @@ __device__ __forceinline__ uint32_t locate(uint32_t x, uint32_t y, uint32_t z, u
 
 __device__ __forceinline__ bool in_bounds(f3 v, float n)
 {
-    return 0.0f <= v.x && v.x <= n && 0.0f <= v.y && v.y <= n && 0.0f <= v.z && v.z <= n;
+    return (0.0f <= v.x) & (v.x <= n) & (0.0f <= v.y) & (v.y <= n) & (0.0f <= v.z) & (v.z <= n);
 }
 
 struct Hit {
@@ -90,12 +90,10 @@ __device__ __forceinline__ f3 primary_dir(const float *cam, uint32_t x, uint32_t
     return normalize(d);
 }
 
-// Traversal state of one ray between DDA steps.
+// Ray state after the setup of internal.glsl:352-366.
 struct DDA {
     f3 o, d, inv, r;
     uint32_t q;
-    uint32_t px, py, pz;   // previous lattice position
-    uint32_t pstop;        // depth of the word that ended the previous descent
     uint32_t mx, my, mz;   // all ones where the axis' inverse direction is positive (far face at pos + size)
     float ex, ey, ez;      // the far face's nudge: +0.001 there, -0.001 on the other faces
 };
@@ -109,8 +107,6 @@ __device__ __forceinline__ int dda_start(DDA &s, f3 o, f3 d, f3 inv, uint32_t de
     s.d = d;
     s.inv = inv;
     s.q = 0;
-    s.px = s.py = s.pz = 0xFFFFFFFFu;
-    s.pstop = 0;
     s.mx = inv.x > 0.0f ? 0xFFFFFFFFu : 0u;
     s.my = inv.y > 0.0f ? 0xFFFFFFFFu : 0u;
     s.mz = inv.z > 0.0f ? 0xFFFFFFFFu : 0u;
@@ -138,7 +134,8 @@ __device__ __forceinline__ int dda_start(DDA &s, f3 o, f3 d, f3 inv, uint32_t de
     return DDA_CONTINUE;
 }
 
-// One iteration of Raycast's while loop (internal.glsl:370-398).
+// RaycastCounted / Raycast (internal.glsl:297-400): the while loop of
+// internal.glsl:370-398 after dda_start.  `checks` receives q.
 //
 // Every DDA step of the reference descends from the root.  The words read at
 // depth dd depend only on coordinate bits D..D-dd (locate() uses bit D-dd and
@@ -150,80 +147,78 @@ __device__ __forceinline__ int dda_start(DDA &s, f3 o, f3 d, f3 inv, uint32_t de
 // outcome outright when it ended above that depth.  Results are identical to
 // the full descent; only the repeated loads are skipped.  `reads` still
 // counts the words the reference's full descent reads (its algorithmic
-// traffic, 4 bytes each).
-template <bool STATS>
-__device__ __forceinline__ int dda_step(DDA &s, const uint32_t *__restrict__ words, uint32_t n_words, uint32_t depth,
-                                        uint32_t controlchecks, uint32_t &checks, uint32_t &reads, Hit &hit,
-                                        uint32_t *cache, uint32_t stride)
-{
-    const uint32_t len = 1u << depth;
-    if (!(in_bounds(s.r, (float)len) && s.q++ <= controlchecks)) {
-        checks = s.q;
-        return DDA_MISS;
-    }
-    const uint32_t ux = (uint32_t)s.r.x, uy = (uint32_t)s.r.y, uz = (uint32_t)s.r.z;
-    const uint32_t diff = (ux ^ s.px) | (uy ^ s.py) | (uz ^ s.pz);
-    // first depth whose word may differ: D - (highest differing bit); the
-    // first step (no previous position) starts at the root
-    const uint32_t hb = 31u - (uint32_t)__clz(diff);
-    const uint32_t resume = diff == 0u ? 0xFFFFFFFFu : (hb >= depth ? 0u : depth - hb);
-    uint32_t dd;
-    if (resume > s.pstop) {
-        dd = s.pstop;   // same words down to the one that ended the last descent
-    } else {
-        uint32_t offset = resume == 0u ? 0u : cache[(resume - 1u) * stride];
-        for (dd = resume; dd < depth; dd++) {
-            offset += locate(ux, uy, uz, len >> dd);
-            const uint32_t raw = offset < n_words ? words[offset] : 0u;
-            if (!(raw & 1u)) break;
-            offset = raw >> 4;
-            cache[dd * stride] = offset;
-        }
-        if (dd == depth) {   // bottom level: the leaf slot
-            offset += locate(ux, uy, uz, 1u);
-            const uint32_t raw = offset < n_words ? words[offset] : 0u;
-            const uint32_t mat = (raw >> 1) & 0x7Fu;
-            if (mat != 0u) {
-                if (STATS) reads += depth + 1u;
-                checks = s.q;
-                hit = Hit{1u, offset, mat, ux, uy, uz, 1u};
-                return DDA_HIT;
-            }
-        }
-        s.pstop = dd;
-    }
-    if (STATS) reads += dd + 1u;   // the reference re-reads depths 0..dd every step
-    s.px = ux;
-    s.py = uy;
-    s.pz = uz;
-    const uint32_t tsize = len >> dd;
-    const uint32_t tpx = ux & ~(tsize - 1u), tpy = uy & ~(tsize - 1u), tpz = uz & ~(tsize - 1u);
-    // intersect_inside(ray, pos, pos + size), internal.glsl:255-261:
-    // tx = min over axes of max(t1, t2) with t1 = ((pos - o) - 0.001) * inv and
-    // t2 = (((pos + size) - o) + 0.001) * inv.  The second numerator exceeds the
-    // first, so the max is t2 where inv > 0 (+inf included) and t1 where inv < 0;
-    // neither can be NaN (for a zero direction component the cell contains the
-    // origin, so the chosen numerator is non-zero), and pos + size is exact as
-    // an integer.  Only the far face is evaluated.
-    const f3 o = s.o, inv = s.inv;
-    const float fx = (float)(tpx + (tsize & s.mx)), fy = (float)(tpy + (tsize & s.my)), fz = (float)(tpz + (tsize & s.mz));
-    const float ax = ((fx - o.x) + s.ex) * inv.x, ay = ((fy - o.y) + s.ey) * inv.y, az = ((fz - o.z) + s.ez) * inv.z;
-    const float tx = tmin(tmin(ax, ay), az);
-    s.r = add(muls(s.d, tx), o);
-    return DDA_CONTINUE;
-}
-
-// RaycastCounted / Raycast (internal.glsl:297-400): `checks` receives q.
+// traffic, 4 bytes each).  The loop carries only the traversal state; the hit
+// record is formed at the exit.
 template <bool STATS>
 __device__ __forceinline__ Hit raycast(const uint32_t *__restrict__ words, uint32_t n_words, uint32_t depth, f3 o,
                                        f3 d, f3 inv, uint32_t controlchecks, uint32_t &checks, uint32_t &reads,
                                        uint32_t *cache, uint32_t stride)
 {
-    Hit hit{0, 0, 0, 0, 0, 0, 0};
     DDA s;
-    int st = dda_start(s, o, d, inv, depth, checks);
-    while (st == DDA_CONTINUE) st = dda_step<STATS>(s, words, n_words, depth, controlchecks, checks, reads, hit, cache, stride);
-    return hit;
+    if (dda_start(s, o, d, inv, depth, checks) != DDA_CONTINUE) return Hit{0, 0, 0, 0, 0, 0, 0};
+    const uint32_t len = 1u << depth;
+    const float fl = (float)len;
+    const f3 oo = s.o;
+    f3 r = s.r;
+    uint32_t q = s.q, px = 0xFFFFFFFFu, py = 0xFFFFFFFFu, pz = 0xFFFFFFFFu, pstop = 0;
+    for (;;) {
+        // inBounds(r) && q++ <= controlchecks, without the short-circuit branch
+        const bool inb = in_bounds(r, fl);
+        const bool go = inb & (q <= controlchecks);
+        q += inb ? 1u : 0u;
+        if (!go) break;
+        const uint32_t ux = (uint32_t)r.x, uy = (uint32_t)r.y, uz = (uint32_t)r.z;
+        const uint32_t diff = (ux ^ px) | (uy ^ py) | (uz ^ pz);
+        // first depth whose word may differ: D - (highest differing bit); the
+        // first step (no previous position) starts at the root
+        const uint32_t hb = 31u - (uint32_t)__clz(diff);
+        const uint32_t resume = diff == 0u ? 0xFFFFFFFFu : (hb >= depth ? 0u : depth - hb);
+        uint32_t dd;
+        if (resume > pstop) {
+            dd = pstop;   // same words down to the one that ended the last descent
+        } else {
+            uint32_t offset = resume == 0u ? 0u : cache[(resume - 1u) * stride];
+            for (dd = resume; dd < depth; dd++) {
+                offset += locate(ux, uy, uz, len >> dd);
+                const uint32_t raw = offset < n_words ? words[offset] : 0u;
+                if (!(raw & 1u)) break;
+                offset = raw >> 4;
+                cache[dd * stride] = offset;
+            }
+            if (dd == depth) {   // bottom level: the leaf slot
+                offset += locate(ux, uy, uz, 1u);
+                const uint32_t raw = offset < n_words ? words[offset] : 0u;
+                const uint32_t mat = (raw >> 1) & 0x7Fu;
+                if (mat != 0u) {
+                    if (STATS) reads += depth + 1u;
+                    checks = q;
+                    return Hit{1u, offset, mat, ux, uy, uz, 1u};
+                }
+            }
+            pstop = dd;
+        }
+        if (STATS) reads += dd + 1u;   // the reference re-reads depths 0..dd every step
+        px = ux;
+        py = uy;
+        pz = uz;
+        const uint32_t tsize = len >> dd;
+        const uint32_t tpx = ux & ~(tsize - 1u), tpy = uy & ~(tsize - 1u), tpz = uz & ~(tsize - 1u);
+        // intersect_inside(ray, pos, pos + size), internal.glsl:255-261:
+        // tx = min over axes of max(t1, t2) with t1 = ((pos - o) - 0.001) * inv and
+        // t2 = (((pos + size) - o) + 0.001) * inv.  The second numerator exceeds
+        // the first, so the max is t2 where inv > 0 (+inf included) and t1 where
+        // inv < 0; neither can be NaN (for a zero direction component the cell
+        // contains the origin, so the chosen numerator is non-zero), and
+        // pos + size is exact as an integer.  Only the far face is evaluated.
+        const float fx = (float)(tpx + (tsize & s.mx)), fy = (float)(tpy + (tsize & s.my)),
+                    fz = (float)(tpz + (tsize & s.mz));
+        const float ax = ((fx - oo.x) + s.ex) * inv.x, ay = ((fy - oo.y) + s.ey) * inv.y,
+                    az = ((fz - oo.z) + s.ez) * inv.z;
+        const float tx = tmin(tmin(ax, ay), az);
+        r = add(muls(d, tx), oo);
+    }
+    checks = q;
+    return Hit{0, 0, 0, 0, 0, 0, 0};
 }
 
 }  // namespace ve

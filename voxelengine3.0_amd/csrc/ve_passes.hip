@@ -51,6 +51,18 @@ __device__ __forceinline__ bool tile_pixel(const FrameParams &p, uint32_t &x, ui
     return x < p.W && y < p.H;
 }
 
+// Per-frame clear of the three ordering tables in one pass.
+__global__ void k_fill_sched(uint32_t *__restrict__ vid, uint32_t *__restrict__ first, uint32_t *__restrict__ cnt,
+                             size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        vid[i] = 0u;
+        first[i] = 0xFFFFFFFFu;
+        cnt[i] = 0u;
+    }
+}
+
 __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -247,10 +259,18 @@ __global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *
         pending &= ~same;
         const uint32_t h = claim_hash(lv) % CLAIM_WIDTH;
         uint32_t *row = tab_vid + (size_t)h * SCHED_SLOTS;
-        const uint32_t a = row[lane], b = row[64u + lane];
+        // rows fill from slot 0 up: the second half is read only when the
+        // first holds neither this id nor a free slot
+        const uint32_t a = row[lane];
+        uint32_t b = NONE;
+        bool b_loaded = false;
         uint32_t from = 0, slot = SCHED_SLOTS;
         while (from < SCHED_SLOTS) {
             const uint64_t ma = __ballot((a == lv || a == 0u) && lane >= from);
+            if (ma == 0ull && !b_loaded) {
+                b = row[64u + lane];
+                b_loaded = true;
+            }
             const uint64_t mb = __ballot((b == lv || b == 0u) && lane + 64u >= from);
             if (!ma && !mb) break;
             const int first = ma ? __ffsll((unsigned long long)ma) - 1 : 64 + __ffsll((unsigned long long)mb) - 1;
@@ -1529,9 +1549,7 @@ int launch_schedule(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
     const uint32_t groups = gw * gh;
     const uint32_t n_vec = groups * 8u;
     const size_t tab = (size_t)CLAIM_WIDTH * SCHED_SLOTS;
-    launch_fill(s, b.sched_vid, 0u, tab);
-    launch_fill(s, b.sched_first, NONE, tab);
-    launch_fill(s, b.sched_cnt, 0u, tab);
+    hipLaunchKernelGGL(k_fill_sched, dim3(cdiv(tab, 256)), dim3(256), 0, s, b.sched_vid, b.sched_first, b.sched_cnt, tab);
     hipLaunchKernelGGL(k_sched_claim, dim3(groups), dim3(64), 0, s, p, b.gbuf, b.sched_vid, b.sched_first, b.sched_cnt,
                        b.cand_at, b.counters);
     hipLaunchKernelGGL(k_sched_rank, dim3(CLAIM_WIDTH), dim3(64), 0, s, b.sched_vid, b.sched_first, b.sched_cnt, b.claim,

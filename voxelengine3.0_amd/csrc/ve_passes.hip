@@ -1,6 +1,6 @@
 // gfx950 kernels for the wavefront passes after the primary pass:
 //   normal.comp     -> k_normal
-//   schedule.comp   -> k_sched_clear, k_sched_claim, k_sched_rank, k_sched_vec_count, (scan), k_sched_write
+//   schedule.comp   -> k_sched_claim, k_sched_gather, k_sched_rank, k_sched_vec_count, (scan), k_sched_write
 //   rank.comp       -> k_rank, (scan), k_rank_write
 //   threshold.comp  -> k_threshold
 //   emit.comp       -> k_emit
@@ -52,17 +52,6 @@ __device__ __forceinline__ bool tile_pixel(const FrameParams &p, uint32_t &x, ui
 }
 
 // Per-frame clear of the three ordering tables in one pass.
-__global__ void k_fill_sched(uint32_t *__restrict__ vid, uint32_t *__restrict__ first, uint32_t *__restrict__ cnt,
-                             size_t n)
-{
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        vid[i] = 0u;
-        first[i] = 0xFFFFFFFFu;
-        cnt[i] = 0u;
-    }
-}
-
 __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -236,14 +225,26 @@ __device__ __forceinline__ uint32_t sched_pixel(uint32_t e, const FrameParams &p
     return x < p.W && y < p.H;
 }
 
-// One wave per work group g; lanes with equal voxel ids are merged first so
-// one claim per distinct id per wave reaches memory, and the whole wave scans
-// the id's table row (128 slots, two per lane) with one coalesced read: the
-// first slot holding the id or 0 is taken, a lost CAS race moves past it.
+// Slot of voxel id v in the open-addressed id table (a mixer unrelated to
+// claim_hash, so ids of one claim row spread over the table).
+__device__ __forceinline__ uint32_t sched_home(uint32_t v, uint32_t mask)
+{
+    v *= 0x9E3779B1u;
+    v ^= v >> 15;
+    v *= 0x85EBCA77u;
+    v ^= v >> 13;
+    return v & mask;
+}
+
+// One wave per work group g. Lanes with equal voxel ids are merged first (the
+// lowest lane of an id carries its lane count), then every distinct id of the
+// wave is entered into the id table at once: linear probing with CAS, then the
+// id's lowest execution rank (kept as its complement, so a zeroed table is
+// "none yet") and pixel count.
 __global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *__restrict__ gbuf,
-                                                    uint32_t *__restrict__ tab_vid, uint32_t *__restrict__ tab_first,
-                                                    uint32_t *__restrict__ tab_cnt, uint32_t *__restrict__ cand_at,
-                                                    uint32_t *__restrict__ counters)
+                                                    uint32_t *__restrict__ key, uint32_t *__restrict__ nfirst,
+                                                    uint32_t *__restrict__ npix, uint32_t mask,
+                                                    uint32_t *__restrict__ cand_at, uint32_t *__restrict__ counters)
 {
     const uint32_t lane = threadIdx.x;
     const uint32_t e = blockIdx.x * 64u + lane;
@@ -252,59 +253,73 @@ __global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *
     uint32_t vid = 0u;
     if (sched_pixel(e, p, pix)) vid = gbuf[pix].w;
     uint64_t pending = __ballot(vid != 0u);
+    uint32_t lanes = 0u;
     while (pending) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
-        const uint32_t lv = __shfl(vid, leader, 64);
+        const uint32_t lv = (uint32_t)__builtin_amdgcn_readlane((int)vid, leader);
         const uint64_t same = __ballot(vid == lv);
         pending &= ~same;
-        const uint32_t h = claim_hash(lv) % CLAIM_WIDTH;
-        uint32_t *row = tab_vid + (size_t)h * SCHED_SLOTS;
-        // rows fill from slot 0 up: the second half is read only when the
-        // first holds neither this id nor a free slot
-        const uint32_t a = row[lane];
-        uint32_t b = NONE;
-        bool b_loaded = false;
-        uint32_t from = 0, slot = SCHED_SLOTS;
-        while (from < SCHED_SLOTS) {
-            const uint64_t ma = __ballot((a == lv || a == 0u) && lane >= from);
-            if (ma == 0ull && !b_loaded) {
-                b = row[64u + lane];
-                b_loaded = true;
-            }
-            const uint64_t mb = __ballot((b == lv || b == 0u) && lane + 64u >= from);
-            if (!ma && !mb) break;
-            const int first = ma ? __ffsll((unsigned long long)ma) - 1 : 64 + __ffsll((unsigned long long)mb) - 1;
-            const uint32_t val = first < 64 ? __shfl(a, first, 64) : __shfl(b, first - 64, 64);
-            if (val == lv) {
-                slot = (uint32_t)first;
-                break;
-            }
-            uint32_t prev = 0u;
-            if (lane == 0u) prev = atomicCAS(&row[first], 0u, lv);
-            prev = __shfl(prev, 0, 64);
-            if (prev == 0u || prev == lv) {
-                slot = (uint32_t)first;
-                break;
-            }
-            from = (uint32_t)first + 1u;
+        if (lane == (uint32_t)leader) lanes = (uint32_t)__popcll(same);
+    }
+    if (lanes == 0u) return;
+    // at most one id per pixel and the table holds twice the pixels, so a
+    // free slot is always reached well inside the bound
+    uint32_t s = sched_home(vid, mask);
+    bool placed = false;
+    for (uint32_t probe = 0; probe <= mask; probe++) {
+        const uint32_t prev = atomicCAS(&key[s], 0u, vid);
+        if (prev == 0u || prev == vid) {
+            placed = true;
+            break;
         }
-        if (lane == 0u) {
-            if (slot < SCHED_SLOTS) {
-                // the group's lowest lane of this id has the lowest execution rank
-                atomicMin(&tab_first[(size_t)h * SCHED_SLOTS + slot], blockIdx.x * 64u + (uint32_t)leader);
-                atomicAdd(&tab_cnt[(size_t)h * SCHED_SLOTS + slot], (uint32_t)__popcll(same));
-            } else {
-                atomicOr(&counters[VE_CNT_OVERFLOW], 1u);
-            }
+        s = (s + 1u) & mask;
+    }
+    if (!placed) {
+        atomicOr(&counters[VE_CNT_OVERFLOW], 1u);
+        return;
+    }
+    atomicMax(&nfirst[s], ~e);
+    atomicAdd(&npix[s], lanes);
+}
+
+// Moves every id of the table into its claim row's member list and clears
+// the table behind it, so the next frame finds it empty.
+__global__ __launch_bounds__(256) void k_sched_gather(uint32_t *__restrict__ key, uint32_t *__restrict__ nfirst,
+                                                      uint32_t *__restrict__ npix, uint32_t n_slots,
+                                                      uint32_t *__restrict__ row_n, uint32_t *__restrict__ tab_vid,
+                                                      uint32_t *__restrict__ tab_first, uint32_t *__restrict__ tab_cnt,
+                                                      uint32_t *__restrict__ counters)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q * 4u >= n_slots) return;
+    const uint4 k4 = reinterpret_cast<const uint4 *>(key)[q];
+    if ((k4.x | k4.y | k4.z | k4.w) == 0u) return;
+    const uint32_t k[4] = {k4.x, k4.y, k4.z, k4.w};
+    for (uint32_t j = 0; j < 4; j++) {
+        if (k[j] == 0u) continue;
+        const uint32_t s = q * 4u + j;
+        const uint32_t f = ~nfirst[s], c = npix[s];
+        nfirst[s] = 0u;
+        npix[s] = 0u;
+        const uint32_t h = claim_hash(k[j]) % CLAIM_WIDTH;
+        const uint32_t at = atomicAdd(&row_n[h], 1u);
+        if (at < SCHED_SLOTS) {
+            tab_vid[(size_t)h * SCHED_SLOTS + at] = k[j];
+            tab_first[(size_t)h * SCHED_SLOTS + at] = f;
+            tab_cnt[(size_t)h * SCHED_SLOTS + at] = c;
+        } else {
+            atomicOr(&counters[VE_CNT_OVERFLOW], 1u);
         }
     }
+    reinterpret_cast<uint4 *>(key)[q] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // One wave per claim row: a voxel's probe index in the reference's claim map is
 // the number of voxels of its row claimed before it (lower execution rank).
 // Writes the reference-layout claim and pixel-count maps (all 64 probes, so
-// no separate clear is needed) and marks each kept voxel's first pixel.
-__global__ __launch_bounds__(64) void k_sched_rank(const uint32_t *__restrict__ tab_vid,
+// no separate clear is needed), marks each kept voxel's first pixel, and
+// empties the row's member list for the next frame.
+__global__ __launch_bounds__(64) void k_sched_rank(uint32_t *__restrict__ row_n, const uint32_t *__restrict__ tab_vid,
                                                    const uint32_t *__restrict__ tab_first,
                                                    const uint32_t *__restrict__ tab_cnt, uint32_t *__restrict__ claim,
                                                    uint32_t *__restrict__ samples, uint32_t *__restrict__ cand_at)
@@ -312,18 +327,18 @@ __global__ __launch_bounds__(64) void k_sched_rank(const uint32_t *__restrict__ 
     __shared__ uint32_t firsts[SCHED_SLOTS];
     const uint32_t h = blockIdx.x, lane = threadIdx.x;
     const size_t base = (size_t)h * SCHED_SLOTS;
-    uint32_t v[2], f[2], c[2];
+    const uint32_t n = min(row_n[h], SCHED_SLOTS);
+    uint32_t v[2] = {0u, 0u}, f[2] = {NONE, NONE}, c[2] = {0u, 0u};
     for (int k = 0; k < 2; k++) {
         const uint32_t s = lane + 64u * k;
-        v[k] = tab_vid[base + s];
-        f[k] = tab_first[base + s];
-        c[k] = tab_cnt[base + s];
-        firsts[s] = v[k] ? f[k] : NONE;
+        if (s < n) {
+            v[k] = tab_vid[base + s];
+            f[k] = tab_first[base + s];
+            c[k] = tab_cnt[base + s];
+        }
+        firsts[s] = f[k];
     }
     __syncthreads();
-    const uint32_t n = (uint32_t)__popcll(__ballot(v[0] != 0u)) + (uint32_t)__popcll(__ballot(v[1] != 0u));
-    // slots fill contiguously from probe 0, so [0, n) are occupied
-    uint32_t out_v = 0u, out_c = 0u;
     for (int k = 0; k < 2; k++) {
         const uint32_t s = lane + 64u * k;
         if (s >= n) continue;
@@ -335,13 +350,12 @@ __global__ __launch_bounds__(64) void k_sched_rank(const uint32_t *__restrict__ 
             cand_at[f[k]] = (rank << 26) | (h << 6) | rank;
         }
     }
-    (void)out_v;
-    (void)out_c;
     // probes past the row's population read 0, as after the per-frame clear
     for (uint32_t s = n + lane; s < CLAIM_PROBES; s += 64u) {
         claim[(size_t)h * CLAIM_PROBES + s] = 0u;
         samples[(size_t)h * CLAIM_PROBES + s] = 0u;
     }
+    if (lane == 0u) row_n[h] = 0u;
 }
 
 // candidates first-claimed in each SIMD-8 vector
@@ -1548,12 +1562,16 @@ int launch_schedule(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
     const uint32_t gw = (p.W + 7u) / 8u, gh = (p.H + 7u) / 8u;
     const uint32_t groups = gw * gh;
     const uint32_t n_vec = groups * 8u;
-    const size_t tab = (size_t)CLAIM_WIDTH * SCHED_SLOTS;
-    hipLaunchKernelGGL(k_fill_sched, dim3(cdiv(tab, 256)), dim3(256), 0, s, b.sched_vid, b.sched_first, b.sched_cnt, tab);
-    hipLaunchKernelGGL(k_sched_claim, dim3(groups), dim3(64), 0, s, p, b.gbuf, b.sched_vid, b.sched_first, b.sched_cnt,
-                       b.cand_at, b.counters);
-    hipLaunchKernelGGL(k_sched_rank, dim3(CLAIM_WIDTH), dim3(64), 0, s, b.sched_vid, b.sched_first, b.sched_cnt, b.claim,
-                       b.samples, b.cand_at);
+    // the part of the id table this frame uses: twice its pixels (the table
+    // is empty between frames, so any power-of-two part of it works)
+    uint32_t slots = 4u;
+    while (slots < 2u * groups * 64u && slots < b.sched_slots) slots <<= 1;
+    hipLaunchKernelGGL(k_sched_claim, dim3(groups), dim3(64), 0, s, p, b.gbuf, b.sched_key, b.sched_nfirst, b.sched_npix,
+                       slots - 1u, b.cand_at, b.counters);
+    hipLaunchKernelGGL(k_sched_gather, dim3(cdiv(slots / 4u, 256)), dim3(256), 0, s, b.sched_key, b.sched_nfirst,
+                       b.sched_npix, slots, b.sched_row_n, b.sched_vid, b.sched_first, b.sched_cnt, b.counters);
+    hipLaunchKernelGGL(k_sched_rank, dim3(CLAIM_WIDTH), dim3(64), 0, s, b.sched_row_n, b.sched_vid, b.sched_first,
+                       b.sched_cnt, b.claim, b.samples, b.cand_at);
     hipLaunchKernelGGL(k_sched_vec_count, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, b.cand_at, n_vec, b.vec_cnt);
     size_t tb = b.scan_temp_bytes;
     if (hipcub::DeviceScan::ExclusiveSum(b.scan_temp, tb, b.vec_cnt, b.vec_base, (int)n_vec, s) != hipSuccess)

@@ -59,26 +59,38 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, size_t n)
 }
 
 // First slot s in [from, n) of a bucket's owner words with owner == vid, or
-// (with STOP_AT_EMPTY) owner == 0; n when there is none.  Owners are read
-// four at a time.  `owner` receives the owner word found.
+// (with STOP_AT_EMPTY) owner == 0; n when there is none.  The whole owner row
+// (n <= 32 words) is read at once, its loads in flight together, and scanned
+// in registers.  `owner` receives the owner word found.
 template <bool STOP_AT_EMPTY>
 __device__ __forceinline__ uint32_t find_owner(const uint32_t *__restrict__ owners, uint32_t from, uint32_t n,
                                                uint32_t vid, uint32_t &owner)
 {
-    for (uint32_t base = from & ~3u; base < n; base += 4) {
-        const uint4 o = *reinterpret_cast<const uint4 *>(owners + base);
-        const uint32_t v[4] = {o.x, o.y, o.z, o.w};
+    uint32_t ov[32];
 #pragma unroll
-        for (uint32_t k = 0; k < 4; k++) {
-            const uint32_t s = base + k;
-            if (s >= from && (v[k] == vid || (STOP_AT_EMPTY && v[k] == 0u))) {
-                owner = v[k];
-                return s;
-            }
-        }
+    for (uint32_t base = 0; base < 32; base += 4) {
+        uint4 o = make_uint4(0u, 0u, 0u, 0u);
+        if (base < n) o = *reinterpret_cast<const uint4 *>(owners + base);
+        ov[base] = o.x;
+        ov[base + 1] = o.y;
+        ov[base + 2] = o.z;
+        ov[base + 3] = o.w;
     }
-    owner = 0xFFFFFFFFu;
-    return n;
+    uint32_t is_vid = 0u, is_free = 0u;
+#pragma unroll
+    for (uint32_t s = 0; s < 32; s++) {
+        is_vid |= (uint32_t)(ov[s] == vid) << s;
+        is_free |= (uint32_t)(STOP_AT_EMPTY && ov[s] == 0u) << s;
+    }
+    const uint32_t in_range = (from >= 32u ? 0u : 0xFFFFFFFFu << from) & (n >= 32u ? 0xFFFFFFFFu : (1u << n) - 1u);
+    const uint32_t hit = (is_vid | is_free) & in_range;
+    if (hit == 0u) {
+        owner = 0xFFFFFFFFu;
+        return n;
+    }
+    const uint32_t s = (uint32_t)__ffs(hit) - 1u;
+    owner = (is_vid >> s) & 1u ? vid : 0u;
+    return s;
 }
 
 // The reference's claim loop `for s: prev = CAS(owner[s], 0, vid); if (prev
@@ -109,25 +121,63 @@ __device__ __forceinline__ uint32_t claim_slot(uint32_t *__restrict__ owners, ui
 // ---------------------------------------------------------------------------
 // normal.comp
 // ---------------------------------------------------------------------------
-__device__ bool lookup_occupied(const uint32_t *__restrict__ w, uint32_t n, uint32_t depth, uint32_t x, uint32_t y,
-                                uint32_t z, uint32_t len)   // normal.comp:55-70
+// Occupancy of the lattice points around a centre (hx, hy, hz) as the
+// reference's point lookup reports it (normal.comp:55-70: descend from the
+// root; a non-internal node, or the leaf slot at the bottom, answers by its
+// material), with the centre's descent shared: depth d of a
+// descent chooses its child by coordinate bit D-d, so a neighbour whose
+// coordinates first differ from the centre's in bit b reads the same nodes as
+// the centre down to depth D-b-1.  The centre's descent records the group
+// offset of every depth (LDS column `cache`, entry d at cache[d * stride]);
+// a neighbour resumes at depth D-b from there, or takes the centre's outcome
+// when the centre's descent ended above that depth.
+struct SharedDescent {
+    uint32_t hx, hy, hz;
+    uint32_t stop;   // depth at which the centre's descent ended
+    bool occ;        // its outcome
+};
+
+__device__ __forceinline__ SharedDescent centre_descent(const uint32_t *__restrict__ w, uint32_t n, uint32_t depth,
+                                                        uint32_t hx, uint32_t hy, uint32_t hz, uint32_t len,
+                                                        uint32_t *cache, uint32_t stride)
 {
-    if (x >= len || y >= len || z >= len) return false;
-    uint32_t offset = 0, raw;
-    for (uint32_t d = 0; d < depth; d++) {
-        offset += locate(x, y, z, len >> d);
-        raw = offset < n ? w[offset] : 0u;
-        if (!(raw & 1u)) return ((raw >> 1) & 0x7Fu) != 0u;
+    SharedDescent c{hx, hy, hz, depth, false};
+    uint32_t offset = 0;
+    for (uint32_t d = 0; d <= depth; d++) {
+        cache[d * stride] = offset;
+        offset += locate(hx, hy, hz, len >> d);
+        const uint32_t raw = offset < n ? w[offset] : 0u;
+        if (d == depth || !(raw & 1u)) {
+            c.stop = d;
+            c.occ = ((raw >> 1) & 0x7Fu) != 0u;
+            break;
+        }
         offset = (raw >> 4) & 0x0FFFFFFFu;
     }
-    offset += locate(x, y, z, len >> depth);
-    raw = offset < n ? w[offset] : 0u;
-    return ((raw >> 1) & 0x7Fu) != 0u;
+    return c;
+}
+
+__device__ __forceinline__ bool neighbour_occupied(const uint32_t *__restrict__ w, uint32_t n, uint32_t depth,
+                                                   const SharedDescent &c, uint32_t x, uint32_t y, uint32_t z,
+                                                   uint32_t len, const uint32_t *cache, uint32_t stride)
+{
+    const uint32_t diff = (x ^ c.hx) | (y ^ c.hy) | (z ^ c.hz);
+    const uint32_t d0 = depth - (31u - (uint32_t)__clz(diff));   // first depth choosing by a differing bit
+    if (c.stop < d0) return c.occ;
+    uint32_t offset = cache[d0 * stride];
+    for (uint32_t d = d0; d <= depth; d++) {
+        offset += locate(x, y, z, len >> d);
+        const uint32_t raw = offset < n ? w[offset] : 0u;
+        if (d == depth || !(raw & 1u)) return ((raw >> 1) & 0x7Fu) != 0u;
+        offset = (raw >> 4) & 0x0FFFFFFFu;
+    }
+    return false;
 }
 
 __device__ f3 refined_normal(const uint32_t *__restrict__ w, uint32_t n, uint32_t depth, uint32_t hx, uint32_t hy,
-                             uint32_t hz, uint32_t len, int r)   // normal.comp:73-87
+                             uint32_t hz, uint32_t len, int r, uint32_t *cache, uint32_t stride)   // normal.comp:73-87
 {
+    const SharedDescent c = centre_descent(w, n, depth, hx, hy, hz, len, cache, stride);
     f3 acc{0.0f, 0.0f, 0.0f};
     for (int dx = -r; dx <= r; dx++)
         for (int dy = -r; dy <= r; dy++)
@@ -136,7 +186,8 @@ __device__ f3 refined_normal(const uint32_t *__restrict__ w, uint32_t n, uint32_
                 if (dx * dx + dy * dy + dz * dz > r * r) continue;
                 const int nx = (int)hx + dx, ny = (int)hy + dy, nz = (int)hz + dz;
                 const bool oob = nx < 0 || ny < 0 || nz < 0 || nx >= (int)len || ny >= (int)len || nz >= (int)len;
-                const bool empty = oob || !lookup_occupied(w, n, depth, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, len);
+                const bool empty =
+                    oob || !neighbour_occupied(w, n, depth, c, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, len, cache, stride);
                 if (empty) acc = add(acc, f3{(float)dx, (float)dy, (float)dz});
             }
     return length3(acc) > 0.001f ? normalize(acc) : f3{0.0f, 1.0f, 0.0f};
@@ -155,6 +206,7 @@ __global__ __launch_bounds__(256) void k_normal(const uint32_t *__restrict__ wor
                                                 const uint4 *__restrict__ gbuf, const uint32_t *__restrict__ gpos,
                                                 uint32_t *__restrict__ nbuf)
 {
+    __shared__ uint32_t path_cache[VE_MAX_DEPTH * 256];
     uint32_t x, y;
     const bool valid = tile_pixel(p, x, y);
     const size_t pix = valid ? (size_t)y * p.W + x : 0;
@@ -173,6 +225,7 @@ __global__ __launch_bounds__(256) void k_normal(const uint32_t *__restrict__ wor
         if (vid == lv && (int)lane != leader) lead = false;
     }
     if (!lead) return;
+    uint32_t *cache = path_cache + threadIdx.x;
     const uint32_t len = 1u << p.depth;
     const uint32_t pk = gpos[pix], bits = p.depth - 1u, mask = (1u << bits) - 1u;
     const uint32_t hx = (pk & mask) * 2u, hy = ((pk >> bits) & mask) * 2u, hz = ((pk >> (2u * bits)) & mask) * 2u;
@@ -184,7 +237,7 @@ __global__ __launch_bounds__(256) void k_normal(const uint32_t *__restrict__ wor
         // The claimant computes the normal; a later claimant that finds the
         // field still empty would compute the same value (normal.comp:151-156).
         if (claimed || __atomic_load_n(&bucket[NB_NRM + s], __ATOMIC_RELAXED) == 0u) {
-            const f3 nrm = refined_normal(words, p.n_words, p.depth, hx, hy, hz, len, p.normal_precision);
+            const f3 nrm = refined_normal(words, p.n_words, p.depth, hx, hy, hz, len, p.normal_precision, cache, 256u);
             atomicCAS(&bucket[NB_NRM + s], 0u, pack_normal(nrm));
         }
         atomicExch(&bucket[NB_TIME + s], p.time);
@@ -199,7 +252,7 @@ __global__ __launch_bounds__(256) void k_normal(const uint32_t *__restrict__ wor
             if (t[k] < oldest) { oldest = t[k]; oslot = q + k; }
     }
     atomicExch(&bucket[oslot], vid);
-    const f3 nrm = refined_normal(words, p.n_words, p.depth, hx, hy, hz, len, p.normal_precision);
+    const f3 nrm = refined_normal(words, p.n_words, p.depth, hx, hy, hz, len, p.normal_precision, cache, 256u);
     atomicExch(&bucket[NB_NRM + oslot], pack_normal(nrm));
     atomicExch(&bucket[NB_TIME + oslot], p.time);
 }

@@ -105,16 +105,7 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.materials, MATERIAL_SLOTS);
     a(&b.node_stats, 4);
     a(&b.ray_totals, 3);
-    // at most one id per pixel: the id table stays at most half full
-    b.sched_slots = 1u;
-    while ((size_t)b.sched_slots < 2 * groups_max * 64) b.sched_slots <<= 1;
-    a(&b.sched_key, b.sched_slots);
-    a(&b.sched_nfirst, b.sched_slots);
-    a(&b.sched_npix, b.sched_slots);
-    a(&b.sched_row_n, CLAIM_WIDTH);
-    a(&b.sched_vid, (size_t)CLAIM_WIDTH * SCHED_SLOTS);
-    a(&b.sched_first, (size_t)CLAIM_WIDTH * SCHED_SLOTS);
-    a(&b.sched_cnt, (size_t)CLAIM_WIDTH * SCHED_SLOTS);
+    a(&b.sched_tab, (size_t)CLAIM_WIDTH * SCHED_SLOTS);
     a(&b.cand_at, groups_max * 64);
     a(&b.vec_cnt, groups_max * 8);
     a(&b.vec_base, groups_max * 8);

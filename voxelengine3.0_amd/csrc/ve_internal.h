@@ -34,6 +34,7 @@ constexpr uint32_t RANK_STATS_WORDS = 264;     // wavefront.glsl:243-255
 constexpr uint32_t NUM_COUNTERS = 16;
 constexpr uint32_t SCHED_SLOTS = 128;          // ordering table: voxels per claim row before overflow
 constexpr uint32_t MATERIAL_SLOTS = 127;       // shade.comp:90
+constexpr uint32_t RANK_BLOCKS = 1024;           // rank pass grid (candidates strided over it)
 constexpr uint32_t RESOLVE_TIMEOUT_BLOCKS = 1024; // 4 waves each, striding over the timed-out pixels
 
 // Device layout of one hash bucket (one column h of the reference's R32UI

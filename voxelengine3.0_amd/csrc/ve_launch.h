@@ -36,14 +36,7 @@ struct DeviceBuffers {
     unsigned long long *node_stats = nullptr;  // 4: primary reads, trace reads, primary steps, trace steps
     unsigned long long *ray_totals = nullptr;  // 3: primary, traced, shadow (cumulative)
     // schedule / rank ordering scratch
-    uint32_t *sched_key = nullptr;       // sched_slots: open-addressed voxel-id table (0 = free)
-    uint32_t *sched_nfirst = nullptr;    // sched_slots: ~(lowest execution rank) of the id
-    uint32_t *sched_npix = nullptr;      // sched_slots: pixels of the id
-    uint32_t sched_slots = 0;            // power of two >= 2 x the context's pixels
-    uint32_t *sched_row_n = nullptr;     // CLAIM_WIDTH: members gathered into each claim row
-    uint32_t *sched_vid = nullptr;       // CLAIM_WIDTH x SCHED_SLOTS member voxel ids
-    uint32_t *sched_first = nullptr;     // member's lowest execution rank
-    uint32_t *sched_cnt = nullptr;       // member's pixel count
+    uint4 *sched_tab = nullptr;          // CLAIM_WIDTH x SCHED_SLOTS: {voxel id, ~lowest execution rank, pixels, 0}
     uint32_t *cand_at = nullptr;         // per execution rank: claim row/probe of the voxel it first claims
     uint32_t *vec_cnt = nullptr;         // per SIMD-8 vector: entries appended
     uint32_t *vec_base = nullptr;        // exclusive scan of vec_cnt

@@ -1,6 +1,6 @@
 // gfx950 kernels for the wavefront passes after the primary pass:
 //   normal.comp     -> k_normal
-//   schedule.comp   -> k_sched_claim, k_sched_gather, k_sched_rank, k_sched_vec_count, (scan), k_sched_write
+//   schedule.comp   -> k_sched_claim, k_sched_rank, k_sched_vec_count, (scan), k_sched_write
 //   rank.comp       -> k_rank, (scan), k_rank_write
 //   threshold.comp  -> k_threshold
 //   emit.comp       -> k_emit
@@ -278,25 +278,24 @@ __device__ __forceinline__ uint32_t sched_pixel(uint32_t e, const FrameParams &p
     return x < p.W && y < p.H;
 }
 
-// Slot of voxel id v in the open-addressed id table (a mixer unrelated to
-// claim_hash, so ids of one claim row spread over the table).
-__device__ __forceinline__ uint32_t sched_home(uint32_t v, uint32_t mask)
+// Home slot of voxel id v inside its claim row's region (a mixer unrelated to
+// claim_hash, which chose the row).
+__device__ __forceinline__ uint32_t sched_home(uint32_t v)
 {
     v *= 0x9E3779B1u;
     v ^= v >> 15;
     v *= 0x85EBCA77u;
     v ^= v >> 13;
-    return v & mask;
+    return v & (SCHED_SLOTS - 1u);
 }
 
 // One wave per work group g. Lanes with equal voxel ids are merged first (the
 // lowest lane of an id carries its lane count), then every distinct id of the
-// wave is entered into the id table at once: linear probing with CAS, then the
-// id's lowest execution rank (kept as its complement, so a zeroed table is
-// "none yet") and pixel count.
-__global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *__restrict__ gbuf,
-                                                    uint32_t *__restrict__ key, uint32_t *__restrict__ nfirst,
-                                                    uint32_t *__restrict__ npix, uint32_t mask,
+// wave enters its claim row's region at once, by linear probing from its home
+// slot. A slot is {id, ~lowest execution rank, pixels, 0}: the complement
+// makes a zeroed slot "free" and lets one 64-bit CAS both claim the slot and
+// keep the lowest rank; the pixel count is added on the same cache line.
+__global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *__restrict__ gbuf, uint4 *__restrict__ tab,
                                                     uint32_t *__restrict__ cand_at, uint32_t *__restrict__ counters)
 {
     const uint32_t lane = threadIdx.x;
@@ -315,92 +314,63 @@ __global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *
         if (lane == (uint32_t)leader) lanes = (uint32_t)__popcll(same);
     }
     if (lanes == 0u) return;
-    // at most one id per pixel and the table holds twice the pixels, so a
-    // free slot is always reached well inside the bound
-    uint32_t s = sched_home(vid, mask);
-    bool placed = false;
-    for (uint32_t probe = 0; probe <= mask; probe++) {
-        const uint32_t prev = atomicCAS(&key[s], 0u, vid);
-        if (prev == 0u || prev == vid) {
-            placed = true;
-            break;
+    uint4 *row = tab + (size_t)(claim_hash(vid) % CLAIM_WIDTH) * SCHED_SLOTS;
+    const unsigned long long want = ((unsigned long long)~e << 32) | vid;
+    uint32_t s = sched_home(vid);
+    for (uint32_t probe = 0; probe < SCHED_SLOTS; probe++, s = (s + 1u) & (SCHED_SLOTS - 1u)) {
+        unsigned long long *kf = reinterpret_cast<unsigned long long *>(row + s);
+        unsigned long long cur = atomicCAS(kf, 0ull, want);
+        if (cur != 0ull && (uint32_t)cur != vid) continue;   // another id's slot
+        // free (now ours) or this id's: keep the lower rank (higher complement)
+        while (cur != 0ull && (uint32_t)(cur >> 32) < ~e) {
+            const unsigned long long prev = atomicCAS(kf, cur, want);
+            if (prev == cur) break;
+            cur = prev;
         }
-        s = (s + 1u) & mask;
-    }
-    if (!placed) {
-        atomicOr(&counters[VE_CNT_OVERFLOW], 1u);
+        atomicAdd(&row[s].z, lanes);
         return;
     }
-    atomicMax(&nfirst[s], ~e);
-    atomicAdd(&npix[s], lanes);
-}
-
-// Moves every id of the table into its claim row's member list and clears
-// the table behind it, so the next frame finds it empty.
-__global__ __launch_bounds__(256) void k_sched_gather(uint32_t *__restrict__ key, uint32_t *__restrict__ nfirst,
-                                                      uint32_t *__restrict__ npix, uint32_t n_slots,
-                                                      uint32_t *__restrict__ row_n, uint32_t *__restrict__ tab_vid,
-                                                      uint32_t *__restrict__ tab_first, uint32_t *__restrict__ tab_cnt,
-                                                      uint32_t *__restrict__ counters)
-{
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q * 4u >= n_slots) return;
-    const uint4 k4 = reinterpret_cast<const uint4 *>(key)[q];
-    if ((k4.x | k4.y | k4.z | k4.w) == 0u) return;
-    const uint32_t k[4] = {k4.x, k4.y, k4.z, k4.w};
-    for (uint32_t j = 0; j < 4; j++) {
-        if (k[j] == 0u) continue;
-        const uint32_t s = q * 4u + j;
-        const uint32_t f = ~nfirst[s], c = npix[s];
-        nfirst[s] = 0u;
-        npix[s] = 0u;
-        const uint32_t h = claim_hash(k[j]) % CLAIM_WIDTH;
-        const uint32_t at = atomicAdd(&row_n[h], 1u);
-        if (at < SCHED_SLOTS) {
-            tab_vid[(size_t)h * SCHED_SLOTS + at] = k[j];
-            tab_first[(size_t)h * SCHED_SLOTS + at] = f;
-            tab_cnt[(size_t)h * SCHED_SLOTS + at] = c;
-        } else {
-            atomicOr(&counters[VE_CNT_OVERFLOW], 1u);
-        }
-    }
-    reinterpret_cast<uint4 *>(key)[q] = make_uint4(0u, 0u, 0u, 0u);
+    atomicOr(&counters[VE_CNT_OVERFLOW], 1u);   // more distinct ids than the row region holds
 }
 
 // One wave per claim row: a voxel's probe index in the reference's claim map is
 // the number of voxels of its row claimed before it (lower execution rank).
 // Writes the reference-layout claim and pixel-count maps (all 64 probes, so
 // no separate clear is needed), marks each kept voxel's first pixel, and
-// empties the row's member list for the next frame.
-__global__ __launch_bounds__(64) void k_sched_rank(uint32_t *__restrict__ row_n, const uint32_t *__restrict__ tab_vid,
-                                                   const uint32_t *__restrict__ tab_first,
-                                                   const uint32_t *__restrict__ tab_cnt, uint32_t *__restrict__ claim,
+// clears the row region for the next frame.
+__global__ __launch_bounds__(64) void k_sched_rank(uint4 *__restrict__ tab, uint32_t *__restrict__ claim,
                                                    uint32_t *__restrict__ samples, uint32_t *__restrict__ cand_at)
 {
+    constexpr uint32_t PER_LANE = SCHED_SLOTS / 64u;
     __shared__ uint32_t firsts[SCHED_SLOTS];
     const uint32_t h = blockIdx.x, lane = threadIdx.x;
-    const size_t base = (size_t)h * SCHED_SLOTS;
-    const uint32_t n = min(row_n[h], SCHED_SLOTS);
-    uint32_t v[2] = {0u, 0u}, f[2] = {NONE, NONE}, c[2] = {0u, 0u};
-    for (int k = 0; k < 2; k++) {
-        const uint32_t s = lane + 64u * k;
-        if (s < n) {
-            v[k] = tab_vid[base + s];
-            f[k] = tab_first[base + s];
-            c[k] = tab_cnt[base + s];
+    uint4 *row = tab + (size_t)h * SCHED_SLOTS;
+    uint4 m[PER_LANE];
+#pragma unroll
+    for (uint32_t k = 0; k < PER_LANE; k++) m[k] = row[lane + 64u * k];
+    // compact the occupied slots: member j of the row sits at firsts[j]
+    uint32_t at[PER_LANE], n = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < PER_LANE; k++) {
+        const uint64_t occ = __ballot(m[k].x != 0u);
+        at[k] = n + (uint32_t)__popcll(occ & ((1ull << lane) - 1ull));
+        n += (uint32_t)__popcll(occ);
+        if (m[k].x != 0u) {
+            firsts[at[k]] = ~m[k].y;
+            row[lane + 64u * k] = make_uint4(0u, 0u, 0u, 0u);
         }
-        firsts[s] = f[k];
     }
     __syncthreads();
-    for (int k = 0; k < 2; k++) {
-        const uint32_t s = lane + 64u * k;
-        if (s >= n) continue;
+#pragma unroll
+    for (uint32_t k = 0; k < PER_LANE; k++) {
+        if (m[k].x == 0u) continue;
+        const uint32_t f = ~m[k].y;
         uint32_t rank = 0;
-        for (uint32_t j = 0; j < n; j++) rank += firsts[j] < f[k];
+        for (uint32_t j = 0; j < n; j++) rank += firsts[j] < f;
         if (rank < CLAIM_PROBES) {
-            claim[(size_t)h * CLAIM_PROBES + rank] = v[k];
-            samples[(size_t)h * CLAIM_PROBES + rank] = c[k];
-            cand_at[f[k]] = (rank << 26) | (h << 6) | rank;
+            claim[(size_t)h * CLAIM_PROBES + rank] = m[k].x;
+            samples[(size_t)h * CLAIM_PROBES + rank] = m[k].z;
+            cand_at[f] = (rank << 26) | (h << 6) | rank;
         }
     }
     // probes past the row's population read 0, as after the per-frame clear
@@ -408,7 +378,6 @@ __global__ __launch_bounds__(64) void k_sched_rank(uint32_t *__restrict__ row_n,
         claim[(size_t)h * CLAIM_PROBES + s] = 0u;
         samples[(size_t)h * CLAIM_PROBES + s] = 0u;
     }
-    if (lane == 0u) row_n[h] = 0u;
 }
 
 // candidates first-claimed in each SIMD-8 vector
@@ -479,6 +448,9 @@ __device__ float indirect_variance(const uint32_t *__restrict__ lbuf, uint32_t h
 
 // Per candidate: samples (0/1/2/minSamples), histogram, min/max; per SIMD-8
 // vector: the number of shade-list entries it appends.
+// Candidates are strided over a fixed grid (the count is known only on the
+// device), so each block merges many candidates into its histogram and
+// min/max before the few global atomics of its flush.
 __global__ __launch_bounds__(256) void k_rank(FrameParams p, const uint4 *__restrict__ candidates,
                                               const uint32_t *__restrict__ samples_map, const uint32_t *__restrict__ lbuf,
                                               uint32_t *__restrict__ rank_stats, const uint32_t *__restrict__ counters,
@@ -492,27 +464,38 @@ __global__ __launch_bounds__(256) void k_rank(FrameParams p, const uint4 *__rest
         bmax = 0u;
     }
     __syncthreads();
-    uint32_t my_min = 0xFFFFFFFFu, my_max = 0u;
+    uint32_t my_min = 0xFFFFFFFFu, my_max = 0u, active = 0u;
     const uint32_t n = counters[VE_CNT_CANDIDATES];
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    int32_t s = 0;
-    if (t < n) {
-        const uint4 cd = candidates[t];
-        const uint32_t pixels = samples_map[(size_t)cd.z * CLAIM_PROBES + cd.w];
-        // with a zero weight the variance term is 0 * vn with vn in [0, 1]
-        // (clamp maps NaN to 0), so skipping the lookup is exact
-        const float var = p.w_variance != 0.0f ? indirect_variance(lbuf, p.hash_width, cd.x) : 0.0f;
-        const float pn = gclamp((float)pixels / gmax(p.pixels_norm, 1.0f), 0.0f, 1.0f);
-        const float vn = gclamp(var / gmax(p.variance_norm, 1.0f), 0.0f, 1.0f);
-        const float imp = p.w_pixels * pn + p.w_variance * vn;
-        const float tHigh = __uint_as_float(rank_stats[0]);
-        const float tMid = __uint_as_float(rank_stats[1]);
-        s = imp >= tHigh ? 2 : (imp >= tMid ? 1 : p.min_samples);
-        uint32_t bin = (uint32_t)(imp * 256.0f);
-        if (bin >= HISTOGRAM_BINS) bin = HISTOGRAM_BINS - 1u;
-        atomicAdd(&hist[bin], 1u);
-        my_min = my_max = __float_as_uint(imp);   // min/max of the float bits as uints (rank.comp:141-143)
-        nsamp[t] = s;
+    const float tHigh = __uint_as_float(rank_stats[0]);
+    const float tMid = __uint_as_float(rank_stats[1]);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t t = base + threadIdx.x;
+        active += min(n - base, (uint32_t)blockDim.x);
+        int32_t s = 0;
+        if (t < n) {
+            const uint4 cd = candidates[t];
+            const uint32_t pixels = samples_map[(size_t)cd.z * CLAIM_PROBES + cd.w];
+            // with a zero weight the variance term is 0 * vn with vn in [0, 1]
+            // (clamp maps NaN to 0), so skipping the lookup is exact
+            const float var = p.w_variance != 0.0f ? indirect_variance(lbuf, p.hash_width, cd.x) : 0.0f;
+            const float pn = gclamp((float)pixels / gmax(p.pixels_norm, 1.0f), 0.0f, 1.0f);
+            const float vn = gclamp(var / gmax(p.variance_norm, 1.0f), 0.0f, 1.0f);
+            const float imp = p.w_pixels * pn + p.w_variance * vn;
+            s = imp >= tHigh ? 2 : (imp >= tMid ? 1 : p.min_samples);
+            uint32_t bin = (uint32_t)(imp * 256.0f);
+            if (bin >= HISTOGRAM_BINS) bin = HISTOGRAM_BINS - 1u;
+            atomicAdd(&hist[bin], 1u);
+            // min/max of the float bits as uints (rank.comp:141-143)
+            my_min = min(my_min, __float_as_uint(imp));
+            my_max = max(my_max, __float_as_uint(imp));
+            nsamp[t] = s;
+        }
+        // vector total = sum of max(s, 0) over the 8 lanes of the vector
+        uint32_t tot = s > 0 ? (uint32_t)s : 0u;
+        tot += __shfl_xor(tot, 1, 8);
+        tot += __shfl_xor(tot, 2, 8);
+        tot += __shfl_xor(tot, 4, 8);
+        if ((t & 7u) == 0u && t < n) vec_tot[t >> 3] = tot;
     }
     for (int off = 32; off > 0; off >>= 1) {
         my_min = min(my_min, (uint32_t)__shfl_xor((int)my_min, off, 64));
@@ -522,14 +505,7 @@ __global__ __launch_bounds__(256) void k_rank(FrameParams p, const uint4 *__rest
         atomicMin(&bmin, my_min);
         atomicMax(&bmax, my_max);
     }
-    // vector total = sum of max(s, 0) over the 8 lanes of the vector
-    uint32_t tot = s > 0 ? (uint32_t)s : 0u;
-    tot += __shfl_xor(tot, 1, 8);
-    tot += __shfl_xor(tot, 2, 8);
-    tot += __shfl_xor(tot, 4, 8);
-    if ((t & 7u) == 0u && t < n) vec_tot[t >> 3] = tot;
     __syncthreads();
-    const uint32_t active = n > blockIdx.x * blockDim.x ? min(n - blockIdx.x * blockDim.x, (uint32_t)blockDim.x) : 0u;
     if (threadIdx.x == 0 && active) {
         atomicAdd(&rank_stats[6], active);
         atomicMin(&rank_stats[4], bmin);
@@ -575,32 +551,48 @@ __global__ __launch_bounds__(256) void k_rank_write(FrameParams p, const uint4 *
     }
 }
 
-// threshold.comp:31-78, one thread
-__global__ void k_threshold(FrameParams p, uint32_t *__restrict__ rank_stats, const uint32_t *__restrict__ counters)
+// threshold.comp:31-78. The reference walks the histogram from the top bin
+// down and stops where the running count first reaches topCount (resp.
+// topMidCount); that bin is the largest b whose suffix count (bins b..255)
+// reaches it, i.e. (number of bins with suffix >= count) - 1, as suffix
+// counts fall with b. One wave: four bins per lane, a suffix scan across lanes.
+static_assert(HISTOGRAM_BINS == 256, "k_threshold holds four bins per lane");
+__global__ __launch_bounds__(64) void k_threshold(FrameParams p, uint32_t *__restrict__ rank_stats,
+                                                  const uint32_t *__restrict__ counters)
 {
-    if (threadIdx.x != 0 || counters[VE_CNT_CANDIDATES] == 0u) return;
-    uint32_t total = 0;
-    for (uint32_t b = 0; b < HISTOGRAM_BINS; b++) total += rank_stats[8 + b];
+    if (counters[VE_CNT_CANDIDATES] == 0u) return;
+    const uint32_t lane = threadIdx.x;
+    const uint4 h = reinterpret_cast<const uint4 *>(rank_stats + 8)[lane];
+    const uint32_t mine = h.x + h.y + h.z + h.w;
+    uint32_t suf = mine;   // bins 4*lane .. 255
+    for (uint32_t o = 1; o < 64u; o <<= 1) {
+        const uint32_t t = __shfl_down(suf, o, 64);
+        if (lane + o < 64u) suf += t;
+    }
+    const uint32_t total = __shfl(suf, 0, 64);
     if (total == 0u) {
-        rank_stats[0] = 0x7F800000u;
-        rank_stats[1] = 0xFF800000u;
+        if (lane == 0u) {
+            rank_stats[0] = 0x7F800000u;
+            rank_stats[1] = 0xFF800000u;
+        }
         return;
     }
     const float topF = gclamp(p.top_fraction, 0.0f, 1.0f);
     const float midF = gclamp(p.mid_fraction, 0.0f, 1.0f - topF);
     const uint32_t topCount = (uint32_t)(topF * (float)total);
     const uint32_t topMidCount = (uint32_t)((topF + midF) * (float)total);
-    uint32_t cum = 0, thb = HISTOGRAM_BINS, tmb = HISTOGRAM_BINS;
-    for (int b = (int)HISTOGRAM_BINS - 1; b >= 0; b--) {
-        cum += rank_stats[8 + b];
-        if (thb == HISTOGRAM_BINS && cum >= topCount) thb = (uint32_t)b;
-        if (tmb == HISTOGRAM_BINS && cum >= topMidCount) tmb = (uint32_t)b;
-        if (thb != HISTOGRAM_BINS && tmb != HISTOGRAM_BINS) break;
+    const uint32_t s3 = suf - mine + h.w, s2 = s3 + h.z, s1 = s2 + h.y, s0 = suf;
+    const uint32_t n_top = (uint32_t)(__popcll(__ballot(s0 >= topCount)) + __popcll(__ballot(s1 >= topCount)) +
+                                      __popcll(__ballot(s2 >= topCount)) + __popcll(__ballot(s3 >= topCount)));
+    const uint32_t n_mid = (uint32_t)(__popcll(__ballot(s0 >= topMidCount)) + __popcll(__ballot(s1 >= topMidCount)) +
+                                      __popcll(__ballot(s2 >= topMidCount)) + __popcll(__ballot(s3 >= topMidCount)));
+    // no bin reaching the count leaves the top bin, as the reference's fallback
+    const uint32_t thb = n_top ? n_top - 1u : HISTOGRAM_BINS - 1u;
+    const uint32_t tmb = n_mid ? n_mid - 1u : HISTOGRAM_BINS - 1u;
+    if (lane == 0u) {
+        rank_stats[0] = __float_as_uint((float)thb / 256.0f);
+        rank_stats[1] = __float_as_uint((float)tmb / 256.0f);
     }
-    if (thb == HISTOGRAM_BINS) thb = HISTOGRAM_BINS - 1u;
-    if (tmb == HISTOGRAM_BINS) tmb = HISTOGRAM_BINS - 1u;
-    rank_stats[0] = __float_as_uint((float)thb / 256.0f);
-    rank_stats[1] = __float_as_uint((float)tmb / 256.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -1615,16 +1607,8 @@ int launch_schedule(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
     const uint32_t gw = (p.W + 7u) / 8u, gh = (p.H + 7u) / 8u;
     const uint32_t groups = gw * gh;
     const uint32_t n_vec = groups * 8u;
-    // the part of the id table this frame uses: twice its pixels (the table
-    // is empty between frames, so any power-of-two part of it works)
-    uint32_t slots = 4u;
-    while (slots < 2u * groups * 64u && slots < b.sched_slots) slots <<= 1;
-    hipLaunchKernelGGL(k_sched_claim, dim3(groups), dim3(64), 0, s, p, b.gbuf, b.sched_key, b.sched_nfirst, b.sched_npix,
-                       slots - 1u, b.cand_at, b.counters);
-    hipLaunchKernelGGL(k_sched_gather, dim3(cdiv(slots / 4u, 256)), dim3(256), 0, s, b.sched_key, b.sched_nfirst,
-                       b.sched_npix, slots, b.sched_row_n, b.sched_vid, b.sched_first, b.sched_cnt, b.counters);
-    hipLaunchKernelGGL(k_sched_rank, dim3(CLAIM_WIDTH), dim3(64), 0, s, b.sched_row_n, b.sched_vid, b.sched_first,
-                       b.sched_cnt, b.claim, b.samples, b.cand_at);
+    hipLaunchKernelGGL(k_sched_claim, dim3(groups), dim3(64), 0, s, p, b.gbuf, b.sched_tab, b.cand_at, b.counters);
+    hipLaunchKernelGGL(k_sched_rank, dim3(CLAIM_WIDTH), dim3(64), 0, s, b.sched_tab, b.claim, b.samples, b.cand_at);
     hipLaunchKernelGGL(k_sched_vec_count, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, b.cand_at, n_vec, b.vec_cnt);
     size_t tb = b.scan_temp_bytes;
     if (hipcub::DeviceScan::ExclusiveSum(b.scan_temp, tb, b.vec_cnt, b.vec_base, (int)n_vec, s) != hipSuccess)
@@ -1640,7 +1624,8 @@ int launch_rank(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
     const uint32_t bound = (uint32_t)std::min<size_t>((size_t)p.W * p.H, CANDIDATE_CAPACITY);
     const uint32_t n_vec = cdiv(bound, 8);
     launch_fill(s, b.vec_cnt, 0u, n_vec);
-    hipLaunchKernelGGL(k_rank, dim3(cdiv(bound, 256)), dim3(256), 0, s, p, b.candidates, b.samples, b.lbuf,
+    hipLaunchKernelGGL(k_rank, dim3(std::min<uint32_t>(cdiv(bound, 256), RANK_BLOCKS)), dim3(256), 0, s, p, b.candidates,
+                       b.samples, b.lbuf,
                        b.rank_stats, b.counters, b.nsamp, b.vec_cnt);
     size_t tb = b.scan_temp_bytes;
     if (hipcub::DeviceScan::ExclusiveSum(b.scan_temp, tb, b.vec_cnt, b.vec_base, (int)n_vec, s) != hipSuccess)

@@ -458,7 +458,6 @@ int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p)
     switch (pass) {
     case VE_PASS_PRIMARY:
         launch_primary(c->stream, s->words, p, c->b, c->node_stats);
-        launch_count_primary(c->stream, c->b, p.W * p.H);
         break;
     case VE_PASS_NORMAL: launch_normal(c->stream, s->words, p, c->b); break;
     case VE_PASS_RESET: launch_reset(c->stream, c->b); break;
@@ -488,13 +487,19 @@ int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p)
     return VE_OK;
 }
 
+// Pass-timing events only carry timestamps (ve_timings reads them after a
+// stream synchronisation), so they skip the system-scope release: that
+// release writes back and invalidates the caches at every pass boundary and
+// stalls the next pass for about 10 us.
+constexpr unsigned TIMING_EVENT_FLAGS = hipEventDisableSystemFence;
+
 // Event slot of the frame being recorded (timing mode only).
 hipEvent_t *frame_slot(ve_ctx *c)
 {
     if (c->frames_recorded >= c->frame_events.size()) {
         std::array<hipEvent_t, 2 * VE_PASS_COUNT * MAX_ITERS> fe{};
         for (auto &e : fe)
-            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            if (hipEventCreateWithFlags(&e, TIMING_EVENT_FLAGS) != hipSuccess) return nullptr;
         c->frame_events.push_back(fe);
     }
     return c->frame_events[c->frames_recorded].data();
@@ -619,7 +624,7 @@ int ve_enable_timing(ve_ctx *c, int enable)
     HIP_OK(c, hipSetDevice(c->device));
     while (enable > 1 && c->frame_events.size() < (size_t)enable) {
         std::array<hipEvent_t, 2 * VE_PASS_COUNT * MAX_ITERS> fe{};
-        for (auto &e : fe) HIP_OK(c, hipEventCreate(&e));
+        for (auto &e : fe) HIP_OK(c, hipEventCreateWithFlags(&e, TIMING_EVENT_FLAGS));
         c->frame_events.push_back(fe);
     }
     return VE_OK;

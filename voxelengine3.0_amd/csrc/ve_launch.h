@@ -49,7 +49,6 @@ void launch_primary(hipStream_t s, const uint32_t *words, const FrameParams &p, 
 void launch_fill(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
 void launch_normal(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b);
 void launch_reset(hipStream_t s, DeviceBuffers &b);
-void launch_count_primary(hipStream_t s, DeviceBuffers &b, uint32_t n);
 int launch_schedule(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 int launch_rank(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 void launch_emit(hipStream_t s, const FrameParams &p, DeviceBuffers &b);

@@ -454,7 +454,7 @@ __device__ float indirect_variance(const uint32_t *__restrict__ lbuf, uint32_t h
 __global__ __launch_bounds__(256) void k_rank(FrameParams p, const uint4 *__restrict__ candidates,
                                               const uint32_t *__restrict__ samples_map, const uint32_t *__restrict__ lbuf,
                                               uint32_t *__restrict__ rank_stats, const uint32_t *__restrict__ counters,
-                                              int32_t *__restrict__ nsamp, uint32_t *__restrict__ vec_tot)
+                                              int32_t *__restrict__ nsamp, uint32_t *__restrict__ vec_tot, uint32_t n_vec)
 {
     __shared__ uint32_t hist[HISTOGRAM_BINS];
     __shared__ uint32_t bmin, bmax;
@@ -497,6 +497,9 @@ __global__ __launch_bounds__(256) void k_rank(FrameParams p, const uint4 *__rest
         tot += __shfl_xor(tot, 4, 8);
         if ((t & 7u) == 0u && t < n) vec_tot[t >> 3] = tot;
     }
+    // vectors past the last candidate append nothing (the scan covers n_vec)
+    for (uint32_t v = (n + 7u) / 8u + blockIdx.x * blockDim.x + threadIdx.x; v < n_vec; v += gridDim.x * blockDim.x)
+        vec_tot[v] = 0u;
     for (int off = 32; off > 0; off >>= 1) {
         my_min = min(my_min, (uint32_t)__shfl_xor((int)my_min, off, 64));
         my_max = max(my_max, (uint32_t)__shfl_xor((int)my_max, off, 64));
@@ -646,10 +649,14 @@ __global__ __launch_bounds__(256) void k_trace(const uint32_t *__restrict__ word
                                                uint32_t *__restrict__ counters, unsigned long long *__restrict__ stats,
                                                unsigned long long *__restrict__ totals,
                                                const Material *__restrict__ mats,
-                                               const uint32_t *__restrict__ em_claim, uint32_t *__restrict__ em_bits)
+                                               const uint32_t *__restrict__ em_claim, uint32_t *__restrict__ em_bits,
+                                               uint32_t *__restrict__ live_out)
 {
     const uint32_t live = *live_ptr;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    // the shade pass of this iteration appends its survivors from zero; the
+    // counter was last read by the previous iteration
+    if (t == 0) *live_out = 0u;
     if (t == 0 && live > 0u) {
         atomicAdd(&counters[VE_CNT_RAYS_TRACED], live);
         atomicMax(&counters[VE_CNT_BOUNCES_RUN], iter + 1u);
@@ -1363,8 +1370,9 @@ __device__ Avg voxel_avg(const uint32_t *__restrict__ lbuf, uint32_t hw, uint32_
 // Lanes of a tile that see the same voxel share one lookup.
 __global__ __launch_bounds__(256) void k_resolve_avg(FrameParams p, const uint4 *__restrict__ gbuf,
                                                      const uint32_t *__restrict__ lbuf, float4 *__restrict__ pix_avg,
-                                                     uint32_t *__restrict__ pix_cnt)
+                                                     uint32_t *__restrict__ pix_cnt, uint32_t *__restrict__ counters)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[VE_CNT_TIMEOUTS] = 0u;   // k_resolve appends after this pass
     uint32_t x = 0, y = 0;
     const bool valid = tile_pixel(p, x, y);
     const size_t pix = valid ? (size_t)y * p.W + x : 0;
@@ -1586,17 +1594,6 @@ __global__ void k_frame_reset(uint32_t *__restrict__ rank_stats, uint32_t *__res
     if (i < NUM_COUNTERS && i != VE_CNT_EMISSIVES) counters[i] = 0u;
 }
 
-__global__ void k_count_primary(uint32_t *__restrict__ counters, unsigned long long *__restrict__ totals, uint32_t n)
-{
-    counters[VE_CNT_PRIMARY_RAYS] = n;
-    totals[0] += n;
-}
-
-void launch_count_primary(hipStream_t s, DeviceBuffers &b, uint32_t n)
-{
-    hipLaunchKernelGGL(k_count_primary, dim3(1), dim3(1), 0, s, b.counters, b.ray_totals, n);
-}
-
 void launch_reset(hipStream_t s, DeviceBuffers &b)
 {
     hipLaunchKernelGGL(k_frame_reset, dim3(1), dim3(256), 0, s, b.rank_stats, b.counters);
@@ -1623,10 +1620,8 @@ int launch_rank(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
     // candidates <= distinct visible voxels <= pixels
     const uint32_t bound = (uint32_t)std::min<size_t>((size_t)p.W * p.H, CANDIDATE_CAPACITY);
     const uint32_t n_vec = cdiv(bound, 8);
-    launch_fill(s, b.vec_cnt, 0u, n_vec);
     hipLaunchKernelGGL(k_rank, dim3(std::min<uint32_t>(cdiv(bound, 256), RANK_BLOCKS)), dim3(256), 0, s, p, b.candidates,
-                       b.samples, b.lbuf,
-                       b.rank_stats, b.counters, b.nsamp, b.vec_cnt);
+                       b.samples, b.lbuf, b.rank_stats, b.counters, b.nsamp, b.vec_cnt, n_vec);
     size_t tb = b.scan_temp_bytes;
     if (hipcub::DeviceScan::ExclusiveSum(b.scan_temp, tb, b.vec_cnt, b.vec_base, (int)n_vec, s) != hipSuccess)
         return VE_EDEVICE;
@@ -1663,11 +1658,12 @@ void launch_trace(hipStream_t s, const uint32_t *words, const FrameParams &p, De
     const uint32_t nb = cdiv(p.shade_budget, 256);
     if (stats)
         hipLaunchKernelGGL(k_trace<true>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, iter), live_in_ptr(b, iter),
-                           iter, b.counters, b.node_stats, b.ray_totals, b.materials, b.emissive_claim, b.em_bits);
+                           iter, b.counters, b.node_stats, b.ray_totals, b.materials, b.emissive_claim, b.em_bits,
+                           live_out_ptr(b, iter));
     else
         hipLaunchKernelGGL(k_trace<false>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, iter),
                            live_in_ptr(b, iter), iter, b.counters, b.node_stats, b.ray_totals, b.materials,
-                           b.emissive_claim, b.em_bits);
+                           b.emissive_claim, b.em_bits, live_out_ptr(b, iter));
     if (iter == 0)
         hipLaunchKernelGGL(k_em_register, dim3(1), dim3(1024), 0, s, bounce_in(b, iter), live_in_ptr(b, iter),
                            b.materials, b.emissive_claim, b.emissives, b.em_bits, b.em_reg, b.counters);
@@ -1676,7 +1672,6 @@ void launch_trace(hipStream_t s, const uint32_t *words, const FrameParams &p, De
 void launch_shade(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t iter)
 {
     const uint32_t nb = cdiv(p.shade_budget, 256);
-    launch_fill(s, live_out_ptr(b, iter), 0u, 1);
     hipLaunchKernelGGL(k_shade, dim3(nb), dim3(256), 0, s, p, words, iter, bounce_in(b, iter), live_in_ptr(b, iter),
                        bounce_out(b, iter), live_out_ptr(b, iter), b.lbuf, b.nbuf, b.rbuf, b.materials, b.emissives,
                        b.em_reg, b.counters, b.ray_totals);
@@ -1702,8 +1697,8 @@ void launch_display(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 
 void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 {
-    hipLaunchKernelGGL(k_resolve_avg, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.pix_avg, b.pix_cnt);
-    launch_fill(s, b.counters + VE_CNT_TIMEOUTS, 0u, 1);
+    hipLaunchKernelGGL(k_resolve_avg, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.pix_avg, b.pix_cnt,
+                       b.counters);
     hipLaunchKernelGGL(k_resolve, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.nbuf, b.materials,
                        b.pix_avg, b.pix_cnt, b.output, b.timeouts, b.counters);
     hipLaunchKernelGGL(k_resolve_timeout, dim3(RESOLVE_TIMEOUT_BLOCKS), dim3(256), 0, s, p, b.gbuf, b.pix_avg,

@@ -10,12 +10,19 @@ emit, the trace/shade bounce loop and resolve.
 
 value = (primary rays + traced bounce rays + NEE shadow rays) of the timed
 frames / wall time of the timed region (inputs resident in HBM; the host only
-enqueues).  The roofline entry is the primary kernel (the dominant one):
-algorithmic node bytes per launch (4 bytes per node word the reference's
-traversal reads, counted by the instrumented kernel on the same frames) over
-its mean launch time from HIP events recorded on the context's stream during
-the timed region.  cpu_baseline: the C oracle (a scalar port of the same
-passes) on the host, whole frames of the same workload.
+enqueues; frames as an application renders them: the context's default
+cross-frame overlap, where a frame's primary pass runs on a second stream
+beside the previous frame's bounce passes, and no instrumentation).  The same
+number of frames then runs again with per-pass HIP events bound to the
+passes' kernel dispatches (pass_ms), and once more with the overlap off (the
+reference's strict pass order) for pass_ms_serial.  The roofline entry is the
+primary kernel (the reference's RaycastCounted traversal): algorithmic node
+bytes per launch (4 bytes per node word the reference's traversal reads,
+counted by the instrumented kernel on the same frames) over its mean launch
+time from those HIP events, in the overlapped pipeline (launch_ms, sharing
+the device with the previous frame's bounce passes) and alone
+(isolated_launch_ms / isolated_frac).  cpu_baseline: the C oracle (a scalar
+port of the same passes) on the host, whole frames of the same workload.
 
 `served` and `batched` (skipped with --no-served): the served form (Poisson
 sessions over resident octrees on the session server, p50/p99 frame latency,
@@ -196,9 +203,8 @@ def main():
     c = r.counters()
     assert c[ve.CNT["OVERFLOW"]] == 0, "claim-row overflow"
 
-    # timed region: K frames, HIP events per pass on the context's stream
+    # timed region: K frames as an application renders them
     r.reset_ray_totals()
-    r.enable_timing(args.steps)
     torch.cuda.synchronize()
     r.sync()
     t0 = time.perf_counter()
@@ -207,11 +213,25 @@ def main():
     r.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    pass_ms = r.timings()
-    r.enable_timing(False)
     totals = r.ray_totals()
     rays = totals["primary"] + totals["traced"] + totals["shadow"]
     value = rays / elapsed / 1e6
+
+    # the same K frames with HIP events on every pass, then with the overlap off
+    def timed_frames(overlap):
+        r.set_overlap(overlap)
+        r.enable_timing(args.steps)
+        r.sync()
+        for i in range(args.warmup, args.warmup + args.steps):
+            frame(i)
+        r.sync()
+        ms = r.timings()
+        r.enable_timing(False)
+        return ms
+
+    pass_ms = timed_frames(True)
+    pass_ms_serial = timed_frames(False)
+    r.set_overlap(True)
 
     # algorithmic node traffic of the primary kernel on the same frames
     r.enable_node_stats(True)
@@ -225,6 +245,8 @@ def main():
     bytes_per_launch = st["primary_reads"] * 4.0 / n_stat
     prim_ms = pass_ms["PRIMARY"]
     achieved = bytes_per_launch / (prim_ms * 1e-3) / 1e9 if prim_ms > 0 else 0.0
+    iso_ms = pass_ms_serial["PRIMARY"]
+    iso_achieved = bytes_per_launch / (iso_ms * 1e-3) / 1e9 if iso_ms > 0 else 0.0
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -258,7 +280,9 @@ def main():
             "rays_per_frame": round(rays / args.steps, 1),
             "primary_rays_per_frame": W * H,
             "traced_rays_per_frame": round(totals["traced"] / args.steps, 1),
+            "overlap": "primary pass of frame k+1 beside the bounce passes of frame k",
             "pass_ms": {k: round(v, 4) for k, v in pass_ms.items()},
+            "pass_ms_serial": {k: round(v, 4) for k, v in pass_ms_serial.items()},
             "scene_build_s": round(t_build, 2),
         },
         "roofline": {
@@ -272,6 +296,9 @@ def main():
             "traffic_source": "profiles/k_primary_extreme_1080p.md (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
             "algorithmic_bytes_per_launch": round(bytes_per_launch),
             "launch_ms": round(prim_ms, 4),
+            "isolated_launch_ms": round(iso_ms, 4),
+            "isolated_achieved": round(iso_achieved, 1),
+            "isolated_frac": round(iso_achieved / HBM_PEAK_GBS, 4),
             "node_reads_per_ray": round(st["primary_reads"] / n_stat / (W * H), 3),
             "steps_per_ray": round(st["primary_steps"] / n_stat / (W * H), 3),
         },

@@ -213,16 +213,26 @@ int ve_read(ve_ctx *ctx, ve_buffer_id id, void *dst, size_t bytes);
 int ve_write(ve_ctx *ctx, ve_buffer_id id, const void *src, size_t bytes);
 int ve_buffer_size(ve_ctx *ctx, ve_buffer_id id, size_t *bytes);
 
-/* Block until the context's stream is idle. */
+/* Block until the context's streams are idle. */
 int ve_sync(ve_ctx *ctx);
 
-/* Optional per-pass timing: while enabled, every ve_frame records HIP events
- * around each pass on the context's stream (no host synchronisation);
- * ve_timings waits for the stream and returns, per pass, the mean
- * milliseconds per frame over the frames recorded since the last call
- * (trace and shade: all bounce iterations of a frame together).  An
+/* Cross-frame overlap (on by default for contexts made by ve_create): each
+ * ve_frame renders its primary pass into a second G-buffer on an auxiliary
+ * stream, beside the previous frame's trace/shade/resolve passes, and the
+ * context's stream waits for it before the normal pass.  Every frame's
+ * results are those of the serial order.  Off: all passes in order on the
+ * context's stream (Renderer::run's sequence).  No counterpart in the
+ * reference, whose GL passes run in submission order; VE_EINVAL for a
+ * context without the second stream (server sessions). */
+int ve_set_overlap(ve_ctx *ctx, int enable);
+
+/* Optional per-pass timing: while enabled, every ve_frame binds HIP events to
+ * the first and last kernel dispatch of each pass (no host synchronisation,
+ * no marker packets); ve_timings waits for the streams and returns, per pass,
+ * the mean milliseconds per frame over the frames recorded since the last
+ * call (trace and shade: all bounce iterations of a frame together).  An
  * `enable` above 1 is the number of frames about to be recorded; their events
- * are created up front so the recording itself only enqueues markers. */
+ * are created up front. */
 int ve_enable_timing(ve_ctx *ctx, int enable);
 int ve_timings(ve_ctx *ctx, float ms_out[VE_PASS_COUNT]);
 

@@ -247,6 +247,9 @@ public:
         check(ve_scene_update(ctx_, scene_, o.data.data(), o.size, o.changed.data(), o.changed.size()), ctx_);
     }
     void setCamera(const CameraUBO &u) { check(ve_set_camera(ctx_, u.v), ctx_); }
+    // Cross-frame overlap of the primary pass (on by default; off = the
+    // reference's strict pass order, same results).
+    void setOverlap(bool on) { check(ve_set_overlap(ctx_, on ? 1 : 0), ctx_); }
     void setMaterials(const void *std140, uint32_t n) { check(ve_set_materials(ctx_, std140, n), ctx_); }
     void setSky(const uint8_t rgb[3])
     {

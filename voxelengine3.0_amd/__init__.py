@@ -145,6 +145,7 @@ def lib():
         "ve_buffer_size": (i32, [p, i32, C.POINTER(sz)]),
         "ve_sync": (i32, [p]),
         "ve_enable_timing": (i32, [p, i32]),
+        "ve_set_overlap": (i32, [p, i32]),
         "ve_timings": (i32, [p, p]),
         "ve_enable_node_stats": (i32, [p, i32]),
         "ve_node_stats": (i32, [p] + [C.POINTER(C.c_uint64)] * 4),
@@ -452,6 +453,10 @@ class Renderer:
 
     def reset_ray_totals(self):
         self._ck(lib().ve_reset_ray_totals(self.h))
+
+    def set_overlap(self, on=True):
+        """Cross-frame overlap of the primary pass (on by default)."""
+        self._ck(lib().ve_set_overlap(self.h, 1 if on else 0))
 
     def enable_timing(self, on=True):
         """on: True/False, or the number of frames about to be recorded."""

@@ -82,11 +82,17 @@ struct Alloc {
     }
 };
 
-void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, uint32_t hash_width)
+void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, uint32_t hash_width, bool overlap)
 {
     const size_t P = (size_t)max_w * max_h;
     const size_t groups_max = (size_t)((max_w + 7) / 8) * ((max_h + 7) / 8);
-    a(&b.gbuf, P); a(&b.gpos, P); a(&b.output, P); a(&b.display, P); a(&b.pix_avg, P); a(&b.pix_cnt, P); a(&b.timeouts, P);
+    for (int k = 0; k < (overlap ? 2 : 1); k++) {
+        a(&b.gbuf_set[k], P);
+        a(&b.gpos_set[k], P);
+    }
+    b.gbuf = b.gbuf_set[0];
+    b.gpos = b.gpos_set[0];
+    a(&b.output, P); a(&b.display, P); a(&b.pix_avg, P); a(&b.pix_cnt, P); a(&b.timeouts, P);
     a(&b.nbuf, (size_t)hash_width * NBUFFER_STRIDE * NBUFFER_SLOTS);
     a(&b.lbuf, (size_t)hash_width * LBUFFER_STRIDE * LBUFFER_SLOTS);
     a(&b.rbuf, (size_t)hash_width * RBUFFER_STRIDE * RBUFFER_SLOTS);
@@ -318,7 +324,7 @@ int ve_create(ve_ctx **out, const ve_renderer_config *cfg)
 {
     ve_clear_global_error();
     std::string err;
-    const int rc = ve::ctx_create(out, cfg, nullptr, err);
+    const int rc = ve::ctx_create(out, cfg, nullptr, true, err);
     return rc == VE_OK ? VE_OK : ve_global_error(rc, err.c_str());
 }
 
@@ -326,11 +332,15 @@ void ve_destroy(ve_ctx *c)
 {
     if (!c) return;
     (void)hipSetDevice(c->device);
+    if (c->aux) (void)hipStreamSynchronize(c->aux);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto &a : c->allocations) (void)hipFree(a.first);
     for (auto &fe : c->frame_events)
         for (hipEvent_t e : fe)
             if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {c->ev_primary, c->ev_emit, c->ev_fence})
+        if (e) (void)hipEventDestroy(e);
+    if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->stream && c->owns_stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -339,6 +349,7 @@ int ve_scene_upload(ve_ctx *c, const uint32_t *words, uint32_t n_words, uint32_t
 {
     if (!c) return ve_global_error(VE_EINVAL, "null context");
     std::string err;
+    c->aux_fence = true;   // the primary pass reads scenes; it follows the upload
     const int rc = ve::scene_upload(c->device, c->stream, words, n_words, depth, scene, err);
     return rc == VE_OK ? VE_OK : ctx_error(c, rc, "%s", err.c_str());
 }
@@ -350,6 +361,7 @@ int ve_scene_update(ve_ctx *c, ve_scene *s, const uint32_t *words, uint32_t n_wo
     if (!s || !words || n_words == 0 || (n_slots && !slots)) return ctx_error(c, VE_EINVAL, "null argument");
     if (n_words < s->n_words) return ctx_error(c, VE_EINVAL, "node array shrank from %u to %u words", s->n_words, n_words);
     HIP_OK(c, hipSetDevice(c->device));
+    c->aux_fence = true;   // the next overlapped primary pass reads the edited words
     if (n_words > s->alloc_words) {   // grow: a new array with headroom, whole image copied
         for (uint32_t i = 0; i < n_words; i++)
             if ((words[i] & 1u) && (uint64_t)(words[i] >> 4) + 8u > n_words)
@@ -452,15 +464,17 @@ int ve_set_sky(ve_ctx *c, const uint8_t *const faces[6], int edge)
 namespace {
 
 // Enqueue one pass; every launch goes on the context's stream.
-int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p)
+// The primary pass runs on `primary_stream` (the context's stream unless a
+// whole frame overlaps it with the previous one).
+int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, hipStream_t primary_stream = nullptr)
 {
     int rc = VE_OK;
     switch (pass) {
     case VE_PASS_PRIMARY:
-        launch_primary(c->stream, s->words, p, c->b, c->node_stats);
+        launch_primary(primary_stream ? primary_stream : c->stream, s->words, p, c->b, c->node_stats);
         break;
     case VE_PASS_NORMAL: launch_normal(c->stream, s->words, p, c->b); break;
-    case VE_PASS_RESET: launch_reset(c->stream, c->b); break;
+    case VE_PASS_RESET: launch_reset(c->stream, p, c->b); break;
     case VE_PASS_SCHEDULE: rc = launch_schedule(c->stream, p, c->b); break;
     case VE_PASS_RANK: rc = launch_rank(c->stream, p, c->b); break;
     case VE_PASS_THRESHOLD: launch_threshold(c->stream, p, c->b); break;
@@ -488,9 +502,8 @@ int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p)
 }
 
 // Pass-timing events only carry timestamps (ve_timings reads them after a
-// stream synchronisation), so they skip the system-scope release: that
-// release writes back and invalidates the caches at every pass boundary and
-// stalls the next pass for about 10 us.
+// stream synchronisation) and are bound to the pass's kernel dispatches, so
+// they need no system-scope release of their own.
 constexpr unsigned TIMING_EVENT_FLAGS = hipEventDisableSystemFence;
 
 // Event slot of the frame being recorded (timing mode only).
@@ -505,14 +518,79 @@ hipEvent_t *frame_slot(ve_ctx *c)
     return c->frame_events[c->frames_recorded].data();
 }
 
-int timed_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, uint32_t iter, hipEvent_t *ev)
+int timed_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, uint32_t iter, hipEvent_t *ev,
+               hipStream_t primary_stream = nullptr)
 {
-    if (!ev) return run_pass(c, s, pass, p);
+    if (!ev) return run_pass(c, s, pass, p, primary_stream);
     hipEvent_t *pair = ev + 2 * (pass * MAX_ITERS + iter);
-    HIP_OK(c, hipEventRecord(pair[0], c->stream));
-    int rc = run_pass(c, s, pass, p);
-    HIP_OK(c, hipEventRecord(pair[1], c->stream));
+    pass_timer = PassTimer{pair[0], pair[1]};
+    const int rc = run_pass(c, s, pass, p, primary_stream);
+    pass_timer = PassTimer{};
     return rc;
+}
+
+// renderer.cpp:282-649 without its host readbacks: counts stay on the device
+// and every kernel sizes its own work from them.  With cross-frame overlap
+// the primary pass goes to the auxiliary stream, into the other G-buffer set,
+// once the previous frame's emit pass is done: it then runs beside that
+// frame's trace/shade/resolve passes, which leave most of the device idle
+// (their time is the tail of the longest rays).  The set's last readers
+// belong to the frame before last, whose passes all precede that emit pass
+// on the main stream.  The main stream waits for it before the normal pass.
+int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &p)
+{
+    hipEvent_t *ev = nullptr;
+    if (c->timing && !(ev = frame_slot(c))) return ctx_error(c, VE_EDEVICE, "hipEventCreate failed");
+    int rc;
+    if (c->overlap) {
+        const uint32_t k = c->gsel ^ 1u;
+        c->gsel = k;
+        c->b.gbuf = c->b.gbuf_set[k];
+        c->b.gpos = c->b.gpos_set[k];
+        if (c->emit_done) HIP_OK(c, hipStreamWaitEvent(c->aux, c->emit_done, 0));
+        if (c->aux_fence) {
+            HIP_OK(c, hipEventRecord(c->ev_fence, c->stream));
+            HIP_OK(c, hipStreamWaitEvent(c->aux, c->ev_fence, 0));
+            c->aux_fence = false;
+        }
+        if ((rc = timed_pass(c, s, VE_PASS_PRIMARY, p, 0, ev, c->aux))) return rc;
+        HIP_OK(c, hipEventRecord(c->ev_primary, c->aux));
+        if ((rc = timed_pass(c, s, VE_PASS_RESET, p, 0, ev))) return rc;
+        HIP_OK(c, hipStreamWaitEvent(c->stream, c->ev_primary, 0));
+    } else {
+        if ((rc = timed_pass(c, s, VE_PASS_RESET, p, 0, ev))) return rc;
+        if ((rc = timed_pass(c, s, VE_PASS_PRIMARY, p, 0, ev))) return rc;
+    }
+    const ve_pass_id head[] = {VE_PASS_NORMAL, VE_PASS_SCHEDULE, VE_PASS_RANK, VE_PASS_THRESHOLD};
+    for (ve_pass_id id : head)
+        if ((rc = timed_pass(c, s, id, p, 0, ev))) return rc;
+    if (c->overlap) {
+        // the emit kernel's own dispatch marks its completion: the timing
+        // event when the pass is timed, ev_emit otherwise
+        if (ev) {
+            if ((rc = timed_pass(c, s, VE_PASS_EMIT, p, 0, ev))) return rc;
+            c->emit_done = ev[2 * (VE_PASS_EMIT * MAX_ITERS) + 1];
+        } else {
+            pass_timer = PassTimer{nullptr, c->ev_emit};
+            rc = run_pass(c, s, VE_PASS_EMIT, p);
+            pass_timer = PassTimer{};
+            if (rc) return rc;
+            c->emit_done = c->ev_emit;
+        }
+    } else if ((rc = timed_pass(c, s, VE_PASS_EMIT, p, 0, ev))) {
+        return rc;
+    }
+    const uint32_t iters = std::min<uint32_t>(c->max_bounces, (uint32_t)MAX_ITERS);   // renderer.cpp:515
+    for (uint32_t i = 0; i < iters; i++) {
+        if ((rc = timed_pass(c, s, VE_PASS_TRACE, p, i, ev))) return rc;
+        if ((rc = timed_pass(c, s, VE_PASS_SHADE, p, i, ev))) return rc;
+    }
+    if ((rc = timed_pass(c, s, VE_PASS_RESOLVE, p, 0, ev))) return rc;
+    if (ev) {
+        c->iters_recorded = iters;
+        c->frames_recorded++;
+    }
+    return VE_OK;
 }
 
 }  // namespace
@@ -526,6 +604,9 @@ int ve_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const ve_frame_config *fc, 
     HIP_OK(c, hipSetDevice(c->device));
     FrameParams p;
     fill_params(c, s, fc, in, p);
+    // an overlapped primary pass must not run ahead of a pass that may touch
+    // the G-buffers; the display pass reads only the resolved image
+    if (pass != VE_PASS_DISPLAY) c->aux_fence = true;
     return run_pass(c, s, pass, p);
 }
 
@@ -536,24 +617,11 @@ int ve_frame(ve_ctx *c, ve_scene *s, const ve_frame_config *fc, const ve_frame_i
     HIP_OK(c, hipSetDevice(c->device));
     FrameParams p;
     fill_params(c, s, fc, in, p);
-    // renderer.cpp:282-649 without its host readbacks: counts stay on the
-    // device and every kernel sizes its own work from them.
-    hipEvent_t *ev = nullptr;
-    if (c->timing && !(ev = frame_slot(c))) return ctx_error(c, VE_EDEVICE, "hipEventCreate failed");
-    const ve_pass_id head[] = {VE_PASS_RESET, VE_PASS_PRIMARY, VE_PASS_NORMAL, VE_PASS_SCHEDULE,
-                               VE_PASS_RANK, VE_PASS_THRESHOLD, VE_PASS_EMIT};
-    for (ve_pass_id id : head)
-        if ((rc = timed_pass(c, s, id, p, 0, ev))) return rc;
-    const uint32_t iters = std::min<uint32_t>(c->max_bounces, (uint32_t)MAX_ITERS);   // renderer.cpp:515
-    for (uint32_t i = 0; i < iters; i++) {
-        if ((rc = timed_pass(c, s, VE_PASS_TRACE, p, i, ev))) return rc;
-        if ((rc = timed_pass(c, s, VE_PASS_SHADE, p, i, ev))) return rc;
-    }
-    rc = timed_pass(c, s, VE_PASS_RESOLVE, p, 0, ev);
-    if (ev) {
-        c->iters_recorded = iters;
-        c->frames_recorded++;
-    }
+    p.count_primary_in_reset = 1;
+    rc = frame_passes(c, s, p);
+    // a frame left half-enqueued may still read either G-buffer set: the next
+    // primary pass waits for everything on the stream
+    if (rc) c->aux_fence = true;
     return rc;
 }
 
@@ -592,6 +660,7 @@ int ve_write(ve_ctx *c, ve_buffer_id id, const void *src, size_t bytes)
     BufView v = buffer_view(c, id);
     if (!v.ptr) return ctx_error(c, VE_ENOTFOUND, "unknown buffer id %d", (int)id);
     if (!src || bytes > v.bytes) return ctx_error(c, VE_EINVAL, "write of %zu bytes to a %zu-byte buffer", bytes, v.bytes);
+    c->aux_fence = true;
     HIP_OK(c, hipSetDevice(c->device));
     if (is_hash_buffer(id)) {   // accepted in the reference layout; whole image only
         if (bytes != v.bytes) return ctx_error(c, VE_EINVAL, "hash images are written whole (%zu bytes)", v.bytes);
@@ -611,6 +680,17 @@ int ve_sync(ve_ctx *c)
     if (!c) return ve_global_error(VE_EINVAL, "null context");
     HIP_OK(c, hipSetDevice(c->device));
     HIP_OK(c, hipStreamSynchronize(c->stream));
+    if (c->aux) HIP_OK(c, hipStreamSynchronize(c->aux));
+    return VE_OK;
+}
+
+int ve_set_overlap(ve_ctx *c, int enable)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    if (enable && !c->aux) return ctx_error(c, VE_EINVAL, "context has no auxiliary stream for overlap");
+    c->overlap = enable != 0;
+    c->emit_done = nullptr;
+    c->aux_fence = true;
     return VE_OK;
 }
 
@@ -635,6 +715,7 @@ int ve_timings(ve_ctx *c, float ms_out[VE_PASS_COUNT])
     if (!c) return ve_global_error(VE_EINVAL, "null context");
     if (!ms_out) return ctx_error(c, VE_EINVAL, "null output");
     HIP_OK(c, hipSetDevice(c->device));
+    if (c->aux) HIP_OK(c, hipStreamSynchronize(c->aux));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     double sum[VE_PASS_COUNT] = {0};
     for (size_t f = 0; f < c->frames_recorded; f++)
@@ -710,11 +791,11 @@ size_t ctx_device_bytes(uint32_t max_w, uint32_t max_h, uint32_t hash_width)
 {
     Alloc a;
     DeviceBuffers b;
-    plan_buffers(a, b, max_w, max_h, hash_width ? hash_width : 16384u);
+    plan_buffers(a, b, max_w, max_h, hash_width ? hash_width : 16384u, false);
     return a.bytes;
 }
 
-int ctx_create(ve_ctx **out, const ve_renderer_config *cfg, hipStream_t shared, std::string &err)
+int ctx_create(ve_ctx **out, const ve_renderer_config *cfg, hipStream_t shared, bool overlap, std::string &err)
 {
     if (!out || !cfg) return err = "null argument", VE_EINVAL;
     if (cfg->max_width == 0 || cfg->max_height == 0) return err = "zero-sized framebuffer", VE_EINVAL;
@@ -737,16 +818,26 @@ int ctx_create(ve_ctx **out, const ve_renderer_config *cfg, hipStream_t shared, 
             c->owns_stream = false;
         } else {
             e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);   // default priority
+            if (e == hipSuccess && overlap) {
+                c->overlap = true;
+                e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);   // default priority
+                // ordering only, between two streams of one device
+                const unsigned fl = hipEventDisableTiming | hipEventReleaseToDevice;
+                if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_primary, fl);
+                // bound to the emit kernel's dispatch, like the pass-timing events
+                if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_emit, hipEventDisableSystemFence);
+                if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fence, fl);
+            }
         }
     }
     if (e != hipSuccess) {
         err = hipGetErrorString(e);
-        delete c;
+        ve_destroy(c);
         return VE_EDEVICE;
     }
     Alloc a;
     a.c = c;
-    plan_buffers(a, c->b, c->max_w, c->max_h, c->hash_width);
+    plan_buffers(a, c->b, c->max_w, c->max_h, c->hash_width, c->overlap);
     if (a.rc == VE_OK) {
         e = hipMemcpyAsync(c->b.rank_stats, INITIAL_THRESHOLDS, sizeof INITIAL_THRESHOLDS, hipMemcpyHostToDevice,
                            c->stream);
@@ -774,6 +865,11 @@ int ctx_reset(ve_ctx *c)
     c->bounce_iter = 0;
     c->frames_recorded = 0;
     c->timing = false;
+    c->gsel = 0;
+    c->b.gbuf = c->b.gbuf_set[0];
+    c->b.gpos = c->b.gpos_set[0];
+    c->emit_done = nullptr;
+    c->aux_fence = true;
     c->node_stats = false;
     std::memset(c->cam, 0, sizeof c->cam);
     std::memset(c->sky, 0, sizeof c->sky);

@@ -47,6 +47,18 @@ struct ve_ctx {
     size_t frames_recorded = 0;
     uint32_t iters_recorded = 0;
     ve::DeviceBuffers b;
+    // Cross-frame overlap (contexts that own their stream): the primary pass,
+    // which reads only the camera and the octree, runs on `aux` into the other
+    // G-buffer set while the previous frame's latency-bound bounce passes
+    // still run on `stream`.
+    bool overlap = false;
+    hipStream_t aux = nullptr;
+    uint32_t gsel = 0;                      // G-buffer set of the latest frame
+    hipEvent_t ev_primary = nullptr;        // on aux: primary pass written
+    hipEvent_t ev_emit = nullptr;           // on stream: emit pass of the latest frame done
+    hipEvent_t emit_done = nullptr;         // the event that marks it (ev_emit, or its timing event)
+    hipEvent_t ev_fence = nullptr;          // on stream: everything enqueued so far
+    bool aux_fence = true;                  // the next primary must follow all of stream's work
     std::vector<std::pair<void *, size_t>> allocations;
     size_t device_bytes = 0;
 };
@@ -56,7 +68,9 @@ namespace ve {
 // Device bytes a context of this shape allocates.
 size_t ctx_device_bytes(uint32_t max_w, uint32_t max_h, uint32_t hash_width);
 // ve_create on a given stream (null: the context creates its own).
-int ctx_create(ve_ctx **out, const ve_renderer_config *cfg, hipStream_t shared, std::string &err);
+// `overlap`: give the context a second stream and G-buffer set for
+// cross-frame overlap (ignored with a shared stream).
+int ctx_create(ve_ctx **out, const ve_renderer_config *cfg, hipStream_t shared, bool overlap, std::string &err);
 // Back to the state of a fresh context (all buffers zero, initial RankStats),
 // enqueued on the context's stream without waiting.
 int ctx_reset(ve_ctx *c);

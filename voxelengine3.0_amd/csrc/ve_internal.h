@@ -111,6 +111,10 @@ struct FrameParams {
     float firefly_k, firefly_floor;
     int32_t render_type;
     int32_t reconstruction_radius;
+    // 1 inside a whole frame: the reset pass counts the frame's primary rays
+    // (the primary pass may run ahead on another stream); 0 for passes run
+    // one by one, where the primary pass counts them itself
+    int32_t count_primary_in_reset;
     float cam[16];
     float sky[3];
 };

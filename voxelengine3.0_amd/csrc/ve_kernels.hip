@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_primary(const uint32_t *__restrict__ wo
                                                  unsigned long long *__restrict__ stats, uint32_t *__restrict__ counters,
                                                  unsigned long long *__restrict__ totals)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {   // one primary ray per pixel
+    if (!p.count_primary_in_reset && blockIdx.x == 0 && threadIdx.x == 0) {   // one primary ray per pixel
         counters[VE_CNT_PRIMARY_RAYS] = p.W * p.H;
         totals[0] += (unsigned long long)p.W * p.H;
     }
@@ -91,10 +91,10 @@ void launch_primary(hipStream_t s, const uint32_t *words, const FrameParams &p, 
     const uint32_t tiles = ((p.W + TILE - 1) / TILE) * ((p.H + TILE - 1) / TILE);
     const uint32_t nb = padded_blocks((tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
     if (stats)
-        hipLaunchKernelGGL(k_primary<true>, dim3(nb), dim3(256), 0, s, words, p, b.gbuf, b.gpos, b.node_stats,
+        launch(k_primary<true>, dim3(nb), dim3(256), 0, s, words, p, b.gbuf, b.gpos, b.node_stats,
                            b.counters, b.ray_totals);
     else
-        hipLaunchKernelGGL(k_primary<false>, dim3(nb), dim3(256), 0, s, words, p, b.gbuf, b.gpos, b.node_stats,
+        launch(k_primary<false>, dim3(nb), dim3(256), 0, s, words, p, b.gbuf, b.gpos, b.node_stats,
                            b.counters, b.ray_totals);
 }
 

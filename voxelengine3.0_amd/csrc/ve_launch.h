@@ -1,6 +1,7 @@
 // Host-side launch wrappers for the kernels in ve_kernels.hip.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -8,10 +9,28 @@
 
 namespace ve {
 
+// Timing of one pass without marker packets in the stream: the events are
+// bound to the pass's own dispatches.  The first kernel launched records
+// `start` as it begins; every kernel records `stop` as it ends, so the last
+// one's stands.  Set around a timed pass, null otherwise.
+struct PassTimer {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local PassTimer pass_timer;
+
+template <typename K, typename... A>
+inline void launch(K kernel, dim3 grid, dim3 block, uint32_t shared_bytes, hipStream_t s, A... args)
+{
+    hipExtLaunchKernelGGL(kernel, grid, block, shared_bytes, s, pass_timer.start, pass_timer.stop, 0u, args...);
+    pass_timer.start = nullptr;
+}
+
 // Device buffers of one context (all owned by the context).
 struct DeviceBuffers {
-    uint4 *gbuf = nullptr;          // P x (checks, material, 0, vid)
-    uint32_t *gpos = nullptr;       // P
+    uint4 *gbuf = nullptr;          // P x (checks, material, 0, vid): the current set
+    uint32_t *gpos = nullptr;       // P: the current set
+    uint4 *gbuf_set[2] = {nullptr, nullptr};     // second set only with cross-frame overlap
+    uint32_t *gpos_set[2] = {nullptr, nullptr};
     float4 *output = nullptr;       // P
     uchar4 *display = nullptr;      // P: tonemapped RGBA8
     float4 *pix_avg = nullptr;      // P: per-pixel voxel average (rgb, indirect variance)
@@ -48,7 +67,7 @@ struct DeviceBuffers {
 void launch_primary(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, bool stats);
 void launch_fill(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
 void launch_normal(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b);
-void launch_reset(hipStream_t s, DeviceBuffers &b);
+void launch_reset(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 int launch_schedule(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 int launch_rank(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 void launch_emit(hipStream_t s, const FrameParams &p, DeviceBuffers &b);

@@ -26,6 +26,9 @@
 
 namespace ve {
 
+thread_local PassTimer pass_timer;
+
+
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr int TILE = 8;
 constexpr int NUM_XCD = 8;
@@ -1575,28 +1578,31 @@ static uint32_t screen_blocks(const FrameParams &p)
 
 void launch_fill(hipStream_t s, uint32_t *p, uint32_t v, size_t n)
 {
-    if (n) hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(n, 256)), dim3(256), 0, s, p, v, n);
+    if (n) launch(k_fill_u32, dim3(cdiv(n, 256)), dim3(256), 0, s, p, v, n);
 }
 
 void launch_normal(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b)
 {
-    hipLaunchKernelGGL(k_normal, dim3(screen_blocks(p)), dim3(256), 0, s, words, p, b.gbuf, b.gpos, b.nbuf);
+    launch(k_normal, dim3(screen_blocks(p)), dim3(256), 0, s, words, p, b.gbuf, b.gpos, b.nbuf);
 }
 
 // Per-frame clear of RankStats' per-frame half and the counters the frame
 // accumulates into (renderer.cpp:343-375); fill kernels, not memsets, so the
 // sequence stays capturable.
-__global__ void k_frame_reset(uint32_t *__restrict__ rank_stats, uint32_t *__restrict__ counters)
+__global__ void k_frame_reset(FrameParams p, uint32_t *__restrict__ rank_stats, uint32_t *__restrict__ counters,
+                              unsigned long long *__restrict__ totals)
 {
     const uint32_t i = threadIdx.x;
+    const uint32_t primary = p.count_primary_in_reset ? p.W * p.H : 0u;   // one primary ray per pixel
     if (i < 4) rank_stats[4 + i] = i == 0 ? 0x7F7FFFFFu : 0u;
     if (i < HISTOGRAM_BINS) rank_stats[8 + i] = 0u;
-    if (i < NUM_COUNTERS && i != VE_CNT_EMISSIVES) counters[i] = 0u;
+    if (i < NUM_COUNTERS && i != VE_CNT_EMISSIVES) counters[i] = i == VE_CNT_PRIMARY_RAYS ? primary : 0u;
+    if (i == 0) totals[0] += primary;
 }
 
-void launch_reset(hipStream_t s, DeviceBuffers &b)
+void launch_reset(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 {
-    hipLaunchKernelGGL(k_frame_reset, dim3(1), dim3(256), 0, s, b.rank_stats, b.counters);
+    launch(k_frame_reset, dim3(1), dim3(256), 0, s, p, b.rank_stats, b.counters, b.ray_totals);
 }
 
 int launch_schedule(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
@@ -1604,13 +1610,13 @@ int launch_schedule(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
     const uint32_t gw = (p.W + 7u) / 8u, gh = (p.H + 7u) / 8u;
     const uint32_t groups = gw * gh;
     const uint32_t n_vec = groups * 8u;
-    hipLaunchKernelGGL(k_sched_claim, dim3(groups), dim3(64), 0, s, p, b.gbuf, b.sched_tab, b.cand_at, b.counters);
-    hipLaunchKernelGGL(k_sched_rank, dim3(CLAIM_WIDTH), dim3(64), 0, s, b.sched_tab, b.claim, b.samples, b.cand_at);
-    hipLaunchKernelGGL(k_sched_vec_count, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, b.cand_at, n_vec, b.vec_cnt);
+    launch(k_sched_claim, dim3(groups), dim3(64), 0, s, p, b.gbuf, b.sched_tab, b.cand_at, b.counters);
+    launch(k_sched_rank, dim3(CLAIM_WIDTH), dim3(64), 0, s, b.sched_tab, b.claim, b.samples, b.cand_at);
+    launch(k_sched_vec_count, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, b.cand_at, n_vec, b.vec_cnt);
     size_t tb = b.scan_temp_bytes;
     if (hipcub::DeviceScan::ExclusiveSum(b.scan_temp, tb, b.vec_cnt, b.vec_base, (int)n_vec, s) != hipSuccess)
         return VE_EDEVICE;
-    hipLaunchKernelGGL(k_sched_write, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, p, b.cand_at, n_vec, b.vec_cnt,
+    launch(k_sched_write, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, p, b.cand_at, n_vec, b.vec_cnt,
                        b.vec_base, b.claim, b.candidates, b.counters);
     return VE_OK;
 }
@@ -1620,24 +1626,24 @@ int launch_rank(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
     // candidates <= distinct visible voxels <= pixels
     const uint32_t bound = (uint32_t)std::min<size_t>((size_t)p.W * p.H, CANDIDATE_CAPACITY);
     const uint32_t n_vec = cdiv(bound, 8);
-    hipLaunchKernelGGL(k_rank, dim3(std::min<uint32_t>(cdiv(bound, 256), RANK_BLOCKS)), dim3(256), 0, s, p, b.candidates,
+    launch(k_rank, dim3(std::min<uint32_t>(cdiv(bound, 256), RANK_BLOCKS)), dim3(256), 0, s, p, b.candidates,
                        b.samples, b.lbuf, b.rank_stats, b.counters, b.nsamp, b.vec_cnt, n_vec);
     size_t tb = b.scan_temp_bytes;
     if (hipcub::DeviceScan::ExclusiveSum(b.scan_temp, tb, b.vec_cnt, b.vec_base, (int)n_vec, s) != hipSuccess)
         return VE_EDEVICE;
-    hipLaunchKernelGGL(k_rank_write, dim3(cdiv(bound, 256)), dim3(256), 0, s, p, b.candidates, b.nsamp, b.vec_cnt,
+    launch(k_rank_write, dim3(cdiv(bound, 256)), dim3(256), 0, s, p, b.candidates, b.nsamp, b.vec_cnt,
                        b.vec_base, b.shade_list, b.counters);
     return VE_OK;
 }
 
 void launch_threshold(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 {
-    hipLaunchKernelGGL(k_threshold, dim3(1), dim3(64), 0, s, p, b.rank_stats, b.counters);
+    launch(k_threshold, dim3(1), dim3(64), 0, s, p, b.rank_stats, b.counters);
 }
 
 void launch_emit(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 {
-    hipLaunchKernelGGL(k_emit, dim3(cdiv(p.shade_budget, 256)), dim3(256), 0, s, p, b.shade_list, b.rays, b.counters);
+    launch(k_emit, dim3(cdiv(p.shade_budget, 256)), dim3(256), 0, s, p, b.shade_list, b.rays, b.counters);
 }
 
 // Bounce iteration `iter` of the trace/shade loop (renderer.cpp:515-602); the
@@ -1657,22 +1663,22 @@ void launch_trace(hipStream_t s, const uint32_t *words, const FrameParams &p, De
 {
     const uint32_t nb = cdiv(p.shade_budget, 256);
     if (stats)
-        hipLaunchKernelGGL(k_trace<true>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, iter), live_in_ptr(b, iter),
+        launch(k_trace<true>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, iter), live_in_ptr(b, iter),
                            iter, b.counters, b.node_stats, b.ray_totals, b.materials, b.emissive_claim, b.em_bits,
                            live_out_ptr(b, iter));
     else
-        hipLaunchKernelGGL(k_trace<false>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, iter),
+        launch(k_trace<false>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, iter),
                            live_in_ptr(b, iter), iter, b.counters, b.node_stats, b.ray_totals, b.materials,
                            b.emissive_claim, b.em_bits, live_out_ptr(b, iter));
     if (iter == 0)
-        hipLaunchKernelGGL(k_em_register, dim3(1), dim3(1024), 0, s, bounce_in(b, iter), live_in_ptr(b, iter),
+        launch(k_em_register, dim3(1), dim3(1024), 0, s, bounce_in(b, iter), live_in_ptr(b, iter),
                            b.materials, b.emissive_claim, b.emissives, b.em_bits, b.em_reg, b.counters);
 }
 
 void launch_shade(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t iter)
 {
     const uint32_t nb = cdiv(p.shade_budget, 256);
-    hipLaunchKernelGGL(k_shade, dim3(nb), dim3(256), 0, s, p, words, iter, bounce_in(b, iter), live_in_ptr(b, iter),
+    launch(k_shade, dim3(nb), dim3(256), 0, s, p, words, iter, bounce_in(b, iter), live_in_ptr(b, iter),
                        bounce_out(b, iter), live_out_ptr(b, iter), b.lbuf, b.nbuf, b.rbuf, b.materials, b.emissives,
                        b.em_reg, b.counters, b.ray_totals);
 }
@@ -1686,22 +1692,22 @@ __global__ void k_scatter_words(uint32_t *__restrict__ words, const uint2 *__res
 
 void launch_scatter_words(hipStream_t s, uint32_t *words, const uint2 *pairs, uint32_t n)
 {
-    if (n) hipLaunchKernelGGL(k_scatter_words, dim3(cdiv(n, 256)), dim3(256), 0, s, words, pairs, n);
+    if (n) launch(k_scatter_words, dim3(cdiv(n, 256)), dim3(256), 0, s, words, pairs, n);
 }
 
 void launch_display(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 {
     const uint32_t n = p.W * p.H;
-    hipLaunchKernelGGL(k_display, dim3(cdiv(n, 256)), dim3(256), 0, s, n, b.output, b.display);
+    launch(k_display, dim3(cdiv(n, 256)), dim3(256), 0, s, n, b.output, b.display);
 }
 
 void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 {
-    hipLaunchKernelGGL(k_resolve_avg, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.pix_avg, b.pix_cnt,
+    launch(k_resolve_avg, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.pix_avg, b.pix_cnt,
                        b.counters);
-    hipLaunchKernelGGL(k_resolve, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.nbuf, b.materials,
+    launch(k_resolve, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.nbuf, b.materials,
                        b.pix_avg, b.pix_cnt, b.output, b.timeouts, b.counters);
-    hipLaunchKernelGGL(k_resolve_timeout, dim3(RESOLVE_TIMEOUT_BLOCKS), dim3(256), 0, s, p, b.gbuf, b.pix_avg,
+    launch(k_resolve_timeout, dim3(RESOLVE_TIMEOUT_BLOCKS), dim3(256), 0, s, p, b.gbuf, b.pix_avg,
                        b.pix_cnt, b.timeouts, b.counters, b.output);
 }
 

@@ -194,7 +194,7 @@ int acquire(ve_server *s, Session &ss)
     hipStream_t q = nullptr;
     if (!s->queues.empty()) q = s->queues[s->next_queue++ % s->queues.size()];
     std::string err;
-    const int rc = ve::ctx_create(&ss.ctx, &cfg, q, err);
+    const int rc = ve::ctx_create(&ss.ctx, &cfg, q, false, err);
     if (rc != VE_OK) return srv_error(s, rc, "session %u: %s", ss.id, err.c_str());
     s->used += ve::ctx_bytes(ss.ctx);
     return VE_OK;

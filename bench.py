@@ -104,7 +104,7 @@ def cpu_baseline(words, depth, ubos, fc_kw, budget_s=20.0):
 # offered loads (frames/s) of the served sweep, per mode (each row also
 # reports the load its Poisson realisation actually offered): the plain loop
 # saturates near 650 frames/s, the concurrent server near 1300
-SERVED_LAMS = {0: [700, 900, 1100, 1200, 1300, 1400, 1500], 1: [300, 400, 450, 500, 550]}
+SERVED_LAMS = {0: [900, 1200, 1500, 1700, 1900, 2100, 2300], 1: [350, 450, 500, 550, 600]}
 
 
 def served_and_batched(ve, extreme):

@@ -9,7 +9,10 @@ Renderer::runWavefrontFrame: primary rays, normals, schedule, rank, threshold,
 emit, the trace/shade bounce loop and resolve.
 
 value = (primary rays + traced bounce rays + NEE shadow rays) of the timed
-frames / wall time of the timed region (inputs resident in HBM; the host only
+frames / wall time of the timed region, counting every live ray of each
+trace pass as the reference does (the bounce-0 camera rays, whose hit records
+come from the primary pass's traversal, are listed as reused_rays_per_frame,
+and mrays_per_s_traversed_only leaves them out) (inputs resident in HBM; the host only
 enqueues; frames as an application renders them: the context's default
 cross-frame overlap, where a frame's primary pass runs on a second stream
 beside the previous frame's bounce passes, and no instrumentation).  The same
@@ -280,6 +283,10 @@ def main():
             "rays_per_frame": round(rays / args.steps, 1),
             "primary_rays_per_frame": W * H,
             "traced_rays_per_frame": round(totals["traced"] / args.steps, 1),
+            # of the traced rays, the bounce-0 camera rays whose hit record the
+            # emit pass took from the primary pass (the same traversal)
+            "reused_rays_per_frame": round(totals["reused"] / args.steps, 1),
+            "mrays_per_s_traversed_only": round((rays - totals["reused"]) / elapsed / 1e6, 3),
             "overlap": "primary pass of frame k+1 beside the bounce passes of frame k",
             "pass_ms": {k: round(v, 4) for k, v in pass_ms.items()},
             "pass_ms_serial": {k: round(v, 4) for k, v in pass_ms_serial.items()},

@@ -204,7 +204,7 @@ typedef enum ve_buffer_id {
 #define VE_CNT_BOUNCES_RUN    7   /* trace/shade iterations with live rays */
 #define VE_CNT_LIVE_B         8   /* live-ray count of the other ray buffer (internal) */
 #define VE_CNT_OVERFLOW       9   /* nonzero: a claim row overflowed the ordering table */
-/* 10: unused */
+#define VE_CNT_REUSED        10   /* bounce-0 rays this frame whose hit record came from the primary pass */
 #define VE_CNT_TIMEOUTS      11   /* pixels whose primary ray timed out (last resolve) */
 #define VE_CNT_EM_BASE       12   /* emissive registry size before this frame's registrations (internal) */
 #define VE_CNT_EM_NEW        13   /* registrations made by this frame's first shade (internal) */
@@ -237,9 +237,16 @@ int ve_enable_timing(ve_ctx *ctx, int enable);
 int ve_timings(ve_ctx *ctx, float ms_out[VE_PASS_COUNT]);
 
 /* Cumulative ray counts since the context was created (or ve_reset_ray_totals):
- * primary rays (W*H per primary pass), rays traced by the bounce passes, and
- * NEE shadow rays.  Kept on the device; reading synchronises the stream. */
+ * primary rays (W*H per primary pass), rays traced by the bounce passes (every
+ * live ray of each trace dispatch, as the reference counts them), and NEE
+ * shadow rays.  Kept on the device; reading synchronises the stream. */
 int ve_ray_totals(ve_ctx *ctx, uint64_t *primary, uint64_t *traced, uint64_t *shadow);
+/* Of the traced rays, those that were not traversed: inside ve_frame a
+ * bounce-0 ray is the pixel's camera ray, which the primary pass already
+ * traversed to the same voxel within the bounce budget, so the emit pass
+ * fills in its hit record from the primary pass (identical to a traversal).
+ * No counterpart in the reference, which traverses it again. */
+int ve_ray_totals_reused(ve_ctx *ctx, uint64_t *reused);
 int ve_reset_ray_totals(ve_ctx *ctx);
 
 /* Node-read instrumentation: when enabled, the primary and trace kernels add

@@ -36,7 +36,7 @@ PASS_NAMES = [k for k, _ in sorted(PASS.items(), key=lambda kv: kv[1])]
 BUF = dict(GBUFFER=0, GBUFFER_POS=1, NBUFFER=2, LBUFFER=3, RBUFFER=4, CANDIDATES=5, SHADE_LIST=6, RAYS=7,
            OUTPUT=8, COUNTERS=9, RANK_STATS=10, SAMPLE_COUNTS=11, CLAIM_MAP=12, EMISSIVES=13, DISPLAY=14)
 CNT = dict(CANDIDATES=0, SHADE=1, EMISSIVES=2, LIVE=3, RAYS_TRACED=4, SHADOW_RAYS=5, PRIMARY_RAYS=6,
-           BOUNCES_RUN=7, LIVE_B=8, OVERFLOW=9, TIMEOUTS=11, EM_BASE=12, EM_NEW=13)
+           BOUNCES_RUN=7, LIVE_B=8, OVERFLOW=9, REUSED=10, TIMEOUTS=11, EM_BASE=12, EM_NEW=13)
 
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("pad0", "<f4"), ("direction", "<f4", 3), ("pad1", "<f4"),
                       ("throughput", "<f4", 3), ("primary_vid", "<u4"), ("direct", "<f4", 3), ("rng", "<u4"),
@@ -150,6 +150,7 @@ def lib():
         "ve_enable_node_stats": (i32, [p, i32]),
         "ve_node_stats": (i32, [p] + [C.POINTER(C.c_uint64)] * 4),
         "ve_ray_totals": (i32, [p] + [C.POINTER(C.c_uint64)] * 3),
+        "ve_ray_totals_reused": (i32, [p, C.POINTER(C.c_uint64)]),
         "ve_reset_ray_totals": (i32, [p]),
         "ve_octree_cpu_create": (i32, [C.POINTER(p), u32, u32]),
         "ve_octree_cpu_destroy": (None, [p]),
@@ -449,7 +450,11 @@ class Renderer:
     def ray_totals(self):
         v = [C.c_uint64() for _ in range(3)]
         self._ck(lib().ve_ray_totals(self.h, *[C.byref(x) for x in v]))
-        return dict(primary=v[0].value, traced=v[1].value, shadow=v[2].value)
+        reused = C.c_uint64()
+        self._ck(lib().ve_ray_totals_reused(self.h, C.byref(reused)))
+        # traced counts every live ray of each trace pass (the reference's
+        # count); reused of them took the primary pass's hit record
+        return dict(primary=v[0].value, traced=v[1].value, shadow=v[2].value, reused=reused.value)
 
     def reset_ray_totals(self):
         self._ck(lib().ve_reset_ray_totals(self.h))

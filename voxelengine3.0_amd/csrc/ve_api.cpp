@@ -89,9 +89,11 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     for (int k = 0; k < (overlap ? 2 : 1); k++) {
         a(&b.gbuf_set[k], P);
         a(&b.gpos_set[k], P);
+        a(&b.ghit_set[k], P);
     }
     b.gbuf = b.gbuf_set[0];
     b.gpos = b.gpos_set[0];
+    b.ghit = b.ghit_set[0];
     a(&b.output, P); a(&b.display, P); a(&b.pix_avg, P); a(&b.pix_cnt, P); a(&b.timeouts, P);
     a(&b.nbuf, (size_t)hash_width * NBUFFER_STRIDE * NBUFFER_SLOTS);
     a(&b.lbuf, (size_t)hash_width * LBUFFER_STRIDE * LBUFFER_SLOTS);
@@ -108,9 +110,10 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.emissive_claim, (size_t)EMISSIVE_CLAIM_W * EMISSIVE_CLAIM_PROBES);
     a(&b.em_bits, SHADE_BUDGET_MAX / 32);
     a(&b.em_reg, EMISSIVE_CAPACITY);
+    a(&b.hit_bits, SHADE_BUDGET_MAX / 32);
     a(&b.materials, MATERIAL_SLOTS);
     a(&b.node_stats, 4);
-    a(&b.ray_totals, 3);
+    a(&b.ray_totals, 4);
     a(&b.sched_tab, (size_t)CLAIM_WIDTH * SCHED_SLOTS);
     a(&b.cand_at, groups_max * 64);
     a(&b.vec_cnt, groups_max * 8);
@@ -547,6 +550,7 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &p)
         c->gsel = k;
         c->b.gbuf = c->b.gbuf_set[k];
         c->b.gpos = c->b.gpos_set[k];
+        c->b.ghit = c->b.ghit_set[k];
         if (c->emit_done) HIP_OK(c, hipStreamWaitEvent(c->aux, c->emit_done, 0));
         if (c->aux_fence) {
             HIP_OK(c, hipEventRecord(c->ev_fence, c->stream));
@@ -618,6 +622,7 @@ int ve_frame(ve_ctx *c, ve_scene *s, const ve_frame_config *fc, const ve_frame_i
     FrameParams p;
     fill_params(c, s, fc, in, p);
     p.count_primary_in_reset = 1;
+    p.reuse_primary_hits = c->node_stats ? 0 : 1;   // node counting traces every ray as the reference does
     rc = frame_passes(c, s, p);
     // a frame left half-enqueued may still read either G-buffer set: the next
     // primary pass waits for everything on the stream
@@ -747,11 +752,23 @@ int ve_ray_totals(ve_ctx *c, uint64_t *primary, uint64_t *traced, uint64_t *shad
     return VE_OK;
 }
 
+int ve_ray_totals_reused(ve_ctx *c, uint64_t *reused)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    if (!reused) return ctx_error(c, VE_EINVAL, "null output");
+    unsigned long long v = 0;
+    HIP_OK(c, hipSetDevice(c->device));
+    HIP_OK(c, hipMemcpyAsync(&v, c->b.ray_totals + 3, sizeof v, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    *reused = v;
+    return VE_OK;
+}
+
 int ve_reset_ray_totals(ve_ctx *c)
 {
     if (!c) return ve_global_error(VE_EINVAL, "null context");
     HIP_OK(c, hipSetDevice(c->device));
-    HIP_OK(c, hipMemsetAsync(c->b.ray_totals, 0, 3 * sizeof(unsigned long long), c->stream));
+    HIP_OK(c, hipMemsetAsync(c->b.ray_totals, 0, 4 * sizeof(unsigned long long), c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     return VE_OK;
 }
@@ -868,6 +885,7 @@ int ctx_reset(ve_ctx *c)
     c->gsel = 0;
     c->b.gbuf = c->b.gbuf_set[0];
     c->b.gpos = c->b.gpos_set[0];
+    c->b.ghit = c->b.ghit_set[0];
     c->emit_done = nullptr;
     c->aux_fence = true;
     c->node_stats = false;

@@ -115,6 +115,10 @@ struct FrameParams {
     // (the primary pass may run ahead on another stream); 0 for passes run
     // one by one, where the primary pass counts them itself
     int32_t count_primary_in_reset;
+    // 1 inside a whole frame (without node counting): the primary pass keeps
+    // each hit's lattice position and the emit pass fills in the hit record
+    // of a bounce-0 ray whose camera ray is the pixel's primary ray
+    int32_t reuse_primary_hits;
     float cam[16];
     float sky[3];
 };

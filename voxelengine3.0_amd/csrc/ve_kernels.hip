@@ -40,7 +40,7 @@ template <bool STATS>
 __global__ __launch_bounds__(256) void k_primary(const uint32_t *__restrict__ words, FrameParams p,
                                                  uint4 *__restrict__ gbuf, uint32_t *__restrict__ gpos,
                                                  unsigned long long *__restrict__ stats, uint32_t *__restrict__ counters,
-                                                 unsigned long long *__restrict__ totals)
+                                                 unsigned long long *__restrict__ totals, uint2 *__restrict__ ghit)
 {
     if (!p.count_primary_in_reset && blockIdx.x == 0 && threadIdx.x == 0) {   // one primary ray per pixel
         counters[VE_CNT_PRIMARY_RAYS] = p.W * p.H;
@@ -66,6 +66,7 @@ __global__ __launch_bounds__(256) void k_primary(const uint32_t *__restrict__ wo
             const uint32_t bits = p.depth - 1u;
             gbuf[pix] = make_uint4(checks, h.material, 0u, voxel_key(h.px, h.py, h.pz) + 1u);
             gpos[pix] = (h.px >> 1) | ((h.py >> 1) << bits) | ((h.pz >> 1) << (2u * bits));
+            if (p.reuse_primary_hits) ghit[pix] = make_uint2(h.px | (h.py << 16), h.pz);
         } else {
             gbuf[pix] = make_uint4(checks, 0u, 0u, 0u);
             gpos[pix] = 0u;
@@ -92,10 +93,10 @@ void launch_primary(hipStream_t s, const uint32_t *words, const FrameParams &p, 
     const uint32_t nb = padded_blocks((tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
     if (stats)
         launch(k_primary<true>, dim3(nb), dim3(256), 0, s, words, p, b.gbuf, b.gpos, b.node_stats,
-                           b.counters, b.ray_totals);
+                           b.counters, b.ray_totals, b.ghit);
     else
         launch(k_primary<false>, dim3(nb), dim3(256), 0, s, words, p, b.gbuf, b.gpos, b.node_stats,
-                           b.counters, b.ray_totals);
+                           b.counters, b.ray_totals, b.ghit);
 }
 
 }  // namespace ve

@@ -29,8 +29,10 @@ inline void launch(K kernel, dim3 grid, dim3 block, uint32_t shared_bytes, hipSt
 struct DeviceBuffers {
     uint4 *gbuf = nullptr;          // P x (checks, material, 0, vid): the current set
     uint32_t *gpos = nullptr;       // P: the current set
+    uint2 *ghit = nullptr;          // P: the primary hit's lattice position (x | y << 16, z), current set
     uint4 *gbuf_set[2] = {nullptr, nullptr};     // second set only with cross-frame overlap
     uint32_t *gpos_set[2] = {nullptr, nullptr};
+    uint2 *ghit_set[2] = {nullptr, nullptr};
     float4 *output = nullptr;       // P
     uchar4 *display = nullptr;      // P: tonemapped RGBA8
     float4 *pix_avg = nullptr;      // P: per-pixel voxel average (rgb, indirect variance)
@@ -51,9 +53,10 @@ struct DeviceBuffers {
     uint32_t *emissive_claim = nullptr;  // EMISSIVE_CLAIM_W x EMISSIVE_CLAIM_PROBES
     uint32_t *em_bits = nullptr;         // SHADE_BUDGET_MAX / 32: first-shade rays that may register a light
     uint32_t *em_reg = nullptr;          // EMISSIVE_CAPACITY: ray index of each registration this frame
+    uint32_t *hit_bits = nullptr;        // SHADE_BUDGET_MAX / 32: bounce-0 rays whose hit record emit filled in
     Material *materials = nullptr;       // MATERIAL_SLOTS
     unsigned long long *node_stats = nullptr;  // 4: primary reads, trace reads, primary steps, trace steps
-    unsigned long long *ray_totals = nullptr;  // 3: primary, traced, shadow (cumulative)
+    unsigned long long *ray_totals = nullptr;  // 4: primary, traced, shadow, traced with a reused hit (cumulative)
     // schedule / rank ordering scratch
     uint4 *sched_tab = nullptr;          // CLAIM_WIDTH x SCHED_SLOTS: {voxel id, ~lowest execution rank, pixels, 0}
     uint32_t *cand_at = nullptr;         // per execution rank: claim row/probe of the voxel it first claims

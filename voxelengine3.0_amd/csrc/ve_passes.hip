@@ -604,35 +604,6 @@ __global__ __launch_bounds__(64) void k_threshold(FrameParams p, uint32_t *__res
 // ---------------------------------------------------------------------------
 // emit.comp: one ray per shade-list entry, ray index = list index
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_emit(FrameParams p, const uint2 *__restrict__ shade_list,
-                                              Ray *__restrict__ rays, uint32_t *__restrict__ counters)
-{
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = counters[VE_CNT_SHADE];
-    if (t == 0) counters[VE_CNT_LIVE] = n;
-    if (t >= n) return;
-    const uint2 en = shade_list[t];
-    const uint32_t px = en.y % p.W, py = en.y / p.W;
-    const f3 d = primary_dir(p.cam, px, py, p.W, p.H);
-    Ray r;
-    r.ox = p.cam[0]; r.oy = p.cam[1]; r.oz = p.cam[2]; r.pad0 = 0.0f;
-    r.dx = d.x; r.dy = d.y; r.dz = d.z; r.pad1 = 0.0f;
-    r.tx = r.ty = r.tz = 1.0f;
-    r.primary_vid = en.x;
-    r.dlx = r.dly = r.dlz = 0.0f;
-    const uint32_t sample_idx = p.frame_idx + en.y * 2654435761u + t * 1442695040u;
-    r.rng = (sample_idx << 8) & 0xFFFFFF00u;
-    r.ilx = r.ily = r.ilz = 0.0f;
-    r.bounce = 0u;
-    r.hit_vid = 0u;
-    r.hit_material = 0u;
-    r.pad2 = r.pad3 = 0u;
-    rays[t] = r;
-}
-
-// ---------------------------------------------------------------------------
-// trace.comp
-// ---------------------------------------------------------------------------
 // Bounce-0 rays that hit an emissive voxel the claim map does not hold yet
 // set their bit in `em_bits`; k_em_register then registers them in ray order.
 __device__ __forceinline__ bool emissive_unknown(const uint32_t *__restrict__ em_claim, uint32_t vid)
@@ -646,6 +617,76 @@ __device__ __forceinline__ bool emissive_unknown(const uint32_t *__restrict__ em
     return false;   // every probe taken by other ids: the reference cannot register it either
 }
 
+// A bounce-0 ray starts at the camera through its pixel: it is the pixel's
+// primary ray, traversed by the primary pass with the same origin, direction
+// and DDA.  When that traversal hit this entry's voxel within the bounce
+// budget (a hit at step q is found by any budget >= q - 1), the trace pass
+// would reach the same hit, so emit writes the ray's post-trace record (hit
+// position, size 1, id, material, emissive flag) and marks it in `hit_bits`;
+// the trace pass then leaves it alone.  The ray still counts as traced (the
+// reference traverses it again); `VE_CNT_REUSED` and ray total 3 count it.
+__global__ __launch_bounds__(256) void k_emit(FrameParams p, const uint2 *__restrict__ shade_list,
+                                              Ray *__restrict__ rays, uint32_t *__restrict__ counters,
+                                              const uint4 *__restrict__ gbuf, const uint2 *__restrict__ ghit,
+                                              const Material *__restrict__ mats, const uint32_t *__restrict__ em_claim,
+                                              uint32_t *__restrict__ em_bits, uint32_t *__restrict__ hit_bits,
+                                              unsigned long long *__restrict__ totals)
+{
+    __shared__ uint32_t block_reused;
+    if (threadIdx.x == 0) block_reused = 0u;
+    __syncthreads();
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = counters[VE_CNT_SHADE];
+    if (t == 0) counters[VE_CNT_LIVE] = n;
+    bool reused = false;
+    if (t < n) {
+        const uint2 en = shade_list[t];
+        const uint32_t px = en.y % p.W, py = en.y / p.W;
+        const f3 d = primary_dir(p.cam, px, py, p.W, p.H);
+        Ray r;
+        r.ox = p.cam[0]; r.oy = p.cam[1]; r.oz = p.cam[2]; r.pad0 = 0.0f;
+        r.dx = d.x; r.dy = d.y; r.dz = d.z; r.pad1 = 0.0f;
+        r.tx = r.ty = r.tz = 1.0f;
+        r.primary_vid = en.x;
+        r.dlx = r.dly = r.dlz = 0.0f;
+        const uint32_t sample_idx = p.frame_idx + en.y * 2654435761u + t * 1442695040u;
+        r.rng = (sample_idx << 8) & 0xFFFFFF00u;
+        r.ilx = r.ily = r.ilz = 0.0f;
+        r.bounce = 0u;
+        r.hit_vid = 0u;
+        r.hit_material = 0u;
+        r.pad2 = r.pad3 = 0u;
+        if (p.reuse_primary_hits && p.max_bounces > 0u) {
+            const uint4 g = gbuf[en.y];
+            if (g.w == en.x && g.w != 0u && g.x - 1u <= p.bounce_cc) {
+                const uint2 hp = ghit[en.y];
+                r.ox = (float)(hp.x & 0xFFFFu);
+                r.oy = (float)(hp.x >> 16);
+                r.oz = (float)hp.y;
+                r.pad0 = 1.0f;
+                r.hit_vid = g.w;
+                r.hit_material = g.y;
+                reused = true;
+                if (mats[g.y < MATERIAL_SLOTS ? g.y : MATERIAL_SLOTS - 1u].emissive && emissive_unknown(em_claim, g.w))
+                    atomicOr(&em_bits[t >> 5], 1u << (t & 31u));
+            }
+        }
+        rays[t] = r;
+    }
+    const uint64_t m = __ballot(reused);
+    const uint32_t lane = threadIdx.x & 63u;
+    if ((lane & 31u) == 0u && t < SHADE_BUDGET_MAX) hit_bits[t >> 5] = (uint32_t)(m >> lane);
+    if (lane == 0u && m) atomicAdd(&block_reused, (uint32_t)__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0 && block_reused) {
+        atomicAdd(&counters[VE_CNT_REUSED], block_reused);
+        atomicAdd(&totals[3], (unsigned long long)block_reused);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// trace.comp
+// ---------------------------------------------------------------------------
 template <bool STATS>
 __global__ __launch_bounds__(256) void k_trace(const uint32_t *__restrict__ words, FrameParams p, Ray *__restrict__ rays,
                                                const uint32_t *__restrict__ live_ptr, uint32_t iter,
@@ -653,7 +694,7 @@ __global__ __launch_bounds__(256) void k_trace(const uint32_t *__restrict__ word
                                                unsigned long long *__restrict__ totals,
                                                const Material *__restrict__ mats,
                                                const uint32_t *__restrict__ em_claim, uint32_t *__restrict__ em_bits,
-                                               uint32_t *__restrict__ live_out)
+                                               uint32_t *__restrict__ live_out, const uint32_t *__restrict__ hit_bits)
 {
     const uint32_t live = *live_ptr;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -667,7 +708,9 @@ __global__ __launch_bounds__(256) void k_trace(const uint32_t *__restrict__ word
     }
     __shared__ uint32_t path_cache[VE_MAX_DEPTH * 256];
     uint32_t checks = 0, reads = 0;
-    if (t < live) {
+    // bounce-0 rays whose hit record the emit pass already filled in
+    const bool filled = iter == 0u && t < live && ((hit_bits[t >> 5] >> (t & 31u)) & 1u);
+    if (t < live && !filled) {
         Ray &r = rays[t];
         if (r.bounce < p.max_bounces) {
             const f3 o{r.ox, r.oy, r.oz}, d{r.dx, r.dy, r.dz};
@@ -1643,7 +1686,8 @@ void launch_threshold(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 
 void launch_emit(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 {
-    launch(k_emit, dim3(cdiv(p.shade_budget, 256)), dim3(256), 0, s, p, b.shade_list, b.rays, b.counters);
+    launch(k_emit, dim3(cdiv(p.shade_budget, 256)), dim3(256), 0, s, p, b.shade_list, b.rays, b.counters, b.gbuf, b.ghit,
+           b.materials, b.emissive_claim, b.em_bits, b.hit_bits, b.ray_totals);
 }
 
 // Bounce iteration `iter` of the trace/shade loop (renderer.cpp:515-602); the
@@ -1665,11 +1709,11 @@ void launch_trace(hipStream_t s, const uint32_t *words, const FrameParams &p, De
     if (stats)
         launch(k_trace<true>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, iter), live_in_ptr(b, iter),
                            iter, b.counters, b.node_stats, b.ray_totals, b.materials, b.emissive_claim, b.em_bits,
-                           live_out_ptr(b, iter));
+                           live_out_ptr(b, iter), b.hit_bits);
     else
         launch(k_trace<false>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, iter),
                            live_in_ptr(b, iter), iter, b.counters, b.node_stats, b.ray_totals, b.materials,
-                           b.emissive_claim, b.em_bits, live_out_ptr(b, iter));
+                           b.emissive_claim, b.em_bits, live_out_ptr(b, iter), b.hit_bits);
     if (iter == 0)
         launch(k_em_register, dim3(1), dim3(1024), 0, s, bounce_in(b, iter), live_in_ptr(b, iter),
                            b.materials, b.emissive_claim, b.emissives, b.em_bits, b.em_reg, b.counters);

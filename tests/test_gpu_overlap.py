@@ -73,3 +73,51 @@ def test_overlap_matches_serial_order(ve, scenes):
         assert np.array_equal(overlapped[name][0], serial[name][0]), name
         assert np.array_equal(overlapped[name][1], serial[name][1]), name
 
+
+
+@pytest.mark.parametrize("bounce_cc", [120, 20])
+def test_reused_bounce0_hits_equal_traversal(ve, scenes, small_scene, bounce_cc):
+    """Bounce-0 rays take their hit record from the primary pass when the
+    pixel's primary ray hit the voxel within the bounce budget; a context with
+    node counting on traverses every ray as the reference does.  With a bounce
+    budget below the primary one, the rays whose primary hit needed more steps
+    fall back to a traversal (and time out the same way)."""
+    knobs = dict(DET_KNOBS, bounce_controlchecks=bounce_cc)
+    res = []
+    for traverse_all in (False, True):
+        rend = ve.Renderer(W, H, hash_width=4096, max_bounces=3)
+        try:
+            rend.set_materials(scenes.palette(1))
+            rend.set_sky_constant((140, 180, 230))
+            rend.enable_node_stats(traverse_all)
+            sc = rend.upload(small_scene["words"], small_scene["depth"], small_scene["info"].size)
+            e, d, fov = scenes.camera_small()
+            fc = ve.FrameConfig(**knobs)
+            reused = []
+            for f in range(4):
+                rend.set_camera(ve.camera_ubo(e, d, fov, W / H))
+                rend.frame(sc, fc, ve.FrameInputs(W, H, f, frame_seed(f), f + 1))
+                c = rend.counters()
+                reused.append((int(c[ve.CNT["REUSED"]]), int(c[ve.CNT["SHADE"]])))
+            print("case reuse bounce_cc", bounce_cc, "traverse_all", traverse_all, reused, flush=True)
+            out = {name: rend.read(name).copy() for name in ("OUTPUT", "COUNTERS", "SHADE_LIST")}
+            out["LBUFFER"] = hash_map(rend.read("LBUFFER"), 12, 32)
+            totals = rend.ray_totals()
+            rend.release(sc)
+            res.append((out, reused, totals))
+        finally:
+            rend.close()
+    (a, ra, ta), (b, rb, tb) = res
+    assert all(r == 0 for r, _ in rb)                      # node counting: every ray traversed
+    assert all(0 < r <= s for r, s in ra)
+    if bounce_cc >= 70:
+        assert all(r == s for r, s in ra)                  # every shade-list pixel hit within 70 steps
+    else:
+        assert any(r < s for r, s in ra)                   # some primary hits took more than 21 steps
+    assert ta["traced"] == tb["traced"] and ta["primary"] == tb["primary"] and ta["reused"] == sum(r for r, _ in ra)
+    for name in ("OUTPUT", "SHADE_LIST"):
+        assert np.array_equal(a[name], b[name]), name
+    ca, cb = a["COUNTERS"].copy(), b["COUNTERS"].copy()
+    ca[ve.CNT["REUSED"]] = cb[ve.CNT["REUSED"]] = 0
+    assert np.array_equal(ca, cb)
+    assert np.array_equal(a["LBUFFER"][0], b["LBUFFER"][0]) and np.array_equal(a["LBUFFER"][1], b["LBUFFER"][1])

@@ -121,3 +121,61 @@ def test_reused_bounce0_hits_equal_traversal(ve, scenes, small_scene, bounce_cc)
     ca[ve.CNT["REUSED"]] = cb[ve.CNT["REUSED"]] = 0
     assert np.array_equal(ca, cb)
     assert np.array_equal(a["LBUFFER"][0], b["LBUFFER"][0]) and np.array_equal(a["LBUFFER"][1], b["LBUFFER"][1])
+
+
+def test_frame_equals_pass_by_pass(ve, scenes, small_scene):
+    """ve_frame -- primary pass overlapped with the previous frame, bounce-0
+    hits taken from the primary pass, bounce iterations 1.. fused per ray with
+    their deposits grouped per lBuffer bucket and applied in iteration order
+    -- renders what the reference's one dispatch per pass and per bounce
+    iteration renders (Renderer::runWavefrontFrame, renderer.cpp:282-649),
+    frame after frame.  Four bounces, so two iterations' deposits are held
+    back and applied in order."""
+    bounces = 4
+    fc_kw = dict(DET_KNOBS)
+    out = []
+    for by_pass in (False, True):
+        rend = ve.Renderer(W, H, hash_width=4096, max_bounces=bounces)
+        try:
+            rend.set_materials(scenes.palette(1))
+            rend.set_sky_constant((140, 180, 230))
+            sc = rend.upload(small_scene["words"], small_scene["depth"], small_scene["info"].size)
+            e, d, fov = scenes.camera_small()
+            fc = ve.FrameConfig(**fc_kw)
+            for f in range(5):
+                eye = np.asarray(e, np.float32) + np.float32(2.0 * f) * np.array([0.5, 0.0, 1.0], np.float32)
+                rend.set_camera(ve.camera_ubo(eye, d, fov, W / H))
+                inputs = ve.FrameInputs(W, H, f, frame_seed(f), f + 1)
+                print("case frame paths by_pass", by_pass, "frame", f, flush=True)
+                if by_pass:
+                    for name in ("RESET", "PRIMARY", "NORMAL", "SCHEDULE", "RANK", "THRESHOLD", "EMIT"):
+                        rend.run_pass(sc, name, fc, inputs)
+                    for _ in range(bounces):
+                        rend.run_pass(sc, "TRACE", fc, inputs)
+                        rend.run_pass(sc, "SHADE", fc, inputs)
+                    rend.run_pass(sc, "RESOLVE", fc, inputs)
+                else:
+                    rend.frame(sc, fc, inputs)
+            res = {name: rend.read(name).copy() for name in ("OUTPUT", "COUNTERS", "CANDIDATES", "SHADE_LIST")}
+            for name, (stride, slots) in HASHED.items():
+                res[name] = hash_map(rend.read(name), stride, slots)
+            res["totals"] = rend.ray_totals()
+            rend.release(sc)
+            out.append(res)
+        finally:
+            rend.close()
+    fused, by_pass = out
+    assert fused["totals"]["reused"] > 0 and by_pass["totals"]["reused"] == 0
+    for k in ("primary", "traced", "shadow"):
+        assert fused["totals"][k] == by_pass["totals"][k], k
+    ca, cb = fused["COUNTERS"].copy(), by_pass["COUNTERS"].copy()
+    assert int(cb[ve.CNT["BOUNCES_RUN"]]) >= 3
+    for k in ("LIVE", "LIVE_B", "REUSED"):   # bookkeeping of the per-iteration dispatch / of the reuse
+        ca[ve.CNT[k]] = cb[ve.CNT[k]] = 0
+    ca[14] = cb[14] = 0                    # deposits held back by the fused pass
+    assert np.array_equal(ca, cb)
+    for name in ("OUTPUT", "CANDIDATES", "SHADE_LIST"):
+        assert np.array_equal(fused[name], by_pass[name]), name
+    for name in HASHED:
+        assert np.array_equal(fused[name][0], by_pass[name][0]), name
+        assert np.array_equal(fused[name][1], by_pass[name][1]), name

@@ -111,6 +111,11 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.em_bits, SHADE_BUDGET_MAX / 32);
     a(&b.em_reg, EMISSIVE_CAPACITY);
     a(&b.hit_bits, SHADE_BUDGET_MAX / 32);
+    a(&b.deferred, SHADE_BUDGET_MAX);
+    a(&b.deferred_sorted, SHADE_BUDGET_MAX);
+    a(&b.bin_count, hash_width);
+    a(&b.bin_fill, hash_width);
+    a(&b.bin_off, hash_width);
     a(&b.materials, MATERIAL_SLOTS);
     a(&b.node_stats, 4);
     a(&b.ray_totals, 4);
@@ -119,7 +124,7 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.vec_cnt, groups_max * 8);
     a(&b.vec_base, groups_max * 8);
     a(&b.nsamp, std::min<size_t>(P, CANDIDATE_CAPACITY));
-    b.scan_temp_bytes = scan_temp_bytes((uint32_t)(groups_max * 8));
+    b.scan_temp_bytes = scan_temp_bytes((uint32_t)std::max<size_t>(groups_max * 8, hash_width));
     a(reinterpret_cast<uint8_t **>(&b.scan_temp), b.scan_temp_bytes);
 }
 
@@ -585,13 +590,24 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &p)
         return rc;
     }
     const uint32_t iters = std::min<uint32_t>(c->max_bounces, (uint32_t)MAX_ITERS);   // renderer.cpp:515
-    for (uint32_t i = 0; i < iters; i++) {
-        if ((rc = timed_pass(c, s, VE_PASS_TRACE, p, i, ev))) return rc;
-        if ((rc = timed_pass(c, s, VE_PASS_SHADE, p, i, ev))) return rc;
+    // iteration 0 (emissive registration, reservoirs) as its own passes; the
+    // later ones fused per ray, timed as trace / shade of iteration 1
+    if (iters > 0) {
+        if ((rc = timed_pass(c, s, VE_PASS_TRACE, p, 0, ev))) return rc;
+        if ((rc = timed_pass(c, s, VE_PASS_SHADE, p, 0, ev))) return rc;
+    }
+    if (iters > 1) {
+        if (ev) pass_timer = PassTimer{ev[2 * (VE_PASS_TRACE * MAX_ITERS + 1)], ev[2 * (VE_PASS_TRACE * MAX_ITERS + 1) + 1]};
+        launch_bounce(c->stream, s->words, p, c->b, 1u, iters, c->node_stats);
+        if (ev) pass_timer = PassTimer{ev[2 * (VE_PASS_SHADE * MAX_ITERS + 1)], ev[2 * (VE_PASS_SHADE * MAX_ITERS + 1) + 1]};
+        rc = launch_deferred(c->stream, p, c->b, 1u, iters);
+        pass_timer = PassTimer{};
+        if (rc) return ctx_error(c, rc, "bounce deposits: device library call failed");
+        HIP_OK(c, hipGetLastError());
     }
     if ((rc = timed_pass(c, s, VE_PASS_RESOLVE, p, 0, ev))) return rc;
     if (ev) {
-        c->iters_recorded = iters;
+        c->iters_recorded = std::min<uint32_t>(iters, 2u);
         c->frames_recorded++;
     }
     return VE_OK;

@@ -34,7 +34,14 @@ constexpr uint32_t RANK_STATS_WORDS = 264;     // wavefront.glsl:243-255
 constexpr uint32_t NUM_COUNTERS = 16;
 constexpr uint32_t SCHED_SLOTS = 128;          // ordering table: voxels per claim row before overflow
 constexpr uint32_t MATERIAL_SLOTS = 127;       // shade.comp:90
-constexpr uint32_t RANK_BLOCKS = 1024;           // rank pass grid (candidates strided over it)
+constexpr uint32_t DEFERRED_BLOCKS = 256;      // deferred-deposit scatter grid (entries strided over it)
+// A deposit of the fused bounce pass (bounce iteration `iter`), applied
+// after it.
+struct Deferred {
+    uint32_t vid, iter;
+    float dl[3], il[3];
+};
+constexpr uint32_t RANK_BLOCKS = 1024;         // rank pass grid (candidates strided over it)
 constexpr uint32_t RESOLVE_TIMEOUT_BLOCKS = 1024; // 4 waves each, striding over the timed-out pixels
 
 // Device layout of one hash bucket (one column h of the reference's R32UI

@@ -54,6 +54,11 @@ struct DeviceBuffers {
     uint32_t *em_bits = nullptr;         // SHADE_BUDGET_MAX / 32: first-shade rays that may register a light
     uint32_t *em_reg = nullptr;          // EMISSIVE_CAPACITY: ray index of each registration this frame
     uint32_t *hit_bits = nullptr;        // SHADE_BUDGET_MAX / 32: bounce-0 rays whose hit record emit filled in
+    Deferred *deferred = nullptr;        // SHADE_BUDGET_MAX: deposits of the fused bounce pass, in append order
+    Deferred *deferred_sorted = nullptr; // SHADE_BUDGET_MAX: the same grouped by lBuffer bucket
+    uint32_t *bin_count = nullptr;       // hash_width: deposits per bucket (zero between frames)
+    uint32_t *bin_fill = nullptr;        // hash_width: entries placed per bucket by the scatter (zero between frames)
+    uint32_t *bin_off = nullptr;         // hash_width: exclusive scan of bin_count
     Material *materials = nullptr;       // MATERIAL_SLOTS
     unsigned long long *node_stats = nullptr;  // 4: primary reads, trace reads, primary steps, trace steps
     unsigned long long *ray_totals = nullptr;  // 4: primary, traced, shadow, traced with a reused hit (cumulative)
@@ -78,6 +83,11 @@ void launch_threshold(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 void launch_trace(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t iter,
                   bool stats);
 void launch_shade(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t iter);
+// Bounce iterations first..iters-1 fused per ray (one launch), then their
+// held-back deposits in iteration order.
+void launch_bounce(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t first,
+                   uint32_t iters, bool stats);
+int launch_deferred(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_t first, uint32_t iters);
 void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 void launch_display(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 void launch_scatter_words(hipStream_t s, uint32_t *words, const uint2 *pairs, uint32_t n);

@@ -687,6 +687,34 @@ __global__ __launch_bounds__(256) void k_emit(FrameParams p, const uint2 *__rest
 // ---------------------------------------------------------------------------
 // trace.comp
 // ---------------------------------------------------------------------------
+// trace.comp for ray t of the batch: the hit record (position, size, id,
+// material) or a miss; first-shade rays that hit an unregistered emissive
+// voxel are flagged for k_em_register.
+template <bool STATS>
+__device__ __forceinline__ void trace_ray(const uint32_t *__restrict__ words, const FrameParams &p, uint32_t iter,
+                                          uint32_t t, Ray &r, const Material *__restrict__ mats,
+                                          const uint32_t *__restrict__ em_claim, uint32_t *__restrict__ em_bits,
+                                          uint32_t *cache, uint32_t &checks, uint32_t &reads)
+{
+    if (r.bounce >= p.max_bounces) return;
+    const f3 o{r.ox, r.oy, r.oz}, d{r.dx, r.dy, r.dz};
+    const Hit h = raycast<STATS>(words, p.n_words, p.depth, o, d, rcp3(d), p.bounce_cc, checks, reads, cache, 256u);
+    if (h.hit) {
+        r.ox = (float)h.px;
+        r.oy = (float)h.py;
+        r.oz = (float)h.pz;
+        r.pad0 = (float)h.size;
+        r.hit_vid = voxel_key(h.px, h.py, h.pz) + 1u;
+        r.hit_material = h.material;
+        if (iter == 0u && mats[h.material < MATERIAL_SLOTS ? h.material : MATERIAL_SLOTS - 1u].emissive &&
+            emissive_unknown(em_claim, r.hit_vid))
+            atomicOr(&em_bits[t >> 5], 1u << (t & 31u));
+    } else {
+        r.hit_vid = 0u;
+        r.hit_material = 0u;
+    }
+}
+
 template <bool STATS>
 __global__ __launch_bounds__(256) void k_trace(const uint32_t *__restrict__ words, FrameParams p, Ray *__restrict__ rays,
                                                const uint32_t *__restrict__ live_ptr, uint32_t iter,
@@ -710,28 +738,8 @@ __global__ __launch_bounds__(256) void k_trace(const uint32_t *__restrict__ word
     uint32_t checks = 0, reads = 0;
     // bounce-0 rays whose hit record the emit pass already filled in
     const bool filled = iter == 0u && t < live && ((hit_bits[t >> 5] >> (t & 31u)) & 1u);
-    if (t < live && !filled) {
-        Ray &r = rays[t];
-        if (r.bounce < p.max_bounces) {
-            const f3 o{r.ox, r.oy, r.oz}, d{r.dx, r.dy, r.dz};
-            const Hit h = raycast<STATS>(words, p.n_words, p.depth, o, d, rcp3(d), p.bounce_cc, checks, reads,
-                                         path_cache + threadIdx.x, 256u);
-            if (h.hit) {
-                r.ox = (float)h.px;
-                r.oy = (float)h.py;
-                r.oz = (float)h.pz;
-                r.pad0 = (float)h.size;
-                r.hit_vid = voxel_key(h.px, h.py, h.pz) + 1u;
-                r.hit_material = h.material;
-                if (iter == 0u && mats[h.material < MATERIAL_SLOTS ? h.material : MATERIAL_SLOTS - 1u].emissive &&
-                    emissive_unknown(em_claim, r.hit_vid))
-                    atomicOr(&em_bits[t >> 5], 1u << (t & 31u));
-            } else {
-                r.hit_vid = 0u;
-                r.hit_material = 0u;
-            }
-        }
-    }
+    if (t < live && !filled) trace_ray<STATS>(words, p, iter, t, rays[t], mats, em_claim, em_bits,
+                                              path_cache + threadIdx.x, checks, reads);
     if (STATS) {
         unsigned long long a = reads, c = checks;
         for (int off = 32; off > 0; off >>= 1) {
@@ -792,13 +800,12 @@ __device__ void reset_lslot(uint32_t *bucket, uint32_t s, uint32_t vid, uint4 d,
 
 // depositSampleDual (shade.comp:162-257) = AccumPass: integer fixed-point sums
 // per voxel id in the lBuffer hash.
-__device__ void deposit(const FrameParams &p, uint32_t *__restrict__ lbuf, uint32_t vid, f3 dir_rgb, f3 ind_rgb)
+// One deposit into the lBuffer bucket of `vid`; `bucket` is the bucket in
+// device memory or a copy of it in LDS.
+__device__ void deposit_into(const FrameParams &p, uint32_t *bucket, uint32_t vid, f3 dir_rgb, f3 ind_rgb)
 {
-    if (vid == 0u) return;
     uint4 d = pack_rgb_l2(dir_rgb);
     uint4 e = pack_rgb_l2(ind_rgb);
-    const uint32_t h = vid % (p.hash_width - 1u) + 1u;
-    uint32_t *bucket = lbuf + (size_t)h * LB_WORDS;
     bool claimed;
     const uint32_t s = claim_slot(bucket, LBUFFER_SLOTS, vid, claimed);
     if (s >= LBUFFER_SLOTS) {
@@ -823,13 +830,31 @@ __device__ void deposit(const FrameParams &p, uint32_t *__restrict__ lbuf, uint3
         for (int f = 0; f <= 4; f++) atomicExch(&pl[f], __atomic_load_n(&pl[f], __ATOMIC_RELAXED) >> 1);
     if (__atomic_load_n(&pl[5], __ATOMIC_RELAXED) >= p.cap_indirect)
         for (int f = 5; f <= 9; f++) atomicExch(&pl[f], __atomic_load_n(&pl[f], __ATOMIC_RELAXED) >> 1);
-    firefly(pl, 0, d, p.firefly_k, p.firefly_floor);
-    firefly(pl, 5, e, p.firefly_k, p.firefly_floor);
-    atomicAdd(&pl[0], 1u); atomicAdd(&pl[1], d.x); atomicAdd(&pl[2], d.y); atomicAdd(&pl[3], d.z);
-    atomicAdd(&pl[4], d.w);
-    atomicAdd(&pl[5], 1u); atomicAdd(&pl[6], e.x); atomicAdd(&pl[7], e.y); atomicAdd(&pl[8], e.z);
-    atomicAdd(&pl[9], e.w);
+    // A sample of zero light is never clamped when the firefly threshold
+    // cannot be negative, and adding zero changes no sum: both are skipped.
+    const bool clamp_zero = p.firefly_k < 0.0f || p.firefly_floor < 0.0f;
+    const bool d_zero = (d.x | d.y | d.z) == 0u, e_zero = (e.x | e.y | e.z) == 0u;
+    if (!d_zero || clamp_zero) firefly(pl, 0, d, p.firefly_k, p.firefly_floor);
+    if (!e_zero || clamp_zero) firefly(pl, 5, e, p.firefly_k, p.firefly_floor);
+    atomicAdd(&pl[0], 1u);
+    if (d.x) atomicAdd(&pl[1], d.x);
+    if (d.y) atomicAdd(&pl[2], d.y);
+    if (d.z) atomicAdd(&pl[3], d.z);
+    if (d.w) atomicAdd(&pl[4], d.w);
+    atomicAdd(&pl[5], 1u);
+    if (e.x) atomicAdd(&pl[6], e.x);
+    if (e.y) atomicAdd(&pl[7], e.y);
+    if (e.z) atomicAdd(&pl[8], e.z);
+    if (e.w) atomicAdd(&pl[9], e.w);
     atomicExch(&bucket[LB_TIME + s], p.time);
+}
+
+__device__ __forceinline__ uint32_t lbuf_row(const FrameParams &p, uint32_t vid) { return vid % (p.hash_width - 1u) + 1u; }
+
+__device__ void deposit(const FrameParams &p, uint32_t *__restrict__ lbuf, uint32_t vid, f3 dir_rgb, f3 ind_rgb)
+{
+    if (vid == 0u) return;
+    deposit_into(p, lbuf + (size_t)lbuf_row(p, vid) * LB_WORDS, vid, dir_rgb, ind_rgb);
 }
 
 __device__ f3 lookup_normal(const uint32_t *__restrict__ nbuf, uint32_t hw, uint32_t vid, f3 rd)   // shade.comp:261-275
@@ -1257,6 +1282,87 @@ __device__ __forceinline__ uint32_t registry_seen(const uint32_t *__restrict__ c
     return counters[VE_CNT_EM_BASE] + lo;
 }
 
+// shade.comp for ray t of the batch at bounce iteration `iter`: NEE, the next
+// direction and Russian roulette.  Returns whether the ray continues (r holds
+// the next bounce's ray), ends with a deposit of its direct/indirect light
+// into its primary voxel (r's dl/il), or ends without one.
+enum : int { RAY_DONE = 0, RAY_SURVIVES = 1, RAY_DEPOSITS = 2 };
+
+__device__ __forceinline__ int shade_ray(const FrameParams &p, uint32_t iter, uint32_t t, Ray &r,
+                                         const uint32_t *__restrict__ nbuf, const Material *__restrict__ mats,
+                                         const uint32_t *__restrict__ em_reg, const uint32_t *__restrict__ counters,
+                                         NeeCtx &nee)
+{
+    const uint32_t MB = p.max_bounces;
+    if (r.bounce >= MB) return RAY_DONE;
+    if (r.hit_vid == 0u) {   // miss: sky * throughput into the indirect channel
+        r.ilx = r.ilx + p.sky[0] * r.tx;
+        r.ily = r.ily + p.sky[1] * r.ty;
+        r.ilz = r.ilz + p.sky[2] * r.tz;
+        return RAY_DEPOSITS;
+    }
+    const uint32_t mi = r.hit_material < MATERIAL_SLOTS ? r.hit_material : MATERIAL_SLOTS - 1u;
+    const Material &m = mats[mi];
+    const uint32_t hx = (uint32_t)r.ox, hy = (uint32_t)r.oy, hz = (uint32_t)r.oz;
+    const uint32_t hsize = (uint32_t)r.pad0;
+    const f3 rd{r.dx, r.dy, r.dz};
+    const f3 nrm = lookup_normal(nbuf, p.hash_width, r.hit_vid, rd);
+    const f3 albedo{m.color[0], m.color[1], m.color[2]};
+    if (m.emissive) {   // shade.comp:576-590 (registration: k_em_register)
+        const f3 em = vmin3(mul(muls(albedo, m.emissive_intensity), f3{r.tx, r.ty, r.tz}), f3{20, 20, 20});
+        if (r.bounce == 0u) { r.dlx += em.x; r.dly += em.y; r.dlz += em.z; }
+        else { r.ilx += em.x; r.ily += em.y; r.ilz += em.z; }
+    }
+    // next-event estimation (shade.comp:631-836)
+    const uint32_t count = iter == 0u ? registry_seen(counters, em_reg, t) : counters[VE_CNT_EMISSIVES];
+    if (!m.emissive && r.bounce < 2u && count > 0u) {
+        const uint32_t n_lights = count < EMISSIVE_CAPACITY ? count : EMISSIVE_CAPACITY;
+        if (p.restir_enabled && r.bounce == 0u)
+            nee_restir(p, nee, r, m, nrm, hx, hy, hz, hsize, n_lights, count);
+        else
+            nee_ris(p, nee, r, m, nrm, hx, hy, hz, n_lights);
+    }
+    if (r.bounce + 1u >= MB) return RAY_DEPOSITS;   // last allowed bounce
+    const bool use_ggx = m.metallic * (1.0f - m.roughness) > 0.5f;
+    uint32_t rng = r.rng;
+    f3 sdir, bt;
+    if (use_ggx) {
+        const f3 Vv{-rd.x, -rd.y, -rd.z};
+        const f3 H = sample_ggx_vndf(nrm, Vv, m.roughness, rng);
+        sdir = reflect3(rd, H);
+        if (dot(sdir, nrm) <= 0.0f) {
+            sdir = normalize(add(nrm, random_direction(rng)));
+            bt = evaluate_brdf(rd, sdir, nrm, albedo, m.roughness, m.metallic, m.specular);
+        } else {
+            const f3 F0 = add(muls(f3{0.04f, 0.04f, 0.04f}, 1.0f - m.metallic), muls(albedo, m.metallic));
+            bt = ggx_throughput(nrm, Vv, sdir, H, m.roughness, F0);
+        }
+    } else {
+        sdir = normalize(add(nrm, random_direction(rng)));
+        bt = evaluate_brdf(rd, sdir, nrm, albedo, m.roughness, m.metallic, m.specular);
+    }
+    r.rng = rng;
+    f3 ntp = mul(f3{r.tx, r.ty, r.tz}, bt);
+    bool dies = false;
+    if (r.bounce + 1u >= RR_START_BOUNCE) {   // Russian roulette (shade.comp:879-889)
+        const float pr = gclamp(gmax(ntp.x, gmax(ntp.y, ntp.z)), 0.05f, 0.95f);
+        const float xi = rand_lds(r.rng);
+        if (xi > pr) dies = true;
+        else ntp = divs(ntp, pr);
+    }
+    if (dies) return RAY_DEPOSITS;
+    // survivor (shade.comp:892-908)
+    r.ox = ((float)hx + 0.5f) + nrm.x;
+    r.oy = ((float)hy + 0.5f) + nrm.y;
+    r.oz = ((float)hz + 0.5f) + nrm.z;
+    r.dx = sdir.x; r.dy = sdir.y; r.dz = sdir.z;
+    r.tx = ntp.x; r.ty = ntp.y; r.tz = ntp.z;
+    r.bounce = r.bounce + 1u;
+    r.hit_vid = 0u;
+    r.hit_material = 0u;
+    return RAY_SURVIVES;
+}
+
 __global__ __launch_bounds__(256) void k_shade(FrameParams p, const uint32_t *__restrict__ words, uint32_t iter,
                                                const Ray *__restrict__ rays_in, const uint32_t *__restrict__ live_in,
                                                Ray *__restrict__ rays_out, uint32_t *__restrict__ live_out,
@@ -1270,85 +1376,13 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, const uint32_t *__
     NeeCtx nee{words, emissives, rbuf, path_cache + threadIdx.x, 0u};
     const uint32_t live = *live_in;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t MB = p.max_bounces;
     bool survive = false;
     Ray r;
     if (t < live) {
         r = rays_in[t];
-        if (r.bounce < MB) {
-            if (r.hit_vid == 0u) {   // miss: sky * throughput into the indirect channel
-                r.ilx = r.ilx + p.sky[0] * r.tx;
-                r.ily = r.ily + p.sky[1] * r.ty;
-                r.ilz = r.ilz + p.sky[2] * r.tz;
-                deposit(p, lbuf, r.primary_vid, f3{r.dlx, r.dly, r.dlz}, f3{r.ilx, r.ily, r.ilz});
-            } else {
-                const uint32_t mi = r.hit_material < MATERIAL_SLOTS ? r.hit_material : MATERIAL_SLOTS - 1u;
-                const Material &m = mats[mi];
-                const uint32_t hx = (uint32_t)r.ox, hy = (uint32_t)r.oy, hz = (uint32_t)r.oz;
-                const uint32_t hsize = (uint32_t)r.pad0;
-                const f3 rd{r.dx, r.dy, r.dz};
-                const f3 nrm = lookup_normal(nbuf, p.hash_width, r.hit_vid, rd);
-                const f3 albedo{m.color[0], m.color[1], m.color[2]};
-                if (m.emissive) {   // shade.comp:576-590 (registration: k_em_register)
-                    const f3 em = vmin3(mul(muls(albedo, m.emissive_intensity), f3{r.tx, r.ty, r.tz}), f3{20, 20, 20});
-                    if (r.bounce == 0u) { r.dlx += em.x; r.dly += em.y; r.dlz += em.z; }
-                    else { r.ilx += em.x; r.ily += em.y; r.ilz += em.z; }
-                }
-                // next-event estimation (shade.comp:631-836)
-                const uint32_t count = iter == 0u ? registry_seen(counters, em_reg, t) : counters[VE_CNT_EMISSIVES];
-                if (!m.emissive && r.bounce < 2u && count > 0u) {
-                    const uint32_t n_lights = count < EMISSIVE_CAPACITY ? count : EMISSIVE_CAPACITY;
-                    if (p.restir_enabled && r.bounce == 0u)
-                        nee_restir(p, nee, r, m, nrm, hx, hy, hz, hsize, n_lights, count);
-                    else
-                        nee_ris(p, nee, r, m, nrm, hx, hy, hz, n_lights);
-                }
-                if (r.bounce + 1u >= MB) {   // last allowed bounce
-                    deposit(p, lbuf, r.primary_vid, f3{r.dlx, r.dly, r.dlz}, f3{r.ilx, r.ily, r.ilz});
-                } else {
-                    const bool use_ggx = m.metallic * (1.0f - m.roughness) > 0.5f;
-                    uint32_t rng = r.rng;
-                    f3 sdir, bt;
-                    if (use_ggx) {
-                        const f3 Vv{-rd.x, -rd.y, -rd.z};
-                        const f3 H = sample_ggx_vndf(nrm, Vv, m.roughness, rng);
-                        sdir = reflect3(rd, H);
-                        if (dot(sdir, nrm) <= 0.0f) {
-                            sdir = normalize(add(nrm, random_direction(rng)));
-                            bt = evaluate_brdf(rd, sdir, nrm, albedo, m.roughness, m.metallic, m.specular);
-                        } else {
-                            const f3 F0 = add(muls(f3{0.04f, 0.04f, 0.04f}, 1.0f - m.metallic), muls(albedo, m.metallic));
-                            bt = ggx_throughput(nrm, Vv, sdir, H, m.roughness, F0);
-                        }
-                    } else {
-                        sdir = normalize(add(nrm, random_direction(rng)));
-                        bt = evaluate_brdf(rd, sdir, nrm, albedo, m.roughness, m.metallic, m.specular);
-                    }
-                    r.rng = rng;
-                    f3 ntp = mul(f3{r.tx, r.ty, r.tz}, bt);
-                    bool dies = false;
-                    if (r.bounce + 1u >= RR_START_BOUNCE) {   // Russian roulette (shade.comp:879-889)
-                        const float pr = gclamp(gmax(ntp.x, gmax(ntp.y, ntp.z)), 0.05f, 0.95f);
-                        const float xi = rand_lds(r.rng);
-                        if (xi > pr) dies = true;
-                        else ntp = divs(ntp, pr);
-                    }
-                    if (dies) {
-                        deposit(p, lbuf, r.primary_vid, f3{r.dlx, r.dly, r.dlz}, f3{r.ilx, r.ily, r.ilz});
-                    } else {   // survivor (shade.comp:892-908)
-                        r.ox = ((float)hx + 0.5f) + nrm.x;
-                        r.oy = ((float)hy + 0.5f) + nrm.y;
-                        r.oz = ((float)hz + 0.5f) + nrm.z;
-                        r.dx = sdir.x; r.dy = sdir.y; r.dz = sdir.z;
-                        r.tx = ntp.x; r.ty = ntp.y; r.tz = ntp.z;
-                        r.bounce = r.bounce + 1u;
-                        r.hit_vid = 0u;
-                        r.hit_material = 0u;
-                        survive = true;
-                    }
-                }
-            }
-        }
+        const int outcome = shade_ray(p, iter, t, r, nbuf, mats, em_reg, counters, nee);
+        if (outcome == RAY_DEPOSITS) deposit(p, lbuf, r.primary_vid, f3{r.dlx, r.dly, r.dlz}, f3{r.ilx, r.ily, r.ilz});
+        survive = outcome == RAY_SURVIVES;
     }
     // shadow rays of the wave: one atomic each for the frame counter and the
     // cumulative total
@@ -1368,6 +1402,146 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, const uint32_t *__
         if ((int)lane == leader) base = atomicAdd(live_out, (uint32_t)__popcll(mask));
         base = __shfl(base, leader, 64);
         if (survive) rays_out[base + before] = r;
+    }
+}
+
+// Bounce iterations first..iters-1 (renderer.cpp:515-602 after the first)
+// fused per ray: each lane takes one ray entering iteration `first` through
+// trace and shade until it ends, so the phase costs about the longest path
+// rather than the sum over iterations of each iteration's longest trace.
+// Shading in these iterations reads nothing a deposit writes (normals,
+// materials, the emissive registry; reservoirs are written at bounce 0
+// only), so deposits can wait: each is appended to `deferred` with its
+// iteration, and after the kernel they are grouped by lBuffer bucket and
+// applied bucket by bucket in LDS, iteration by iteration -- the order the
+// reference's one dispatch per iteration gives.  Traversals then run without
+// a claim and up to ten adds on global memory per deposit beside them (the
+// kernel is kept this lean: extra bookkeeping here has cost the traversal
+// loop 20% through its register allocation).
+// Ray counts: every ray entering an iteration counts as traced, as in the
+// per-iteration trace dispatch.
+template <bool STATS>
+__global__ __launch_bounds__(256) void k_bounce(const uint32_t *__restrict__ words, FrameParams p,
+                                                const Ray *__restrict__ rays_in, const uint32_t *__restrict__ live_ptr,
+                                                uint32_t first, uint32_t iters, uint32_t *__restrict__ counters,
+                                                unsigned long long *__restrict__ stats,
+                                                unsigned long long *__restrict__ totals,
+                                                const Material *__restrict__ mats, const uint32_t *__restrict__ nbuf,
+                                                uint32_t *__restrict__ rbuf, const EmissiveVoxel *__restrict__ emissives,
+                                                const uint32_t *__restrict__ em_reg,
+                                                const uint32_t *__restrict__ em_claim, uint32_t *__restrict__ em_bits,
+                                                Deferred *__restrict__ deferred)
+{
+    const uint32_t live = *live_ptr;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x * blockDim.x >= live) return;   // whole block past the live rays
+    __shared__ uint32_t path_cache[VE_MAX_DEPTH * 256];
+    NeeCtx nee{words, emissives, rbuf, path_cache + threadIdx.x, 0u};
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t checks_sum = 0, reads = 0;
+    bool active = t < live;
+    Ray r;
+    if (active) r = rays_in[t];
+    for (uint32_t iter = first; iter < iters; iter++) {
+        const uint64_t m = __ballot(active);
+        if (m == 0ull) break;
+        if ((int)lane == __ffsll((unsigned long long)m) - 1) {
+            const uint32_t n = (uint32_t)__popcll(m);
+            atomicAdd(&counters[VE_CNT_RAYS_TRACED], n);
+            atomicAdd(&totals[1], (unsigned long long)n);
+            atomicMax(&counters[VE_CNT_BOUNCES_RUN], iter + 1u);
+        }
+        if (!active) continue;
+        uint32_t checks = 0;
+        trace_ray<STATS>(words, p, iter, t, r, mats, em_claim, em_bits, path_cache + threadIdx.x, checks, reads);
+        checks_sum += checks;
+        const int outcome = shade_ray(p, iter, t, r, nbuf, mats, em_reg, counters, nee);
+        if (outcome == RAY_DEPOSITS) {   // at most one deposit per ray: the slot is always in range
+            const uint32_t slot = atomicAdd(&counters[VE_CNT_DEFERRED], 1u);
+            if (slot < SHADE_BUDGET_MAX)
+                deferred[slot] = Deferred{r.primary_vid, iter, {r.dlx, r.dly, r.dlz}, {r.ilx, r.ily, r.ilz}};
+        }
+        active = outcome == RAY_SURVIVES;
+    }
+    unsigned long long sr = nee.shadow_rays;
+    for (int off = 32; off > 0; off >>= 1) sr += __shfl_xor(sr, off, 64);
+    if (lane == 0u && sr) {
+        atomicAdd(&counters[VE_CNT_SHADOW_RAYS], (uint32_t)sr);
+        atomicAdd(&totals[2], sr);
+    }
+    if (STATS) {
+        unsigned long long a = reads, c = checks_sum;
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off, 64);
+            c += __shfl_xor(c, off, 64);
+        }
+        if (lane == 0u) {
+            atomicAdd(&stats[1], a);
+            atomicAdd(&stats[3], c);
+        }
+    }
+}
+
+// Deposits per lBuffer bucket (entries of id 0 deposit nothing).
+__global__ __launch_bounds__(256) void k_bin_count(FrameParams p, const Deferred *__restrict__ deferred,
+                                                   const uint32_t *__restrict__ counters, uint32_t *__restrict__ bin_count)
+{
+    const uint32_t n = min(counters[VE_CNT_DEFERRED], SHADE_BUDGET_MAX);
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const uint32_t vid = deferred[e].vid;
+        if (vid != 0u) atomicAdd(&bin_count[lbuf_row(p, vid)], 1u);
+    }
+}
+
+// Groups the held-back deposits by lBuffer bucket (counting sort: an entry
+// goes to its bucket's offset plus the number of its bucket's entries placed
+// before it).
+__global__ __launch_bounds__(256) void k_bin_scatter(FrameParams p, const Deferred *__restrict__ deferred,
+                                                     const uint32_t *__restrict__ counters,
+                                                     const uint32_t *__restrict__ bin_off,
+                                                     uint32_t *__restrict__ bin_fill, Deferred *__restrict__ sorted)
+{
+    const uint32_t n = min(counters[VE_CNT_DEFERRED], SHADE_BUDGET_MAX);
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const Deferred d = deferred[e];
+        if (d.vid == 0u) continue;
+        const uint32_t h = lbuf_row(p, d.vid);
+        sorted[bin_off[h] + atomicAdd(&bin_fill[h], 1u)] = d;
+    }
+}
+
+// One wave per lBuffer bucket with deposits: the bucket is copied to LDS,
+// its deposits are applied there iteration by iteration (the lanes of one
+// iteration concurrently, as a dispatch's invocations are), and it is written
+// back; the bucket's count is cleared for the next frame.
+__global__ __launch_bounds__(64) void k_bin_apply(FrameParams p, const Deferred *__restrict__ sorted,
+                                                  uint32_t *__restrict__ bin_count, uint32_t *__restrict__ bin_fill,
+                                                  const uint32_t *__restrict__ bin_off, uint32_t *__restrict__ lbuf,
+                                                  uint32_t first, uint32_t iters)
+{
+    __shared__ __align__(16) uint32_t bk[LB_WORDS];   // 16-byte aligned: owner rows are read as uint4
+    const uint32_t h = blockIdx.x, lane = threadIdx.x;
+    const uint32_t n = bin_count[h];
+    if (n == 0u) return;
+    uint32_t *gb = lbuf + (size_t)h * LB_WORDS;
+    for (uint32_t i = lane; i < LB_WORDS; i += 64u) bk[i] = gb[i];
+    __syncthreads();
+    const Deferred *list = sorted + bin_off[h];
+    for (uint32_t it = first; it < iters; it++) {
+        for (uint32_t base = 0; base < n; base += 64u) {
+            const uint32_t e = base + lane;
+            if (e < n) {
+                const Deferred d = list[e];
+                if (d.iter == it)
+                    deposit_into(p, bk, d.vid, f3{d.dl[0], d.dl[1], d.dl[2]}, f3{d.il[0], d.il[1], d.il[2]});
+            }
+        }
+        __syncthreads();
+    }
+    for (uint32_t i = lane; i < LB_WORDS; i += 64u) gb[i] = bk[i];
+    if (lane == 0u) {
+        bin_count[h] = 0u;
+        bin_fill[h] = 0u;
     }
 }
 
@@ -1725,6 +1899,33 @@ void launch_shade(hipStream_t s, const uint32_t *words, const FrameParams &p, De
     launch(k_shade, dim3(nb), dim3(256), 0, s, p, words, iter, bounce_in(b, iter), live_in_ptr(b, iter),
                        bounce_out(b, iter), live_out_ptr(b, iter), b.lbuf, b.nbuf, b.rbuf, b.materials, b.emissives,
                        b.em_reg, b.counters, b.ray_totals);
+}
+
+void launch_bounce(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t first,
+                   uint32_t iters, bool stats)
+{
+    const uint32_t nb = cdiv(p.shade_budget, 256);
+    if (stats)
+        launch(k_bounce<true>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, first), live_in_ptr(b, first), first,
+               iters, b.counters, b.node_stats, b.ray_totals, b.materials, b.nbuf, b.rbuf, b.emissives, b.em_reg,
+               b.emissive_claim, b.em_bits, b.deferred);
+    else
+        launch(k_bounce<false>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, first), live_in_ptr(b, first), first,
+               iters, b.counters, b.node_stats, b.ray_totals, b.materials, b.nbuf, b.rbuf, b.emissives, b.em_reg,
+               b.emissive_claim, b.em_bits, b.deferred);
+}
+
+int launch_deferred(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_t first, uint32_t iters)
+{
+    launch(k_bin_count, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, p, b.deferred, b.counters, b.bin_count);
+    size_t tb = b.scan_temp_bytes;
+    if (hipcub::DeviceScan::ExclusiveSum(b.scan_temp, tb, b.bin_count, b.bin_off, (int)p.hash_width, s) != hipSuccess)
+        return VE_EDEVICE;
+    launch(k_bin_scatter, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, p, b.deferred, b.counters, b.bin_off, b.bin_fill,
+           b.deferred_sorted);
+    launch(k_bin_apply, dim3(p.hash_width), dim3(64), 0, s, p, b.deferred_sorted, b.bin_count, b.bin_fill, b.bin_off,
+           b.lbuf, first, iters);
+    return VE_OK;
 }
 
 // Partial node-array update (Octree::UpdateNode's glBufferSubData per word).

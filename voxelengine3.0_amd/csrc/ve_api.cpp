@@ -106,6 +106,8 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.rays_next, SHADE_BUDGET_MAX);
     a(&b.counters, NUM_COUNTERS);
     a(&b.rank_stats, RANK_STATS_WORDS);
+    a(&b.rank_part, (size_t)RANK_COPIES * RANK_COPY_WORDS);
+    a(&b.bounce_part, (size_t)BOUNCE_COPIES * 2);
     a(&b.emissives, EMISSIVE_CAPACITY);
     a(&b.emissive_claim, (size_t)EMISSIVE_CLAIM_W * EMISSIVE_CLAIM_PROBES);
     a(&b.em_bits, SHADE_BUDGET_MAX / 32);

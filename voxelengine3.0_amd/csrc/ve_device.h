@@ -229,13 +229,15 @@ namespace ve {
 // log = log2 * ln2, exp = exp2(x * log2e), pow = exp2(log2(x) * y); the log2 /
 // exp2 / sin / cos cores are evaluated in double and rounded once, which is
 // the same definition the C oracle uses.
-__device__ __forceinline__ float m_log2(float x) { return (float)log2((double)x); }
-__device__ __forceinline__ float m_exp2(float x) { return (float)exp2((double)x); }
+// The double-precision cores are kept out of line: inlined at every call
+// site they made the bounce kernels larger than the instruction cache.
+__device__ __noinline__ float m_log2(float x) { return (float)log2((double)x); }
+__device__ __noinline__ float m_exp2(float x) { return (float)exp2((double)x); }
 __device__ __forceinline__ float m_log(float x) { return m_log2(x) * 0.6931471805599453f; }
 __device__ __forceinline__ float m_exp(float x) { return m_exp2(x * 1.4426950408889634f); }
 __device__ __forceinline__ float m_pow(float x, float y) { return m_exp2(m_log2(x) * y); }
-__device__ __forceinline__ float m_cos(float x) { return (float)cos((double)x); }
-__device__ __forceinline__ float m_sin(float x) { return (float)sin((double)x); }
+__device__ __noinline__ float m_cos(float x) { return (float)cos((double)x); }
+__device__ __noinline__ float m_sin(float x) { return (float)sin((double)x); }
 __device__ __forceinline__ float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
 __device__ __forceinline__ float gsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
 __device__ __forceinline__ float length3(f3 a) { return sqrtf(dot(a, a)); }
@@ -320,7 +322,9 @@ __device__ __forceinline__ float smith_g1(float NdotX, float a)
 }
 
 // sampleGGXVNDF (internal.glsl:182-219)
-__device__ __forceinline__ f3 sample_ggx_vndf(f3 N, f3 Vv, float roughness, uint32_t &state)
+// (out of line, with its two random numbers drawn by the caller, so that no
+// reference escapes into the call)
+__device__ __noinline__ f3 sample_ggx_vndf_at(f3 N, f3 Vv, float roughness, float r1, float r2)
 {
     const float a = gmax(roughness * roughness, 0.001f);
     const f3 up = fabsf(N.z) < 0.999f ? f3{0, 0, 1} : f3{1, 0, 0};
@@ -331,8 +335,6 @@ __device__ __forceinline__ f3 sample_ggx_vndf(f3 N, f3 Vv, float roughness, uint
     const float lensq = Vh.x * Vh.x + Vh.y * Vh.y;
     const f3 T1 = lensq > 0.0f ? muls(f3{-Vh.y, Vh.x, 0.0f}, 1.0f / sqrtf(lensq)) : f3{1, 0, 0};
     const f3 T2 = cross3(Vh, T1);
-    const float r1 = rand_lds(state);
-    const float r2 = rand_lds(state);
     const float r = sqrtf(r1);
     const float phi = 6.2831853f * r2;
     const float t1 = r * m_cos(phi);
@@ -344,7 +346,14 @@ __device__ __forceinline__ f3 sample_ggx_vndf(f3 N, f3 Vv, float roughness, uint
     return add(add(muls(T, Hl.x), muls(B, Hl.y)), muls(N, Hl.z));
 }
 
-__device__ __forceinline__ f3 ggx_throughput(f3 N, f3 Vv, f3 L, f3 H, float roughness, f3 F0)
+__device__ __forceinline__ f3 sample_ggx_vndf(f3 N, f3 Vv, float roughness, uint32_t &state)
+{
+    const float r1 = rand_lds(state);   // internal.glsl draws r1 then r2
+    const float r2 = rand_lds(state);
+    return sample_ggx_vndf_at(N, Vv, roughness, r1, r2);
+}
+
+__device__ __noinline__ f3 ggx_throughput(f3 N, f3 Vv, f3 L, f3 H, float roughness, f3 F0)
 {
     const float NdotL = gmax(dot(N, L), 0.001f);
     const float VdotH = gmax(dot(Vv, H), 0.001f);

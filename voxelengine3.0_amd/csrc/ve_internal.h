@@ -42,6 +42,14 @@ struct Deferred {
     float dl[3], il[3];
 };
 constexpr uint32_t RANK_BLOCKS = 1024;         // rank pass grid (candidates strided over it)
+// The rank blocks flush their counts, min/max and histogram into RANK_COPIES
+// copies (block b into copy b % RANK_COPIES), so no address takes more than
+// RANK_BLOCKS / RANK_COPIES of those atomics; copy layout: count, ~min, max,
+// histogram[256].
+constexpr uint32_t RANK_COPIES = 32, RANK_COPY_WORDS = 3 + HISTOGRAM_BINS;
+// The fused bounce pass's waves flush their traced-ray counts and deepest
+// iteration into BOUNCE_COPIES copies of {rays, iterations run}.
+constexpr uint32_t BOUNCE_COPIES = 32;
 constexpr uint32_t RESOLVE_TIMEOUT_BLOCKS = 1024; // 4 waves each, striding over the timed-out pixels
 
 // Device layout of one hash bucket (one column h of the reference's R32UI

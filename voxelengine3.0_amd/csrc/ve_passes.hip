@@ -456,7 +456,8 @@ __device__ float indirect_variance(const uint32_t *__restrict__ lbuf, uint32_t h
 // min/max before the few global atomics of its flush.
 __global__ __launch_bounds__(256) void k_rank(FrameParams p, const uint4 *__restrict__ candidates,
                                               const uint32_t *__restrict__ samples_map, const uint32_t *__restrict__ lbuf,
-                                              uint32_t *__restrict__ rank_stats, const uint32_t *__restrict__ counters,
+                                              const uint32_t *__restrict__ rank_stats,
+                                              uint32_t *__restrict__ rank_part, const uint32_t *__restrict__ counters,
                                               int32_t *__restrict__ nsamp, uint32_t *__restrict__ vec_tot, uint32_t n_vec)
 {
     __shared__ uint32_t hist[HISTOGRAM_BINS];
@@ -512,13 +513,42 @@ __global__ __launch_bounds__(256) void k_rank(FrameParams p, const uint4 *__rest
         atomicMax(&bmax, my_max);
     }
     __syncthreads();
+    uint32_t *part = rank_part + (size_t)(blockIdx.x % RANK_COPIES) * RANK_COPY_WORDS;
     if (threadIdx.x == 0 && active) {
-        atomicAdd(&rank_stats[6], active);
-        atomicMin(&rank_stats[4], bmin);
-        atomicMax(&rank_stats[5], bmax);
+        atomicAdd(&part[0], active);
+        atomicMax(&part[1], ~bmin);   // the complement: a zeroed copy holds "no minimum"
+        atomicMax(&part[2], bmax);
     }
     for (uint32_t i = threadIdx.x; i < HISTOGRAM_BINS; i += blockDim.x)
-        if (hist[i]) atomicAdd(&rank_stats[8 + i], hist[i]);
+        if (hist[i]) atomicAdd(&part[3 + i], hist[i]);
+}
+
+// RankStats from the rank blocks' copies (rank.comp:141-145: count, min and
+// max of the importance bits, histogram), which are cleared for the next
+// pass.  One block of HISTOGRAM_BINS threads, after k_rank.
+__device__ void rank_fold_copies(uint32_t *__restrict__ rank_stats, uint32_t *__restrict__ rank_part)
+{
+    const uint32_t i = threadIdx.x;
+    uint32_t h = 0;
+    for (uint32_t c = 0; c < RANK_COPIES; c++) {
+        uint32_t *part = rank_part + (size_t)c * RANK_COPY_WORDS;
+        h += part[3 + i];
+        part[3 + i] = 0u;
+    }
+    rank_stats[8 + i] += h;
+    if (i == 0) {
+        uint32_t cnt = 0, nmin = 0, mx = 0;
+        for (uint32_t c = 0; c < RANK_COPIES; c++) {
+            uint32_t *part = rank_part + (size_t)c * RANK_COPY_WORDS;
+            cnt += part[0];
+            nmin = max(nmin, part[1]);
+            mx = max(mx, part[2]);
+            part[0] = part[1] = part[2] = 0u;
+        }
+        rank_stats[6] += cnt;
+        rank_stats[4] = min(rank_stats[4], ~nmin);
+        rank_stats[5] = max(rank_stats[5], mx);
+    }
 }
 
 // Lockstep order of rank.comp's sample loop inside a vector: iteration i
@@ -527,8 +557,11 @@ __global__ __launch_bounds__(256) void k_rank_write(FrameParams p, const uint4 *
                                                     const int32_t *__restrict__ nsamp,
                                                     const uint32_t *__restrict__ vec_tot,
                                                     const uint32_t *__restrict__ vec_base, uint2 *__restrict__ shade_list,
-                                                    uint32_t *__restrict__ counters)
+                                                    uint32_t *__restrict__ counters, uint32_t *__restrict__ rank_stats,
+                                                    uint32_t *__restrict__ rank_part)
 {
+    static_assert(HISTOGRAM_BINS == 256, "the fold takes one bin per thread of a 256-thread block");
+    if (blockIdx.x == 0) rank_fold_copies(rank_stats, rank_part);
     const uint32_t n = counters[VE_CNT_CANDIDATES];
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -1286,8 +1319,11 @@ __device__ __forceinline__ uint32_t registry_seen(const uint32_t *__restrict__ c
 // direction and Russian roulette.  Returns whether the ray continues (r holds
 // the next bounce's ray), ends with a deposit of its direct/indirect light
 // into its primary voxel (r's dl/il), or ends without one.
+// ANY_BOUNCE false: the ray is past bounce 0 (reservoir reuse and the
+// first-shade registry view are then never needed and not compiled in).
 enum : int { RAY_DONE = 0, RAY_SURVIVES = 1, RAY_DEPOSITS = 2 };
 
+template <bool ANY_BOUNCE>
 __device__ __forceinline__ int shade_ray(const FrameParams &p, uint32_t iter, uint32_t t, Ray &r,
                                          const uint32_t *__restrict__ nbuf, const Material *__restrict__ mats,
                                          const uint32_t *__restrict__ em_reg, const uint32_t *__restrict__ counters,
@@ -1314,10 +1350,10 @@ __device__ __forceinline__ int shade_ray(const FrameParams &p, uint32_t iter, ui
         else { r.ilx += em.x; r.ily += em.y; r.ilz += em.z; }
     }
     // next-event estimation (shade.comp:631-836)
-    const uint32_t count = iter == 0u ? registry_seen(counters, em_reg, t) : counters[VE_CNT_EMISSIVES];
+    const uint32_t count = ANY_BOUNCE && iter == 0u ? registry_seen(counters, em_reg, t) : counters[VE_CNT_EMISSIVES];
     if (!m.emissive && r.bounce < 2u && count > 0u) {
         const uint32_t n_lights = count < EMISSIVE_CAPACITY ? count : EMISSIVE_CAPACITY;
-        if (p.restir_enabled && r.bounce == 0u)
+        if (ANY_BOUNCE && p.restir_enabled && r.bounce == 0u)
             nee_restir(p, nee, r, m, nrm, hx, hy, hz, hsize, n_lights, count);
         else
             nee_ris(p, nee, r, m, nrm, hx, hy, hz, n_lights);
@@ -1380,7 +1416,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, const uint32_t *__
     Ray r;
     if (t < live) {
         r = rays_in[t];
-        const int outcome = shade_ray(p, iter, t, r, nbuf, mats, em_reg, counters, nee);
+        const int outcome = shade_ray<true>(p, iter, t, r, nbuf, mats, em_reg, counters, nee);
         if (outcome == RAY_DEPOSITS) deposit(p, lbuf, r.primary_vid, f3{r.dlx, r.dly, r.dlz}, f3{r.ilx, r.ily, r.ilz});
         survive = outcome == RAY_SURVIVES;
     }
@@ -1419,7 +1455,10 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, const uint32_t *__
 // kernel is kept this lean: extra bookkeeping here has cost the traversal
 // loop 20% through its register allocation).
 // Ray counts: every ray entering an iteration counts as traced, as in the
-// per-iteration trace dispatch.
+// per-iteration trace dispatch; each lane counts its own, and each wave adds
+// its sums to one of BOUNCE_COPIES copies at the end (k_bin_count folds
+// them), since per-iteration atomics on one counter from every wave cost the
+// traversals a quarter of their time.
 template <bool STATS>
 __global__ __launch_bounds__(256) void k_bounce(const uint32_t *__restrict__ words, FrameParams p,
                                                 const Ray *__restrict__ rays_in, const uint32_t *__restrict__ live_ptr,
@@ -1430,7 +1469,7 @@ __global__ __launch_bounds__(256) void k_bounce(const uint32_t *__restrict__ wor
                                                 uint32_t *__restrict__ rbuf, const EmissiveVoxel *__restrict__ emissives,
                                                 const uint32_t *__restrict__ em_reg,
                                                 const uint32_t *__restrict__ em_claim, uint32_t *__restrict__ em_bits,
-                                                Deferred *__restrict__ deferred)
+                                                Deferred *__restrict__ deferred, uint32_t *__restrict__ bounce_part)
 {
     const uint32_t live = *live_ptr;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1438,30 +1477,34 @@ __global__ __launch_bounds__(256) void k_bounce(const uint32_t *__restrict__ wor
     __shared__ uint32_t path_cache[VE_MAX_DEPTH * 256];
     NeeCtx nee{words, emissives, rbuf, path_cache + threadIdx.x, 0u};
     const uint32_t lane = threadIdx.x & 63u;
-    uint32_t checks_sum = 0, reads = 0;
+    uint32_t checks_sum = 0, reads = 0, entered = 0, reached = 0;
     bool active = t < live;
     Ray r;
     if (active) r = rays_in[t];
     for (uint32_t iter = first; iter < iters; iter++) {
-        const uint64_t m = __ballot(active);
-        if (m == 0ull) break;
-        if ((int)lane == __ffsll((unsigned long long)m) - 1) {
-            const uint32_t n = (uint32_t)__popcll(m);
-            atomicAdd(&counters[VE_CNT_RAYS_TRACED], n);
-            atomicAdd(&totals[1], (unsigned long long)n);
-            atomicMax(&counters[VE_CNT_BOUNCES_RUN], iter + 1u);
-        }
+        if (__ballot(active) == 0ull) break;
         if (!active) continue;
+        entered++;
+        reached = iter + 1u;
         uint32_t checks = 0;
         trace_ray<STATS>(words, p, iter, t, r, mats, em_claim, em_bits, path_cache + threadIdx.x, checks, reads);
         checks_sum += checks;
-        const int outcome = shade_ray(p, iter, t, r, nbuf, mats, em_reg, counters, nee);
+        const int outcome = shade_ray<false>(p, iter, t, r, nbuf, mats, em_reg, counters, nee);
         if (outcome == RAY_DEPOSITS) {   // at most one deposit per ray: the slot is always in range
             const uint32_t slot = atomicAdd(&counters[VE_CNT_DEFERRED], 1u);
             if (slot < SHADE_BUDGET_MAX)
                 deferred[slot] = Deferred{r.primary_vid, iter, {r.dlx, r.dly, r.dlz}, {r.ilx, r.ily, r.ilz}};
         }
         active = outcome == RAY_SURVIVES;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        entered += __shfl_xor(entered, off, 64);
+        reached = max(reached, (uint32_t)__shfl_xor((int)reached, off, 64));
+    }
+    if (lane == 0u && entered) {
+        uint32_t *part = bounce_part + 2u * ((t >> 6) % BOUNCE_COPIES);
+        atomicAdd(&part[0], entered);
+        atomicMax(&part[1], reached);
     }
     unsigned long long sr = nee.shadow_rays;
     for (int off = 32; off > 0; off >>= 1) sr += __shfl_xor(sr, off, 64);
@@ -1482,10 +1525,25 @@ __global__ __launch_bounds__(256) void k_bounce(const uint32_t *__restrict__ wor
     }
 }
 
-// Deposits per lBuffer bucket (entries of id 0 deposit nothing).
+// Deposits per lBuffer bucket (entries of id 0 deposit nothing).  Its first
+// thread folds k_bounce's ray-count copies into the frame counters and the
+// cumulative totals and clears them.
 __global__ __launch_bounds__(256) void k_bin_count(FrameParams p, const Deferred *__restrict__ deferred,
-                                                   const uint32_t *__restrict__ counters, uint32_t *__restrict__ bin_count)
+                                                   uint32_t *__restrict__ counters, uint32_t *__restrict__ bin_count,
+                                                   uint32_t *__restrict__ bounce_part,
+                                                   unsigned long long *__restrict__ totals)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        uint32_t rays = 0, reached = 0;
+        for (uint32_t c = 0; c < BOUNCE_COPIES; c++) {
+            rays += bounce_part[2u * c];
+            reached = max(reached, bounce_part[2u * c + 1u]);
+            bounce_part[2u * c] = bounce_part[2u * c + 1u] = 0u;
+        }
+        counters[VE_CNT_RAYS_TRACED] += rays;
+        totals[1] += rays;
+        counters[VE_CNT_BOUNCES_RUN] = max(counters[VE_CNT_BOUNCES_RUN], reached);
+    }
     const uint32_t n = min(counters[VE_CNT_DEFERRED], SHADE_BUDGET_MAX);
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
         const uint32_t vid = deferred[e].vid;
@@ -1844,12 +1902,12 @@ int launch_rank(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
     const uint32_t bound = (uint32_t)std::min<size_t>((size_t)p.W * p.H, CANDIDATE_CAPACITY);
     const uint32_t n_vec = cdiv(bound, 8);
     launch(k_rank, dim3(std::min<uint32_t>(cdiv(bound, 256), RANK_BLOCKS)), dim3(256), 0, s, p, b.candidates,
-                       b.samples, b.lbuf, b.rank_stats, b.counters, b.nsamp, b.vec_cnt, n_vec);
+                       b.samples, b.lbuf, b.rank_stats, b.rank_part, b.counters, b.nsamp, b.vec_cnt, n_vec);
     size_t tb = b.scan_temp_bytes;
     if (hipcub::DeviceScan::ExclusiveSum(b.scan_temp, tb, b.vec_cnt, b.vec_base, (int)n_vec, s) != hipSuccess)
         return VE_EDEVICE;
     launch(k_rank_write, dim3(cdiv(bound, 256)), dim3(256), 0, s, p, b.candidates, b.nsamp, b.vec_cnt,
-                       b.vec_base, b.shade_list, b.counters);
+                       b.vec_base, b.shade_list, b.counters, b.rank_stats, b.rank_part);
     return VE_OK;
 }
 
@@ -1908,16 +1966,17 @@ void launch_bounce(hipStream_t s, const uint32_t *words, const FrameParams &p, D
     if (stats)
         launch(k_bounce<true>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, first), live_in_ptr(b, first), first,
                iters, b.counters, b.node_stats, b.ray_totals, b.materials, b.nbuf, b.rbuf, b.emissives, b.em_reg,
-               b.emissive_claim, b.em_bits, b.deferred);
+               b.emissive_claim, b.em_bits, b.deferred, b.bounce_part);
     else
         launch(k_bounce<false>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, first), live_in_ptr(b, first), first,
                iters, b.counters, b.node_stats, b.ray_totals, b.materials, b.nbuf, b.rbuf, b.emissives, b.em_reg,
-               b.emissive_claim, b.em_bits, b.deferred);
+               b.emissive_claim, b.em_bits, b.deferred, b.bounce_part);
 }
 
 int launch_deferred(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_t first, uint32_t iters)
 {
-    launch(k_bin_count, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, p, b.deferred, b.counters, b.bin_count);
+    launch(k_bin_count, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, p, b.deferred, b.counters, b.bin_count, b.bounce_part,
+           b.ray_totals);
     size_t tb = b.scan_temp_bytes;
     if (hipcub::DeviceScan::ExclusiveSum(b.scan_temp, tb, b.bin_count, b.bin_off, (int)p.hash_width, s) != hipSuccess)
         return VE_EDEVICE;

@@ -69,6 +69,22 @@ __device__ __forceinline__ uint32_t locate(uint32_t x, uint32_t y, uint32_t z, u
     return ((uint32_t)((x & p2) != 0u) << 2) | ((uint32_t)((y & p2) != 0u) << 1) | (uint32_t)((z & p2) != 0u);
 }
 
+// locate() for p2 = 1 << bit: the three coordinate bits, extracted directly.
+__device__ __forceinline__ uint32_t locate_bit(uint32_t x, uint32_t y, uint32_t z, uint32_t bit)
+{
+    return (__builtin_amdgcn_ubfe(x, bit, 1u) << 2) | (__builtin_amdgcn_ubfe(y, bit, 1u) << 1) |
+           __builtin_amdgcn_ubfe(z, bit, 1u);
+}
+
+// Node word `offset` of a validated array: the upload and every update check
+// that each internal node's child block lies inside the array, so a descent
+// never computes an offset past it (the reference's `offset < size ? w : 0`
+// guard is then always true); the clamp only keeps the load in bounds.
+__device__ __forceinline__ uint32_t node_word(const uint32_t *__restrict__ words, uint32_t n_words, uint32_t offset)
+{
+    return words[min(offset, n_words - 1u)];
+}
+
 __device__ __forceinline__ bool in_bounds(f3 v, float n)
 {
     return (0.0f <= v.x) & (v.x <= n) & (0.0f <= v.y) & (v.y <= n) & (0.0f <= v.z) & (v.z <= n);
@@ -179,15 +195,15 @@ __device__ __forceinline__ Hit raycast(const uint32_t *__restrict__ words, uint3
         } else {
             uint32_t offset = resume == 0u ? 0u : cache[(resume - 1u) * stride];
             for (dd = resume; dd < depth; dd++) {
-                offset += locate(ux, uy, uz, len >> dd);
-                const uint32_t raw = offset < n_words ? words[offset] : 0u;
+                offset += locate_bit(ux, uy, uz, depth - dd);
+                const uint32_t raw = node_word(words, n_words, offset);
                 if (!(raw & 1u)) break;
                 offset = raw >> 4;
                 cache[dd * stride] = offset;
             }
             if (dd == depth) {   // bottom level: the leaf slot
-                offset += locate(ux, uy, uz, 1u);
-                const uint32_t raw = offset < n_words ? words[offset] : 0u;
+                offset += locate_bit(ux, uy, uz, 0u);
+                const uint32_t raw = node_word(words, n_words, offset);
                 const uint32_t mat = (raw >> 1) & 0x7Fu;
                 if (mat != 0u) {
                     if (STATS) reads += depth + 1u;
@@ -249,26 +265,32 @@ __device__ __forceinline__ f3 cross3(f3 a, f3 b)
 }
 
 // ---- Halton sampler (internal.glsl:57-89) -------------------------------------
-__device__ __forceinline__ float halton(uint32_t i, uint32_t b)
+// Radical inverse of i in base B, digit by digit as the reference computes it.
+// B is a compile-time constant, so i / B is a multiply-high, not a division.
+template <uint32_t B>
+__device__ __forceinline__ float halton_b(uint32_t i)
 {
     float f = 1.0f, r = 0.0f;
     for (int k = 0; k < 24 && i > 0u; k++) {
-        f /= (float)b;
-        const uint32_t q = i / b;
-        r += f * (float)(i - q * b);
+        f /= (float)B;
+        const uint32_t q = i / B;
+        r += f * (float)(i - q * B);
         i = q;
     }
     return r;
 }
 
-__device__ __forceinline__ uint32_t halton_base(uint32_t k)
+// The base of dimension `dim` is the (dim mod 16)-th prime (internal.glsl:
+// 57-60).  Out of line: one copy of the sixteen loops serves every draw.
+__device__ __noinline__ float halton_dim(uint32_t i, uint32_t dim)
 {
-    // first 16 primes (internal.glsl:57-60)
-    switch (k & 15u) {
-    case 0: return 2u;   case 1: return 3u;   case 2: return 5u;   case 3: return 7u;
-    case 4: return 11u;  case 5: return 13u;  case 6: return 17u;  case 7: return 19u;
-    case 8: return 23u;  case 9: return 29u;  case 10: return 31u; case 11: return 37u;
-    case 12: return 41u; case 13: return 43u; case 14: return 47u; default: return 53u;
+    switch (dim & 15u) {
+    case 0: return halton_b<2>(i);    case 1: return halton_b<3>(i);    case 2: return halton_b<5>(i);
+    case 3: return halton_b<7>(i);    case 4: return halton_b<11>(i);   case 5: return halton_b<13>(i);
+    case 6: return halton_b<17>(i);   case 7: return halton_b<19>(i);   case 8: return halton_b<23>(i);
+    case 9: return halton_b<29>(i);   case 10: return halton_b<31>(i);  case 11: return halton_b<37>(i);
+    case 12: return halton_b<41>(i);  case 13: return halton_b<43>(i);  case 14: return halton_b<47>(i);
+    default: return halton_b<53>(i);
     }
 }
 
@@ -278,7 +300,7 @@ __device__ __forceinline__ float rand_lds(uint32_t &state)
     const uint32_t sample_idx = state >> 8;
     state = (sample_idx << 8) | ((dim + 1u) & 0xFFu);
     const uint32_t effective = sample_idx ^ ((dim >> 4) * 0x9E3779B9u);
-    return halton(effective, halton_base(dim));
+    return halton_dim(effective, dim);
 }
 
 __device__ __forceinline__ float rand_normal(uint32_t &state)   // internal.glsl:101-105

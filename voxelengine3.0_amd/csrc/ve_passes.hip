@@ -148,8 +148,8 @@ __device__ __forceinline__ SharedDescent centre_descent(const uint32_t *__restri
     uint32_t offset = 0;
     for (uint32_t d = 0; d <= depth; d++) {
         cache[d * stride] = offset;
-        offset += locate(hx, hy, hz, len >> d);
-        const uint32_t raw = offset < n ? w[offset] : 0u;
+        offset += locate_bit(hx, hy, hz, depth - d);
+        const uint32_t raw = node_word(w, n, offset);
         if (d == depth || !(raw & 1u)) {
             c.stop = d;
             c.occ = ((raw >> 1) & 0x7Fu) != 0u;
@@ -169,8 +169,8 @@ __device__ __forceinline__ bool neighbour_occupied(const uint32_t *__restrict__ 
     if (c.stop < d0) return c.occ;
     uint32_t offset = cache[d0 * stride];
     for (uint32_t d = d0; d <= depth; d++) {
-        offset += locate(x, y, z, len >> d);
-        const uint32_t raw = offset < n ? w[offset] : 0u;
+        offset += locate_bit(x, y, z, depth - d);
+        const uint32_t raw = node_word(w, n, offset);
         if (d == depth || !(raw & 1u)) return ((raw >> 1) & 0x7Fu) != 0u;
         offset = (raw >> 4) & 0x0FFFFFFFu;
     }

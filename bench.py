@@ -107,7 +107,7 @@ def cpu_baseline(words, depth, ubos, fc_kw, budget_s=20.0):
 # offered loads (frames/s) of the served sweep, per mode (each row also
 # reports the load its Poisson realisation actually offered): the plain loop
 # saturates near 650 frames/s, the concurrent server near 1300
-SERVED_LAMS = {0: [900, 1200, 1500, 1700, 1900, 2100, 2300], 1: [350, 450, 500, 550, 600]}
+SERVED_LAMS = {0: [1200, 1500, 1700, 1800, 1900, 2000], 1: [500, 600, 700, 800, 900, 1000]}
 
 
 def served_and_batched(ve, extreme):

@@ -86,7 +86,7 @@ def cpu_baseline(words, depth, ubos, fc_kw, budget_s=20.0):
     from voxelengine_amd import scenes
     of = O.Frame(W, H, 16384, BOUNCES + 1)
     of.set_materials(scenes.palette(1))
-    of.set_sky(np.array([140, 180, 230], np.float32) / np.float32(255))
+    of.set_sky_rgb8((140, 180, 230))
     fc = O.FrameConfig(**fc_kw)
     rays, frames = 0, 0
     t0 = time.perf_counter()

@@ -104,8 +104,15 @@ void ve_camera_ubo(const float position[3], const float direction[3], float fov_
  * up to 127 entries (shade.comp:84-90). */
 int ve_set_materials(ve_ctx *ctx, const void *std140_entries, uint32_t n_entries);
 
-/* Sky: six RGB8 faces of `edge`² texels sampled with bilinear filtering on a
- * miss (skybox.cpp:87-122).  A constant sky is edge = 1. */
+/* Sky (Skybox, skybox.cpp:71-122): six faces +X -X +Y -Y +Z -Z of edge x edge
+ * RGB8 texels, rows tightly packed in glTexImage2D order (the reference
+ * uploads rows 4-byte aligned, the same bytes when edge is a multiple of 4),
+ * sampled along the ray on every miss (shade.comp:559) and along the pixel's
+ * primary direction for sky pixels (resolve.comp:218-288) with the
+ * reference's sampler state: GL_LINEAR, CLAMP_TO_EDGE, no mipmaps, no
+ * seamless filtering.  A constant sky is edge = 1 (any uniform cube is kept
+ * as a constant).  edge <= 16384.  Waits for the frames already enqueued on
+ * the context before replacing a cube map. */
 int ve_set_sky(ve_ctx *ctx, const uint8_t *const faces_rgb8[6], int edge);
 
 /* Per-frame knobs = core::FrameConfig (src/renderer/core.hpp:83-158). */

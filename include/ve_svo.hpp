@@ -256,6 +256,8 @@ public:
         const uint8_t *faces[6] = {rgb, rgb, rgb, rgb, rgb, rgb};
         check(ve_set_sky(ctx_, faces, 1), ctx_);
     }
+    // Cube-map sky: six faces (+X -X +Y -Y +Z -Z) of edge x edge RGB8 texels.
+    void setSkyCube(const uint8_t *const faces[6], int edge) { check(ve_set_sky(ctx_, faces, edge), ctx_); }
 
     // Renderer::run (renderer.cpp:654-712): one wavefront frame, then the
     // final blit's tonemap into the RGBA8 display image; returns false (and

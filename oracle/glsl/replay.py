@@ -45,7 +45,7 @@ def _prog(name, defines=None):
 class GLPipeline:
     """The reference renderer's GPU state on llvmpipe for one scene."""
 
-    def __init__(self, words, depth, W, H, max_bounces=8, materials=None, sky_rgb8=(0, 0, 0)):
+    def __init__(self, words, depth, W, H, max_bounces=8, materials=None, sky_rgb8=(0, 0, 0), sky_faces=None):
         self.gl = G.GL.get()
         self.W, self.H, self.depth = W, H, depth
         self.max_bounces = max_bounces
@@ -64,8 +64,12 @@ class GLPipeline:
         mb = np.frombuffer(np.ascontiguousarray(materials).tobytes(), np.uint8)
         mat[:len(mb)] = mb
         self.mat_ubo = G.Buffer(data=mat)
-        face = np.array(sky_rgb8, np.uint8).reshape(1, 1, 3)
-        self.sky = G.CubeMap([face] * 6, 1)
+        if sky_faces is None:   # constant sky: one texel per face
+            face = np.array(sky_rgb8, np.uint8).reshape(1, 1, 3)
+            self.sky = G.CubeMap([face] * 6, 1)
+        else:                   # six (edge, edge, 3) RGB8 faces, +X -X +Y -Y +Z -Z
+            faces = np.asarray(sky_faces, np.uint8)
+            self.sky = G.CubeMap([faces[i] for i in range(6)], faces.shape[1])
         # screen images (framebufferEvent)
         self.gbuf = G.Image2D(W, H, G.GL_RGBA32UI)
         self.gpos = G.Image2D(W, H, G.GL_R32UI)

@@ -153,6 +153,9 @@ class Frame:
             L.so_state_destroy.argtypes = [C.c_void_p]
             L.so_state_set_materials.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
             L.so_state_set_sky.argtypes = [C.c_void_p, C.c_void_p]
+            L.so_state_set_sky_cube.restype = C.c_int
+            L.so_state_set_sky_cube.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
+            L.so_sky_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
             L.so_frame.restype = C.c_int
             L.so_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
                                    C.POINTER(FrameConfig), C.c_uint32, C.c_uint32, C.c_uint32]
@@ -181,6 +184,25 @@ class Frame:
     def set_sky(self, rgb):
         a = np.ascontiguousarray(rgb, np.float32)
         self.L.so_state_set_sky(self.h, a.ctypes.data)
+
+    def set_sky_cube(self, faces):
+        """Six (edge, edge, 3) RGB8 faces, +X -X +Y -Y +Z -Z (skybox.cpp:87-122)."""
+        f = np.ascontiguousarray(faces, np.uint8)
+        if f.ndim != 4 or f.shape[0] != 6 or f.shape[1] != f.shape[2] or f.shape[3] != 3:
+            raise ValueError("faces must be (6, edge, edge, 3) uint8")
+        if self.L.so_state_set_sky_cube(self.h, f.ctypes.data, f.shape[1]) != 0:
+            raise MemoryError("oracle sky")
+
+    def set_sky_rgb8(self, rgb8):
+        """A constant sky given as RGB8, sampled like every cube map (one texel per face)."""
+        self.set_sky_cube(np.broadcast_to(np.asarray(rgb8, np.uint8).reshape(1, 1, 1, 3), (6, 1, 1, 3)))
+
+    def sky_sample(self, dirs):
+        """texture(skybox, dir).rgb for an (n, 3) float32 array of directions."""
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.empty_like(d)
+        self.L.so_sky_sample(self.h, d.ctypes.data, len(d), out.ctypes.data)
+        return out
 
     def frame(self, words, depth, ubo, fc, frame_idx, frame_seed, time, n_words=None):
         self._words = np.ascontiguousarray(words, np.uint32)

@@ -556,6 +556,8 @@ struct so_state {
     uint32_t rank_stats[264];
     mat_t materials[128];
     float sky[3];
+    uint8_t *sky_tex;     /* cube-map faces (6 * edge * edge RGB8), NULL for a constant sky */
+    uint32_t sky_edge;
     uint32_t counters[16];
     /* frame constants */
     const uint32_t *words;
@@ -603,6 +605,7 @@ void so_state_destroy(so_state *s)
     free(s->gbuf); free(s->gpos); free(s->out); free(s->nbuf); free(s->lbuf); free(s->rbuf);
     free(s->claim); free(s->samples); free(s->cands); free(s->shade_list); free(s->ring);
     free(s->emissives); free(s->emissive_claim);
+    free(s->sky_tex);
     free(s);
 }
 
@@ -613,7 +616,102 @@ void so_state_set_materials(so_state *s, const void *std140, uint32_t n)
     memcpy(s->materials, std140, (size_t)n * 64);
 }
 
-void so_state_set_sky(so_state *s, const float rgb[3]) { memcpy(s->sky, rgb, sizeof s->sky); }
+void so_state_set_sky(so_state *s, const float rgb[3])
+{
+    memcpy(s->sky, rgb, sizeof s->sky);
+    free(s->sky_tex);
+    s->sky_tex = NULL;
+    s->sky_edge = 0;
+}
+
+int so_state_set_sky_cube(so_state *s, const uint8_t *faces_rgb8, uint32_t edge)
+{
+    size_t n = (size_t)6 * edge * edge * 3;
+    uint8_t *t = malloc(n);
+    if (!t || edge == 0) {
+        free(t);
+        return -1;
+    }
+    memcpy(t, faces_rgb8, n);
+    free(s->sky_tex);
+    s->sky_tex = t;
+    s->sky_edge = edge;
+    return 0;
+}
+
+/* ---- cube-map sampling: texture(samplerCube, dir) ------------------------
+ * The sampler state of skybox.cpp:113-117 (GL_LINEAR min/mag, CLAMP_TO_EDGE,
+ * level 0 only, seamless filtering off), as the GL driver the reference runs
+ * on evaluates it:
+ *  - face: the major axis of the GL spec's cube-map face table (GL 4.5
+ *    section 8.13, table 8.19), ties resolved toward z, then y;
+ *  - s = (sc / |ma| + 1) / 2, t = (tc / |ma| + 1) / 2;
+ *  - texel coordinate u = s * edge - 1/2 taken to 24.8 fixed point with
+ *    round-to-nearest-even: integer part = left texel, fraction = an 8-bit
+ *    weight; both texels clamped to [0, edge - 1];
+ *  - 8-bit texels blended in integers: lerp(a, b, w) = a + (((b - a) w + 128)
+ *    >> 8), along s first, then t;
+ *  - the blended byte c returned as c * (1/255) in single precision.
+ * The driver-specific steps (tie order, fixed-point weights, byte lerps, the
+ * final scaling) are pinned by the driver's own outputs over random faces and
+ * directions, edge and corner cases included. */
+static int lerp8(int a, int b, int w) { return a + (((b - a) * w + 128) >> 8); }
+
+static void cube_axis(float st, uint32_t edge, int *i0, int *i1, int *w)
+{
+    float u = st * (float)(edge * 256u) - 128.0f;
+    int x = (int)nearbyintf(u);
+    int lo = x >> 8, hi = lo + 1, top = (int)edge - 1;
+    *w = x & 255;
+    *i0 = lo < 0 ? 0 : lo > top ? top : lo;
+    *i1 = hi < 0 ? 0 : hi > top ? top : hi;
+}
+
+static v3 sky_sample(const so_state *s, v3 d)
+{
+    if (!s->sky_tex) return V(s->sky[0], s->sky[1], s->sky[2]);
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z), sc, tc, ma;
+    int face;
+    if (az >= ax && az >= ay) {
+        face = d.z >= 0.0f ? 4 : 5;
+        sc = d.z >= 0.0f ? d.x : -d.x;
+        tc = -d.y;
+        ma = az;
+    } else if (ay >= ax) {
+        face = d.y >= 0.0f ? 2 : 3;
+        sc = d.x;
+        tc = d.y >= 0.0f ? d.z : -d.z;
+        ma = ay;
+    } else {
+        face = d.x >= 0.0f ? 0 : 1;
+        sc = d.x >= 0.0f ? -d.z : d.z;
+        tc = -d.y;
+        ma = ax;
+    }
+    float fs = (sc / ma + 1.0f) * 0.5f, ft = (tc / ma + 1.0f) * 0.5f;
+    int i0, i1, wa, j0, j1, wb;
+    cube_axis(fs, s->sky_edge, &i0, &i1, &wa);
+    cube_axis(ft, s->sky_edge, &j0, &j1, &wb);
+    const uint8_t *f = s->sky_tex + (size_t)face * s->sky_edge * s->sky_edge * 3;
+    float out[3];
+    for (int c = 0; c < 3; c++) {
+        int t00 = f[((size_t)j0 * s->sky_edge + i0) * 3 + c], t10 = f[((size_t)j0 * s->sky_edge + i1) * 3 + c];
+        int t01 = f[((size_t)j1 * s->sky_edge + i0) * 3 + c], t11 = f[((size_t)j1 * s->sky_edge + i1) * 3 + c];
+        int v = lerp8(lerp8(t00, t10, wa), lerp8(t01, t11, wa), wb);
+        out[c] = (float)v * (1.0f / 255.0f);
+    }
+    return V(out[0], out[1], out[2]);
+}
+
+void so_sky_sample(const so_state *s, const float *dirs, size_t n, float *rgb)
+{
+    for (size_t i = 0; i < n; i++) {
+        v3 c = sky_sample(s, V(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+        rgb[3 * i] = c.x;
+        rgb[3 * i + 1] = c.y;
+        rgb[3 * i + 2] = c.z;
+    }
+}
 
 /* nBuffer / lBuffer / rBuffer are stored bucket-major: bucket h, field y at
  * h * height + y (the reference's image (x = h, y)). */
@@ -1465,9 +1563,10 @@ static void shade_pass(so_state *s, uint32_t base, uint32_t size)
         /* case 1: misses */
         for (uint32_t l = 0; l < 8; l++) {
             if (!live[l] || r[l].hit_vid != 0) continue;
-            r[l].ilx = r[l].ilx + s->sky[0] * r[l].tx;
-            r[l].ily = r[l].ily + s->sky[1] * r[l].ty;
-            r[l].ilz = r[l].ilz + s->sky[2] * r[l].tz;
+            v3 sky = sky_sample(s, V(r[l].dx, r[l].dy, r[l].dz));   /* shade.comp:559 */
+            r[l].ilx = r[l].ilx + sky.x * r[l].tx;
+            r[l].ily = r[l].ily + sky.y * r[l].ty;
+            r[l].ilz = r[l].ilz + sky.z * r[l].tz;
             deposit(s, r[l].primary_vid, V(r[l].dlx, r[l].dly, r[l].dlz), V(r[l].ilx, r[l].ily, r[l].ilz));
             r[l].bounce = MB;
             s->ring[slot[l]] = r[l];
@@ -1654,7 +1753,8 @@ static int lookup_normal_resolve(const so_state *s, uint32_t vid, v3 *n)   /* re
     return 0;
 }
 
-static v3 sky_rgb(const so_state *s) { return V(s->sky[0], s->sky[1], s->sky[2]); }
+/* texture(skybox, pixelDirection(pix)) (resolve.comp:170-180) */
+static v3 sky_rgb(const so_state *s, int px, int py) { return sky_sample(s, pixel_dir(s, (uint32_t)px, (uint32_t)py)); }
 
 static v3 reconstruct_timeout(const so_state *s, int px, int py)   /* resolve.comp:193-219 */
 {
@@ -1677,14 +1777,14 @@ static v3 reconstruct_timeout(const so_state *s, int px, int py)   /* resolve.co
             sum = vadd(sum, vmuls(na, w));
             wsum += w;
         }
-    return wsum > 0.0001f ? vdivs(sum, wsum) : sky_rgb(s);
+    return wsum > 0.0001f ? vdivs(sum, wsum) : sky_rgb(s, px, py);
 }
 
 static v3 resolve_default(const so_state *s, int px, int py, uint32_t vid, uint32_t checks)   /* resolve.comp:224-261 */
 {
     if (vid == 0) {
         if (checks >= (uint32_t)s->fc->primary_controlchecks) return reconstruct_timeout(s, px, py);
-        return sky_rgb(s);
+        return sky_rgb(s, px, py);
     }
     uint32_t cc;
     float cv;
@@ -1741,19 +1841,19 @@ static void resolve_pixel(so_state *s, uint32_t x, uint32_t y)   /* resolve.comp
     switch (s->fc->render_type) {
     case 1: col = V((float)checks / 65.0f, (float)checks / 65.0f, (float)checks / 65.0f); break;
     case 2:
-        if (vid == 0) col = timeout ? reconstruct_timeout(s, (int)x, (int)y) : sky_rgb(s);
+        if (vid == 0) col = timeout ? reconstruct_timeout(s, (int)x, (int)y) : sky_rgb(s, (int)x, (int)y);
         else if (mi == 0 || mi >= 127u) col = V(0, 0, 0);
         else col = V(s->materials[mi].color[0], s->materials[mi].color[1], s->materials[mi].color[2]);
         break;
     case 3:
-        if (vid == 0) col = timeout ? reconstruct_timeout(s, (int)x, (int)y) : sky_rgb(s);
+        if (vid == 0) col = timeout ? reconstruct_timeout(s, (int)x, (int)y) : sky_rgb(s, (int)x, (int)y);
         else {
             v3 n;
             col = lookup_normal_resolve(s, vid, &n) ? vadd(vmuls(n, 0.5f), V(0.5f, 0.5f, 0.5f)) : V(0, 0, 0);
         }
         break;
     case 4:
-        if (vid == 0) col = timeout ? reconstruct_timeout(s, (int)x, (int)y) : sky_rgb(s);
+        if (vid == 0) col = timeout ? reconstruct_timeout(s, (int)x, (int)y) : sky_rgb(s, (int)x, (int)y);
         else {
             uint32_t hid = vid - 1u;
             col = V((float)((hid + 1u) % 255u) / 255.0f, (float)(((hid + 1u) / 255u) % 255u) / 255.0f,
@@ -1761,7 +1861,7 @@ static void resolve_pixel(so_state *s, uint32_t x, uint32_t y)   /* resolve.comp
         }
         break;
     case 5:
-        if (vid == 0) col = timeout ? reconstruct_timeout(s, (int)x, (int)y) : sky_rgb(s);
+        if (vid == 0) col = timeout ? reconstruct_timeout(s, (int)x, (int)y) : sky_rgb(s, (int)x, (int)y);
         else {
             float sc = raw_variance(s, vid) * 65025.0f / gmax(s->fc->variance_norm, 1.0f);
             sc = gclamp(sc, 0.0f, 1.0f);

@@ -81,6 +81,13 @@ so_state *so_state_create(uint32_t hash_width, uint32_t max_bounces, uint32_t W,
 void so_state_destroy(so_state *s);
 void so_state_set_materials(so_state *s, const void *std140, uint32_t n);
 void so_state_set_sky(so_state *s, const float rgb[3]);
+/* Cube-map sky (skybox.cpp:87-122): six faces +X -X +Y -Y +Z -Z of edge x edge
+ * RGB8 texels, rows in upload order, sampled as GL_LINEAR / CLAMP_TO_EDGE on
+ * every miss; replaces the constant colour of so_state_set_sky.  Returns 0, or
+ * -1 when out of memory. */
+int so_state_set_sky_cube(so_state *s, const uint8_t *faces_rgb8, uint32_t edge);
+/* texture(skybox, dir).rgb of the current sky for n directions (3 floats each). */
+void so_sky_sample(const so_state *s, const float *dirs, size_t n, float *rgb);
 
 /* Renderer::runWavefrontFrame (renderer.cpp:282-649) for one frame. */
 int so_frame(so_state *s, const uint32_t *words, uint32_t n_words, uint32_t depth, const float ubo[16],

@@ -6,7 +6,7 @@ Runs the reference shaders (read from $VE_REF_SHADERS, default
 scenes of voxelengine3.0_amd/scenes.py, and stores inputs' digests plus the
 pass outputs as compressed .npz files next to this script.
 
-    LP_NUM_THREADS=1 python tests/golden/make_golden.py [primary] [frame]
+    LP_NUM_THREADS=1 python tests/golden/make_golden.py [fp] [cube] [primary] [frame] [lights] [sky]
 
 Only the outputs are stored; the shaders are never copied.
 """
@@ -113,6 +113,76 @@ def make_fp():
     np.savez_compressed(os.path.join(HERE, "fp_probes.npz"), **arrays)
 
 
+# Cube-map sky probes: `texture(samplerCube, dir).rgb` on llvmpipe over six
+# random RGB8 faces with the reference skybox's sampler state (GL_LINEAR,
+# CLAMP_TO_EDGE, no mipmaps, non-seamless; skybox.cpp:87-122), per face edge.
+CUBE_EDGES = (1, 2, 4, 7, 16)
+
+
+def cube_directions(rng, n):
+    """Random unit vectors (float32, normalised as the passes normalise), plus
+    the cases that select a face or a texel edge: the six axes, equal
+    magnitudes (face ties), directions just off the cube's edges and corners,
+    unnormalised and tiny vectors, and texel-centre hits."""
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d /= np.sqrt((d.astype(np.float64) ** 2).sum(1, keepdims=True)).astype(np.float32)
+    special = []
+    for ax in range(3):
+        for sg in (1.0, -1.0):
+            v = [0.0, 0.0, 0.0]
+            v[ax] = sg
+            special.append(v)
+    for a in (1.0, -1.0):
+        for b in (1.0, -1.0):
+            special += [[a, b, 0.0], [a, 0.0, b], [0.0, a, b], [a, b, 1.0], [a, b, -1.0]]
+    for e in (1e-6, 1e-3, 0.25):
+        special += [[1.0, 1.0 - e, 0.3], [1.0 - e, 1.0, -0.2], [-0.4, 1.0 - e, -1.0], [1.0, -1.0, 1.0 - e],
+                    [0.2, 1.0, 1.0 - e], [-1.0 + e, 0.5, 1.0]]
+    special += [[3.7, -1.2, 0.4], [0.0, -2.5e-3, 1e-3], [1e-20, 2e-20, -3e-20], [-5.0, 4.999, 0.0]]
+    special = np.asarray(special, np.float32)
+    d[: len(special)] = special
+    return d
+
+
+def make_cube():
+    from oracle.glsl import gl as G
+    rng = np.random.default_rng(77)
+    n = 4096
+    arrays = {}
+    src = ("#version 430 core\nlayout(local_size_x=64) in;\nuniform samplerCube sky;\n"
+           "layout(std430,binding=0) buffer I{float inp[];};\nlayout(std430,binding=1) buffer O{float o[];};\n"
+           "void main(){ uint i = gl_GlobalInvocationID.x; vec3 d = vec3(inp[3u*i], inp[3u*i+1u], inp[3u*i+2u]); "
+           "vec3 c = texture(sky, d).rgb; o[3u*i] = c.x; o[3u*i+1u] = c.y; o[3u*i+2u] = c.z; }\n")
+    prog = G.Program(src, "probe_cube")
+    for edge in CUBE_EDGES:
+        faces = rng.integers(0, 256, (6, edge, edge, 3)).astype(np.uint8)
+        dirs = cube_directions(rng, n)
+        # texel centres of random faces: the bilinear weights are exact there
+        k = np.arange(64, 128)
+        f, ti, tj = rng.integers(0, 6, 64), rng.integers(0, edge, 64), rng.integers(0, edge, 64)
+        sc = ((ti + 0.5) / edge * 2.0 - 1.0).astype(np.float32)
+        tc = ((tj + 0.5) / edge * 2.0 - 1.0).astype(np.float32)
+        one = np.ones(64, np.float32)
+        table = {0: (one, -tc, -sc), 1: (-one, -tc, sc), 2: (sc, one, tc), 3: (sc, -one, -tc), 4: (sc, -tc, one),
+                 5: (-sc, -tc, -one)}
+        for i in range(64):
+            x, y, z = table[int(f[i])]
+            dirs[k[i]] = (x[i], y[i], z[i])
+        cube = G.CubeMap([faces[i] for i in range(6)], edge)
+        prog.use()
+        cube.bind(prog, "sky", 1)
+        bi = G.Buffer(data=dirs.ravel())
+        bo = G.Buffer(nbytes=n * 12)
+        bi.bind_ssbo(0)
+        bo.bind_ssbo(1)
+        prog.dispatch(n // 64)
+        arrays["faces_%d" % edge] = faces
+        arrays["dirs_%d" % edge] = dirs
+        arrays["rgb_%d" % edge] = bo.read(np.float32).reshape(n, 3)
+        print("cube probe edge", edge, flush=True)
+    np.savez_compressed(os.path.join(HERE, "cube_probes.npz"), **arrays)
+
+
 # Frame-level runs: the whole runWavefrontFrame sequence over several frames.
 # "det" = the deterministic parity knobs (one sample per visible voxel,
 # caps above the sample count, no stale reset, no firefly clamp); "default" =
@@ -186,6 +256,55 @@ def make_frame():
         json.dump(meta, fh, indent=1, sort_keys=True)
 
 
+# Cube-map sky: the small scene under six random 8x8 RGB8 faces, so escaping
+# bounce rays (shade.comp:559) and sky pixels (resolve.comp:227) sample it;
+# a run with a tiny primary budget times pixels out, and the ones with no
+# usable neighbour within a 2-pixel radius fall back to the sky too
+# (resolve.comp:218).
+SKY_RUNS = {
+    "sky_det": dict(W=128, H=128, frames=4, max_bounces=3, knobs=DET_KNOBS),
+    "sky_timeout": dict(W=96, H=64, frames=2, max_bounces=2,
+                        knobs=dict(DET_KNOBS, primary_controlchecks=14, reconstruction_radius=2)),
+}
+
+
+def sky_faces(edge=8, seed=5):
+    return np.random.default_rng(seed).integers(0, 256, (6, edge, edge, 3)).astype(np.uint8)
+
+
+def make_sky():
+    from oracle import oracle as O
+    depth, ops = scenes.scene_small(1)
+    words, info = ve.octree_build(ops, depth)
+    mats = scenes.palette(1)
+    faces = sky_faces()
+    meta = {"depth": depth, "words_sha256": words_digest(words, info.size), "runs": {}}
+    arrays = {"sky_faces": faces}
+    for name, run in SKY_RUNS.items():
+        W, H = run["W"], run["H"]
+        fc = O.FrameConfig(**run["knobs"])
+        eye, d, fov = scenes.camera_small()
+        ubo = ve.camera_ubo(eye, d, fov, W / H)
+        pl = R.GLPipeline(words, depth, W, H, max_bounces=run["max_bounces"], materials=mats, sky_faces=faces)
+        pl.set_camera(ubo)
+        counters = []
+        for f in range(run["frames"]):
+            st = pl.frame(fc.as_dict(), f, frame_seed(f), f + 1)
+            counters.append([st["candidates"], st["scheduled"], st["bounces"], st["traced"]])
+            print(name, "frame", f, st, flush=True)
+        arrays[name + "_ubo"] = ubo
+        arrays[name + "_counters"] = np.array(counters, np.uint32)
+        g, _ = pl.read_gbuffer()
+        arrays[name + "_gbuf"] = g
+        arrays[name + "_out"] = pl.out.read()
+        lv, lval = hash_map(pl.read_hash(pl.lbuf, 384).reshape(-1), 12, 32)
+        arrays[name + "_lbuf_vids"], arrays[name + "_lbuf_vals"] = lv, lval
+        meta["runs"][name] = dict(W=W, H=H, frames=run["frames"], max_bounces=run["max_bounces"], knobs=run["knobs"])
+    np.savez_compressed(os.path.join(HERE, "frame_sky.npz"), **arrays)
+    with open(os.path.join(HERE, "frame_sky.json"), "w") as fh:
+        json.dump(meta, fh, indent=1, sort_keys=True)
+
+
 # Emissive scene (next-event estimation): ReSTIR with temporal reuse only,
 # and with spatial reuse as well; one sample per visible voxel per frame.
 LIGHT_KNOBS = dict(DET_KNOBS, restir_spatial=0)
@@ -236,12 +355,16 @@ if __name__ == "__main__":
         sys.exit("run with LP_NUM_THREADS=1 (deterministic llvmpipe dispatch order)")
     if not R.shaders_available():
         sys.exit("reference shaders not found at %s" % R.SHADER_DIR)
-    what = sys.argv[1:] or ["fp", "primary", "frame", "lights"]
+    what = sys.argv[1:] or ["fp", "cube", "primary", "frame", "lights", "sky"]
     if "fp" in what:
         make_fp()
+    if "cube" in what:
+        make_cube()
     if "primary" in what:
         make_primary()
     if "frame" in what:
         make_frame()
     if "lights" in what:
         make_lights()
+    if "sky" in what:
+        make_sky()

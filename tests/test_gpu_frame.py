@@ -97,7 +97,7 @@ def test_frames_vs_oracle_random(ve, scenes, oracle, small_scene):
             rend.set_materials(mats)
             of.set_materials(mats)
             rend.set_sky_constant((90, 150, 240))
-            of.set_sky(np.array([90, 150, 240], np.float32) / np.float32(255))
+            of.set_sky_rgb8((90, 150, 240))
             rend.set_camera(ubo)
             scene = rend.upload(words, depth, small_scene["info"].size)
             fc_g, fc_o = ve.FrameConfig(**knobs), oracle.FrameConfig(**knobs)
@@ -137,7 +137,7 @@ def test_timeout_reconstruction(ve, scenes, oracle, small_scene, radius):
         rend.set_materials(scenes.palette(1))
         of.set_materials(scenes.palette(1))
         rend.set_sky_constant((140, 180, 230))
-        of.set_sky(np.array([140, 180, 230], np.float32) / np.float32(255))
+        of.set_sky_rgb8((140, 180, 230))
         rend.set_camera(ubo)
         scene = rend.upload(words, depth, small_scene["info"].size)
         fc_g, fc_o = ve.FrameConfig(**knobs), oracle.FrameConfig(**knobs)

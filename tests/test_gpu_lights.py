@@ -116,7 +116,7 @@ def test_registry_persists_and_matches_oracle(ve, scenes, oracle, lights_scene):
         for x in (rend, of):
             x.set_materials(scenes.palette_lights(1))
         rend.set_sky_constant((140, 180, 230))
-        of.set_sky(np.array([140, 180, 230], np.float32) / np.float32(255))
+        of.set_sky_rgb8((140, 180, 230))
         sc = rend.upload(lights_scene["words"], lights_scene["depth"], lights_scene["info"].size)
         fc_g, fc_o = ve.FrameConfig(**knobs), oracle.FrameConfig(**knobs)
         for f, (e, d) in enumerate(views * 2):

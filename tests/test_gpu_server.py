@@ -4,7 +4,8 @@ Each session's frames must equal the same frames rendered alone through
 ve_frame on a private context (bit-exact under the order-independent knobs),
 whether the sessions overlap on their own streams, share one stream (the
 plain loop), wait for admission under a tight memory budget, or reuse a
-pooled context that an earlier session left behind."""
+pooled context that an earlier session left behind (queues=2: three
+sessions on two shared streams)."""
 import numpy as np
 import pytest
 
@@ -87,7 +88,7 @@ def _served(ve, scenes, small_scene, views, frames, fc, **server_kw):
         srv.close()
 
 
-@pytest.mark.parametrize("queues", [0, 1])
+@pytest.mark.parametrize("queues", [0, 1, 2])
 def test_sessions_alone_equal_in_mix(ve, scenes, small_scene, queues):
     print("case server mix queues", queues, flush=True)
     fc = ve.FrameConfig(**DET_KNOBS)

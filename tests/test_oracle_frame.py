@@ -24,7 +24,7 @@ def test_oracle_frames_match_llvmpipe(gold, oracle, small_scene, scenes, run):
     fc = oracle.FrameConfig(**r["knobs"])
     of = oracle.Frame(W, H, 16384, r["max_bounces"])
     of.set_materials(scenes.palette(1))
-    of.set_sky(np.array(meta["sky_rgb8"], np.float32) / np.float32(255))
+    of.set_sky_rgb8(meta["sky_rgb8"])
     words, depth = small_scene["words"], small_scene["depth"]
     counters = []
     for f in range(r["frames"]):

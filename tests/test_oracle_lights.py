@@ -34,7 +34,7 @@ def run_oracle(oracle, scenes, meta, arr, run, scene):
     fc = oracle.FrameConfig(**r["knobs"])
     of = oracle.Frame(W, H, 16384, r["max_bounces"])
     of.set_materials(scenes.palette_lights(1))
-    of.set_sky(np.array(meta["sky_rgb8"], np.float32) / np.float32(255))
+    of.set_sky_rgb8(meta["sky_rgb8"])
     counters = []
     for f in range(r["frames"]):
         of.frame(scene["words"], scene["depth"], arr[run + "_ubo"], fc, f, frame_seed(f), f + 1)

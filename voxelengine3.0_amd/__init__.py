@@ -406,6 +406,14 @@ class Renderer:
         self._ck(lib().ve_set_sky(self.h, arr, 1))
         self._sky_keep = face
 
+    def set_sky_cube(self, faces):
+        """Six (edge, edge, 3) uint8 faces, +X -X +Y -Y +Z -Z (skybox.cpp:87-122)."""
+        f = np.ascontiguousarray(faces, np.uint8)
+        if f.ndim != 4 or f.shape[0] != 6 or f.shape[1] != f.shape[2] or f.shape[3] != 3:
+            raise ValueError("faces must be (6, edge, edge, 3) uint8")
+        arr = (C.c_void_p * 6)(*[f[i].ctypes.data for i in range(6)])
+        self._ck(lib().ve_set_sky(self.h, arr, f.shape[1]))
+
     def run_pass(self, scene, name, fc, inputs):
         self._ck(lib().ve_pass(self.h, scene, PASS[name], C.byref(fc), C.byref(inputs)))
 

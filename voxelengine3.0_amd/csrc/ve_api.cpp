@@ -203,6 +203,8 @@ void fill_params(ve_ctx *c, const ve_scene *s, const ve_frame_config *fc, const 
     p.reconstruction_radius = fc->reconstruction_radius;
     std::memcpy(p.cam, c->cam, sizeof p.cam);
     std::memcpy(p.sky, c->sky, sizeof p.sky);
+    p.sky_edge = c->sky_edge;
+    p.sky_tex = c->sky_tex;
 }
 
 int check_frame(ve_ctx *c, const ve_scene *s, const ve_frame_config *fc, const ve_frame_inputs *in)
@@ -345,6 +347,7 @@ void ve_destroy(ve_ctx *c)
     if (c->aux) (void)hipStreamSynchronize(c->aux);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto &a : c->allocations) (void)hipFree(a.first);
+    if (c->sky_tex) (void)hipFree(c->sky_tex);
     for (auto &fe : c->frame_events)
         for (hipEvent_t e : fe)
             if (e) (void)hipEventDestroy(e);
@@ -455,17 +458,41 @@ int ve_set_sky(ve_ctx *c, const uint8_t *const faces[6], int edge)
 {
     if (!c) return ve_global_error(VE_EINVAL, "null context");
     if (!faces || edge < 1) return ctx_error(c, VE_EINVAL, "null faces or edge < 1");
-    // Constant skies only: every texel of every face must hold the same RGB8
-    // value, which bilinear filtering returns unchanged (c / 255).
-    const uint8_t *c0 = faces[0];
-    if (!c0) return ctx_error(c, VE_EINVAL, "null face");
-    for (int f = 0; f < 6; f++) {
+    if (edge > 16384) return ctx_error(c, VE_EINVAL, "cube-map edge %d above 16384", edge);
+    for (int f = 0; f < 6; f++)
         if (!faces[f]) return ctx_error(c, VE_EINVAL, "null face");
-        for (int i = 0; i < edge * edge; i++)
-            if (std::memcmp(faces[f] + 3 * i, c0, 3) != 0)
-                return ctx_error(c, VE_EINVAL, "only constant skies are supported (face %d texel %d differs)", f, i);
+    // A uniform sky (every texel of every face the same RGB8 value) filters to
+    // that texel for every direction: kept as a constant, c * (1/255) like
+    // the sampler's own result.
+    const uint8_t *c0 = faces[0];
+    const size_t texels = (size_t)edge * (size_t)edge;
+    bool uniform = true;
+    for (int f = 0; f < 6 && uniform; f++)
+        for (size_t i = 0; i < texels && uniform; i++) uniform = std::memcmp(faces[f] + 3 * i, c0, 3) == 0;
+    if (uniform) {
+        for (int i = 0; i < 3; i++) c->sky[i] = (float)c0[i] * (1.0f / 255.0f);
+        c->sky_edge = 0;
+        return VE_OK;
     }
-    for (int i = 0; i < 3; i++) c->sky[i] = (float)c0[i] / 255.0f;
+    std::vector<uint32_t> tex(6 * texels);
+    for (int f = 0; f < 6; f++)
+        for (size_t i = 0; i < texels; i++) {
+            const uint8_t *t = faces[f] + 3 * i;
+            tex[f * texels + i] = (uint32_t)t[0] | (uint32_t)t[1] << 8 | (uint32_t)t[2] << 16;
+        }
+    HIP_OK(c, hipSetDevice(c->device));
+    // frames already enqueued may still sample the current texture
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    if (tex.size() > c->sky_tex_words) {
+        if (c->sky_tex) HIP_OK(c, hipFree(c->sky_tex));
+        c->sky_tex = nullptr;
+        c->sky_tex_words = 0;
+        c->sky_edge = 0;
+        HIP_OK(c, hipMalloc(&c->sky_tex, tex.size() * sizeof(uint32_t)));
+        c->sky_tex_words = tex.size();
+    }
+    HIP_OK(c, hipMemcpy(c->sky_tex, tex.data(), tex.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    c->sky_edge = (uint32_t)edge;
     return VE_OK;
 }
 
@@ -909,6 +936,7 @@ int ctx_reset(ve_ctx *c)
     c->node_stats = false;
     std::memset(c->cam, 0, sizeof c->cam);
     std::memset(c->sky, 0, sizeof c->sky);
+    c->sky_edge = 0;
     return VE_OK;
 }
 

@@ -38,7 +38,10 @@ struct ve_ctx {
     uint32_t max_bounces = 8;
     std::string err;
     float cam[16] = {0};
-    float sky[3] = {0, 0, 0};
+    float sky[3] = {0, 0, 0};            // constant sky (sky_edge 0)
+    uint32_t sky_edge = 0;               // cube-map sky: texels per face edge
+    uint32_t *sky_tex = nullptr;         // 6 x sky_edge^2 texels, r | g << 8 | b << 16
+    size_t sky_tex_words = 0;            // allocated
     bool node_stats = false;
     bool timing = false;
     uint32_t bounce_iter = 0;           // next trace/shade iteration of per-pass dispatch

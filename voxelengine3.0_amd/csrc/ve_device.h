@@ -106,6 +106,73 @@ __device__ __forceinline__ f3 primary_dir(const float *cam, uint32_t x, uint32_t
     return normalize(d);
 }
 
+// texture(skybox, d).rgb under the reference skybox's sampler state
+// (skybox.cpp:113-117: GL_LINEAR, CLAMP_TO_EDGE, level 0, no seamless
+// filtering), evaluated as the GL driver the reference runs on does:
+//   face: major axis of the GL cube-map face table, ties toward z, then y;
+//   s, t = (sc / |ma| + 1) / 2, (tc / |ma| + 1) / 2;
+//   u = s * edge - 1/2 in 24.8 fixed point, rounded to nearest even: integer
+//   part = left texel, fraction = 8-bit weight; texels clamped to the face;
+//   bytes blended as a + (((b - a) w + 128) >> 8), along s, then t;
+//   result c * (1/255).
+__device__ __forceinline__ void cube_axis(float st, uint32_t edge, uint32_t &i0, uint32_t &i1, int &w)
+{
+    const float u = st * (float)(edge * 256u) - 128.0f;
+    const int x = (int)__builtin_rintf(u);
+    const int lo = x >> 8, top = (int)edge - 1;
+    w = x & 255;
+    i0 = (uint32_t)min(max(lo, 0), top);
+    i1 = (uint32_t)min(max(lo + 1, 0), top);
+}
+
+__device__ __forceinline__ int lerp8(int a, int b, int w) { return a + (((b - a) * w + 128) >> 8); }
+
+__device__ __noinline__ f3 sky_cube(const uint32_t *__restrict__ tex, uint32_t edge, float x, float y, float z)
+{
+    const float ax = fabsf(x), ay = fabsf(y), az = fabsf(z);
+    uint32_t face;
+    float sc, tc, ma;
+    if (az >= ax && az >= ay) {
+        face = z >= 0.0f ? 4u : 5u;
+        sc = z >= 0.0f ? x : -x;
+        tc = -y;
+        ma = az;
+    } else if (ay >= ax) {
+        face = y >= 0.0f ? 2u : 3u;
+        sc = x;
+        tc = y >= 0.0f ? z : -z;
+        ma = ay;
+    } else {
+        face = x >= 0.0f ? 0u : 1u;
+        sc = x >= 0.0f ? -z : z;
+        tc = -y;
+        ma = ax;
+    }
+    const float s = (sc / ma + 1.0f) * 0.5f, t = (tc / ma + 1.0f) * 0.5f;
+    uint32_t i0, i1, j0, j1;
+    int wa, wb;
+    cube_axis(s, edge, i0, i1, wa);
+    cube_axis(t, edge, j0, j1, wb);
+    const uint32_t *f = tex + (size_t)face * edge * edge;
+    const uint32_t t00 = f[j0 * edge + i0], t10 = f[j0 * edge + i1], t01 = f[j1 * edge + i0], t11 = f[j1 * edge + i1];
+    float c[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int sh = 8 * k;
+        const int top = lerp8((int)((t00 >> sh) & 255u), (int)((t10 >> sh) & 255u), wa);
+        const int bot = lerp8((int)((t01 >> sh) & 255u), (int)((t11 >> sh) & 255u), wa);
+        c[k] = (float)lerp8(top, bot, wb) * (1.0f / 255.0f);
+    }
+    return f3{c[0], c[1], c[2]};
+}
+
+// The sky seen along d: the cube map, or the constant colour.
+__device__ __forceinline__ f3 sky_color(const FrameParams &p, float x, float y, float z)
+{
+    if (p.sky_edge == 0u) return f3{p.sky[0], p.sky[1], p.sky[2]};
+    return sky_cube(p.sky_tex, p.sky_edge, x, y, z);
+}
+
 // Ray state after the setup of internal.glsl:352-366.
 struct DDA {
     f3 o, d, inv, r;

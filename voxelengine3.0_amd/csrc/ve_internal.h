@@ -134,6 +134,10 @@ struct FrameParams {
     // each hit's lattice position and the emit pass fills in the hit record
     // of a bounce-0 ray whose camera ray is the pixel's primary ray
     int32_t reuse_primary_hits;
+    // sky: a cube map of sky_edge^2 texels per face (faces +X -X +Y -Y +Z -Z,
+    // texel r | g << 8 | b << 16), or, with sky_edge 0, the constant `sky`
+    uint32_t sky_edge;
+    const uint32_t *sky_tex;
     float cam[16];
     float sky[3];
 };

@@ -1332,9 +1332,10 @@ __device__ __forceinline__ int shade_ray(const FrameParams &p, uint32_t iter, ui
     const uint32_t MB = p.max_bounces;
     if (r.bounce >= MB) return RAY_DONE;
     if (r.hit_vid == 0u) {   // miss: sky * throughput into the indirect channel
-        r.ilx = r.ilx + p.sky[0] * r.tx;
-        r.ily = r.ily + p.sky[1] * r.ty;
-        r.ilz = r.ilz + p.sky[2] * r.tz;
+        const f3 sky = sky_color(p, r.dx, r.dy, r.dz);
+        r.ilx = r.ilx + sky.x * r.tx;
+        r.ily = r.ily + sky.y * r.ty;
+        r.ilz = r.ilz + sky.z * r.tz;
         return RAY_DEPOSITS;
     }
     const uint32_t mi = r.hit_material < MATERIAL_SLOTS ? r.hit_material : MATERIAL_SLOTS - 1u;
@@ -1705,13 +1706,13 @@ __global__ __launch_bounds__(256) void k_resolve(FrameParams p, const uint4 *__r
         if (timeout) timeouts[base + (uint32_t)__popcll(tmask & ((1ull << lane) - 1ull))] = (uint32_t)pix;
     }
     if (!valid || timeout) return;
-    const f3 sky{p.sky[0], p.sky[1], p.sky[2]};
     f3 col;
     if (p.render_type == 1) {
         const float v = (float)checks / 65.0f;
         col = f3{v, v, v};
-    } else if (vid == 0u) {
-        col = sky;
+    } else if (vid == 0u) {   // texture(skybox, pixelDirection(pix)) (resolve.comp:170-180,227)
+        const f3 d = primary_dir(p.cam, x, y, p.W, p.H);
+        col = sky_color(p, d.x, d.y, d.z);
     } else if (p.render_type == 2) {
         col = (mi == 0u || mi >= 127u) ? f3{0, 0, 0} : f3{mats[mi].color[0], mats[mi].color[1], mats[mi].color[2]};
     } else if (p.render_type == 3) {
@@ -1815,7 +1816,13 @@ __global__ __launch_bounds__(256) void k_resolve_timeout(FrameParams p, const ui
         sb = wave_fsum(sb);
         ws = wave_fsum(ws);
         if (lane == 0u) {
-            const f3 col = ws > 0.0001f ? f3{sr / ws, sg / ws, sb / ws} : f3{p.sky[0], p.sky[1], p.sky[2]};
+            f3 col;
+            if (ws > 0.0001f) {
+                col = f3{sr / ws, sg / ws, sb / ws};
+            } else {   // resolve.comp:218
+                const f3 d = primary_dir(p.cam, (uint32_t)px, (uint32_t)py, p.W, p.H);
+                col = sky_color(p, d.x, d.y, d.z);
+            }
             out[pix] = make_float4(col.x, col.y, col.z, 1.0f);
         }
     }

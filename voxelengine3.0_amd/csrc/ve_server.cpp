@@ -213,7 +213,9 @@ int admit(ve_server *s)
         ve_ctx *c = ss.ctx;
         SRV_HIP(s, hipMemcpyAsync(c->b.materials, ss.materials.data(), ss.materials.size(), hipMemcpyHostToDevice,
                                   c->stream));
-        for (int i = 0; i < 3; i++) c->sky[i] = (float)ss.desc.sky_rgb8[i] / 255.0f;
+        // a constant sky, as ve_set_sky keeps a uniform cube
+        for (int i = 0; i < 3; i++) c->sky[i] = (float)ss.desc.sky_rgb8[i] * (1.0f / 255.0f);
+        c->sky_edge = 0;
         ss.state = State::Active;
         s->admitted++;
         while (!ss.pending.empty()) {

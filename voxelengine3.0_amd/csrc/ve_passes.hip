@@ -1569,38 +1569,48 @@ __global__ __launch_bounds__(256) void k_bin_scatter(FrameParams p, const Deferr
     }
 }
 
-// One wave per lBuffer bucket with deposits: the bucket is copied to LDS,
-// its deposits are applied there iteration by iteration (the lanes of one
-// iteration concurrently, as a dispatch's invocations are), and it is written
-// back; the bucket's count is cleared for the next frame.
+// APPLY_GROUP lanes per lBuffer bucket with deposits, 64 / APPLY_GROUP
+// buckets per wave: each bucket is copied to LDS, its deposits are applied
+// there iteration by iteration (the group's lanes concurrently within an
+// iteration, as a dispatch's invocations are), and it is written back; the
+// bucket's count is cleared for the next frame.  A bucket holds about a dozen
+// deferred deposits on average, so a whole wave per bucket would leave most
+// lanes idle through the owner scans and atomics.
+constexpr uint32_t APPLY_GROUP = 16, APPLY_BUCKETS = 64 / APPLY_GROUP;
+
 __global__ __launch_bounds__(64) void k_bin_apply(FrameParams p, const Deferred *__restrict__ sorted,
                                                   uint32_t *__restrict__ bin_count, uint32_t *__restrict__ bin_fill,
                                                   const uint32_t *__restrict__ bin_off, uint32_t *__restrict__ lbuf,
                                                   uint32_t first, uint32_t iters)
 {
-    __shared__ __align__(16) uint32_t bk[LB_WORDS];   // 16-byte aligned: owner rows are read as uint4
-    const uint32_t h = blockIdx.x, lane = threadIdx.x;
-    const uint32_t n = bin_count[h];
-    if (n == 0u) return;
-    uint32_t *gb = lbuf + (size_t)h * LB_WORDS;
-    for (uint32_t i = lane; i < LB_WORDS; i += 64u) bk[i] = gb[i];
+    __shared__ __align__(16) uint32_t bk[APPLY_BUCKETS][LB_WORDS];   // owner rows are read as uint4
+    const uint32_t g = threadIdx.x / APPLY_GROUP, gl = threadIdx.x % APPLY_GROUP;
+    const uint32_t h = blockIdx.x * APPLY_BUCKETS + g;
+    const uint32_t n = h < p.hash_width ? bin_count[h] : 0u;
+    if (__ballot(n != 0u) == 0ull) return;   // the whole wave (one wave per block)
+    uint4 *lb = reinterpret_cast<uint4 *>(bk[g]);
+    uint4 *gb = reinterpret_cast<uint4 *>(lbuf + (size_t)h * LB_WORDS);
+    if (n != 0u)
+        for (uint32_t i = gl; i < LB_WORDS / 4u; i += APPLY_GROUP) lb[i] = gb[i];
     __syncthreads();
-    const Deferred *list = sorted + bin_off[h];
+    const Deferred *list = sorted + (n != 0u ? bin_off[h] : 0u);
     for (uint32_t it = first; it < iters; it++) {
-        for (uint32_t base = 0; base < n; base += 64u) {
-            const uint32_t e = base + lane;
+        for (uint32_t base = 0; base < n; base += APPLY_GROUP) {
+            const uint32_t e = base + gl;
             if (e < n) {
                 const Deferred d = list[e];
                 if (d.iter == it)
-                    deposit_into(p, bk, d.vid, f3{d.dl[0], d.dl[1], d.dl[2]}, f3{d.il[0], d.il[1], d.il[2]});
+                    deposit_into(p, bk[g], d.vid, f3{d.dl[0], d.dl[1], d.dl[2]}, f3{d.il[0], d.il[1], d.il[2]});
             }
         }
         __syncthreads();
     }
-    for (uint32_t i = lane; i < LB_WORDS; i += 64u) gb[i] = bk[i];
-    if (lane == 0u) {
-        bin_count[h] = 0u;
-        bin_fill[h] = 0u;
+    if (n != 0u) {
+        for (uint32_t i = gl; i < LB_WORDS / 4u; i += APPLY_GROUP) gb[i] = lb[i];
+        if (gl == 0u) {
+            bin_count[h] = 0u;
+            bin_fill[h] = 0u;
+        }
     }
 }
 
@@ -1989,8 +1999,8 @@ int launch_deferred(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint3
         return VE_EDEVICE;
     launch(k_bin_scatter, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, p, b.deferred, b.counters, b.bin_off, b.bin_fill,
            b.deferred_sorted);
-    launch(k_bin_apply, dim3(p.hash_width), dim3(64), 0, s, p, b.deferred_sorted, b.bin_count, b.bin_fill, b.bin_off,
-           b.lbuf, first, iters);
+    launch(k_bin_apply, dim3(cdiv(p.hash_width, APPLY_BUCKETS)), dim3(64), 0, s, p, b.deferred_sorted, b.bin_count,
+           b.bin_fill, b.bin_off, b.lbuf, first, iters);
     return VE_OK;
 }
 

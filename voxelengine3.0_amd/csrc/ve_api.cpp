@@ -50,35 +50,19 @@ int ctx_error(ve_ctx *c, int code, const char *fmt, ...)
         if (e_ != hipSuccess) return ctx_error(ctx, VE_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
-template <typename T>
-int dalloc(ve_ctx *c, T **p, size_t count)
-{
-    void *ptr = nullptr;
-    hipError_t e = hipMalloc(&ptr, count * sizeof(T) ? count * sizeof(T) : 16);
-    if (e != hipSuccess) return ctx_error(c, VE_ENOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
-    // cleared on the context's stream, ahead of everything that uses it
-    e = hipMemsetAsync(ptr, 0, count * sizeof(T), c->stream);
-    if (e != hipSuccess) {
-        (void)hipFree(ptr);
-        return ctx_error(c, VE_EDEVICE, "hipMemsetAsync: %s", hipGetErrorString(e));
-    }
-    c->allocations.push_back({ptr, count * sizeof(T)});
-    *p = static_cast<T *>(ptr);
-    return VE_OK;
-}
-
-// Every device buffer of a context, in one list: allocates them (c set) or
-// only adds up their bytes (c null), so admission control can price a
-// context before creating it.
+// Every device buffer of a context, in one list, carved out of one
+// allocation at 256-byte offsets: with `base` null the list only adds up the
+// bytes (admission control prices a context before creating it), with `base`
+// set it hands out the pointers.  One hipMalloc and one clearing fill per
+// context instead of one of each per buffer.
 struct Alloc {
-    ve_ctx *c = nullptr;
+    uint8_t *base = nullptr;
     size_t bytes = 0;
-    int rc = VE_OK;
     template <typename T>
     void operator()(T **p, size_t count)
     {
+        if (base) *p = reinterpret_cast<T *>(base + bytes);
         bytes += (count * sizeof(T) + 255) & ~(size_t)255;
-        if (c && rc == VE_OK) rc = dalloc(c, p, count);
     }
 };
 
@@ -898,20 +882,27 @@ int ctx_create(ve_ctx **out, const ve_renderer_config *cfg, hipStream_t shared, 
         return VE_EDEVICE;
     }
     Alloc a;
-    a.c = c;
-    plan_buffers(a, c->b, c->max_w, c->max_h, c->hash_width, c->overlap);
-    if (a.rc == VE_OK) {
+    plan_buffers(a, c->b, c->max_w, c->max_h, c->hash_width, c->overlap);   // size
+    void *block = nullptr;
+    e = hipMalloc(&block, a.bytes);
+    if (e != hipSuccess) {
+        err = std::string("hipMalloc(") + std::to_string(a.bytes) + " bytes): " + hipGetErrorString(e);
+        ve_destroy(c);
+        return VE_ENOMEM;
+    }
+    c->allocations.push_back({block, a.bytes});
+    Alloc carve;
+    carve.base = static_cast<uint8_t *>(block);
+    plan_buffers(carve, c->b, c->max_w, c->max_h, c->hash_width, c->overlap);
+    // cleared on the context's stream, ahead of everything that uses it
+    e = hipMemsetAsync(block, 0, a.bytes, c->stream);
+    if (e == hipSuccess)
         e = hipMemcpyAsync(c->b.rank_stats, INITIAL_THRESHOLDS, sizeof INITIAL_THRESHOLDS, hipMemcpyHostToDevice,
                            c->stream);
-        if (e != hipSuccess) {
-            a.rc = VE_EDEVICE;
-            c->err = hipGetErrorString(e);
-        }
-    }
-    if (a.rc != VE_OK) {
-        err = c->err;
+    if (e != hipSuccess) {
+        err = hipGetErrorString(e);
         ve_destroy(c);
-        return a.rc;
+        return VE_EDEVICE;
     }
     c->device_bytes = a.bytes;
     *out = c;

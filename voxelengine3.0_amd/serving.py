@@ -72,7 +72,7 @@ def make_workload(lam, duration_s, seed=0, n_scenes=3):
         sc = int(rng.integers(0, n_scenes))
         W, H = int(rng.integers(256, 2561)), int(rng.integers(256, 1441))
         s = dict(arrival_s=t0, scene=sc, W=W, H=H, spp=int(rng.integers(1, 9)), bounces=int(rng.integers(1, 5)),
-                 length=length, seed=int(rng.integers(1, 1 << 30)))
+                 length=length, seed=int(rng.integers(1, 1 << 30)), running=bool(residual))
         s["frames_s"] = [t0 + k / CLIENT_FPS for k in range(length) if t0 + k / CLIENT_FPS < duration_s]
         if keep_rng.uniform() < keep and s["frames_s"]:
             sessions.append(s)
@@ -95,7 +95,13 @@ def _ubo(ve, s, depth, k):
 def run_workload(ve, srv, scene_ids, scene_depths, sessions, materials, sky=(140, 180, 230)):
     """Replay `sessions` in real time against `srv`; returns per-frame records
     (latency measured from each frame's scheduled arrival to the end of its
-    last kernel on the device)."""
+    last kernel on the device).  The sessions already running when the window
+    starts are opened before the clock starts, as a server in steady state
+    holds them; every later session is opened at its arrival.  Python's
+    cyclic garbage collector is held off during the replay (the loop stands
+    in for a native serving thread; a collection pass would pause
+    submissions for tens of milliseconds)."""
+    import gc
     events = []
     for i, s in enumerate(sessions):
         for k, ts in enumerate(s["frames_s"]):
@@ -103,7 +109,21 @@ def run_workload(ve, srv, scene_ids, scene_depths, sessions, materials, sky=(140
     events.sort()
     fcs = [ve.FrameConfig(**session_knobs(s["spp"])) for s in sessions]
     sid = [None] * len(sessions)
+    for i, s in enumerate(sessions):
+        if s.get("running"):
+            sid[i] = srv.open(scene_ids[s["scene"]], s["W"], s["H"], s["bounces"] + 1, materials, sky)
     records = []
+    gc.collect()
+    gc_was_on = gc.isenabled()
+    gc.disable()
+    try:
+        return _replay(ve, srv, scene_depths, scene_ids, sessions, events, fcs, sid, records, materials, sky)
+    finally:
+        if gc_was_on:
+            gc.enable()
+
+
+def _replay(ve, srv, scene_depths, scene_ids, sessions, events, fcs, sid, records, materials, sky):
     t_base = srv.now_ms() + 50.0              # schedule starts 50 ms from now
     run_workload.t_base = t_base
     n_events, e = len(events), 0
@@ -122,14 +142,15 @@ def run_workload(ve, srv, scene_ids, scene_depths, sessions, materials, sky=(140
             if k == len(s["frames_s"]) - 1:
                 srv.close_session(sid[i])
             e += 1
-        got = srv.poll(wait=False)
-        if not got and outstanding and (e >= n_events or t_base + events[e][0] * 1e3 > srv.now_ms() + 0.5):
-            got = srv.poll(wait=True)
+        # never block on the device while arrivals are still to come: a wait
+        # for the oldest frame in flight would hold back the frames falling
+        # due meanwhile
+        got = srv.poll(wait=e >= n_events and outstanding > 0)
         for d in got:
             outstanding -= 1
             records.append(d)
         if not got and e < n_events:
-            time.sleep(max(0.0, min(0.0005, (t_base + events[e][0] * 1e3 - srv.now_ms()) * 1e-3)))
+            time.sleep(max(0.0, min(0.0002, (t_base + events[e][0] * 1e3 - srv.now_ms()) * 1e-3)))
     return records
 
 

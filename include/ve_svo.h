@@ -67,6 +67,25 @@ int ve_octree_build(const ve_insert *ops, size_t n_ops, uint32_t depth, uint32_t
                     uint32_t **words_out, ve_octree_info *info_out);
 void ve_free(void *p);
 
+/* ---- synthetic cave terrain -------------------------------------------------
+ * No counterpart in the reference: the generator of the depth-12 divergence
+ * scene (a heightfield ground with 3D value-noise caves under its surface),
+ * a data source for tests and benchmarks whose voxels go through
+ * ve_octree_build.  Per column (x, z) of an n x n heightfield (hi = one past
+ * the top ground voxel, lo = first voxel of the watertight surface shell,
+ * row-major [x][z]):
+ *   cave   = hi - band <= y < hi and value_noise((x+.5)/cell, (y+.5)/cell, (z+.5)/cell) > threshold
+ *   solid  = 0 <= y < hi and not cave
+ *   output = solid and (y >= lo or a 6-neighbour inside the grid is a cave)
+ * value_noise: smoothstep trilinear blend (x, then y, then z) of lattice values
+ * (hash4(i, j, k, noise_seed) >> 8) / 2^24, in double precision; material
+ * 1 + (hash4(x, y, z, material_seed ^ 0x5EED) & 7); hash4 as in
+ * voxelengine3.0_amd/scenes.py.  band <= 63.  With out null or capacity below
+ * the voxel count, only *count is set.  threads 0 = up to 16. */
+int ve_gen_caves(uint32_t n, const int32_t *hi, const int32_t *lo, uint32_t band, uint32_t cell, double threshold,
+                 uint32_t noise_seed, uint32_t material_seed, uint32_t threads, ve_insert *out, size_t capacity,
+                 size_t *count);
+
 /* ---- renderer ---------------------------------------------------------------- */
 typedef struct ve_ctx ve_ctx;
 typedef struct ve_scene ve_scene;

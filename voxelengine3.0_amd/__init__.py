@@ -128,6 +128,7 @@ def lib():
     sig = {
         "ve_octree_build": (i32, [p, sz, u32, u32, C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(OctreeInfo)]),
         "ve_free": (None, [p]),
+        "ve_gen_caves": (i32, [u32, p, p, u32, u32, C.c_double, u32, u32, u32, p, sz, C.POINTER(sz)]),
         "ve_create": (i32, [C.POINTER(p), C.POINTER(RendererConfig)]),
         "ve_destroy": (None, [p]),
         "ve_last_error": (C.c_char_p, [p]),
@@ -214,6 +215,25 @@ def octree_build(ops, depth, max_words=1 << 27):
     finally:
         lib().ve_free(out)
     return words, info
+
+
+def gen_caves(hi, lo, band, cell, threshold, noise_seed, material_seed, threads=0):
+    """Synthetic cave terrain (ve_gen_caves): the voxels of a heightfield
+    ground with value-noise caves in a band under its surface, as an
+    insert-dtype array (hi, lo: n x n int32 column bounds, [x][z])."""
+    hi = np.ascontiguousarray(hi, np.int32)
+    lo = np.ascontiguousarray(lo, np.int32)
+    n = hi.shape[0]
+    if hi.shape != (n, n) or lo.shape != (n, n):
+        raise ValueError("hi and lo must be n x n")
+    cnt = C.c_size_t()
+    args = (n, hi.ctypes.data, lo.ctypes.data, band, cell, float(threshold), noise_seed & 0xFFFFFFFF,
+            material_seed & 0xFFFFFFFF, threads)
+    _check(lib().ve_gen_caves(*args, None, 0, C.byref(cnt)))
+    from .scenes import INSERT_DTYPE
+    out = np.empty(cnt.value, INSERT_DTYPE)
+    _check(lib().ve_gen_caves(*args, out.ctypes.data if len(out) else None, len(out), C.byref(cnt)))
+    return out
 
 
 class OctreeCPU:

@@ -195,6 +195,92 @@ def scene_extreme(seed=1, power=EXTREME_SPHERE_POWER):
 LIGHT_BOXES = (((28, 78, 40), (33, 83, 45)), ((88, 74, 70), (91, 77, 73)), ((52, 98, 56), (70, 99, 74)))
 
 
+def value_noise3(x, y, z, seed):
+    """3D value noise: smoothstep-weighted trilinear blend (x, then y, then z)
+    of lattice values unit(hash4(i, j, k, seed)), float64."""
+    ix, iy, iz = np.floor(x), np.floor(y), np.floor(z)
+    sx, sy, sz = [f * f * (3.0 - 2.0 * f) for f in (x - ix, y - iy, z - iz)]
+    ix, iy, iz = ix.astype(np.int64), iy.astype(np.int64), iz.astype(np.int64)
+    v = {(a, b, c): unit(hash4(ix + a, iy + b, iz + c, seed)) for a in (0, 1) for b in (0, 1) for c in (0, 1)}
+    a00 = v[0, 0, 0] + sx * (v[1, 0, 0] - v[0, 0, 0])
+    a10 = v[0, 1, 0] + sx * (v[1, 1, 0] - v[0, 1, 0])
+    a01 = v[0, 0, 1] + sx * (v[1, 0, 1] - v[0, 0, 1])
+    a11 = v[0, 1, 1] + sx * (v[1, 1, 1] - v[0, 1, 1])
+    b0 = a00 + sy * (a10 - a00)
+    b1 = a01 + sy * (a11 - a01)
+    return b0 + sz * (b1 - b0)
+
+
+# The divergence configuration: a depth-12 fbm ground whose top CAVE_BAND
+# voxels are hollowed by 3D value noise over a CAVE_CELL-voxel lattice above
+# CAVE_THRESHOLD (only the surface shell and the cave walls are voxels), plus
+# PILLAR_COUNT seeded 1x1 pillars standing on the ground, U[16, 512] tall.
+CAVE_BAND, CAVE_CELL, CAVE_THRESHOLD = 48, 32, 0.55
+PILLAR_COUNT, PILLAR_MIN, PILLAR_MAX = 2048, 16, 512
+
+
+def shell_columns(n, seed, thickness=1):
+    """(hi, lo) per column [x][z] of terrain_shell: hi = one past the top
+    voxel, lo = the first voxel of the watertight shell."""
+    H = heightfield(n, seed)
+    hi = np.clip(np.ceil(H), 0, n).astype(np.int64)
+    pad = np.pad(hi, 1, mode="edge")
+    nb = np.minimum.reduce([pad[2:, 1:-1], pad[:-2, 1:-1], pad[1:-1, 2:], pad[1:-1, :-2]])
+    lo = np.clip(np.minimum(hi - thickness, nb), 0, None)
+    return hi, lo
+
+
+def caves_numpy(hi, lo, band, cell, threshold, noise_seed, material_seed):
+    """The cave-terrain rule of ve_gen_caves restated over whole arrays (small
+    grids only): voxels sorted by (x, z, y)."""
+    n = hi.shape[0]
+    X, Y, Z = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    HI = hi[:, None, :].repeat(n, 1)          # [x][y][z]
+    LO = lo[:, None, :].repeat(n, 1)
+    v = value_noise3((X + 0.5) / cell, (Y + 0.5) / cell, (Z + 0.5) / cell, noise_seed)
+    cave = (Y >= HI - band) & (Y < HI) & (v > threshold)
+    solid = (Y < HI) & ~cave
+    pad = np.pad(cave, 1)
+    touch = (pad[2:, 1:-1, 1:-1] | pad[:-2, 1:-1, 1:-1] | pad[1:-1, 2:, 1:-1] | pad[1:-1, :-2, 1:-1] |
+             pad[1:-1, 1:-1, 2:] | pad[1:-1, 1:-1, :-2])
+    keep = solid & ((Y >= LO) | touch)
+    x, y, z = X[keep], Y[keep], Z[keep]
+    order = np.lexsort((y, z, x))
+    x, y, z = x[order], y[order], z[order]
+    return _ops(x, y, z, voxel_materials(x, y, z, material_seed))
+
+
+def pillar_ops(n, seed, hi, count=PILLAR_COUNT, hmin=PILLAR_MIN, hmax=PILLAR_MAX):
+    """`count` seeded 1x1 pillars: column x, z uniform in [0.05N, 0.95N),
+    standing on the ground (from hi up), height uniform in [hmin, hmax]."""
+    i = np.arange(count)
+    px = (0.05 * n + 0.9 * n * unit(hash4(i, 1, 0x9111, seed))).astype(np.int64)
+    pz = (0.05 * n + 0.9 * n * unit(hash4(i, 2, 0x9111, seed))).astype(np.int64)
+    h = hmin + (hash4(i, 3, 0x9111, seed) % U32(hmax - hmin + 1)).astype(np.int64)
+    base = hi[px, pz]
+    top = np.minimum(base + h, n)
+    cnt = np.maximum(top - base, 0)
+    k = np.repeat(i, cnt)
+    y = base[k] + (np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt))
+    x, z = px[k], pz[k]
+    return _ops(x, y, z, voxel_materials(x, y, z, seed))
+
+
+def scene_caves(seed=1, depth=12, patch=None, pillars=PILLAR_COUNT, threads=0):
+    """The divergence configuration at `depth` (12: 4096^3): fbm ground with
+    value-noise caves under its surface and `pillars` pillars.  With `patch`,
+    the ground covers only the patch x patch columns at the origin (its
+    heightfield that of a patch-sized grid) and the pillars stand on the
+    empty floor elsewhere: the same rules on a small voxel count."""
+    from . import gen_caves
+    n = 1 << depth
+    m = n if patch is None else int(patch)
+    hi_p, lo_p = shell_columns(m, seed)
+    caves = gen_caves(hi_p, lo_p, CAVE_BAND, CAVE_CELL, CAVE_THRESHOLD, seed ^ 0xCA7E, seed, threads)
+    hi = hi_p if m == n else np.pad(hi_p, ((0, n - m), (0, n - m)))
+    return depth, np.concatenate([caves, pillar_ops(n, seed, hi, count=pillars)])
+
+
 def scene_lights(seed=1):
     """The depth-7 terrain of scene_small plus three emissive blocks above it
     (a 5^3 cube, a 3^3 cube and an 18x1x18 ceiling panel) using materials 9

@@ -117,12 +117,57 @@ def cpu_baseline(words, depth, ubos, fc_kw, budget_s=20.0):
 SERVED_LAMS = {0: [1200, 1500, 1800, 2000, 2200], 1: [500, 700, 800, 900, 1000]}
 
 
-def served_and_batched(ve, extreme):
+def divergence(ve, caves, frames=32, warmup=4):
+    """The divergence configuration: the depth-12 cave scene (fbm ground with
+    value-noise caves and 2048 pillars), 2560x1440, 4 samples per visible
+    voxel (bottom rank tier), MAX_BOUNCES 5 (4 bounces), a 32-frame seeded
+    orbit; whole frames as an application renders them."""
+    from voxelengine_amd import scenes
+    depth, words, info = caves
+    W2, H2 = 2560, 1440
+    r = ve.Renderer(W2, H2, hash_width=16384, max_bounces=5)
+    try:
+        scene = r.upload(words, depth, info.size)
+        r.set_materials(scenes.palette(1))
+        r.set_sky_constant((140, 180, 230))
+        fc = ve.FrameConfig(**dict(knobs(), top_fraction=0.0, mid_fraction=0.0, min_samples=4))
+        ubos = []
+        for f in range(frames):
+            eye, d, fov = scenes.orbit_camera(1 << depth, f, frames, 1)
+            ubos.append(ve.camera_ubo(eye, d, fov, W2 / H2))
+
+        def frame(i):
+            r.set_camera(ubos[i % frames])
+            r.frame(scene, fc, ve.FrameInputs(W2, H2, i, frame_seed(i), i + 1))
+
+        for i in range(warmup):
+            frame(i)
+        r.sync()
+        r.reset_ray_totals()
+        t0 = time.perf_counter()
+        for i in range(warmup, warmup + frames):
+            frame(i)
+        r.sync()
+        dt = time.perf_counter() - t0
+        t = r.ray_totals()
+        rays = t["primary"] + t["traced"] + t["shadow"]
+        out = dict(workload="svo-depth12-caves-1440p-4spp-4bounce-orbit32", voxels=int(info.num_voxels),
+                   node_words=int(info.size), frames=frames, ms_per_frame=round(dt / frames * 1e3, 3),
+                   mrays_per_s=round(rays / dt / 1e6, 1), rays_per_frame=round(rays / frames, 1),
+                   traced_rays_per_frame=round(t["traced"] / frames, 1))
+        r.release(scene)
+        return out
+    finally:
+        r.close()
+
+
+def served_and_batched(ve, extreme, caves):
     """Served: Poisson sessions over three resident fbm octrees (depth 9/10/11),
     3 s of measured arrivals per offered load (after 0.5 s of warm-up), on the
     concurrent server and on the plain
-    loop (one stream); batched: 64 uneven viewports over the depth-10 scene
-    and fbm depth 9 / 11 octrees, 6 frames each, both ways."""
+    loop (one stream); batched: 64 uneven viewports over the depth-10 scene,
+    the depth-12 cave scene and fbm depth 9 / 11 octrees, 6 frames each, both
+    ways."""
     from voxelengine_amd import serving
 
     def log(m):
@@ -140,9 +185,9 @@ def served_and_batched(ve, extreme):
                                          "ratio": round(best_c / best_p, 3) if best_p else None},
     }
     d9, d11 = serving.served_scenes(((9, 7), (11, 8)))
-    b = serving.batched(ve, [extreme, d9, d11], 64, 6, 0, (0, 1), log=log)
+    b = serving.batched(ve, [extreme, caves, d9, d11], 64, 6, 0, (0, 1), log=log)
     batched = {
-        "workload": "64 viewports on {depth-10 extreme, fbm depth 9 seed 7, fbm depth 11 seed 8}, res "
+        "workload": "64 viewports on {depth-10 extreme, depth-12 caves, fbm depth 9 seed 7, fbm depth 11 seed 8}, res "
                     "U[128..1920]x[128..1080], spp {1,2,4}, bounces {1,2,4}, 6 frames per viewport",
         "concurrent": b[0],
         "plain": b[1],
@@ -262,9 +307,24 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(words, depth, ubos, knobs())
 
+    # the divergence configuration (depth-12 cave scene), also resident in
+    # the batched set
+    r.release(scene)
+    r.close()
+    del ubos
+    from voxelengine_amd import scenes as S
+    t0 = time.perf_counter()
+    cdepth, cops = S.scene_caves(1)
+    cwords, cinfo = ve.octree_build(cops, cdepth, max_words=1 << 28)
+    t_caves = time.perf_counter() - t0
+    del cops
+    caves = (cdepth, cwords, cinfo)
+    div = divergence(ve, caves)
+    div["scene_build_s"] = round(t_caves, 2)
+
     served = batched = None
     if not args.no_served:
-        served, batched = served_and_batched(ve, (depth, words, info))
+        served, batched = served_and_batched(ve, (depth, words, info), caves)
 
     out = {
         "metric": "Mrays/s @1080p 10.4M-voxel SVO 2-bounce",
@@ -317,11 +377,10 @@ def main():
             "steps_per_ray": round(st["primary_steps"] / n_stat / (W * H), 3),
         },
         "cpu_baseline": cpu,
+        "divergence": div,
         "served": served,
         "batched": batched,
     }
-    r.release(scene)
-    r.close()
     print(json.dumps(out))
 
 

@@ -151,7 +151,9 @@ def divergence(ve, caves, frames=32, warmup=4):
         dt = time.perf_counter() - t0
         t = r.ray_totals()
         rays = t["primary"] + t["traced"] + t["shadow"]
+        c = r.counters()
         out = dict(workload="svo-depth12-caves-1440p-4spp-4bounce-orbit32", voxels=int(info.num_voxels),
+                   candidates_per_frame=int(c[ve.CNT["CANDIDATES"]]), claim_order_exact=bool(c[ve.CNT["OVERFLOW"]] == 0),
                    node_words=int(info.size), frames=frames, ms_per_frame=round(dt / frames * 1e3, 3),
                    mrays_per_s=round(rays / dt / 1e6, 1), rays_per_frame=round(rays / frames, 1),
                    traced_rays_per_frame=round(t["traced"] / frames, 1))

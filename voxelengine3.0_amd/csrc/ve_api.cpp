@@ -106,6 +106,10 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.node_stats, 4);
     a(&b.ray_totals, 4);
     a(&b.sched_tab, (size_t)CLAIM_WIDTH * SCHED_SLOTS);
+    a(&b.sched_over, SCHED_OVER_CAP);
+    a(&b.sched_over_n, 2);
+    a(&b.sched_row_over, CLAIM_WIDTH);
+    a(&b.sched_rows, CLAIM_WIDTH);
     a(&b.cand_at, groups_max * 64);
     a(&b.vec_cnt, groups_max * 8);
     a(&b.vec_base, groups_max * 8);

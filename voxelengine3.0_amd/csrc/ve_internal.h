@@ -33,6 +33,11 @@ constexpr uint32_t TRUSTED_THRESHOLD = 128;    // resolve.comp:82
 constexpr uint32_t RANK_STATS_WORDS = 264;     // wavefront.glsl:243-255
 constexpr uint32_t NUM_COUNTERS = 16;
 constexpr uint32_t SCHED_SLOTS = 128;          // ordering table: voxels per claim row before overflow
+// Voxels past a claim row's table go to a per-frame overflow list, and their
+// rows are ranked exactly by a separate pass: up to SCHED_OVER_CAP records
+// per frame and SCHED_ROW_OVER of them in one row (beyond either, the claim
+// order is no longer the reference's and VE_CNT_OVERFLOW is set).
+constexpr uint32_t SCHED_OVER_CAP = 4096, SCHED_ROW_OVER = 2048;
 constexpr uint32_t MATERIAL_SLOTS = 127;       // shade.comp:90
 constexpr uint32_t DEFERRED_BLOCKS = 256;      // deferred-deposit scatter grid (entries strided over it)
 // A deposit of the fused bounce pass (bounce iteration `iter`), applied

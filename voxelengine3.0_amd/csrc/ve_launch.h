@@ -66,6 +66,10 @@ struct DeviceBuffers {
     unsigned long long *ray_totals = nullptr;  // 4: primary, traced, shadow, traced with a reused hit (cumulative)
     // schedule / rank ordering scratch
     uint4 *sched_tab = nullptr;          // CLAIM_WIDTH x SCHED_SLOTS: {voxel id, ~lowest execution rank, pixels, 0}
+    uint4 *sched_over = nullptr;         // SCHED_OVER_CAP: {voxel id, ~rank, pixels, claim row} past a full row table
+    uint32_t *sched_over_n = nullptr;    // 2: overflow records, flagged rows (zero between frames)
+    uint32_t *sched_row_over = nullptr;  // CLAIM_WIDTH: 1 where the row has overflow records (zero between frames)
+    uint32_t *sched_rows = nullptr;      // CLAIM_WIDTH: the flagged rows
     uint32_t *cand_at = nullptr;         // per execution rank: claim row/probe of the voxel it first claims
     uint32_t *vec_cnt = nullptr;         // per SIMD-8 vector: entries appended
     uint32_t *vec_base = nullptr;        // exclusive scan of vec_cnt

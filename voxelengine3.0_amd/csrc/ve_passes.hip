@@ -299,7 +299,9 @@ __device__ __forceinline__ uint32_t sched_home(uint32_t v)
 // makes a zeroed slot "free" and lets one 64-bit CAS both claim the slot and
 // keep the lowest rank; the pixel count is added on the same cache line.
 __global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *__restrict__ gbuf, uint4 *__restrict__ tab,
-                                                    uint32_t *__restrict__ cand_at, uint32_t *__restrict__ counters)
+                                                    uint32_t *__restrict__ cand_at, uint32_t *__restrict__ counters,
+                                                    uint4 *__restrict__ over, uint32_t *__restrict__ over_n,
+                                                    uint32_t *__restrict__ row_over, uint32_t *__restrict__ rows)
 {
     const uint32_t lane = threadIdx.x;
     const uint32_t e = blockIdx.x * 64u + lane;
@@ -317,7 +319,8 @@ __global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *
         if (lane == (uint32_t)leader) lanes = (uint32_t)__popcll(same);
     }
     if (lanes == 0u) return;
-    uint4 *row = tab + (size_t)(claim_hash(vid) % CLAIM_WIDTH) * SCHED_SLOTS;
+    const uint32_t h = claim_hash(vid) % CLAIM_WIDTH;
+    uint4 *row = tab + (size_t)h * SCHED_SLOTS;
     const unsigned long long want = ((unsigned long long)~e << 32) | vid;
     uint32_t s = sched_home(vid);
     for (uint32_t probe = 0; probe < SCHED_SLOTS; probe++, s = (s + 1u) & (SCHED_SLOTS - 1u)) {
@@ -333,7 +336,14 @@ __global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *
         atomicAdd(&row[s].z, lanes);
         return;
     }
-    atomicOr(&counters[VE_CNT_OVERFLOW], 1u);   // more distinct ids than the row region holds
+    // the row's table is full: an overflow record, and the row is ranked by
+    // k_sched_rank_over
+    const uint32_t i = atomicAdd(&over_n[0], 1u);
+    if (i < SCHED_OVER_CAP)
+        over[i] = make_uint4(vid, ~e, lanes, h);
+    else
+        atomicOr(&counters[VE_CNT_OVERFLOW], 1u);
+    if (atomicExch(&row_over[h], 1u) == 0u) rows[atomicAdd(&over_n[1], 1u)] = h;
 }
 
 // One wave per claim row: a voxel's probe index in the reference's claim map is
@@ -342,11 +352,13 @@ __global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *
 // no separate clear is needed), marks each kept voxel's first pixel, and
 // clears the row region for the next frame.
 __global__ __launch_bounds__(64) void k_sched_rank(uint4 *__restrict__ tab, uint32_t *__restrict__ claim,
-                                                   uint32_t *__restrict__ samples, uint32_t *__restrict__ cand_at)
+                                                   uint32_t *__restrict__ samples, uint32_t *__restrict__ cand_at,
+                                                   const uint32_t *__restrict__ row_over)
 {
     constexpr uint32_t PER_LANE = SCHED_SLOTS / 64u;
     __shared__ uint32_t firsts[SCHED_SLOTS];
     const uint32_t h = blockIdx.x, lane = threadIdx.x;
+    if (row_over[h] != 0u) return;   // k_sched_rank_over ranks it
     uint4 *row = tab + (size_t)h * SCHED_SLOTS;
     uint4 m[PER_LANE];
 #pragma unroll
@@ -383,10 +395,117 @@ __global__ __launch_bounds__(64) void k_sched_rank(uint4 *__restrict__ tab, uint
     }
 }
 
+constexpr uint32_t SCHED_OVER_BLOCKS = 64;   // blocks striding over the flagged rows
+
+// The rows whose table filled up: members are the table's voxels plus the
+// row's overflow records, where one voxel may have several records (one per
+// work group that saw it first in the row's probe order); a voxel's first
+// execution rank is its records' lowest and its pixel count their sum. Then
+// the same ranking as k_sched_rank. One block per flagged row; the table row
+// and the flag are cleared for the next frame.
+__global__ __launch_bounds__(256) void k_sched_rank_over(uint4 *__restrict__ tab, uint32_t *__restrict__ claim,
+                                                         uint32_t *__restrict__ samples, uint32_t *__restrict__ cand_at,
+                                                         const uint4 *__restrict__ over, const uint32_t *__restrict__ over_n,
+                                                         uint32_t *__restrict__ row_over, const uint32_t *__restrict__ rows,
+                                                         uint32_t *__restrict__ counters)
+{
+    __shared__ uint32_t tv[SCHED_SLOTS], tf[SCHED_SLOTS], tc[SCHED_SLOTS];
+    __shared__ uint32_t rv[SCHED_ROW_OVER], rf[SCHED_ROW_OVER], rc[SCHED_ROW_OVER];
+    __shared__ uint32_t nt, nr, nrep;
+    const uint32_t t = threadIdx.x;
+    const uint32_t no = min(over_n[0], SCHED_OVER_CAP), nrows = over_n[1];
+    for (uint32_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+        const uint32_t h = rows[r];
+        uint4 *row = tab + (size_t)h * SCHED_SLOTS;
+        if (t == 0u) {
+            nt = 0u;
+            nr = 0u;
+            nrep = 0u;
+        }
+        __syncthreads();
+        for (uint32_t k = t; k < SCHED_SLOTS; k += blockDim.x) {
+            const uint4 m = row[k];
+            if (m.x != 0u) {
+                const uint32_t j = atomicAdd(&nt, 1u);
+                tv[j] = m.x;
+                tf[j] = ~m.y;
+                tc[j] = m.z;
+            }
+            row[k] = make_uint4(0u, 0u, 0u, 0u);
+        }
+        for (uint32_t i = t; i < no; i += blockDim.x) {
+            const uint4 o = over[i];
+            if (o.w != h) continue;
+            const uint32_t j = atomicAdd(&nr, 1u);
+            if (j < SCHED_ROW_OVER) {
+                rv[j] = o.x;
+                rf[j] = ~o.y;
+                rc[j] = o.z;
+            }
+        }
+        __syncthreads();
+        const uint32_t n_t = nt, n_r = min(nr, SCHED_ROW_OVER);
+        if (t == 0u && nr > SCHED_ROW_OVER) atomicOr(&counters[VE_CNT_OVERFLOW], 1u);
+        // representative record of each voxel: its lowest first rank (ranks
+        // are distinct, so exactly one record per voxel)
+        for (uint32_t i = t; i < n_r; i += blockDim.x) {
+            bool rep = true;
+            for (uint32_t j = 0; j < n_r && rep; j++) rep = !(rv[j] == rv[i] && rf[j] < rf[i]);
+            if (rep) atomicAdd(&nrep, 1u);
+        }
+        __syncthreads();
+        const uint32_t n_all = n_t + nrep;
+        for (uint32_t m = t; m < n_t + n_r; m += blockDim.x) {
+            uint32_t vid, f, cnt;
+            if (m < n_t) {
+                vid = tv[m];
+                f = tf[m];
+                cnt = tc[m];
+            } else {
+                const uint32_t i = m - n_t;
+                vid = rv[i];
+                f = rf[i];
+                cnt = 0u;
+                bool rep = true;
+                for (uint32_t j = 0; j < n_r; j++) {
+                    if (rv[j] != vid) continue;
+                    cnt += rc[j];
+                    if (rf[j] < f) rep = false;
+                }
+                if (!rep) continue;
+            }
+            uint32_t rank = 0;
+            for (uint32_t j = 0; j < n_t; j++) rank += tf[j] < f;
+            for (uint32_t j = 0; j < n_r; j++) {
+                if (rf[j] >= f) continue;
+                bool rep = true;   // count each other voxel once, at its representative
+                for (uint32_t q = 0; q < n_r && rep; q++) rep = !(rv[q] == rv[j] && rf[q] < rf[j]);
+                rank += rep;
+            }
+            if (rank < CLAIM_PROBES) {
+                claim[(size_t)h * CLAIM_PROBES + rank] = vid;
+                samples[(size_t)h * CLAIM_PROBES + rank] = cnt;
+                cand_at[f] = (rank << 26) | (h << 6) | rank;
+            }
+        }
+        for (uint32_t q = n_all + t; q < CLAIM_PROBES; q += blockDim.x) {
+            claim[(size_t)h * CLAIM_PROBES + q] = 0u;
+            samples[(size_t)h * CLAIM_PROBES + q] = 0u;
+        }
+        if (t == 0u) row_over[h] = 0u;
+        __syncthreads();
+    }
+}
+
 // candidates first-claimed in each SIMD-8 vector
-__global__ void k_sched_vec_count(const uint32_t *__restrict__ cand_at, uint32_t n_vec, uint32_t *__restrict__ vec_cnt)
+__global__ void k_sched_vec_count(const uint32_t *__restrict__ cand_at, uint32_t n_vec, uint32_t *__restrict__ vec_cnt,
+                                  uint32_t *__restrict__ over_n)
 {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v == 0u) {   // the overflow records of this frame are consumed
+        over_n[0] = 0u;
+        over_n[1] = 0u;
+    }
     if (v >= n_vec) return;
     uint32_t c = 0;
     for (uint32_t l = 0; l < 8; l++) c += cand_at[v * 8u + l] != NONE;
@@ -1902,9 +2021,13 @@ int launch_schedule(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
     const uint32_t gw = (p.W + 7u) / 8u, gh = (p.H + 7u) / 8u;
     const uint32_t groups = gw * gh;
     const uint32_t n_vec = groups * 8u;
-    launch(k_sched_claim, dim3(groups), dim3(64), 0, s, p, b.gbuf, b.sched_tab, b.cand_at, b.counters);
-    launch(k_sched_rank, dim3(CLAIM_WIDTH), dim3(64), 0, s, b.sched_tab, b.claim, b.samples, b.cand_at);
-    launch(k_sched_vec_count, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, b.cand_at, n_vec, b.vec_cnt);
+    launch(k_sched_claim, dim3(groups), dim3(64), 0, s, p, b.gbuf, b.sched_tab, b.cand_at, b.counters, b.sched_over,
+           b.sched_over_n, b.sched_row_over, b.sched_rows);
+    launch(k_sched_rank, dim3(CLAIM_WIDTH), dim3(64), 0, s, b.sched_tab, b.claim, b.samples, b.cand_at,
+           b.sched_row_over);
+    launch(k_sched_rank_over, dim3(SCHED_OVER_BLOCKS), dim3(256), 0, s, b.sched_tab, b.claim, b.samples, b.cand_at,
+           b.sched_over, b.sched_over_n, b.sched_row_over, b.sched_rows, b.counters);
+    launch(k_sched_vec_count, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, b.cand_at, n_vec, b.vec_cnt, b.sched_over_n);
     size_t tb = b.scan_temp_bytes;
     if (hipcub::DeviceScan::ExclusiveSum(b.scan_temp, tb, b.vec_cnt, b.vec_base, (int)n_vec, s) != hipSuccess)
         return VE_EDEVICE;

@@ -235,6 +235,7 @@ typedef enum ve_buffer_id {
 #define VE_CNT_EM_BASE       12   /* emissive registry size before this frame's registrations (internal) */
 #define VE_CNT_EM_NEW        13   /* registrations made by this frame's first shade (internal) */
 #define VE_CNT_DEFERRED      14   /* deposits of later bounce iterations held back in the fused bounce pass (internal) */
+#define VE_CNT_TIMEOUT_TILES 15   /* 16x16 screen tiles holding a timed-out pixel (last resolve; internal) */
 
 int ve_read(ve_ctx *ctx, ve_buffer_id id, void *dst, size_t bytes);
 int ve_write(ve_ctx *ctx, ve_buffer_id id, const void *src, size_t bytes);

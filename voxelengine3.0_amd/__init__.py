@@ -36,7 +36,8 @@ PASS_NAMES = [k for k, _ in sorted(PASS.items(), key=lambda kv: kv[1])]
 BUF = dict(GBUFFER=0, GBUFFER_POS=1, NBUFFER=2, LBUFFER=3, RBUFFER=4, CANDIDATES=5, SHADE_LIST=6, RAYS=7,
            OUTPUT=8, COUNTERS=9, RANK_STATS=10, SAMPLE_COUNTS=11, CLAIM_MAP=12, EMISSIVES=13, DISPLAY=14)
 CNT = dict(CANDIDATES=0, SHADE=1, EMISSIVES=2, LIVE=3, RAYS_TRACED=4, SHADOW_RAYS=5, PRIMARY_RAYS=6,
-           BOUNCES_RUN=7, LIVE_B=8, OVERFLOW=9, REUSED=10, TIMEOUTS=11, EM_BASE=12, EM_NEW=13)
+           BOUNCES_RUN=7, LIVE_B=8, OVERFLOW=9, REUSED=10, TIMEOUTS=11, EM_BASE=12, EM_NEW=13, DEFERRED=14,
+           TIMEOUT_TILES=15)
 
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("pad0", "<f4"), ("direction", "<f4", 3), ("pad1", "<f4"),
                       ("throughput", "<f4", 3), ("primary_vid", "<u4"), ("direct", "<f4", 3), ("rng", "<u4"),

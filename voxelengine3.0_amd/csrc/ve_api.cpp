@@ -79,6 +79,8 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     b.gpos = b.gpos_set[0];
     b.ghit = b.ghit_set[0];
     a(&b.output, P); a(&b.display, P); a(&b.pix_avg, P); a(&b.pix_cnt, P); a(&b.timeouts, P);
+    a(&b.timeout_tiles, (size_t)((max_w + RECON_TILE - 1) / RECON_TILE) * ((max_h + RECON_TILE - 1) / RECON_TILE));
+    a(&b.timeout_list, (size_t)((max_w + RECON_TILE - 1) / RECON_TILE) * ((max_h + RECON_TILE - 1) / RECON_TILE));
     a(&b.nbuf, (size_t)hash_width * NBUFFER_STRIDE * NBUFFER_SLOTS);
     a(&b.lbuf, (size_t)hash_width * LBUFFER_STRIDE * LBUFFER_SLOTS);
     a(&b.rbuf, (size_t)hash_width * RBUFFER_STRIDE * RBUFFER_SLOTS);

@@ -56,6 +56,13 @@ constexpr uint32_t RANK_COPIES = 32, RANK_COPY_WORDS = 3 + HISTOGRAM_BINS;
 // iteration into BOUNCE_COPIES copies of {rays, iterations run}.
 constexpr uint32_t BOUNCE_COPIES = 32;
 constexpr uint32_t RESOLVE_TIMEOUT_BLOCKS = 1024; // 4 waves each, striding over the timed-out pixels
+// Timed-out pixels are reconstructed per 16x16 tile from LDS for radii up to
+// RECON_TILED_MAX (the default is 10); larger radii gather per pixel.
+constexpr uint32_t RECON_TILE = 16;
+constexpr int RECON_TILED_MAX = 12;
+constexpr uint32_t RECON_TILE_BLOCKS = 4096;  // blocks striding over the listed tiles
+constexpr uint32_t RECON_DENSE = 24;          // timed-out pixels from which a tile stages its window in LDS
+constexpr uint32_t RECON_TILED_MIN = 65536;   // timed-out pixels in a frame from which dense tiles are tiled
 
 // Device layout of one hash bucket (one column h of the reference's R32UI
 // images).  The reference interleaves each slot's fields (slot s, field f at

@@ -38,6 +38,8 @@ struct DeviceBuffers {
     float4 *pix_avg = nullptr;      // P: per-pixel voxel average (rgb, indirect variance)
     uint32_t *pix_cnt = nullptr;    // P: per-pixel voxel sample count
     uint32_t *timeouts = nullptr;   // P: timed-out pixels (resolve)
+    uint32_t *timeout_tiles = nullptr;  // per 16x16 tile: holds a timed-out pixel (zero between frames)
+    uint32_t *timeout_list = nullptr;   // the flagged 16x16 tiles, in listing order
     uint32_t *nbuf = nullptr;       // hash_width x 96
     uint32_t *lbuf = nullptr;       // hash_width x 384
     uint32_t *rbuf = nullptr;       // hash_width x 128

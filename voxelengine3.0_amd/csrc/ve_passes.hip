@@ -325,8 +325,16 @@ __global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *
     uint32_t s = sched_home(vid);
     for (uint32_t probe = 0; probe < SCHED_SLOTS; probe++, s = (s + 1u) & (SCHED_SLOTS - 1u)) {
         unsigned long long *kf = reinterpret_cast<unsigned long long *>(row + s);
-        unsigned long long cur = atomicCAS(kf, 0ull, want);
+        // the home slot is claimed outright (usually free); past it, an id
+        // never leaves its slot, so a plain read passes other ids' slots
+        // without an atomic and only a free-looking slot is claimed
+        unsigned long long cur = probe == 0u ? atomicCAS(kf, 0ull, want)
+                                             : __hip_atomic_load(kf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (cur != 0ull && (uint32_t)cur != vid) continue;   // another id's slot
+        if (cur == 0ull && probe != 0u) {
+            cur = atomicCAS(kf, 0ull, want);
+            if (cur != 0ull && (uint32_t)cur != vid) continue;   // another id took it first
+        }
         // free (now ours) or this id's: keep the lower rank (higher complement)
         while (cur != 0ull && (uint32_t)(cur >> 32) < ~e) {
             const unsigned long long prev = atomicCAS(kf, cur, want);
@@ -395,7 +403,7 @@ __global__ __launch_bounds__(64) void k_sched_rank(uint4 *__restrict__ tab, uint
     }
 }
 
-constexpr uint32_t SCHED_OVER_BLOCKS = 64;   // blocks striding over the flagged rows
+constexpr uint32_t SCHED_OVER_BLOCKS = 256;  // blocks striding over the flagged rows
 
 // The rows whose table filled up: members are the table's voxels plus the
 // row's overflow records, where one voxel may have several records (one per
@@ -1778,9 +1786,17 @@ __device__ Avg voxel_avg(const uint32_t *__restrict__ lbuf, uint32_t hw, uint32_
 // Lanes of a tile that see the same voxel share one lookup.
 __global__ __launch_bounds__(256) void k_resolve_avg(FrameParams p, const uint4 *__restrict__ gbuf,
                                                      const uint32_t *__restrict__ lbuf, float4 *__restrict__ pix_avg,
-                                                     uint32_t *__restrict__ pix_cnt, uint32_t *__restrict__ counters)
+                                                     uint32_t *__restrict__ pix_cnt, uint32_t *__restrict__ counters,
+                                                     uint32_t *__restrict__ tcount)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) counters[VE_CNT_TIMEOUTS] = 0u;   // k_resolve appends after this pass
+    if (blockIdx.x == 0 && threadIdx.x == 0) {   // k_resolve appends after this pass
+        counters[VE_CNT_TIMEOUTS] = 0u;
+        counters[VE_CNT_TIMEOUT_TILES] = 0u;
+    }
+    {   // and counts timed-out pixels per 16x16 tile from zero
+        const uint32_t n_t = ((p.W + RECON_TILE - 1u) / RECON_TILE) * ((p.H + RECON_TILE - 1u) / RECON_TILE);
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_t; i += gridDim.x * blockDim.x) tcount[i] = 0u;
+    }
     uint32_t x = 0, y = 0;
     const bool valid = tile_pixel(p, x, y);
     const size_t pix = valid ? (size_t)y * p.W + x : 0;
@@ -1817,7 +1833,8 @@ __global__ __launch_bounds__(256) void k_resolve(FrameParams p, const uint4 *__r
                                                  const uint32_t *__restrict__ lbuf, const uint32_t *__restrict__ nbuf,
                                                  const Material *__restrict__ mats, const float4 *__restrict__ pix_avg,
                                                  const uint32_t *__restrict__ pix_cnt, float4 *__restrict__ out,
-                                                 uint32_t *__restrict__ timeouts, uint32_t *__restrict__ counters)
+                                                 uint32_t *__restrict__ timeouts, uint32_t *__restrict__ counters,
+                                                 uint32_t *__restrict__ tflag, uint32_t *__restrict__ tlist)
 {
     uint32_t x = 0, y = 0;
     const bool valid = tile_pixel(p, x, y);
@@ -1833,6 +1850,13 @@ __global__ __launch_bounds__(256) void k_resolve(FrameParams p, const uint4 *__r
         if ((int)lane == first) base = atomicAdd(&counters[VE_CNT_TIMEOUTS], (uint32_t)__popcll(tmask));
         base = __shfl(base, first, 64);
         if (timeout) timeouts[base + (uint32_t)__popcll(tmask & ((1ull << lane) - 1ull))] = (uint32_t)pix;
+        // the 16x16 tiles of the tiled gather that hold a timed-out pixel (a
+        // wave's 8x8 tile lies in one of them), listed once each
+        if ((int)lane == first && p.reconstruction_radius <= RECON_TILED_MAX) {
+            const uint32_t t16 = (y / RECON_TILE) * ((p.W + RECON_TILE - 1u) / RECON_TILE) + x / RECON_TILE;
+            if (atomicAdd(&tflag[t16], (uint32_t)__popcll(tmask)) == 0u)
+                tlist[atomicAdd(&counters[VE_CNT_TIMEOUT_TILES], 1u)] = t16;
+        }
     }
     if (!valid || timeout) return;
     f3 col;
@@ -1908,21 +1932,134 @@ __device__ __forceinline__ float wave_fsum(float v)
 // only the order of the final float additions differs from its sequential
 // loop (all terms are non-negative, so the sums agree to a few ulp).  Waves
 // stride over the list and exit once it is exhausted.
+// reconstructTimeout (resolve.comp:193-219) for radii up to RECON_TILED_MAX,
+// on the 16x16 screen tiles holding RECON_DENSE or more timed-out pixels
+// (k_resolve lists and counts them; sparser tiles go to k_resolve_timeout):
+// blocks stride over the listed tiles, and a dense one stages, for the tile
+// and its R-pixel halo, each pixel's tap weight base min(count, 32) (0 where
+// the tap is skipped: no voxel, timed out itself, no samples, or outside the
+// screen) with its voxel average in LDS; each timed-out pixel then sums its
+// (2R+1)^2 window over 16 lanes (partial sums reduced across them; the image
+// is compared within 1e-4).  Only in frames with RECON_TILED_MIN timed-out
+// pixels or more: below that, staging windows costs more than it saves.
+__global__ __launch_bounds__(256) void k_resolve_timeout_tiled(FrameParams p, const uint4 *__restrict__ gbuf,
+                                                               const float4 *__restrict__ pix_avg,
+                                                               const uint32_t *__restrict__ pix_cnt,
+                                                               uint32_t *__restrict__ tflag,
+                                                               const uint32_t *__restrict__ tlist,
+                                                               const uint32_t *__restrict__ counters,
+                                                               float4 *__restrict__ out)
+{
+    extern __shared__ float4 tap[];   // (16 + 2R)^2 taps
+    __shared__ uint32_t todo[RECON_TILE * RECON_TILE];
+    __shared__ uint32_t n_todo;
+    if (counters[VE_CNT_TIMEOUTS] < RECON_TILED_MIN) return;   // k_resolve_timeout did every pixel
+    const uint32_t tw = (p.W + RECON_TILE - 1u) / RECON_TILE;
+    const uint32_t n_tiles = counters[VE_CNT_TIMEOUT_TILES];
+    const int R = min(max(p.reconstruction_radius, 0), (int)RECON_TILED_MAX);
+    const int D = (int)RECON_TILE + 2 * R;
+    for (uint32_t it = blockIdx.x; it < n_tiles; it += gridDim.x) {
+        const uint32_t tile = tlist[it];
+        const int tx0 = (int)((tile % tw) * RECON_TILE), ty0 = (int)((tile / tw) * RECON_TILE);
+        if (tflag[tile] < RECON_DENSE) continue;   // sparse tiles: k_resolve_timeout did them
+        if (threadIdx.x == 0u) n_todo = 0u;
+        __syncthreads();
+        {   // the tile's timed-out pixels, packed onto the first threads
+            const int px = tx0 + (int)(threadIdx.x % RECON_TILE), py = ty0 + (int)(threadIdx.x / RECON_TILE);
+            if (px < (int)p.W && py < (int)p.H && p.render_type != 1) {
+                const uint4 g = gbuf[(size_t)py * p.W + px];
+                if (g.w == 0u && g.x >= p.primary_cc) todo[atomicAdd(&n_todo, 1u)] = threadIdx.x;
+            }
+        }
+        __syncthreads();
+        const uint32_t n = n_todo;
+        {
+            // dense tile: stage the window's taps once, sum each pixel's
+            // window sequentially in the reference's order (dy, then dx)
+            for (int i = (int)threadIdx.x; i < D * D; i += (int)blockDim.x) {
+                const int qx = tx0 - R + i % D, qy = ty0 - R + i / D;
+                float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (qx >= 0 && qy >= 0 && qx < (int)p.W && qy < (int)p.H) {
+                    const size_t q = (size_t)qy * p.W + qx;
+                    const uint4 nd = gbuf[q];
+                    const uint32_t nc = pix_cnt[q];
+                    if (nd.w != 0u && nd.x < p.primary_cc && nc != 0u) {
+                        const float4 na = pix_avg[q];
+                        v = make_float4(na.x, na.y, na.z, (float)min(nc, 32u));
+                    }
+                }
+                tap[i] = v;
+            }
+            __syncthreads();
+            // 16 lanes per timed-out pixel, each summing every 16th tap of
+            // the window; the partial sums meet in a 16-lane reduction
+            const uint32_t g = threadIdx.x / 16u, gl = threadIdx.x % 16u;
+            const int K = 2 * R + 1, NT = K * K;
+            for (uint32_t base = 0; base < n; base += blockDim.x / 16u) {
+                const uint32_t k = base + g;
+                float sr = 0.0f, sg = 0.0f, sb = 0.0f, ws = 0.0f;
+                int lx = 0, ly = 0;
+                if (k < n) {
+                    const uint32_t me = todo[k];
+                    lx = (int)(me % RECON_TILE);
+                    ly = (int)(me / RECON_TILE);
+                    for (int t = (int)gl; t < NT; t += 16) {
+                        const int dx = t % K - R, dy = t / K - R;
+                        if (dx == 0 && dy == 0) continue;
+                        const float4 e = tap[(ly + R + dy) * D + (lx + R + dx)];
+                        if (e.w == 0.0f) continue;
+                        const float dist = (float)(dx * dx + dy * dy);
+                        const float w = e.w / (1.0f + dist * 0.15f);
+                        sr = sr + e.x * w;
+                        sg = sg + e.y * w;
+                        sb = sb + e.z * w;
+                        ws = ws + w;
+                    }
+                }
+                for (int off = 8; off > 0; off >>= 1) {
+                    sr += __shfl_xor(sr, off, 16);
+                    sg += __shfl_xor(sg, off, 16);
+                    sb += __shfl_xor(sb, off, 16);
+                    ws += __shfl_xor(ws, off, 16);
+                }
+                if (k < n && gl == 0u) {
+                    const int px = tx0 + lx, py = ty0 + ly;
+                    f3 col;
+                    if (ws > 0.0001f) {
+                        col = f3{sr / ws, sg / ws, sb / ws};
+                    } else {   // resolve.comp:218
+                        const f3 d = primary_dir(p.cam, (uint32_t)px, (uint32_t)py, p.W, p.H);
+                        col = sky_color(p, d.x, d.y, d.z);
+                    }
+                    out[(size_t)py * p.W + px] = make_float4(col.x, col.y, col.z, 1.0f);
+                }
+            }
+        }
+        __syncthreads();   // before the next tile reuses the LDS
+    }
+}
+
 __global__ __launch_bounds__(256) void k_resolve_timeout(FrameParams p, const uint4 *__restrict__ gbuf,
                                                          const float4 *__restrict__ pix_avg,
                                                          const uint32_t *__restrict__ pix_cnt,
                                                          const uint32_t *__restrict__ timeouts,
                                                          const uint32_t *__restrict__ counters,
+                                                         const uint32_t *__restrict__ tcount,
                                                          float4 *__restrict__ out)
 {
     const int R = p.reconstruction_radius < 0 ? 0 : p.reconstruction_radius;
+    const uint32_t tw = (p.W + RECON_TILE - 1u) / RECON_TILE;
     const int D = 2 * R + 1, NT = D * D;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t n = counters[VE_CNT_TIMEOUTS];
+    // in a frame with many timed-out pixels, those of dense tiles are left to
+    // k_resolve_timeout_tiled
+    const bool tiled = R <= RECON_TILED_MAX && n >= RECON_TILED_MIN;
     const uint32_t waves = gridDim.x * (blockDim.x >> 6);
     for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += waves) {
         const uint32_t pix = timeouts[i];
         const int px = (int)(pix % p.W), py = (int)(pix / p.W);
+        if (tiled && tcount[(uint32_t)(py / (int)RECON_TILE) * tw + (uint32_t)px / RECON_TILE] >= RECON_DENSE) continue;
         float sr = 0.0f, sg = 0.0f, sb = 0.0f, ws = 0.0f;
         for (int t = (int)lane; t < NT; t += 64) {
             const int dx = t % D - R, dy = t / D - R;
@@ -2148,17 +2285,26 @@ void launch_display(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 {
     launch(k_resolve_avg, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.pix_avg, b.pix_cnt,
-                       b.counters);
+                       b.counters, b.timeout_tiles);
     launch(k_resolve, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.nbuf, b.materials,
-                       b.pix_avg, b.pix_cnt, b.output, b.timeouts, b.counters);
-    launch(k_resolve_timeout, dim3(RESOLVE_TIMEOUT_BLOCKS), dim3(256), 0, s, p, b.gbuf, b.pix_avg,
-                       b.pix_cnt, b.timeouts, b.counters, b.output);
+                       b.pix_avg, b.pix_cnt, b.output, b.timeouts, b.counters, b.timeout_tiles, b.timeout_list);
+    // timed-out pixels one wave each; in timeout-heavy frames, the tiles
+    // holding RECON_DENSE or more go to the tiled pass instead
+    launch(k_resolve_timeout, dim3(RESOLVE_TIMEOUT_BLOCKS), dim3(256), 0, s, p, b.gbuf, b.pix_avg, b.pix_cnt,
+           b.timeouts, b.counters, b.timeout_tiles, b.output);
+    if (p.reconstruction_radius <= (int)RECON_TILED_MAX) {
+        const uint32_t D = RECON_TILE + 2u * (uint32_t)std::max(p.reconstruction_radius, 0);
+        const uint32_t tiles = cdiv(p.W, RECON_TILE) * cdiv(p.H, RECON_TILE);
+        launch(k_resolve_timeout_tiled, dim3(std::min(tiles, RECON_TILE_BLOCKS)), dim3(256),
+               (uint32_t)(D * D * sizeof(float4)), s, p, b.gbuf, b.pix_avg, b.pix_cnt, b.timeout_tiles,
+               b.timeout_list, b.counters, b.output);
+    }
 }
 
 size_t scan_temp_bytes(uint32_t max_items)
 {
     size_t bytes = 0;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (uint32_t *)nullptr, (uint32_t *)nullptr, (int)max_items);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (uint32_t *)nullptr, (uint32_t *)nullptr, (int)max_items);
     return bytes;
 }
 

@@ -159,6 +159,8 @@ void fill_params(ve_ctx *c, const ve_scene *s, const ve_frame_config *fc, const 
     p.H = in->height;
     p.depth = s ? s->depth : 0;
     p.n_words = s ? s->n_words : 0;
+    p.jump = s ? s->jump : nullptr;
+    p.jump_level = s ? s->jump_level : 0;
     p.hash_width = c->hash_width;
     p.max_bounces = c->max_bounces;
     p.frame_idx = in->frame_idx;
@@ -385,7 +387,10 @@ int ve_scene_update(ve_ctx *c, ve_scene *s, const uint32_t *words, uint32_t n_wo
         s->words = static_cast<uint32_t *>(d);
         s->alloc_words = alloc;
         s->n_words = n_words;
-        s->device_bytes = (size_t)alloc * 4;
+        s->device_bytes = (size_t)alloc * 4 + ((size_t)4 << (3u * s->jump_level));
+        launch_build_jump(c->stream, s->words, s->n_words, s->depth, s->jump_level, s->jump);
+        HIP_OK(c, hipGetLastError());
+        HIP_OK(c, hipStreamSynchronize(c->stream));
         return VE_OK;
     }
     std::vector<uint2> pairs;
@@ -409,10 +414,11 @@ int ve_scene_update(ve_ctx *c, ve_scene *s, const uint32_t *words, uint32_t n_wo
         HIP_OK(c, hipMemcpyAsync(s->staging, pairs.data(), pairs.size() * sizeof(uint2), hipMemcpyHostToDevice,
                                  c->stream));
         launch_scatter_words(c->stream, s->words, s->staging, (uint32_t)pairs.size());
-        HIP_OK(c, hipGetLastError());
-        HIP_OK(c, hipStreamSynchronize(c->stream));
     }
     s->n_words = n_words;
+    launch_build_jump(c->stream, s->words, s->n_words, s->depth, s->jump_level, s->jump);   // after the new words
+    HIP_OK(c, hipGetLastError());
+    HIP_OK(c, hipStreamSynchronize(c->stream));
     return VE_OK;
 }
 
@@ -965,10 +971,23 @@ int scene_upload(int device, hipStream_t stream, const uint32_t *words, uint32_t
         delete s;
         return err = std::string("hipMalloc node array: ") + hipGetErrorString(e), VE_ENOMEM;
     }
+    const uint32_t jl = std::min(JUMP_LEVEL, depth - 1u);
+    void *j = nullptr;
+    e = hipMalloc(&j, (size_t)4 << (3u * jl));
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        delete s;
+        return err = std::string("hipMalloc jump table: ") + hipGetErrorString(e), VE_ENOMEM;
+    }
     e = hipMemcpyAsync(d, words, (size_t)n_words * 4, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) {
+        launch_build_jump(stream, static_cast<const uint32_t *>(d), n_words, depth, jl, static_cast<uint32_t *>(j));
+        e = hipGetLastError();
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) {
         (void)hipFree(d);
+        (void)hipFree(j);
         delete s;
         return err = std::string("node upload: ") + hipGetErrorString(e), VE_EDEVICE;
     }
@@ -976,7 +995,9 @@ int scene_upload(int device, hipStream_t stream, const uint32_t *words, uint32_t
     s->n_words = n_words;
     s->alloc_words = n_words;
     s->depth = depth;
-    s->device_bytes = (size_t)n_words * 4;
+    s->jump = static_cast<uint32_t *>(j);
+    s->jump_level = jl;
+    s->device_bytes = (size_t)n_words * 4 + ((size_t)4 << (3u * jl));
     *scene = s;
     return VE_OK;
 }
@@ -985,6 +1006,7 @@ void scene_free(ve_scene *s)
 {
     if (!s) return;
     (void)hipFree(s->words);
+    if (s->jump) (void)hipFree(s->jump);
     if (s->staging) (void)hipFree(s->staging);
     delete s;
 }

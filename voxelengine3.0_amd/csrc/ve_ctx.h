@@ -23,6 +23,11 @@ struct ve_scene {
     size_t device_bytes = 0;
     uint2 *staging = nullptr;          // (slot, word) pairs of ve_scene_update
     size_t staging_cap = 0;
+    // Jump table: for each cell of the grid at depth jump_level (8^level
+    // cells), the outcome of the descent from the root through that depth
+    // (rebuilt whenever the words change).
+    uint32_t *jump = nullptr;
+    uint32_t jump_level = 0;
 };
 
 constexpr int MAX_ITERS = 8;   // host bounce-loop cap (renderer.cpp:515)

@@ -227,15 +227,21 @@ __device__ __forceinline__ int dda_start(DDA &s, f3 o, f3 d, f3 inv, uint32_t de
 // 0..D-hb-1 read exactly the words they read last step.  The descent resumes
 // at depth D-hb from the `next` pointer cached for depth D-hb-1 (in LDS, one
 // column per thread: entry dd at cache[dd * stride]), or reuses the last
-// outcome outright when it ended above that depth.  Results are identical to
-// the full descent; only the repeated loads are skipped.  `reads` still
+// outcome outright when it ended above that depth.  A descent that would
+// resume at or above depth jl (the step left its cell of the depth-jl grid)
+// takes the outcome through depth jl from the scene's jump table in one load
+// and continues below it (positions on the grid's far faces, whose
+// coordinate bit D takes part in the reference's locate(), descend from the
+// root instead).  Results are identical to the full descent; only the
+// repeated loads are skipped.  `reads` still
 // counts the words the reference's full descent reads (its algorithmic
 // traffic, 4 bytes each).  The loop carries only the traversal state; the hit
 // record is formed at the exit.
 template <bool STATS>
-__device__ __forceinline__ Hit raycast(const uint32_t *__restrict__ words, uint32_t n_words, uint32_t depth, f3 o,
-                                       f3 d, f3 inv, uint32_t controlchecks, uint32_t &checks, uint32_t &reads,
-                                       uint32_t *cache, uint32_t stride)
+__device__ __forceinline__ Hit raycast(const uint32_t *__restrict__ words, const uint32_t *__restrict__ jump,
+                                       uint32_t jl, uint32_t n_words, uint32_t depth, f3 o, f3 d, f3 inv,
+                                       uint32_t controlchecks, uint32_t &checks, uint32_t &reads, uint32_t *cache,
+                                       uint32_t stride)
 {
     DDA s;
     if (dda_start(s, o, d, inv, depth, checks) != DDA_CONTINUE) return Hit{0, 0, 0, 0, 0, 0, 0};
@@ -260,14 +266,28 @@ __device__ __forceinline__ Hit raycast(const uint32_t *__restrict__ words, uint3
         if (resume > pstop) {
             dd = pstop;   // same words down to the one that ended the last descent
         } else {
-            uint32_t offset = resume == 0u ? 0u : cache[(resume - 1u) * stride];
-            for (dd = resume; dd < depth; dd++) {
+            uint32_t offset, from = resume;
+            if (resume > jl) {
+                offset = cache[(resume - 1u) * stride];
+            } else if (((ux | uy | uz) >> depth) == 0u) {
+                const uint32_t sh = depth - jl;
+                const uint32_t e = jump[((ux >> sh) << (2u * jl)) | ((uy >> sh) << jl) | (uz >> sh)];
+                offset = e;
+                from = jl + 1u;
+                if (e & JUMP_ENDED) from = depth + 1u;   // ended at depth (e & 0xFF): no loop, no leaf
+                else cache[jl * stride] = e;
+            } else {
+                offset = 0u;
+                from = 0u;
+            }
+            for (dd = from; dd < depth; dd++) {
                 offset += locate_bit(ux, uy, uz, depth - dd);
                 const uint32_t raw = node_word(words, n_words, offset);
                 if (!(raw & 1u)) break;
                 offset = raw >> 4;
                 cache[dd * stride] = offset;
             }
+            if (from == depth + 1u) dd = offset & 0xFFu;
             if (dd == depth) {   // bottom level: the leaf slot
                 offset += locate_bit(ux, uy, uz, 0u);
                 const uint32_t raw = node_word(words, n_words, offset);

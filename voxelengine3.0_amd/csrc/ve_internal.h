@@ -55,6 +55,10 @@ constexpr uint32_t RANK_COPIES = 32, RANK_COPY_WORDS = 3 + HISTOGRAM_BINS;
 // The fused bounce pass's waves flush their traced-ray counts and deepest
 // iteration into BOUNCE_COPIES copies of {rays, iterations run}.
 constexpr uint32_t BOUNCE_COPIES = 32;
+// Jump tables are built at depth min(JUMP_LEVEL, scene depth - 1): 8^6
+// entries (1 MiB) stay resident in each XCD's L2.
+constexpr uint32_t JUMP_LEVEL = 6;
+constexpr uint32_t JUMP_ENDED = 0x80000000u;   // entry flag: the descent ended at depth (entry & 0xFF)
 constexpr uint32_t RESOLVE_TIMEOUT_BLOCKS = 1024; // 4 waves each, striding over the timed-out pixels
 // Timed-out pixels are reconstructed per 16x16 tile from LDS for radii up to
 // RECON_TILED_MAX (the default is 10); larger radii gather per pixel.
@@ -150,6 +154,9 @@ struct FrameParams {
     // texel r | g << 8 | b << 16), or, with sky_edge 0, the constant `sky`
     uint32_t sky_edge;
     const uint32_t *sky_tex;
+    // the scene's jump table (see raycast)
+    const uint32_t *jump;
+    uint32_t jump_level;
     float cam[16];
     float sky[3];
 };

@@ -59,7 +59,7 @@ __global__ __launch_bounds__(256) void k_primary(const uint32_t *__restrict__ wo
     if (x < p.W && y < p.H) {
         const f3 dir = primary_dir(p.cam, x, y, p.W, p.H);
         const f3 org = mk(p.cam[0], p.cam[1], p.cam[2]);
-        const Hit h = raycast<STATS>(words, p.n_words, p.depth, org, dir, rcp3(dir), p.primary_cc, checks, reads,
+        const Hit h = raycast<STATS>(words, p.jump, p.jump_level, p.n_words, p.depth, org, dir, rcp3(dir), p.primary_cc, checks, reads,
                                      path_cache + threadIdx.x, 256u);
         const size_t pix = (size_t)y * p.W + x;
         if (h.hit) {
@@ -97,6 +97,41 @@ void launch_primary(hipStream_t s, const uint32_t *words, const FrameParams &p, 
     else
         launch(k_primary<false>, dim3(nb), dim3(256), 0, s, words, p, b.gbuf, b.gpos, b.node_stats,
                            b.counters, b.ray_totals, b.ghit);
+}
+
+// Jump table: one thread per cell of the grid at depth `level` (cells of
+// 2^(depth - level) voxels, index x | y | z bits of the cell coordinates):
+// the descent of raycast from the root through depth `level` depends only on
+// the coordinate bits above the cell, so its outcome is the same for every
+// position in the cell -- the child block of the node at depth `level`, or
+// the depth at which an empty node ended it (JUMP_ENDED | depth).
+__global__ void k_build_jump(const uint32_t *__restrict__ words, uint32_t n_words, uint32_t depth, uint32_t level,
+                             uint32_t *__restrict__ jump)
+{
+    const uint32_t cells = 1u << (3u * level);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    const uint32_t sh = depth - level;
+    const uint32_t ux = (i >> (2u * level)) << sh, uy = ((i >> level) & ((1u << level) - 1u)) << sh,
+                   uz = (i & ((1u << level) - 1u)) << sh;
+    uint32_t offset = 0;
+    for (uint32_t dd = 0; dd <= level; dd++) {
+        offset += locate_bit(ux, uy, uz, depth - dd);
+        const uint32_t raw = node_word(words, n_words, offset);
+        if (!(raw & 1u)) {
+            jump[i] = JUMP_ENDED | dd;
+            return;
+        }
+        offset = raw >> 4;
+    }
+    jump[i] = offset;
+}
+
+void launch_build_jump(hipStream_t s, const uint32_t *words, uint32_t n_words, uint32_t depth, uint32_t level,
+                       uint32_t *jump)
+{
+    const uint32_t cells = 1u << (3u * level);
+    launch(k_build_jump, dim3((cells + 255u) / 256u), dim3(256), 0, s, words, n_words, depth, level, jump);
 }
 
 }  // namespace ve

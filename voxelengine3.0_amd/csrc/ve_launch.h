@@ -99,6 +99,9 @@ int launch_deferred(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint3
 void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 void launch_display(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 void launch_scatter_words(hipStream_t s, uint32_t *words, const uint2 *pairs, uint32_t n);
+// The jump table of a node array (8^level entries).
+void launch_build_jump(hipStream_t s, const uint32_t *words, uint32_t n_words, uint32_t depth, uint32_t level,
+                       uint32_t *jump);
 size_t scan_temp_bytes(uint32_t max_items);
 
 }  // namespace ve

@@ -858,7 +858,8 @@ __device__ __forceinline__ void trace_ray(const uint32_t *__restrict__ words, co
 {
     if (r.bounce >= p.max_bounces) return;
     const f3 o{r.ox, r.oy, r.oz}, d{r.dx, r.dy, r.dz};
-    const Hit h = raycast<STATS>(words, p.n_words, p.depth, o, d, rcp3(d), p.bounce_cc, checks, reads, cache, 256u);
+    const Hit h = raycast<STATS>(words, p.jump, p.jump_level, p.n_words, p.depth, o, d, rcp3(d), p.bounce_cc, checks, reads,
+                                 cache, 256u);
     if (h.hit) {
         r.ox = (float)h.px;
         r.oy = (float)h.py;
@@ -1289,7 +1290,8 @@ __device__ bool visible_to(const FrameParams &p, NeeCtx &c, f3 o, f3 d, uint32_t
 {
     uint32_t checks = 0, reads = 0;
     c.shadow_rays++;
-    const Hit h = raycast<false>(c.words, p.n_words, p.depth, o, d, rcp3(d), p.bounce_cc, checks, reads, c.cache, 256u);
+    const Hit h = raycast<false>(c.words, p.jump, p.jump_level, p.n_words, p.depth, o, d, rcp3(d), p.bounce_cc, checks,
+                                 reads, c.cache, 256u);
     return h.hit && voxel_key(h.px, h.py, h.pz) + 1u == target;
 }
 

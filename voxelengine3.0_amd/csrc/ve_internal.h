@@ -55,9 +55,11 @@ constexpr uint32_t RANK_COPIES = 32, RANK_COPY_WORDS = 3 + HISTOGRAM_BINS;
 // The fused bounce pass's waves flush their traced-ray counts and deepest
 // iteration into BOUNCE_COPIES copies of {rays, iterations run}.
 constexpr uint32_t BOUNCE_COPIES = 32;
-// Jump tables are built at depth min(JUMP_LEVEL, scene depth - 1): 8^6
-// entries (1 MiB) stay resident in each XCD's L2.
-constexpr uint32_t JUMP_LEVEL = 6;
+// Jump tables are built at depth min(JUMP_LEVEL, scene depth - 1): 8^8
+// entries (64 MiB, resident in the MALL), rebuilt in well under a
+// millisecond after an edit.  Measured on the depth-10 frame: level 6 (1 MiB)
+// 3015, 7 3110, 8 3207, 9 (512 MiB) 3244 Mrays/s.
+constexpr uint32_t JUMP_LEVEL = 8;
 constexpr uint32_t JUMP_ENDED = 0x80000000u;   // entry flag: the descent ended at depth (entry & 0xFF)
 constexpr uint32_t RESOLVE_TIMEOUT_BLOCKS = 1024; // 4 waves each, striding over the timed-out pixels
 // Timed-out pixels are reconstructed per 16x16 tile from LDS for radii up to

@@ -114,7 +114,7 @@ def cpu_baseline(words, depth, ubos, fc_kw, budget_s=20.0):
 # offered loads (frames/s) of the served sweep, per mode (each row also
 # reports the load its Poisson realisation actually offered); the sweep then
 # bisects between the highest load within the p99 bound and the next one
-SERVED_LAMS = {0: [1200, 1500, 1800, 2000, 2200], 1: [500, 700, 800, 900, 1000]}
+SERVED_LAMS = {0: [1200, 1600, 2000, 2200, 2400], 1: [600, 900, 1100, 1200, 1300, 1400]}
 
 
 def divergence(ve, caves, frames=32, warmup=4):

@@ -157,6 +157,43 @@ def test_admission_waits_for_budget(ve, scenes, small_scene):
         srv.close()
 
 
+def test_closed_waiting_session_still_renders(ve, scenes, small_scene):
+    """ve_server_close closes after the session's submitted frames, also for a
+    session still waiting for a slot: it keeps its place, its frames run once
+    it is admitted (when the running session retires), and they render as
+    alone; every submitted frame is reported."""
+    print("case server close while waiting", flush=True)
+    fc = ve.FrameConfig(**DET_KNOBS)
+    v = _views(scenes)[0]
+    W, H, mb = v[:3]
+    ref_outs, _ = _alone(ve, scenes, small_scene, v, 2, fc)
+    srv = ve.Server(max_sessions=1)
+    try:
+        sc = srv.add_scene(small_scene["words"], small_scene["depth"], small_scene["info"].size)
+        a = srv.open(sc, W, H, mb, scenes.palette(1), SKY)
+        b = srv.open(sc, W, H, mb, scenes.palette(1), SKY)
+        assert srv.stats()["waiting"] == 1
+        ubos = _ubos(ve, scenes, v, 2)
+        for s in (a, b):
+            for f in range(2):
+                srv.submit(s, fc, ve.FrameInputs(W, H, f, frame_seed(f), f + 1), ubos[f])
+        srv.close_session(b)          # still waiting: its two frames stay queued
+        srv.close_session(a)
+        done = []
+        while len(done) < 4:
+            got = srv.poll(wait=True)
+            assert got or srv.stats()["frames_in_flight"] > 0 or srv.stats()["waiting"] > 0
+            done += got
+        assert sorted((d["session"], d["frame"]) for d in done) == [(a, 0), (a, 1), (b, 0), (b, 1)]
+        db = [d for d in done if d["session"] == b]
+        da = [d for d in done if d["session"] == a]
+        assert min(d["start_ms"] for d in db) >= max(d["done_ms"] for d in da) - 1e-3
+        st = srv.stats()
+        assert st["admitted"] == 0 and st["waiting"] == 0 and st["frames_in_flight"] == 0
+    finally:
+        srv.close()
+
+
 def test_server_rejects_bad_requests(ve, scenes, small_scene):
     srv = ve.Server(hbm_budget=1 << 20)
     try:

@@ -422,11 +422,13 @@ int ve_server_close(ve_server *s, uint32_t session)
     if (it == s->sessions.end()) return srv_error(s, VE_ENOTFOUND, "unknown session %u", session);
     Session &ss = it->second;
     ss.close_requested = true;
-    if (ss.state == State::Waiting) {   // never admitted: its queued frames never ran
+    if (ss.state == State::Waiting && ss.pending.empty()) {   // never admitted and nothing queued
         s->waiting.erase(std::find(s->waiting.begin(), s->waiting.end(), session));
         s->sessions.erase(it);
         return admit(s);
     }
+    // a waiting session keeps its place: its queued frames run once it is
+    // admitted, and it retires after them like an admitted one
     SRV_HIP(s, hipSetDevice(s->device));
     collect(s);
     return admit(s);

@@ -61,6 +61,7 @@ constexpr uint32_t BOUNCE_COPIES = 32;
 // 3015, 7 3110, 8 3207, 9 (512 MiB) 3244 Mrays/s.
 constexpr uint32_t JUMP_LEVEL = 8;
 constexpr uint32_t JUMP_ENDED = 0x80000000u;   // entry flag: the descent ended at depth (entry & 0xFF)
+constexpr uint32_t JUMP_OCC = 0x100u;          // with JUMP_ENDED: the word that ended it has a material
 constexpr uint32_t RESOLVE_TIMEOUT_BLOCKS = 1024; // 4 waves each, striding over the timed-out pixels
 // Timed-out pixels are reconstructed per 16x16 tile from LDS for radii up to
 // RECON_TILED_MAX (the default is 10); larger radii gather per pixel.

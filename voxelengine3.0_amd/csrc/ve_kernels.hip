@@ -104,7 +104,8 @@ void launch_primary(hipStream_t s, const uint32_t *words, const FrameParams &p, 
 // the descent of raycast from the root through depth `level` depends only on
 // the coordinate bits above the cell, so its outcome is the same for every
 // position in the cell -- the child block of the node at depth `level`, or
-// the depth at which an empty node ended it (JUMP_ENDED | depth).
+// the depth at which a non-internal word ended it (JUMP_ENDED | depth, with
+// JUMP_OCC when that word carries a material).
 __global__ void k_build_jump(const uint32_t *__restrict__ words, uint32_t n_words, uint32_t depth, uint32_t level,
                              uint32_t *__restrict__ jump)
 {
@@ -118,8 +119,8 @@ __global__ void k_build_jump(const uint32_t *__restrict__ words, uint32_t n_word
     for (uint32_t dd = 0; dd <= level; dd++) {
         offset += locate_bit(ux, uy, uz, depth - dd);
         const uint32_t raw = node_word(words, n_words, offset);
-        if (!(raw & 1u)) {
-            jump[i] = JUMP_ENDED | dd;
+        if (!(raw & 1u)) {   // the ending word's material bits decide an occupancy lookup
+            jump[i] = JUMP_ENDED | (((raw >> 1) & 0x7Fu) != 0u ? JUMP_OCC : 0u) | dd;
             return;
         }
         offset = raw >> 4;

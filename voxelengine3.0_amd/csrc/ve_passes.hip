@@ -138,15 +138,37 @@ struct SharedDescent {
     uint32_t hx, hy, hz;
     uint32_t stop;   // depth at which the centre's descent ended
     bool occ;        // its outcome
+    uint32_t from;   // first depth whose block base the cache holds (jl + 1 after a jump-table start)
 };
 
-__device__ __forceinline__ SharedDescent centre_descent(const uint32_t *__restrict__ w, uint32_t n, uint32_t depth,
-                                                        uint32_t hx, uint32_t hy, uint32_t hz, uint32_t len,
-                                                        uint32_t *cache, uint32_t stride)
+// The jump table's outcome through depth jl for (x, y, z): true when the
+// descent ended there (occ set), else `offset` = the block base of depth jl + 1.
+__device__ __forceinline__ bool jump_start(const uint32_t *__restrict__ jump, uint32_t jl, uint32_t depth, uint32_t x,
+                                           uint32_t y, uint32_t z, uint32_t &offset, uint32_t &stop, bool &occ)
 {
-    SharedDescent c{hx, hy, hz, depth, false};
+    const uint32_t sh = depth - jl;
+    const uint32_t e = jump[((x >> sh) << (2u * jl)) | ((y >> sh) << jl) | (z >> sh)];
+    if (e & JUMP_ENDED) {
+        stop = e & 0xFFu;
+        occ = (e & JUMP_OCC) != 0u;
+        return true;
+    }
+    offset = e;
+    return false;
+}
+
+// The centre voxel's descent (normal.comp's lookupOccupied): from the jump
+// table through depth jl, then word by word, keeping each depth's block base
+// for the neighbours' lookups.
+__device__ __forceinline__ SharedDescent centre_descent(const uint32_t *__restrict__ w, const uint32_t *__restrict__ jump,
+                                                        uint32_t jl, uint32_t n, uint32_t depth, uint32_t hx,
+                                                        uint32_t hy, uint32_t hz, uint32_t len, uint32_t *cache,
+                                                        uint32_t stride)
+{
+    SharedDescent c{hx, hy, hz, depth, false, jl + 1u};
     uint32_t offset = 0;
-    for (uint32_t d = 0; d <= depth; d++) {
+    if (jump_start(jump, jl, depth, hx, hy, hz, offset, c.stop, c.occ)) return c;
+    for (uint32_t d = jl + 1u; d <= depth; d++) {
         cache[d * stride] = offset;
         offset += locate_bit(hx, hy, hz, depth - d);
         const uint32_t raw = node_word(w, n, offset);
@@ -160,14 +182,23 @@ __device__ __forceinline__ SharedDescent centre_descent(const uint32_t *__restri
     return c;
 }
 
-__device__ __forceinline__ bool neighbour_occupied(const uint32_t *__restrict__ w, uint32_t n, uint32_t depth,
-                                                   const SharedDescent &c, uint32_t x, uint32_t y, uint32_t z,
-                                                   uint32_t len, const uint32_t *cache, uint32_t stride)
+__device__ __forceinline__ bool neighbour_occupied(const uint32_t *__restrict__ w, const uint32_t *__restrict__ jump,
+                                                   uint32_t jl, uint32_t n, uint32_t depth, const SharedDescent &c,
+                                                   uint32_t x, uint32_t y, uint32_t z, uint32_t len,
+                                                   const uint32_t *cache, uint32_t stride)
 {
     const uint32_t diff = (x ^ c.hx) | (y ^ c.hy) | (z ^ c.hz);
-    const uint32_t d0 = depth - (31u - (uint32_t)__clz(diff));   // first depth choosing by a differing bit
+    uint32_t d0 = depth - (31u - (uint32_t)__clz(diff));   // first depth choosing by a differing bit
     if (c.stop < d0) return c.occ;
-    uint32_t offset = cache[d0 * stride];
+    uint32_t offset;
+    if (d0 < c.from) {   // the paths part above the cached depths: this voxel's own table entry
+        uint32_t stop;
+        bool occ;
+        if (jump_start(jump, jl, depth, x, y, z, offset, stop, occ)) return occ;
+        d0 = jl + 1u;
+    } else {
+        offset = cache[d0 * stride];
+    }
     for (uint32_t d = d0; d <= depth; d++) {
         offset += locate_bit(x, y, z, depth - d);
         const uint32_t raw = node_word(w, n, offset);
@@ -177,10 +208,11 @@ __device__ __forceinline__ bool neighbour_occupied(const uint32_t *__restrict__ 
     return false;
 }
 
-__device__ f3 refined_normal(const uint32_t *__restrict__ w, uint32_t n, uint32_t depth, uint32_t hx, uint32_t hy,
-                             uint32_t hz, uint32_t len, int r, uint32_t *cache, uint32_t stride)   // normal.comp:73-87
+__device__ f3 refined_normal(const uint32_t *__restrict__ w, const uint32_t *__restrict__ jump, uint32_t jl, uint32_t n,
+                             uint32_t depth, uint32_t hx, uint32_t hy, uint32_t hz, uint32_t len, int r,
+                             uint32_t *cache, uint32_t stride)   // normal.comp:73-87
 {
-    const SharedDescent c = centre_descent(w, n, depth, hx, hy, hz, len, cache, stride);
+    const SharedDescent c = centre_descent(w, jump, jl, n, depth, hx, hy, hz, len, cache, stride);
     f3 acc{0.0f, 0.0f, 0.0f};
     for (int dx = -r; dx <= r; dx++)
         for (int dy = -r; dy <= r; dy++)
@@ -190,7 +222,8 @@ __device__ f3 refined_normal(const uint32_t *__restrict__ w, uint32_t n, uint32_
                 const int nx = (int)hx + dx, ny = (int)hy + dy, nz = (int)hz + dz;
                 const bool oob = nx < 0 || ny < 0 || nz < 0 || nx >= (int)len || ny >= (int)len || nz >= (int)len;
                 const bool empty =
-                    oob || !neighbour_occupied(w, n, depth, c, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, len, cache, stride);
+                    oob || !neighbour_occupied(w, jump, jl, n, depth, c, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, len,
+                                               cache, stride);
                 if (empty) acc = add(acc, f3{(float)dx, (float)dy, (float)dz});
             }
     return length3(acc) > 0.001f ? normalize(acc) : f3{0.0f, 1.0f, 0.0f};
@@ -240,7 +273,8 @@ __global__ __launch_bounds__(256) void k_normal(const uint32_t *__restrict__ wor
         // The claimant computes the normal; a later claimant that finds the
         // field still empty would compute the same value (normal.comp:151-156).
         if (claimed || __atomic_load_n(&bucket[NB_NRM + s], __ATOMIC_RELAXED) == 0u) {
-            const f3 nrm = refined_normal(words, p.n_words, p.depth, hx, hy, hz, len, p.normal_precision, cache, 256u);
+            const f3 nrm = refined_normal(words, p.jump, p.jump_level, p.n_words, p.depth, hx, hy, hz, len, p.normal_precision,
+                                          cache, 256u);
             atomicCAS(&bucket[NB_NRM + s], 0u, pack_normal(nrm));
         }
         atomicExch(&bucket[NB_TIME + s], p.time);
@@ -255,7 +289,8 @@ __global__ __launch_bounds__(256) void k_normal(const uint32_t *__restrict__ wor
             if (t[k] < oldest) { oldest = t[k]; oslot = q + k; }
     }
     atomicExch(&bucket[oslot], vid);
-    const f3 nrm = refined_normal(words, p.n_words, p.depth, hx, hy, hz, len, p.normal_precision, cache, 256u);
+    const f3 nrm = refined_normal(words, p.jump, p.jump_level, p.n_words, p.depth, hx, hy, hz, len, p.normal_precision,
+                                          cache, 256u);
     atomicExch(&bucket[NB_NRM + oslot], pack_normal(nrm));
     atomicExch(&bucket[NB_TIME + oslot], p.time);
 }

@@ -107,7 +107,10 @@ const char *ve_last_error(const ve_ctx *ctx);
 
 /* Octree::set upload (octree.cpp:127-166): copies `n_words` node words to the
  * device.  The words are validated (every `next` block must lie inside the
- * array) so a malformed array fails with VE_EFORMAT instead of faulting. */
+ * array) so a malformed array fails with VE_EFORMAT instead of faulting.
+ * Besides the words, a scene holds a traversal jump table of
+ * 8^min(8, depth - 1) words (64 MiB from depth 9 up), built here and rebuilt by
+ * every ve_scene_update; results do not depend on it. */
 int ve_scene_upload(ve_ctx *ctx, const uint32_t *words, uint32_t n_words, uint32_t depth,
                     ve_scene **scene);
 int ve_scene_release(ve_ctx *ctx, ve_scene *scene);
@@ -319,8 +322,9 @@ uint32_t ve_octree_find_slot(const ve_octree_cpu *t, uint32_t x, uint32_t y, uin
 /* Bring a device node array up to date after ve_octree_apply_edits: `words`
  * is the new image (n_words in use) and `slots` the words that changed since
  * the scene was uploaded or last updated (words appended since then and not
- * listed are zero).  Grows the device array when n_words exceeds it.  Ordered
- * on the context's stream after the frames already enqueued there. */
+ * listed are zero).  Grows the device array when n_words exceeds it, and
+ * rebuilds the scene's jump table.  Ordered on the context's stream after the
+ * frames already enqueued there; returns when the update is complete. */
 int  ve_scene_update(ve_ctx *ctx, ve_scene *scene, const uint32_t *words, uint32_t n_words,
                      const uint32_t *slots, size_t n_slots);
 

@@ -123,16 +123,19 @@ def test_reused_bounce0_hits_equal_traversal(ve, scenes, small_scene, bounce_cc)
     assert np.array_equal(a["LBUFFER"][0], b["LBUFFER"][0]) and np.array_equal(a["LBUFFER"][1], b["LBUFFER"][1])
 
 
-def test_frame_equals_pass_by_pass(ve, scenes, small_scene):
+@pytest.mark.parametrize("min_samples", [1, 48])
+def test_frame_equals_pass_by_pass(ve, scenes, small_scene, min_samples):
     """ve_frame -- primary pass overlapped with the previous frame, bounce-0
     hits taken from the primary pass, bounce iterations 1.. fused per ray with
     their deposits grouped per lBuffer bucket and applied in iteration order
     -- renders what the reference's one dispatch per pass and per bounce
     iteration renders (Renderer::runWavefrontFrame, renderer.cpp:282-649),
     frame after frame.  Four bounces, so two iterations' deposits are held
-    back and applied in order."""
+    back and applied in order.  With 48 samples per visible voxel a voxel's
+    bucket holds back more deposits than it keeps in place, so the overflow
+    list is grouped and applied too."""
     bounces = 4
-    fc_kw = dict(DET_KNOBS)
+    fc_kw = dict(DET_KNOBS, min_samples=min_samples)
     out = []
     for by_pass in (False, True):
         rend = ve.Renderer(W, H, hash_width=4096, max_bounces=bounces)
@@ -146,7 +149,7 @@ def test_frame_equals_pass_by_pass(ve, scenes, small_scene):
                 eye = np.asarray(e, np.float32) + np.float32(2.0 * f) * np.array([0.5, 0.0, 1.0], np.float32)
                 rend.set_camera(ve.camera_ubo(eye, d, fov, W / H))
                 inputs = ve.FrameInputs(W, H, f, frame_seed(f), f + 1)
-                print("case frame paths by_pass", by_pass, "frame", f, flush=True)
+                print("case frame paths min_samples", min_samples, "by_pass", by_pass, "frame", f, flush=True)
                 if by_pass:
                     for name in ("RESET", "PRIMARY", "NORMAL", "SCHEDULE", "RANK", "THRESHOLD", "EMIT"):
                         rend.run_pass(sc, name, fc, inputs)

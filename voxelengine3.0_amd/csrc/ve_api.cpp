@@ -100,10 +100,12 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.em_reg, EMISSIVE_CAPACITY);
     a(&b.hit_bits, SHADE_BUDGET_MAX / 32);
     a(&b.deferred, SHADE_BUDGET_MAX);
-    a(&b.deferred_sorted, SHADE_BUDGET_MAX);
+    a(&b.bin_over, SHADE_BUDGET_MAX);
+    a(&b.bin_over_sorted, SHADE_BUDGET_MAX);
+    a(&b.binned, (size_t)hash_width * BIN_CAP);
     a(&b.bin_count, hash_width);
-    a(&b.bin_fill, hash_width);
     a(&b.bin_off, hash_width);
+    a(&b.bin_over_n, 2);
     a(&b.materials, MATERIAL_SLOTS);
     a(&b.node_stats, 4);
     a(&b.ray_totals, 4);
@@ -116,7 +118,7 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.vec_cnt, groups_max * 8);
     a(&b.vec_base, groups_max * 8);
     a(&b.nsamp, std::min<size_t>(P, CANDIDATE_CAPACITY));
-    b.scan_temp_bytes = scan_temp_bytes((uint32_t)std::max<size_t>(groups_max * 8, hash_width));
+    b.scan_temp_bytes = scan_temp_bytes((uint32_t)(groups_max * 8));
     a(reinterpret_cast<uint8_t **>(&b.scan_temp), b.scan_temp_bytes);
 }
 

@@ -39,13 +39,17 @@ constexpr uint32_t SCHED_SLOTS = 128;          // ordering table: voxels per cla
 // order is no longer the reference's and VE_CNT_OVERFLOW is set).
 constexpr uint32_t SCHED_OVER_CAP = 4096, SCHED_ROW_OVER = 2048;
 constexpr uint32_t MATERIAL_SLOTS = 127;       // shade.comp:90
-constexpr uint32_t DEFERRED_BLOCKS = 256;      // deferred-deposit scatter grid (entries strided over it)
-// A deposit of the fused bounce pass (bounce iteration `iter`), applied
-// after it.
+constexpr uint32_t DEFERRED_BLOCKS = 256;      // deferred-deposit overflow grid (entries strided over it)
+// A deposit of the fused bounce pass, applied after it: `iter` is the bounce
+// iteration (bits 0-7) and the deposit's index among its lBuffer bucket's
+// held-back deposits (bits 8-31).
 struct Deferred {
     uint32_t vid, iter;
     float dl[3], il[3];
 };
+// Held-back deposits kept in place per lBuffer bucket; a bucket's further
+// ones go through an overflow list grouped per bucket after the pass.
+constexpr uint32_t BIN_CAP = 32;
 constexpr uint32_t RANK_BLOCKS = 1024;         // rank pass grid (candidates strided over it)
 // The rank blocks flush their counts, min/max and histogram into RANK_COPIES
 // copies (block b into copy b % RANK_COPIES), so no address takes more than

@@ -25,6 +25,17 @@ inline void launch(K kernel, dim3 grid, dim3 block, uint32_t shared_bytes, hipSt
     pass_timer.start = nullptr;
 }
 
+// The deposits held back by the fused bounce pass, per lBuffer bucket: its
+// first BIN_CAP in `binned` (row h), the rest listed in `over` and then
+// grouped per bucket into `over_sorted` at `over_off[h]`.  A record's `iter`
+// holds the bounce iteration in bits 0-7 and the bucket-local index above.
+struct BinLists {
+    uint32_t hash_width;
+    uint32_t *count;
+    Deferred *binned, *over, *over_sorted;
+    uint32_t *over_off, *over_n;
+};
+
 // Device buffers of one context (all owned by the context).
 struct DeviceBuffers {
     uint4 *gbuf = nullptr;          // P x (checks, material, 0, vid): the current set
@@ -58,11 +69,14 @@ struct DeviceBuffers {
     uint32_t *em_bits = nullptr;         // SHADE_BUDGET_MAX / 32: first-shade rays that may register a light
     uint32_t *em_reg = nullptr;          // EMISSIVE_CAPACITY: ray index of each registration this frame
     uint32_t *hit_bits = nullptr;        // SHADE_BUDGET_MAX / 32: bounce-0 rays whose hit record emit filled in
-    Deferred *deferred = nullptr;        // SHADE_BUDGET_MAX: deposits of the fused bounce pass, in append order
-    Deferred *deferred_sorted = nullptr; // SHADE_BUDGET_MAX: the same grouped by lBuffer bucket
+    // deposits held back by the fused bounce pass (see BinLists)
+    Deferred *binned = nullptr;          // hash_width x BIN_CAP
+    Deferred *deferred = nullptr;        // SHADE_BUDGET_MAX: in append order
+    Deferred *bin_over = nullptr;        // SHADE_BUDGET_MAX: past BIN_CAP in a bucket, in placing order
+    Deferred *bin_over_sorted = nullptr; // SHADE_BUDGET_MAX: the overflow grouped by lBuffer bucket
     uint32_t *bin_count = nullptr;       // hash_width: deposits per bucket (zero between frames)
-    uint32_t *bin_fill = nullptr;        // hash_width: entries placed per bucket by the scatter (zero between frames)
-    uint32_t *bin_off = nullptr;         // hash_width: exclusive scan of bin_count
+    uint32_t *bin_off = nullptr;         // hash_width: overflow region of a bucket past BIN_CAP
+    uint32_t *bin_over_n = nullptr;      // 2: overflow entries, overflow entries placed (zero between frames)
     Material *materials = nullptr;       // MATERIAL_SLOTS
     unsigned long long *node_stats = nullptr;  // 4: primary reads, trace reads, primary steps, trace steps
     unsigned long long *ray_totals = nullptr;  // 4: primary, traced, shadow, traced with a reused hit (cumulative)

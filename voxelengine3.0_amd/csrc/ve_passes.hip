@@ -1045,7 +1045,8 @@ __device__ void deposit_into(const FrameParams &p, uint32_t *bucket, uint32_t vi
     atomicExch(&bucket[LB_TIME + s], p.time);
 }
 
-__device__ __forceinline__ uint32_t lbuf_row(const FrameParams &p, uint32_t vid) { return vid % (p.hash_width - 1u) + 1u; }
+__device__ __forceinline__ uint32_t lbuf_row_of(uint32_t hash_width, uint32_t vid) { return vid % (hash_width - 1u) + 1u; }
+__device__ __forceinline__ uint32_t lbuf_row(const FrameParams &p, uint32_t vid) { return lbuf_row_of(p.hash_width, vid); }
 
 __device__ void deposit(const FrameParams &p, uint32_t *__restrict__ lbuf, uint32_t vid, f3 dir_rgb, f3 ind_rgb)
 {
@@ -1690,12 +1691,13 @@ __global__ __launch_bounds__(256) void k_bounce(const uint32_t *__restrict__ wor
     }
 }
 
-// Deposits per lBuffer bucket (entries of id 0 deposit nothing).  Its first
-// thread folds k_bounce's ray-count copies into the frame counters and the
-// cumulative totals and clears them.
-__global__ __launch_bounds__(256) void k_bin_count(FrameParams p, const Deferred *__restrict__ deferred,
-                                                   uint32_t *__restrict__ counters, uint32_t *__restrict__ bin_count,
-                                                   uint32_t *__restrict__ bounce_part,
+// Puts each deposit held back by the fused bounce pass (append order) into
+// its lBuffer bucket: the first BIN_CAP in place, the rest into the overflow
+// list (entries of id 0 deposit nothing).  Its first thread folds k_bounce's
+// ray-count copies into the frame counters and the cumulative totals and
+// clears them.
+__global__ __launch_bounds__(256) void k_bin_place(BinLists bins, const Deferred *__restrict__ deferred,
+                                                   uint32_t *__restrict__ counters, uint32_t *__restrict__ bounce_part,
                                                    unsigned long long *__restrict__ totals)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1711,25 +1713,42 @@ __global__ __launch_bounds__(256) void k_bin_count(FrameParams p, const Deferred
     }
     const uint32_t n = min(counters[VE_CNT_DEFERRED], SHADE_BUDGET_MAX);
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        const uint32_t vid = deferred[e].vid;
-        if (vid != 0u) atomicAdd(&bin_count[lbuf_row(p, vid)], 1u);
+        Deferred d = deferred[e];
+        if (d.vid == 0u) continue;
+        const uint32_t h = lbuf_row_of(bins.hash_width, d.vid);
+        const uint32_t local = atomicAdd(&bins.count[h], 1u);
+        d.iter |= local << 8;
+        if (local < BIN_CAP) {
+            bins.binned[(size_t)h * BIN_CAP + local] = d;
+        } else {   // at most n entries overflow: always in range
+            bins.over[atomicAdd(&bins.over_n[0], 1u)] = d;
+        }
     }
 }
 
-// Groups the held-back deposits by lBuffer bucket (counting sort: an entry
-// goes to its bucket's offset plus the number of its bucket's entries placed
-// before it).
-__global__ __launch_bounds__(256) void k_bin_scatter(FrameParams p, const Deferred *__restrict__ deferred,
-                                                     const uint32_t *__restrict__ counters,
-                                                     const uint32_t *__restrict__ bin_off,
-                                                     uint32_t *__restrict__ bin_fill, Deferred *__restrict__ sorted)
+// Gives each lBuffer bucket with more than BIN_CAP held-back deposits its
+// region of the overflow-sorted list (the bucket's entry of local index
+// BIN_CAP allocates it; the counts are final after k_bin_place).
+__global__ __launch_bounds__(256) void k_bin_over_alloc(BinLists bins)
 {
-    const uint32_t n = min(counters[VE_CNT_DEFERRED], SHADE_BUDGET_MAX);
+    const uint32_t n = min(bins.over_n[0], SHADE_BUDGET_MAX);
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        const Deferred d = deferred[e];
-        if (d.vid == 0u) continue;
-        const uint32_t h = lbuf_row(p, d.vid);
-        sorted[bin_off[h] + atomicAdd(&bin_fill[h], 1u)] = d;
+        const Deferred d = bins.over[e];
+        if ((d.iter >> 8) != BIN_CAP) continue;
+        const uint32_t h = lbuf_row_of(bins.hash_width, d.vid);
+        bins.over_off[h] = atomicAdd(&bins.over_n[1], bins.count[h] - BIN_CAP);
+    }
+}
+
+// Places each overflow entry at its bucket's region plus its local index past
+// BIN_CAP, so a bucket's overflow deposits lie together in append order.
+__global__ __launch_bounds__(256) void k_bin_over_place(BinLists bins)
+{
+    const uint32_t n = min(bins.over_n[0], SHADE_BUDGET_MAX);
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const Deferred d = bins.over[e];
+        const uint32_t h = lbuf_row_of(bins.hash_width, d.vid);
+        bins.over_sorted[bins.over_off[h] + (d.iter >> 8) - BIN_CAP] = d;
     }
 }
 
@@ -1742,28 +1761,28 @@ __global__ __launch_bounds__(256) void k_bin_scatter(FrameParams p, const Deferr
 // lanes idle through the owner scans and atomics.
 constexpr uint32_t APPLY_GROUP = 16, APPLY_BUCKETS = 64 / APPLY_GROUP;
 
-__global__ __launch_bounds__(64) void k_bin_apply(FrameParams p, const Deferred *__restrict__ sorted,
-                                                  uint32_t *__restrict__ bin_count, uint32_t *__restrict__ bin_fill,
-                                                  const uint32_t *__restrict__ bin_off, uint32_t *__restrict__ lbuf,
+__global__ __launch_bounds__(64) void k_bin_apply(FrameParams p, BinLists bins, uint32_t *__restrict__ lbuf,
                                                   uint32_t first, uint32_t iters)
 {
     __shared__ __align__(16) uint32_t bk[APPLY_BUCKETS][LB_WORDS];   // owner rows are read as uint4
+    if (blockIdx.x == 0 && threadIdx.x == 0) bins.over_n[0] = bins.over_n[1] = 0u;   // read by earlier passes only
     const uint32_t g = threadIdx.x / APPLY_GROUP, gl = threadIdx.x % APPLY_GROUP;
     const uint32_t h = blockIdx.x * APPLY_BUCKETS + g;
-    const uint32_t n = h < p.hash_width ? bin_count[h] : 0u;
+    const uint32_t n = h < p.hash_width ? bins.count[h] : 0u;
     if (__ballot(n != 0u) == 0ull) return;   // the whole wave (one wave per block)
     uint4 *lb = reinterpret_cast<uint4 *>(bk[g]);
     uint4 *gb = reinterpret_cast<uint4 *>(lbuf + (size_t)h * LB_WORDS);
     if (n != 0u)
         for (uint32_t i = gl; i < LB_WORDS / 4u; i += APPLY_GROUP) lb[i] = gb[i];
     __syncthreads();
-    const Deferred *list = sorted + (n != 0u ? bin_off[h] : 0u);
+    const Deferred *held = bins.binned + (size_t)h * BIN_CAP;
+    const Deferred *over = bins.over_sorted + (n > BIN_CAP ? bins.over_off[h] : 0u);
     for (uint32_t it = first; it < iters; it++) {
         for (uint32_t base = 0; base < n; base += APPLY_GROUP) {
             const uint32_t e = base + gl;
             if (e < n) {
-                const Deferred d = list[e];
-                if (d.iter == it)
+                const Deferred d = e < BIN_CAP ? held[e] : over[e - BIN_CAP];
+                if ((d.iter & 0xFFu) == it)
                     deposit_into(p, bk[g], d.vid, f3{d.dl[0], d.dl[1], d.dl[2]}, f3{d.il[0], d.il[1], d.il[2]});
             }
         }
@@ -1771,10 +1790,7 @@ __global__ __launch_bounds__(64) void k_bin_apply(FrameParams p, const Deferred 
     }
     if (n != 0u) {
         for (uint32_t i = gl; i < LB_WORDS / 4u; i += APPLY_GROUP) gb[i] = lb[i];
-        if (gl == 0u) {
-            bin_count[h] = 0u;
-            bin_fill[h] = 0u;
-        }
+        if (gl == 0u) bins.count[h] = 0u;
     }
 }
 
@@ -2273,6 +2289,11 @@ void launch_shade(hipStream_t s, const uint32_t *words, const FrameParams &p, De
                        b.em_reg, b.counters, b.ray_totals);
 }
 
+static BinLists bin_lists(const FrameParams &p, const DeviceBuffers &b)
+{
+    return BinLists{p.hash_width, b.bin_count, b.binned, b.bin_over, b.bin_over_sorted, b.bin_off, b.bin_over_n};
+}
+
 void launch_bounce(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t first,
                    uint32_t iters, bool stats)
 {
@@ -2289,15 +2310,12 @@ void launch_bounce(hipStream_t s, const uint32_t *words, const FrameParams &p, D
 
 int launch_deferred(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_t first, uint32_t iters)
 {
-    launch(k_bin_count, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, p, b.deferred, b.counters, b.bin_count, b.bounce_part,
+    const BinLists bins = bin_lists(p, b);
+    launch(k_bin_place, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, bins, b.deferred, b.counters, b.bounce_part,
            b.ray_totals);
-    size_t tb = b.scan_temp_bytes;
-    if (hipcub::DeviceScan::ExclusiveSum(b.scan_temp, tb, b.bin_count, b.bin_off, (int)p.hash_width, s) != hipSuccess)
-        return VE_EDEVICE;
-    launch(k_bin_scatter, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, p, b.deferred, b.counters, b.bin_off, b.bin_fill,
-           b.deferred_sorted);
-    launch(k_bin_apply, dim3(cdiv(p.hash_width, APPLY_BUCKETS)), dim3(64), 0, s, p, b.deferred_sorted, b.bin_count,
-           b.bin_fill, b.bin_off, b.lbuf, first, iters);
+    launch(k_bin_over_alloc, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, bins);
+    launch(k_bin_over_place, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, bins);
+    launch(k_bin_apply, dim3(cdiv(p.hash_width, APPLY_BUCKETS)), dim3(64), 0, s, p, bins, b.lbuf, first, iters);
     return VE_OK;
 }
 

@@ -1626,7 +1626,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, const uint32_t *__
 // them), since per-iteration atomics on one counter from every wave cost the
 // traversals a quarter of their time.
 template <bool STATS>
-__global__ __launch_bounds__(256) void k_bounce(const uint32_t *__restrict__ words, FrameParams p,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_bounce(const uint32_t *__restrict__ words, FrameParams p,
                                                 const Ray *__restrict__ rays_in, const uint32_t *__restrict__ live_ptr,
                                                 uint32_t first, uint32_t iters, uint32_t *__restrict__ counters,
                                                 unsigned long long *__restrict__ stats,

@@ -565,30 +565,37 @@ __global__ void k_sched_write(FrameParams p, const uint32_t *__restrict__ cand_a
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n_vec) return;
     if (v == n_vec - 1u) counters[VE_CNT_CANDIDATES] = min(vec_base[v] + vec_cnt[v], CANDIDATE_CAPACITY);
-    uint32_t key[8], lane_of[8];
+    // the vector's claim-map words are loaded together up front (independent
+    // loads in flight at once), then written in (probe, lane) order
+    uint32_t at[8], word[8];
+    for (uint32_t l = 0; l < 8; l++) at[l] = cand_at[v * 8u + l];
+    for (uint32_t l = 0; l < 8; l++)
+        word[l] = at[l] != NONE ? claim[(size_t)((at[l] >> 6) & 0xFFFFFu) * CLAIM_PROBES + (at[l] & 63u)] : 0u;
+    uint32_t key[8];   // (probe, lane), sorted
     uint32_t m = 0;
     for (uint32_t l = 0; l < 8; l++) {
-        const uint32_t a = cand_at[v * 8u + l];
-        if (a == NONE) continue;
-        uint32_t k = ((a >> 26) << 3) | l;   // (probe, lane)
+        if (at[l] == NONE) continue;
+        const uint32_t k = ((at[l] >> 26) << 3) | l;
         uint32_t j = m++;
         while (j > 0 && key[j - 1] > k) {
             key[j] = key[j - 1];
-            lane_of[j] = lane_of[j - 1];
             j--;
         }
         key[j] = k;
-        lane_of[j] = l;
     }
     uint32_t pos = vec_base[v];
     for (uint32_t j = 0; j < m; j++, pos++) {
         if (pos >= CANDIDATE_CAPACITY) break;
-        const uint32_t e = v * 8u + lane_of[j];
-        const uint32_t a = cand_at[e];
-        const uint32_t h = (a >> 6) & 0xFFFFFu, probe = a & 63u;
+        const uint32_t l = key[j] & 7u, e = v * 8u + l;
+        uint32_t a = 0u, w = 0u;
+        for (uint32_t q = 0; q < 8; q++)   // select, not an indexed register array
+            if (q == l) {
+                a = at[q];
+                w = word[q];
+            }
         uint32_t pix;
         sched_pixel(e, p, pix);
-        candidates[pos] = make_uint4(claim[(size_t)h * CLAIM_PROBES + probe], pix, h, probe);
+        candidates[pos] = make_uint4(w, pix, (a >> 6) & 0xFFFFFu, a & 63u);
     }
 }
 

@@ -339,15 +339,17 @@ void ve_destroy(ve_ctx *c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->aux) (void)hipStreamSynchronize(c->aux);
+    if (c->aux2) (void)hipStreamSynchronize(c->aux2);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto &a : c->allocations) (void)hipFree(a.first);
     if (c->sky_tex) (void)hipFree(c->sky_tex);
     for (auto &fe : c->frame_events)
         for (hipEvent_t e : fe)
             if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {c->ev_primary, c->ev_emit, c->ev_fence})
+    for (hipEvent_t e : {c->ev_primary, c->ev_emit, c->ev_fence, c->ev_head, c->ev_normal})
         if (e) (void)hipEventDestroy(e);
     if (c->aux) (void)hipStreamDestroy(c->aux);
+    if (c->aux2) (void)hipStreamDestroy(c->aux2);
     if (c->stream && c->owns_stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -498,17 +500,17 @@ int ve_set_sky(ve_ctx *c, const uint8_t *const faces[6], int edge)
 
 namespace {
 
-// Enqueue one pass; every launch goes on the context's stream.
-// The primary pass runs on `primary_stream` (the context's stream unless a
-// whole frame overlaps it with the previous one).
-int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, hipStream_t primary_stream = nullptr)
+// Enqueue one pass; every launch goes on the context's stream.  The primary
+// and normal passes run on `side_stream` when given (an overlapped frame's
+// auxiliary streams).
+int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, hipStream_t side_stream = nullptr)
 {
     int rc = VE_OK;
     switch (pass) {
     case VE_PASS_PRIMARY:
-        launch_primary(primary_stream ? primary_stream : c->stream, s->words, p, c->b, c->node_stats);
+        launch_primary(side_stream ? side_stream : c->stream, s->words, p, c->b, c->node_stats);
         break;
-    case VE_PASS_NORMAL: launch_normal(c->stream, s->words, p, c->b); break;
+    case VE_PASS_NORMAL: launch_normal(side_stream ? side_stream : c->stream, s->words, p, c->b); break;
     case VE_PASS_RESET: launch_reset(c->stream, p, c->b); break;
     case VE_PASS_SCHEDULE: rc = launch_schedule(c->stream, p, c->b); break;
     case VE_PASS_RANK: rc = launch_rank(c->stream, p, c->b); break;
@@ -554,12 +556,12 @@ hipEvent_t *frame_slot(ve_ctx *c)
 }
 
 int timed_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, uint32_t iter, hipEvent_t *ev,
-               hipStream_t primary_stream = nullptr)
+               hipStream_t side_stream = nullptr)
 {
-    if (!ev) return run_pass(c, s, pass, p, primary_stream);
+    if (!ev) return run_pass(c, s, pass, p, side_stream);
     hipEvent_t *pair = ev + 2 * (pass * MAX_ITERS + iter);
     pass_timer = PassTimer{pair[0], pair[1]};
-    const int rc = run_pass(c, s, pass, p, primary_stream);
+    const int rc = run_pass(c, s, pass, p, side_stream);
     pass_timer = PassTimer{};
     return rc;
 }
@@ -597,7 +599,15 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &p)
         if ((rc = timed_pass(c, s, VE_PASS_RESET, p, 0, ev))) return rc;
         if ((rc = timed_pass(c, s, VE_PASS_PRIMARY, p, 0, ev))) return rc;
     }
-    const ve_pass_id head[] = {VE_PASS_NORMAL, VE_PASS_SCHEDULE, VE_PASS_RANK, VE_PASS_THRESHOLD};
+    if (c->overlap) {   // normal pass beside schedule .. emit
+        HIP_OK(c, hipEventRecord(c->ev_head, c->stream));
+        HIP_OK(c, hipStreamWaitEvent(c->aux2, c->ev_head, 0));
+        if ((rc = timed_pass(c, s, VE_PASS_NORMAL, p, 0, ev, c->aux2))) return rc;
+        HIP_OK(c, hipEventRecord(c->ev_normal, c->aux2));
+    } else if ((rc = timed_pass(c, s, VE_PASS_NORMAL, p, 0, ev))) {
+        return rc;
+    }
+    const ve_pass_id head[] = {VE_PASS_SCHEDULE, VE_PASS_RANK, VE_PASS_THRESHOLD};
     for (ve_pass_id id : head)
         if ((rc = timed_pass(c, s, id, p, 0, ev))) return rc;
     if (c->overlap) {
@@ -616,6 +626,7 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &p)
     } else if ((rc = timed_pass(c, s, VE_PASS_EMIT, p, 0, ev))) {
         return rc;
     }
+    if (c->overlap) HIP_OK(c, hipStreamWaitEvent(c->stream, c->ev_normal, 0));   // shading and resolve read normals
     const uint32_t iters = std::min<uint32_t>(c->max_bounces, (uint32_t)MAX_ITERS);   // renderer.cpp:515
     // iteration 0 (emissive registration, reservoirs) as its own passes; the
     // later ones fused per ray, timed as trace / shade of iteration 1
@@ -729,6 +740,7 @@ int ve_sync(ve_ctx *c)
     HIP_OK(c, hipSetDevice(c->device));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     if (c->aux) HIP_OK(c, hipStreamSynchronize(c->aux));
+    if (c->aux2) HIP_OK(c, hipStreamSynchronize(c->aux2));
     return VE_OK;
 }
 
@@ -764,6 +776,7 @@ int ve_timings(ve_ctx *c, float ms_out[VE_PASS_COUNT])
     if (!ms_out) return ctx_error(c, VE_EINVAL, "null output");
     HIP_OK(c, hipSetDevice(c->device));
     if (c->aux) HIP_OK(c, hipStreamSynchronize(c->aux));
+    if (c->aux2) HIP_OK(c, hipStreamSynchronize(c->aux2));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     double sum[VE_PASS_COUNT] = {0};
     for (size_t f = 0; f < c->frames_recorded; f++)
@@ -887,6 +900,9 @@ int ctx_create(ve_ctx **out, const ve_renderer_config *cfg, hipStream_t shared, 
                 // bound to the emit kernel's dispatch, like the pass-timing events
                 if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_emit, hipEventDisableSystemFence);
                 if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fence, fl);
+                if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->aux2, hipStreamNonBlocking);   // default priority
+                if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_head, fl);
+                if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_normal, fl);
             }
         }
     }

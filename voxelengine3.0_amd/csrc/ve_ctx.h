@@ -66,6 +66,14 @@ struct ve_ctx {
     hipEvent_t ev_emit = nullptr;           // on stream: emit pass of the latest frame done
     hipEvent_t emit_done = nullptr;         // the event that marks it (ev_emit, or its timing event)
     hipEvent_t ev_fence = nullptr;          // on stream: everything enqueued so far
+    // The normal pass writes only the nBuffer, which the schedule, rank,
+    // threshold and emit passes never touch: inside an overlapped frame it
+    // runs on aux2 beside them, after the frame's primary pass and the
+    // previous frame's passes (ev_head), and the main stream waits for it
+    // (ev_normal) before the first pass that reads normals.
+    hipStream_t aux2 = nullptr;
+    hipEvent_t ev_head = nullptr;           // on stream: the frame's reset pass, after its primary pass
+    hipEvent_t ev_normal = nullptr;         // on aux2: normal pass written
     bool aux_fence = true;                  // the next primary must follow all of stream's work
     std::vector<std::pair<void *, size_t>> allocations;
     size_t device_bytes = 0;

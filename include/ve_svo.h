@@ -250,8 +250,10 @@ int ve_sync(ve_ctx *ctx);
 /* Cross-frame overlap (on by default for contexts made by ve_create): each
  * ve_frame renders its primary pass into a second G-buffer on an auxiliary
  * stream, beside the previous frame's trace/shade/resolve passes, and the
- * context's stream waits for it before the normal pass.  Every frame's
- * results are those of the serial order.  Off: all passes in order on the
+ * context's stream waits for it before the normal pass; the normal pass runs
+ * on a second auxiliary stream beside the schedule, rank, threshold and emit
+ * passes (it writes only the nBuffer, which they do not use), joined before
+ * the trace pass.  Every frame's results are those of the serial order.  Off: all passes in order on the
  * context's stream (Renderer::run's sequence).  No counterpart in the
  * reference, whose GL passes run in submission order; VE_EINVAL for a
  * context without the second stream (server sessions). */

@@ -1626,10 +1626,12 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, const uint32_t *__
 // reference's one dispatch per iteration gives.  Traversals then run without
 // a claim and up to ten adds on global memory per deposit beside them (the
 // kernel is kept this lean: extra bookkeeping here has cost the traversal
-// loop 20% through its register allocation).
+// loop 20% through its register allocation, and placing each deposit in its
+// lBuffer bucket from here 27 VGPRs and a wave per SIMD).  It is held to 5
+// waves per SIMD (96 VGPRs with 5 spilled; 109 VGPRs and 4 waves otherwise).
 // Ray counts: every ray entering an iteration counts as traced, as in the
 // per-iteration trace dispatch; each lane counts its own, and each wave adds
-// its sums to one of BOUNCE_COPIES copies at the end (k_bin_count folds
+// its sums to one of BOUNCE_COPIES copies at the end (k_bin_place folds
 // them), since per-iteration atomics on one counter from every wave cost the
 // traversals a quarter of their time.
 template <bool STATS>

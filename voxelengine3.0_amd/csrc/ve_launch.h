@@ -89,6 +89,7 @@ struct DeviceBuffers {
     uint32_t *cand_at = nullptr;         // per execution rank: claim row/probe of the voxel it first claims
     uint32_t *vec_cnt = nullptr;         // per SIMD-8 vector: entries appended
     uint32_t *vec_base = nullptr;        // exclusive scan of vec_cnt
+    uint32_t *vec_chunk = nullptr;       // per 256 SIMD-8 vectors: sum of vec_cnt
     int32_t *nsamp = nullptr;            // per candidate: samples
     void *scan_temp = nullptr;
     size_t scan_temp_bytes = 0;

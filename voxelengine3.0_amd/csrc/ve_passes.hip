@@ -540,31 +540,75 @@ __global__ __launch_bounds__(256) void k_sched_rank_over(uint4 *__restrict__ tab
     }
 }
 
-// candidates first-claimed in each SIMD-8 vector
-__global__ void k_sched_vec_count(const uint32_t *__restrict__ cand_at, uint32_t n_vec, uint32_t *__restrict__ vec_cnt,
-                                  uint32_t *__restrict__ over_n)
+// Sum over a 256-thread block (every thread gets it); `lds` holds 4 words.
+__device__ __forceinline__ uint32_t block_sum256(uint32_t x, uint32_t *lds)
 {
+    for (int off = 32; off > 0; off >>= 1) x += (uint32_t)__shfl_xor((int)x, off, 64);
+    if ((threadIdx.x & 63u) == 0u) lds[threadIdx.x >> 6] = x;
+    __syncthreads();
+    const uint32_t t = lds[0] + lds[1] + lds[2] + lds[3];
+    __syncthreads();
+    return t;
+}
+
+// Exclusive prefix over a 256-thread block, in thread order; `lds` holds 4 words.
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t x, uint32_t *lds)
+{
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint32_t inc = x;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)inc, off, 64);
+        if (lane >= off) inc += y;
+    }
+    if (lane == 63u) lds[w] = inc;
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t k = 0; k < w; k++) base += lds[k];
+    __syncthreads();
+    return base + inc - x;
+}
+
+// Candidates first-claimed in each SIMD-8 vector, and their sum per block of
+// 256 vectors (k_sched_write turns these into the vectors' offsets).
+__global__ __launch_bounds__(256) void k_sched_vec_count(const uint32_t *__restrict__ cand_at, uint32_t n_vec,
+                                                         uint32_t *__restrict__ vec_cnt,
+                                                         uint32_t *__restrict__ vec_chunk,
+                                                         uint32_t *__restrict__ over_n)
+{
+    __shared__ uint32_t lds[4];
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v == 0u) {   // the overflow records of this frame are consumed
         over_n[0] = 0u;
         over_n[1] = 0u;
     }
-    if (v >= n_vec) return;
     uint32_t c = 0;
-    for (uint32_t l = 0; l < 8; l++) c += cand_at[v * 8u + l] != NONE;
-    vec_cnt[v] = c;
+    if (v < n_vec) {
+        for (uint32_t l = 0; l < 8; l++) c += cand_at[v * 8u + l] != NONE;
+        vec_cnt[v] = c;
+    }
+    const uint32_t t = block_sum256(c, lds);
+    if (threadIdx.x == 0) vec_chunk[blockIdx.x] = t;
 }
 
 // Inside a vector, wins happen probe by probe (lockstep loop) and, within a
 // probe, in lane order: sort the vector's <= 8 winners by (probe, lane).
-__global__ void k_sched_write(FrameParams p, const uint32_t *__restrict__ cand_at, uint32_t n_vec,
-                              const uint32_t *__restrict__ vec_cnt, const uint32_t *__restrict__ vec_base,
-                              const uint32_t *__restrict__ claim, uint4 *__restrict__ candidates,
-                              uint32_t *__restrict__ counters)
+// A vector's first candidate index: the block's offset (the sum of the
+// earlier blocks' totals) plus the prefix inside the block.
+__global__ __launch_bounds__(256) void k_sched_write(FrameParams p, const uint32_t *__restrict__ cand_at, uint32_t n_vec,
+                                                     const uint32_t *__restrict__ vec_cnt,
+                                                     const uint32_t *__restrict__ vec_chunk,
+                                                     const uint32_t *__restrict__ claim, uint4 *__restrict__ candidates,
+                                                     uint32_t *__restrict__ counters)
 {
+    __shared__ uint32_t lds[4];
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t before = 0;
+    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += blockDim.x) before += vec_chunk[i];
+    const uint32_t off = block_sum256(before, lds);
+    const uint32_t cnt = v < n_vec ? vec_cnt[v] : 0u;
+    const uint32_t vbase = off + block_excl_scan256(cnt, lds);
     if (v >= n_vec) return;
-    if (v == n_vec - 1u) counters[VE_CNT_CANDIDATES] = min(vec_base[v] + vec_cnt[v], CANDIDATE_CAPACITY);
+    if (v == n_vec - 1u) counters[VE_CNT_CANDIDATES] = min(vbase + cnt, CANDIDATE_CAPACITY);
     // the vector's claim-map words are loaded together up front (independent
     // loads in flight at once), then written in (probe, lane) order
     uint32_t at[8], word[8];
@@ -583,7 +627,7 @@ __global__ void k_sched_write(FrameParams p, const uint32_t *__restrict__ cand_a
         }
         key[j] = k;
     }
-    uint32_t pos = vec_base[v];
+    uint32_t pos = vbase;
     for (uint32_t j = 0; j < m; j++, pos++) {
         if (pos >= CANDIDATE_CAPACITY) break;
         const uint32_t l = key[j] & 7u, e = v * 8u + l;
@@ -2226,12 +2270,10 @@ int launch_schedule(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
            b.sched_row_over);
     launch(k_sched_rank_over, dim3(SCHED_OVER_BLOCKS), dim3(256), 0, s, b.sched_tab, b.claim, b.samples, b.cand_at,
            b.sched_over, b.sched_over_n, b.sched_row_over, b.sched_rows, b.counters);
-    launch(k_sched_vec_count, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, b.cand_at, n_vec, b.vec_cnt, b.sched_over_n);
-    size_t tb = b.scan_temp_bytes;
-    if (hipcub::DeviceScan::ExclusiveSum(b.scan_temp, tb, b.vec_cnt, b.vec_base, (int)n_vec, s) != hipSuccess)
-        return VE_EDEVICE;
-    launch(k_sched_write, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, p, b.cand_at, n_vec, b.vec_cnt,
-                       b.vec_base, b.claim, b.candidates, b.counters);
+    launch(k_sched_vec_count, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, b.cand_at, n_vec, b.vec_cnt, b.vec_chunk,
+           b.sched_over_n);
+    launch(k_sched_write, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, p, b.cand_at, n_vec, b.vec_cnt, b.vec_chunk,
+           b.claim, b.candidates, b.counters);
     return VE_OK;
 }
 

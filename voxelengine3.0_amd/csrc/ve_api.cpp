@@ -116,11 +116,9 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.sched_rows, CLAIM_WIDTH);
     a(&b.cand_at, groups_max * 64);
     a(&b.vec_cnt, groups_max * 8);
-    a(&b.vec_base, groups_max * 8);
     a(&b.vec_chunk, (groups_max * 8 + 255) / 256);
+    a(&b.rank_chunk, (std::min<size_t>(P, CANDIDATE_CAPACITY) + 255) / 256);
     a(&b.nsamp, std::min<size_t>(P, CANDIDATE_CAPACITY));
-    b.scan_temp_bytes = scan_temp_bytes((uint32_t)(groups_max * 8));
-    a(reinterpret_cast<uint8_t **>(&b.scan_temp), b.scan_temp_bytes);
 }
 
 // RankStats thresholds start at (+inf, -inf): every voxel in the mid tier on

@@ -88,11 +88,9 @@ struct DeviceBuffers {
     uint32_t *sched_rows = nullptr;      // CLAIM_WIDTH: the flagged rows
     uint32_t *cand_at = nullptr;         // per execution rank: claim row/probe of the voxel it first claims
     uint32_t *vec_cnt = nullptr;         // per SIMD-8 vector: entries appended
-    uint32_t *vec_base = nullptr;        // exclusive scan of vec_cnt
-    uint32_t *vec_chunk = nullptr;       // per 256 SIMD-8 vectors: sum of vec_cnt
+    uint32_t *vec_chunk = nullptr;       // per 256 SIMD-8 vectors: sum of vec_cnt (schedule)
+    uint32_t *rank_chunk = nullptr;      // per 256 candidates: shade-list entries they append (rank)
     int32_t *nsamp = nullptr;            // per candidate: samples
-    void *scan_temp = nullptr;
-    size_t scan_temp_bytes = 0;
 };
 
 void launch_primary(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, bool stats);
@@ -117,6 +115,5 @@ void launch_scatter_words(hipStream_t s, uint32_t *words, const uint2 *pairs, ui
 // The jump table of a node array (8^level entries).
 void launch_build_jump(hipStream_t s, const uint32_t *words, uint32_t n_words, uint32_t depth, uint32_t level,
                        uint32_t *jump);
-size_t scan_temp_bytes(uint32_t max_items);
 
 }  // namespace ve

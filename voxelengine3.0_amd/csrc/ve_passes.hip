@@ -1,7 +1,7 @@
 // gfx950 kernels for the wavefront passes after the primary pass:
 //   normal.comp     -> k_normal
-//   schedule.comp   -> k_sched_claim, k_sched_rank, k_sched_vec_count, (scan), k_sched_write
-//   rank.comp       -> k_rank, (scan), k_rank_write
+//   schedule.comp   -> k_sched_claim, k_sched_rank(_over), k_sched_vec_count, k_sched_write
+//   rank.comp       -> k_rank, k_rank_write
 //   threshold.comp  -> k_threshold
 //   emit.comp       -> k_emit
 //   trace.comp      -> k_trace
@@ -19,7 +19,6 @@
 // come from exclusive scans over SIMD-8 vectors.  The result is deterministic
 // and equal to the reference's output on the same inputs.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include "ve_device.h"
 #include "ve_launch.h"
@@ -671,9 +670,11 @@ __global__ __launch_bounds__(256) void k_rank(FrameParams p, const uint4 *__rest
                                               const uint32_t *__restrict__ samples_map, const uint32_t *__restrict__ lbuf,
                                               const uint32_t *__restrict__ rank_stats,
                                               uint32_t *__restrict__ rank_part, const uint32_t *__restrict__ counters,
-                                              int32_t *__restrict__ nsamp, uint32_t *__restrict__ vec_tot, uint32_t n_vec)
+                                              int32_t *__restrict__ nsamp, uint32_t *__restrict__ vec_tot,
+                                              uint32_t *__restrict__ chunk_tot)
 {
     __shared__ uint32_t hist[HISTOGRAM_BINS];
+    __shared__ uint32_t sum_lds[4];
     __shared__ uint32_t bmin, bmax;
     for (uint32_t i = threadIdx.x; i < HISTOGRAM_BINS; i += blockDim.x) hist[i] = 0u;
     if (threadIdx.x == 0) {
@@ -713,10 +714,11 @@ __global__ __launch_bounds__(256) void k_rank(FrameParams p, const uint4 *__rest
         tot += __shfl_xor(tot, 2, 8);
         tot += __shfl_xor(tot, 4, 8);
         if ((t & 7u) == 0u && t < n) vec_tot[t >> 3] = tot;
+        // the chunk of 256 candidates this iteration covers: its total, from
+        // which k_rank_write offsets its vectors
+        const uint32_t c = block_sum256(s > 0 ? (uint32_t)s : 0u, sum_lds);
+        if (threadIdx.x == 0) chunk_tot[base / 256u] = c;
     }
-    // vectors past the last candidate append nothing (the scan covers n_vec)
-    for (uint32_t v = (n + 7u) / 8u + blockIdx.x * blockDim.x + threadIdx.x; v < n_vec; v += gridDim.x * blockDim.x)
-        vec_tot[v] = 0u;
     for (int off = 32; off > 0; off >>= 1) {
         my_min = min(my_min, (uint32_t)__shfl_xor((int)my_min, off, 64));
         my_max = max(my_max, (uint32_t)__shfl_xor((int)my_max, off, 64));
@@ -769,25 +771,36 @@ __device__ void rank_fold_copies(uint32_t *__restrict__ rank_stats, uint32_t *__
 __global__ __launch_bounds__(256) void k_rank_write(FrameParams p, const uint4 *__restrict__ candidates,
                                                     const int32_t *__restrict__ nsamp,
                                                     const uint32_t *__restrict__ vec_tot,
-                                                    const uint32_t *__restrict__ vec_base, uint2 *__restrict__ shade_list,
+                                                    const uint32_t *__restrict__ chunk_tot, uint2 *__restrict__ shade_list,
                                                     uint32_t *__restrict__ counters, uint32_t *__restrict__ rank_stats,
                                                     uint32_t *__restrict__ rank_part)
 {
     static_assert(HISTOGRAM_BINS == 256, "the fold takes one bin per thread of a 256-thread block");
+    __shared__ uint32_t lds[4];
     if (blockIdx.x == 0) rank_fold_copies(rank_stats, rank_part);
     const uint32_t n = counters[VE_CNT_CANDIDATES];
+    if (blockIdx.x * blockDim.x >= n) return;   // the whole block
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
     const uint32_t v = t >> 3, l = t & 7u, v0 = v * 8u;
+    // the vector's first shade-list index: the totals of the earlier 256-
+    // candidate chunks plus the vectors before it in this one
+    uint32_t before = 0;
+    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += blockDim.x) before += chunk_tot[i];
+    const uint32_t off = block_sum256(before, lds);
+    const uint32_t mine = (l == 0u && t < n) ? vec_tot[v] : 0u;
+    const uint32_t excl = block_excl_scan256(mine, lds);
+    const uint32_t vbase = off + (uint32_t)__shfl((int)excl, (int)((threadIdx.x & 63u) & ~7u), 64);
+    const uint32_t vtot = (uint32_t)__shfl((int)mine, (int)((threadIdx.x & 63u) & ~7u), 64);
+    if (t >= n) return;
     int32_t s[8];
     int32_t smax = 0;
     for (uint32_t k = 0; k < 8; k++) {
         s[k] = v0 + k < n ? nsamp[v0 + k] : 0;
         smax = max(smax, s[k]);
     }
-    if (t == n - 1u) counters[VE_CNT_SHADE] = min(vec_base[v] + vec_tot[v], p.shade_budget);
+    if (t == n - 1u) counters[VE_CNT_SHADE] = min(vbase + vtot, p.shade_budget);
     const uint4 cd = candidates[t];
-    uint32_t pos = vec_base[v];
+    uint32_t pos = vbase;
     for (int32_t i = 0; i < smax; i++) {
         uint32_t before = 0, all = 0;
         for (uint32_t k = 0; k < 8; k++) {
@@ -2281,14 +2294,10 @@ int launch_rank(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 {
     // candidates <= distinct visible voxels <= pixels
     const uint32_t bound = (uint32_t)std::min<size_t>((size_t)p.W * p.H, CANDIDATE_CAPACITY);
-    const uint32_t n_vec = cdiv(bound, 8);
     launch(k_rank, dim3(std::min<uint32_t>(cdiv(bound, 256), RANK_BLOCKS)), dim3(256), 0, s, p, b.candidates,
-                       b.samples, b.lbuf, b.rank_stats, b.rank_part, b.counters, b.nsamp, b.vec_cnt, n_vec);
-    size_t tb = b.scan_temp_bytes;
-    if (hipcub::DeviceScan::ExclusiveSum(b.scan_temp, tb, b.vec_cnt, b.vec_base, (int)n_vec, s) != hipSuccess)
-        return VE_EDEVICE;
+                       b.samples, b.lbuf, b.rank_stats, b.rank_part, b.counters, b.nsamp, b.vec_cnt, b.rank_chunk);
     launch(k_rank_write, dim3(cdiv(bound, 256)), dim3(256), 0, s, p, b.candidates, b.nsamp, b.vec_cnt,
-                       b.vec_base, b.shade_list, b.counters, b.rank_stats, b.rank_part);
+                       b.rank_chunk, b.shade_list, b.counters, b.rank_stats, b.rank_part);
     return VE_OK;
 }
 
@@ -2407,11 +2416,5 @@ void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
     }
 }
 
-size_t scan_temp_bytes(uint32_t max_items)
-{
-    size_t bytes = 0;
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (uint32_t *)nullptr, (uint32_t *)nullptr, (int)max_items);
-    return bytes;
-}
 
 }  // namespace ve

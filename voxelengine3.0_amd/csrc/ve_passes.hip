@@ -2061,13 +2061,13 @@ __device__ __forceinline__ float wave_fsum(float v)
 // (2R+1)^2 window over 16 lanes (partial sums reduced across them; the image
 // is compared within 1e-4).  Only in frames with RECON_TILED_MIN timed-out
 // pixels or more: below that, staging windows costs more than it saves.
-__global__ __launch_bounds__(256) void k_resolve_timeout_tiled(FrameParams p, const uint4 *__restrict__ gbuf,
+__device__ __forceinline__ void resolve_timeout_tiles(FrameParams p, const uint4 *__restrict__ gbuf,
                                                                const float4 *__restrict__ pix_avg,
                                                                const uint32_t *__restrict__ pix_cnt,
                                                                uint32_t *__restrict__ tflag,
                                                                const uint32_t *__restrict__ tlist,
                                                                const uint32_t *__restrict__ counters,
-                                                               float4 *__restrict__ out)
+                                                               float4 *__restrict__ out, uint32_t blk, uint32_t nblk)
 {
     extern __shared__ float4 tap[];   // (16 + 2R)^2 taps
     __shared__ uint32_t todo[RECON_TILE * RECON_TILE];
@@ -2077,7 +2077,7 @@ __global__ __launch_bounds__(256) void k_resolve_timeout_tiled(FrameParams p, co
     const uint32_t n_tiles = counters[VE_CNT_TIMEOUT_TILES];
     const int R = min(max(p.reconstruction_radius, 0), (int)RECON_TILED_MAX);
     const int D = (int)RECON_TILE + 2 * R;
-    for (uint32_t it = blockIdx.x; it < n_tiles; it += gridDim.x) {
+    for (uint32_t it = blk; it < n_tiles; it += nblk) {
         const uint32_t tile = tlist[it];
         const int tx0 = (int)((tile % tw) * RECON_TILE), ty0 = (int)((tile / tw) * RECON_TILE);
         if (tflag[tile] < RECON_DENSE) continue;   // sparse tiles: k_resolve_timeout did them
@@ -2158,13 +2158,13 @@ __global__ __launch_bounds__(256) void k_resolve_timeout_tiled(FrameParams p, co
     }
 }
 
-__global__ __launch_bounds__(256) void k_resolve_timeout(FrameParams p, const uint4 *__restrict__ gbuf,
+__device__ __forceinline__ void resolve_timeout_list(FrameParams p, const uint4 *__restrict__ gbuf,
                                                          const float4 *__restrict__ pix_avg,
                                                          const uint32_t *__restrict__ pix_cnt,
                                                          const uint32_t *__restrict__ timeouts,
                                                          const uint32_t *__restrict__ counters,
                                                          const uint32_t *__restrict__ tcount,
-                                                         float4 *__restrict__ out)
+                                                         float4 *__restrict__ out, uint32_t blk, uint32_t nblk)
 {
     const int R = p.reconstruction_radius < 0 ? 0 : p.reconstruction_radius;
     const uint32_t tw = (p.W + RECON_TILE - 1u) / RECON_TILE;
@@ -2174,8 +2174,8 @@ __global__ __launch_bounds__(256) void k_resolve_timeout(FrameParams p, const ui
     // in a frame with many timed-out pixels, those of dense tiles are left to
     // k_resolve_timeout_tiled
     const bool tiled = R <= RECON_TILED_MAX && n >= RECON_TILED_MIN;
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-    for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += waves) {
+    const uint32_t waves = nblk * (blockDim.x >> 6);
+    for (uint32_t i = blk * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += waves) {
         const uint32_t pix = timeouts[i];
         const int px = (int)(pix % p.W), py = (int)(pix / p.W);
         if (tiled && tcount[(uint32_t)(py / (int)RECON_TILE) * tw + (uint32_t)px / RECON_TILE] >= RECON_DENSE) continue;
@@ -2211,6 +2211,26 @@ __global__ __launch_bounds__(256) void k_resolve_timeout(FrameParams p, const ui
             out[pix] = make_float4(col.x, col.y, col.z, 1.0f);
         }
     }
+}
+
+// The timed-out pixels' reconstruction in one launch: blocks below
+// `list_blocks` take the listed pixels one wave each, the rest the dense
+// 16x16 tiles from LDS (with the tiled path on; the two roles write disjoint
+// pixels and only read the per-tile counts).
+__global__ __launch_bounds__(256) void k_resolve_timeouts(FrameParams p, const uint4 *__restrict__ gbuf,
+                                                          const float4 *__restrict__ pix_avg,
+                                                          const uint32_t *__restrict__ pix_cnt,
+                                                          const uint32_t *__restrict__ timeouts,
+                                                          uint32_t *__restrict__ tflag,
+                                                          const uint32_t *__restrict__ tlist,
+                                                          const uint32_t *__restrict__ counters,
+                                                          float4 *__restrict__ out, uint32_t list_blocks)
+{
+    if (blockIdx.x < list_blocks)
+        resolve_timeout_list(p, gbuf, pix_avg, pix_cnt, timeouts, counters, tflag, out, blockIdx.x, list_blocks);
+    else
+        resolve_timeout_tiles(p, gbuf, pix_avg, pix_cnt, tflag, tlist, counters, out, blockIdx.x - list_blocks,
+                              gridDim.x - list_blocks);
 }
 
 // RenderPass (final.frag:11-19): ACES filmic curve per channel, clamped to
@@ -2404,16 +2424,15 @@ void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
     launch(k_resolve, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.nbuf, b.materials,
                        b.pix_avg, b.pix_cnt, b.output, b.timeouts, b.counters, b.timeout_tiles, b.timeout_list);
     // timed-out pixels one wave each; in timeout-heavy frames, the tiles
-    // holding RECON_DENSE or more go to the tiled pass instead
-    launch(k_resolve_timeout, dim3(RESOLVE_TIMEOUT_BLOCKS), dim3(256), 0, s, p, b.gbuf, b.pix_avg, b.pix_cnt,
-           b.timeouts, b.counters, b.timeout_tiles, b.output);
+    // holding RECON_DENSE or more go to the tiled blocks instead
+    uint32_t tile_blocks = 0, lds = 0;
     if (p.reconstruction_radius <= (int)RECON_TILED_MAX) {
         const uint32_t D = RECON_TILE + 2u * (uint32_t)std::max(p.reconstruction_radius, 0);
-        const uint32_t tiles = cdiv(p.W, RECON_TILE) * cdiv(p.H, RECON_TILE);
-        launch(k_resolve_timeout_tiled, dim3(std::min(tiles, RECON_TILE_BLOCKS)), dim3(256),
-               (uint32_t)(D * D * sizeof(float4)), s, p, b.gbuf, b.pix_avg, b.pix_cnt, b.timeout_tiles,
-               b.timeout_list, b.counters, b.output);
+        tile_blocks = std::min(cdiv(p.W, RECON_TILE) * cdiv(p.H, RECON_TILE), RECON_TILE_BLOCKS);
+        lds = (uint32_t)(D * D * sizeof(float4));
     }
+    launch(k_resolve_timeouts, dim3(RESOLVE_TIMEOUT_BLOCKS + tile_blocks), dim3(256), lds, s, p, b.gbuf, b.pix_avg,
+           b.pix_cnt, b.timeouts, b.timeout_tiles, b.timeout_list, b.counters, b.output, RESOLVE_TIMEOUT_BLOCKS);
 }
 
 

@@ -1947,7 +1947,7 @@ __global__ __launch_bounds__(256) void k_resolve_avg(FrameParams p, const uint4 
 
 // AvgPass part 2 (resolve.comp:362-382): per-pixel colour of the selected view.
 // Timed-out pixels (reconstructTimeout) are appended to `timeouts` and
-// coloured by k_resolve_timeout.
+// coloured by k_resolve_timeouts.
 __global__ __launch_bounds__(256) void k_resolve(FrameParams p, const uint4 *__restrict__ gbuf,
                                                  const uint32_t *__restrict__ lbuf, const uint32_t *__restrict__ nbuf,
                                                  const Material *__restrict__ mats, const float4 *__restrict__ pix_avg,
@@ -2053,7 +2053,7 @@ __device__ __forceinline__ float wave_fsum(float v)
 // stride over the list and exit once it is exhausted.
 // reconstructTimeout (resolve.comp:193-219) for radii up to RECON_TILED_MAX,
 // on the 16x16 screen tiles holding RECON_DENSE or more timed-out pixels
-// (k_resolve lists and counts them; sparser tiles go to k_resolve_timeout):
+// (k_resolve lists and counts them; sparser tiles go to the listed form):
 // blocks stride over the listed tiles, and a dense one stages, for the tile
 // and its R-pixel halo, each pixel's tap weight base min(count, 32) (0 where
 // the tap is skipped: no voxel, timed out itself, no samples, or outside the
@@ -2072,7 +2072,7 @@ __device__ __forceinline__ void resolve_timeout_tiles(FrameParams p, const uint4
     extern __shared__ float4 tap[];   // (16 + 2R)^2 taps
     __shared__ uint32_t todo[RECON_TILE * RECON_TILE];
     __shared__ uint32_t n_todo;
-    if (counters[VE_CNT_TIMEOUTS] < RECON_TILED_MIN) return;   // k_resolve_timeout did every pixel
+    if (counters[VE_CNT_TIMEOUTS] < RECON_TILED_MIN) return;   // the listed form did every pixel
     const uint32_t tw = (p.W + RECON_TILE - 1u) / RECON_TILE;
     const uint32_t n_tiles = counters[VE_CNT_TIMEOUT_TILES];
     const int R = min(max(p.reconstruction_radius, 0), (int)RECON_TILED_MAX);
@@ -2080,7 +2080,7 @@ __device__ __forceinline__ void resolve_timeout_tiles(FrameParams p, const uint4
     for (uint32_t it = blk; it < n_tiles; it += nblk) {
         const uint32_t tile = tlist[it];
         const int tx0 = (int)((tile % tw) * RECON_TILE), ty0 = (int)((tile / tw) * RECON_TILE);
-        if (tflag[tile] < RECON_DENSE) continue;   // sparse tiles: k_resolve_timeout did them
+        if (tflag[tile] < RECON_DENSE) continue;   // sparse tiles: the listed form did them
         if (threadIdx.x == 0u) n_todo = 0u;
         __syncthreads();
         {   // the tile's timed-out pixels, packed onto the first threads
@@ -2172,7 +2172,7 @@ __device__ __forceinline__ void resolve_timeout_list(FrameParams p, const uint4 
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t n = counters[VE_CNT_TIMEOUTS];
     // in a frame with many timed-out pixels, those of dense tiles are left to
-    // k_resolve_timeout_tiled
+    // the tiled blocks
     const bool tiled = R <= RECON_TILED_MAX && n >= RECON_TILED_MIN;
     const uint32_t waves = nblk * (blockDim.x >> 6);
     for (uint32_t i = blk * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += waves) {

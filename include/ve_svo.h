@@ -300,7 +300,10 @@ int ve_node_stats(ve_ctx *ctx, uint64_t *primary_reads, uint64_t *trace_reads,
  * not thread-safe. */
 typedef struct ve_octree_cpu ve_octree_cpu;
 
-int  ve_octree_cpu_create(ve_octree_cpu **out, uint32_t depth, uint32_t max_words);   /* max_words 0 = 2^27 */
+/* max_words: the word limit (GL_MAX_TEXTURE_BUFFER_SIZE, octree.cpp:102-105);
+ * 0 = 2^27, values above 2^28 (what the 28-bit `next` field addresses) are
+ * clamped to 2^28, values below 9 are VE_EINVAL. */
+int  ve_octree_cpu_create(ve_octree_cpu **out, uint32_t depth, uint32_t max_words);
 void ve_octree_cpu_destroy(ve_octree_cpu *t);
 /* OctreeCPU::insert per entry, in order (octree_cpu.cpp:40-64) */
 int  ve_octree_cpu_insert(ve_octree_cpu *t, const ve_insert *ops, size_t n);
@@ -312,7 +315,13 @@ int  ve_octree_cpu_lookup(const ve_octree_cpu *t, uint32_t x, uint32_t y, uint32
 /* OctreeCPU::raycast (octree_cpu.cpp:128-215), the picking ray of the main loop */
 int  ve_octree_cpu_raycast(const ve_octree_cpu *t, const float origin[3], const float direction[3],
                            uint32_t max_steps, int *hit, uint32_t position[3], uint32_t *material);
+/* VE_ECAPACITY when the tree needs more words than max_words (the reference
+ * writes past its array there): nothing is written past the image, which is
+ * incomplete until a set succeeds (ve_octree_apply_edits fails until then). */
 int  ve_octree_set(ve_octree_cpu *t, ve_octree_info *info);
+/* VE_ECAPACITY when a new subtree finds no block under max_words: as in the
+ * reference (octree.cpp:290-294) its slot keeps its old word, and here its
+ * path also stays dirty, so a later apply_edits (after removes) writes it. */
 int  ve_octree_apply_edits(ve_octree_cpu *t, ve_octree_info *info);
 /* The node-word image (capacity words; info.size of them in use). */
 int  ve_octree_words(const ve_octree_cpu *t, const uint32_t **words, uint32_t *capacity);
@@ -405,6 +414,9 @@ int  ve_server_close(ve_server *srv, uint32_t session);
 int  ve_server_poll(ve_server *srv, ve_frame_done *out, uint32_t max, uint32_t *n_out, int wait);
 /* RGBA32F output of the session's last finished frame (waits for its frames). */
 int  ve_server_read_output(ve_server *srv, uint32_t session, float *rgba, size_t bytes);
+/* Any buffer of an admitted session's state, as ve_read on a lone context
+ * (same ids and layouts; waits for the session's frames). */
+int  ve_server_read(ve_server *srv, uint32_t session, ve_buffer_id id, void *dst, size_t bytes);
 int  ve_server_get_stats(ve_server *srv, ve_server_stats *out);
 
 #ifdef __cplusplus

@@ -137,3 +137,44 @@ def test_picking_ray_matches_node_word_raycast(ve, scenes, oracle):
         assert h1 == h2 and (not h1 or (p1 == p2 and m1 == m2))
         hits += h1
     assert hits > 50
+
+
+def test_word_limit_is_an_error_not_an_overflow(ve, scenes):
+    """A tree that needs more words than max_words: set() and applyEdits()
+    fail with VE_ECAPACITY, the image never grows past the limit, and after
+    removes bring the tree back under it the dirty path is written."""
+    cpu = ve.OctreeCPU(4, max_words=16)
+    rng = np.random.default_rng(3)
+    ops = np.zeros(64, scenes.INSERT_DTYPE)
+    ops["x"], ops["y"], ops["z"] = rng.integers(0, 16, (3, 64))
+    ops["material"] = 1
+    cpu.insert_ops(ops)
+    o = ve.Octree(4, max_words=16)
+    with pytest.raises(ve.VeError) as e:
+        o.set(cpu)
+    assert e.value.code == ve.VE_ECAPACITY
+    # the image the library holds never grew past the limit
+    import ctypes as C
+    w, cap = C.POINTER(C.c_uint32)(), C.c_uint32()
+    assert ve.lib().ve_octree_words(cpu.h, C.byref(w), C.byref(cap)) == 0
+    assert cap.value <= 16
+    with pytest.raises(ve.VeError) as e1:      # no incremental sync of an incomplete image
+        o.applyEdits(cpu)
+    assert e1.value.code == ve.VE_ECAPACITY
+
+    # one voxel fits in 9 + 8 * depth words; an edit that needs more fails,
+    # removing it again lets the next apply succeed
+    n_fit = 9 + 8 * 3
+    cpu2 = ve.OctreeCPU(4, max_words=n_fit)
+    cpu2.insert(1, 2, 3, 4)
+    o3 = ve.Octree(4).set(cpu2)
+    assert o3.size == n_fit and decode(o3.data, 4, 1, 2, 3) == 4
+    cpu2.insert(14, 13, 12, 5)                 # a second path: 3 more blocks
+    with pytest.raises(ve.VeError) as e3:
+        o3.applyEdits(cpu2)
+    assert e3.value.code == ve.VE_ECAPACITY
+    assert o3.capacity <= n_fit and decode(o3.data, 4, 1, 2, 3) == 4
+    cpu2.remove(1, 2, 3)                       # frees the first path's blocks
+    o3.applyEdits(cpu2)
+    assert decode(o3.data, 4, 14, 13, 12) == 5 and decode(o3.data, 4, 1, 2, 3) == 0
+    assert o3.size <= n_fit

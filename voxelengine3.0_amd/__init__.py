@@ -178,6 +178,7 @@ def lib():
         "ve_server_poll": (i32, [p, p, u32, C.POINTER(u32), i32]),
         "ve_server_read_output": (i32, [p, u32, p, sz]),
         "ve_server_get_stats": (i32, [p, C.POINTER(ServerStats)]),
+        "ve_server_read": (i32, [p, u32, C.c_int, p, sz]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -569,6 +570,14 @@ class Server:
         out = np.empty(width * height * 4, np.float32)
         self._ck(lib().ve_server_read_output(self.h, session, out.ctypes.data, out.nbytes))
         return out
+
+    def read(self, session, buf, dtype=np.uint32, nbytes=None):
+        """Any buffer of a session's state (ve_server_read), as Renderer.read."""
+        if nbytes is None:
+            raise ValueError("nbytes is required")
+        out = np.empty(int(nbytes), np.uint8)
+        self._ck(lib().ve_server_read(self.h, session, BUF[buf], out.ctypes.data, out.nbytes))
+        return out.view(dtype)
 
     def stats(self):
         st = ServerStats()

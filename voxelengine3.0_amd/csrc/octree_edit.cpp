@@ -54,6 +54,7 @@ struct ve_octree_cpu {
     std::vector<uint32_t> free_blocks;  // Octree::freeBlocks (a stack: back() is top())
     std::vector<uint32_t> changed;      // slots written by the last set / apply_edits
     bool full_upload = false;           // the last call rewrote the whole image
+    bool incomplete = false;            // the last set hit the word limit: no image until a set succeeds
 
     ~ve_octree_cpu() { free_subtree(root); }
 
@@ -102,14 +103,19 @@ struct ve_octree_cpu {
     }
 
     // Octree::resizeDataIfNeeded (octree.cpp:95-119): doubling, clamped to the
-    // word limit; a resize re-uploads the whole image
-    void resize(uint32_t required)
+    // word limit; a resize re-uploads the whole image.  Returns false when
+    // `required` exceeds the limit (the reference then writes past its
+    // array); nothing is resized then.
+    bool resize(uint32_t required)
     {
-        if (required <= capacity) return;
-        while (capacity < required) capacity *= 2;
-        if (capacity > max_words) capacity = max_words;
+        if (required <= capacity) return true;
+        if (required > max_words) return false;
+        uint64_t cap = capacity;
+        while (cap < required) cap *= 2;
+        capacity = (uint32_t)std::min<uint64_t>(cap, max_words);
         data.resize(capacity, 0u);
         full_upload = true;
+        return true;
     }
 
     void put(uint32_t slot, uint32_t w)
@@ -119,8 +125,9 @@ struct ve_octree_cpu {
     }
 
     // Octree::linearizeNode (octree.cpp:170-215): explicit-stack DFS,
-    // children pushed 0..7 (so 7 is linearised first)
-    void linearize(ENode *start, uint32_t off0, uint32_t level0)
+    // children pushed 0..7 (so 7 is linearised first).  Returns false, having
+    // written nothing past the array, when a block would cross the word limit.
+    bool linearize(ENode *start, uint32_t off0, uint32_t level0)
     {
         struct E {
             ENode *n;
@@ -134,29 +141,33 @@ struct ve_octree_cpu {
             e.n->dirty = false;
             if (e.lv > depth || (e.n->material != 0 && e.n->count == 0)) {
                 if (e.n->material == 0) continue;
-                resize(e.off + 1);
+                if (!resize(e.off + 1)) return false;
                 data[e.off] = leaf_word(e.n->material);
                 num_voxels++;
             } else {
-                resize(e.off + 1);
+                if ((uint64_t)size + 8u > max_words || !resize(size + 8u) || e.off >= capacity) return false;
                 const uint32_t block = size;
                 size += 8;
-                resize(size);
                 e.n->gpu_block = block;
                 data[e.off] = internal_word(block, e.n->count);
                 for (uint32_t i = 0; i < 8; i++) stk.push_back({e.n->child[i], block + i, e.lv + 1});
             }
         }
+        return true;
     }
 
-    // Octree::applyEditsNode (octree.cpp:252-312)
-    void apply_node(ENode *node, uint32_t slot, uint32_t level)
+    // Octree::applyEditsNode (octree.cpp:252-312).  Returns false when a
+    // subtree could not be given a block under the word limit: its slot keeps
+    // its old word, as in the reference (octree.cpp:290-294), but the path to
+    // it stays dirty so a later apply (after removes, or with a set into a
+    // larger limit) writes it.
+    bool apply_node(ENode *node, uint32_t slot, uint32_t level)
     {
         if (!node) {
             if (data[slot] != 0u) put(slot, 0u);
-            return;
+            return true;
         }
-        if (!node->dirty) return;
+        if (!node->dirty) return true;
         node->dirty = false;
         if (level > depth || (node->material != 0 && node->count == 0)) {
             put(slot, leaf_word(node->material));
@@ -165,7 +176,7 @@ struct ve_octree_cpu {
                 free_blocks.push_back(node->gpu_block);
                 node->gpu_block = NONE;
             }
-            return;
+            return true;
         }
         if (node->gpu_block == NONE) {
             uint32_t block;
@@ -174,15 +185,20 @@ struct ve_octree_cpu {
                 free_blocks.pop_back();
                 for (uint32_t i = 0; i < 8; i++) put(block + i, 0u);   // recycled block zeroed
             } else {
-                if ((uint64_t)size + 8u > max_words) return;   // the word limit: subtree left unsynced
+                if ((uint64_t)size + 8u > max_words || !resize(size + 8u)) {   // the word limit
+                    node->dirty = true;
+                    return false;
+                }
                 block = size;
                 size += 8;
-                resize(size);
             }
             node->gpu_block = block;
         }
         put(slot, internal_word(node->gpu_block, node->count));
-        for (uint32_t i = 0; i < 8; i++) apply_node(node->child[i], node->gpu_block + i, level + 1);
+        bool ok = true;
+        for (uint32_t i = 0; i < 8; i++) ok &= apply_node(node->child[i], node->gpu_block + i, level + 1);
+        if (!ok) node->dirty = true;
+        return ok;
     }
 };
 
@@ -196,7 +212,12 @@ int ve_octree_cpu_create(ve_octree_cpu **out, uint32_t depth, uint32_t max_words
     ve_octree_cpu *t = new (std::nothrow) ve_octree_cpu();
     if (!t) return ve_global_error(VE_ENOMEM, "octree allocation");
     t->depth = depth;
-    t->max_words = max_words ? max_words : (1u << 27);
+    // the 28-bit `next` field addresses at most 2^28 words
+    t->max_words = max_words ? std::min<uint32_t>(max_words, 1u << 28) : (1u << 27);
+    if (t->max_words < 9u) {
+        delete t;
+        return ve_global_error(VE_EINVAL, "max_words below 9 (the root and its first block)");
+    }
     t->data.assign(t->capacity, 0u);
     *out = t;
     return VE_OK;
@@ -341,8 +362,14 @@ int ve_octree_set(ve_octree_cpu *t, ve_octree_info *info)
         t->root->gpu_block = 1;
         t->root->dirty = false;
         t->data[0] = internal_word(1, t->root->count);
-        for (uint32_t i = 0; i < 8; i++) t->linearize(t->root->child[i], i + 1, 2);
+        for (uint32_t i = 0; i < 8; i++)
+            if (!t->linearize(t->root->child[i], i + 1, 2)) {
+                t->incomplete = true;
+                if (info) *info = ve_octree_info{t->depth, t->size, t->capacity, t->num_voxels};
+                return ve_global_error(VE_ECAPACITY, "octree needs more node words than the limit");
+            }
     }
+    t->incomplete = false;
     if (info) *info = ve_octree_info{t->depth, t->size, t->capacity, t->num_voxels};
     return VE_OK;
 }
@@ -351,6 +378,7 @@ int ve_octree_set(ve_octree_cpu *t, ve_octree_info *info)
 int ve_octree_apply_edits(ve_octree_cpu *t, ve_octree_info *info)
 {
     if (!t) return ve_global_error(VE_EINVAL, "null argument");
+    if (t->incomplete) return ve_global_error(VE_ECAPACITY, "the last set hit the word limit; set again first");
     t->changed.clear();
     t->full_upload = false;
     if (t->root) {
@@ -359,7 +387,14 @@ int ve_octree_apply_edits(ve_octree_cpu *t, ve_octree_info *info)
         if (t->root->dirty) {
             t->put(0, internal_word(1, t->root->count));
             t->root->dirty = false;
-            for (uint32_t i = 0; i < 8; i++) t->apply_node(t->root->child[i], i + 1, 2);
+            bool ok = true;
+            for (uint32_t i = 0; i < 8; i++) ok &= t->apply_node(t->root->child[i], i + 1, 2);
+            if (!ok) {
+                t->root->dirty = true;
+                if (info) *info = ve_octree_info{t->depth, t->size, t->capacity, t->num_voxels};
+                return ve_global_error(VE_ECAPACITY, "edits need more node words than the limit; "
+                                                     "the subtrees without a block are left out of the image");
+            }
         }
     }
     if (info) *info = ve_octree_info{t->depth, t->size, t->capacity, t->num_voxels};

@@ -477,6 +477,17 @@ int ve_server_read_output(ve_server *s, uint32_t session, float *rgba, size_t by
     return VE_OK;
 }
 
+int ve_server_read(ve_server *s, uint32_t session, ve_buffer_id id, void *dst, size_t bytes)
+{
+    if (!s || !dst) return ve_global_error(VE_EINVAL, "null argument");
+    auto it = s->sessions.find(session);
+    if (it == s->sessions.end() || !it->second.ctx)
+        return srv_error(s, VE_ENOTFOUND, "session %u holds no device state", session);
+    const int rc = ve_read(it->second.ctx, id, dst, bytes);   // ordered after the session's frames
+    if (rc != VE_OK) return srv_error(s, rc, "session %u: %s", session, it->second.ctx->err.c_str());
+    return VE_OK;
+}
+
 int ve_server_get_stats(ve_server *s, ve_server_stats *o)
 {
     if (!s || !o) return ve_global_error(VE_EINVAL, "null argument");

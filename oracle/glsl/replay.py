@@ -45,7 +45,13 @@ def _prog(name, defines=None):
 class GLPipeline:
     """The reference renderer's GPU state on llvmpipe for one scene."""
 
-    def __init__(self, words, depth, W, H, max_bounces=8, materials=None, sky_rgb8=(0, 0, 0), sky_faces=None):
+    def __init__(self, words, depth, W, H, max_bounces=8, materials=None, sky_rgb8=(0, 0, 0), sky_faces=None,
+                 hash_width=None):
+        """hash_width: the width of the l/n/rBuffer images and of the
+        lBufferWidth / nBufferWidth / rBufferWidth uniforms, which the
+        reference takes from GL_MAX_TEXTURE_SIZE (renderer.cpp:27-40); None =
+        this driver's value.  A smaller width stands in for a driver with a
+        smaller limit (every bucket then holds more voxels)."""
         self.gl = G.GL.get()
         self.W, self.H, self.depth = W, H, depth
         self.max_bounces = max_bounces
@@ -55,7 +61,7 @@ class GLPipeline:
                      "emit.comp", "trace.comp", "shade.comp", "resolve.comp"):
             self.progs[name] = None
         self.defines = defines
-        self.hash_w = self.gl.integer(0x0D33)   # GL_MAX_TEXTURE_SIZE (renderer.cpp:27)
+        self.hash_w = self.gl.integer(0x0D33) if hash_width is None else int(hash_width)   # renderer.cpp:27
         self.tbo = G.TextureBuffer(np.ascontiguousarray(words, np.uint32))
         self.cam_ubo = G.Buffer(nbytes=64)
         if materials is None:

@@ -213,6 +213,18 @@ class Frame:
         if rc != 0:
             raise RuntimeError("oracle frame failed")
 
+    def resolve(self, words, depth, ubo, fc, n_words=None):
+        """resolve.comp alone on the current state (render type etc. from fc)."""
+        L = self.L
+        L.so_resolve.restype = C.c_int
+        L.so_resolve.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.POINTER(FrameConfig)]
+        self._words = np.ascontiguousarray(words, np.uint32)
+        ubo = np.ascontiguousarray(ubo, np.float32)
+        self._fc = fc
+        n = len(self._words) if n_words is None else n_words
+        if L.so_resolve(self.h, self._words.ctypes.data, n, depth, ubo.ctypes.data, C.byref(fc)) != 0:
+            raise RuntimeError("oracle resolve failed")
+
     def read(self, name, dtype=np.uint32):
         p = C.c_void_p()
         n = self.L.so_buffer(self.h, BUFFER_IDS[name], C.byref(p))
@@ -224,3 +236,15 @@ class Frame:
         out = np.zeros(16, np.uint32)
         self.L.so_counters(self.h, out.ctypes.data)
         return out
+
+    def emit_records(self, base=0, n=None):
+        """emit.comp's Ray records (24 uint32 each) for shade-list entries
+        [base, base + n) of the last frame."""
+        L = self.L
+        L.so_emit_records.restype = C.c_size_t
+        L.so_emit_records.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+        if n is None:
+            n = int(self.counters()[1])
+        out = np.zeros((max(n, 1), 24), np.uint32)
+        got = L.so_emit_records(self.h, base, n, out.ctypes.data)
+        return out[:got]

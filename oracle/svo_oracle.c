@@ -267,9 +267,23 @@ static inline uint32_t gl_locate(const uint32_t p[3], uint32_t p2)   /* internal
 
 static inline uint32_t fetch(const uint32_t *w, uint32_t n, uint32_t i) { return i < n ? w[i] : 0u; }
 
+/* log(x) = log2(x) * ln2, exp(x) = exp2(x * log2e), pow(x, y) =
+ * exp2(log2(x) * y) (probed on llvmpipe: these compositions match 100%).
+ * The log2 / exp2 / sin / cos cores are evaluated in double and rounded
+ * once to float; the GPU kernels use the same definition, so the oracle and
+ * the product agree bit for bit, and both stay within a few ulp of
+ * llvmpipe's polynomial cores. */
+static inline float m_log2(float x) { return (float)log2((double)x); }
+static inline float m_exp2(float x) { return (float)exp2((double)x); }
+static inline float m_log(float x) { return m_log2(x) * 0.6931471805599453f; }
+static inline float m_exp(float x) { return m_exp2(x * 1.4426950408889634f); }
+static inline float m_pow(float x, float y) { return m_exp2(m_log2(x) * y); }
+static inline float m_cos(float x) { return (float)cos((double)x); }
+static inline float m_sin(float x) { return (float)sin((double)x); }
 /* Elementwise FP primitives of the GLSL lowering, for comparison with
  * llvmpipe probe outputs.  op 0: dot(v,v) of vec3 in[3i..3i+2]; 1: normalize
- * (3 outputs); 2: a/b; 3: a*b+c; 4: min(a,b); 5: max(a,b). */
+ * (3 outputs); 2: a/b; 3: a*b+c; 4: min(a,b); 5: max(a,b); 6-14: inversesqrt,
+ * sqrt, pow, exp, log, exp2, log2, sin, cos of the probe shaders' arguments. */
 void so_fp_probe(int op, const float *in, float *out, size_t n)
 {
     for (size_t i = 0; i < n; i++) {
@@ -286,6 +300,16 @@ void so_fp_probe(int op, const float *in, float *out, size_t n)
         case 3: out[i] = madd(a[0], a[1], a[2]); break;
         case 4: out[i] = gmin(a[0], a[1]); break;
         case 5: out[i] = gmax(a[0], a[1]); break;
+        /* the built-in probes of tests/golden/make_golden.py, same arguments */
+        case 6: out[i] = 1.0f / sqrtf(fabsf(a[0]) + 1e-6f); break;                     /* inversesqrt */
+        case 7: out[i] = sqrtf(fabsf(a[0])); break;
+        case 8: out[i] = m_pow(fabsf(a[0]) * 0.125f, fabsf(a[1]) * 16.0f); break;
+        case 9: out[i] = m_exp(a[0] * a[2]); break;
+        case 10: out[i] = m_log(fabsf(a[0]) + 1e-6f); break;
+        case 11: out[i] = m_exp2(a[0] * 2.0f); break;
+        case 12: out[i] = m_log2(fabsf(a[0]) + 1e-6f); break;
+        case 13: out[i] = m_sin(a[0] * a[2]); break;
+        case 14: out[i] = m_cos(a[0] * a[2]); break;
         default: out[i] = 0.0f;
         }
     }
@@ -472,19 +496,6 @@ void so_generate_primary(const uint32_t *words, uint32_t n_words, uint32_t depth
 /* ======================================================================== */
 /* GLSL built-ins as llvmpipe composes them                                  */
 /* ======================================================================== */
-/* log(x) = log2(x) * ln2, exp(x) = exp2(x * log2e), pow(x, y) =
- * exp2(log2(x) * y) (probed on llvmpipe: these compositions match 100%).
- * The log2 / exp2 / sin / cos cores are evaluated in double and rounded
- * once to float; the GPU kernels use the same definition, so the oracle and
- * the product agree bit for bit, and both stay within a few ulp of
- * llvmpipe's polynomial cores. */
-static inline float m_log2(float x) { return (float)log2((double)x); }
-static inline float m_exp2(float x) { return (float)exp2((double)x); }
-static inline float m_log(float x) { return m_log2(x) * 0.6931471805599453f; }
-static inline float m_exp(float x) { return m_exp2(x * 1.4426950408889634f); }
-static inline float m_pow(float x, float y) { return m_exp2(m_log2(x) * y); }
-static inline float m_cos(float x) { return (float)cos((double)x); }
-static inline float m_sin(float x) { return (float)sin((double)x); }
 static inline float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
 static inline float gsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
 
@@ -868,36 +879,84 @@ static uint32_t pack_normal(v3 n)   /* normal.comp:90-95, round() = round-half-e
     return ((uint32_t)x << 22) | ((uint32_t)y << 12) | ((uint32_t)z << 2);
 }
 
-static void normal_pixel(so_state *s, uint32_t x, uint32_t y)   /* normal.comp:124-177 */
+/* normal.comp:124-177 for the eight invocations of one SIMD-8 vector (a row
+ * of eight pixels of a work group), in llvmpipe's lockstep order: per
+ * iteration of the claim loop every active lane issues its compare-and-swap
+ * (lane order), then the lanes that found their slot read its normal,
+ * compute and store it if it is empty (lane order) and stamp the time; the
+ * lanes that found no slot then all scan the slot times before any of them
+ * writes, so lanes of one vector that evict in one bucket pick the same
+ * least recently used slot and the last of them owns it. */
+static void normal_vec(so_state *s, uint32_t y, uint32_t x0, uint32_t nl)
 {
-    size_t pix = (size_t)y * s->W + x;
-    uint32_t vid = s->gbuf[4 * pix + 3];
-    if (vid == 0) return;
-    uint32_t len = 1u << s->depth;
-    uint32_t pk = s->gpos[pix], bits = s->depth - 1u, mask = (1u << bits) - 1u;
-    uint32_t pos[3] = {(pk & mask) * 2u, ((pk >> bits) & mask) * 2u, ((pk >> (2u * bits)) & mask) * 2u};
-    uint32_t h = vid % (s->hw - 1u) + 1u;
+    uint32_t vid[8], h[8], pos[8][3];
+    int active[8], evict[8];
+    uint32_t len = 1u << s->depth, bits = s->depth - 1u, mask = (1u << bits) - 1u;
+    for (uint32_t l = 0; l < nl; l++) {
+        size_t pix = (size_t)y * s->W + x0 + l;
+        vid[l] = s->gbuf[4 * pix + 3];
+        active[l] = vid[l] != 0;
+        evict[l] = 0;
+        if (!active[l]) continue;
+        uint32_t pk = s->gpos[pix];
+        pos[l][0] = (pk & mask) * 2u;
+        pos[l][1] = ((pk >> bits) & mask) * 2u;
+        pos[l][2] = ((pk >> (2u * bits)) & mask) * 2u;
+        h[l] = vid[l] % (s->hw - 1u) + 1u;
+    }
     for (uint32_t sl = 0; sl < NSLOTS; sl++) {
-        uint32_t *own = &NB(s, h, sl * NSTRIDE);
-        uint32_t prev = *own;
-        if (prev == 0) *own = vid;
-        if (prev == 0 || prev == vid) {
-            if (NB(s, h, sl * NSTRIDE + 1) == 0) {
-                v3 n = refined_normal(s, pos, len);
-                if (NB(s, h, sl * NSTRIDE + 1) == 0) NB(s, h, sl * NSTRIDE + 1) = pack_normal(n);
+        int found[8];
+        for (uint32_t l = 0; l < nl; l++) {   /* imageAtomicCompSwap(owner, 0, vid), lane order */
+            found[l] = 0;
+            if (!active[l]) continue;
+            uint32_t *own = &NB(s, h[l], sl * NSTRIDE);
+            uint32_t prev = *own;
+            if (prev == 0) *own = vid[l];
+            found[l] = prev == 0 || prev == vid[l];
+        }
+        uint32_t existing[8];
+        for (uint32_t l = 0; l < nl; l++)
+            if (found[l]) existing[l] = NB(s, h[l], sl * NSTRIDE + 1);
+        for (uint32_t l = 0; l < nl; l++)   /* compSwap(normal, 0, n) in lane order */
+            if (found[l] && existing[l] == 0) {
+                uint32_t n = pack_normal(refined_normal(s, pos[l], len));
+                if (NB(s, h[l], sl * NSTRIDE + 1) == 0) NB(s, h[l], sl * NSTRIDE + 1) = n;
             }
-            NB(s, h, sl * NSTRIDE + 2) = s->time;
-            return;
+        for (uint32_t l = 0; l < nl; l++)
+            if (found[l]) {
+                NB(s, h[l], sl * NSTRIDE + 2) = s->time;
+                active[l] = 0;
+            }
+    }
+    uint32_t oslot[8];
+    for (uint32_t l = 0; l < nl; l++) {   /* the time scans of every evicting lane first */
+        if (!active[l]) continue;
+        evict[l] = 1;
+        uint32_t oldest = 0xFFFFFFFFu;
+        oslot[l] = 0;
+        for (uint32_t sl = 0; sl < NSLOTS; sl++) {
+            uint32_t t = NB(s, h[l], sl * NSTRIDE + 2);
+            if (t < oldest) { oldest = t; oslot[l] = sl; }
         }
     }
-    uint32_t oldest = 0xFFFFFFFFu, oslot = 0;
-    for (uint32_t sl = 0; sl < NSLOTS; sl++) {
-        uint32_t t = NB(s, h, sl * NSTRIDE + 2);
-        if (t < oldest) { oldest = t; oslot = sl; }
-    }
-    NB(s, h, oslot * NSTRIDE) = vid;
-    NB(s, h, oslot * NSTRIDE + 1) = pack_normal(refined_normal(s, pos, len));
-    NB(s, h, oslot * NSTRIDE + 2) = s->time;
+    for (uint32_t l = 0; l < nl; l++)
+        if (evict[l]) NB(s, h[l], oslot[l] * NSTRIDE) = vid[l];
+    for (uint32_t l = 0; l < nl; l++)
+        if (evict[l]) NB(s, h[l], oslot[l] * NSTRIDE + 1) = pack_normal(refined_normal(s, pos[l], len));
+    for (uint32_t l = 0; l < nl; l++)
+        if (evict[l]) NB(s, h[l], oslot[l] * NSTRIDE + 2) = s->time;
+}
+
+static void normal_pass(so_state *s)   /* 8x8 work groups in dispatch order, one SIMD-8 vector per row */
+{
+    uint32_t gw = (s->W + 7u) / 8u, gh = (s->H + 7u) / 8u;
+    for (uint32_t gy = 0; gy < gh; gy++)
+        for (uint32_t gx = 0; gx < gw; gx++)
+            for (uint32_t ly = 0; ly < 8u; ly++) {
+                uint32_t y = gy * 8u + ly, x0 = gx * 8u;
+                if (y >= s->H || x0 >= s->W) continue;
+                normal_vec(s, y, x0, s->W - x0 < 8u ? s->W - x0 : 8u);
+            }
 }
 
 /* 8x8 work groups in dispatch order (x fastest), invocations in local order */
@@ -1108,24 +1167,39 @@ static v3 pixel_dir(const so_state *s, uint32_t px, uint32_t py)
     return V(d[0], d[1], d[2]);
 }
 
+/* The Ray record emit.comp:55-92 writes for shade-list entry li. */
+static ray_t emit_record(const so_state *s, uint32_t li)
+{
+    uint32_t vid = s->shade_list[2 * (size_t)li], pix = s->shade_list[2 * (size_t)li + 1];
+    uint32_t px = pix % s->W, py = pix / s->W;
+    v3 d = pixel_dir(s, px, py);
+    ray_t r;
+    memset(&r, 0, sizeof r);
+    r.ox = s->cam[0]; r.oy = s->cam[1]; r.oz = s->cam[2];
+    r.dx = d.x; r.dy = d.y; r.dz = d.z;
+    r.tx = r.ty = r.tz = 1.0f;
+    r.primary_vid = vid;
+    uint32_t sample_idx = s->frame_idx + pix * 2654435761u + li * 1442695040u;
+    r.rng = (sample_idx << 8) & 0xFFFFFF00u;
+    return r;
+}
+
 static void emit_pass(so_state *s, uint32_t chunk_base, uint32_t chunk_size)   /* emit.comp:39-100 */
 {
     for (uint32_t t = 0; t < chunk_size; t++) {
-        uint32_t li = chunk_base + t;
-        uint32_t vid = s->shade_list[2 * (size_t)li], pix = s->shade_list[2 * (size_t)li + 1];
-        uint32_t px = pix % s->W, py = pix / s->W;
-        v3 d = pixel_dir(s, px, py);
-        ray_t r;
-        memset(&r, 0, sizeof r);
-        r.ox = s->cam[0]; r.oy = s->cam[1]; r.oz = s->cam[2];
-        r.dx = d.x; r.dy = d.y; r.dz = d.z;
-        r.tx = r.ty = r.tz = 1.0f;
-        r.primary_vid = vid;
-        uint32_t sample_idx = s->frame_idx + pix * 2654435761u + li * 1442695040u;
-        r.rng = (sample_idx << 8) & 0xFFFFFF00u;
+        /* one atomicAdd(ring_head) per invocation, in invocation order */
         uint32_t slot = s->ring_head++;
-        s->ring[slot % RAY_CAP] = r;
+        s->ring[slot % RAY_CAP] = emit_record(s, chunk_base + t);
     }
+}
+
+size_t so_emit_records(const so_state *s, uint32_t base, uint32_t n, void *out)
+{
+    if (base >= s->shade_count) return 0;
+    if (n > s->shade_count - base) n = s->shade_count - base;
+    ray_t *o = (ray_t *)out;
+    for (uint32_t t = 0; t < n; t++) o[t] = emit_record(s, base + t);
+    return n;
 }
 
 static void trace_pass(so_state *s, uint32_t base, uint32_t size)   /* trace.comp:25-70 */
@@ -1186,40 +1260,98 @@ static void firefly(const so_state *s, uint32_t h, uint32_t i, uint32_t cnt_fiel
     }
 }
 
-static void deposit(so_state *s, uint32_t vid, v3 dir_rgb, v3 ind_rgb)   /* shade.comp:162-257 */
+/* depositSampleDual (shade.comp:162-257) for the lanes of one SIMD-8 vector
+ * that reach the same call site together, in llvmpipe's lockstep order: per
+ * iteration of the slot loop every active lane issues its compare-and-swap
+ * (lane order), then every lane loads the slot's owner and time, then the
+ * lanes whose id matches deposit (each step of the deposit body for all of
+ * them before the next step: halving loads before halving stores, firefly
+ * loads before any add) and leave the loop, the others fold the loaded time
+ * into their eviction candidate; lanes that never match reset their
+ * candidate slot after the loop, in lane order.  Lanes of one vector thus
+ * see each other's claims of this iteration but not the time stamps their
+ * deposits write later in it: a lane can evict the slot another lane of its
+ * vector has just claimed (its loaded time is still the old one).  With one
+ * lane this is the sequential deposit. */
+static void deposit_vec(so_state *s, uint32_t n, const uint32_t *vid, const v3 *dir_rgb, const v3 *ind_rgb)
 {
-    if (vid == 0) return;
-    uint32_t d[4], e[4];
-    pack_rgb_l2(dir_rgb, d);
-    pack_rgb_l2(ind_rgb, e);
-    uint32_t h = vid % (s->hw - 1u) + 1u;
-    uint32_t cand_t = 0xFFFFFFFFu, cand_i = 0;
-    int have = 0;
-    for (uint32_t i = 0; i < LSTRIDE * LSLOTS; i += LSTRIDE) {
-        if (LB(s, h, i) == 0) LB(s, h, i) = vid;
-        uint32_t id = LB(s, h, i), st = LB(s, h, i + 11);
-        if (id != vid) {
-            if (st < cand_t) { cand_t = st; cand_i = i; have = 1; }
-            continue;
-        }
-        if (!s->fc->disable_stale_reset && st != 0 && st + s->fc->update_time < s->time) {
-            reset_lslot(s, h, i, vid, d, e);
-            return;
-        }
-        if (LB(s, h, i + 1) >= (uint32_t)s->fc->sample_cap_direct)
-            for (int f = 1; f <= 5; f++) LB(s, h, i + f) >>= 1;
-        if (LB(s, h, i + 6) >= (uint32_t)s->fc->sample_cap_indirect)
-            for (int f = 6; f <= 10; f++) LB(s, h, i + f) >>= 1;
-        firefly(s, h, i, 1, d);
-        firefly(s, h, i, 6, e);
-        LB(s, h, i + 1) += 1; LB(s, h, i + 2) += d[0]; LB(s, h, i + 3) += d[1]; LB(s, h, i + 4) += d[2];
-        LB(s, h, i + 5) += d[3];
-        LB(s, h, i + 6) += 1; LB(s, h, i + 7) += e[0]; LB(s, h, i + 8) += e[1]; LB(s, h, i + 9) += e[2];
-        LB(s, h, i + 10) += e[3];
-        LB(s, h, i + 11) = s->time;
-        return;
+    uint32_t d[8][4], e[8][4], h[8], cand_t[8], cand_i[8];
+    int active[8];
+    for (uint32_t l = 0; l < n; l++) {
+        active[l] = vid[l] != 0;
+        if (!active[l]) continue;
+        pack_rgb_l2(dir_rgb[l], d[l]);
+        pack_rgb_l2(ind_rgb[l], e[l]);
+        h[l] = vid[l] % (s->hw - 1u) + 1u;
+        cand_t[l] = 0xFFFFFFFFu;
+        cand_i[l] = 0;
     }
-    if (have) reset_lslot(s, h, cand_i, vid, d, e);
+    for (uint32_t i = 0; i < LSTRIDE * LSLOTS; i += LSTRIDE) {
+        uint32_t id[8], st[8];
+        int match[8];
+        for (uint32_t l = 0; l < n; l++)   /* imageAtomicCompSwap(owner, 0, vid), lane order */
+            if (active[l] && LB(s, h[l], i) == 0) LB(s, h[l], i) = vid[l];
+        for (uint32_t l = 0; l < n; l++) {   /* owner and time loads */
+            if (!active[l]) continue;
+            id[l] = LB(s, h[l], i);
+            st[l] = LB(s, h[l], i + 11);
+        }
+        int any = 0;
+        for (uint32_t l = 0; l < n; l++) {
+            match[l] = 0;
+            if (!active[l]) continue;
+            if (id[l] != vid[l]) {
+                if (st[l] < cand_t[l]) { cand_t[l] = st[l]; cand_i[l] = i; }
+                continue;
+            }
+            match[l] = 1;
+            any = 1;
+        }
+        if (!any) continue;
+        /* stale reset (shade.comp:183-186): those lanes leave */
+        for (uint32_t l = 0; l < n; l++) {
+            if (!match[l]) continue;
+            if (!s->fc->disable_stale_reset && st[l] != 0 && st[l] + s->fc->update_time < s->time) {
+                reset_lslot(s, h[l], i, vid[l], d[l], e[l]);
+                match[l] = 0;
+                active[l] = 0;
+            }
+        }
+        /* halving at the caps (shade.comp:188-194): every count load, then
+         * the halving lanes' field loads, then their stores */
+        for (int ch = 0; ch < 2; ch++) {
+            uint32_t f0 = ch == 0 ? 1u : 6u, cap = ch == 0 ? (uint32_t)s->fc->sample_cap_direct
+                                                          : (uint32_t)s->fc->sample_cap_indirect;
+            uint32_t cnt[8], old[8][5];
+            for (uint32_t l = 0; l < n; l++)
+                if (match[l]) cnt[l] = LB(s, h[l], i + f0);
+            for (uint32_t l = 0; l < n; l++)
+                if (match[l] && cnt[l] >= cap)
+                    for (uint32_t f = 0; f < 5; f++) old[l][f] = LB(s, h[l], i + f0 + f);
+            for (uint32_t f = 0; f < 5; f++)
+                for (uint32_t l = 0; l < n; l++)
+                    if (match[l] && cnt[l] >= cap) LB(s, h[l], i + f0 + f) = old[l][f] >> 1;
+        }
+        /* firefly clamps (shade.comp:201-234) read the sums before any lane adds */
+        for (uint32_t l = 0; l < n; l++)
+            if (match[l]) firefly(s, h[l], i, 1, d[l]);
+        for (uint32_t l = 0; l < n; l++)
+            if (match[l]) firefly(s, h[l], i, 6, e[l]);
+        for (uint32_t l = 0; l < n; l++) {   /* the adds commute; time last */
+            if (!match[l]) continue;
+            LB(s, h[l], i + 1) += 1; LB(s, h[l], i + 2) += d[l][0]; LB(s, h[l], i + 3) += d[l][1];
+            LB(s, h[l], i + 4) += d[l][2]; LB(s, h[l], i + 5) += d[l][3];
+            LB(s, h[l], i + 6) += 1; LB(s, h[l], i + 7) += e[l][0]; LB(s, h[l], i + 8) += e[l][1];
+            LB(s, h[l], i + 9) += e[l][2]; LB(s, h[l], i + 10) += e[l][3];
+        }
+        for (uint32_t l = 0; l < n; l++)
+            if (match[l]) {
+                LB(s, h[l], i + 11) = s->time;
+                active[l] = 0;
+            }
+    }
+    for (uint32_t l = 0; l < n; l++)   /* bucket full: reset the candidate (shade.comp:254-256) */
+        if (active[l] && cand_t[l] != 0xFFFFFFFFu) reset_lslot(s, h[l], cand_i[l], vid[l], d[l], e[l]);
 }
 
 static v3 unpack_normal(uint32_t p)
@@ -1561,17 +1693,22 @@ static void shade_pass(so_state *s, uint32_t base, uint32_t size)
             live[l] = r[l].bounce < MB;
         }
         /* case 1: misses */
+        uint32_t dn = 0, dvid[8];
+        v3 ddir[8], dind[8];
         for (uint32_t l = 0; l < 8; l++) {
             if (!live[l] || r[l].hit_vid != 0) continue;
             v3 sky = sky_sample(s, V(r[l].dx, r[l].dy, r[l].dz));   /* shade.comp:559 */
             r[l].ilx = r[l].ilx + sky.x * r[l].tx;
             r[l].ily = r[l].ily + sky.y * r[l].ty;
             r[l].ilz = r[l].ilz + sky.z * r[l].tz;
-            deposit(s, r[l].primary_vid, V(r[l].dlx, r[l].dly, r[l].dlz), V(r[l].ilx, r[l].ily, r[l].ilz));
+            dvid[dn] = r[l].primary_vid;
+            ddir[dn] = V(r[l].dlx, r[l].dly, r[l].dlz);
+            dind[dn++] = V(r[l].ilx, r[l].ily, r[l].ilz);
             r[l].bounce = MB;
             s->ring[slot[l]] = r[l];
             live[l] = 0;
         }
+        deposit_vec(s, dn, dvid, ddir, dind);
         v3 normal[8];
         uint32_t hp[8][3], hsize[8];
         const mat_t *mat[8];
@@ -1634,13 +1771,17 @@ static void shade_pass(so_state *s, uint32_t base, uint32_t size)
                 if (st[l].pending) rbuf_store(s, st[l].vid, &st[l].res);
         }
         /* last allowed bounce: deposit and die */
+        dn = 0;
         for (uint32_t l = 0; l < 8; l++) {
             if (!live[l] || r[l].bounce + 1u < MB) continue;
-            deposit(s, r[l].primary_vid, V(r[l].dlx, r[l].dly, r[l].dlz), V(r[l].ilx, r[l].ily, r[l].ilz));
+            dvid[dn] = r[l].primary_vid;
+            ddir[dn] = V(r[l].dlx, r[l].dly, r[l].dlz);
+            dind[dn++] = V(r[l].ilx, r[l].ily, r[l].ilz);
             r[l].bounce = MB;
             s->ring[slot[l]] = r[l];
             live[l] = 0;
         }
+        deposit_vec(s, dn, dvid, ddir, dind);
         /* BRDF sampling + Russian roulette */
         v3 sdir[8], ntp[8];
         for (uint32_t l = 0; l < 8; l++) {
@@ -1669,13 +1810,16 @@ static void shade_pass(so_state *s, uint32_t base, uint32_t size)
             r[l].rng = rng;
             ntp[l] = vmul(V(r[l].tx, r[l].ty, r[l].tz), bt);
         }
+        dn = 0;
         for (uint32_t l = 0; l < 8; l++) {
             if (!live[l]) continue;
             if (r[l].bounce + 1u >= 2u) {   /* RR_START_BOUNCE */
                 float p = gclamp(gmax(ntp[l].x, gmax(ntp[l].y, ntp[l].z)), 0.05f, 0.95f);
                 float xi = rand_lds(&r[l].rng);
                 if (xi > p) {
-                    deposit(s, r[l].primary_vid, V(r[l].dlx, r[l].dly, r[l].dlz), V(r[l].ilx, r[l].ily, r[l].ilz));
+                    dvid[dn] = r[l].primary_vid;
+                    ddir[dn] = V(r[l].dlx, r[l].dly, r[l].dlz);
+                    dind[dn++] = V(r[l].ilx, r[l].ily, r[l].ilz);
                     r[l].bounce = MB;
                     s->ring[slot[l]] = r[l];
                     live[l] = 0;
@@ -1684,6 +1828,7 @@ static void shade_pass(so_state *s, uint32_t base, uint32_t size)
                 ntp[l] = vdivs(ntp[l], p);
             }
         }
+        deposit_vec(s, dn, dvid, ddir, dind);
         /* survivors: mark the slot dead, append the continuation at ring_head */
         for (uint32_t l = 0; l < 8; l++) {
             if (!live[l]) continue;
@@ -1898,7 +2043,7 @@ int so_frame(so_state *s, const uint32_t *words, uint32_t n_words, uint32_t dept
                         NULL, NULL, 1);
     s->counters[6] = s->W * s->H;
     /* 2. normal */
-    FOR_GROUPS_2D(s, X, Y, normal_pixel(s, X, Y));
+    normal_pass(s);
     /* per-frame reset (renderer.cpp:343-375) */
     s->ring_head = s->ring_tail = 0;
     s->shade_count = 0;
@@ -1942,6 +2087,18 @@ int so_frame(so_state *s, const uint32_t *words, uint32_t n_words, uint32_t dept
     s->counters[7] = bounces_run;
     s->counters[2] = s->emissive_count;
     /* 6. resolve */
+    FOR_GROUPS_2D(s, X, Y, resolve_pixel(s, X, Y));
+    return 0;
+}
+
+int so_resolve(so_state *s, const uint32_t *words, uint32_t n_words, uint32_t depth, const float ubo[16],
+               const so_frame_config *fc)
+{
+    s->words = words;
+    s->n_words = n_words;
+    s->depth = depth;
+    memcpy(s->cam, ubo, sizeof s->cam);
+    s->fc = fc;
     FOR_GROUPS_2D(s, X, Y, resolve_pixel(s, X, Y));
     return 0;
 }

@@ -93,6 +93,17 @@ void so_sky_sample(const so_state *s, const float *dirs, size_t n, float *rgb);
 int so_frame(so_state *s, const uint32_t *words, uint32_t n_words, uint32_t depth, const float ubo[16],
              const so_frame_config *fc, uint32_t frame_idx, uint32_t frame_seed, uint32_t time);
 
+/* resolve.comp alone (renderer.cpp:612-648) on the current state: the last
+ * frame's G-buffer and the l/nBuffer, with `fc`'s render type, primary
+ * control checks and reconstruction radius. */
+int so_resolve(so_state *s, const uint32_t *words, uint32_t n_words, uint32_t depth, const float ubo[16],
+               const so_frame_config *fc);
+
+/* The 96-byte Ray records emit.comp writes for shade-list entries
+ * [base, base + n) of the last frame (its camera, frame index and shade
+ * list), in ring order; returns the number written. */
+size_t so_emit_records(const so_state *s, uint32_t base, uint32_t n, void *out);
+
 /* Buffer access by the same ids as the product's ve_buffer_id. */
 size_t so_buffer(so_state *s, int id, void **ptr);
 void so_counters(so_state *s, uint32_t out[16]);

@@ -7,6 +7,7 @@ scenes of voxelengine3.0_amd/scenes.py, and stores inputs' digests plus the
 pass outputs as compressed .npz files next to this script.
 
     LP_NUM_THREADS=1 python tests/golden/make_golden.py [fp] [cube] [primary] [frame] [lights] [sky]
+                                                        [overflow] [material] [loops]
 
 Only the outputs are stored; the shaders are never copied.
 """
@@ -82,6 +83,19 @@ FP_PROBES = {
     "mad": "o[i] = a * b + c;",
     "min": "o[i] = min(a, b);",
     "max": "o[i] = max(a, b);",
+    # built-ins of the sampling and shading code (internal.glsl:57-239,
+    # resolve.comp:235-260): inversesqrt in the VNDF frame, sqrt, pow in the
+    # Blinn-Phong lobe and the Fresnel term, exp in the bilateral weights,
+    # log / cos / sin in Box-Muller and the VNDF disk
+    "inversesqrt": "o[i] = inversesqrt(abs(a) + 1e-6);",
+    "sqrt": "o[i] = sqrt(abs(a));",
+    "pow": "o[i] = pow(abs(a) * 0.125, abs(b) * 16.0);",
+    "exp": "o[i] = exp(a * c);",
+    "log": "o[i] = log(abs(a) + 1e-6);",
+    "exp2": "o[i] = exp2(a * 2.0);",
+    "log2": "o[i] = log2(abs(a) + 1e-6);",
+    "sin": "o[i] = sin(a * c);",
+    "cos": "o[i] = cos(a * c);",
 }
 
 
@@ -350,12 +364,161 @@ def make_lights():
         json.dump(meta, fh, indent=1, sort_keys=True)
 
 
+# llvmpipe's loop limit: gallivm stops a shader's loops after 65535
+# iterations in total per SIMD-8 vector (nested loops and divergent lanes
+# count together), a safety limit of the driver that GLSL does not have.  A
+# run whose shaders exceed it differs from the GLSL semantics the oracle and
+# the kernels follow, so no parity run may reach it; these probes record it.
+LOOP_PROBES = {
+    "single_100000": "uint c = 0u; for (int k = 0; k < 100000; k++) c++; o[i] = c;",
+    "nested_300x300": "uint c = 0u; for (int a = 0; a < 300; a++) for (int b = 0; b < 300; b++) c++; o[i] = c;",
+    "nested_40x40x40": ("uint c = 0u; for (int a = 0; a < 40; a++) for (int b = 0; b < 40; b++) "
+                        "for (int d = 0; d < 40; d++) c++; o[i] = c;"),
+    "single_60000": "uint c = 0u; for (int k = 0; k < 60000; k++) c++; o[i] = c;",
+}
+
+
+def make_loop_limit():
+    from oracle.glsl import gl as G
+    arrays = {}
+    for name, body in LOOP_PROBES.items():
+        src = ("#version 430 core\nlayout(local_size_x=64) in;\nlayout(std430,binding=1) buffer O{uint o[];};\n"
+               "void main(){ uint i = gl_GlobalInvocationID.x; " + body + " }\n")
+        prog = G.Program(src, "loop_" + name)
+        prog.use()
+        bo = G.Buffer(nbytes=64 * 4)
+        bo.bind_ssbo(1)
+        prog.dispatch(1)
+        arrays[name] = bo.read(np.uint32)
+        print("loop probe", name, arrays[name][:2], flush=True)
+    np.savez_compressed(os.path.join(HERE, "loop_limit.npz"), **arrays)
+
+
+# Hash-bucket overflow: the small scene with hash images 64 buckets wide (a
+# driver whose GL_MAX_TEXTURE_SIZE is 64), so the normal pass's and the
+# deposit's bucket-full paths run all the time: LRU eviction in the nBuffer
+# (normal.comp:160-176) and the lBuffer (shade.comp:170-181,254-256),
+# including several lanes of one SIMD-8 vector depositing into one full
+# bucket.  The camera moves every frame, so new voxels keep arriving.  The raw
+# hash images (slot order included) are stored.
+OVERFLOW_HASH_W = 64
+OVERFLOW_RUNS = {
+    "ovf_det": dict(W=128, H=128, frames=4, max_bounces=3, knobs=DET_KNOBS),
+    "ovf_spp": dict(W=96, H=96, frames=4, max_bounces=3,
+                    knobs=dict(DET_KNOBS, top_fraction=0.0, mid_fraction=0.0, min_samples=3, sample_cap_direct=16,
+                               sample_cap_indirect=8, firefly_floor=60.0)),
+    # the reference defaults with normal radius 1: with radius 6, lanes that
+    # compute several normals in one SIMD-8 vector run past llvmpipe's loop
+    # limit (make_loop_limit) under this much eviction
+    "ovf_default": dict(W=96, H=96, frames=5, max_bounces=3, knobs=dict(DEFAULT_KNOBS, normal_precision=1)),
+}
+
+
+def moving_ubo(W, H, f):
+    eye, d, fov = scenes.camera_small()
+    e = np.asarray(eye, np.float32) + np.float32(2.0 * f) * np.array([0.5, 0.0, 1.0], np.float32)
+    return ve.camera_ubo(e, d, fov, W / H)
+
+
+def make_overflow():
+    from oracle import oracle as O
+    depth, ops = scenes.scene_small(1)
+    words, info = ve.octree_build(ops, depth)
+    mats = scenes.palette(1)
+    meta = {"depth": depth, "words_sha256": words_digest(words, info.size), "sky_rgb8": SKY_RGB8,
+            "hash_width": OVERFLOW_HASH_W, "runs": {}}
+    arrays = {}
+    for name, run in OVERFLOW_RUNS.items():
+        W, H = run["W"], run["H"]
+        fc = O.FrameConfig(**run["knobs"])
+        pl = R.GLPipeline(words, depth, W, H, max_bounces=run["max_bounces"], materials=mats, sky_rgb8=SKY_RGB8,
+                          hash_width=OVERFLOW_HASH_W)
+        counters = []
+        for f in range(run["frames"]):
+            pl.set_camera(moving_ubo(W, H, f))
+            st = pl.frame(fc.as_dict(), f, frame_seed(f), f + 1)
+            counters.append([st["candidates"], st["scheduled"], st["bounces"], st["traced"]])
+            print(name, "frame", f, st, flush=True)
+        arrays[name + "_counters"] = np.array(counters, np.uint32)
+        arrays[name + "_out"] = pl.out.read()
+        g, p = pl.read_gbuffer()
+        arrays[name + "_gbuf"] = g
+        arrays[name + "_nbuf"] = pl.read_hash(pl.nbuf, 96)     # [bucket][slot * 3 + field]
+        arrays[name + "_lbuf"] = pl.read_hash(pl.lbuf, 384)    # [bucket][slot * 12 + field]
+        meta["runs"][name] = dict(W=W, H=H, frames=run["frames"], max_bounces=run["max_bounces"], knobs=run["knobs"])
+    np.savez_compressed(os.path.join(HERE, "frame_overflow.npz"), **arrays)
+    with open(os.path.join(HERE, "frame_overflow.json"), "w") as fh:
+        json.dump(meta, fh, indent=1, sort_keys=True)
+
+
+# Materials and accumulation paths the other runs do not reach:
+#   spec_det  - GGX VNDF sampling and the Blinn-Phong lobe (palette_spec);
+#   caps_det  - the reference demo's moving-camera caps 16 / 8
+#               (voxelengine.cpp:197-200) over 20 frames, so both channels are
+#               halved at their caps (shade.comp:188-194), plus a firefly floor
+#               low enough for the clamp to act (shade.comp:201-234);
+#   long_det  - 72 frames of one sample per visible voxel, so voxels pass the
+#               trusted threshold of 128 samples (resolve.comp:82,235); its
+#               final state is then resolved with every render type
+#               (resolve.comp:266-382: DEFAULT, STRUCTURE, ALBEDO, NORMAL,
+#               VOXELID, VARIANCE), timed-out pixels included (primary
+#               control checks 40 in the resolve).
+MATERIAL_RUNS = {
+    "spec_det": dict(W=128, H=128, frames=6, max_bounces=3, knobs=DET_KNOBS, palette="spec"),
+    "caps_det": dict(W=96, H=96, frames=20, max_bounces=3,
+                     knobs=dict(DET_KNOBS, sample_cap_direct=16, sample_cap_indirect=8, firefly_floor=40.0),
+                     palette="spec"),
+    "long_det": dict(W=64, H=64, frames=72, max_bounces=2, knobs=DET_KNOBS, palette="base"),
+}
+RESOLVE_TYPES = (0, 1, 2, 3, 4, 5)
+RESOLVE_CHECKS = 40
+
+
+def make_material():
+    from oracle import oracle as O
+    depth, ops = scenes.scene_small(1)
+    words, info = ve.octree_build(ops, depth)
+    meta = {"depth": depth, "words_sha256": words_digest(words, info.size), "sky_rgb8": SKY_RGB8,
+            "resolve_types": list(RESOLVE_TYPES), "resolve_checks": RESOLVE_CHECKS, "runs": {}}
+    arrays = {}
+    for name, run in MATERIAL_RUNS.items():
+        W, H = run["W"], run["H"]
+        mats = scenes.palette_spec(1) if run["palette"] == "spec" else scenes.palette(1)
+        fc = O.FrameConfig(**run["knobs"])
+        eye, d, fov = scenes.camera_small()
+        ubo = ve.camera_ubo(eye, d, fov, W / H)
+        pl = R.GLPipeline(words, depth, W, H, max_bounces=run["max_bounces"], materials=mats, sky_rgb8=SKY_RGB8)
+        pl.set_camera(ubo)
+        counters = []
+        for f in range(run["frames"]):
+            st = pl.frame(fc.as_dict(), f, frame_seed(f), f + 1)
+            counters.append([st["candidates"], st["scheduled"], st["bounces"], st["traced"]])
+            if f % 10 == 0 or f == run["frames"] - 1:
+                print(name, "frame", f, st, flush=True)
+        arrays[name + "_ubo"] = ubo
+        arrays[name + "_materials"] = mats.view(np.uint8)
+        arrays[name + "_counters"] = np.array(counters, np.uint32)
+        arrays[name + "_out"] = pl.out.read()
+        lv, lval = hash_map(pl.read_hash(pl.lbuf, 384).reshape(-1), 12, 32)
+        arrays[name + "_lbuf_vids"], arrays[name + "_lbuf_vals"] = lv, lval
+        if name == "long_det":
+            for rt in RESOLVE_TYPES:
+                kn = dict(fc.as_dict(), render_type=rt, primary_controlchecks=RESOLVE_CHECKS)
+                pl.resolve(kn)
+                arrays["%s_resolve%d" % (name, rt)] = pl.out.read()
+        meta["runs"][name] = dict(W=W, H=H, frames=run["frames"], max_bounces=run["max_bounces"], knobs=run["knobs"],
+                                  palette=run["palette"])
+    np.savez_compressed(os.path.join(HERE, "frame_material.npz"), **arrays)
+    with open(os.path.join(HERE, "frame_material.json"), "w") as fh:
+        json.dump(meta, fh, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
     if os.environ.get("LP_NUM_THREADS") != "1":
         sys.exit("run with LP_NUM_THREADS=1 (deterministic llvmpipe dispatch order)")
     if not R.shaders_available():
         sys.exit("reference shaders not found at %s" % R.SHADER_DIR)
-    what = sys.argv[1:] or ["fp", "cube", "primary", "frame", "lights", "sky"]
+    what = sys.argv[1:] or ["fp", "cube", "primary", "frame", "lights", "sky", "overflow", "material", "loops"]
     if "fp" in what:
         make_fp()
     if "cube" in what:
@@ -368,3 +531,9 @@ if __name__ == "__main__":
         make_lights()
     if "sky" in what:
         make_sky()
+    if "overflow" in what:
+        make_overflow()
+    if "material" in what:
+        make_material()
+    if "loops" in what:
+        make_loop_limit()

@@ -323,6 +323,25 @@ def palette(seed=1, count=8):
     return m
 
 
+def palette_spec(seed=1):
+    """palette(seed) with three specular entries: 3 = the reference demo's
+    metallic_m (albedo 0.8, roughness 0.15, specular 1, metallic 0.7:
+    metallic * (1 - roughness) > 0.5, so shade.comp samples GGX VNDF),
+    5 = a specular dielectric (specular 0.9, metallic 0.2, roughness 0.5) and
+    7 = a rough metal (specular 1, metallic 0.9, roughness 0.5): the last two
+    take the cosine sample with a Blinn-Phong lobe of weight specular *
+    metallic (internal.glsl:120-149)."""
+    m = palette(seed)
+    for i, col, rough, spec, metal in ((3, (0.8, 0.8, 0.8), 0.15, 1.0, 0.7), (5, None, 0.5, 0.9, 0.2),
+                                          (7, None, 0.5, 1.0, 0.9)):
+        if col is not None:
+            m[i]["color"] = [col[0], col[1], col[2], 0.0]
+        m[i]["roughness"] = rough
+        m[i]["specular"] = spec
+        m[i]["metallic"] = metal
+    return m
+
+
 def palette_lights(seed=1):
     """palette(seed) plus two emissive entries: 9 warm white (intensity 4),
     10 blue (intensity 6)."""

@@ -56,6 +56,7 @@ HBM_PEAK_GBS = 8000.0
 W, H = 1920, 1080
 BOUNCES = 2
 ORBIT_FRAMES = 64
+SKY = (140, 180, 230)
 
 
 def knobs():
@@ -93,7 +94,7 @@ def cpu_baseline(words, depth, ubos, fc_kw, budget_s=20.0):
     from voxelengine_amd import scenes
     of = O.Frame(W, H, 16384, BOUNCES + 1)
     of.set_materials(scenes.palette(1))
-    of.set_sky_rgb8((140, 180, 230))
+    of.set_sky_rgb8(SKY)
     fc = O.FrameConfig(**fc_kw)
     rays, frames = 0, 0
     t0 = time.perf_counter()
@@ -242,7 +243,7 @@ def main():
     r = ve.Renderer(W, H, hash_width=16384, max_bounces=BOUNCES + 1)
     scene = r.upload(words, depth, info.size)
     r.set_materials(scenes.palette(1))
-    r.set_sky_constant((140, 180, 230))
+    r.set_sky_constant(SKY)
     fc = ve.FrameConfig(**knobs())
     ubos = []
     for f in range(ORBIT_FRAMES):

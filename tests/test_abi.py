@@ -25,3 +25,14 @@ def test_errors_are_codes_not_crashes(ve):
     info = ve.OctreeInfo()
     assert L.ve_octree_build(None, 0, 0, 0, C.byref(out), C.byref(info)) == ve.VE_EINVAL
     assert L.ve_octree_build(None, 0, 17, 0, C.byref(out), C.byref(info)) == ve.VE_EINVAL
+
+
+def test_headline_knobs_are_the_benchmarks():
+    """The GPU parity tests of the measured workload (test_gpu_headline.py)
+    run the benchmark's own configuration."""
+    import bench
+    from frame_common import HEADLINE_KNOBS, HEADLINE_SKY
+    import test_gpu_headline as t
+    assert bench.knobs() == HEADLINE_KNOBS
+    assert (bench.W, bench.H, bench.BOUNCES + 1, bench.ORBIT_FRAMES) == (t.W, t.H, t.MAX_BOUNCES, t.ORBIT)
+    assert tuple(bench.SKY) == HEADLINE_SKY

@@ -100,12 +100,14 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.em_reg, EMISSIVE_CAPACITY);
     a(&b.hit_bits, SHADE_BUDGET_MAX / 32);
     a(&b.deferred, SHADE_BUDGET_MAX);
-    a(&b.bin_over, SHADE_BUDGET_MAX);
-    a(&b.bin_over_sorted, SHADE_BUDGET_MAX);
-    a(&b.binned, (size_t)hash_width * BIN_CAP);
-    a(&b.bin_count, hash_width);
-    a(&b.bin_off, hash_width);
-    a(&b.bin_over_n, 2);
+    a(&b.ray_li[0], SHADE_BUDGET_MAX);
+    a(&b.ray_li[1], SHADE_BUDGET_MAX);
+    a(&b.dep_death, SHADE_BUDGET_MAX);
+    a(&b.dep_wpos, SHADE_BUDGET_MAX);
+    a(&b.dep_cnt, hash_width);
+    a(&b.dep_off, (size_t)hash_width + 1);
+    a(&b.dep_ev, SHADE_BUDGET_MAX);
+    a(&b.dep_spill, SHADE_BUDGET_MAX);
     a(&b.materials, MATERIAL_SLOTS);
     a(&b.node_stats, 4);
     a(&b.ray_totals, 4);
@@ -119,6 +121,10 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.vec_chunk, (groups_max * 8 + 255) / 256);
     a(&b.rank_chunk, (std::min<size_t>(P, CANDIDATE_CAPACITY) + 255) / 256);
     a(&b.nsamp, std::min<size_t>(P, CANDIDATE_CAPACITY));
+    a(&b.ord_cnt, hash_width);
+    a(&b.ord_off, (size_t)hash_width + 1);
+    a(&b.ord_ev, P);
+    a(&b.ord_spill, P);
 }
 
 // RankStats thresholds start at (+inf, -inf): every voxel in the mid tier on
@@ -499,12 +505,30 @@ int ve_set_sky(ve_ctx *c, const uint8_t *const faces[6], int edge)
 
 namespace {
 
+// Apply the deposits that pass-by-pass shade dispatches recorded, with the
+// parameters of their frame.
+int flush_deposits(ve_ctx *c)
+{
+    if (c->dep_iters == 0u) return VE_OK;
+    const int rc = launch_deposits(c->stream, c->dep_params, c->b, c->dep_iters);
+    c->dep_iters = 0u;
+    if (rc) return ctx_error(c, rc, "deposits: device library call failed");
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return ctx_error(c, VE_EDEVICE, "deposit launch: %s", hipGetErrorString(e));
+    return VE_OK;
+}
+
 // Enqueue one pass; every launch goes on the context's stream.  The primary
 // and normal passes run on `side_stream` when given (an overlapped frame's
 // auxiliary streams).
 int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, hipStream_t side_stream = nullptr)
 {
     int rc = VE_OK;
+    // passes that read or clear what a frame's deposits change, or the
+    // records themselves, come after the pending deposits
+    if (pass == VE_PASS_RESET || pass == VE_PASS_SCHEDULE || pass == VE_PASS_RANK || pass == VE_PASS_THRESHOLD ||
+        pass == VE_PASS_EMIT || pass == VE_PASS_RESOLVE)
+        if ((rc = flush_deposits(c))) return rc;
     switch (pass) {
     case VE_PASS_PRIMARY:
         launch_primary(side_stream ? side_stream : c->stream, s->words, p, c->b, c->node_stats);
@@ -526,6 +550,10 @@ int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, hipS
         if (c->bounce_iter >= 8u) return ctx_error(c, VE_ESTATE, "shade after 8 bounce iterations (emit first)");
         launch_shade(c->stream, s->words, p, c->b, c->bounce_iter);
         c->bounce_iter++;
+        if (!p.count_primary_in_reset) {   // a lone dispatch: its deposits wait for the frame's others
+            c->dep_iters = std::max(c->dep_iters, c->bounce_iter);
+            c->dep_params = p;
+        }
         break;
     case VE_PASS_RESOLVE: launch_resolve(c->stream, p, c->b); break;
     case VE_PASS_DISPLAY: launch_display(c->stream, p, c->b); break;
@@ -636,10 +664,13 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &p)
     if (iters > 1) {
         if (ev) pass_timer = PassTimer{ev[2 * (VE_PASS_TRACE * MAX_ITERS + 1)], ev[2 * (VE_PASS_TRACE * MAX_ITERS + 1) + 1]};
         launch_bounce(c->stream, s->words, p, c->b, 1u, iters, c->node_stats);
-        if (ev) pass_timer = PassTimer{ev[2 * (VE_PASS_SHADE * MAX_ITERS + 1)], ev[2 * (VE_PASS_SHADE * MAX_ITERS + 1) + 1]};
-        rc = launch_deferred(c->stream, p, c->b, 1u, iters);
+    }
+    if (iters > 0) {   // the frame's deposits, timed with the last shade pass
+        const uint32_t slot = 2 * (VE_PASS_SHADE * MAX_ITERS + (iters > 1 ? 1 : 0));
+        if (ev) pass_timer = PassTimer{iters > 1 ? ev[slot] : nullptr, ev[slot + 1]};
+        rc = launch_deposits(c->stream, p, c->b, iters);
         pass_timer = PassTimer{};
-        if (rc) return ctx_error(c, rc, "bounce deposits: device library call failed");
+        if (rc) return ctx_error(c, rc, "deposits: device library call failed");
         HIP_OK(c, hipGetLastError());
     }
     if ((rc = timed_pass(c, s, VE_PASS_RESOLVE, p, 0, ev))) return rc;
@@ -699,6 +730,7 @@ int ve_read(ve_ctx *c, ve_buffer_id id, void *dst, size_t bytes)
     if (!v.ptr) return ctx_error(c, VE_ENOTFOUND, "unknown buffer id %d", (int)id);
     if (!dst || bytes > v.bytes) return ctx_error(c, VE_EINVAL, "read of %zu bytes from a %zu-byte buffer", bytes, v.bytes);
     HIP_OK(c, hipSetDevice(c->device));
+    if (int rc = flush_deposits(c)) return rc;
     if (is_hash_buffer(id)) {   // returned in the reference's slot-interleaved layout
         std::vector<uint32_t> tmp(v.bytes / 4), ref(v.bytes / 4);
         HIP_OK(c, hipMemcpyAsync(tmp.data(), v.ptr, v.bytes, hipMemcpyDeviceToHost, c->stream));
@@ -720,6 +752,7 @@ int ve_write(ve_ctx *c, ve_buffer_id id, const void *src, size_t bytes)
     if (!src || bytes > v.bytes) return ctx_error(c, VE_EINVAL, "write of %zu bytes to a %zu-byte buffer", bytes, v.bytes);
     c->aux_fence = true;
     HIP_OK(c, hipSetDevice(c->device));
+    if (int rc = flush_deposits(c)) return rc;
     if (is_hash_buffer(id)) {   // accepted in the reference layout; whole image only
         if (bytes != v.bytes) return ctx_error(c, VE_EINVAL, "hash images are written whole (%zu bytes)", v.bytes);
         std::vector<uint32_t> dev(v.bytes / 4);
@@ -945,6 +978,7 @@ int ctx_reset(ve_ctx *c)
     HIP_OK(c, hipMemcpyAsync(c->b.rank_stats, INITIAL_THRESHOLDS, sizeof INITIAL_THRESHOLDS, hipMemcpyHostToDevice,
                              c->stream));
     c->bounce_iter = 0;
+    c->dep_iters = 0;
     c->frames_recorded = 0;
     c->timing = false;
     c->gsel = 0;

@@ -50,6 +50,11 @@ struct ve_ctx {
     bool node_stats = false;
     bool timing = false;
     uint32_t bounce_iter = 0;           // next trace/shade iteration of per-pass dispatch
+    // Deposits recorded by pass-by-pass shade dispatches and not yet applied
+    // (the reference's chunk order needs all of a frame's deposits): the
+    // iterations recorded and the parameters of the frame that recorded them.
+    uint32_t dep_iters = 0;
+    ve::FrameParams dep_params{};
     // timing: one event pair per pass and bounce iteration per recorded frame
     std::vector<std::array<hipEvent_t, 2 * VE_PASS_COUNT * MAX_ITERS>> frame_events;
     size_t frames_recorded = 0;

@@ -39,17 +39,18 @@ constexpr uint32_t SCHED_SLOTS = 128;          // ordering table: voxels per cla
 // order is no longer the reference's and VE_CNT_OVERFLOW is set).
 constexpr uint32_t SCHED_OVER_CAP = 4096, SCHED_ROW_OVER = 2048;
 constexpr uint32_t MATERIAL_SLOTS = 127;       // shade.comp:90
-constexpr uint32_t DEFERRED_BLOCKS = 256;      // deferred-deposit overflow grid (entries strided over it)
-// A deposit of the fused bounce pass, applied after it: `iter` is the bounce
-// iteration (bits 0-7) and the deposit's index among its lBuffer bucket's
-// held-back deposits (bits 8-31).
+constexpr uint32_t DEFERRED_BLOCKS = 256;      // deposit-record passes' grid (entries strided over it)
+// One deposit of the frame (depositSampleDual, shade.comp:162-257), held
+// until the frame's bounces are done and then replayed per lBuffer bucket in
+// the reference's order.  `meta`: the ray's shade-list index (bits 0-17), the
+// bounce iteration (bits 18-21) and the deposit's call site in shade.comp
+// (bits 22-23: DEP_MISS, DEP_LAST, DEP_RR).
 struct Deferred {
-    uint32_t vid, iter;
+    uint32_t vid, meta;
     float dl[3], il[3];
 };
-// Held-back deposits kept in place per lBuffer bucket; a bucket's further
-// ones go through an overflow list grouped per bucket after the pass.
-constexpr uint32_t BIN_CAP = 32;
+constexpr uint32_t DEP_MISS = 0, DEP_LAST = 1, DEP_RR = 2;   // shade.comp:558-564, 840-845, 877-889
+constexpr uint32_t RAY_CHUNK = 49152;          // renderer.hpp:94: rays per emit chunk (renderer.cpp:476-606)
 constexpr uint32_t RANK_BLOCKS = 1024;         // rank pass grid (candidates strided over it)
 // The rank blocks flush their counts, min/max and histogram into RANK_COPIES
 // copies (block b into copy b % RANK_COPIES), so no address takes more than

@@ -25,17 +25,6 @@ inline void launch(K kernel, dim3 grid, dim3 block, uint32_t shared_bytes, hipSt
     pass_timer.start = nullptr;
 }
 
-// The deposits held back by the fused bounce pass, per lBuffer bucket: its
-// first BIN_CAP in `binned` (row h), the rest listed in `over` and then
-// grouped per bucket into `over_sorted` at `over_off[h]`.  A record's `iter`
-// holds the bounce iteration in bits 0-7 and the bucket-local index above.
-struct BinLists {
-    uint32_t hash_width;
-    uint32_t *count;
-    Deferred *binned, *over, *over_sorted;
-    uint32_t *over_off, *over_n;
-};
-
 // Device buffers of one context (all owned by the context).
 struct DeviceBuffers {
     uint4 *gbuf = nullptr;          // P x (checks, material, 0, vid): the current set
@@ -69,14 +58,15 @@ struct DeviceBuffers {
     uint32_t *em_bits = nullptr;         // SHADE_BUDGET_MAX / 32: first-shade rays that may register a light
     uint32_t *em_reg = nullptr;          // EMISSIVE_CAPACITY: ray index of each registration this frame
     uint32_t *hit_bits = nullptr;        // SHADE_BUDGET_MAX / 32: bounce-0 rays whose hit record emit filled in
-    // deposits held back by the fused bounce pass (see BinLists)
-    Deferred *binned = nullptr;          // hash_width x BIN_CAP
+    // the frame's deposits, replayed per lBuffer bucket in the reference's order
     Deferred *deferred = nullptr;        // SHADE_BUDGET_MAX: in append order
-    Deferred *bin_over = nullptr;        // SHADE_BUDGET_MAX: past BIN_CAP in a bucket, in placing order
-    Deferred *bin_over_sorted = nullptr; // SHADE_BUDGET_MAX: the overflow grouped by lBuffer bucket
-    uint32_t *bin_count = nullptr;       // hash_width: deposits per bucket (zero between frames)
-    uint32_t *bin_off = nullptr;         // hash_width: overflow region of a bucket past BIN_CAP
-    uint32_t *bin_over_n = nullptr;      // 2: overflow entries, overflow entries placed (zero between frames)
+    uint32_t *ray_li[2] = {nullptr, nullptr};  // SHADE_BUDGET_MAX each: shade-list index of each ray of rays / rays_next
+    uint32_t *dep_death = nullptr;       // SHADE_BUDGET_MAX: bounce iteration at which each list entry's ray deposited
+    uint32_t *dep_wpos = nullptr;        // SHADE_BUDGET_MAX: its position in the reference's ray window then
+    uint32_t *dep_cnt = nullptr;         // hash_width: deposits per lBuffer bucket (zero between frames)
+    uint32_t *dep_off = nullptr;         // hash_width + 1
+    uint2 *dep_ev = nullptr;             // SHADE_BUDGET_MAX: {order key, record} grouped per bucket
+    uint64_t *dep_spill = nullptr;       // SHADE_BUDGET_MAX: sort keys of buckets too large to sort in LDS
     Material *materials = nullptr;       // MATERIAL_SLOTS
     unsigned long long *node_stats = nullptr;  // 4: primary reads, trace reads, primary steps, trace steps
     unsigned long long *ray_totals = nullptr;  // 4: primary, traced, shadow, traced with a reused hit (cumulative)
@@ -91,6 +81,11 @@ struct DeviceBuffers {
     uint32_t *vec_chunk = nullptr;       // per 256 SIMD-8 vectors: sum of vec_cnt (schedule)
     uint32_t *rank_chunk = nullptr;      // per 256 candidates: shade-list entries they append (rank)
     int32_t *nsamp = nullptr;            // per candidate: samples
+    // ordered per-bucket replay of the normal pass (events of one frame)
+    uint32_t *ord_cnt = nullptr;         // hash_width: events per nBuffer bucket (zero between passes)
+    uint32_t *ord_off = nullptr;         // hash_width + 1: each bucket's first event
+    uint4 *ord_ev = nullptr;             // max_w * max_h: {execution rank, pixel, voxel id, packed position}
+    uint64_t *ord_spill = nullptr;       // max_w * max_h: sort keys of buckets too large to sort in LDS
 };
 
 void launch_primary(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, bool stats);
@@ -108,7 +103,10 @@ void launch_shade(hipStream_t s, const uint32_t *words, const FrameParams &p, De
 // held-back deposits in iteration order.
 void launch_bounce(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t first,
                    uint32_t iters, bool stats);
-int launch_deferred(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_t first, uint32_t iters);
+// The frame's deposits (records of every shade pass and the fused bounce
+// pass, bounce iterations 0..iters-1), replayed into the lBuffer in the
+// reference's order.
+int launch_deposits(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_t iters);
 void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 void launch_display(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 void launch_scatter_words(hipStream_t s, uint32_t *words, const uint2 *pairs, uint32_t n);

@@ -237,61 +237,255 @@ __device__ __forceinline__ uint32_t pack_normal(f3 v)   // normal.comp:90-95, ro
     return ((uint32_t)x << 22) | ((uint32_t)y << 12) | ((uint32_t)z << 2);
 }
 
-__global__ __launch_bounds__(256) void k_normal(const uint32_t *__restrict__ words, FrameParams p,
-                                                const uint4 *__restrict__ gbuf, const uint32_t *__restrict__ gpos,
-                                                uint32_t *__restrict__ nbuf)
+// ---------------------------------------------------------------------------
+// Ordered per-bucket replay
+// ---------------------------------------------------------------------------
+// The nBuffer and lBuffer are 32-slot hash buckets whose claims and
+// least-recently-used evictions the reference makes with atomics from many
+// invocations at once.  When a bucket fills up, which slot a voxel takes or
+// loses depends on the order of those atomics.  Its order on the GL driver
+// the parity is defined against (single-threaded llvmpipe: work groups in
+// dispatch order, the invocations of a work group in SIMD-8 vectors, each
+// vector's lanes in lockstep) is replayed exactly: the work for each bucket is
+// listed as events with their execution order, every bucket's events are
+// sorted by it, and one wave per bucket replays them against the bucket,
+// lanes holding its slots.
+
+// Ascending sort of n 64-bit keys by one wave: the bitonic network of the
+// next power of two in the form whose every compare-exchange puts the
+// smaller key at the lower index (the first step of each merge pairs i with
+// its mirror in the block), so the positions past n act as +infinity and
+// every exchange that touches them is skipped.  GLOBAL: the keys are in
+// device memory (loads and stores bypass the non-coherent L1 between steps).
+template <bool GLOBAL>
+__device__ __forceinline__ uint64_t key_ld(const uint64_t *k, uint32_t i)
 {
-    __shared__ uint32_t path_cache[VE_MAX_DEPTH * 256];
-    uint32_t x, y;
+    if (GLOBAL) return __hip_atomic_load(k + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return k[i];
+}
+template <bool GLOBAL>
+__device__ __forceinline__ void key_st(uint64_t *k, uint32_t i, uint64_t v)
+{
+    if (GLOBAL) __hip_atomic_store(k + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else k[i] = v;
+}
+template <bool GLOBAL>
+__device__ void wave_sort_keys(uint64_t *keys, uint32_t n)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t np = 1;
+    while (np < n) np <<= 1;
+    for (uint32_t k = 2; k <= np; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = lane; i < np / 2u; i += 64u) {
+                const uint32_t a = (i / j) * 2u * j + (i % j);
+                const uint32_t b = j == (k >> 1) ? a ^ (k - 1u) : a + j;   // mirror, then half-cleaners
+                if (b < n) {
+                    const uint64_t x = key_ld<GLOBAL>(keys, a), y = key_ld<GLOBAL>(keys, b);
+                    if (y < x) {
+                        key_st<GLOBAL>(keys, a, y);
+                        key_st<GLOBAL>(keys, b, x);
+                    }
+                }
+            }
+            if (GLOBAL) __atomic_thread_fence(__ATOMIC_SEQ_CST);
+            __syncthreads();
+        }
+    }
+}
+
+// Execution rank of pixel (x, y) in an 8x8-group dispatch: work groups in
+// dispatch order (x fastest), invocations in local order.
+__device__ __forceinline__ uint32_t exec_rank(uint32_t x, uint32_t y, uint32_t W)
+{
+    const uint32_t gw = (W + 7u) / 8u;
+    return (((y >> 3) * gw + (x >> 3)) << 6) | ((y & 7u) << 3) | (x & 7u);
+}
+
+// Buckets sorted and replayed from LDS up to this many events; larger ones
+// (a voxel covering many work groups of a close view) from device memory.
+constexpr uint32_t ORDER_LDS_EVENTS = 1024;
+
+// ---------------------------------------------------------------------------
+// normal.comp
+// ---------------------------------------------------------------------------
+// Replay of normal.comp:124-177 per nBuffer bucket.  Per pixel: claim the
+// first slot that is free or holds the voxel (computing the normal if the
+// slot has none) and stamp the time; when every slot holds another voxel,
+// take the least recently stamped one.  Within one SIMD-8 vector the claim
+// loops of the lanes give the same slots as running the lanes one after
+// another, but the lanes that find no slot all scan the time stamps before
+// any of them writes: they pick the same slot, and the last of them owns it.
+//
+// 1. k_nrm_events (count, then place): one wave per 8x8 work group lists its
+//    hit pixels per bucket as events {execution rank, pixel, voxel id, packed
+//    position}.  A pixel whose closest earlier pixel of the group in the
+//    same bucket carries the same voxel (and position) is left out: it finds
+//    the slot that pixel just claimed, or evicts with it, to the same effect.
+// 2. k_order_scan: per-bucket event counts -> offsets.
+// 3. k_nrm_replay: one wave per bucket sorts its events and replays them;
+//    then the slots that took a new normal compute it, in parallel.
+template <bool PLACE>
+__global__ __launch_bounds__(256) void k_nrm_events(FrameParams p, const uint4 *__restrict__ gbuf,
+                                                    const uint32_t *__restrict__ gpos, uint32_t *__restrict__ cnt,
+                                                    const uint32_t *__restrict__ off, uint4 *__restrict__ ev)
+{
+    uint32_t x = 0, y = 0;
     const bool valid = tile_pixel(p, x, y);
+    const uint32_t lane = threadIdx.x & 63u;
     const size_t pix = valid ? (size_t)y * p.W + x : 0;
     const uint32_t vid = valid ? gbuf[pix].w : 0u;
-    // Pixels of one voxel carry the same id and the same packed position, and
-    // their claims, normal and time stamp are identical: one lane per distinct
-    // id in the wave does the work (the lowest, as the first claimant would).
-    const uint32_t lane = threadIdx.x & 63u;
-    bool lead = vid != 0u;
-    uint64_t pending = __ballot(lead);
+    const bool hit = vid != 0u;
+    const uint32_t pk = hit ? gpos[pix] : 0u;
+    const uint32_t h = hit ? vid % (p.hash_width - 1u) + 1u : 0u;
+    uint64_t pending = __ballot(hit);
     while (pending) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
-        const uint32_t lv = __shfl(vid, leader, 64);
-        const uint64_t same = __ballot(vid == lv);
+        const uint32_t hb = (uint32_t)__builtin_amdgcn_readlane((int)h, leader);
+        const bool mine = hit && h == hb;
+        const uint64_t same = __ballot(mine);
         pending &= ~same;
-        if (vid == lv && (int)lane != leader) lead = false;
+        const uint64_t before = same & ((1ull << lane) - 1ull);
+        const int j = before ? 63 - __clzll((long long)before) : (int)lane;
+        const uint32_t pv = __shfl(vid, j, 64), pp = __shfl(pk, j, 64);
+        const bool keep = mine && !(before && pv == vid && pp == pk);
+        const uint64_t kept = __ballot(keep);
+        uint32_t b = 0;
+        if ((int)lane == leader) b = atomicAdd(&cnt[hb], (uint32_t)__popcll(kept));
+        b = __shfl(b, leader, 64);
+        if (PLACE && keep)
+            ev[off[hb] + b + (uint32_t)__popcll(kept & ((1ull << lane) - 1ull))] =
+                make_uint4(exec_rank(x, y, p.W), (uint32_t)pix, vid, pk);
     }
-    if (!lead) return;
-    uint32_t *cache = path_cache + threadIdx.x;
-    const uint32_t len = 1u << p.depth;
-    const uint32_t pk = gpos[pix], bits = p.depth - 1u, mask = (1u << bits) - 1u;
-    const uint32_t hx = (pk & mask) * 2u, hy = ((pk >> bits) & mask) * 2u, hz = ((pk >> (2u * bits)) & mask) * 2u;
-    const uint32_t h = vid % (p.hash_width - 1u) + 1u;
+}
+
+// Exclusive scan of n per-bucket counts into off[0..n] (one block), clearing
+// the counts for the placing pass, which uses them as cursors.
+__global__ __launch_bounds__(1024) void k_order_scan(uint32_t *__restrict__ cnt, uint32_t *__restrict__ off, uint32_t n)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (n + 1023u) / 1024u, lo = min(t * per, n), hi = min(lo + per, n);
+    uint32_t sum = 0;
+    for (uint32_t i = lo; i < hi; i++) sum += cnt[i];
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024u; d <<= 1) {
+        const uint32_t v = t >= d ? part[t - d] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (uint32_t i = lo; i < hi; i++) {
+        const uint32_t c = cnt[i];
+        off[i] = run;
+        run += c;
+        cnt[i] = 0u;
+    }
+    if (t == 1023u) off[n] = part[1023];
+}
+
+// Sorts bucket b's events by execution rank into `order` (indices into ev,
+// sorted; LDS up to ORDER_LDS_EVENTS, else the bucket's range of `spill`).
+__device__ __forceinline__ const uint64_t *sort_bucket_events(const uint4 *__restrict__ ev, uint32_t base, uint32_t n,
+                                                              uint64_t *lds_keys, uint64_t *__restrict__ spill)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    uint64_t *keys = n <= ORDER_LDS_EVENTS ? lds_keys : spill + base;
+    for (uint32_t i = lane; i < n; i += 64u) {
+        const uint64_t k = ((uint64_t)ev[base + i].x << 32) | i;
+        if (n <= ORDER_LDS_EVENTS) keys[i] = k;
+        else __hip_atomic_store(keys + i, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (n > ORDER_LDS_EVENTS) __atomic_thread_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();
+    if (n <= ORDER_LDS_EVENTS) wave_sort_keys<false>(keys, n);
+    else wave_sort_keys<true>(keys, n);
+    return keys;
+}
+
+__global__ __launch_bounds__(64) void k_nrm_replay(const uint32_t *__restrict__ words, FrameParams p,
+                                                   uint32_t *__restrict__ nbuf, const uint32_t *__restrict__ off,
+                                                   const uint4 *__restrict__ ev, uint64_t *__restrict__ spill,
+                                                   uint32_t *__restrict__ cnt)
+{
+    __shared__ uint64_t lds_keys[ORDER_LDS_EVENTS];
+    __shared__ uint4 lds_ev[ORDER_LDS_EVENTS];
+    __shared__ uint32_t path_cache[VE_MAX_DEPTH * 64];
+    const uint32_t h = blockIdx.x;
+    const uint32_t base = off[h], n = off[h + 1] - base;
+    if (n == 0u) return;
+    const uint32_t lane = threadIdx.x;
+    if (lane == 0u) cnt[h] = 0u;   // the next pass's count starts from zero
+    const bool in_lds = n <= ORDER_LDS_EVENTS;
+    const uint64_t *keys = sort_bucket_events(ev, base, n, lds_keys, spill);
+    if (in_lds) {   // the events in sorted order, gathered at once
+        for (uint32_t i = lane; i < n; i += 64u) lds_ev[i] = ev[base + (uint32_t)keys[i]];
+        __syncthreads();
+    }
     uint32_t *bucket = nbuf + (size_t)h * NB_WORDS;
-    bool claimed;
-    const uint32_t s = claim_slot(bucket, NBUFFER_SLOTS, vid, claimed);
-    if (s < NBUFFER_SLOTS) {
-        // The claimant computes the normal; a later claimant that finds the
-        // field still empty would compute the same value (normal.comp:151-156).
-        if (claimed || __atomic_load_n(&bucket[NB_NRM + s], __ATOMIC_RELAXED) == 0u) {
-            const f3 nrm = refined_normal(words, p.jump, p.jump_level, p.n_words, p.depth, hx, hy, hz, len, p.normal_precision,
-                                          cache, 256u);
-            atomicCAS(&bucket[NB_NRM + s], 0u, pack_normal(nrm));
+    const bool slot_lane = lane < NBUFFER_SLOTS;
+    uint32_t owner = slot_lane ? bucket[lane] : NONE;
+    uint32_t tstamp = slot_lane ? bucket[NB_TIME + lane] : NONE;
+    uint32_t nrm = slot_lane ? bucket[NB_NRM + lane] : 0u;
+    bool has_src = false;   // the slot takes the normal at packed position `src`
+    uint32_t src = 0u;
+    uint32_t cur_vec = NONE, ev_vid = 0u, ev_pos = 0u, last_vid = 0u, last_pos = 0u;
+    auto flush = [&]() {   // the lockstep eviction of one vector's slotless lanes
+        if (ev_vid == 0u) return;
+        uint32_t t = slot_lane ? tstamp : NONE;   // first slot with the smallest stamp
+        uint32_t best = t;
+        for (int o = 32; o > 0; o >>= 1) best = min(best, (uint32_t)__shfl_xor((int)best, o, 64));
+        const uint64_t at = __ballot(slot_lane && t == best);
+        const uint32_t s = (uint32_t)__ffsll((unsigned long long)at) - 1u;
+        if (lane == s) {
+            owner = ev_vid;
+            has_src = true;
+            src = ev_pos;
+            tstamp = p.time;
         }
-        atomicExch(&bucket[NB_TIME + s], p.time);
-        return;
+        ev_vid = 0u;
+    };
+    for (uint32_t k = 0; k < n; k++) {
+        const uint4 e = in_lds ? lds_ev[k]
+                               : ev[base + (uint32_t)__hip_atomic_load(keys + k, __ATOMIC_RELAXED,
+                                                                      __HIP_MEMORY_SCOPE_AGENT)];
+        const uint32_t vec = e.x >> 3;
+        if (vec != cur_vec) {
+            flush();
+            cur_vec = vec;
+        }
+        if (e.z == last_vid && e.w == last_pos) continue;   // finds what the last event left: no change
+        last_vid = e.z;
+        last_pos = e.w;
+        const uint64_t m = __ballot(slot_lane && (owner == e.z || owner == 0u));
+        if (m) {
+            if (lane == (uint32_t)__ffsll((unsigned long long)m) - 1u) {
+                if (owner == 0u) owner = e.z;
+                if (nrm == 0u && !has_src) {   // normal.comp:151-156: computed only into an empty field
+                    has_src = true;
+                    src = e.w;
+                }
+                tstamp = p.time;
+            }
+        } else {   // lockstep: the vector's last slotless lane wins the evicted slot
+            ev_vid = e.z;
+            ev_pos = e.w;
+        }
     }
-    // bucket full: take the least recently used slot (normal.comp:162-176)
-    uint32_t oldest = 0xFFFFFFFFu, oslot = 0;
-    for (uint32_t q = 0; q < NBUFFER_SLOTS; q += 4) {
-        const uint4 t4 = *reinterpret_cast<const uint4 *>(bucket + NB_TIME + q);
-        const uint32_t t[4] = {t4.x, t4.y, t4.z, t4.w};
-        for (uint32_t k = 0; k < 4; k++)
-            if (t[k] < oldest) { oldest = t[k]; oslot = q + k; }
+    flush();
+    if (has_src) {   // the new normals, one lane per slot
+        const uint32_t len = 1u << p.depth, bits = p.depth - 1u, mask = (1u << bits) - 1u;
+        const uint32_t hx = (src & mask) * 2u, hy = ((src >> bits) & mask) * 2u, hz = ((src >> (2u * bits)) & mask) * 2u;
+        nrm = pack_normal(refined_normal(words, p.jump, p.jump_level, p.n_words, p.depth, hx, hy, hz, len,
+                                         p.normal_precision, path_cache + lane, 64u));
     }
-    atomicExch(&bucket[oslot], vid);
-    const f3 nrm = refined_normal(words, p.jump, p.jump_level, p.n_words, p.depth, hx, hy, hz, len, p.normal_precision,
-                                          cache, 256u);
-    atomicExch(&bucket[NB_NRM + oslot], pack_normal(nrm));
-    atomicExch(&bucket[NB_TIME + oslot], p.time);
+    if (slot_lane) {
+        bucket[lane] = owner;
+        bucket[NB_NRM + lane] = nrm;
+        bucket[NB_TIME + lane] = tstamp;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -889,7 +1083,8 @@ __global__ __launch_bounds__(256) void k_emit(FrameParams p, const uint2 *__rest
                                               const uint4 *__restrict__ gbuf, const uint2 *__restrict__ ghit,
                                               const Material *__restrict__ mats, const uint32_t *__restrict__ em_claim,
                                               uint32_t *__restrict__ em_bits, uint32_t *__restrict__ hit_bits,
-                                              unsigned long long *__restrict__ totals)
+                                              unsigned long long *__restrict__ totals, uint32_t *__restrict__ ray_li,
+                                              uint32_t *__restrict__ death)
 {
     __shared__ uint32_t block_reused;
     if (threadIdx.x == 0) block_reused = 0u;
@@ -931,6 +1126,8 @@ __global__ __launch_bounds__(256) void k_emit(FrameParams p, const uint2 *__rest
             }
         }
         rays[t] = r;
+        ray_li[t] = t;      // shade-list index, carried through the survivor compaction
+        death[t] = NONE;    // the iteration of its deposit, recorded after the bounces
     }
     const uint64_t m = __ballot(reused);
     const uint32_t lane = threadIdx.x & 63u;
@@ -1027,96 +1224,8 @@ __device__ __forceinline__ uint4 pack_rgb_l2(f3 c)   // shade.comp:154-160
     return o;
 }
 
-__device__ __forceinline__ void firefly(const uint32_t *sl, uint32_t cf, uint4 &v, float k, float floor_)
-{
-    const uint32_t c = __atomic_load_n(&sl[cf], __ATOMIC_RELAXED);
-    if (c <= 4u) return;
-    const uint32_t rs = __atomic_load_n(&sl[cf + 1], __ATOMIC_RELAXED);
-    const uint32_t gs = __atomic_load_n(&sl[cf + 2], __ATOMIC_RELAXED);
-    const uint32_t bs = __atomic_load_n(&sl[cf + 3], __ATOMIC_RELAXED);
-    const float meanL = (float)(rs + gs + bs) / (float)(3u * c);
-    const float sampleL = (float)(v.x + v.y + v.z) / 3.0f;
-    const float thresh = gmax(meanL * k, floor_);
-    if (sampleL > thresh) {
-        const float scale = thresh / sampleL;
-        v.x = (uint32_t)((float)v.x * scale);
-        v.y = (uint32_t)((float)v.y * scale);
-        v.z = (uint32_t)((float)v.z * scale);
-        const uint32_t L = (v.x + v.y + v.z) / 3u;
-        v.w = L * L;
-    }
-}
-
-__device__ void reset_lslot(uint32_t *bucket, uint32_t s, uint32_t vid, uint4 d, uint4 e, uint32_t time)
-{
-    uint32_t *pl = bucket + LB_PAY + s * LB_PAYW;
-    atomicExch(&bucket[s], vid);
-    atomicExch(&pl[0], 1u); atomicExch(&pl[1], d.x); atomicExch(&pl[2], d.y); atomicExch(&pl[3], d.z);
-    atomicExch(&pl[4], d.w);
-    atomicExch(&pl[5], 1u); atomicExch(&pl[6], e.x); atomicExch(&pl[7], e.y); atomicExch(&pl[8], e.z);
-    atomicExch(&pl[9], e.w);
-    atomicExch(&bucket[LB_TIME + s], time);
-}
-
-// depositSampleDual (shade.comp:162-257) = AccumPass: integer fixed-point sums
-// per voxel id in the lBuffer hash.
-// One deposit into the lBuffer bucket of `vid`; `bucket` is the bucket in
-// device memory or a copy of it in LDS.
-__device__ void deposit_into(const FrameParams &p, uint32_t *bucket, uint32_t vid, f3 dir_rgb, f3 ind_rgb)
-{
-    uint4 d = pack_rgb_l2(dir_rgb);
-    uint4 e = pack_rgb_l2(ind_rgb);
-    bool claimed;
-    const uint32_t s = claim_slot(bucket, LBUFFER_SLOTS, vid, claimed);
-    if (s >= LBUFFER_SLOTS) {
-        // every slot owned by another id: reset the least recently written one
-        uint32_t cand_t = 0xFFFFFFFFu, cand = LBUFFER_SLOTS;
-        for (uint32_t q = 0; q < LBUFFER_SLOTS; q += 4) {
-            const uint4 t4 = *reinterpret_cast<const uint4 *>(bucket + LB_TIME + q);
-            const uint32_t t[4] = {t4.x, t4.y, t4.z, t4.w};
-            for (uint32_t k = 0; k < 4; k++)
-                if (t[k] < cand_t) { cand_t = t[k]; cand = q + k; }
-        }
-        if (cand < LBUFFER_SLOTS) reset_lslot(bucket, cand, vid, d, e, p.time);
-        return;
-    }
-    uint32_t *pl = bucket + LB_PAY + s * LB_PAYW;
-    const uint32_t st = __atomic_load_n(&bucket[LB_TIME + s], __ATOMIC_RELAXED);
-    if (!p.disable_stale_reset && st != 0u && st + p.update_time < p.time) {
-        reset_lslot(bucket, s, vid, d, e, p.time);
-        return;
-    }
-    if (__atomic_load_n(&pl[0], __ATOMIC_RELAXED) >= p.cap_direct)
-        for (int f = 0; f <= 4; f++) atomicExch(&pl[f], __atomic_load_n(&pl[f], __ATOMIC_RELAXED) >> 1);
-    if (__atomic_load_n(&pl[5], __ATOMIC_RELAXED) >= p.cap_indirect)
-        for (int f = 5; f <= 9; f++) atomicExch(&pl[f], __atomic_load_n(&pl[f], __ATOMIC_RELAXED) >> 1);
-    // A sample of zero light is never clamped when the firefly threshold
-    // cannot be negative, and adding zero changes no sum: both are skipped.
-    const bool clamp_zero = p.firefly_k < 0.0f || p.firefly_floor < 0.0f;
-    const bool d_zero = (d.x | d.y | d.z) == 0u, e_zero = (e.x | e.y | e.z) == 0u;
-    if (!d_zero || clamp_zero) firefly(pl, 0, d, p.firefly_k, p.firefly_floor);
-    if (!e_zero || clamp_zero) firefly(pl, 5, e, p.firefly_k, p.firefly_floor);
-    atomicAdd(&pl[0], 1u);
-    if (d.x) atomicAdd(&pl[1], d.x);
-    if (d.y) atomicAdd(&pl[2], d.y);
-    if (d.z) atomicAdd(&pl[3], d.z);
-    if (d.w) atomicAdd(&pl[4], d.w);
-    atomicAdd(&pl[5], 1u);
-    if (e.x) atomicAdd(&pl[6], e.x);
-    if (e.y) atomicAdd(&pl[7], e.y);
-    if (e.z) atomicAdd(&pl[8], e.z);
-    if (e.w) atomicAdd(&pl[9], e.w);
-    atomicExch(&bucket[LB_TIME + s], p.time);
-}
-
 __device__ __forceinline__ uint32_t lbuf_row_of(uint32_t hash_width, uint32_t vid) { return vid % (hash_width - 1u) + 1u; }
 __device__ __forceinline__ uint32_t lbuf_row(const FrameParams &p, uint32_t vid) { return lbuf_row_of(p.hash_width, vid); }
-
-__device__ void deposit(const FrameParams &p, uint32_t *__restrict__ lbuf, uint32_t vid, f3 dir_rgb, f3 ind_rgb)
-{
-    if (vid == 0u) return;
-    deposit_into(p, lbuf + (size_t)lbuf_row(p, vid) * LB_WORDS, vid, dir_rgb, ind_rgb);
-}
 
 __device__ f3 lookup_normal(const uint32_t *__restrict__ nbuf, uint32_t hw, uint32_t vid, f3 rd)   // shade.comp:261-275
 {
@@ -1550,6 +1659,9 @@ __device__ __forceinline__ uint32_t registry_seen(const uint32_t *__restrict__ c
 // into its primary voxel (r's dl/il), or ends without one.
 // ANY_BOUNCE false: the ray is past bounce 0 (reservoir reuse and the
 // first-shade registry view are then never needed and not compiled in).
+// shade_ray outcomes: the ray ends without depositing, survives, or deposits
+// and ends at one of shade.comp's three call sites (miss, last bounce,
+// Russian roulette), RAY_DEPOSITS + DEP_*
 enum : int { RAY_DONE = 0, RAY_SURVIVES = 1, RAY_DEPOSITS = 2 };
 
 template <bool ANY_BOUNCE>
@@ -1565,7 +1677,7 @@ __device__ __forceinline__ int shade_ray(const FrameParams &p, uint32_t iter, ui
         r.ilx = r.ilx + sky.x * r.tx;
         r.ily = r.ily + sky.y * r.ty;
         r.ilz = r.ilz + sky.z * r.tz;
-        return RAY_DEPOSITS;
+        return RAY_DEPOSITS + (int)DEP_MISS;
     }
     const uint32_t mi = r.hit_material < MATERIAL_SLOTS ? r.hit_material : MATERIAL_SLOTS - 1u;
     const Material &m = mats[mi];
@@ -1588,7 +1700,7 @@ __device__ __forceinline__ int shade_ray(const FrameParams &p, uint32_t iter, ui
         else
             nee_ris(p, nee, r, m, nrm, hx, hy, hz, n_lights);
     }
-    if (r.bounce + 1u >= MB) return RAY_DEPOSITS;   // last allowed bounce
+    if (r.bounce + 1u >= MB) return RAY_DEPOSITS + (int)DEP_LAST;   // last allowed bounce
     const bool use_ggx = m.metallic * (1.0f - m.roughness) > 0.5f;
     uint32_t rng = r.rng;
     f3 sdir, bt;
@@ -1616,7 +1728,7 @@ __device__ __forceinline__ int shade_ray(const FrameParams &p, uint32_t iter, ui
         if (xi > pr) dies = true;
         else ntp = divs(ntp, pr);
     }
-    if (dies) return RAY_DEPOSITS;
+    if (dies) return RAY_DEPOSITS + (int)DEP_RR;
     // survivor (shade.comp:892-908)
     r.ox = ((float)hx + 0.5f) + nrm.x;
     r.oy = ((float)hy + 0.5f) + nrm.y;
@@ -1629,10 +1741,29 @@ __device__ __forceinline__ int shade_ray(const FrameParams &p, uint32_t iter, ui
     return RAY_SURVIVES;
 }
 
+// Appends a deposit record for each lane with `dep` (the wave's records
+// take consecutive slots: one atomic per wave).
+__device__ __forceinline__ void record_deposit(bool dep, const Ray &r, uint32_t li, uint32_t iter, int outcome,
+                                               uint32_t *__restrict__ counters, Deferred *__restrict__ deferred)
+{
+    const uint64_t m = __ballot(dep);
+    if (!m) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(&counters[VE_CNT_DEFERRED], (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (dep && slot < SHADE_BUDGET_MAX)   // one deposit per ray and frame: always in range
+        deferred[slot] = Deferred{r.primary_vid, li | (iter << 18) | ((uint32_t)(outcome - RAY_DEPOSITS) << 22),
+                                  {r.dlx, r.dly, r.dlz}, {r.ilx, r.ily, r.ilz}};
+}
+
 __global__ __launch_bounds__(256) void k_shade(FrameParams p, const uint32_t *__restrict__ words, uint32_t iter,
                                                const Ray *__restrict__ rays_in, const uint32_t *__restrict__ live_in,
-                                               Ray *__restrict__ rays_out, uint32_t *__restrict__ live_out,
-                                               uint32_t *__restrict__ lbuf, const uint32_t *__restrict__ nbuf,
+                                               const uint32_t *__restrict__ li_in, Ray *__restrict__ rays_out,
+                                               uint32_t *__restrict__ live_out, uint32_t *__restrict__ li_out,
+                                               Deferred *__restrict__ deferred, const uint32_t *__restrict__ nbuf,
                                                uint32_t *__restrict__ rbuf, const Material *__restrict__ mats,
                                                const EmissiveVoxel *__restrict__ emissives,
                                                const uint32_t *__restrict__ em_reg, uint32_t *__restrict__ counters,
@@ -1643,13 +1774,16 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, const uint32_t *__
     const uint32_t live = *live_in;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     bool survive = false;
+    int outcome = RAY_DONE;
+    uint32_t li = 0u;
     Ray r;
     if (t < live) {
         r = rays_in[t];
-        const int outcome = shade_ray<true>(p, iter, t, r, nbuf, mats, em_reg, counters, nee);
-        if (outcome == RAY_DEPOSITS) deposit(p, lbuf, r.primary_vid, f3{r.dlx, r.dly, r.dlz}, f3{r.ilx, r.ily, r.ilz});
+        li = li_in[t];
+        outcome = shade_ray<true>(p, iter, t, r, nbuf, mats, em_reg, counters, nee);
         survive = outcome == RAY_SURVIVES;
     }
+    record_deposit(outcome >= RAY_DEPOSITS, r, li, iter, outcome, counters, deferred);
     // shadow rays of the wave: one atomic each for the frame counter and the
     // cumulative total
     unsigned long long sr = nee.shadow_rays;
@@ -1667,7 +1801,10 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, const uint32_t *__
         uint32_t base = 0;
         if ((int)lane == leader) base = atomicAdd(live_out, (uint32_t)__popcll(mask));
         base = __shfl(base, leader, 64);
-        if (survive) rays_out[base + before] = r;
+        if (survive) {
+            rays_out[base + before] = r;
+            li_out[base + before] = li;
+        }
     }
 }
 
@@ -1701,6 +1838,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                                                 uint32_t *__restrict__ rbuf, const EmissiveVoxel *__restrict__ emissives,
                                                 const uint32_t *__restrict__ em_reg,
                                                 const uint32_t *__restrict__ em_claim, uint32_t *__restrict__ em_bits,
+                                                const uint32_t *__restrict__ li_in,
                                                 Deferred *__restrict__ deferred, uint32_t *__restrict__ bounce_part)
 {
     const uint32_t live = *live_ptr;
@@ -1712,7 +1850,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     uint32_t checks_sum = 0, reads = 0, entered = 0, reached = 0;
     bool active = t < live;
     Ray r;
-    if (active) r = rays_in[t];
+    uint32_t li = 0u;
+    if (active) {
+        r = rays_in[t];
+        li = li_in[t];
+    }
     for (uint32_t iter = first; iter < iters; iter++) {
         if (__ballot(active) == 0ull) break;
         if (!active) continue;
@@ -1722,10 +1864,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         trace_ray<STATS>(words, p, iter, t, r, mats, em_claim, em_bits, path_cache + threadIdx.x, checks, reads);
         checks_sum += checks;
         const int outcome = shade_ray<false>(p, iter, t, r, nbuf, mats, em_reg, counters, nee);
-        if (outcome == RAY_DEPOSITS) {   // at most one deposit per ray: the slot is always in range
+        if (outcome >= RAY_DEPOSITS) {   // at most one deposit per ray: the slot is always in range
             const uint32_t slot = atomicAdd(&counters[VE_CNT_DEFERRED], 1u);
             if (slot < SHADE_BUDGET_MAX)
-                deferred[slot] = Deferred{r.primary_vid, iter, {r.dlx, r.dly, r.dlz}, {r.ilx, r.ily, r.ilz}};
+                deferred[slot] = Deferred{r.primary_vid,
+                                          li | (iter << 18) | ((uint32_t)(outcome - RAY_DEPOSITS) << 22),
+                                          {r.dlx, r.dly, r.dlz}, {r.ilx, r.ily, r.ilz}};
         }
         active = outcome == RAY_SURVIVES;
     }
@@ -1757,16 +1901,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     }
 }
 
-// Puts each deposit held back by the fused bounce pass (append order) into
-// its lBuffer bucket: the first BIN_CAP in place, the rest into the overflow
-// list (entries of id 0 deposit nothing).  Its first thread folds k_bounce's
-// ray-count copies into the frame counters and the cumulative totals and
-// clears them.
-__global__ __launch_bounds__(256) void k_bin_place(BinLists bins, const Deferred *__restrict__ deferred,
-                                                   uint32_t *__restrict__ counters, uint32_t *__restrict__ bounce_part,
-                                                   unsigned long long *__restrict__ totals)
+// ---------------------------------------------------------------------------
+// The frame's deposits in the reference's order (AccumPass)
+// ---------------------------------------------------------------------------
+// The reference deposits from its shade dispatches: chunk by chunk of 49152
+// shade-list entries (renderer.cpp:476-606), bounce iteration by iteration
+// within a chunk, each dispatch over the window of that chunk's rays still
+// alive, in SIMD-8 vectors, each vector's lanes in lockstep through its three
+// call sites (misses, last bounce, Russian roulette).  A window keeps the
+// shade-list order (survivors are appended in invocation order), so a ray's
+// place in the window of iteration b is the number of rays of its chunk
+// before it in the list still alive at b.  Every deposit of the frame is
+// recorded with its ray's list index, iteration and call site; after the
+// bounces:
+//   k_dep_prep    records each ray's last iteration and counts the deposits
+//                 per lBuffer bucket (and folds the fused pass's ray counts);
+//   k_dep_wpos    window positions: one block per (chunk, iteration) scans
+//                 "alive at that iteration" over the chunk;
+//   k_order_scan  per-bucket offsets;
+//   k_dep_place   each deposit's order key {chunk, iteration, vector, call
+//                 site, lane} into its bucket's list;
+//   k_dep_replay  one wave per bucket sorts its deposits and replays
+//                 depositSampleDual on an LDS copy of the bucket: a call site
+//                 of a vector with one deposit in the bucket as one
+//                 invocation, several in lockstep (deposit_lockstep).
+__global__ __launch_bounds__(256) void k_dep_prep(FrameParams p, const Deferred *__restrict__ deferred,
+                                                  uint32_t *__restrict__ counters, uint32_t *__restrict__ bounce_part,
+                                                  unsigned long long *__restrict__ totals, uint32_t *__restrict__ death,
+                                                  uint32_t *__restrict__ cnt)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {   // the fused bounce pass's per-wave ray counts
         uint32_t rays = 0, reached = 0;
         for (uint32_t c = 0; c < BOUNCE_COPIES; c++) {
             rays += bounce_part[2u * c];
@@ -1779,85 +1943,245 @@ __global__ __launch_bounds__(256) void k_bin_place(BinLists bins, const Deferred
     }
     const uint32_t n = min(counters[VE_CNT_DEFERRED], SHADE_BUDGET_MAX);
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        Deferred d = deferred[e];
-        if (d.vid == 0u) continue;
-        const uint32_t h = lbuf_row_of(bins.hash_width, d.vid);
-        const uint32_t local = atomicAdd(&bins.count[h], 1u);
-        d.iter |= local << 8;
-        if (local < BIN_CAP) {
-            bins.binned[(size_t)h * BIN_CAP + local] = d;
-        } else {   // at most n entries overflow: always in range
-            bins.over[atomicAdd(&bins.over_n[0], 1u)] = d;
-        }
+        const Deferred d = deferred[e];
+        death[d.meta & 0x3FFFFu] = (d.meta >> 18) & 0xFu;
+        if (d.vid != 0u) atomicAdd(&cnt[lbuf_row_of(p.hash_width, d.vid)], 1u);
     }
 }
 
-// Gives each lBuffer bucket with more than BIN_CAP held-back deposits its
-// region of the overflow-sorted list (the bucket's entry of local index
-// BIN_CAP allocates it; the counts are final after k_bin_place).
-__global__ __launch_bounds__(256) void k_bin_over_alloc(BinLists bins)
+// Block (chunk c, iteration b): for each list entry of chunk c whose ray
+// deposited at iteration b, the number of the chunk's entries before it
+// whose rays were alive at b (deposited at b or later).
+__global__ __launch_bounds__(1024) void k_dep_wpos(const uint32_t *__restrict__ counters,
+                                                   const uint32_t *__restrict__ death, uint32_t *__restrict__ wpos)
 {
-    const uint32_t n = min(bins.over_n[0], SHADE_BUDGET_MAX);
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        const Deferred d = bins.over[e];
-        if ((d.iter >> 8) != BIN_CAP) continue;
-        const uint32_t h = lbuf_row_of(bins.hash_width, d.vid);
-        bins.over_off[h] = atomicAdd(&bins.over_n[1], bins.count[h] - BIN_CAP);
-    }
-}
-
-// Places each overflow entry at its bucket's region plus its local index past
-// BIN_CAP, so a bucket's overflow deposits lie together in append order.
-__global__ __launch_bounds__(256) void k_bin_over_place(BinLists bins)
-{
-    const uint32_t n = min(bins.over_n[0], SHADE_BUDGET_MAX);
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        const Deferred d = bins.over[e];
-        const uint32_t h = lbuf_row_of(bins.hash_width, d.vid);
-        bins.over_sorted[bins.over_off[h] + (d.iter >> 8) - BIN_CAP] = d;
-    }
-}
-
-// APPLY_GROUP lanes per lBuffer bucket with deposits, 64 / APPLY_GROUP
-// buckets per wave: each bucket is copied to LDS, its deposits are applied
-// there iteration by iteration (the group's lanes concurrently within an
-// iteration, as a dispatch's invocations are), and it is written back; the
-// bucket's count is cleared for the next frame.  A bucket holds about a dozen
-// deferred deposits on average, so a whole wave per bucket would leave most
-// lanes idle through the owner scans and atomics.
-constexpr uint32_t APPLY_GROUP = 16, APPLY_BUCKETS = 64 / APPLY_GROUP;
-
-__global__ __launch_bounds__(64) void k_bin_apply(FrameParams p, BinLists bins, uint32_t *__restrict__ lbuf,
-                                                  uint32_t first, uint32_t iters)
-{
-    __shared__ __align__(16) uint32_t bk[APPLY_BUCKETS][LB_WORDS];   // owner rows are read as uint4
-    if (blockIdx.x == 0 && threadIdx.x == 0) bins.over_n[0] = bins.over_n[1] = 0u;   // read by earlier passes only
-    const uint32_t g = threadIdx.x / APPLY_GROUP, gl = threadIdx.x % APPLY_GROUP;
-    const uint32_t h = blockIdx.x * APPLY_BUCKETS + g;
-    const uint32_t n = h < p.hash_width ? bins.count[h] : 0u;
-    if (__ballot(n != 0u) == 0ull) return;   // the whole wave (one wave per block)
-    uint4 *lb = reinterpret_cast<uint4 *>(bk[g]);
-    uint4 *gb = reinterpret_cast<uint4 *>(lbuf + (size_t)h * LB_WORDS);
-    if (n != 0u)
-        for (uint32_t i = gl; i < LB_WORDS / 4u; i += APPLY_GROUP) lb[i] = gb[i];
+    __shared__ uint32_t part[1024];
+    const uint32_t ns = counters[VE_CNT_SHADE], c = blockIdx.x, b = blockIdx.y, t = threadIdx.x;
+    const uint32_t lo = c * RAY_CHUNK;
+    if (lo >= ns) return;
+    const uint32_t hi = min(lo + RAY_CHUNK, ns);
+    const uint32_t per = (hi - lo + 1023u) / 1024u, a = min(lo + t * per, hi), z = min(a + per, hi);
+    uint32_t cnt = 0;
+    for (uint32_t i = a; i < z; i++) cnt += death[i] >= b ? 1u : 0u;
+    part[t] = cnt;
     __syncthreads();
-    const Deferred *held = bins.binned + (size_t)h * BIN_CAP;
-    const Deferred *over = bins.over_sorted + (n > BIN_CAP ? bins.over_off[h] : 0u);
-    for (uint32_t it = first; it < iters; it++) {
-        for (uint32_t base = 0; base < n; base += APPLY_GROUP) {
-            const uint32_t e = base + gl;
-            if (e < n) {
-                const Deferred d = e < BIN_CAP ? held[e] : over[e - BIN_CAP];
-                if ((d.iter & 0xFFu) == it)
-                    deposit_into(p, bk[g], d.vid, f3{d.dl[0], d.dl[1], d.dl[2]}, f3{d.il[0], d.il[1], d.il[2]});
-            }
-        }
+    for (uint32_t d = 1; d < 1024u; d <<= 1) {
+        const uint32_t v = t >= d ? part[t - d] : 0u;
+        __syncthreads();
+        part[t] += v;
         __syncthreads();
     }
-    if (n != 0u) {
-        for (uint32_t i = gl; i < LB_WORDS / 4u; i += APPLY_GROUP) gb[i] = lb[i];
-        if (gl == 0u) bins.count[h] = 0u;
+    uint32_t run = part[t] - cnt;
+    for (uint32_t i = a; i < z; i++) {
+        const uint32_t di = death[i];
+        if (di == b) wpos[i] = run;
+        run += di >= b ? 1u : 0u;
     }
+}
+
+__global__ __launch_bounds__(256) void k_dep_place(FrameParams p, const Deferred *__restrict__ deferred,
+                                                   const uint32_t *__restrict__ counters,
+                                                   const uint32_t *__restrict__ wpos, uint32_t *__restrict__ cnt,
+                                                   const uint32_t *__restrict__ off, uint2 *__restrict__ ev)
+{
+    const uint32_t n = min(counters[VE_CNT_DEFERRED], SHADE_BUDGET_MAX);
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const Deferred d = deferred[e];
+        if (d.vid == 0u) continue;
+        const uint32_t li = d.meta & 0x3FFFFu, it = (d.meta >> 18) & 0xFu, site = (d.meta >> 22) & 3u;
+        const uint32_t w = wpos[li];
+        const uint32_t key = ((li / RAY_CHUNK) << 22) | (it << 18) | ((w >> 3) << 5) | (site << 3) | (w & 7u);
+        const uint32_t h = lbuf_row_of(p.hash_width, d.vid);
+        ev[off[h] + atomicAdd(&cnt[h], 1u)] = make_uint2(key, e);
+    }
+}
+
+// Slot state of an lBuffer bucket copy (device layout: owner[32] | time[32] |
+// payload[32][10], payload = dCnt dR dG dB dL2 iCnt iR iG iB iL2).
+__device__ __forceinline__ void lb_reset(uint32_t *bk, uint32_t s, uint32_t vid, uint4 d, uint4 e, uint32_t time)
+{   // resetLSlotDual (shade.comp:108-126)
+    uint32_t *pl = bk + LB_PAY + s * LB_PAYW;
+    bk[s] = vid;
+    pl[0] = 1u; pl[1] = d.x; pl[2] = d.y; pl[3] = d.z; pl[4] = d.w;
+    pl[5] = 1u; pl[6] = e.x; pl[7] = e.y; pl[8] = e.z; pl[9] = e.w;
+    bk[LB_TIME + s] = time;
+}
+
+__device__ __forceinline__ void lb_firefly(const uint32_t *pl, uint32_t cf, uint4 &v, float k, float floor_)
+{   // shade.comp:201-234 for one channel, on the slot's current sums
+    const uint32_t c = pl[cf];
+    if (c <= 4u) return;
+    const float meanL = (float)(pl[cf + 1] + pl[cf + 2] + pl[cf + 3]) / (float)(3u * c);
+    const float sampleL = (float)(v.x + v.y + v.z) / 3.0f;
+    const float thresh = gmax(meanL * k, floor_);
+    if (sampleL > thresh) {
+        const float scale = thresh / sampleL;
+        v.x = (uint32_t)((float)v.x * scale);
+        v.y = (uint32_t)((float)v.y * scale);
+        v.z = (uint32_t)((float)v.z * scale);
+        const uint32_t L = (v.x + v.y + v.z) / 3u;
+        v.w = L * L;
+    }
+}
+
+__device__ __forceinline__ void lb_add(uint32_t *pl, uint4 d, uint4 e)
+{
+    pl[0] += 1u; pl[1] += d.x; pl[2] += d.y; pl[3] += d.z; pl[4] += d.w;
+    pl[5] += 1u; pl[6] += e.x; pl[7] += e.y; pl[8] += e.z; pl[9] += e.w;
+}
+
+// depositSampleDual for k (2..8) lanes of one vector at one call site, all
+// into this bucket, in llvmpipe's lockstep order: per slot iteration every
+// active lane's compare-and-swap (lane order), then every lane's owner and
+// time loads; lanes whose id matches deposit (all halving loads before the
+// halving stores, all firefly reads before any add) and leave; the others
+// fold the loaded time into their eviction candidate.  Lanes that never
+// match reset their candidate after the loop, in lane order.  One thread.
+__device__ void deposit_lockstep(const FrameParams &p, uint32_t *bk, uint32_t k, const uint32_t *vid, uint4 *d,
+                                 uint4 *e)
+{
+    bool active[8];
+    uint32_t cand_t[8], cand_i[8];
+    for (uint32_t l = 0; l < k; l++) {
+        active[l] = true;
+        cand_t[l] = NONE;
+        cand_i[l] = 0u;
+    }
+    for (uint32_t i = 0; i < LBUFFER_SLOTS; i++) {
+        bool any = false;
+        for (uint32_t l = 0; l < k; l++) any |= active[l];
+        if (!any) break;
+        for (uint32_t l = 0; l < k; l++)
+            if (active[l] && bk[i] == 0u) bk[i] = vid[l];
+        const uint32_t id = bk[i], st = bk[LB_TIME + i];   // every active lane loads the same slot
+        bool match[8];
+        for (uint32_t l = 0; l < k; l++) {
+            match[l] = active[l] && id == vid[l];
+            if (active[l] && !match[l] && st < cand_t[l]) {
+                cand_t[l] = st;
+                cand_i[l] = i;
+            }
+        }
+        uint32_t *pl = bk + LB_PAY + i * LB_PAYW;
+        for (uint32_t l = 0; l < k; l++)   // stale reset (shade.comp:183-186)
+            if (match[l] && !p.disable_stale_reset && st != 0u && st + p.update_time < p.time) {
+                lb_reset(bk, i, vid[l], d[l], e[l], p.time);
+                match[l] = active[l] = false;
+            }
+        for (uint32_t ch = 0; ch < 2; ch++) {   // halving at the caps (shade.comp:188-194)
+            const uint32_t f0 = ch ? 5u : 0u, cap = ch ? p.cap_indirect : p.cap_direct;
+            bool halve = false;
+            for (uint32_t l = 0; l < k; l++) halve |= match[l] && pl[f0] >= cap;
+            if (halve)   // every halving lane loads the same values before any stores
+                for (uint32_t f = 0; f < 5; f++) pl[f0 + f] >>= 1;
+        }
+        for (uint32_t l = 0; l < k; l++)
+            if (match[l]) lb_firefly(pl, 0, d[l], p.firefly_k, p.firefly_floor);
+        for (uint32_t l = 0; l < k; l++)
+            if (match[l]) lb_firefly(pl, 5, e[l], p.firefly_k, p.firefly_floor);
+        for (uint32_t l = 0; l < k; l++)
+            if (match[l]) {
+                lb_add(pl, d[l], e[l]);
+                bk[LB_TIME + i] = p.time;
+                active[l] = false;
+            }
+    }
+    for (uint32_t l = 0; l < k; l++)   // bucket full: reset the candidate (shade.comp:254-256)
+        if (active[l] && cand_t[l] != NONE) lb_reset(bk, cand_i[l], vid[l], d[l], e[l], p.time);
+}
+
+constexpr uint32_t DEP_LDS_EVENTS = 512;   // deposits of one bucket sorted in LDS (more: device memory)
+
+__global__ __launch_bounds__(64) void k_dep_replay(FrameParams p, uint32_t *__restrict__ lbuf,
+                                                   const uint32_t *__restrict__ off, const uint2 *__restrict__ ev,
+                                                   const Deferred *__restrict__ deferred, uint64_t *__restrict__ spill,
+                                                   uint32_t *__restrict__ cnt)
+{
+    __shared__ uint64_t lds_keys[DEP_LDS_EVENTS];
+    __shared__ __align__(16) uint32_t bk[LB_WORDS];
+    __shared__ uint32_t g_vid[8];
+    __shared__ uint4 g_d[8], g_e[8];
+    const uint32_t h = blockIdx.x;
+    const uint32_t base = off[h], n = off[h + 1] - base;
+    if (n == 0u) return;
+    const uint32_t lane = threadIdx.x;
+    if (lane == 0u) cnt[h] = 0u;   // the next frame's count starts from zero
+    const bool in_lds = n <= DEP_LDS_EVENTS;
+    uint64_t *keys = in_lds ? lds_keys : spill + base;
+    for (uint32_t i = lane; i < n; i += 64u) {
+        const uint64_t k = ((uint64_t)ev[base + i].x << 32) | i;
+        if (in_lds) keys[i] = k;
+        else __hip_atomic_store(keys + i, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    uint32_t *gb = lbuf + (size_t)h * LB_WORDS;
+    for (uint32_t i = lane; i < LB_WORDS / 4u; i += 64u)
+        reinterpret_cast<uint4 *>(bk)[i] = reinterpret_cast<const uint4 *>(gb)[i];
+    if (!in_lds) __atomic_thread_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();
+    if (in_lds) wave_sort_keys<false>(keys, n);
+    else wave_sort_keys<true>(keys, n);
+    auto key_at = [&](uint32_t k) -> uint64_t {
+        return in_lds ? keys[k] : __hip_atomic_load(keys + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    for (uint32_t k = 0; k < n;) {
+        // the group: consecutive deposits of one vector's call site (same key but the lane)
+        const uint32_t group = (uint32_t)(key_at(k) >> 35);
+        uint32_t m = 1;
+        while (k + m < n && m < 8u && (uint32_t)(key_at(k + m) >> 35) == group) m++;
+        if (lane < m) {
+            const Deferred dd = deferred[ev[base + (uint32_t)key_at(k + lane)].y];
+            g_vid[lane] = dd.vid;
+            g_d[lane] = pack_rgb_l2(f3{dd.dl[0], dd.dl[1], dd.dl[2]});
+            g_e[lane] = pack_rgb_l2(f3{dd.il[0], dd.il[1], dd.il[2]});
+        }
+        __syncthreads();
+        if (m == 1u) {   // one invocation: the first slot that is free or holds the voxel, else the LRU slot
+            const uint32_t vid = g_vid[0];
+            const bool sl = lane < LBUFFER_SLOTS;
+            const uint32_t own = sl ? bk[lane] : NONE;
+            const uint64_t hit = __ballot(sl && (own == vid || own == 0u));
+            const uint32_t t = sl ? bk[LB_TIME + lane] : NONE;
+            uint32_t best = t;
+            for (int o = 32; o > 0; o >>= 1) best = min(best, (uint32_t)__shfl_xor((int)best, o, 64));
+            const uint64_t oldest = __ballot(sl && t == best);
+            if (lane == 0u) {
+                uint4 d = g_d[0], e = g_e[0];
+                if (hit) {
+                    const uint32_t s = (uint32_t)__ffsll((unsigned long long)hit) - 1u;
+                    if (bk[s] == 0u) bk[s] = vid;
+                    const uint32_t st = bk[LB_TIME + s];
+                    uint32_t *pl = bk + LB_PAY + s * LB_PAYW;
+                    if (!p.disable_stale_reset && st != 0u && st + p.update_time < p.time) {
+                        lb_reset(bk, s, vid, d, e, p.time);
+                    } else {
+                        if (pl[0] >= p.cap_direct)
+                            for (uint32_t f = 0; f < 5; f++) pl[f] >>= 1;
+                        if (pl[5] >= p.cap_indirect)
+                            for (uint32_t f = 5; f < 10; f++) pl[f] >>= 1;
+                        lb_firefly(pl, 0, d, p.firefly_k, p.firefly_floor);
+                        lb_firefly(pl, 5, e, p.firefly_k, p.firefly_floor);
+                        lb_add(pl, d, e);
+                        bk[LB_TIME + s] = p.time;
+                    }
+                } else if (best != NONE) {
+                    lb_reset(bk, (uint32_t)__ffsll((unsigned long long)oldest) - 1u, vid, d, e, p.time);
+                }
+            }
+        } else if (lane == 0u) {
+            uint32_t vids[8];
+            uint4 ds[8], es[8];
+            for (uint32_t l = 0; l < m; l++) {
+                vids[l] = g_vid[l];
+                ds[l] = g_d[l];
+                es[l] = g_e[l];
+            }
+            deposit_lockstep(p, bk, m, vids, ds, es);
+        }
+        __syncthreads();
+        k += m;
+    }
+    for (uint32_t i = lane; i < LB_WORDS / 4u; i += 64u)
+        reinterpret_cast<uint4 *>(gb)[i] = reinterpret_cast<const uint4 *>(bk)[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -2270,7 +2594,13 @@ void launch_fill(hipStream_t s, uint32_t *p, uint32_t v, size_t n)
 
 void launch_normal(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b)
 {
-    launch(k_normal, dim3(screen_blocks(p)), dim3(256), 0, s, words, p, b.gbuf, b.gpos, b.nbuf);
+    launch(k_nrm_events<false>, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.gpos, b.ord_cnt, b.ord_off,
+           b.ord_ev);
+    launch(k_order_scan, dim3(1), dim3(1024), 0, s, b.ord_cnt, b.ord_off, p.hash_width);
+    launch(k_nrm_events<true>, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.gpos, b.ord_cnt, b.ord_off,
+           b.ord_ev);
+    launch(k_nrm_replay, dim3(p.hash_width), dim3(64), 0, s, words, p, b.nbuf, b.ord_off, b.ord_ev, b.ord_spill,
+           b.ord_cnt);
 }
 
 // Per-frame clear of RankStats' per-frame half and the counters the frame
@@ -2329,7 +2659,7 @@ void launch_threshold(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 void launch_emit(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 {
     launch(k_emit, dim3(cdiv(p.shade_budget, 256)), dim3(256), 0, s, p, b.shade_list, b.rays, b.counters, b.gbuf, b.ghit,
-           b.materials, b.emissive_claim, b.em_bits, b.hit_bits, b.ray_totals);
+           b.materials, b.emissive_claim, b.em_bits, b.hit_bits, b.ray_totals, b.ray_li[0], b.dep_death);
 }
 
 // Bounce iteration `iter` of the trace/shade loop (renderer.cpp:515-602); the
@@ -2344,6 +2674,8 @@ static inline uint32_t *live_out_ptr(DeviceBuffers &b, uint32_t iter)
 {
     return b.counters + ((iter & 1u) ? VE_CNT_LIVE : VE_CNT_LIVE_B);
 }
+static inline uint32_t *li_in(DeviceBuffers &b, uint32_t iter) { return b.ray_li[iter & 1u]; }
+static inline uint32_t *li_out(DeviceBuffers &b, uint32_t iter) { return b.ray_li[(iter & 1u) ^ 1u]; }
 
 void launch_trace(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t iter, bool stats)
 {
@@ -2365,13 +2697,8 @@ void launch_shade(hipStream_t s, const uint32_t *words, const FrameParams &p, De
 {
     const uint32_t nb = cdiv(p.shade_budget, 256);
     launch(k_shade, dim3(nb), dim3(256), 0, s, p, words, iter, bounce_in(b, iter), live_in_ptr(b, iter),
-                       bounce_out(b, iter), live_out_ptr(b, iter), b.lbuf, b.nbuf, b.rbuf, b.materials, b.emissives,
-                       b.em_reg, b.counters, b.ray_totals);
-}
-
-static BinLists bin_lists(const FrameParams &p, const DeviceBuffers &b)
-{
-    return BinLists{p.hash_width, b.bin_count, b.binned, b.bin_over, b.bin_over_sorted, b.bin_off, b.bin_over_n};
+           li_in(b, iter), bounce_out(b, iter), live_out_ptr(b, iter), li_out(b, iter), b.deferred, b.nbuf, b.rbuf,
+           b.materials, b.emissives, b.em_reg, b.counters, b.ray_totals);
 }
 
 void launch_bounce(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t first,
@@ -2381,21 +2708,24 @@ void launch_bounce(hipStream_t s, const uint32_t *words, const FrameParams &p, D
     if (stats)
         launch(k_bounce<true>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, first), live_in_ptr(b, first), first,
                iters, b.counters, b.node_stats, b.ray_totals, b.materials, b.nbuf, b.rbuf, b.emissives, b.em_reg,
-               b.emissive_claim, b.em_bits, b.deferred, b.bounce_part);
+               b.emissive_claim, b.em_bits, li_in(b, first), b.deferred, b.bounce_part);
     else
         launch(k_bounce<false>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, first), live_in_ptr(b, first), first,
                iters, b.counters, b.node_stats, b.ray_totals, b.materials, b.nbuf, b.rbuf, b.emissives, b.em_reg,
-               b.emissive_claim, b.em_bits, b.deferred, b.bounce_part);
+               b.emissive_claim, b.em_bits, li_in(b, first), b.deferred, b.bounce_part);
 }
 
-int launch_deferred(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_t first, uint32_t iters)
+int launch_deposits(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_t iters)
 {
-    const BinLists bins = bin_lists(p, b);
-    launch(k_bin_place, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, bins, b.deferred, b.counters, b.bounce_part,
-           b.ray_totals);
-    launch(k_bin_over_alloc, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, bins);
-    launch(k_bin_over_place, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, bins);
-    launch(k_bin_apply, dim3(cdiv(p.hash_width, APPLY_BUCKETS)), dim3(64), 0, s, p, bins, b.lbuf, first, iters);
+    const uint32_t chunks = (p.shade_budget + RAY_CHUNK - 1u) / RAY_CHUNK;
+    launch(k_dep_prep, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, p, b.deferred, b.counters, b.bounce_part, b.ray_totals,
+           b.dep_death, b.dep_cnt);
+    launch(k_dep_wpos, dim3(chunks, iters > 0u ? iters : 1u), dim3(1024), 0, s, b.counters, b.dep_death, b.dep_wpos);
+    launch(k_order_scan, dim3(1), dim3(1024), 0, s, b.dep_cnt, b.dep_off, p.hash_width);
+    launch(k_dep_place, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, p, b.deferred, b.counters, b.dep_wpos, b.dep_cnt,
+           b.dep_off, b.dep_ev);
+    launch(k_dep_replay, dim3(p.hash_width), dim3(64), 0, s, p, b.lbuf, b.dep_off, b.dep_ev, b.deferred, b.dep_spill,
+           b.dep_cnt);
     return VE_OK;
 }
 

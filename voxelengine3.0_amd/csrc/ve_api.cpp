@@ -105,8 +105,9 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.dep_death, SHADE_BUDGET_MAX);
     a(&b.dep_wpos, SHADE_BUDGET_MAX);
     a(&b.dep_cnt, hash_width);
-    a(&b.dep_off, (size_t)hash_width + 1);
-    a(&b.dep_ev, SHADE_BUDGET_MAX);
+    a(&b.dep_slots, (size_t)hash_width * DEP_CAP);
+    a(&b.dep_over, SHADE_BUDGET_MAX);
+    a(&b.dep_over_n, 2);
     a(&b.dep_spill, SHADE_BUDGET_MAX);
     a(&b.materials, MATERIAL_SLOTS);
     a(&b.node_stats, 4);
@@ -121,10 +122,15 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.vec_chunk, (groups_max * 8 + 255) / 256);
     a(&b.rank_chunk, (std::min<size_t>(P, CANDIDATE_CAPACITY) + 255) / 256);
     a(&b.nsamp, std::min<size_t>(P, CANDIDATE_CAPACITY));
+    b.ord_cap = 64u;
+    while (b.ord_cap < ORDER_LDS_EVENTS && (size_t)b.ord_cap * hash_width < P) b.ord_cap *= 2u;
+    b.ord_over_cap = b.ord_spill_cap = (uint32_t)P;
     a(&b.ord_cnt, hash_width);
-    a(&b.ord_off, (size_t)hash_width + 1);
-    a(&b.ord_ev, P);
+    a(&b.ord_ev, (size_t)hash_width * b.ord_cap);
+    a(&b.ord_over, P);
+    a(&b.ord_over_n, 2);
     a(&b.ord_spill, P);
+    a(&b.ord_work, (size_t)hash_width * NBUFFER_SLOTS);
 }
 
 // RankStats thresholds start at (+inf, -inf): every voxel in the mid tier on

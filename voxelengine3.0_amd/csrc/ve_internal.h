@@ -40,6 +40,8 @@ constexpr uint32_t SCHED_SLOTS = 128;          // ordering table: voxels per cla
 constexpr uint32_t SCHED_OVER_CAP = 4096, SCHED_ROW_OVER = 2048;
 constexpr uint32_t MATERIAL_SLOTS = 127;       // shade.comp:90
 constexpr uint32_t DEFERRED_BLOCKS = 256;      // deposit-record passes' grid (entries strided over it)
+constexpr uint32_t ORDER_LDS_EVENTS = 512;     // per-bucket event lists up to this length sort in LDS
+constexpr uint32_t DEP_CAP = 128;              // deposits per lBuffer bucket listed in place (the rest overflow)
 // One deposit of the frame (depositSampleDual, shade.comp:162-257), held
 // until the frame's bounces are done and then replayed per lBuffer bucket in
 // the reference's order.  `meta`: the ray's shade-list index (bits 0-17), the
@@ -51,6 +53,7 @@ struct Deferred {
 };
 constexpr uint32_t DEP_MISS = 0, DEP_LAST = 1, DEP_RR = 2;   // shade.comp:558-564, 840-845, 877-889
 constexpr uint32_t RAY_CHUNK = 49152;          // renderer.hpp:94: rays per emit chunk (renderer.cpp:476-606)
+constexpr uint32_t MAX_BOUNCE_ITERS = 8;       // renderer.cpp:515: bounce iterations per chunk
 constexpr uint32_t RANK_BLOCKS = 1024;         // rank pass grid (candidates strided over it)
 // The rank blocks flush their counts, min/max and histogram into RANK_COPIES
 // copies (block b into copy b % RANK_COPIES), so no address takes more than

@@ -64,9 +64,10 @@ struct DeviceBuffers {
     uint32_t *dep_death = nullptr;       // SHADE_BUDGET_MAX: bounce iteration at which each list entry's ray deposited
     uint32_t *dep_wpos = nullptr;        // SHADE_BUDGET_MAX: its position in the reference's ray window then
     uint32_t *dep_cnt = nullptr;         // hash_width: deposits per lBuffer bucket (zero between frames)
-    uint32_t *dep_off = nullptr;         // hash_width + 1
-    uint2 *dep_ev = nullptr;             // SHADE_BUDGET_MAX: {order key, record} grouped per bucket
-    uint64_t *dep_spill = nullptr;       // SHADE_BUDGET_MAX: sort keys of buckets too large to sort in LDS
+    uint32_t *dep_slots = nullptr;       // hash_width x DEP_CAP: record indices of each bucket's deposits
+    uint2 *dep_over = nullptr;           // SHADE_BUDGET_MAX: {bucket, record} past a bucket's DEP_CAP
+    uint32_t *dep_over_n = nullptr;      // 2: overflow entries, spill words handed out (zero between frames)
+    uint64_t *dep_spill = nullptr;       // SHADE_BUDGET_MAX: sort keys of buckets past DEP_CAP
     Material *materials = nullptr;       // MATERIAL_SLOTS
     unsigned long long *node_stats = nullptr;  // 4: primary reads, trace reads, primary steps, trace steps
     unsigned long long *ray_totals = nullptr;  // 4: primary, traced, shadow, traced with a reused hit (cumulative)
@@ -82,10 +83,15 @@ struct DeviceBuffers {
     uint32_t *rank_chunk = nullptr;      // per 256 candidates: shade-list entries they append (rank)
     int32_t *nsamp = nullptr;            // per candidate: samples
     // ordered per-bucket replay of the normal pass (events of one frame)
+    uint32_t ord_cap = 0;                // events per bucket in place: a power of two >= max(64, pixels / hash_width)
+    uint32_t ord_over_cap = 0;           // = ord_spill_cap = max_w * max_h
+    uint32_t ord_spill_cap = 0;
     uint32_t *ord_cnt = nullptr;         // hash_width: events per nBuffer bucket (zero between passes)
-    uint32_t *ord_off = nullptr;         // hash_width + 1: each bucket's first event
-    uint4 *ord_ev = nullptr;             // max_w * max_h: {execution rank, pixel, voxel id, packed position}
-    uint64_t *ord_spill = nullptr;       // max_w * max_h: sort keys of buckets too large to sort in LDS
+    uint4 *ord_ev = nullptr;             // hash_width x ord_cap: {execution rank, 0, voxel id, packed position}
+    uint4 *ord_over = nullptr;           // ord_over_cap: {execution rank, bucket, voxel id, packed position}
+    uint32_t *ord_over_n = nullptr;      // 2: overflow events, spill words handed out (zero between passes)
+    uint64_t *ord_spill = nullptr;       // ord_spill_cap: sort keys of buckets past ord_cap
+    uint32_t *ord_work = nullptr;        // hash_width x 32: packed position of each slot's new normal, or NONE
 };
 
 void launch_primary(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, bool stats);

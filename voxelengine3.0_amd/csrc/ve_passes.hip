@@ -269,6 +269,14 @@ __device__ __forceinline__ void key_st(uint64_t *k, uint32_t i, uint64_t v)
     if (GLOBAL) __hip_atomic_store(k + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else k[i] = v;
 }
+// Orders a wave's LDS accesses before its next ones (the sorts and replays
+// are per wave, several waves to a block, so no block barrier).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <bool GLOBAL>
 __device__ void wave_sort_keys(uint64_t *keys, uint32_t n)
 {
@@ -288,10 +296,32 @@ __device__ void wave_sort_keys(uint64_t *keys, uint32_t n)
                     }
                 }
             }
-            if (GLOBAL) __atomic_thread_fence(__ATOMIC_SEQ_CST);
-            __syncthreads();
+            if (GLOBAL) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+            wave_lds_sync();
         }
     }
+}
+
+// The same network over one key per lane (n <= 64; absent keys ~0): lane i
+// receives the i-th smallest.
+__device__ __forceinline__ uint64_t wave_sort64(uint64_t key)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (uint32_t k = 2; k <= 64; k <<= 1)
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t partner = j == (k >> 1) ? lane ^ (k - 1u) : lane ^ j;
+            const uint64_t other = __shfl((unsigned long long)key, (int)partner, 64);
+            key = (lane < partner) ? (other < key ? other : key) : (other < key ? key : other);
+        }
+    return key;
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
 }
 
 // Execution rank of pixel (x, y) in an 8x8-group dispatch: work groups in
@@ -302,9 +332,48 @@ __device__ __forceinline__ uint32_t exec_rank(uint32_t x, uint32_t y, uint32_t W
     return (((y >> 3) * gw + (x >> 3)) << 6) | ((y & 7u) << 3) | (x & 7u);
 }
 
-// Buckets sorted and replayed from LDS up to this many events; larger ones
-// (a voxel covering many work groups of a close view) from device memory.
-constexpr uint32_t ORDER_LDS_EVENTS = 1024;
+// A bucket's events in sorted order: buckets of up to 64 events are sorted
+// in registers (lane i holds the i-th), up to ORDER_LDS_EVENTS in LDS, larger
+// ones (a voxel covering many work groups of a close view) in device memory.
+static_assert(DEP_CAP <= ORDER_LDS_EVENTS, "in-place deposit lists sort in LDS");
+
+struct SortedKeys {
+    uint32_t n;
+    uint64_t reg;        // keys == nullptr (n <= 64): this lane's key
+    uint64_t *keys;      // else: all n keys, sorted (LDS, or device memory when glob)
+    bool glob;
+    __device__ __forceinline__ uint64_t at(uint32_t q) const   // q = window base + lane
+    {
+        if (keys == nullptr) return reg;
+        if (!glob) return keys[q];
+        return __hip_atomic_load(keys + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
+// Sort keys (high word from key_hi(i), low word i) of the bucket's n events.
+template <typename KeyHi>
+__device__ __forceinline__ SortedKeys sort_bucket(uint32_t n, KeyHi key_hi, uint64_t *lds_keys, uint64_t *spill)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    SortedKeys sk{n, ~0ull, nullptr, false};
+    if (n <= 64u) {
+        sk.reg = wave_sort64(lane < n ? ((uint64_t)key_hi(lane) << 32) | lane : ~0ull);
+        return sk;
+    }
+    const bool in_lds = n <= ORDER_LDS_EVENTS;
+    sk.keys = in_lds ? lds_keys : spill;
+    sk.glob = !in_lds;
+    for (uint32_t i = lane; i < n; i += 64u) {
+        const uint64_t k = ((uint64_t)key_hi(i) << 32) | i;
+        if (in_lds) sk.keys[i] = k;
+        else __hip_atomic_store(sk.keys + i, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (!in_lds) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+    wave_lds_sync();
+    if (in_lds) wave_sort_keys<false>(sk.keys, n);
+    else wave_sort_keys<true>(sk.keys, n);
+    return sk;
+}
 
 // ---------------------------------------------------------------------------
 // normal.comp
@@ -317,129 +386,150 @@ constexpr uint32_t ORDER_LDS_EVENTS = 1024;
 // another, but the lanes that find no slot all scan the time stamps before
 // any of them writes: they pick the same slot, and the last of them owns it.
 //
-// 1. k_nrm_events (count, then place): one wave per 8x8 work group lists its
-//    hit pixels per bucket as events {execution rank, pixel, voxel id, packed
-//    position}.  A pixel whose closest earlier pixel of the group in the
-//    same bucket carries the same voxel (and position) is left out: it finds
-//    the slot that pixel just claimed, or evicts with it, to the same effect.
-// 2. k_order_scan: per-bucket event counts -> offsets.
-// 3. k_nrm_replay: one wave per bucket sorts its events and replays them;
-//    then the slots that took a new normal compute it, in parallel.
-template <bool PLACE>
+// 1. k_nrm_events: one wave per 8x8 work group lists its hit pixels per
+//    bucket as events {execution rank, -, voxel id, packed position}.  A pixel
+//    whose closest earlier pixel of the group in the same bucket carries the
+//    same voxel (and position) is left out: it finds the slot that pixel just
+//    claimed, or evicts with it, to the same effect.  One atomic per distinct
+//    bucket, all of a wave's in one instruction; each bucket has room for
+//    `cap` events, more go to an overflow list {rank, bucket, id, position}.
+// 2. k_nrm_replay: one wave per bucket (four per block) sorts its events and
+//    replays them, lanes holding the slots, events streamed through
+//    registers; each slot's new normal (if any) is noted in its work word.
+// 3. k_nrm_compute: one lane per slot with a new normal computes it.
+struct OrderLists {
+    uint32_t cap;          // events per bucket in place
+    uint32_t over_cap;     // overflow list capacity
+    uint32_t *cnt;         // per bucket: events (zero between passes)
+    uint4 *ev;             // bucket-major, cap each
+    uint4 *over;           // overflow events
+    uint32_t *over_n;      // [0] overflow entries, [1] spill words handed out (zero between passes)
+    uint64_t *spill;       // sort keys of buckets whose events overflowed
+    uint32_t spill_cap;
+};
+
 __global__ __launch_bounds__(256) void k_nrm_events(FrameParams p, const uint4 *__restrict__ gbuf,
-                                                    const uint32_t *__restrict__ gpos, uint32_t *__restrict__ cnt,
-                                                    const uint32_t *__restrict__ off, uint4 *__restrict__ ev)
+                                                    const uint32_t *__restrict__ gpos, OrderLists ol,
+                                                    uint32_t *__restrict__ counters)
 {
     uint32_t x = 0, y = 0;
     const bool valid = tile_pixel(p, x, y);
     const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (1ull << lane) - 1ull;
     const size_t pix = valid ? (size_t)y * p.W + x : 0;
     const uint32_t vid = valid ? gbuf[pix].w : 0u;
     const bool hit = vid != 0u;
     const uint32_t pk = hit ? gpos[pix] : 0u;
     const uint32_t h = hit ? vid % (p.hash_width - 1u) + 1u : 0u;
     uint64_t pending = __ballot(hit);
+    bool keep = false, leads = false;
+    uint32_t rank = 0, group = 0, leader_of = lane;
     while (pending) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
         const uint32_t hb = (uint32_t)__builtin_amdgcn_readlane((int)h, leader);
         const bool mine = hit && h == hb;
         const uint64_t same = __ballot(mine);
         pending &= ~same;
-        const uint64_t before = same & ((1ull << lane) - 1ull);
+        const uint64_t before = same & below;
         const int j = before ? 63 - __clzll((long long)before) : (int)lane;
         const uint32_t pv = __shfl(vid, j, 64), pp = __shfl(pk, j, 64);
-        const bool keep = mine && !(before && pv == vid && pp == pk);
-        const uint64_t kept = __ballot(keep);
-        uint32_t b = 0;
-        if ((int)lane == leader) b = atomicAdd(&cnt[hb], (uint32_t)__popcll(kept));
-        b = __shfl(b, leader, 64);
-        if (PLACE && keep)
-            ev[off[hb] + b + (uint32_t)__popcll(kept & ((1ull << lane) - 1ull))] =
-                make_uint4(exec_rank(x, y, p.W), (uint32_t)pix, vid, pk);
+        const bool k = mine && !(before && pv == vid && pp == pk);
+        const uint64_t kept = __ballot(k);
+        if (mine) {
+            keep = k;
+            rank = (uint32_t)__popcll(kept & below);
+            leader_of = (uint32_t)leader;
+        }
+        if ((int)lane == leader) {
+            leads = true;
+            group = (uint32_t)__popcll(kept);
+        }
+    }
+    uint32_t b = 0;
+    if (leads) b = atomicAdd(&ol.cnt[h], group);
+    b = __shfl(b, (int)leader_of, 64);
+    if (!keep) return;
+    const uint32_t at = b + rank, e = exec_rank(x, y, p.W);
+    if (at < ol.cap) {
+        ol.ev[(size_t)h * ol.cap + at] = make_uint4(e, 0u, vid, pk);
+    } else {
+        const uint32_t o = atomicAdd(&ol.over_n[0], 1u);
+        if (o < ol.over_cap) ol.over[o] = make_uint4(e, h, vid, pk);
+        else atomicOr(&counters[VE_CNT_OVERFLOW], 2u);   // past the list: no longer the reference's order
     }
 }
 
-// Exclusive scan of n per-bucket counts into off[0..n] (one block), clearing
-// the counts for the placing pass, which uses them as cursors.
-__global__ __launch_bounds__(1024) void k_order_scan(uint32_t *__restrict__ cnt, uint32_t *__restrict__ off, uint32_t n)
-{
-    __shared__ uint32_t part[1024];
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (n + 1023u) / 1024u, lo = min(t * per, n), hi = min(lo + per, n);
-    uint32_t sum = 0;
-    for (uint32_t i = lo; i < hi; i++) sum += cnt[i];
-    part[t] = sum;
-    __syncthreads();
-    for (uint32_t d = 1; d < 1024u; d <<= 1) {
-        const uint32_t v = t >= d ? part[t - d] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[t] - sum;
-    for (uint32_t i = lo; i < hi; i++) {
-        const uint32_t c = cnt[i];
-        off[i] = run;
-        run += c;
-        cnt[i] = 0u;
-    }
-    if (t == 1023u) off[n] = part[1023];
-}
-
-// Sorts bucket b's events by execution rank into `order` (indices into ev,
-// sorted; LDS up to ORDER_LDS_EVENTS, else the bucket's range of `spill`).
-__device__ __forceinline__ const uint64_t *sort_bucket_events(const uint4 *__restrict__ ev, uint32_t base, uint32_t n,
-                                                              uint64_t *lds_keys, uint64_t *__restrict__ spill)
+// The sorted events of bucket h: its in-place ones plus, when it has more
+// than `cap`, its entries of the overflow list (found by scanning it, keys in
+// spill words handed out for the bucket).  A key's low word indexes the
+// bucket's row, or with bit 31 set the overflow list.
+__device__ __forceinline__ SortedKeys sort_listed(const OrderLists &ol, uint32_t h, uint32_t n_all, uint64_t *lds_keys)
 {
     const uint32_t lane = threadIdx.x & 63u;
-    uint64_t *keys = n <= ORDER_LDS_EVENTS ? lds_keys : spill + base;
-    for (uint32_t i = lane; i < n; i += 64u) {
-        const uint64_t k = ((uint64_t)ev[base + i].x << 32) | i;
-        if (n <= ORDER_LDS_EVENTS) keys[i] = k;
-        else __hip_atomic_store(keys + i, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint4 *row = ol.ev + (size_t)h * ol.cap;
+    if (n_all <= ol.cap) return sort_bucket(n_all, [&](uint32_t i) { return row[i].x; }, lds_keys, nullptr);
+    uint32_t base = 0;
+    if (lane == 0u) base = atomicAdd(&ol.over_n[1], n_all);
+    base = __shfl(base, 0, 64);
+    if (base + n_all > ol.spill_cap) return SortedKeys{0u, ~0ull, nullptr, false};   // reported by the caller
+    SortedKeys sk{n_all, ~0ull, ol.spill + base, true};
+    for (uint32_t i = lane; i < ol.cap; i += 64u)
+        __hip_atomic_store(sk.keys + i, ((uint64_t)row[i].x << 32) | i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t L = min(ol.over_n[0], ol.over_cap);
+    uint32_t put = ol.cap;
+    for (uint32_t j0 = 0; j0 < L; j0 += 64u) {
+        const uint32_t j = j0 + lane;
+        const bool m = j < L && ol.over[j].y == h;
+        const uint64_t mb = __ballot(m);
+        const uint32_t at = put + (uint32_t)__popcll(mb & ((1ull << lane) - 1ull));
+        if (m && at < n_all)
+            __hip_atomic_store(sk.keys + at, ((uint64_t)ol.over[j].x << 32) | 0x80000000u | j, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        put += (uint32_t)__popcll(mb);
     }
-    if (n > ORDER_LDS_EVENTS) __atomic_thread_fence(__ATOMIC_SEQ_CST);
-    __syncthreads();
-    if (n <= ORDER_LDS_EVENTS) wave_sort_keys<false>(keys, n);
-    else wave_sort_keys<true>(keys, n);
-    return keys;
+    sk.n = min(put, n_all);   // fewer when the overflow list lost events (reported by the caller)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+    wave_sort_keys<true>(sk.keys, sk.n);
+    return sk;
 }
 
-__global__ __launch_bounds__(64) void k_nrm_replay(const uint32_t *__restrict__ words, FrameParams p,
-                                                   uint32_t *__restrict__ nbuf, const uint32_t *__restrict__ off,
-                                                   const uint4 *__restrict__ ev, uint64_t *__restrict__ spill,
-                                                   uint32_t *__restrict__ cnt)
+__device__ __forceinline__ uint4 listed_event(const OrderLists &ol, uint32_t h, uint64_t key)
 {
-    __shared__ uint64_t lds_keys[ORDER_LDS_EVENTS];
-    __shared__ uint4 lds_ev[ORDER_LDS_EVENTS];
-    __shared__ uint32_t path_cache[VE_MAX_DEPTH * 64];
-    const uint32_t h = blockIdx.x;
-    const uint32_t base = off[h], n = off[h + 1] - base;
+    const uint32_t lo = (uint32_t)key;
+    return (lo & 0x80000000u) ? ol.over[lo & 0x7FFFFFFFu] : ol.ev[(size_t)h * ol.cap + lo];
+}
+
+constexpr uint32_t REPLAY_WAVES = 4;   // buckets per block of the replays (one per wave)
+
+__global__ __launch_bounds__(64 * REPLAY_WAVES) void k_nrm_replay(FrameParams p, uint32_t *__restrict__ nbuf,
+                                                                  OrderLists ol, uint32_t *__restrict__ work,
+                                                                  uint32_t *__restrict__ counters)
+{
+    __shared__ uint64_t lds_keys_all[REPLAY_WAVES][ORDER_LDS_EVENTS];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const uint32_t h = blockIdx.x * REPLAY_WAVES + wv;
+    if (h >= p.hash_width) return;
+    const bool slot_lane = lane < NBUFFER_SLOTS;
+    const uint32_t n = ol.cnt[h];
+    if (slot_lane) work[h * NBUFFER_SLOTS + lane] = NONE;   // no new normal unless the replay gives one
     if (n == 0u) return;
-    const uint32_t lane = threadIdx.x;
-    if (lane == 0u) cnt[h] = 0u;   // the next pass's count starts from zero
-    const bool in_lds = n <= ORDER_LDS_EVENTS;
-    const uint64_t *keys = sort_bucket_events(ev, base, n, lds_keys, spill);
-    if (in_lds) {   // the events in sorted order, gathered at once
-        for (uint32_t i = lane; i < n; i += 64u) lds_ev[i] = ev[base + (uint32_t)keys[i]];
-        __syncthreads();
+    if (lane == 0u) ol.cnt[h] = 0u;   // the next pass's count starts from zero
+    const SortedKeys sk = sort_listed(ol, h, n, lds_keys_all[wv]);
+    if (sk.n < n) {   // events lost past the lists: no longer the reference's order
+        if (lane == 0u) atomicOr(&counters[VE_CNT_OVERFLOW], 2u);
+        if (sk.n == 0u) return;
     }
     uint32_t *bucket = nbuf + (size_t)h * NB_WORDS;
-    const bool slot_lane = lane < NBUFFER_SLOTS;
     uint32_t owner = slot_lane ? bucket[lane] : NONE;
     uint32_t tstamp = slot_lane ? bucket[NB_TIME + lane] : NONE;
-    uint32_t nrm = slot_lane ? bucket[NB_NRM + lane] : 0u;
+    const uint32_t nrm = slot_lane ? bucket[NB_NRM + lane] : 0u;
     bool has_src = false;   // the slot takes the normal at packed position `src`
     uint32_t src = 0u;
     uint32_t cur_vec = NONE, ev_vid = 0u, ev_pos = 0u, last_vid = 0u, last_pos = 0u;
-    auto flush = [&]() {   // the lockstep eviction of one vector's slotless lanes
-        if (ev_vid == 0u) return;
-        uint32_t t = slot_lane ? tstamp : NONE;   // first slot with the smallest stamp
-        uint32_t best = t;
-        for (int o = 32; o > 0; o >>= 1) best = min(best, (uint32_t)__shfl_xor((int)best, o, 64));
-        const uint64_t at = __ballot(slot_lane && t == best);
-        const uint32_t s = (uint32_t)__ffsll((unsigned long long)at) - 1u;
-        if (lane == s) {
+    auto evict = [&]() {   // the lockstep eviction of one vector's slotless lanes
+        const uint32_t best = wave_min_u32(slot_lane ? tstamp : NONE);
+        const uint64_t at = __ballot(slot_lane && tstamp == best);
+        if (lane == (uint32_t)__ffsll((unsigned long long)at) - 1u) {
             owner = ev_vid;
             has_src = true;
             src = ev_pos;
@@ -447,45 +537,64 @@ __global__ __launch_bounds__(64) void k_nrm_replay(const uint32_t *__restrict__ 
         }
         ev_vid = 0u;
     };
-    for (uint32_t k = 0; k < n; k++) {
-        const uint4 e = in_lds ? lds_ev[k]
-                               : ev[base + (uint32_t)__hip_atomic_load(keys + k, __ATOMIC_RELAXED,
-                                                                      __HIP_MEMORY_SCOPE_AGENT)];
-        const uint32_t vec = e.x >> 3;
-        if (vec != cur_vec) {
-            flush();
-            cur_vec = vec;
-        }
-        if (e.z == last_vid && e.w == last_pos) continue;   // finds what the last event left: no change
-        last_vid = e.z;
-        last_pos = e.w;
-        const uint64_t m = __ballot(slot_lane && (owner == e.z || owner == 0u));
-        if (m) {
-            if (lane == (uint32_t)__ffsll((unsigned long long)m) - 1u) {
-                if (owner == 0u) owner = e.z;
-                if (nrm == 0u && !has_src) {   // normal.comp:151-156: computed only into an empty field
-                    has_src = true;
-                    src = e.w;
-                }
-                tstamp = p.time;
+    for (uint32_t w0 = 0; w0 < sk.n; w0 += 64u) {
+        const uint32_t q = w0 + lane, cnt_w = min(64u, sk.n - w0);
+        uint4 e = make_uint4(0u, 0u, 0u, 0u);   // this lane's event of the window, in sorted order
+        if (q < sk.n) e = listed_event(ol, h, sk.at(q));
+        for (uint32_t j = 0; j < cnt_w; j++) {
+            const uint32_t vec = (uint32_t)__builtin_amdgcn_readlane((int)e.x, (int)j) >> 3;
+            const uint32_t vid = (uint32_t)__builtin_amdgcn_readlane((int)e.z, (int)j);
+            const uint32_t pos = (uint32_t)__builtin_amdgcn_readlane((int)e.w, (int)j);
+            if (vec != cur_vec) {
+                if (ev_vid != 0u) evict();
+                cur_vec = vec;
             }
-        } else {   // lockstep: the vector's last slotless lane wins the evicted slot
-            ev_vid = e.z;
-            ev_pos = e.w;
+            if (vid == last_vid && pos == last_pos) continue;   // finds what the last event left: no change
+            last_vid = vid;
+            last_pos = pos;
+            const uint64_t m = __ballot(slot_lane && (owner == vid || owner == 0u));
+            if (m) {
+                if (lane == (uint32_t)__ffsll((unsigned long long)m) - 1u) {
+                    if (owner == 0u) owner = vid;
+                    if (nrm == 0u && !has_src) {   // normal.comp:151-156: computed only into an empty field
+                        has_src = true;
+                        src = pos;
+                    }
+                    tstamp = p.time;
+                }
+            } else {   // lockstep: the vector's last slotless lane wins the evicted slot
+                ev_vid = vid;
+                ev_pos = pos;
+            }
         }
     }
-    flush();
-    if (has_src) {   // the new normals, one lane per slot
-        const uint32_t len = 1u << p.depth, bits = p.depth - 1u, mask = (1u << bits) - 1u;
-        const uint32_t hx = (src & mask) * 2u, hy = ((src >> bits) & mask) * 2u, hz = ((src >> (2u * bits)) & mask) * 2u;
-        nrm = pack_normal(refined_normal(words, p.jump, p.jump_level, p.n_words, p.depth, hx, hy, hz, len,
-                                         p.normal_precision, path_cache + lane, 64u));
-    }
+    if (ev_vid != 0u) evict();
     if (slot_lane) {
         bucket[lane] = owner;
-        bucket[NB_NRM + lane] = nrm;
         bucket[NB_TIME + lane] = tstamp;
+        if (has_src) work[h * NBUFFER_SLOTS + lane] = src;
     }
+}
+
+// Work word NONE: the slot keeps its normal.  (A packed position of NONE
+// would need every coordinate bit set past depth 11, where the reference's
+// packing has already overflowed; such a slot keeps its old normal.)
+__global__ __launch_bounds__(256) void k_nrm_compute(const uint32_t *__restrict__ words, FrameParams p,
+                                                     uint32_t *__restrict__ nbuf, const uint32_t *__restrict__ work,
+                                                     uint32_t *__restrict__ over_n)
+{
+    __shared__ uint32_t path_cache[VE_MAX_DEPTH * 256];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) over_n[0] = over_n[1] = 0u;   // the replay, the lists' last reader, is done
+    if (t >= p.hash_width * NBUFFER_SLOTS) return;
+    const uint32_t src = work[t];
+    if (src == NONE) return;
+    const uint32_t h = t / NBUFFER_SLOTS, sl = t % NBUFFER_SLOTS;
+    const uint32_t len = 1u << p.depth, bits = p.depth - 1u, mask = (1u << bits) - 1u;
+    const uint32_t hx = (src & mask) * 2u, hy = ((src >> bits) & mask) * 2u, hz = ((src >> (2u * bits)) & mask) * 2u;
+    nbuf[(size_t)h * NB_WORDS + NB_NRM + sl] =
+        pack_normal(refined_normal(words, p.jump, p.jump_level, p.n_words, p.depth, hx, hy, hz, len,
+                                   p.normal_precision, path_cache + threadIdx.x, 256u));
 }
 
 // ---------------------------------------------------------------------------
@@ -1914,21 +2023,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
 // before it in the list still alive at b.  Every deposit of the frame is
 // recorded with its ray's list index, iteration and call site; after the
 // bounces:
-//   k_dep_prep    records each ray's last iteration and counts the deposits
-//                 per lBuffer bucket (and folds the fused pass's ray counts);
-//   k_dep_wpos    window positions: one block per (chunk, iteration) scans
-//                 "alive at that iteration" over the chunk;
-//   k_order_scan  per-bucket offsets;
-//   k_dep_place   each deposit's order key {chunk, iteration, vector, call
-//                 site, lane} into its bucket's list;
-//   k_dep_replay  one wave per bucket sorts its deposits and replays
-//                 depositSampleDual on an LDS copy of the bucket: a call site
-//                 of a vector with one deposit in the bucket as one
-//                 invocation, several in lockstep (deposit_lockstep).
+//   k_dep_prep    records each ray's last iteration and lists each deposit
+//                 under its lBuffer bucket (and folds the fused pass's ray
+//                 counts);
+//   k_dep_wpos    window positions, one block per 1024 list entries;
+//   k_dep_replay  one wave per bucket sorts its deposits by order key
+//                 {chunk, iteration, vector, call site, lane} and replays
+//                 depositSampleDual with lane s holding slot s: a vector's
+//                 call site with one deposit here as one invocation, several
+//                 in lockstep (deposit_lockstep, on an LDS copy);
+//   k_dep_done    clears the frame's record and overflow counts.
+struct DepLists {
+    uint32_t cap;          // deposits per bucket in place
+    uint32_t *cnt;         // per bucket (zero between frames)
+    uint32_t *slots;       // bucket-major record indices, cap each
+    uint2 *over;           // {bucket, record} of the rest
+    uint32_t *over_n;      // [0] overflow entries, [1] spill words handed out
+    uint64_t *spill;       // sort keys of buckets that overflowed
+    uint32_t cap_total;    // over / spill capacity (SHADE_BUDGET_MAX)
+};
+
 __global__ __launch_bounds__(256) void k_dep_prep(FrameParams p, const Deferred *__restrict__ deferred,
                                                   uint32_t *__restrict__ counters, uint32_t *__restrict__ bounce_part,
                                                   unsigned long long *__restrict__ totals, uint32_t *__restrict__ death,
-                                                  uint32_t *__restrict__ cnt)
+                                                  DepLists dl)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) {   // the fused bounce pass's per-wave ray counts
         uint32_t rays = 0, reached = 0;
@@ -1945,54 +2063,65 @@ __global__ __launch_bounds__(256) void k_dep_prep(FrameParams p, const Deferred 
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
         const Deferred d = deferred[e];
         death[d.meta & 0x3FFFFu] = (d.meta >> 18) & 0xFu;
-        if (d.vid != 0u) atomicAdd(&cnt[lbuf_row_of(p.hash_width, d.vid)], 1u);
-    }
-}
-
-// Block (chunk c, iteration b): for each list entry of chunk c whose ray
-// deposited at iteration b, the number of the chunk's entries before it
-// whose rays were alive at b (deposited at b or later).
-__global__ __launch_bounds__(1024) void k_dep_wpos(const uint32_t *__restrict__ counters,
-                                                   const uint32_t *__restrict__ death, uint32_t *__restrict__ wpos)
-{
-    __shared__ uint32_t part[1024];
-    const uint32_t ns = counters[VE_CNT_SHADE], c = blockIdx.x, b = blockIdx.y, t = threadIdx.x;
-    const uint32_t lo = c * RAY_CHUNK;
-    if (lo >= ns) return;
-    const uint32_t hi = min(lo + RAY_CHUNK, ns);
-    const uint32_t per = (hi - lo + 1023u) / 1024u, a = min(lo + t * per, hi), z = min(a + per, hi);
-    uint32_t cnt = 0;
-    for (uint32_t i = a; i < z; i++) cnt += death[i] >= b ? 1u : 0u;
-    part[t] = cnt;
-    __syncthreads();
-    for (uint32_t d = 1; d < 1024u; d <<= 1) {
-        const uint32_t v = t >= d ? part[t - d] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[t] - cnt;
-    for (uint32_t i = a; i < z; i++) {
-        const uint32_t di = death[i];
-        if (di == b) wpos[i] = run;
-        run += di >= b ? 1u : 0u;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_dep_place(FrameParams p, const Deferred *__restrict__ deferred,
-                                                   const uint32_t *__restrict__ counters,
-                                                   const uint32_t *__restrict__ wpos, uint32_t *__restrict__ cnt,
-                                                   const uint32_t *__restrict__ off, uint2 *__restrict__ ev)
-{
-    const uint32_t n = min(counters[VE_CNT_DEFERRED], SHADE_BUDGET_MAX);
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        const Deferred d = deferred[e];
         if (d.vid == 0u) continue;
-        const uint32_t li = d.meta & 0x3FFFFu, it = (d.meta >> 18) & 0xFu, site = (d.meta >> 22) & 3u;
-        const uint32_t w = wpos[li];
-        const uint32_t key = ((li / RAY_CHUNK) << 22) | (it << 18) | ((w >> 3) << 5) | (site << 3) | (w & 7u);
         const uint32_t h = lbuf_row_of(p.hash_width, d.vid);
-        ev[off[h] + atomicAdd(&cnt[h], 1u)] = make_uint2(key, e);
+        const uint32_t at = atomicAdd(&dl.cnt[h], 1u);
+        if (at < dl.cap) dl.slots[(size_t)h * dl.cap + at] = e;
+        else dl.over[atomicAdd(&dl.over_n[0], 1u)] = make_uint2(h, e);   // at most one entry per record
+    }
+}
+
+// Block k: list entries [1024k, 1024k + 1024) of chunk c = k / 48: for each
+// entry whose ray deposited at iteration b, the number of the chunk's entries
+// before it whose rays were alive at b (deposited at b or later).  The
+// chunk's earlier entries are counted by every block (at most 47 x 1024
+// loads a thread-block, all in flight), the block's own with ballots.
+constexpr uint32_t WPOS_BLOCKS = RAY_CHUNK / 1024u;
+__global__ __launch_bounds__(1024) void k_dep_wpos(const uint32_t *__restrict__ counters,
+                                                   const uint32_t *__restrict__ death, uint32_t *__restrict__ wpos,
+                                                   uint32_t iters)
+{
+    __shared__ uint32_t wsum[16][MAX_BOUNCE_ITERS];
+    __shared__ uint32_t before_blk[MAX_BOUNCE_ITERS];
+    const uint32_t ns = counters[VE_CNT_SHADE], t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    const uint32_t c = blockIdx.x / WPOS_BLOCKS, k = blockIdx.x % WPOS_BLOCKS;
+    const uint32_t lo = c * RAY_CHUNK, i = lo + k * 1024u + t;
+    if (lo + k * 1024u >= ns) return;
+    const uint64_t below = (1ull << lane) - 1ull;
+    // entries of the chunk's earlier blocks alive at each iteration
+    uint32_t acc[MAX_BOUNCE_ITERS] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t q = 0; q < k; q++) {
+        const uint32_t d = death[lo + q * 1024u + t];   // earlier blocks lie below ns
+#pragma unroll
+        for (uint32_t b = 0; b < MAX_BOUNCE_ITERS; b++) acc[b] += (d != NONE && d >= b) ? 1u : 0u;
+    }
+#pragma unroll
+    for (uint32_t b = 0; b < MAX_BOUNCE_ITERS; b++) {
+        uint32_t v = acc[b];
+        for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
+        if (lane == 0u) wsum[w][b] = v;
+    }
+    __syncthreads();
+    if (t < MAX_BOUNCE_ITERS) {
+        uint32_t v = 0;
+        for (uint32_t q = 0; q < 16; q++) v += wsum[q][t];
+        before_blk[t] = v;
+    }
+    __syncthreads();
+    const bool here = i < ns;
+    const uint32_t d = here ? death[i] : NONE;
+    uint32_t mine_before = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < MAX_BOUNCE_ITERS; b++) {
+        const uint64_t alive = __ballot(here && d != NONE && d >= b);
+        if (lane == 0u) wsum[w][b] = (uint32_t)__popcll(alive);
+        if (d == b) mine_before = (uint32_t)__popcll(alive & below);
+    }
+    __syncthreads();
+    if (here && d < iters) {
+        uint32_t pos = before_blk[d] + mine_before;
+        for (uint32_t q = 0; q < w; q++) pos += wsum[q][d];
+        wpos[i] = pos;
     }
 }
 
@@ -2090,98 +2219,190 @@ __device__ void deposit_lockstep(const FrameParams &p, uint32_t *bk, uint32_t k,
         if (active[l] && cand_t[l] != NONE) lb_reset(bk, cand_i[l], vid[l], d[l], e[l], p.time);
 }
 
-constexpr uint32_t DEP_LDS_EVENTS = 512;   // deposits of one bucket sorted in LDS (more: device memory)
+// Lane-held slot of an lBuffer bucket: owner, time stamp, payload.
+struct LSlot {
+    uint32_t owner, tstamp, pl[LB_PAYW];
+};
 
-__global__ __launch_bounds__(64) void k_dep_replay(FrameParams p, uint32_t *__restrict__ lbuf,
-                                                   const uint32_t *__restrict__ off, const uint2 *__restrict__ ev,
-                                                   const Deferred *__restrict__ deferred, uint64_t *__restrict__ spill,
-                                                   uint32_t *__restrict__ cnt)
+__device__ __forceinline__ void lslot_reset(LSlot &s, uint32_t vid, uint4 d, uint4 e, uint32_t time)
+{   // resetLSlotDual (shade.comp:108-126)
+    s.owner = vid;
+    s.pl[0] = 1u; s.pl[1] = d.x; s.pl[2] = d.y; s.pl[3] = d.z; s.pl[4] = d.w;
+    s.pl[5] = 1u; s.pl[6] = e.x; s.pl[7] = e.y; s.pl[8] = e.z; s.pl[9] = e.w;
+    s.tstamp = time;
+}
+
+// One invocation's depositSampleDual into slot s (found or claimed), on the
+// lane that holds it.
+__device__ __forceinline__ void lslot_deposit(const FrameParams &p, LSlot &s, uint32_t vid, uint4 d, uint4 e)
 {
-    __shared__ uint64_t lds_keys[DEP_LDS_EVENTS];
-    __shared__ __align__(16) uint32_t bk[LB_WORDS];
-    __shared__ uint32_t g_vid[8];
-    __shared__ uint4 g_d[8], g_e[8];
-    const uint32_t h = blockIdx.x;
-    const uint32_t base = off[h], n = off[h + 1] - base;
-    if (n == 0u) return;
-    const uint32_t lane = threadIdx.x;
-    if (lane == 0u) cnt[h] = 0u;   // the next frame's count starts from zero
-    const bool in_lds = n <= DEP_LDS_EVENTS;
-    uint64_t *keys = in_lds ? lds_keys : spill + base;
-    for (uint32_t i = lane; i < n; i += 64u) {
-        const uint64_t k = ((uint64_t)ev[base + i].x << 32) | i;
-        if (in_lds) keys[i] = k;
-        else __hip_atomic_store(keys + i, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t st = s.tstamp;
+    if (s.owner == 0u) s.owner = vid;
+    if (!p.disable_stale_reset && st != 0u && st + p.update_time < p.time) {   // shade.comp:183-186
+        lslot_reset(s, vid, d, e, p.time);
+        return;
     }
-    uint32_t *gb = lbuf + (size_t)h * LB_WORDS;
-    for (uint32_t i = lane; i < LB_WORDS / 4u; i += 64u)
-        reinterpret_cast<uint4 *>(bk)[i] = reinterpret_cast<const uint4 *>(gb)[i];
-    if (!in_lds) __atomic_thread_fence(__ATOMIC_SEQ_CST);
-    __syncthreads();
-    if (in_lds) wave_sort_keys<false>(keys, n);
-    else wave_sort_keys<true>(keys, n);
-    auto key_at = [&](uint32_t k) -> uint64_t {
-        return in_lds ? keys[k] : __hip_atomic_load(keys + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    };
-    for (uint32_t k = 0; k < n;) {
-        // the group: consecutive deposits of one vector's call site (same key but the lane)
-        const uint32_t group = (uint32_t)(key_at(k) >> 35);
-        uint32_t m = 1;
-        while (k + m < n && m < 8u && (uint32_t)(key_at(k + m) >> 35) == group) m++;
-        if (lane < m) {
-            const Deferred dd = deferred[ev[base + (uint32_t)key_at(k + lane)].y];
-            g_vid[lane] = dd.vid;
-            g_d[lane] = pack_rgb_l2(f3{dd.dl[0], dd.dl[1], dd.dl[2]});
-            g_e[lane] = pack_rgb_l2(f3{dd.il[0], dd.il[1], dd.il[2]});
+    if (s.pl[0] >= p.cap_direct)   // shade.comp:188-194
+        for (uint32_t f = 0; f < 5; f++) s.pl[f] >>= 1;
+    if (s.pl[5] >= p.cap_indirect)
+        for (uint32_t f = 5; f < 10; f++) s.pl[f] >>= 1;
+    lb_firefly(s.pl, 0, d, p.firefly_k, p.firefly_floor);
+    lb_firefly(s.pl, 5, e, p.firefly_k, p.firefly_floor);
+    lb_add(s.pl, d, e);
+    s.tstamp = p.time;
+}
+
+// The order key of record `rec` (without its list index): chunk, bounce
+// iteration, SIMD-8 vector, call site, lane.
+__device__ __forceinline__ uint32_t dep_key(const Deferred *__restrict__ deferred, const uint32_t *__restrict__ wpos,
+                                            uint32_t rec)
+{
+    const uint32_t meta = deferred[rec].meta;
+    const uint32_t li = meta & 0x3FFFFu, it = (meta >> 18) & 0xFu, site = (meta >> 22) & 3u;
+    const uint32_t w = wpos[li];
+    return ((li / RAY_CHUNK) << 22) | (it << 18) | ((w >> 3) << 5) | (site << 3) | (w & 7u);
+}
+
+__global__ __launch_bounds__(64 * REPLAY_WAVES) void k_dep_replay(FrameParams p, uint32_t *__restrict__ lbuf,
+                                                                  DepLists dl, const Deferred *__restrict__ deferred,
+                                                                  const uint32_t *__restrict__ wpos,
+                                                                  uint32_t *__restrict__ counters)
+{
+    __shared__ uint64_t lds_keys_all[REPLAY_WAVES][ORDER_LDS_EVENTS];
+    __shared__ __align__(16) uint32_t bk_all[REPLAY_WAVES][LB_WORDS];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const uint32_t h = blockIdx.x * REPLAY_WAVES + wv;
+    if (h >= p.hash_width) return;
+    const uint32_t n = dl.cnt[h];
+    if (n == 0u) return;
+    if (lane == 0u) dl.cnt[h] = 0u;   // the next frame's count starts from zero
+    uint32_t *bk = bk_all[wv];
+    const uint32_t *row = dl.slots + (size_t)h * dl.cap;
+    // records: row[i] in place, then this bucket's overflow entries (low word bit 31)
+    SortedKeys sk{n, ~0ull, nullptr, false};
+    if (n <= dl.cap) {
+        sk = sort_bucket(n, [&](uint32_t i) { return dep_key(deferred, wpos, row[i]); }, lds_keys_all[wv], nullptr);
+    } else {
+        uint32_t base = 0;
+        if (lane == 0u) base = atomicAdd(&dl.over_n[1], n);
+        base = __shfl(base, 0, 64);
+        sk.keys = dl.spill + base;   // spill words for all overflowing buckets sum to at most the record count
+        sk.glob = true;
+        for (uint32_t i = lane; i < dl.cap; i += 64u)
+            __hip_atomic_store(sk.keys + i, ((uint64_t)dep_key(deferred, wpos, row[i]) << 32) | i, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t L = dl.over_n[0];
+        uint32_t put = dl.cap;
+        for (uint32_t j0 = 0; j0 < L; j0 += 64u) {
+            const uint32_t j = j0 + lane;
+            const bool m = j < L && dl.over[j].x == h;
+            const uint64_t mb = __ballot(m);
+            const uint32_t at = put + (uint32_t)__popcll(mb & ((1ull << lane) - 1ull));
+            if (m && at < n)
+                __hip_atomic_store(sk.keys + at, ((uint64_t)dep_key(deferred, wpos, dl.over[j].y) << 32) | 0x80000000u | j,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            put += (uint32_t)__popcll(mb);
         }
-        __syncthreads();
-        if (m == 1u) {   // one invocation: the first slot that is free or holds the voxel, else the LRU slot
-            const uint32_t vid = g_vid[0];
-            const bool sl = lane < LBUFFER_SLOTS;
-            const uint32_t own = sl ? bk[lane] : NONE;
-            const uint64_t hit = __ballot(sl && (own == vid || own == 0u));
-            const uint32_t t = sl ? bk[LB_TIME + lane] : NONE;
-            uint32_t best = t;
-            for (int o = 32; o > 0; o >>= 1) best = min(best, (uint32_t)__shfl_xor((int)best, o, 64));
-            const uint64_t oldest = __ballot(sl && t == best);
-            if (lane == 0u) {
-                uint4 d = g_d[0], e = g_e[0];
-                if (hit) {
-                    const uint32_t s = (uint32_t)__ffsll((unsigned long long)hit) - 1u;
-                    if (bk[s] == 0u) bk[s] = vid;
-                    const uint32_t st = bk[LB_TIME + s];
-                    uint32_t *pl = bk + LB_PAY + s * LB_PAYW;
-                    if (!p.disable_stale_reset && st != 0u && st + p.update_time < p.time) {
-                        lb_reset(bk, s, vid, d, e, p.time);
-                    } else {
-                        if (pl[0] >= p.cap_direct)
-                            for (uint32_t f = 0; f < 5; f++) pl[f] >>= 1;
-                        if (pl[5] >= p.cap_indirect)
-                            for (uint32_t f = 5; f < 10; f++) pl[f] >>= 1;
-                        lb_firefly(pl, 0, d, p.firefly_k, p.firefly_floor);
-                        lb_firefly(pl, 5, e, p.firefly_k, p.firefly_floor);
-                        lb_add(pl, d, e);
-                        bk[LB_TIME + s] = p.time;
-                    }
-                } else if (best != NONE) {
-                    lb_reset(bk, (uint32_t)__ffsll((unsigned long long)oldest) - 1u, vid, d, e, p.time);
-                }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+        wave_sort_keys<true>(sk.keys, n);
+    }
+    const bool sl = lane < LBUFFER_SLOTS;
+    uint32_t *gb = lbuf + (size_t)h * LB_WORDS;
+    LSlot st{NONE, NONE, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+    if (sl) {
+        st.owner = gb[lane];
+        st.tstamp = gb[LB_TIME + lane];
+#pragma unroll
+        for (uint32_t f = 0; f < LB_PAYW; f++) st.pl[f] = gb[LB_PAY + lane * LB_PAYW + f];
+    }
+    // this lane's record of the window [w0, w0 + 64)
+    uint32_t w0 = 0, r_grp = NONE, r_vid = 0u;
+    uint4 r_d = make_uint4(0u, 0u, 0u, 0u), r_e = r_d;
+    auto load_window = [&](uint32_t at) {
+        w0 = at;
+        const uint32_t q = at + lane;
+        r_grp = NONE;
+        if (q < n) {
+            const uint64_t key = sk.at(q);   // n <= 64: q is this lane, which holds the q-th key
+            const uint32_t lo = (uint32_t)key;
+            const uint32_t rec = (lo & 0x80000000u) ? dl.over[lo & 0x7FFFFFFFu].y : row[lo];
+            const Deferred dd = deferred[rec];
+            r_grp = (uint32_t)(key >> 35);   // the order key without its lane bits
+            r_vid = dd.vid;
+            r_d = pack_rgb_l2(f3{dd.dl[0], dd.dl[1], dd.dl[2]});
+            r_e = pack_rgb_l2(f3{dd.il[0], dd.il[1], dd.il[2]});
+        }
+    };
+    load_window(0);
+    for (uint32_t k = 0; k < n;) {
+        if (k + 8u > w0 + 64u && w0 + 64u < n) load_window(k);   // a whole group inside the window
+        const uint32_t j = k - w0;
+        const uint32_t g = (uint32_t)__builtin_amdgcn_readlane((int)r_grp, (int)j);
+        uint32_t m = 1;
+        while (m < 8u && k + m < n && (uint32_t)__builtin_amdgcn_readlane((int)r_grp, (int)(j + m)) == g) m++;
+        if (m == 1u) {
+            const uint32_t vid = (uint32_t)__builtin_amdgcn_readlane((int)r_vid, (int)j);
+            const uint4 d = make_uint4((uint32_t)__builtin_amdgcn_readlane((int)r_d.x, (int)j),
+                                       (uint32_t)__builtin_amdgcn_readlane((int)r_d.y, (int)j),
+                                       (uint32_t)__builtin_amdgcn_readlane((int)r_d.z, (int)j),
+                                       (uint32_t)__builtin_amdgcn_readlane((int)r_d.w, (int)j));
+            const uint4 e = make_uint4((uint32_t)__builtin_amdgcn_readlane((int)r_e.x, (int)j),
+                                       (uint32_t)__builtin_amdgcn_readlane((int)r_e.y, (int)j),
+                                       (uint32_t)__builtin_amdgcn_readlane((int)r_e.z, (int)j),
+                                       (uint32_t)__builtin_amdgcn_readlane((int)r_e.w, (int)j));
+            const uint64_t hit = __ballot(sl && (st.owner == vid || st.owner == 0u));
+            if (hit) {
+                if (lane == (uint32_t)__ffsll((unsigned long long)hit) - 1u) lslot_deposit(p, st, vid, d, e);
+            } else {   // every slot holds another voxel: reset the least recently stamped one
+                const uint32_t best = wave_min_u32(sl ? st.tstamp : NONE);
+                const uint64_t at = __ballot(sl && st.tstamp == best);
+                if (best != NONE && lane == (uint32_t)__ffsll((unsigned long long)at) - 1u)
+                    lslot_reset(st, vid, d, e, p.time);
             }
-        } else if (lane == 0u) {
+        } else {   // several lanes of one vector here: lockstep, on an LDS copy
+            if (sl) {
+                bk[lane] = st.owner;
+                bk[LB_TIME + lane] = st.tstamp;
+                for (uint32_t f = 0; f < LB_PAYW; f++) bk[LB_PAY + lane * LB_PAYW + f] = st.pl[f];
+            }
             uint32_t vids[8];
             uint4 ds[8], es[8];
             for (uint32_t l = 0; l < m; l++) {
-                vids[l] = g_vid[l];
-                ds[l] = g_d[l];
-                es[l] = g_e[l];
+                const int src = (int)(j + l);
+                vids[l] = (uint32_t)__builtin_amdgcn_readlane((int)r_vid, src);
+                ds[l] = make_uint4((uint32_t)__builtin_amdgcn_readlane((int)r_d.x, src),
+                                   (uint32_t)__builtin_amdgcn_readlane((int)r_d.y, src),
+                                   (uint32_t)__builtin_amdgcn_readlane((int)r_d.z, src),
+                                   (uint32_t)__builtin_amdgcn_readlane((int)r_d.w, src));
+                es[l] = make_uint4((uint32_t)__builtin_amdgcn_readlane((int)r_e.x, src),
+                                   (uint32_t)__builtin_amdgcn_readlane((int)r_e.y, src),
+                                   (uint32_t)__builtin_amdgcn_readlane((int)r_e.z, src),
+                                   (uint32_t)__builtin_amdgcn_readlane((int)r_e.w, src));
             }
-            deposit_lockstep(p, bk, m, vids, ds, es);
+            wave_lds_sync();
+            if (lane == 0u) deposit_lockstep(p, bk, m, vids, ds, es);
+            wave_lds_sync();
+            if (sl) {
+                st.owner = bk[lane];
+                st.tstamp = bk[LB_TIME + lane];
+                for (uint32_t f = 0; f < LB_PAYW; f++) st.pl[f] = bk[LB_PAY + lane * LB_PAYW + f];
+            }
+            wave_lds_sync();
         }
-        __syncthreads();
         k += m;
     }
-    for (uint32_t i = lane; i < LB_WORDS / 4u; i += 64u)
-        reinterpret_cast<uint4 *>(gb)[i] = reinterpret_cast<const uint4 *>(bk)[i];
+    if (sl) {
+        gb[lane] = st.owner;
+        gb[LB_TIME + lane] = st.tstamp;
+#pragma unroll
+        for (uint32_t f = 0; f < LB_PAYW; f++) gb[LB_PAY + lane * LB_PAYW + f] = st.pl[f];
+    }
+}
+
+__global__ void k_dep_done(uint32_t *__restrict__ counters, uint32_t *__restrict__ over_n)
+{
+    counters[VE_CNT_DEFERRED] = 0u;   // the frame's records are applied
+    over_n[0] = over_n[1] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -2592,15 +2813,20 @@ void launch_fill(hipStream_t s, uint32_t *p, uint32_t v, size_t n)
     if (n) launch(k_fill_u32, dim3(cdiv(n, 256)), dim3(256), 0, s, p, v, n);
 }
 
+static OrderLists nrm_lists(const DeviceBuffers &b)
+{
+    return OrderLists{b.ord_cap, b.ord_over_cap, b.ord_cnt, b.ord_ev, b.ord_over, b.ord_over_n, b.ord_spill,
+                      b.ord_spill_cap};
+}
+
 void launch_normal(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b)
 {
-    launch(k_nrm_events<false>, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.gpos, b.ord_cnt, b.ord_off,
-           b.ord_ev);
-    launch(k_order_scan, dim3(1), dim3(1024), 0, s, b.ord_cnt, b.ord_off, p.hash_width);
-    launch(k_nrm_events<true>, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.gpos, b.ord_cnt, b.ord_off,
-           b.ord_ev);
-    launch(k_nrm_replay, dim3(p.hash_width), dim3(64), 0, s, words, p, b.nbuf, b.ord_off, b.ord_ev, b.ord_spill,
-           b.ord_cnt);
+    const OrderLists ol = nrm_lists(b);
+    launch(k_nrm_events, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.gpos, ol, b.counters);
+    launch(k_nrm_replay, dim3(cdiv(p.hash_width, REPLAY_WAVES)), dim3(64 * REPLAY_WAVES), 0, s, p, b.nbuf, ol,
+           b.ord_work, b.counters);
+    launch(k_nrm_compute, dim3(cdiv((size_t)p.hash_width * NBUFFER_SLOTS, 256)), dim3(256), 0, s, words, p, b.nbuf,
+           b.ord_work, b.ord_over_n);
 }
 
 // Per-frame clear of RankStats' per-frame half and the counters the frame
@@ -2718,14 +2944,13 @@ void launch_bounce(hipStream_t s, const uint32_t *words, const FrameParams &p, D
 int launch_deposits(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_t iters)
 {
     const uint32_t chunks = (p.shade_budget + RAY_CHUNK - 1u) / RAY_CHUNK;
+    const DepLists dl{DEP_CAP, b.dep_cnt, b.dep_slots, b.dep_over, b.dep_over_n, b.dep_spill, SHADE_BUDGET_MAX};
     launch(k_dep_prep, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, p, b.deferred, b.counters, b.bounce_part, b.ray_totals,
-           b.dep_death, b.dep_cnt);
-    launch(k_dep_wpos, dim3(chunks, iters > 0u ? iters : 1u), dim3(1024), 0, s, b.counters, b.dep_death, b.dep_wpos);
-    launch(k_order_scan, dim3(1), dim3(1024), 0, s, b.dep_cnt, b.dep_off, p.hash_width);
-    launch(k_dep_place, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, p, b.deferred, b.counters, b.dep_wpos, b.dep_cnt,
-           b.dep_off, b.dep_ev);
-    launch(k_dep_replay, dim3(p.hash_width), dim3(64), 0, s, p, b.lbuf, b.dep_off, b.dep_ev, b.deferred, b.dep_spill,
-           b.dep_cnt);
+           b.dep_death, dl);
+    launch(k_dep_wpos, dim3(chunks * WPOS_BLOCKS), dim3(1024), 0, s, b.counters, b.dep_death, b.dep_wpos, iters);
+    launch(k_dep_replay, dim3(cdiv(p.hash_width, REPLAY_WAVES)), dim3(64 * REPLAY_WAVES), 0, s, p, b.lbuf, dl,
+           b.deferred, b.dep_wpos, b.counters);
+    launch(k_dep_done, dim3(1), dim3(1), 0, s, b.counters, b.dep_over_n);
     return VE_OK;
 }
 

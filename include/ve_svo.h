@@ -202,6 +202,24 @@ typedef enum ve_pass_id {
 int ve_pass(ve_ctx *ctx, ve_scene *scene, ve_pass_id pass, const ve_frame_config *fc,
             const ve_frame_inputs *in);
 
+/* No counterpart in the reference: the VE_PASS_PRIMARY dispatch of ve_pass
+ * (generate_primary.comp) for callers that cannot allocate device memory --
+ * code under HIP graph capture, serving loops that forbid hipMalloc.  It needs
+ * no context or scene: `words` (n_words node words) and the outputs are the
+ * caller's device buffers (gbuffer: uint32 x4 per pixel as VE_BUF_GBUFFER,
+ * gbuffer_pos: uint32 per pixel as VE_BUF_GBUFFER_POS), `ubo` is the host
+ * camera of ve_set_camera, `stream` a hipStream_t (NULL = the null stream).
+ * The traversal jump table is rebuilt on every call in a static per-device
+ * scratch array of 1 MiB + 256 bytes (a depth-min(6, depth-1) table; the
+ * call fails with VE_ECAPACITY if a table would not fit), so the call makes
+ * no hipMalloc, hipFree or other allocation; results equal ve_pass's bit for
+ * bit.  Calls that overlap on one device share that scratch and race.
+ * Asynchronous: returns after enqueueing; errors of the arguments return
+ * VE_EINVAL with ve_last_error(NULL). */
+int ve_pass_primary_noalloc(const uint32_t *words, uint32_t n_words, uint32_t depth, const float ubo[16],
+                            uint32_t width, uint32_t height, int primary_controlchecks, uint32_t *gbuffer,
+                            uint32_t *gbuffer_pos, void *stream);
+
 /* Buffers readable / writable by id (device copies, synchronous). */
 typedef enum ve_buffer_id {
     VE_BUF_GBUFFER = 0,    /* uint32 x4 per pixel: checks, material, 0, voxelKey+1 */

@@ -179,6 +179,7 @@ def lib():
         "ve_server_read_output": (i32, [p, u32, p, sz]),
         "ve_server_get_stats": (i32, [p, C.POINTER(ServerStats)]),
         "ve_server_read": (i32, [p, u32, C.c_int, p, sz]),
+        "ve_pass_primary_noalloc": (i32, [p, u32, u32, p, u32, u32, i32, p, p, p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -364,6 +365,17 @@ def camera_ubo(position, direction, fov_deg, aspect):
     out = np.zeros(16, np.float32)
     lib().ve_camera_ubo(pos.ctypes.data, d.ctypes.data, float(fov_deg), float(aspect), out.ctypes.data)
     return out
+
+
+def primary_noalloc(words_dev, n_words, depth, ubo, width, height, controlchecks, gbuffer_dev, gbuffer_pos_dev,
+                    stream=None):
+    """ve_pass_primary_noalloc: the primary pass over device buffers given as
+    addresses (words, G-buffer uint32 x4 per pixel, positions uint32 per
+    pixel), with no context and no device allocation; enqueued on `stream`
+    (a hipStream_t address, None = the null stream)."""
+    ubo = np.ascontiguousarray(ubo, np.float32)
+    _check(lib().ve_pass_primary_noalloc(words_dev, n_words, depth, ubo.ctypes.data, width, height, controlchecks,
+                                         gbuffer_dev, gbuffer_pos_dev, stream))
 
 
 class Renderer:

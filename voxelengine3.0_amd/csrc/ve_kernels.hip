@@ -10,6 +10,10 @@
 // contiguous band of tiles so the nodes a band touches stay in that XCD's L2.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
 #include "ve_device.h"
 #include "ve_launch.h"
 
@@ -135,4 +139,92 @@ void launch_build_jump(hipStream_t s, const uint32_t *words, uint32_t n_words, u
     launch(k_build_jump, dim3((cells + 255u) / 256u), dim3(256), 0, s, words, n_words, depth, level, jump);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free primary pass
+// ---------------------------------------------------------------------------
+// The jump table of ve_pass_primary_noalloc lives in a file-scope device
+// array instead of the scene: a depth-min(6, depth-1) table (at most 8^6
+// words, 1 MiB) in `scratch`, after 64 words reserved for the kernel's
+// counter arguments.  Each device holds its own copy; its address comes from
+// hipGetSymbolAddress on the first call on that device (the array's name is
+// a host-side stand-in, never a device address).  Every call on a device
+// rebuilds the table over the same memory, so calls that overlap on one
+// device race.
+constexpr uint32_t NOALLOC_JUMP_LEVEL = 6;
+constexpr size_t NOALLOC_HEAD_WORDS = 64;   // counters (16), node stats (8), ray totals (8), spare
+constexpr size_t NOALLOC_SCRATCH_WORDS = NOALLOC_HEAD_WORDS + ((size_t)1 << (3u * NOALLOC_JUMP_LEVEL));
+static __device__ __align__(256) uint32_t scratch[NOALLOC_SCRATCH_WORDS];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex noalloc_mutex;
+static uint32_t *noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+static int noalloc_scratch(int dev, uint32_t **out)
+{
+    std::lock_guard<std::mutex> lock(noalloc_mutex);
+    if (!noalloc_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            char buf[256];
+            snprintf(buf, sizeof buf, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+            return ve_global_error(VE_EDEVICE, buf);
+        }
+        noalloc_ptr[dev] = static_cast<uint32_t *>(ptr);
+    }
+    *out = noalloc_ptr[dev];
+    return VE_OK;
+}
+
 }  // namespace ve
+
+extern "C" int ve_pass_primary_noalloc(const uint32_t *words, uint32_t n_words, uint32_t depth, const float ubo[16],
+                                       uint32_t width, uint32_t height, int primary_controlchecks, uint32_t *gbuffer,
+                                       uint32_t *gbuffer_pos, void *stream)
+{
+    using namespace ve;
+    ve_clear_global_error();
+    if (!words || !ubo || !gbuffer || !gbuffer_pos) return ve_global_error(VE_EINVAL, "null argument");
+    if (n_words == 0u || n_words > (1u << 28)) return ve_global_error(VE_EINVAL, "n_words outside [1, 2^28]");
+    if (depth < 1u || depth > VE_MAX_DEPTH) return ve_global_error(VE_EINVAL, "depth out of range [1,16]");
+    if (width < 1u || height < 1u || width > 16384u || height > 16384u)
+        return ve_global_error(VE_EINVAL, "framebuffer size outside [1, 16384]");
+    const uint32_t jl = depth - 1u < NOALLOC_JUMP_LEVEL ? depth - 1u : NOALLOC_JUMP_LEVEL;
+    const size_t need = NOALLOC_HEAD_WORDS + ((size_t)1 << (3u * jl));
+    if (need > NOALLOC_SCRATCH_WORDS) {
+        char buf[160];
+        snprintf(buf, sizeof buf, "jump table needs %zu scratch words, above the %zu available", need,
+                 NOALLOC_SCRATCH_WORDS);
+        return ve_global_error(VE_ECAPACITY, buf);
+    }
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return ve_global_error(VE_EDEVICE, hipGetErrorString(e));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return ve_global_error(VE_EDEVICE, "device ordinal above 63");
+    uint32_t *sp = nullptr;
+    const int rc = noalloc_scratch(dev, &sp);
+    if (rc != VE_OK) return rc;
+    FrameParams p;
+    std::memset(&p, 0, sizeof p);
+    p.W = width;
+    p.H = height;
+    p.depth = depth;
+    p.n_words = n_words;
+    uint32_t *const jump = sp + NOALLOC_HEAD_WORDS;
+    p.jump = jump;
+    p.jump_level = jl;
+    p.primary_cc = (uint32_t)primary_controlchecks;
+    p.count_primary_in_reset = 1;   // the ray counters are the context's, not written here
+    p.reuse_primary_hits = 0;       // no hit records kept
+    std::memcpy(p.cam, ubo, sizeof p.cam);
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    launch_build_jump(s, words, n_words, depth, jl, jump);
+    const uint32_t tiles = ((width + TILE - 1) / TILE) * ((height + TILE - 1) / TILE);
+    const uint32_t nb = padded_blocks((tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
+    launch(k_primary<false>, dim3(nb), dim3(256), 0, s, words, p, reinterpret_cast<uint4 *>(gbuffer), gbuffer_pos,
+           reinterpret_cast<unsigned long long *>(sp + 16), sp, reinterpret_cast<unsigned long long *>(sp + 24),
+           static_cast<uint2 *>(nullptr));
+    e = hipGetLastError();
+    if (e != hipSuccess) return ve_global_error(VE_EDEVICE, hipGetErrorString(e));
+    return VE_OK;
+}

@@ -86,30 +86,64 @@ def frame_seed(f):
     return (1 + 7919 * f) & 0xFFFFFFFF
 
 
+def host_cpu():
+    """(threads this process may use, capped at the 16 a one-GPU share
+    allows; nproc; CPU model)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return max(1, min(n, 16)), os.cpu_count(), model
+
+
 def cpu_baseline(words, depth, ubos, fc_kw, budget_s=20.0):
-    """C oracle (scalar port) on the host: whole frames of the same workload,
-    as many as fit in ~budget_s seconds (at least one)."""
+    """C oracle (a scalar port of the same passes) on the host cores: one
+    renderer per thread, each accumulating its own orbit sequence of whole
+    frames of the same workload, until ~budget_s seconds have passed (at least
+    one frame each); value = all threads' rays / wall time."""
     import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
     from oracle import oracle as O
     from voxelengine_amd import scenes
-    of = O.Frame(W, H, 16384, BOUNCES + 1)
-    of.set_materials(scenes.palette(1))
-    of.set_sky_rgb8(SKY)
+    threads, nproc, model = host_cpu()
+    words = np.ascontiguousarray(words, np.uint32)
     fc = O.FrameConfig(**fc_kw)
-    rays, frames = 0, 0
     t0 = time.perf_counter()
-    while True:
-        f = frames % len(ubos)
-        of.frame(words, depth, ubos[f], fc, f, frame_seed(f), f + 1)
-        c = of.counters()
-        rays += int(c[6]) + int(c[4]) + int(c[5])
-        frames += 1
-        if time.perf_counter() - t0 > budget_s or frames >= 8:
-            break
+
+    def worker(t):
+        of = O.Frame(W, H, 16384, BOUNCES + 1)
+        try:
+            of.set_materials(scenes.palette(1))
+            of.set_sky_rgb8(SKY)
+            rays = frames = 0
+            while True:
+                i = t * 7 + frames
+                f = i % len(ubos)
+                of.frame(words, depth, ubos[f], fc, i, frame_seed(i), i + 1)   # the C call releases the GIL
+                c = of.counters()
+                rays += int(c[6]) + int(c[4]) + int(c[5])
+                frames += 1
+                if time.perf_counter() - t0 > budget_s:
+                    return rays, frames
+        finally:
+            of.close()
+
+    with ThreadPoolExecutor(threads) as ex:
+        res = list(ex.map(worker, range(threads)))
     dt = time.perf_counter() - t0
-    of.close()
-    return dict(value=rays / dt / 1e6, unit="Mrays/s", cores=1, kind="port",
-                sample="%d whole frame(s) of the same 1080p workload, single thread, %.1f s" % (frames, dt))
+    rays, frames = sum(r for r, _ in res), sum(f for _, f in res)
+    return dict(value=rays / dt / 1e6, unit="Mrays/s", cores=threads, kind="port", nproc=nproc, cpu=model,
+                sample="%d whole frames of the same 1080p workload, %d threads (one renderer each), %.1f s"
+                       % (frames, threads, dt))
 
 
 # offered loads (frames/s) of the served sweep, per mode (each row also
@@ -153,11 +187,30 @@ def divergence(ve, caves, frames=32, warmup=4):
         t = r.ray_totals()
         rays = t["primary"] + t["traced"] + t["shadow"]
         c = r.counters()
+        # per-ray traversal figures: the same frames again with node counting
+        # (which traverses every bounce-0 ray too, as the reference does)
+        n_st = 4
+        r.enable_node_stats(True)
+        r.reset_ray_totals()
+        for i in range(warmup, warmup + n_st):
+            frame(i)
+        r.sync()
+        st = r.node_stats()
+        ts = r.ray_totals()
+        r.enable_node_stats(False)
+        prim, traced = ts["primary"], ts["traced"]
+        per_ray = dict(primary_steps_per_ray=round(st["primary_steps"] / prim, 3),
+                       primary_node_bytes_per_ray=round(4.0 * st["primary_reads"] / prim, 2),
+                       trace_steps_per_ray=round(st["trace_steps"] / max(traced, 1), 3),
+                       trace_node_bytes_per_ray=round(4.0 * st["trace_reads"] / max(traced, 1), 2),
+                       steps_per_ray=round((st["primary_steps"] + st["trace_steps"]) / (prim + traced), 3),
+                       node_bytes_per_ray=round(4.0 * (st["primary_reads"] + st["trace_reads"]) / (prim + traced), 2),
+                       per_ray_frames=n_st)
         out = dict(workload="svo-depth12-caves-1440p-4spp-4bounce-orbit32", voxels=int(info.num_voxels),
                    candidates_per_frame=int(c[ve.CNT["CANDIDATES"]]), claim_order_exact=bool(c[ve.CNT["OVERFLOW"]] == 0),
                    node_words=int(info.size), frames=frames, ms_per_frame=round(dt / frames * 1e3, 3),
                    mrays_per_s=round(rays / dt / 1e6, 1), rays_per_frame=round(rays / frames, 1),
-                   traced_rays_per_frame=round(t["traced"] / frames, 1))
+                   traced_rays_per_frame=round(t["traced"] / frames, 1), **per_ray)
         r.release(scene)
         return out
     finally:
@@ -207,6 +260,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-served", action="store_true",
                     help="skip the served (Poisson sessions) and batched (64 viewports) measurements")
+    ap.add_argument("--no-divergence", action="store_true",
+                    help="skip the depth-12 divergence configuration (and, with it, the batched set's cave scene)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device copy (4 launches), the reference kernel for scaling "
                          "the FETCH_SIZE / WRITE_SIZE counters")
@@ -283,12 +338,13 @@ def main():
         for i in range(args.warmup, args.warmup + args.steps):
             frame(i)
         r.sync()
+        later = r.timings_iter(1)   # the fused kernel of bounce iterations >= 1, under TRACE
         ms = r.timings()
         r.enable_timing(False)
-        return ms
+        return ms, later
 
-    pass_ms = timed_frames(True)
-    pass_ms_serial = timed_frames(False)
+    pass_ms, later_ms = timed_frames(True)
+    pass_ms_serial, later_ms_serial = timed_frames(False)
     r.set_overlap(True)
 
     # algorithmic node traffic of the primary kernel on the same frames
@@ -298,6 +354,14 @@ def main():
         r.set_camera(ubos[f])
         r.run_pass(scene, "PRIMARY", fc, ve.FrameInputs(W, H, i, frame_seed(i), i + 1))
     st = r.node_stats()
+    # the fused bounce kernel's node traffic: whole frames with node counting
+    n_bst = min(args.steps, 8)
+    r.enable_node_stats(True)
+    for i in range(args.warmup, args.warmup + n_bst):
+        frame(i)
+    r.sync()
+    bst = r.node_stats()
+    cb = r.counters()
     r.enable_node_stats(False)
     n_stat = min(args.steps, ORBIT_FRAMES)
     bytes_per_launch = st["primary_reads"] * 4.0 / n_stat
@@ -305,6 +369,25 @@ def main():
     achieved = bytes_per_launch / (prim_ms * 1e-3) / 1e9 if prim_ms > 0 else 0.0
     iso_ms = pass_ms_serial["PRIMARY"]
     iso_achieved = bytes_per_launch / (iso_ms * 1e-3) / 1e9 if iso_ms > 0 else 0.0
+    # the fused bounce kernel (trace + shade of iterations >= 1 per ray)
+    b_bytes = bst["bounce_reads"] * 4.0 / n_bst
+    b_ms, b_ms_iso = later_ms["TRACE"], later_ms_serial["TRACE"]
+    b_rays = totals["traced"] / args.steps - int(cb[ve.CNT["SHADE"]])
+    bounce_kernel = {
+        "kernel": "k_bounce (trace.comp + shade.comp, bounce iterations >= 1, fused per ray)",
+        "bound": "hbm",
+        "algorithmic_bytes_per_launch": round(b_bytes),
+        "rays_per_launch": round(b_rays, 1),
+        "node_reads_per_ray": round(bst["bounce_reads"] / n_bst / max(b_rays, 1.0), 3),
+        "steps_per_ray": round(bst["bounce_steps"] / n_bst / max(b_rays, 1.0), 3),
+        "launch_ms": round(b_ms, 4),
+        "achieved": round(b_bytes / (b_ms * 1e-3) / 1e9, 1) if b_ms > 0 else None,
+        "frac": round(b_bytes / (b_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if b_ms > 0 else None,
+        "isolated_launch_ms": round(b_ms_iso, 4),
+        "isolated_frac": round(b_bytes / (b_ms_iso * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if b_ms_iso > 0 else None,
+        "traffic": profiled_traffic("k_bounce"),
+        "grays_per_s": round(b_rays / (b_ms * 1e-3) / 1e9, 3) if b_ms > 0 else None,
+    }
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -316,14 +399,17 @@ def main():
     r.close()
     del ubos
     from voxelengine_amd import scenes as S
-    t0 = time.perf_counter()
-    cdepth, cops = S.scene_caves(1)
-    cwords, cinfo = ve.octree_build(cops, cdepth, max_words=1 << 28)
-    t_caves = time.perf_counter() - t0
-    del cops
-    caves = (cdepth, cwords, cinfo)
-    div = divergence(ve, caves)
-    div["scene_build_s"] = round(t_caves, 2)
+    div = caves = None
+    if not (args.no_divergence and args.no_served):
+        t0 = time.perf_counter()
+        cdepth, cops = S.scene_caves(1)
+        cwords, cinfo = ve.octree_build(cops, cdepth, max_words=1 << 28)
+        t_caves = time.perf_counter() - t0
+        del cops
+        caves = (cdepth, cwords, cinfo)
+    if not args.no_divergence:
+        div = divergence(ve, caves)
+        div["scene_build_s"] = round(t_caves, 2)
 
     served = batched = None
     if not args.no_served:
@@ -379,6 +465,7 @@ def main():
             "node_reads_per_ray": round(st["primary_reads"] / n_stat / (W * H), 3),
             "steps_per_ray": round(st["primary_steps"] / n_stat / (W * H), 3),
         },
+        "bounce_kernel": bounce_kernel,
         "cpu_baseline": cpu,
         "divergence": div,
         "served": served,

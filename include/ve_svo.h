@@ -115,6 +115,20 @@ int ve_scene_upload(ve_ctx *ctx, const uint32_t *words, uint32_t n_words, uint32
                     ve_scene **scene);
 int ve_scene_release(ve_ctx *ctx, ve_scene *scene);
 
+/* No counterpart in the reference: launch-shape knobs, which change speed
+ * only, never results.  tile_chunk: how the 8x8-pixel tiles of the screen
+ * passes are dealt to the chip's 8 XCDs -- 0 gives each XCD one contiguous
+ * band of rows; n > 0 (at most 4096; default 16) deals runs of n groups of
+ * 4 tiles to the XCDs in turn.  jump_level: depth of the traversal jump
+ * table of scenes uploaded on this context from now on (0 = default 8;
+ * 1..8: 8^level words). */
+typedef struct ve_tuning {
+    uint32_t tile_chunk;
+    uint32_t jump_level;
+} ve_tuning;
+int ve_set_tuning(ve_ctx *ctx, const ve_tuning *t);
+int ve_get_tuning(ve_ctx *ctx, ve_tuning *t);
+
 /* 64-byte std140 CameraUniform (camera.cpp:39-50): position, cameraPlane
  * (forward), cameraPlaneRight, cameraPlaneUp, each a vec4 with w = 0. */
 int ve_set_camera(ve_ctx *ctx, const float ubo[16]);
@@ -286,6 +300,12 @@ int ve_set_overlap(ve_ctx *ctx, int enable);
  * are created up front. */
 int ve_enable_timing(ve_ctx *ctx, int enable);
 int ve_timings(ve_ctx *ctx, float ms_out[VE_PASS_COUNT]);
+/* No counterpart in the reference: the same means split by bounce iteration,
+ * read before ve_timings (which starts a new recording).  iter 0: the first
+ * bounce's trace and shade passes and every other pass; iter 1: the fused
+ * kernel of the later iterations under VE_PASS_TRACE and the frame's ordered
+ * deposits under VE_PASS_SHADE (0 elsewhere). */
+int ve_timings_iter(ve_ctx *ctx, uint32_t iter, float ms_out[VE_PASS_COUNT]);
 
 /* Cumulative ray counts since the context was created (or ve_reset_ray_totals):
  * primary rays (W*H per primary pass), rays traced by the bounce passes (every
@@ -306,6 +326,9 @@ int ve_reset_ray_totals(ve_ctx *ctx);
 int ve_enable_node_stats(ve_ctx *ctx, int enable);
 int ve_node_stats(ve_ctx *ctx, uint64_t *primary_reads, uint64_t *trace_reads,
                   uint64_t *primary_steps, uint64_t *trace_steps);
+/* No counterpart in the reference: the share of trace_reads / trace_steps made
+ * by the fused kernel of bounce iterations 1 and later. */
+int ve_node_stats_bounce(ve_ctx *ctx, uint64_t *reads, uint64_t *steps);
 
 /* ---- editable octree: OctreeCPU + Octree::set / applyEdits -------------------
  * The persistent pointer tree of OctreeCPU (src/octree_cpu.hpp:9-60) with the

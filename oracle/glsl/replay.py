@@ -211,9 +211,15 @@ class GLPipeline:
         return int(self.shade_meta.read()[0])
 
     # --- 4+5. emit / trace / shade chunk loop (renderer.cpp:476-606)
-    def bounces(self, fc, scheduled, frame_seed, frame_idx, time):
+    def bounces(self, fc, scheduled, frame_seed, frame_idx, time, count_live=False):
+        """Returns (iterations run, rays still bouncing after their trace
+        dispatch, rays in all trace dispatch windows, and -- with count_live --
+        rays that each trace dispatch traverses: its window's rays whose
+        bounce is below MAX_BOUNCES, trace.comp:36)."""
         bounces_run = 0
         traced = 0
+        window = 0
+        live_in = 0
         for base in range(0, scheduled, RAY_CAPACITY_HOST):
             size = min(scheduled - base, RAY_CAPACITY_HOST)
             self.ring_meta.write(np.zeros(4, np.uint32))
@@ -232,6 +238,10 @@ class GLPipeline:
                 if head == tail:
                     break
                 win = head - tail
+                window += win
+                if count_live:
+                    ring = self.ring.read(np.uint32).reshape(-1, 24)
+                    live_in += int((ring[(tail + np.arange(win)) % RAY_CAPACITY_HOST, 19] < self.max_bounces).sum())
                 tr = self.prog("trace.comp")
                 tr.use()
                 self.tbo.bind(tr, "octreeTexture", 0)
@@ -279,7 +289,7 @@ class GLPipeline:
                 sh.dispatch((win + 63) // 64)
                 self.ring_meta.write(np.array([head], np.uint32), offset=4)
                 bounces_run = max(bounces_run, b + 1)
-        return bounces_run, traced
+        return bounces_run, traced, window, live_in
 
     # --- 6. resolve.comp (renderer.cpp:612-648)
     def resolve(self, fc):
@@ -304,7 +314,7 @@ class GLPipeline:
         p.set1f("varianceNormForView", fc["variance_norm"])
         p.dispatch(*self.groups2d())
 
-    def frame(self, fc, frame_idx, frame_seed, time):
+    def frame(self, fc, frame_idx, frame_seed, time, count_live=False):
         """One Renderer::runWavefrontFrame; returns the per-frame counters."""
         budget = max(1, min(int(fc["shading_budget"]), SHADE_BUDGET_MAX))
         self.primary(fc["primary_controlchecks"])
@@ -315,10 +325,10 @@ class GLPipeline:
         if cand > 0:
             scheduled = self.rank_threshold(fc, cand, budget)
         scheduled = min(scheduled, budget)
-        bounces_run, traced = self.bounces(fc, scheduled, frame_seed, frame_idx, time)
+        bounces_run, traced, window, live_in = self.bounces(fc, scheduled, frame_seed, frame_idx, time, count_live)
         self.resolve(fc)
-        return dict(candidates=cand, scheduled=scheduled, bounces=bounces_run, traced=traced,
-                    emissives=int(self.emissive_meta.read()[0]))
+        return dict(candidates=cand, scheduled=scheduled, bounces=bounces_run, traced=traced, window=window,
+                    live_in=live_in, emissives=int(self.emissive_meta.read()[0]))
 
     def read_hash(self, img, height):
         """Image (x = bucket, y = field) -> bucket-major [bucket][field]."""

@@ -34,6 +34,7 @@ def lib():
                                           C.POINTER(C.c_uint64), C.c_int]
         L.so_raycast.restype = SoHit
         L.so_raycast.argtypes = [p, u32, u32, p, p, p, u32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]
+        L.so_retrace_f64.argtypes = [p, u32, u32, p, u32, u32, u32, p, C.c_size_t, p]
         _lib = L
     return _lib
 
@@ -53,6 +54,22 @@ def raycast(words, depth, origin, direction, controlchecks, n_words=None):
     h = lib().so_raycast(w.ctypes.data, len(w) if n_words is None else n_words, depth, o.ctypes.data, d.ctypes.data,
                          inv.ctypes.data, controlchecks, C.byref(checks), None)
     return bool(h.hit), tuple(h.pos), h.material, checks.value
+
+
+RETRACE_DTYPE = np.dtype([("key32", "<u4"), ("key64", "<u4"), ("steps32", "<u4"), ("steps64", "<u4"),
+                          ("diverge_step", "<u4"), ("pad", "<u4"), ("face_dist", "<f8"), ("pos_diff", "<f8")])
+
+
+def retrace_f64(words, depth, ubo, W, H, controlchecks, pixels, n_words=None):
+    """FP64 twin of Raycast for the primary rays of `pixels` (y*W+x) beside
+    the FP32 walk: a RETRACE_DTYPE record per pixel."""
+    w = np.ascontiguousarray(words, np.uint32)
+    u = np.ascontiguousarray(ubo, np.float32)
+    px = np.ascontiguousarray(pixels, np.uint32)
+    out = np.zeros(len(px), RETRACE_DTYPE)
+    lib().so_retrace_f64(w.ctypes.data, len(w) if n_words is None else n_words, depth, u.ctypes.data, W, H,
+                         controlchecks, px.ctypes.data, len(px), out.ctypes.data)
+    return out
 
 
 def octree_build(ops, depth, max_words=1 << 27):

@@ -228,7 +228,7 @@ static inline float gmax(float a, float b) { return (a > b || isnan(b)) ? a : b;
  * reject each plausible alternative lowering.  0 = llvmpipe's (the oracle). */
 static int g_fp_variant = 0;
 void so_set_fp_variant(int v) { g_fp_variant = v; }
-enum { FPV_LLVMPIPE = 0, FPV_FMA = 1, FPV_DOT_LEFT = 2, FPV_NORM_DIV = 3, FPV_DIV_RCP = 4 };
+enum { FPV_LLVMPIPE = 0, FPV_FMA = 1, FPV_DOT_LEFT = 2, FPV_NORM_DIV = 3, FPV_DIV_RCP = 4, FPV_SRC_ORDER = 5 };
 
 /* a*b + c, contracted only under the FMA variant */
 static inline float madd(float a, float b, float c) { return g_fp_variant == FPV_FMA ? fmaf(a, b, c) : a * b + c; }
@@ -329,9 +329,39 @@ uint32_t so_voxel_key(const uint32_t pos[3])   /* internal.glsl:263-291 */
 /* float -> uint conversion of a value in [0, 2^24] (GLSL uint(float)). */
 static inline uint32_t f2u(float f) { return (uint32_t)f; }
 
+/* The positions a traversal tests, in order: one entry per test of the loop
+ * condition of internal.glsl:370 (cell = the truncated position, or NONE
+ * when the position is out of bounds and ends the walk). */
+typedef struct trail {
+    uint32_t cap, n;
+    uint32_t (*cell)[3];
+    double (*pos)[3];
+} trail;
+
+static inline void trail_add(trail *t, const double r[3], int inb)
+{
+    if (!t || t->n >= t->cap) return;
+    for (int i = 0; i < 3; i++) {
+        t->pos[t->n][i] = r[i];
+        t->cell[t->n][i] = inb ? (uint32_t)r[i] : 0xFFFFFFFFu;
+    }
+    t->n++;
+}
+
+static so_hit raycast32(const uint32_t *words, uint32_t n_words, uint32_t depth, const float origin_in[3],
+                        const float dir[3], const float inv[3], uint32_t controlchecks, uint32_t *checks_out,
+                        uint64_t *node_reads, trail *tr);
+
 so_hit so_raycast(const uint32_t *words, uint32_t n_words, uint32_t depth, const float origin_in[3],
                   const float dir[3], const float inv[3], uint32_t controlchecks, uint32_t *checks_out,
                   uint64_t *node_reads)
+{
+    return raycast32(words, n_words, depth, origin_in, dir, inv, controlchecks, checks_out, node_reads, NULL);
+}
+
+static so_hit raycast32(const uint32_t *words, uint32_t n_words, uint32_t depth, const float origin_in[3],
+                        const float dir[3], const float inv[3], uint32_t controlchecks, uint32_t *checks_out,
+                        uint64_t *node_reads, trail *tr)
 {
     so_hit miss = {0};
     uint32_t p2c[17];
@@ -346,10 +376,13 @@ so_hit so_raycast(const uint32_t *words, uint32_t n_words, uint32_t depth, const
     if (in_bounds(o, fl)) {
         r[0] = o[0]; r[1] = o[1]; r[2] = o[2];
     } else {   /* intersect(ray, 0, N), internal.glsl:245-253 */
+        /* llvmpipe's compiler reassociates the constant nudges: (0 - o) + 0.001
+         * becomes 0.001 - o (the same value) and (N - o) - 0.001 becomes
+         * (N - 0.001) - o */
         float tmn[3], tmx[3];
         for (int i = 0; i < 3; i++) {
-            float t1 = ((0.0f - o[i]) + 0.001f) * inv[i];
-            float t2 = ((fl - o[i]) - 0.001f) * inv[i];
+            float t1 = (0.001f - o[i]) * inv[i];
+            float t2 = (g_fp_variant == FPV_SRC_ORDER ? (fl - o[i]) - 0.001f : (fl - 0.001f) - o[i]) * inv[i];
             tmn[i] = gmin(t1, t2);
             tmx[i] = gmax(t1, t2);
         }
@@ -364,7 +397,13 @@ so_hit so_raycast(const uint32_t *words, uint32_t n_words, uint32_t depth, const
     }
 
     uint32_t tpos[3] = {0, 0, 0}, tsize = 0;
-    while (in_bounds(r, fl) && q++ <= controlchecks) {   /* internal.glsl:370 */
+    for (;;) {
+        const int inb = in_bounds(r, fl);
+        if (tr) {
+            const double rd[3] = {r[0], r[1], r[2]};
+            trail_add(tr, rd, inb);
+        }
+        if (!(inb && q++ <= controlchecks)) break;   /* internal.glsl:370 */
         uint32_t ur[3] = {f2u(r[0]), f2u(r[1]), f2u(r[2])};
         uint32_t offset = 0, d = 0;
         int found = 0;
@@ -402,10 +441,15 @@ so_hit so_raycast(const uint32_t *words, uint32_t n_words, uint32_t depth, const
         /* intersect_inside(ray, target.position, target.position + size), internal.glsl:255-261 */
         float tmx[3];
         for (int i = 0; i < 3; i++) {
+            /* reassociated by llvmpipe's compiler: (pos - 0.001) - o and
+             * ((pos + 0.001) + size) - o */
             float bmin = (float)tpos[i];
-            float bmax = bmin + (float)tsize;
-            float t1 = ((bmin - o[i]) - 0.001f) * inv[i];
-            float t2 = ((bmax - o[i]) + 0.001f) * inv[i];
+            float t1 = ((bmin - 0.001f) - o[i]) * inv[i];
+            float t2 = (((bmin + 0.001f) + (float)tsize) - o[i]) * inv[i];
+            if (g_fp_variant == FPV_SRC_ORDER) {   /* GLSL source order (a test variant) */
+                t1 = ((bmin - o[i]) - 0.001f) * inv[i];
+                t2 = (((bmin + (float)tsize) - o[i]) + 0.001f) * inv[i];
+            }
             tmx[i] = gmax(t1, t2);
         }
         float tx = gmin(gmin(tmx[0], tmx[1]), tmx[2]);
@@ -429,6 +473,165 @@ static void primary_dir(const float ubo[16], uint32_t x, uint32_t y, uint32_t W,
     }
     for (int i = 0; i < 3; i++) d[i] = madd(-ny, ubo[12 + i], madd(nx, ubo[8 + i], ubo[4 + i]));
     gl_normalize(d);
+}
+
+/* ---- FP64 twin ------------------------------------------------------------
+ * Raycast (internal.glsl:351-400) and the primary ray of
+ * generate_primary.comp:44-58 in double precision: the camera widened exactly
+ * from float, the same 0.001 nudges, traversal and step limit.  Where the
+ * FP32 walk and this one visit different cells, the first such step shows
+ * whether FP32 rounding decided it: there the FP64 position lies within a
+ * rounding-sized distance of the face separating the two cells. */
+static inline int in_bounds64(const double v[3], double n)
+{
+    return 0.0 <= v[0] && v[0] <= n && 0.0 <= v[1] && v[1] <= n && 0.0 <= v[2] && v[2] <= n;
+}
+
+static so_hit raycast64(const uint32_t *words, uint32_t n_words, uint32_t depth, const double origin_in[3],
+                        const double dir[3], const double inv[3], uint32_t controlchecks, uint32_t *checks_out,
+                        trail *tr)
+{
+    so_hit miss = {0};
+    uint32_t p2c[17];
+    const uint32_t len = 1u << depth;
+    for (uint32_t i = 0; i <= depth; i++) p2c[i] = len >> i;
+    const double fl = (double)len;
+    uint32_t q = 0;
+    double o[3], r[3];
+    for (int i = 0; i < 3; i++) o[i] = origin_in[i] + dir[i] * 4.0;
+    if (in_bounds64(o, fl)) {
+        r[0] = o[0]; r[1] = o[1]; r[2] = o[2];
+    } else {
+        double te = -INFINITY, tx = INFINITY;
+        for (int i = 0; i < 3; i++) {
+            const double t1 = (0.001 - o[i]) * inv[i], t2 = ((fl - 0.001) - o[i]) * inv[i];
+            const double mn = t1 < t2 ? t1 : t2, mx = t1 > t2 ? t1 : t2;
+            if (mn > te) te = mn;
+            if (mx < tx) tx = mx;
+        }
+        q++;
+        if (tx < te || tx < 0.0) {
+            if (checks_out) *checks_out = q;
+            return miss;
+        }
+        for (int i = 0; i < 3; i++) r[i] = dir[i] * te + o[i];
+    }
+    for (;;) {
+        const int inb = in_bounds64(r, fl);
+        trail_add(tr, r, inb);
+        if (!(inb && q++ <= controlchecks)) break;
+        const uint32_t ur[3] = {(uint32_t)r[0], (uint32_t)r[1], (uint32_t)r[2]};
+        uint32_t offset = 0, d = 0, tsize = 0, tpos[3];
+        int found = 0;
+        for (; d < depth; d++) {
+            offset += gl_locate(ur, p2c[d]);
+            const uint32_t raw = fetch(words, n_words, offset);
+            if (!(raw & 1u)) {
+                found = 1;
+                break;
+            }
+            offset = (raw & 0xFFFFFFF0u) >> 4;
+        }
+        if (!found) offset += gl_locate(ur, p2c[d]);
+        tsize = p2c[d];
+        for (int i = 0; i < 3; i++) tpos[i] = ur[i] & ~(tsize - 1u);
+        if (!found) {
+            const uint32_t mat = (fetch(words, n_words, offset) & 254u) >> 1;
+            if (mat != 0) {
+                so_hit h = {1u, offset, mat, {tpos[0], tpos[1], tpos[2]}, tsize};
+                if (checks_out) *checks_out = q;
+                return h;
+            }
+        }
+        double tx = INFINITY;
+        for (int i = 0; i < 3; i++) {
+            const double bmin = (double)tpos[i];
+            const double t1 = ((bmin - 0.001) - o[i]) * inv[i], t2 = (((bmin + 0.001) + (double)tsize) - o[i]) * inv[i];
+            const double mx = t1 > t2 ? t1 : t2;
+            if (mx < tx) tx = mx;
+        }
+        for (int i = 0; i < 3; i++) r[i] = dir[i] * tx + o[i];
+    }
+    if (checks_out) *checks_out = q;
+    return miss;
+}
+
+/* Distance from FP64 position p to the face that separates its cell from
+ * cell c (the largest over the axes where they differ); an out-of-bounds
+ * side counts the grid's own faces. */
+static double separating_face(const double p[3], const uint32_t c64[3], const uint32_t c32[3], double n)
+{
+    double dist = 0.0;
+    if (c64[0] == 0xFFFFFFFFu || c32[0] == 0xFFFFFFFFu) {   /* one walk left the grid: its nearest face */
+        dist = INFINITY;
+        for (int i = 0; i < 3; i++) {
+            const double a = fabs(p[i]), b = fabs(n - p[i]);
+            if (a < dist) dist = a;
+            if (b < dist) dist = b;
+        }
+        return dist;
+    }
+    for (int i = 0; i < 3; i++) {
+        if (c64[i] == c32[i]) continue;
+        const double face = c32[i] > c64[i] ? (double)c64[i] + 1.0 : (double)c64[i];
+        const double di = fabs(face - p[i]);
+        if (di > dist) dist = di;
+    }
+    return dist;
+}
+
+void so_retrace_f64(const uint32_t *words, uint32_t n_words, uint32_t depth, const float ubo[16], uint32_t W,
+                    uint32_t H, uint32_t controlchecks, const uint32_t *pixels, size_t n, so_retrace *out)
+{
+    const uint32_t cap = controlchecks + 4u;
+    trail t32 = {cap, 0, malloc(sizeof(uint32_t[3]) * cap), malloc(sizeof(double[3]) * cap)};
+    trail t64 = {cap, 0, malloc(sizeof(uint32_t[3]) * cap), malloc(sizeof(double[3]) * cap)};
+    const double fl = (double)(1u << depth);
+    for (size_t k = 0; k < n; k++) {
+        const uint32_t x = pixels[k] % W, y = pixels[k] / W;
+        so_retrace *o = out + k;
+        float d[3], inv[3];
+        primary_dir(ubo, x, y, W, H, d);
+        for (int i = 0; i < 3; i++) inv[i] = 1.0f / d[i];
+        t32.n = t64.n = 0;
+        uint32_t c32 = 0, c64 = 0;
+        const so_hit h32 = raycast32(words, n_words, depth, ubo, d, inv, controlchecks, &c32, NULL, &t32);
+        const double nx = ((double)x + 0.5) / (double)W * 2.0 - 1.0, ny = ((double)y + 0.5) / (double)H * 2.0 - 1.0;
+        double dd[3], ii[3], org[3], l2 = 0.0;
+        for (int i = 0; i < 3; i++) {
+            dd[i] = (double)ubo[4 + i] + nx * (double)ubo[8 + i] - ny * (double)ubo[12 + i];
+            l2 += dd[i] * dd[i];
+            org[i] = (double)ubo[i];
+        }
+        for (int i = 0; i < 3; i++) {
+            dd[i] /= sqrt(l2);
+            ii[i] = 1.0 / dd[i];
+        }
+        const so_hit h64 = raycast64(words, n_words, depth, org, dd, ii, controlchecks, &c64, &t64);
+        o->key32 = h32.hit ? so_voxel_key(h32.pos) + 1u : 0u;
+        o->key64 = h64.hit ? so_voxel_key(h64.pos) + 1u : 0u;
+        o->steps32 = c32;
+        o->steps64 = c64;
+        o->diverge_step = 0xFFFFFFFFu;
+        o->face_dist = -1.0;
+        o->pos_diff = 0.0;
+        const uint32_t m = t32.n < t64.n ? t32.n : t64.n;
+        uint32_t s = 0;
+        while (s < m && t32.cell[s][0] == t64.cell[s][0] && t32.cell[s][1] == t64.cell[s][1] &&
+               t32.cell[s][2] == t64.cell[s][2])
+            s++;
+        if (s < m) {
+            o->diverge_step = s;
+            o->face_dist = separating_face(t64.pos[s], t64.cell[s], t32.cell[s], fl);
+            for (int i = 0; i < 3; i++) {
+                const double e = fabs(t64.pos[s][i] - t32.pos[s][i]);
+                if (e > o->pos_diff) o->pos_diff = e;
+            }
+        } else if (t32.n != t64.n) {   /* one walk stopped at the step limit or a hit the other passed */
+            o->diverge_step = s;
+        }
+    }
+    free(t32.cell); free(t32.pos); free(t64.cell); free(t64.pos);
 }
 
 typedef struct prim_job {
@@ -785,7 +988,9 @@ static v3 evaluate_brdf(v3 Vd, v3 L, v3 N, v3 albedo, float roughness, float met
     float specular = spec_norm * m_pow(NdotH, spec_exp);
     float kS = spec_int * metallic;
     float kD = 1.0f - kS * 0.5f;
-    v3 diffuse = vmuls(vmuls(albedo, kD), 0.31831f);
+    /* albedo * kD * 0.31831, reassociated by llvmpipe's compiler (a constant
+     * factor of a product chain moves next to its first operand) */
+    v3 diffuse = g_fp_variant == FPV_SRC_ORDER ? vmuls(vmuls(albedo, kD), 0.31831f) : vmuls(vmuls(albedo, 0.31831f), kD);
     float sp = specular * kS;
     return vmuls(vadd(diffuse, V(sp, sp, sp)), NdotL);
 }

@@ -58,6 +58,19 @@ void so_generate_primary(const uint32_t *words, uint32_t n_words, uint32_t depth
                          uint32_t W, uint32_t H, uint32_t controlchecks, uint32_t *gbuf, uint32_t *pos,
                          uint64_t *node_reads, uint64_t *steps, int threads);
 
+/* FP64 twin of Raycast (internal.glsl:351-400) for the primary rays of the
+ * given pixels (index y*W+x), beside the FP32 walk.  key*: voxelKey+1 of the
+ * hit or 0; steps*: q; diverge_step: first loop test at which the two walks
+ * are in different cells (0xFFFFFFFF: never); there, face_dist is the FP64
+ * position's distance to the face separating the two cells and pos_diff the
+ * largest per-axis difference of the two positions. */
+typedef struct so_retrace {
+    uint32_t key32, key64, steps32, steps64, diverge_step, pad;
+    double face_dist, pos_diff;
+} so_retrace;
+void so_retrace_f64(const uint32_t *words, uint32_t n_words, uint32_t depth, const float ubo[16], uint32_t W,
+                    uint32_t H, uint32_t controlchecks, const uint32_t *pixels, size_t n, so_retrace *out);
+
 /* ---- full wavefront frame --------------------------------------------------- */
 typedef struct so_frame_config {
     int32_t primary_controlchecks, bounce_controlchecks, reconstruction_radius, normal_precision;

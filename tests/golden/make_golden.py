@@ -7,7 +7,7 @@ scenes of voxelengine3.0_amd/scenes.py, and stores inputs' digests plus the
 pass outputs as compressed .npz files next to this script.
 
     LP_NUM_THREADS=1 python tests/golden/make_golden.py [fp] [cube] [primary] [frame] [lights] [sky]
-                                                        [overflow] [material] [loops]
+                                                        [overflow] [material] [loops] [headline]
 
 Only the outputs are stored; the shaders are never copied.
 """
@@ -513,6 +513,65 @@ def make_material():
         json.dump(meta, fh, indent=1, sort_keys=True)
 
 
+def make_headline():
+    """The measured workload itself (depth-10 scene of 10.4 M voxels, 1920x1080,
+    MAX_BOUNCES 3, the benchmark's knobs, orbit frames 0..3): digests of every
+    exact structure, 4096 sampled output pixels per frame, and the lBuffer
+    entries where the oracle's deposit lands one unit from llvmpipe's (ulp
+    flips of the transcendental cores), with the output pixels they reach."""
+    from oracle import oracle as O
+    from frame_common import (HEADLINE_FRAMES, HEADLINE_KNOBS, HEADLINE_SAMPLES, HEADLINE_SKY, OUT_ATOL, OUT_RTOL,
+                              hash_map as hmap, headline_digests)
+    depth, ops = scenes.scene_extreme(1)
+    words, info = ve.octree_build(ops, depth)
+    del ops
+    W, H = 1920, 1080
+    mats = scenes.palette(1)
+    fc = O.FrameConfig(**HEADLINE_KNOBS)
+    pl = R.GLPipeline(words, depth, W, H, max_bounces=3, materials=mats, sky_rgb8=HEADLINE_SKY)
+    of = O.Frame(W, H, 16384, 3)
+    of.set_materials(mats)
+    of.set_sky_rgb8(HEADLINE_SKY)
+    rng = np.random.default_rng(1080)
+    meta = {"depth": depth, "words_sha256": words_digest(words, info.size), "W": W, "H": H, "max_bounces": 3,
+            "sky_rgb8": list(HEADLINE_SKY), "frames": []}
+    arrays = {}
+    for f in range(HEADLINE_FRAMES):
+        eye, d, fov = scenes.orbit_camera(1 << depth, f, 64, 1)
+        ubo = ve.camera_ubo(eye, d, fov, W / H)
+        pl.set_camera(ubo)
+        st = pl.frame(fc.as_dict(), f, frame_seed(f), f + 1, count_live=True)
+        of.frame(words, depth, ubo, fc, f, frame_seed(f), f + 1, n_words=info.size)
+        g, p = pl.read_gbuffer()
+        nc, ns = st["candidates"], st["scheduled"]
+        lmap = hmap(pl.read_hash(pl.lbuf, 384).reshape(-1), 12, 32)
+        nmap = hmap(pl.read_hash(pl.nbuf, 96).reshape(-1), 3, 32)
+        dg = headline_digests(g.reshape(-1), p.reshape(-1), pl.candidates.read()[:4 * nc],
+                              pl.shade_list.read()[:2 * ns], pl.rank_stats.read(), nmap, lmap)
+        omap = hmap(of.read("LBUFFER"), 12, 32)
+        assert np.array_equal(omap[0], lmap[0])
+        flip = np.nonzero((omap[1] != lmap[1]).any(1))[0]
+        assert np.abs(omap[1][flip].astype(np.int64) - lmap[1][flip].astype(np.int64)).max(initial=0) <= 1
+        arrays["f%d_flip_vids" % f] = lmap[0][flip]
+        arrays["f%d_flip_llvm" % f] = lmap[1][flip]
+        arrays["f%d_flip_oracle" % f] = omap[1][flip]
+        out = pl.out.read().reshape(-1, 4)
+        oout = of.read("OUTPUT", np.float32).reshape(-1, 4)
+        px = np.sort(rng.choice(W * H, HEADLINE_SAMPLES, replace=False)).astype(np.uint32)
+        arrays["f%d_out_pixels" % f] = px
+        arrays["f%d_out_values" % f] = out[px]
+        bad = ~np.isclose(oout[px], out[px], rtol=OUT_RTOL, atol=OUT_ATOL).all(1)
+        assert bad.sum() == 0 or len(flip) > 0, "output beyond tolerance without an lBuffer flip"
+        arrays["f%d_out_flip" % f] = bad
+        meta["frames"].append(dict(counters=[int(nc), int(ns), int(st["bounces"]), int(st["live_in"])],
+                                   digests=dg, lbuf_flips=int(len(flip)), out_flip_pixels=int(bad.sum())))
+        print("headline frame", f, st, "flips", len(flip), "output pixels reached", int(bad.sum()), flush=True)
+    of.close()
+    np.savez_compressed(os.path.join(HERE, "headline_llvmpipe.npz"), **arrays)
+    with open(os.path.join(HERE, "headline_llvmpipe.json"), "w") as fh:
+        json.dump(meta, fh, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
     if os.environ.get("LP_NUM_THREADS") != "1":
         sys.exit("run with LP_NUM_THREADS=1 (deterministic llvmpipe dispatch order)")
@@ -537,3 +596,5 @@ if __name__ == "__main__":
         make_material()
     if "loops" in what:
         make_loop_limit()
+    if "headline" in what:
+        make_headline()

@@ -11,7 +11,8 @@ in the shade list, the emitted rays and every lBuffer bucket.
 Bit-exact: G-buffer and packed positions, counters, candidate list (order
 and representative pixel), shade list, per-voxel pixel counts, claim map,
 RankStats, l/nBuffer, emit.comp's ray records.  Resolved RGBA32F within
-1e-4 relative."""
+1e-4 relative.  Sampled primary rays re-traced in FP64 hit the same voxel
+except at knife edges."""
 import numpy as np
 import pytest
 
@@ -115,3 +116,75 @@ def test_headline_orbit_primaries_match_oracle(ve, scenes, oracle, extreme):
     finally:
         rend.close()
 
+
+
+FACE_TOL = 1e-3   # voxel units: FP32 positions near coordinate 1024 carry an ulp of 1.2e-4
+
+
+def test_headline_fp64_retrace(ve, scenes, oracle, extreme):
+    """FP64 re-trace of 4096 sampled primary rays per frame of the GPU
+    G-buffer (Raycast, internal.glsl:351-400, in double precision): the
+    same voxel, or a knife edge -- the FP32 and FP64 walks part where the
+    FP64 position lies within FACE_TOL of the face between their cells."""
+    depth, words, info = extreme
+    rend = ve.Renderer(W, H)
+    rng = np.random.default_rng(2024)
+    knife = same = 0
+    try:
+        scene = rend.upload(words, depth, info.size)
+        for f in (0, 17, 42):
+            print("case headline fp64 retrace frame", f, flush=True)
+            ubo = _ubo(ve, scenes, depth, f)
+            rend.set_camera(ubo)
+            rend.run_pass(scene, "PRIMARY", ve.FrameConfig(**HEADLINE_KNOBS), ve.FrameInputs(W, H, f, 1, 1))
+            g, _ = rend.gbuffer(W, H)
+            pix = rng.choice(W * H, 4096, replace=False).astype(np.uint32)
+            r = oracle.retrace_f64(words, depth, ubo, W, H, HEADLINE_KNOBS["primary_controlchecks"], pix,
+                                   n_words=info.size)
+            gpu_key = g.reshape(-1, 4)[pix, 3]
+            assert np.array_equal(r["key32"], gpu_key)   # the FP32 walk beside the twin is the GPU's
+            diff = r["key64"] != gpu_key
+            for i in np.nonzero(diff)[0]:
+                print("  knife edge: pixel", pix[i], r[i], flush=True)
+                assert r["diverge_step"][i] != 0xFFFFFFFF and 0.0 <= r["face_dist"][i] <= FACE_TOL, r[i]
+            knife += int(diff.sum())
+            same += int((~diff).sum())
+        rend.release(scene)
+    finally:
+        rend.close()
+    print("fp64 retrace: %d same voxel, %d knife edges" % (same, knife), flush=True)
+    assert knife <= 12   # 3 x 4096 rays
+
+
+def test_headline_frames_match_llvmpipe_fixture(ve, scenes, extreme):
+    """Orbit frames 0..3 through ve_frame against the reference's own GLSL on
+    llvmpipe (tests/golden/headline_llvmpipe.*): every structure bit-exact,
+    the lBuffer up to the fixture's recorded ulp flips, the output within the
+    radiance tolerance."""
+    from frame_common import (HEADLINE_FRAMES, compare_headline_frame, hash_map, headline_digests,
+                              load_headline_gold)
+    meta, arr = load_headline_gold()
+    depth, words, info = extreme
+    rend = ve.Renderer(W, H, hash_width=16384, max_bounces=MAX_BOUNCES)
+    try:
+        rend.set_materials(scenes.palette(1))
+        rend.set_sky_constant(HEADLINE_SKY)
+        scene = rend.upload(words, depth, info.size)
+        fc = ve.FrameConfig(**HEADLINE_KNOBS)
+        for f in range(HEADLINE_FRAMES):
+            print("case headline llvmpipe frame", f, flush=True)
+            rend.set_camera(_ubo(ve, scenes, depth, f))
+            rend.frame(scene, fc, ve.FrameInputs(W, H, f, frame_seed(f), f + 1))
+            c = rend.counters()
+            nc, ns = int(c[0]), int(c[1])
+            g, p = rend.gbuffer(W, H)
+            lmap = hash_map(rend.read("LBUFFER"), 12, 32)
+            dg = headline_digests(g.reshape(-1), p.reshape(-1), rend.read("CANDIDATES", np.uint32, nc * 16),
+                                  rend.read("SHADE_LIST", np.uint32, ns * 8), rend.read("RANK_STATS"),
+                                  hash_map(rend.read("NBUFFER"), 3, 32), lmap)
+            compare_headline_frame(meta, arr, f, [nc, ns, int(c[ve.CNT["BOUNCES_RUN"]]),
+                                                  int(c[ve.CNT["RAYS_TRACED"]])], dg, lmap,
+                                   rend.read("OUTPUT", np.float32, W * H * 16))
+        rend.release(scene)
+    finally:
+        rend.close()

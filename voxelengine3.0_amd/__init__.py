@@ -151,6 +151,10 @@ def lib():
         "ve_timings": (i32, [p, p]),
         "ve_enable_node_stats": (i32, [p, i32]),
         "ve_node_stats": (i32, [p] + [C.POINTER(C.c_uint64)] * 4),
+        "ve_node_stats_bounce": (i32, [p] + [C.POINTER(C.c_uint64)] * 2),
+        "ve_timings_iter": (i32, [p, u32, p]),
+        "ve_set_tuning": (i32, [p, p]),
+        "ve_get_tuning": (i32, [p, p]),
         "ve_ray_totals": (i32, [p] + [C.POINTER(C.c_uint64)] * 3),
         "ve_ray_totals_reused": (i32, [p, C.POINTER(C.c_uint64)]),
         "ve_reset_ray_totals": (i32, [p]),
@@ -426,6 +430,16 @@ class Renderer:
         self._ck(lib().ve_scene_update(self.h, scene, words.ctypes.data, octree.size,
                                        slots.ctypes.data if len(slots) else None, len(slots)))
 
+    def set_tuning(self, tile_chunk=0, jump_level=0):
+        """ve_set_tuning: launch-shape knobs (speed only, never results)."""
+        t = (C.c_uint32 * 2)(tile_chunk, jump_level)
+        self._ck(lib().ve_set_tuning(self.h, t))
+
+    def tuning(self):
+        t = (C.c_uint32 * 2)()
+        self._ck(lib().ve_get_tuning(self.h, t))
+        return dict(tile_chunk=t[0], jump_level=t[1])
+
     def set_camera(self, ubo):
         ubo = np.ascontiguousarray(ubo, np.float32)
         self._ck(lib().ve_set_camera(self.h, ubo.ctypes.data))
@@ -486,8 +500,10 @@ class Renderer:
     def node_stats(self):
         v = [C.c_uint64() for _ in range(4)]
         self._ck(lib().ve_node_stats(self.h, *[C.byref(x) for x in v]))
+        b = [C.c_uint64() for _ in range(2)]
+        self._ck(lib().ve_node_stats_bounce(self.h, *[C.byref(x) for x in b]))
         return dict(primary_reads=v[0].value, trace_reads=v[1].value, primary_steps=v[2].value,
-                    trace_steps=v[3].value)
+                    trace_steps=v[3].value, bounce_reads=b[0].value, bounce_steps=b[1].value)
 
     def ray_totals(self):
         v = [C.c_uint64() for _ in range(3)]
@@ -513,6 +529,12 @@ class Renderer:
     def timings(self):
         out = np.zeros(len(PASS), np.float32)
         self._ck(lib().ve_timings(self.h, out.ctypes.data))
+        return dict(zip(PASS_NAMES, out.tolist()))
+
+    def timings_iter(self, it):
+        """Per-pass means split by bounce iteration (read before timings())."""
+        out = np.zeros(len(PASS), np.float32)
+        self._ck(lib().ve_timings_iter(self.h, it, out.ctypes.data))
         return dict(zip(PASS_NAMES, out.tolist()))
 
 

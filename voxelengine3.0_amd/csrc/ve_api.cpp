@@ -110,7 +110,7 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.dep_over_n, 2);
     a(&b.dep_spill, SHADE_BUDGET_MAX);
     a(&b.materials, MATERIAL_SLOTS);
-    a(&b.node_stats, 4);
+    a(&b.node_stats, 8);
     a(&b.ray_totals, 4);
     a(&b.sched_tab, (size_t)CLAIM_WIDTH * SCHED_SLOTS);
     a(&b.sched_over, SCHED_OVER_CAP);
@@ -174,6 +174,7 @@ void fill_params(ve_ctx *c, const ve_scene *s, const ve_frame_config *fc, const 
     p.n_words = s ? s->n_words : 0;
     p.jump = s ? s->jump : nullptr;
     p.jump_level = s ? s->jump_level : 0;
+    p.tile_chunk = c->tuning.tile_chunk;
     p.hash_width = c->hash_width;
     p.max_bounces = c->max_bounces;
     p.frame_idx = in->frame_idx;
@@ -370,7 +371,7 @@ int ve_scene_upload(ve_ctx *c, const uint32_t *words, uint32_t n_words, uint32_t
     if (!c) return ve_global_error(VE_EINVAL, "null context");
     std::string err;
     c->aux_fence = true;   // the primary pass reads scenes; it follows the upload
-    const int rc = ve::scene_upload(c->device, c->stream, words, n_words, depth, scene, err);
+    const int rc = ve::scene_upload(c->device, c->stream, words, n_words, depth, scene, err, c->tuning.jump_level);
     return rc == VE_OK ? VE_OK : ctx_error(c, rc, "%s", err.c_str());
 }
 
@@ -444,6 +445,24 @@ int ve_scene_release(ve_ctx *c, ve_scene *s)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     ve::scene_free(s);
+    return VE_OK;
+}
+
+int ve_set_tuning(ve_ctx *c, const ve_tuning *t)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    if (!t) return ctx_error(c, VE_EINVAL, "null tuning");
+    if (t->tile_chunk > 4096u) return ctx_error(c, VE_EINVAL, "tile_chunk %u above 4096", t->tile_chunk);
+    if (t->jump_level > JUMP_LEVEL) return ctx_error(c, VE_EINVAL, "jump_level %u above %u", t->jump_level, JUMP_LEVEL);
+    c->tuning = *t;
+    return VE_OK;
+}
+
+int ve_get_tuning(ve_ctx *c, ve_tuning *t)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    if (!t) return ctx_error(c, VE_EINVAL, "null tuning");
+    *t = c->tuning;
     return VE_OK;
 }
 
@@ -808,6 +827,29 @@ int ve_enable_timing(ve_ctx *c, int enable)
     return VE_OK;
 }
 
+int ve_timings_iter(ve_ctx *c, uint32_t iter, float ms_out[VE_PASS_COUNT])
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    if (!ms_out) return ctx_error(c, VE_EINVAL, "null output");
+    if (iter > 1) return ctx_error(c, VE_EINVAL, "iter %u above 1", iter);
+    HIP_OK(c, hipSetDevice(c->device));
+    if (c->aux) HIP_OK(c, hipStreamSynchronize(c->aux));
+    if (c->aux2) HIP_OK(c, hipStreamSynchronize(c->aux2));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < VE_PASS_COUNT; i++) {
+        double sum = 0.0;
+        const bool split = i == VE_PASS_TRACE || i == VE_PASS_SHADE;
+        if (i != VE_PASS_DISPLAY && (iter == 0 || (split && iter < c->iters_recorded)))
+            for (size_t f = 0; f < c->frames_recorded; f++) {
+                const hipEvent_t *pair = &c->frame_events[f][2 * (i * MAX_ITERS + iter)];
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, pair[0], pair[1]) == hipSuccess) sum += ms;
+            }
+        ms_out[i] = c->frames_recorded ? (float)(sum / (double)c->frames_recorded) : 0.0f;
+    }
+    return VE_OK;
+}
+
 int ve_timings(ve_ctx *c, float ms_out[VE_PASS_COUNT])
 {
     if (!c) return ve_global_error(VE_EINVAL, "null context");
@@ -872,7 +914,7 @@ int ve_enable_node_stats(ve_ctx *c, int enable)
     if (!c) return ve_global_error(VE_EINVAL, "null context");
     c->node_stats = enable != 0;
     HIP_OK(c, hipSetDevice(c->device));
-    HIP_OK(c, hipMemsetAsync(c->b.node_stats, 0, 4 * sizeof(unsigned long long), c->stream));
+    HIP_OK(c, hipMemsetAsync(c->b.node_stats, 0, 8 * sizeof(unsigned long long), c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     return VE_OK;
 }
@@ -880,14 +922,26 @@ int ve_enable_node_stats(ve_ctx *c, int enable)
 int ve_node_stats(ve_ctx *c, uint64_t *pr, uint64_t *tr, uint64_t *ps, uint64_t *ts)
 {
     if (!c) return ve_global_error(VE_EINVAL, "null context");
-    unsigned long long v[4];
+    unsigned long long v[6];
     HIP_OK(c, hipSetDevice(c->device));
     HIP_OK(c, hipMemcpyAsync(v, c->b.node_stats, sizeof v, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     if (pr) *pr = v[0];
-    if (tr) *tr = v[1];
+    if (tr) *tr = v[1] + v[4];
     if (ps) *ps = v[2];
-    if (ts) *ts = v[3];
+    if (ts) *ts = v[3] + v[5];
+    return VE_OK;
+}
+
+int ve_node_stats_bounce(ve_ctx *c, uint64_t *reads, uint64_t *steps)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    unsigned long long v[6];
+    HIP_OK(c, hipSetDevice(c->device));
+    HIP_OK(c, hipMemcpyAsync(v, c->b.node_stats, sizeof v, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    if (reads) *reads = v[4];
+    if (steps) *steps = v[5];
     return VE_OK;
 }
 
@@ -1001,7 +1055,7 @@ int ctx_reset(ve_ctx *c)
 }
 
 int scene_upload(int device, hipStream_t stream, const uint32_t *words, uint32_t n_words, uint32_t depth,
-                 ve_scene **scene, std::string &err)
+                 ve_scene **scene, std::string &err, uint32_t jump_level)
 {
     if (!words || !scene || n_words == 0) return err = "null words / scene or empty array", VE_EINVAL;
     char buf[256];
@@ -1028,7 +1082,7 @@ int scene_upload(int device, hipStream_t stream, const uint32_t *words, uint32_t
         delete s;
         return err = std::string("hipMalloc node array: ") + hipGetErrorString(e), VE_ENOMEM;
     }
-    const uint32_t jl = std::min(JUMP_LEVEL, depth - 1u);
+    const uint32_t jl = std::min(jump_level ? std::min(jump_level, JUMP_LEVEL) : JUMP_LEVEL, depth - 1u);
     void *j = nullptr;
     e = hipMalloc(&j, (size_t)4 << (3u * jl));
     if (e != hipSuccess) {

@@ -48,6 +48,7 @@ struct ve_ctx {
     uint32_t *sky_tex = nullptr;         // 6 x sky_edge^2 texels, r | g << 8 | b << 16
     size_t sky_tex_words = 0;            // allocated
     bool node_stats = false;
+    ve_tuning tuning{ve::DEFAULT_TILE_CHUNK, 0u};
     bool timing = false;
     uint32_t bounce_iter = 0;           // next trace/shade iteration of per-pass dispatch
     // Deposits recorded by pass-by-pass shade dispatches and not yet applied
@@ -96,7 +97,7 @@ int ctx_create(ve_ctx **out, const ve_renderer_config *cfg, hipStream_t shared, 
 // enqueued on the context's stream without waiting.
 int ctx_reset(ve_ctx *c);
 int scene_upload(int device, hipStream_t stream, const uint32_t *words, uint32_t n_words, uint32_t depth,
-                 ve_scene **scene, std::string &err);
+                 ve_scene **scene, std::string &err, uint32_t jump_level = JUMP_LEVEL);
 void scene_free(ve_scene *s);
 hipStream_t ctx_stream(ve_ctx *c);
 size_t ctx_bytes(const ve_ctx *c);

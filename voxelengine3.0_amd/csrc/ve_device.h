@@ -51,6 +51,20 @@ __device__ __forceinline__ f3 normalize(f3 v)
 }
 __device__ __forceinline__ f3 rcp3(f3 d) { return f3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z}; }
 
+// Block b of a screen pass (4 tiles of 8x8 pixels per block) -> its tile
+// group.  The hardware deals blocks to the 8 XCDs round-robin (XCD b % 8).
+// chunk 0: XCD x takes the x-th contiguous eighth of the groups (one band of
+// rows: its L2 holds that band's nodes); chunk n: XCD x takes runs of n
+// consecutive groups in turn (runs x, x + 8, ...), so every XCD gets work
+// from every band.  The grid is padded to a multiple of 8 * max(chunk, 1)
+// blocks, which makes the map a bijection.
+__device__ __forceinline__ uint32_t xcd_group(uint32_t b, uint32_t nb_padded, uint32_t chunk)
+{
+    const uint32_t x = b & 7u, k = b >> 3;
+    if (chunk == 0u) return x * (nb_padded >> 3) + k;
+    return ((k / chunk) * 8u + x) * chunk + k % chunk;
+}
+
 // voxelKey + hash (internal.glsl:263-291); the 10-bit packing aliases when a
 // coordinate reaches 1024, exactly as in the reference.
 __device__ __forceinline__ uint32_t voxel_key(uint32_t x, uint32_t y, uint32_t z)
@@ -179,6 +193,7 @@ struct DDA {
     uint32_t q;
     uint32_t mx, my, mz;   // all ones where the axis' inverse direction is positive (far face at pos + size)
     float ex, ey, ez;      // the far face's nudge: +0.001 there, -0.001 on the other faces
+    bool exact;            // an infinite inverse component: exits take the reference's full min/max form
 };
 
 enum : int { DDA_CONTINUE = 0, DDA_MISS = 1, DDA_HIT = 2 };
@@ -196,16 +211,20 @@ __device__ __forceinline__ int dda_start(DDA &s, f3 o, f3 d, f3 inv, uint32_t de
     s.ex = inv.x > 0.0f ? 0.001f : -0.001f;
     s.ey = inv.y > 0.0f ? 0.001f : -0.001f;
     s.ez = inv.z > 0.0f ? 0.001f : -0.001f;
+    s.exact = __builtin_isinf(inv.x) || __builtin_isinf(inv.y) || __builtin_isinf(inv.z);
     s.o = add(o, muls(d, 4.0f));   // internal.glsl:361
     if (in_bounds(s.o, fl)) {
         s.r = s.o;
         return DDA_CONTINUE;
     }
-    // intersect(ray, vec3(0), vec3(N)), internal.glsl:245-253
+    // intersect(ray, vec3(0), vec3(N)), internal.glsl:245-253, with the
+    // association llvmpipe's compiler gives it: t1 = (0.001 - o) * inv,
+    // t2 = ((N - 0.001) - o) * inv
     const f3 oo = s.o;
-    const float t1x = ((0.0f - oo.x) + 0.001f) * inv.x, t2x = ((fl - oo.x) - 0.001f) * inv.x;
-    const float t1y = ((0.0f - oo.y) + 0.001f) * inv.y, t2y = ((fl - oo.y) - 0.001f) * inv.y;
-    const float t1z = ((0.0f - oo.z) + 0.001f) * inv.z, t2z = ((fl - oo.z) - 0.001f) * inv.z;
+    const float fe = fl - 0.001f;
+    const float t1x = (0.001f - oo.x) * inv.x, t2x = (fe - oo.x) * inv.x;
+    const float t1y = (0.001f - oo.y) * inv.y, t2y = (fe - oo.y) * inv.y;
+    const float t1z = (0.001f - oo.z) * inv.z, t2z = (fe - oo.z) * inv.z;
     const float te = tmax(tmax(tmin(t1x, t2x), tmin(t1y, t2y)), tmin(t1z, t2z));
     const float tx = tmin(tmin(tmax(t1x, t2x), tmax(t1y, t2y)), tmax(t1z, t2z));
     s.q = 1;
@@ -306,18 +325,29 @@ __device__ __forceinline__ Hit raycast(const uint32_t *__restrict__ words, const
         pz = uz;
         const uint32_t tsize = len >> dd;
         const uint32_t tpx = ux & ~(tsize - 1u), tpy = uy & ~(tsize - 1u), tpz = uz & ~(tsize - 1u);
-        // intersect_inside(ray, pos, pos + size), internal.glsl:255-261:
-        // tx = min over axes of max(t1, t2) with t1 = ((pos - o) - 0.001) * inv and
-        // t2 = (((pos + size) - o) + 0.001) * inv.  The second numerator exceeds
-        // the first, so the max is t2 where inv > 0 (+inf included) and t1 where
-        // inv < 0; neither can be NaN (for a zero direction component the cell
-        // contains the origin, so the chosen numerator is non-zero), and
-        // pos + size is exact as an integer.  Only the far face is evaluated.
-        const float fx = (float)(tpx + (tsize & s.mx)), fy = (float)(tpy + (tsize & s.my)),
-                    fz = (float)(tpz + (tsize & s.mz));
-        const float ax = ((fx - oo.x) + s.ex) * inv.x, ay = ((fy - oo.y) + s.ey) * inv.y,
-                    az = ((fz - oo.z) + s.ez) * inv.z;
-        const float tx = tmin(tmin(ax, ay), az);
+        // intersect_inside(ray, pos, pos + size), internal.glsl:255-261, with
+        // the association llvmpipe's compiler gives it: tx = min over axes of
+        // max(t1, t2), t1 = ((pos - 0.001) - o) * inv, t2 = (((pos + 0.001) +
+        // size) - o) * inv.  The second numerator exceeds the first, so for a
+        // finite inv the max is t2 where inv > 0 and t1 where inv < 0: only the
+        // far face is evaluated ((pos -+ 0.001) + 0 is pos -+ 0.001 exactly).
+        // An infinite component (zero direction) can make a numerator zero
+        // (pos - 0.001 rounds to pos from 32768 up) and a product NaN, so
+        // such rays take the full form.
+        const float pxf = (float)tpx, pyf = (float)tpy, pzf = (float)tpz;
+        float tx;
+        if (!s.exact) {
+            const float ax = (((pxf + s.ex) + (float)(tsize & s.mx)) - oo.x) * inv.x,
+                        ay = (((pyf + s.ey) + (float)(tsize & s.my)) - oo.y) * inv.y,
+                        az = (((pzf + s.ez) + (float)(tsize & s.mz)) - oo.z) * inv.z;
+            tx = tmin(tmin(ax, ay), az);
+        } else {
+            const float szf = (float)tsize;
+            const float ax = tmax(((pxf - 0.001f) - oo.x) * inv.x, (((pxf + 0.001f) + szf) - oo.x) * inv.x),
+                        ay = tmax(((pyf - 0.001f) - oo.y) * inv.y, (((pyf + 0.001f) + szf) - oo.y) * inv.y),
+                        az = tmax(((pzf - 0.001f) - oo.z) * inv.z, (((pzf + 0.001f) + szf) - oo.z) * inv.z);
+            tx = tmin(tmin(ax, ay), az);
+        }
         r = add(muls(d, tx), oo);
     }
     checks = q;
@@ -419,7 +449,9 @@ __device__ __forceinline__ f3 evaluate_brdf(f3 Vd, f3 L, f3 N, f3 albedo, float 
     const float specular = spec_norm * m_pow(NdotH, spec_exp);
     const float kS = spec_int * metallic;
     const float kD = 1.0f - kS * 0.5f;
-    const f3 diffuse = muls(muls(albedo, kD), 0.31831f);
+    // albedo * kD * 0.31831 as llvmpipe's compiler reassociates it: the
+    // constant factor moves next to the first operand
+    const f3 diffuse = muls(muls(albedo, 0.31831f), kD);
     const float sp = specular * kS;
     return muls(add(diffuse, f3{sp, sp, sp}), NdotL);
 }

@@ -6,8 +6,9 @@
 // Launch shape.  Screen passes run one 64-lane wavefront per 8x8 pixel tile
 // (a tile's rays are coherent, so the lanes walk the same octree nodes), four
 // wavefronts per 256-thread block.  Blocks are handed out round-robin to the
-// 8 XCDs by the hardware; the block -> tile map below gives each XCD one
-// contiguous band of tiles so the nodes a band touches stay in that XCD's L2.
+// 8 XCDs by the hardware; the block -> tile map (xcd_group) gives each XCD
+// runs of consecutive tiles, so neighbouring rays share an L2, while every
+// XCD gets runs from every part of the image (even load: sky and terrain).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -22,14 +23,6 @@ namespace ve {
 constexpr int TILE = 8;
 constexpr int WAVES_PER_BLOCK = 4;
 constexpr int NUM_XCD = 8;
-
-// Logical tile-group index of this block with the XCD-banded mapping; the
-// grid is padded to a multiple of NUM_XCD blocks.
-__device__ __forceinline__ uint32_t xcd_banded_block(uint32_t b, uint32_t nblocks_padded)
-{
-    const uint32_t per = nblocks_padded / NUM_XCD;
-    return (b % NUM_XCD) * per + b / NUM_XCD;
-}
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
 {
@@ -52,7 +45,7 @@ __global__ __launch_bounds__(256) void k_primary(const uint32_t *__restrict__ wo
     }
     const uint32_t tiles_x = (p.W + TILE - 1) / TILE;
     const uint32_t tiles_y = (p.H + TILE - 1) / TILE;
-    const uint32_t group = xcd_banded_block(blockIdx.x, gridDim.x);
+    const uint32_t group = xcd_group(blockIdx.x, gridDim.x, p.tile_chunk);
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t tile = group * WAVES_PER_BLOCK + wave;
     if (tile >= tiles_x * tiles_y) return;
@@ -85,16 +78,17 @@ __global__ __launch_bounds__(256) void k_primary(const uint32_t *__restrict__ wo
     }
 }
 
-static uint32_t padded_blocks(uint32_t work_groups)
+static uint32_t padded_blocks(uint32_t work_groups, uint32_t chunk)
 {
     uint32_t nb = work_groups == 0 ? 1 : work_groups;
-    return (nb + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+    const uint32_t unit = NUM_XCD * (chunk ? chunk : 1u);
+    return (nb + unit - 1) / unit * unit;
 }
 
 void launch_primary(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, bool stats)
 {
     const uint32_t tiles = ((p.W + TILE - 1) / TILE) * ((p.H + TILE - 1) / TILE);
-    const uint32_t nb = padded_blocks((tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
+    const uint32_t nb = padded_blocks((tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK, p.tile_chunk);
     if (stats)
         launch(k_primary<true>, dim3(nb), dim3(256), 0, s, words, p, b.gbuf, b.gpos, b.node_stats,
                            b.counters, b.ray_totals, b.ghit);
@@ -216,11 +210,12 @@ extern "C" int ve_pass_primary_noalloc(const uint32_t *words, uint32_t n_words, 
     p.primary_cc = (uint32_t)primary_controlchecks;
     p.count_primary_in_reset = 1;   // the ray counters are the context's, not written here
     p.reuse_primary_hits = 0;       // no hit records kept
+    p.tile_chunk = DEFAULT_TILE_CHUNK;
     std::memcpy(p.cam, ubo, sizeof p.cam);
     const hipStream_t s = static_cast<hipStream_t>(stream);
     launch_build_jump(s, words, n_words, depth, jl, jump);
     const uint32_t tiles = ((width + TILE - 1) / TILE) * ((height + TILE - 1) / TILE);
-    const uint32_t nb = padded_blocks((tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
+    const uint32_t nb = padded_blocks((tiles + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK, p.tile_chunk);
     launch(k_primary<false>, dim3(nb), dim3(256), 0, s, words, p, reinterpret_cast<uint4 *>(gbuffer), gbuffer_pos,
            reinterpret_cast<unsigned long long *>(sp + 16), sp, reinterpret_cast<unsigned long long *>(sp + 24),
            static_cast<uint2 *>(nullptr));

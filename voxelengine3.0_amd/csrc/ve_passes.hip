@@ -35,17 +35,11 @@ constexpr int NUM_XCD = 8;
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t xcd_banded(uint32_t b, uint32_t nb_padded)
-{
-    const uint32_t per = nb_padded / NUM_XCD;
-    return (b % NUM_XCD) * per + b / NUM_XCD;
-}
-
 __device__ __forceinline__ bool tile_pixel(const FrameParams &p, uint32_t &x, uint32_t &y)
 {
     const uint32_t tiles_x = (p.W + TILE - 1) / TILE;
     const uint32_t tiles_y = (p.H + TILE - 1) / TILE;
-    const uint32_t tile = xcd_banded(blockIdx.x, gridDim.x) * 4u + (threadIdx.x >> 6);
+    const uint32_t tile = xcd_group(blockIdx.x, gridDim.x, p.tile_chunk) * 4u + (threadIdx.x >> 6);
     if (tile >= tiles_x * tiles_y) return false;
     const uint32_t lane = threadIdx.x & 63u;
     x = (tile % tiles_x) * TILE + (lane & 7u);
@@ -2003,9 +1997,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
             a += __shfl_xor(a, off, 64);
             c += __shfl_xor(c, off, 64);
         }
-        if (lane == 0u) {
-            atomicAdd(&stats[1], a);
-            atomicAdd(&stats[3], c);
+        if (lane == 0u) {   // the fused kernel's share, kept apart from k_trace's
+            atomicAdd(&stats[4], a);
+            atomicAdd(&stats[5], c);
         }
     }
 }
@@ -2805,7 +2799,8 @@ static uint32_t screen_blocks(const FrameParams &p)
     const uint32_t tiles = ((p.W + TILE - 1) / TILE) * ((p.H + TILE - 1) / TILE);
     uint32_t nb = (tiles + 3u) / 4u;
     nb = nb == 0 ? 1 : nb;
-    return (nb + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+    const uint32_t unit = NUM_XCD * (p.tile_chunk ? p.tile_chunk : 1u);
+    return (nb + unit - 1) / unit * unit;
 }
 
 void launch_fill(hipStream_t s, uint32_t *p, uint32_t v, size_t n)

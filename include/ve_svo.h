@@ -291,6 +291,17 @@ int ve_sync(ve_ctx *ctx);
  * context without the second stream (server sessions). */
 int ve_set_overlap(ve_ctx *ctx, int enable);
 
+/* No counterpart in the reference: graph replay.  While enabled (and the
+ * overlap is off, and timing is off), ve_frame captures the frame's launches
+ * once into a HIP graph and replays it for later frames; the frame's varying
+ * inputs (frame_idx, frame_seed, time, camera) reach the kernels through a
+ * small device block copied before each replay, so results equal those of
+ * direct launches bit for bit.  A frame whose other parameters differ
+ * (size, knobs, scene, sky) is captured anew.  ve_graph_stats counts the
+ * captures and replays. */
+int ve_set_graph(ve_ctx *ctx, int enable);
+int ve_graph_stats(ve_ctx *ctx, uint64_t *captures, uint64_t *replays);
+
 /* Optional per-pass timing: while enabled, every ve_frame binds HIP events to
  * the first and last kernel dispatch of each pass (no host synchronisation,
  * no marker packets); ve_timings waits for the streams and returns, per pass,

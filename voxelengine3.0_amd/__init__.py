@@ -154,6 +154,8 @@ def lib():
         "ve_node_stats_bounce": (i32, [p] + [C.POINTER(C.c_uint64)] * 2),
         "ve_timings_iter": (i32, [p, u32, p]),
         "ve_set_tuning": (i32, [p, p]),
+        "ve_set_graph": (i32, [p, i32]),
+        "ve_graph_stats": (i32, [p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "ve_get_tuning": (i32, [p, p]),
         "ve_ray_totals": (i32, [p] + [C.POINTER(C.c_uint64)] * 3),
         "ve_ray_totals_reused": (i32, [p, C.POINTER(C.c_uint64)]),
@@ -429,6 +431,15 @@ class Renderer:
             slots = np.ascontiguousarray(octree.changed, np.uint32)
         self._ck(lib().ve_scene_update(self.h, scene, words.ctypes.data, octree.size,
                                        slots.ctypes.data if len(slots) else None, len(slots)))
+
+    def set_graph(self, on=True):
+        """ve_set_graph: replay frames from a captured HIP graph."""
+        self._ck(lib().ve_set_graph(self.h, 1 if on else 0))
+
+    def graph_stats(self):
+        a, b = C.c_uint64(), C.c_uint64()
+        self._ck(lib().ve_graph_stats(self.h, C.byref(a), C.byref(b)))
+        return dict(captures=a.value, replays=b.value)
 
     def set_tuning(self, tile_chunk=0, jump_level=0):
         """ve_set_tuning: launch-shape knobs (speed only, never results)."""

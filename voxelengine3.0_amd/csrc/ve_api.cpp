@@ -131,6 +131,7 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.ord_over_n, 2);
     a(&b.ord_spill, P);
     a(&b.ord_work, (size_t)hash_width * NBUFFER_SLOTS);
+    a(&b.dyn, 1);
 }
 
 // RankStats thresholds start at (+inf, -inf): every voxel in the mid tier on
@@ -360,6 +361,10 @@ void ve_destroy(ve_ctx *c)
             if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : {c->ev_primary, c->ev_emit, c->ev_fence, c->ev_head, c->ev_normal})
         if (e) (void)hipEventDestroy(e);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    for (hipEvent_t e : c->dyn_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->dyn_host) (void)hipHostFree(c->dyn_host);
     if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->aux2) (void)hipStreamDestroy(c->aux2);
     if (c->stream && c->owns_stream) (void)hipStreamDestroy(c->stream);
@@ -649,7 +654,7 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &p)
         HIP_OK(c, hipStreamWaitEvent(c->stream, c->ev_primary, 0));
     } else {
         if ((rc = timed_pass(c, s, VE_PASS_RESET, p, 0, ev))) return rc;
-        if ((rc = timed_pass(c, s, VE_PASS_PRIMARY, p, 0, ev))) return rc;
+        if (!c->skip_primary && (rc = timed_pass(c, s, VE_PASS_PRIMARY, p, 0, ev))) return rc;
     }
     if (c->overlap) {   // normal pass beside schedule .. emit
         HIP_OK(c, hipEventRecord(c->ev_head, c->stream));
@@ -706,9 +711,86 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &p)
     return VE_OK;
 }
 
+// One frame as a graph replay: the per-frame inputs go to the device block
+// b.dyn (a copy on the context's stream from a pinned staging slot), the
+// kernels read them through FrameParams::dyn, and the frame's launches --
+// everything else in their parameters unchanged -- are replayed from the
+// captured graph; a frame whose other parameters differ is captured anew.
+// The staging slot is rewritten only once its previous copy has run.
+int frame_graph(ve_ctx *c, ve_scene *s, const FrameParams &p)
+{
+    if (!c->dyn_host) {
+        void *h = nullptr;
+        HIP_OK(c, hipHostMalloc(&h, ve_ctx::DYN_RING * sizeof(FrameDyn), hipHostMallocDefault));
+        c->dyn_host = static_cast<FrameDyn *>(h);
+        for (auto &e : c->dyn_ev) HIP_OK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    const uint32_t j = c->dyn_next++ % ve_ctx::DYN_RING;
+    HIP_OK(c, hipEventSynchronize(c->dyn_ev[j]));   // returns at once for a slot never copied from
+    FrameDyn &d = c->dyn_host[j];
+    d.frame_idx = p.frame_idx;
+    d.frame_seed = p.frame_seed;
+    d.time = p.time;
+    d.pad = 0;
+    std::memcpy(d.cam, p.cam, sizeof d.cam);
+    HIP_OK(c, hipMemcpyAsync(c->b.dyn, &d, sizeof d, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(c, hipEventRecord(c->dyn_ev[j], c->stream));
+    FrameParams q = p;
+    q.frame_idx = q.frame_seed = q.time = 0;
+    std::memset(q.cam, 0, sizeof q.cam);
+    q.dyn = c->b.dyn;
+    // the node array's address is a kernel argument beside the parameters
+    std::vector<uint8_t> key(sizeof q + sizeof(void *) + 2);
+    std::memcpy(key.data(), &q, sizeof q);
+    const void *words = s->words;
+    std::memcpy(key.data() + sizeof q, &words, sizeof words);
+    key[sizeof q + sizeof words] = c->skip_primary ? 1 : 0;
+    key[sizeof q + sizeof words + 1] = c->node_stats ? 1 : 0;
+    if (!c->gexec || key != c->gkey) {
+        if (c->gexec) {
+            HIP_OK(c, hipGraphExecDestroy(c->gexec));
+            c->gexec = nullptr;
+        }
+        HIP_OK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        const int rc = frame_passes(c, s, q);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : ctx_error(c, VE_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+        }
+        const hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            c->gexec = nullptr;
+            return ctx_error(c, VE_EDEVICE, "graph instantiate: %s", hipGetErrorString(ei));
+        }
+        c->gkey = std::move(key);
+        c->graph_captures++;
+    }
+    HIP_OK(c, hipGraphLaunch(c->gexec, c->stream));
+    c->graph_replays++;
+    return VE_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int ve_set_graph(ve_ctx *c, int enable)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    c->graph = enable != 0;
+    return VE_OK;
+}
+
+int ve_graph_stats(ve_ctx *c, uint64_t *captures, uint64_t *replays)
+{
+    if (!c) return ve_global_error(VE_EINVAL, "null context");
+    if (captures) *captures = c->graph_captures;
+    if (replays) *replays = c->graph_replays;
+    return VE_OK;
+}
 
 int ve_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const ve_frame_config *fc, const ve_frame_inputs *in)
 {
@@ -732,6 +814,7 @@ int ve_frame(ve_ctx *c, ve_scene *s, const ve_frame_config *fc, const ve_frame_i
     fill_params(c, s, fc, in, p);
     p.count_primary_in_reset = 1;
     p.reuse_primary_hits = c->node_stats ? 0 : 1;   // node counting traces every ray as the reference does
+    if (c->graph && !c->overlap && !c->timing) return frame_graph(c, s, p);
     rc = frame_passes(c, s, p);
     // a frame left half-enqueued may still read either G-buffer set: the next
     // primary pass waits for everything on the stream

@@ -81,6 +81,20 @@ struct ve_ctx {
     hipEvent_t ev_head = nullptr;           // on stream: the frame's reset pass, after its primary pass
     hipEvent_t ev_normal = nullptr;         // on aux2: normal pass written
     bool aux_fence = true;                  // the next primary must follow all of stream's work
+    // Graph replay (ve_set_graph): the frame's launches are captured once into
+    // a HIP graph and replayed; the per-frame inputs travel in the device
+    // block b.dyn, copied from a pinned staging ring before each replay.  A
+    // new capture is made when anything else in the frame's parameters changes.
+    static constexpr uint32_t DYN_RING = 8;
+    bool graph = false;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uint8_t> gkey;              // parameters of the captured frame (per-frame inputs cleared)
+    ve::FrameDyn *dyn_host = nullptr;       // pinned, DYN_RING slots
+    hipEvent_t dyn_ev[DYN_RING] = {};       // the copy from each slot
+    uint32_t dyn_next = 0;
+    uint64_t graph_captures = 0, graph_replays = 0;
+    // the frame's primary pass was run by a batch launch (ve_batch_frame)
+    bool skip_primary = false;
     std::vector<std::pair<void *, size_t>> allocations;
     size_t device_bytes = 0;
 };

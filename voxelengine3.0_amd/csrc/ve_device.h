@@ -51,6 +51,12 @@ __device__ __forceinline__ f3 normalize(f3 v)
 }
 __device__ __forceinline__ f3 rcp3(f3 d) { return f3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z}; }
 
+// The per-frame inputs: kernel arguments, or a graph replay's device block.
+__device__ __forceinline__ uint32_t fp_frame_idx(const FrameParams &p) { return p.dyn ? p.dyn->frame_idx : p.frame_idx; }
+__device__ __forceinline__ uint32_t fp_frame_seed(const FrameParams &p) { return p.dyn ? p.dyn->frame_seed : p.frame_seed; }
+__device__ __forceinline__ uint32_t fp_time(const FrameParams &p) { return p.dyn ? p.dyn->time : p.time; }
+__device__ __forceinline__ const float *fp_cam(const FrameParams &p) { return p.dyn ? p.dyn->cam : p.cam; }
+
 // Block b of a screen pass (4 tiles of 8x8 pixels per block) -> its tile
 // group.  The hardware deals blocks to the 8 XCDs round-robin (XCD b % 8).
 // chunk 0: XCD x takes the x-th contiguous eighth of the groups (one band of

@@ -138,6 +138,14 @@ struct EmissiveVoxel {
 static_assert(sizeof(EmissiveVoxel) == 32, "EmissiveVoxel is 32 bytes");
 
 // Frame parameters handed to every kernel by value.
+// The inputs that change from frame to frame (Renderer::frameID, frameSeed,
+// glTime and the camera UBO).  A frame replayed from a captured graph reads
+// them from a device copy (FrameParams::dyn) instead of its kernel arguments.
+struct FrameDyn {
+    uint32_t frame_idx, frame_seed, time, pad;
+    float cam[16];
+};
+
 struct FrameParams {
     uint32_t W, H;
     uint32_t depth;
@@ -176,6 +184,8 @@ struct FrameParams {
     uint32_t jump_level;
     // screen passes: tile groups per XCD run (0 = one band per XCD)
     uint32_t tile_chunk;
+    // non-null: frame_idx, frame_seed, time and cam come from this device block
+    const FrameDyn *dyn;
     float cam[16];
     float sky[3];
 };

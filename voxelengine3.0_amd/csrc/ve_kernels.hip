@@ -54,8 +54,8 @@ __global__ __launch_bounds__(256) void k_primary(const uint32_t *__restrict__ wo
     __shared__ uint32_t path_cache[VE_MAX_DEPTH * 256];
     uint32_t checks = 0, reads = 0;
     if (x < p.W && y < p.H) {
-        const f3 dir = primary_dir(p.cam, x, y, p.W, p.H);
-        const f3 org = mk(p.cam[0], p.cam[1], p.cam[2]);
+        const f3 dir = primary_dir(fp_cam(p), x, y, p.W, p.H);
+        const f3 org = mk(fp_cam(p)[0], fp_cam(p)[1], fp_cam(p)[2]);
         const Hit h = raycast<STATS>(words, p.jump, p.jump_level, p.n_words, p.depth, org, dir, rcp3(dir), p.primary_cc, checks, reads,
                                      path_cache + threadIdx.x, 256u);
         const size_t pix = (size_t)y * p.W + x;

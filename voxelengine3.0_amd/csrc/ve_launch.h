@@ -92,6 +92,7 @@ struct DeviceBuffers {
     uint32_t *ord_over_n = nullptr;      // 2: overflow events, spill words handed out (zero between passes)
     uint64_t *ord_spill = nullptr;       // ord_spill_cap: sort keys of buckets past ord_cap
     uint32_t *ord_work = nullptr;        // hash_width x 32: packed position of each slot's new normal, or NONE
+    FrameDyn *dyn = nullptr;             // 1: per-frame inputs of a graph replay
 };
 
 void launch_primary(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, bool stats);

@@ -527,7 +527,7 @@ __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_nrm_replay(FrameParams p,
             owner = ev_vid;
             has_src = true;
             src = ev_pos;
-            tstamp = p.time;
+            tstamp = fp_time(p);
         }
         ev_vid = 0u;
     };
@@ -554,7 +554,7 @@ __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_nrm_replay(FrameParams p,
                         has_src = true;
                         src = pos;
                     }
-                    tstamp = p.time;
+                    tstamp = fp_time(p);
                 }
             } else {   // lockstep: the vector's last slotless lane wins the evicted slot
                 ev_vid = vid;
@@ -606,7 +606,7 @@ __device__ __forceinline__ uint32_t sched_tile(uint32_t g, uint32_t total, uint3
 __device__ __forceinline__ uint32_t sched_pixel(uint32_t e, const FrameParams &p, uint32_t &pix)
 {
     const uint32_t gw = (p.W + 7u) / 8u, gh = (p.H + 7u) / 8u;
-    const uint32_t tile = sched_tile(e >> 6, gw * gh, p.frame_seed);
+    const uint32_t tile = sched_tile(e >> 6, gw * gh, fp_frame_seed(p));
     const uint32_t x = (tile % gw) * 8u + (e & 7u), y = (tile / gw) * 8u + ((e >> 3) & 7u);
     pix = y * p.W + x;
     return x < p.W && y < p.H;
@@ -1199,14 +1199,14 @@ __global__ __launch_bounds__(256) void k_emit(FrameParams p, const uint2 *__rest
     if (t < n) {
         const uint2 en = shade_list[t];
         const uint32_t px = en.y % p.W, py = en.y / p.W;
-        const f3 d = primary_dir(p.cam, px, py, p.W, p.H);
+        const f3 d = primary_dir(fp_cam(p), px, py, p.W, p.H);
         Ray r;
-        r.ox = p.cam[0]; r.oy = p.cam[1]; r.oz = p.cam[2]; r.pad0 = 0.0f;
+        r.ox = fp_cam(p)[0]; r.oy = fp_cam(p)[1]; r.oz = fp_cam(p)[2]; r.pad0 = 0.0f;
         r.dx = d.x; r.dy = d.y; r.dz = d.z; r.pad1 = 0.0f;
         r.tx = r.ty = r.tz = 1.0f;
         r.primary_vid = en.x;
         r.dlx = r.dly = r.dlz = 0.0f;
-        const uint32_t sample_idx = p.frame_idx + en.y * 2654435761u + t * 1442695040u;
+        const uint32_t sample_idx = fp_frame_idx(p) + en.y * 2654435761u + t * 1442695040u;
         r.rng = (sample_idx << 8) & 0xFFFFFF00u;
         r.ilx = r.ily = r.ilz = 0.0f;
         r.bounce = 0u;
@@ -1537,7 +1537,7 @@ __device__ bool rbuf_load(const FrameParams &p, const uint32_t *__restrict__ rbu
     r.m = pl[4];
     r.target_pdf = __uint_as_float(pl[5]);
     const uint32_t t = bucket[RB_TIME + s];
-    if (!p.disable_stale_reset && t != 0u && t + p.update_time < p.time) {
+    if (!p.disable_stale_reset && t != 0u && t + p.update_time < fp_time(p)) {
         r = Reservoir{0u, 0u, 0u, 0.0f, 0u, 0.0f};
         return false;
     }
@@ -1570,7 +1570,7 @@ __device__ void rbuf_store(const FrameParams &p, uint32_t *__restrict__ rbuf, ui
     pl[3] = __float_as_uint(r.wsum);
     pl[4] = r.m;
     pl[5] = __float_as_uint(r.target_pdf);
-    bucket[RB_TIME + s] = p.time;
+    bucket[RB_TIME + s] = fp_time(p);
 }
 
 __device__ __forceinline__ bool light_valid(const Reservoir &r, const EmissiveVoxel *__restrict__ em, uint32_t count)
@@ -2187,8 +2187,8 @@ __device__ void deposit_lockstep(const FrameParams &p, uint32_t *bk, uint32_t k,
         }
         uint32_t *pl = bk + LB_PAY + i * LB_PAYW;
         for (uint32_t l = 0; l < k; l++)   // stale reset (shade.comp:183-186)
-            if (match[l] && !p.disable_stale_reset && st != 0u && st + p.update_time < p.time) {
-                lb_reset(bk, i, vid[l], d[l], e[l], p.time);
+            if (match[l] && !p.disable_stale_reset && st != 0u && st + p.update_time < fp_time(p)) {
+                lb_reset(bk, i, vid[l], d[l], e[l], fp_time(p));
                 match[l] = active[l] = false;
             }
         for (uint32_t ch = 0; ch < 2; ch++) {   // halving at the caps (shade.comp:188-194)
@@ -2205,12 +2205,12 @@ __device__ void deposit_lockstep(const FrameParams &p, uint32_t *bk, uint32_t k,
         for (uint32_t l = 0; l < k; l++)
             if (match[l]) {
                 lb_add(pl, d[l], e[l]);
-                bk[LB_TIME + i] = p.time;
+                bk[LB_TIME + i] = fp_time(p);
                 active[l] = false;
             }
     }
     for (uint32_t l = 0; l < k; l++)   // bucket full: reset the candidate (shade.comp:254-256)
-        if (active[l] && cand_t[l] != NONE) lb_reset(bk, cand_i[l], vid[l], d[l], e[l], p.time);
+        if (active[l] && cand_t[l] != NONE) lb_reset(bk, cand_i[l], vid[l], d[l], e[l], fp_time(p));
 }
 
 // Lane-held slot of an lBuffer bucket: owner, time stamp, payload.
@@ -2232,8 +2232,8 @@ __device__ __forceinline__ void lslot_deposit(const FrameParams &p, LSlot &s, ui
 {
     const uint32_t st = s.tstamp;
     if (s.owner == 0u) s.owner = vid;
-    if (!p.disable_stale_reset && st != 0u && st + p.update_time < p.time) {   // shade.comp:183-186
-        lslot_reset(s, vid, d, e, p.time);
+    if (!p.disable_stale_reset && st != 0u && st + p.update_time < fp_time(p)) {   // shade.comp:183-186
+        lslot_reset(s, vid, d, e, fp_time(p));
         return;
     }
     if (s.pl[0] >= p.cap_direct)   // shade.comp:188-194
@@ -2243,7 +2243,7 @@ __device__ __forceinline__ void lslot_deposit(const FrameParams &p, LSlot &s, ui
     lb_firefly(s.pl, 0, d, p.firefly_k, p.firefly_floor);
     lb_firefly(s.pl, 5, e, p.firefly_k, p.firefly_floor);
     lb_add(s.pl, d, e);
-    s.tstamp = p.time;
+    s.tstamp = fp_time(p);
 }
 
 // The order key of record `rec` (without its list index): chunk, bounce
@@ -2351,7 +2351,7 @@ __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_dep_replay(FrameParams p,
                 const uint32_t best = wave_min_u32(sl ? st.tstamp : NONE);
                 const uint64_t at = __ballot(sl && st.tstamp == best);
                 if (best != NONE && lane == (uint32_t)__ffsll((unsigned long long)at) - 1u)
-                    lslot_reset(st, vid, d, e, p.time);
+                    lslot_reset(st, vid, d, e, fp_time(p));
             }
         } else {   // several lanes of one vector here: lockstep, on an LDS copy
             if (sl) {
@@ -2522,7 +2522,7 @@ __global__ __launch_bounds__(256) void k_resolve(FrameParams p, const uint4 *__r
         const float v = (float)checks / 65.0f;
         col = f3{v, v, v};
     } else if (vid == 0u) {   // texture(skybox, pixelDirection(pix)) (resolve.comp:170-180,227)
-        const f3 d = primary_dir(p.cam, x, y, p.W, p.H);
+        const f3 d = primary_dir(fp_cam(p), x, y, p.W, p.H);
         col = sky_color(p, d.x, d.y, d.z);
     } else if (p.render_type == 2) {
         col = (mi == 0u || mi >= 127u) ? f3{0, 0, 0} : f3{mats[mi].color[0], mats[mi].color[1], mats[mi].color[2]};
@@ -2686,7 +2686,7 @@ __device__ __forceinline__ void resolve_timeout_tiles(FrameParams p, const uint4
                     if (ws > 0.0001f) {
                         col = f3{sr / ws, sg / ws, sb / ws};
                     } else {   // resolve.comp:218
-                        const f3 d = primary_dir(p.cam, (uint32_t)px, (uint32_t)py, p.W, p.H);
+                        const f3 d = primary_dir(fp_cam(p), (uint32_t)px, (uint32_t)py, p.W, p.H);
                         col = sky_color(p, d.x, d.y, d.z);
                     }
                     out[(size_t)py * p.W + px] = make_float4(col.x, col.y, col.z, 1.0f);
@@ -2744,7 +2744,7 @@ __device__ __forceinline__ void resolve_timeout_list(FrameParams p, const uint4 
             if (ws > 0.0001f) {
                 col = f3{sr / ws, sg / ws, sb / ws};
             } else {   // resolve.comp:218
-                const f3 d = primary_dir(p.cam, (uint32_t)px, (uint32_t)py, p.W, p.H);
+                const f3 d = primary_dir(fp_cam(p), (uint32_t)px, (uint32_t)py, p.W, p.H);
                 col = sky_color(p, d.x, d.y, d.z);
             }
             out[pix] = make_float4(col.x, col.y, col.z, 1.0f);

@@ -302,6 +302,36 @@ int ve_set_overlap(ve_ctx *ctx, int enable);
 int ve_set_graph(ve_ctx *ctx, int enable);
 int ve_graph_stats(ve_ctx *ctx, uint64_t *captures, uint64_t *replays);
 
+/* ---- batched form -------------------------------------------------------------
+ * No counterpart in the reference: frame k of many views at once.  Their
+ * primary passes (generate_primary.comp) run in one persistent launch whose
+ * waves pull tiles from a global queue over all the views' tiles; each
+ * view's other passes then run on its own context's stream, replayed from its
+ * frame graph when ve_set_graph is on.  Every view's results equal those of
+ * ve_frame on its context bit for bit.  The contexts need the cross-frame
+ * overlap, timing and node counting off, and each view's camera is its
+ * context's (ve_set_camera).  Launch shape: tiles of tile_w x (64 / tile_w)
+ * pixels (tile_w 8, 16, 32 or 64; 0 = 8), `grab` tiles per queue pull (0 = 4),
+ * a grid of waves_per_cu waves per compute unit (0 = 16); at most max_views
+ * views per call (0 = 256).  voxelengine3.0_amd/autotune.py picks them. */
+typedef struct ve_batch ve_batch;
+typedef struct ve_batch_config {
+    int32_t  device;
+    uint32_t max_views;
+    uint32_t tile_w;
+    uint32_t grab;
+    uint32_t waves_per_cu;
+} ve_batch_config;
+int  ve_batch_create(ve_batch **out, const ve_batch_config *cfg);
+void ve_batch_destroy(ve_batch *b);
+const char *ve_batch_last_error(const ve_batch *b);
+/* Enqueues one frame of each view (fcs[i], ins[i] as for ve_frame); no host
+ * synchronisation. */
+int  ve_batch_frame(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const *scenes,
+                    const ve_frame_config *fcs, const ve_frame_inputs *ins);
+/* Milliseconds of the last batched primary launch (waits for it). */
+int  ve_batch_primary_ms(ve_batch *b, float *ms);
+
 /* Optional per-pass timing: while enabled, every ve_frame binds HIP events to
  * the first and last kernel dispatch of each pass (no host synchronisation,
  * no marker packets); ve_timings waits for the streams and returns, per pass,

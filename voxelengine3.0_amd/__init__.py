@@ -114,6 +114,11 @@ class ServerStats(C.Structure):
                 ("pooled", C.c_uint32), ("bytes_in_use", C.c_uint64), ("budget", C.c_uint64)]
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("max_views", C.c_uint32), ("tile_w", C.c_uint32), ("grab", C.c_uint32),
+                ("waves_per_cu", C.c_uint32)]
+
+
 _lib = None
 
 
@@ -155,6 +160,11 @@ def lib():
         "ve_timings_iter": (i32, [p, u32, p]),
         "ve_set_tuning": (i32, [p, p]),
         "ve_set_graph": (i32, [p, i32]),
+        "ve_batch_create": (i32, [C.POINTER(p), C.POINTER(BatchConfig)]),
+        "ve_batch_destroy": (None, [p]),
+        "ve_batch_last_error": (C.c_char_p, [p]),
+        "ve_batch_frame": (i32, [p, u32, p, p, p, p]),
+        "ve_batch_primary_ms": (i32, [p, C.POINTER(C.c_float)]),
         "ve_graph_stats": (i32, [p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "ve_get_tuning": (i32, [p, p]),
         "ve_ray_totals": (i32, [p] + [C.POINTER(C.c_uint64)] * 3),
@@ -382,6 +392,45 @@ def primary_noalloc(words_dev, n_words, depth, ubo, width, height, controlchecks
     ubo = np.ascontiguousarray(ubo, np.float32)
     _check(lib().ve_pass_primary_noalloc(words_dev, n_words, depth, ubo.ctypes.data, width, height, controlchecks,
                                          gbuffer_dev, gbuffer_pos_dev, stream))
+
+
+class Batch:
+    """ve_batch_*: frame k of many Renderers at once (one persistent primary
+    launch over all views, the other passes on each view's stream)."""
+
+    def __init__(self, device=0, max_views=256, tile_w=8, grab=4, waves_per_cu=16):
+        cfg = BatchConfig(device, max_views, tile_w, grab, waves_per_cu)
+        h = C.c_void_p()
+        _check(lib().ve_batch_create(C.byref(h), C.byref(cfg)))
+        self.h = h
+        self.config = dict(tile_w=tile_w, grab=grab, waves_per_cu=waves_per_cu)
+
+    def close(self):
+        if self.h:
+            lib().ve_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def frame(self, renderers, scenes, fcs, inputs):
+        n = len(renderers)
+        ctxs = (C.c_void_p * n)(*[r.h.value for r in renderers])
+        scs = (C.c_void_p * n)(*[s.value for s in scenes])
+        fca = (FrameConfig * n)(*fcs)
+        ina = (FrameInputs * n)(*inputs)
+        rc = lib().ve_batch_frame(self.h, n, ctxs, scs, fca, ina)
+        if rc != VE_OK:
+            msg = lib().ve_batch_last_error(self.h)
+            raise VeError(rc, msg.decode() if msg else "")
+
+    def primary_ms(self):
+        ms = C.c_float()
+        _check(lib().ve_batch_primary_ms(self.h, C.byref(ms)))
+        return ms.value
 
 
 class Renderer:

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libve_svo.so")
 
 HIP_SOURCES = ["ve_kernels.hip", "ve_passes.hip"]
-CXX_SOURCES = ["ve_api.cpp", "ve_server.cpp", "octree_build.cpp", "octree_edit.cpp", "scene_gen.cpp"]
+CXX_SOURCES = ["ve_api.cpp", "ve_server.cpp", "ve_batch.cpp", "octree_build.cpp", "octree_edit.cpp", "scene_gen.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function"]
 DEVICE = ["--offload-arch=gfx950", "-ffp-contract=off", "-fno-fast-math", "-Xclang", "-target-feature",

@@ -1114,6 +1114,27 @@ int ctx_create(ve_ctx **out, const ve_renderer_config *cfg, hipStream_t shared, 
     return VE_OK;
 }
 
+int frame_begin(ve_ctx *c, ve_scene *s, const ve_frame_config *fc, const ve_frame_inputs *in, FrameParams &p)
+{
+    int rc = check_frame(c, s, fc, in);
+    if (rc) return rc;
+    if (c->overlap) return ctx_error(c, VE_ESTATE, "batched frames need the cross-frame overlap off");
+    if (c->timing || c->node_stats) return ctx_error(c, VE_ESTATE, "batched frames run without timing or node counting");
+    HIP_OK(c, hipSetDevice(c->device));
+    fill_params(c, s, fc, in, p);
+    p.count_primary_in_reset = 1;
+    p.reuse_primary_hits = 1;
+    return VE_OK;
+}
+
+int frame_rest(ve_ctx *c, ve_scene *s, const FrameParams &p)
+{
+    c->skip_primary = true;
+    const int rc = c->graph ? frame_graph(c, s, p) : frame_passes(c, s, p);
+    c->skip_primary = false;
+    return rc;
+}
+
 int ctx_reset(ve_ctx *c)
 {
     HIP_OK(c, hipSetDevice(c->device));

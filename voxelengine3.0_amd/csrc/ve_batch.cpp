@@ -1,0 +1,237 @@
+// The batched form: frame k of many views at once.  Their primary passes run
+// in one persistent launch that pulls tiles from a global queue over all the
+// views (launch_primary_batch); every other pass of each view runs on its own
+// context's stream after that launch -- replayed from the context's captured
+// frame graph when graph replay is on.  No counterpart in the reference,
+// which renders one view per Renderer::run.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ve_ctx.h"
+#include "ve_internal.h"
+#include "ve_launch.h"
+
+struct ve_batch {
+    int device = 0;
+    uint32_t max_views = 0;
+    ve_batch_config cfg{};
+    uint32_t blocks = 0;                     // persistent grid
+    hipStream_t stream = nullptr;
+    ve::BatchView *views_dev = nullptr;
+    uint32_t *bases_dev = nullptr;           // max_views + 1
+    uint32_t *queue_dev = nullptr;
+    // pinned staging, two sets: a set is rewritten once its previous copy ran
+    ve::BatchView *views_host[2] = {nullptr, nullptr};
+    uint32_t *bases_host[2] = {nullptr, nullptr};
+    hipEvent_t staged[2] = {nullptr, nullptr};
+    uint32_t flip = 0;
+    std::vector<hipEvent_t> ctx_ev;          // per view: its stream's work before the batch
+    hipEvent_t done = nullptr;               // the batch's primary launch finished
+    hipEvent_t t0 = nullptr, t1 = nullptr;   // around the primary launch
+    std::vector<ve::FrameParams> params;
+    std::string err;
+};
+
+namespace {
+
+int batch_error(ve_batch *b, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (b) b->err = buf;
+    return code;
+}
+
+#define B_HIP(b, call)                                                                              \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) return batch_error(b, VE_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+bool tile_width_ok(uint32_t w)
+{
+    for (uint32_t v : ve::BATCH_TILE_WIDTHS)
+        if (v == w) return true;
+    return false;
+}
+
+}  // namespace
+
+namespace ve {
+
+// The batch's primary launch for views whose parameters frame_begin filled:
+// it waits for everything already on the views' streams, and the views'
+// streams wait for it.  `starts` (optional, one per view) are recorded on the
+// batch stream just before the launch.
+int batch_primary(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const *scenes, const FrameParams *ps,
+                  hipEvent_t *starts)
+{
+    const uint32_t th = 64u / b->cfg.tile_w;
+    const uint32_t k = b->flip;
+    b->flip ^= 1u;
+    B_HIP(b, hipEventSynchronize(b->staged[k]));
+    BatchView *vh = b->views_host[k];
+    uint32_t *bh = b->bases_host[k];
+    uint32_t total = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const FrameParams &p = ps[i];
+        BatchView &v = vh[i];
+        v.words = scenes[i]->words;
+        v.jump = p.jump;
+        v.n_words = p.n_words;
+        v.depth = p.depth;
+        v.jump_level = p.jump_level;
+        v.cc = p.primary_cc;
+        v.W = p.W;
+        v.H = p.H;
+        v.tiles_x = (p.W + b->cfg.tile_w - 1) / b->cfg.tile_w;
+        v.reuse = (uint32_t)p.reuse_primary_hits;
+        std::memcpy(v.cam, p.cam, sizeof v.cam);
+        v.gbuf = ctxs[i]->b.gbuf;
+        v.gpos = ctxs[i]->b.gpos;
+        v.ghit = ctxs[i]->b.ghit;
+        bh[i] = total;
+        total += v.tiles_x * ((p.H + th - 1) / th);
+    }
+    bh[n] = total;
+    for (uint32_t i = 0; i < n; i++) {
+        B_HIP(b, hipEventRecord(b->ctx_ev[i], ctxs[i]->stream));
+        B_HIP(b, hipStreamWaitEvent(b->stream, b->ctx_ev[i], 0));
+    }
+    B_HIP(b, hipMemcpyAsync(b->views_dev, vh, n * sizeof(BatchView), hipMemcpyHostToDevice, b->stream));
+    B_HIP(b, hipMemcpyAsync(b->bases_dev, bh, (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+    B_HIP(b, hipEventRecord(b->staged[k], b->stream));
+    B_HIP(b, hipMemsetAsync(b->queue_dev, 0, sizeof(uint32_t), b->stream));
+    if (starts)
+        for (uint32_t i = 0; i < n; i++) B_HIP(b, hipEventRecord(starts[i], b->stream));
+    B_HIP(b, hipEventRecord(b->t0, b->stream));
+    launch_primary_batch(b->stream, b->views_dev, b->bases_dev, n, total, b->queue_dev, b->cfg.tile_w, b->cfg.grab,
+                         b->blocks);
+    B_HIP(b, hipGetLastError());
+    B_HIP(b, hipEventRecord(b->t1, b->stream));
+    B_HIP(b, hipEventRecord(b->done, b->stream));
+    for (uint32_t i = 0; i < n; i++) B_HIP(b, hipStreamWaitEvent(ctxs[i]->stream, b->done, 0));
+    return VE_OK;
+}
+
+const char *batch_error_text(const ve_batch *b) { return b->err.c_str(); }
+
+}  // namespace ve
+
+extern "C" {
+
+int ve_batch_create(ve_batch **out, const ve_batch_config *cfg)
+{
+    if (!out || !cfg) return ve_global_error(VE_EINVAL, "null argument");
+    ve_batch_config c = *cfg;
+    if (c.max_views == 0) c.max_views = 256;
+    if (c.tile_w == 0) c.tile_w = 8;
+    if (c.grab == 0) c.grab = 4;
+    if (c.waves_per_cu == 0) c.waves_per_cu = 16;
+    if (!tile_width_ok(c.tile_w)) return ve_global_error(VE_EINVAL, "tile_w must be 8, 16, 32 or 64");
+    if (c.max_views > 65536 || c.grab > 1024 || c.waves_per_cu > 32)
+        return ve_global_error(VE_EINVAL, "max_views above 65536, grab above 1024 or waves_per_cu above 32");
+    ve_batch *b = new (std::nothrow) ve_batch();
+    if (!b) return ve_global_error(VE_ENOMEM, "batch allocation");
+    b->device = c.device;
+    b->max_views = c.max_views;
+    b->cfg = c;
+    hipDeviceProp_t prop{};
+    hipError_t e = hipSetDevice(c.device);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, c.device);
+    if (e == hipSuccess) {
+        const uint32_t cus = prop.multiProcessorCount > 0 ? (uint32_t)prop.multiProcessorCount : 256u;
+        b->blocks = (cus * c.waves_per_cu + 3u) / 4u;   // 4 waves per block
+    }
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);   // default priority
+    if (e == hipSuccess) e = hipMalloc(&b->views_dev, (size_t)c.max_views * sizeof(ve::BatchView));
+    if (e == hipSuccess) e = hipMalloc(&b->bases_dev, ((size_t)c.max_views + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&b->queue_dev, sizeof(uint32_t));
+    for (int k = 0; k < 2 && e == hipSuccess; k++) {
+        void *h = nullptr;
+        e = hipHostMalloc(&h, (size_t)c.max_views * sizeof(ve::BatchView), hipHostMallocDefault);
+        b->views_host[k] = static_cast<ve::BatchView *>(h);
+        if (e == hipSuccess) e = hipHostMalloc(&h, ((size_t)c.max_views + 1) * sizeof(uint32_t), hipHostMallocDefault);
+        if (e == hipSuccess) b->bases_host[k] = static_cast<uint32_t *>(h);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged[k], hipEventDisableTiming);
+    }
+    b->ctx_ev.assign(c.max_views, nullptr);
+    for (uint32_t i = 0; i < c.max_views && e == hipSuccess; i++)
+        e = hipEventCreateWithFlags(&b->ctx_ev[i], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreate(&b->t0);
+    if (e == hipSuccess) e = hipEventCreate(&b->t1);
+    if (e != hipSuccess) {
+        ve_global_error(VE_EDEVICE, hipGetErrorString(e));
+        ve_batch_destroy(b);
+        return VE_EDEVICE;
+    }
+    *out = b;
+    return VE_OK;
+}
+
+void ve_batch_destroy(ve_batch *b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (hipEvent_t e : b->ctx_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {b->staged[0], b->staged[1], b->done, b->t0, b->t1})
+        if (e) (void)hipEventDestroy(e);
+    for (int k = 0; k < 2; k++) {
+        if (b->views_host[k]) (void)hipHostFree(b->views_host[k]);
+        if (b->bases_host[k]) (void)hipHostFree(b->bases_host[k]);
+    }
+    if (b->views_dev) (void)hipFree(b->views_dev);
+    if (b->bases_dev) (void)hipFree(b->bases_dev);
+    if (b->queue_dev) (void)hipFree(b->queue_dev);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char *ve_batch_last_error(const ve_batch *b) { return b ? b->err.c_str() : ve_last_error(nullptr); }
+
+int ve_batch_frame(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const *scenes, const ve_frame_config *fcs,
+                   const ve_frame_inputs *ins)
+{
+    if (!b) return ve_global_error(VE_EINVAL, "null batch");
+    if (n == 0) return VE_OK;
+    if (!ctxs || !scenes || !fcs || !ins) return batch_error(b, VE_EINVAL, "null argument");
+    if (n > b->max_views) return batch_error(b, VE_EINVAL, "%u views above the batch's %u", n, b->max_views);
+    b->params.resize(n);
+    for (uint32_t i = 0; i < n; i++) {
+        if (!ctxs[i]) return batch_error(b, VE_EINVAL, "null context %u", i);
+        if (ctxs[i]->device != b->device) return batch_error(b, VE_EINVAL, "view %u is on another device", i);
+        for (uint32_t j = 0; j < i; j++)
+            if (ctxs[j] == ctxs[i]) return batch_error(b, VE_EINVAL, "context of view %u repeated", i);
+        const int rc = ve::frame_begin(ctxs[i], scenes[i], &fcs[i], &ins[i], b->params[i]);
+        if (rc) return batch_error(b, rc, "view %u: %s", i, ctxs[i]->err.c_str());
+    }
+    B_HIP(b, hipSetDevice(b->device));
+    int rc = ve::batch_primary(b, n, ctxs, scenes, b->params.data(), nullptr);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < n; i++)
+        if ((rc = ve::frame_rest(ctxs[i], scenes[i], b->params[i])))
+            return batch_error(b, rc, "view %u: %s", i, ctxs[i]->err.c_str());
+    return VE_OK;
+}
+
+int ve_batch_primary_ms(ve_batch *b, float *ms)
+{
+    if (!b || !ms) return ve_global_error(VE_EINVAL, "null argument");
+    B_HIP(b, hipSetDevice(b->device));
+    B_HIP(b, hipEventSynchronize(b->t1));
+    B_HIP(b, hipEventElapsedTime(ms, b->t0, b->t1));
+    return VE_OK;
+}
+
+}  // extern "C"

@@ -110,6 +110,17 @@ int ctx_create(ve_ctx **out, const ve_renderer_config *cfg, hipStream_t shared, 
 // Back to the state of a fresh context (all buffers zero, initial RankStats),
 // enqueued on the context's stream without waiting.
 int ctx_reset(ve_ctx *c);
+// A frame whose primary pass runs in a batch (ve_batch_frame): the checks and
+// parameters of ve_frame (the context must have the overlap, timing and node
+// counting off), then every other pass on the context's stream, replayed
+// from its graph when graph replay is on.
+int frame_begin(ve_ctx *c, ve_scene *s, const ve_frame_config *fc, const ve_frame_inputs *in, FrameParams &p);
+int frame_rest(ve_ctx *c, ve_scene *s, const FrameParams &p);
+// The batched primary launch of ve_batch_frame for views prepared by
+// frame_begin (ve_batch.cpp); `starts` (optional) are recorded just before it.
+int batch_primary(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const *scenes, const FrameParams *ps,
+                  hipEvent_t *starts);
+const char *batch_error_text(const ve_batch *b);
 int scene_upload(int device, hipStream_t stream, const uint32_t *words, uint32_t n_words, uint32_t depth,
                  ve_scene **scene, std::string &err, uint32_t jump_level = JUMP_LEVEL);
 void scene_free(ve_scene *s);

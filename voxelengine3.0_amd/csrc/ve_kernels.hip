@@ -97,6 +97,67 @@ void launch_primary(hipStream_t s, const uint32_t *words, const FrameParams &p, 
                            b.counters, b.ray_totals, b.ghit);
 }
 
+// Primary rays of many views in one launch (the batched form): a persistent
+// grid whose waves pull runs of tiles from a global counter over the
+// concatenated tile lists of all views, so views of every size share the
+// chip and a small view's tail overlaps the next view's work.  A tile is
+// tile_w x (64 / tile_w) pixels, one per lane; each pixel's ray is the one
+// k_primary casts for it (same direction, same traversal), so every view's
+// G-buffer equals its own primary pass bit for bit.
+template <uint32_t TW>
+__global__ __launch_bounds__(256) void k_primary_batch(const BatchView *__restrict__ views,
+                                                       const uint32_t *__restrict__ bases, uint32_t n_views,
+                                                       uint32_t total, uint32_t *__restrict__ queue, uint32_t grab)
+{
+    constexpr uint32_t TH = 64u / TW;
+    __shared__ uint32_t path_cache[VE_MAX_DEPTH * 256];
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t v = 0;
+    for (;;) {
+        uint32_t t0 = 0;
+        if (lane == 0u) t0 = atomicAdd(queue, grab);
+        t0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl((int)t0, 0, 64));
+        if (t0 >= total) break;   // every wave reaches the end of the list
+        const uint32_t t1 = min(t0 + grab, total);
+        if (t0 < bases[v]) v = 0;   // runs are handed out in increasing order per wave
+        for (uint32_t t = t0; t < t1; t++) {
+            while (v + 1u < n_views && t >= bases[v + 1u]) v++;
+            const BatchView &bv = views[v];
+            const uint32_t local = t - bases[v];
+            const uint32_t x = (local % bv.tiles_x) * TW + (lane % TW);
+            const uint32_t y = (local / bv.tiles_x) * TH + lane / TW;
+            if (x >= bv.W || y >= bv.H) continue;
+            uint32_t checks = 0, reads = 0;
+            const f3 dir = primary_dir(bv.cam, x, y, bv.W, bv.H);
+            const f3 org = mk(bv.cam[0], bv.cam[1], bv.cam[2]);
+            const Hit h = raycast<false>(bv.words, bv.jump, bv.jump_level, bv.n_words, bv.depth, org, dir, rcp3(dir),
+                                         bv.cc, checks, reads, path_cache + threadIdx.x, 256u);
+            const size_t pix = (size_t)y * bv.W + x;
+            if (h.hit) {
+                const uint32_t bits = bv.depth - 1u;
+                bv.gbuf[pix] = make_uint4(checks, h.material, 0u, voxel_key(h.px, h.py, h.pz) + 1u);
+                bv.gpos[pix] = (h.px >> 1) | ((h.py >> 1) << bits) | ((h.pz >> 1) << (2u * bits));
+                if (bv.reuse) bv.ghit[pix] = make_uint2(h.px | (h.py << 16), h.pz);
+            } else {
+                bv.gbuf[pix] = make_uint4(checks, 0u, 0u, 0u);
+                bv.gpos[pix] = 0u;
+            }
+        }
+    }
+}
+
+void launch_primary_batch(hipStream_t s, const BatchView *views, const uint32_t *bases, uint32_t n, uint32_t total,
+                          uint32_t *queue, uint32_t tile_w, uint32_t grab, uint32_t blocks)
+{
+    const dim3 g(blocks ? blocks : 1u), b(256);
+    switch (tile_w) {
+    case 16: launch(k_primary_batch<16>, g, b, 0, s, views, bases, n, total, queue, grab); break;
+    case 32: launch(k_primary_batch<32>, g, b, 0, s, views, bases, n, total, queue, grab); break;
+    case 64: launch(k_primary_batch<64>, g, b, 0, s, views, bases, n, total, queue, grab); break;
+    default: launch(k_primary_batch<8>, g, b, 0, s, views, bases, n, total, queue, grab); break;
+    }
+}
+
 // Jump table: one thread per cell of the grid at depth `level` (cells of
 // 2^(depth - level) voxels, index x | y | z bits of the cell coordinates):
 // the descent of raycast from the root through depth `level` depends only on

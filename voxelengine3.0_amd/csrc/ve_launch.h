@@ -96,6 +96,26 @@ struct DeviceBuffers {
 };
 
 void launch_primary(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, bool stats);
+
+// One view of a batched primary pass: what generate_primary.comp reads for
+// it, and where its G-buffer goes.
+struct BatchView {
+    const uint32_t *words;
+    const uint32_t *jump;
+    uint32_t n_words, depth, jump_level, cc;
+    uint32_t W, H, tiles_x, reuse;
+    float cam[16];
+    uint4 *gbuf;
+    uint32_t *gpos;
+    uint2 *ghit;
+};
+constexpr uint32_t BATCH_TILE_WIDTHS[4] = {8u, 16u, 32u, 64u};
+// The primary passes of n views in one persistent launch of `blocks` blocks
+// of 4 waves: bases[v] = first tile of view v in the concatenated tile list
+// (bases[n] = total), tiles of tile_w x (64 / tile_w) pixels, each wave
+// taking runs of `grab` tiles from the counter *queue (zero at launch).
+void launch_primary_batch(hipStream_t s, const BatchView *views, const uint32_t *bases, uint32_t n, uint32_t total,
+                          uint32_t *queue, uint32_t tile_w, uint32_t grab, uint32_t blocks);
 void launch_fill(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
 void launch_normal(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b);
 void launch_reset(hipStream_t s, const FrameParams &p, DeviceBuffers &b);

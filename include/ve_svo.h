@@ -313,7 +313,10 @@ int ve_graph_stats(ve_ctx *ctx, uint64_t *captures, uint64_t *replays);
  * context's (ve_set_camera).  Launch shape: tiles of tile_w x (64 / tile_w)
  * pixels (tile_w 8, 16, 32 or 64; 0 = 8), `grab` tiles per queue pull (0 = 4),
  * a grid of waves_per_cu waves per compute unit (0 = 16); at most max_views
- * views per call (0 = 256).  voxelengine3.0_amd/autotune.py picks them. */
+ * views per call (0 = 256).  The views are split into `groups` launches (0 =
+ * 4) of about equal work, largest views first: a group's views start their
+ * other passes as soon as its launch ends, beside the next group's launch.
+ * voxelengine3.0_amd/autotune.py picks them. */
 typedef struct ve_batch ve_batch;
 typedef struct ve_batch_config {
     int32_t  device;
@@ -321,6 +324,7 @@ typedef struct ve_batch_config {
     uint32_t tile_w;
     uint32_t grab;
     uint32_t waves_per_cu;
+    uint32_t groups;
 } ve_batch_config;
 int  ve_batch_create(ve_batch **out, const ve_batch_config *cfg);
 void ve_batch_destroy(ve_batch *b);
@@ -500,6 +504,22 @@ int  ve_server_read_output(ve_server *srv, uint32_t session, float *rgba, size_t
  * (same ids and layouts; waits for the session's frames). */
 int  ve_server_read(ve_server *srv, uint32_t session, ve_buffer_id id, void *dst, size_t bytes);
 int  ve_server_get_stats(ve_server *srv, ve_server_stats *out);
+/* No counterpart in the reference: the batched and graph-replayed forms of the
+ * server.  ve_server_set_graph: sessions replay their frame graphs
+ * (ve_set_graph).  ve_server_set_batch: the launch shape of the batched
+ * primary pass (ve_batch_config; device and max_views are the server's).
+ * ve_server_submit_batch: one frame for each of n distinct sessions (camera
+ * UBOs in ubos[16 * i]; tags and arrival times optional, an arrival below 0
+ * meaning now) -- the primary passes of the admitted sessions whose earlier
+ * frames are done run in one launch (a session still busy goes alone on its
+ * stream), the rest of each frame on its session's stream;
+ * frames of sessions still waiting for admission queue as with
+ * ve_server_submit.  Each frame's result equals ve_server_submit's. */
+int  ve_server_set_graph(ve_server *srv, int enable);
+int  ve_server_set_batch(ve_server *srv, const ve_batch_config *cfg);
+int  ve_server_submit_batch(ve_server *srv, uint32_t n, const uint32_t *sessions, const ve_frame_config *fcs,
+                            const ve_frame_inputs *ins, const float *ubos, const uint64_t *tags,
+                            const double *arrival_ms);
 
 #ifdef __cplusplus
 }

@@ -2,7 +2,7 @@
 (ve_frame): uneven views on different scenes, sizes, bounce counts and knobs,
 their primary passes in one persistent launch pulling tiles from a global
 queue; every structure of every view bit-exact frame by frame, for several
-launch shapes, with and without graph replay."""
+launch shapes and groupings, with and without graph replay."""
 import numpy as np
 import pytest
 
@@ -44,10 +44,10 @@ def _make(ve, vs, graph):
     return out
 
 
-@pytest.mark.parametrize("tile_w,grab,waves,graph", [(8, 4, 16, False), (32, 1, 4, True), (64, 16, 8, False),
-                                                     (16, 2, 32, True)])
-def test_batch_matches_views_alone(ve, scenes, views, tile_w, grab, waves, graph):
-    batch = ve.Batch(tile_w=tile_w, grab=grab, waves_per_cu=waves)
+@pytest.mark.parametrize("tile_w,grab,waves,groups,graph", [(8, 4, 16, 1, False), (32, 1, 4, 4, True),
+                                                            (64, 16, 8, 64, False), (16, 2, 32, 2, True)])
+def test_batch_matches_views_alone(ve, scenes, views, tile_w, grab, waves, groups, graph):
+    batch = ve.Batch(tile_w=tile_w, grab=grab, waves_per_cu=waves, groups=groups)
     a, b = _make(ve, views, graph), _make(ve, views, False)
     try:
         for f in range(3):
@@ -62,7 +62,7 @@ def test_batch_matches_views_alone(ve, scenes, views, tile_w, grab, waves, graph
                 fcs.append(fc)
                 ins.append(fi)
             batch.frame([r for r, _ in a], [s for _, s in a], fcs, ins)
-            print("case batch", tile_w, grab, waves, graph, "frame", f, flush=True)
+            print("case batch", tile_w, grab, waves, groups, graph, "frame", f, flush=True)
             for i in range(len(views)):
                 for name in BUFS:
                     assert np.array_equal(a[i][0].read(name), b[i][0].read(name)), "view %d frame %d %s" % (i, f, name)

@@ -76,9 +76,12 @@ def _alone(ve, scenes, scene, sess, seed):
         r.close()
 
 
-def test_sessions_alone_equal_in_mix(ve, scenes, resident):
+@pytest.mark.parametrize("batched", [False, True])
+def test_sessions_alone_equal_in_mix(ve, scenes, resident, batched):
     """Five sessions overlap on their own streams, frame k of each submitted
-    together; every frame equals the session rendered alone."""
+    together -- one by one, or as one batch (their primary passes in one
+    persistent launch, the rest replayed from each session's frame graph);
+    every frame equals the session rendered alone."""
     seeds = [101 + 17 * i for i in range(len(SESSIONS))]
     alone = []
     for i, s in enumerate(SESSIONS):
@@ -87,13 +90,20 @@ def test_sessions_alone_equal_in_mix(ve, scenes, resident):
     srv = ve.Server(queues=0)
     try:
         ids = {key: srv.add_scene(w, d, info.size) for key, (d, w, info) in resident.items()}
+        if batched:
+            srv.set_graph(True)
+            srv.set_batch(tile_w=16, grab=4, waves_per_cu=16)
         sids = [srv.open(ids[s[0]], s[1], s[2], s[4] + 1, scenes.palette(1), SKY) for s in SESSIONS]
         for k in range(FRAMES):
-            print("case served mix frame", k, flush=True)
-            for i, s in enumerate(SESSIONS):
-                d = resident[s[0]][0]
-                srv.submit(sids[i], ve.FrameConfig(**_knobs(s[3])), _inputs(ve, s[1], s[2], seeds[i], k),
-                           _ubo(ve, scenes, d, s[1], s[2], seeds[i], k), tag=i)
+            print("case served mix frame", k, "batched" if batched else "", flush=True)
+            args = [(sids[i], ve.FrameConfig(**_knobs(s[3])), _inputs(ve, s[1], s[2], seeds[i], k),
+                     _ubo(ve, scenes, resident[s[0]][0], s[1], s[2], seeds[i], k)) for i, s in enumerate(SESSIONS)]
+            if batched:
+                srv.submit_batch([a[0] for a in args], [a[1] for a in args], [a[2] for a in args],
+                                 [a[3] for a in args], tags=list(range(len(args))))
+            else:
+                for i, a in enumerate(args):
+                    srv.submit(*a, tag=i)
             got = 0
             while got < len(SESSIONS):
                 got += len(srv.poll(wait=True))

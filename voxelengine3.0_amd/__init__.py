@@ -116,7 +116,7 @@ class ServerStats(C.Structure):
 
 class BatchConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("max_views", C.c_uint32), ("tile_w", C.c_uint32), ("grab", C.c_uint32),
-                ("waves_per_cu", C.c_uint32)]
+                ("waves_per_cu", C.c_uint32), ("groups", C.c_uint32)]
 
 
 _lib = None
@@ -166,6 +166,9 @@ def lib():
         "ve_batch_frame": (i32, [p, u32, p, p, p, p]),
         "ve_batch_primary_ms": (i32, [p, C.POINTER(C.c_float)]),
         "ve_graph_stats": (i32, [p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "ve_server_set_graph": (i32, [p, i32]),
+        "ve_server_set_batch": (i32, [p, C.POINTER(BatchConfig)]),
+        "ve_server_submit_batch": (i32, [p, u32, p, p, p, p, p, p]),
         "ve_get_tuning": (i32, [p, p]),
         "ve_ray_totals": (i32, [p] + [C.POINTER(C.c_uint64)] * 3),
         "ve_ray_totals_reused": (i32, [p, C.POINTER(C.c_uint64)]),
@@ -398,12 +401,12 @@ class Batch:
     """ve_batch_*: frame k of many Renderers at once (one persistent primary
     launch over all views, the other passes on each view's stream)."""
 
-    def __init__(self, device=0, max_views=256, tile_w=8, grab=4, waves_per_cu=16):
-        cfg = BatchConfig(device, max_views, tile_w, grab, waves_per_cu)
+    def __init__(self, device=0, max_views=256, tile_w=8, grab=4, waves_per_cu=16, groups=4):
+        cfg = BatchConfig(device, max_views, tile_w, grab, waves_per_cu, groups)
         h = C.c_void_p()
         _check(lib().ve_batch_create(C.byref(h), C.byref(cfg)))
         self.h = h
-        self.config = dict(tile_w=tile_w, grab=grab, waves_per_cu=waves_per_cu)
+        self.config = dict(tile_w=tile_w, grab=grab, waves_per_cu=waves_per_cu, groups=groups)
 
     def close(self):
         if self.h:
@@ -649,6 +652,27 @@ class Server:
         ubo = np.ascontiguousarray(ubo, np.float32)
         self._ck(lib().ve_server_submit(self.h, session, C.byref(fc), C.byref(inputs), ubo.ctypes.data, tag,
                                         float(arrival_ms)))
+
+    def submit_batch(self, sessions, fcs, inputs, ubos, tags=None, arrival_ms=None):
+        """ve_server_submit_batch: one frame for each of the distinct sessions
+        (arrival_ms: one time per frame, None = now)."""
+        n = len(sessions)
+        if n == 0:
+            return
+        sa = (C.c_uint32 * n)(*sessions)
+        fca = (FrameConfig * n)(*fcs)
+        ina = (FrameInputs * n)(*inputs)
+        ub = np.ascontiguousarray(np.asarray(ubos, np.float32).reshape(n, 16))
+        ta = (C.c_uint64 * n)(*(tags if tags is not None else [0] * n))
+        aa = (C.c_double * n)(*(arrival_ms if arrival_ms is not None else [-1.0] * n))
+        self._ck(lib().ve_server_submit_batch(self.h, n, sa, fca, ina, ub.ctypes.data, ta, aa))
+
+    def set_graph(self, on=True):
+        self._ck(lib().ve_server_set_graph(self.h, 1 if on else 0))
+
+    def set_batch(self, tile_w=8, grab=4, waves_per_cu=16, groups=4):
+        cfg = BatchConfig(0, 0, tile_w, grab, waves_per_cu, groups)
+        self._ck(lib().ve_server_set_batch(self.h, C.byref(cfg)))
 
     def close_session(self, session):
         self._ck(lib().ve_server_close(self.h, session))

@@ -6,6 +6,7 @@
 // which renders one view per Renderer::run.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -23,15 +24,19 @@ struct ve_batch {
     uint32_t blocks = 0;                     // persistent grid
     hipStream_t stream = nullptr;
     ve::BatchView *views_dev = nullptr;
-    uint32_t *bases_dev = nullptr;           // max_views + 1
-    uint32_t *queue_dev = nullptr;
-    // pinned staging, two sets: a set is rewritten once its previous copy ran
-    ve::BatchView *views_host[2] = {nullptr, nullptr};
-    uint32_t *bases_host[2] = {nullptr, nullptr};
-    hipEvent_t staged[2] = {nullptr, nullptr};
-    uint32_t flip = 0;
+    uint32_t *bases_dev = nullptr;           // max_views + MAX_GROUPS
+    uint32_t *queue_dev = nullptr;           // MAX_GROUPS counters
+    // pinned staging sets: a set is rewritten only once its copy has run (a
+    // query, never a host wait: a serving loop must not block); a new set is
+    // added when none is free
+    struct Staging {
+        ve::BatchView *views = nullptr;
+        uint32_t *bases = nullptr;
+        hipEvent_t copied = nullptr;
+    };
+    std::vector<Staging> staging;
     std::vector<hipEvent_t> ctx_ev;          // per view: its stream's work before the batch
-    hipEvent_t done = nullptr;               // the batch's primary launch finished
+    hipEvent_t done[64] = {};                // per group: its primary launch finished
     hipEvent_t t0 = nullptr, t1 = nullptr;   // around the primary launch
     std::vector<ve::FrameParams> params;
     std::string err;
@@ -56,6 +61,8 @@ int batch_error(ve_batch *b, int code, const char *fmt, ...)
         if (e_ != hipSuccess) return batch_error(b, VE_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
+constexpr uint32_t MAX_GROUPS = 64;
+
 bool tile_width_ok(uint32_t w)
 {
     for (uint32_t v : ve::BATCH_TILE_WIDTHS)
@@ -67,58 +74,108 @@ bool tile_width_ok(uint32_t w)
 
 namespace ve {
 
-// The batch's primary launch for views whose parameters frame_begin filled:
-// it waits for everything already on the views' streams, and the views'
-// streams wait for it.  `starts` (optional, one per view) are recorded on the
-// batch stream just before the launch.
+// The batch's primary launches for views whose parameters frame_begin filled.
+// They wait for everything already on the views' streams.  The views, largest
+// first, are split into groups of about equal work, one persistent launch
+// each on the batch stream; a view's stream waits for its group's launch only,
+// so the views of early groups run their other passes beside the later
+// launches.  `starts` (optional, one per view) are recorded on the batch
+// stream before the first launch.
 int batch_primary(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const *scenes, const FrameParams *ps,
                   hipEvent_t *starts)
 {
-    const uint32_t th = 64u / b->cfg.tile_w;
-    const uint32_t k = b->flip;
-    b->flip ^= 1u;
-    B_HIP(b, hipEventSynchronize(b->staged[k]));
-    BatchView *vh = b->views_host[k];
-    uint32_t *bh = b->bases_host[k];
-    uint32_t total = 0;
+    if (n > b->max_views) return batch_error(b, VE_EINVAL, "%u views above the batch's %u", n, b->max_views);
+    const uint32_t tw = b->cfg.tile_w, th = 64u / tw;
+    std::vector<uint32_t> tiles(n), order(n);
+    uint64_t all = 0;
     for (uint32_t i = 0; i < n; i++) {
-        const FrameParams &p = ps[i];
-        BatchView &v = vh[i];
-        v.words = scenes[i]->words;
-        v.jump = p.jump;
-        v.n_words = p.n_words;
-        v.depth = p.depth;
-        v.jump_level = p.jump_level;
-        v.cc = p.primary_cc;
-        v.W = p.W;
-        v.H = p.H;
-        v.tiles_x = (p.W + b->cfg.tile_w - 1) / b->cfg.tile_w;
-        v.reuse = (uint32_t)p.reuse_primary_hits;
-        std::memcpy(v.cam, p.cam, sizeof v.cam);
-        v.gbuf = ctxs[i]->b.gbuf;
-        v.gpos = ctxs[i]->b.gpos;
-        v.ghit = ctxs[i]->b.ghit;
-        bh[i] = total;
-        total += v.tiles_x * ((p.H + th - 1) / th);
+        tiles[i] = ((ps[i].W + tw - 1) / tw) * ((ps[i].H + th - 1) / th);
+        all += tiles[i];
+        order[i] = i;
     }
-    bh[n] = total;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return tiles[a] > tiles[c]; });
+    const uint32_t groups = std::max(1u, std::min(std::min(b->cfg.groups, n), MAX_GROUPS));
+    // group g: sorted views [first[g], first[g + 1]), cut where the running work passes (g + 1) / groups
+    std::vector<uint32_t> first(groups + 1, n);
+    first[0] = 0;
+    uint64_t run = 0;
+    uint32_t g = 1;
+    for (uint32_t k = 0; k < n && g < groups; k++) {
+        run += tiles[order[k]];
+        if (run * groups >= all * g && k + 1 < n) first[g++] = k + 1;
+    }
+    for (; g < groups; g++) first[g] = n;   // fewer cuts than groups: the rest are empty
+    ve_batch::Staging *st = nullptr;
+    for (auto &x : b->staging)
+        if (hipEventQuery(x.copied) == hipSuccess) {
+            st = &x;
+            break;
+        }
+    if (!st) {
+        ve_batch::Staging x;
+        void *h = nullptr;
+        B_HIP(b, hipHostMalloc(&h, (size_t)b->max_views * sizeof(BatchView), hipHostMallocDefault));
+        x.views = static_cast<BatchView *>(h);
+        B_HIP(b, hipHostMalloc(&h, ((size_t)b->max_views + MAX_GROUPS) * sizeof(uint32_t), hipHostMallocDefault));
+        x.bases = static_cast<uint32_t *>(h);
+        B_HIP(b, hipEventCreateWithFlags(&x.copied, hipEventDisableTiming));
+        b->staging.push_back(x);
+        st = &b->staging.back();
+    }
+    BatchView *vh = st->views;
+    uint32_t *bh = st->bases;   // group g's bases at first[g] + g, first[g + 1] - first[g] + 1 of them
+    std::vector<uint32_t> totals(groups, 0);
+    for (uint32_t gi = 0; gi < groups; gi++) {
+        uint32_t total = 0;
+        for (uint32_t j = first[gi]; j < first[gi + 1]; j++) {
+            const uint32_t i = order[j];
+            const FrameParams &p = ps[i];
+            BatchView &v = vh[j];
+            v.words = scenes[i]->words;
+            v.jump = p.jump;
+            v.n_words = p.n_words;
+            v.depth = p.depth;
+            v.jump_level = p.jump_level;
+            v.cc = p.primary_cc;
+            v.W = p.W;
+            v.H = p.H;
+            v.tiles_x = (p.W + tw - 1) / tw;
+            v.reuse = (uint32_t)p.reuse_primary_hits;
+            std::memcpy(v.cam, p.cam, sizeof v.cam);
+            v.gbuf = ctxs[i]->b.gbuf;
+            v.gpos = ctxs[i]->b.gpos;
+            v.ghit = ctxs[i]->b.ghit;
+            bh[j + gi] = total;
+            total += tiles[i];
+        }
+        bh[first[gi + 1] + gi] = total;
+        totals[gi] = total;
+    }
     for (uint32_t i = 0; i < n; i++) {
         B_HIP(b, hipEventRecord(b->ctx_ev[i], ctxs[i]->stream));
         B_HIP(b, hipStreamWaitEvent(b->stream, b->ctx_ev[i], 0));
     }
     B_HIP(b, hipMemcpyAsync(b->views_dev, vh, n * sizeof(BatchView), hipMemcpyHostToDevice, b->stream));
-    B_HIP(b, hipMemcpyAsync(b->bases_dev, bh, (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
-    B_HIP(b, hipEventRecord(b->staged[k], b->stream));
-    B_HIP(b, hipMemsetAsync(b->queue_dev, 0, sizeof(uint32_t), b->stream));
+    B_HIP(b, hipMemcpyAsync(b->bases_dev, bh, (n + groups) * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+    B_HIP(b, hipEventRecord(st->copied, b->stream));
+    B_HIP(b, hipMemsetAsync(b->queue_dev, 0, groups * sizeof(uint32_t), b->stream));
     if (starts)
         for (uint32_t i = 0; i < n; i++) B_HIP(b, hipEventRecord(starts[i], b->stream));
     B_HIP(b, hipEventRecord(b->t0, b->stream));
-    launch_primary_batch(b->stream, b->views_dev, b->bases_dev, n, total, b->queue_dev, b->cfg.tile_w, b->cfg.grab,
-                         b->blocks);
-    B_HIP(b, hipGetLastError());
+    for (uint32_t gi = 0; gi < groups; gi++) {
+        const uint32_t m = first[gi + 1] - first[gi];
+        if (m == 0) continue;
+        // no more blocks than the group's runs of tiles need (one run per wave)
+        const uint64_t runs = ((uint64_t)totals[gi] + b->cfg.grab - 1) / b->cfg.grab;
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>(b->blocks, (runs + 3) / 4);
+        launch_primary_batch(b->stream, b->views_dev + first[gi], b->bases_dev + first[gi] + gi, m, totals[gi],
+                             b->queue_dev + gi, tw, b->cfg.grab, blocks);
+        B_HIP(b, hipGetLastError());
+        B_HIP(b, hipEventRecord(b->done[gi], b->stream));
+        for (uint32_t j = first[gi]; j < first[gi + 1]; j++)
+            B_HIP(b, hipStreamWaitEvent(ctxs[order[j]]->stream, b->done[gi], 0));
+    }
     B_HIP(b, hipEventRecord(b->t1, b->stream));
-    B_HIP(b, hipEventRecord(b->done, b->stream));
-    for (uint32_t i = 0; i < n; i++) B_HIP(b, hipStreamWaitEvent(ctxs[i]->stream, b->done, 0));
     return VE_OK;
 }
 
@@ -136,9 +193,10 @@ int ve_batch_create(ve_batch **out, const ve_batch_config *cfg)
     if (c.tile_w == 0) c.tile_w = 8;
     if (c.grab == 0) c.grab = 4;
     if (c.waves_per_cu == 0) c.waves_per_cu = 16;
+    if (c.groups == 0) c.groups = 4;
     if (!tile_width_ok(c.tile_w)) return ve_global_error(VE_EINVAL, "tile_w must be 8, 16, 32 or 64");
-    if (c.max_views > 65536 || c.grab > 1024 || c.waves_per_cu > 32)
-        return ve_global_error(VE_EINVAL, "max_views above 65536, grab above 1024 or waves_per_cu above 32");
+    if (c.max_views > 65536 || c.grab > 1024 || c.waves_per_cu > 32 || c.groups > MAX_GROUPS)
+        return ve_global_error(VE_EINVAL, "max_views above 65536, grab above 1024, waves_per_cu above 32 or groups above 64");
     ve_batch *b = new (std::nothrow) ve_batch();
     if (!b) return ve_global_error(VE_ENOMEM, "batch allocation");
     b->device = c.device;
@@ -153,20 +211,13 @@ int ve_batch_create(ve_batch **out, const ve_batch_config *cfg)
     }
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);   // default priority
     if (e == hipSuccess) e = hipMalloc(&b->views_dev, (size_t)c.max_views * sizeof(ve::BatchView));
-    if (e == hipSuccess) e = hipMalloc(&b->bases_dev, ((size_t)c.max_views + 1) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&b->queue_dev, sizeof(uint32_t));
-    for (int k = 0; k < 2 && e == hipSuccess; k++) {
-        void *h = nullptr;
-        e = hipHostMalloc(&h, (size_t)c.max_views * sizeof(ve::BatchView), hipHostMallocDefault);
-        b->views_host[k] = static_cast<ve::BatchView *>(h);
-        if (e == hipSuccess) e = hipHostMalloc(&h, ((size_t)c.max_views + 1) * sizeof(uint32_t), hipHostMallocDefault);
-        if (e == hipSuccess) b->bases_host[k] = static_cast<uint32_t *>(h);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged[k], hipEventDisableTiming);
-    }
+    if (e == hipSuccess) e = hipMalloc(&b->bases_dev, ((size_t)c.max_views + MAX_GROUPS) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&b->queue_dev, MAX_GROUPS * sizeof(uint32_t));
     b->ctx_ev.assign(c.max_views, nullptr);
     for (uint32_t i = 0; i < c.max_views && e == hipSuccess; i++)
         e = hipEventCreateWithFlags(&b->ctx_ev[i], hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+    for (uint32_t g = 0; g < MAX_GROUPS && e == hipSuccess; g++)
+        e = hipEventCreateWithFlags(&b->done[g], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreate(&b->t0);
     if (e == hipSuccess) e = hipEventCreate(&b->t1);
     if (e != hipSuccess) {
@@ -185,11 +236,14 @@ void ve_batch_destroy(ve_batch *b)
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (hipEvent_t e : b->ctx_ev)
         if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {b->staged[0], b->staged[1], b->done, b->t0, b->t1})
+    for (hipEvent_t e : b->done)
         if (e) (void)hipEventDestroy(e);
-    for (int k = 0; k < 2; k++) {
-        if (b->views_host[k]) (void)hipHostFree(b->views_host[k]);
-        if (b->bases_host[k]) (void)hipHostFree(b->bases_host[k]);
+    for (hipEvent_t e : {b->t0, b->t1})
+        if (e) (void)hipEventDestroy(e);
+    for (auto &x : b->staging) {
+        if (x.copied) (void)hipEventDestroy(x.copied);
+        if (x.views) (void)hipHostFree(x.views);
+        if (x.bases) (void)hipHostFree(x.bases);
     }
     if (b->views_dev) (void)hipFree(b->views_dev);
     if (b->bases_dev) (void)hipFree(b->bases_dev);

@@ -84,6 +84,9 @@ struct ve_server {
     std::vector<RaySlot *> free_slots;
     std::vector<RaySlot *> slot_chunks;
     std::deque<ve_frame_done> done;        // finished, not yet returned
+    bool graph = false;                    // sessions replay their frame graphs (ve_set_graph)
+    ve_batch *batch = nullptr;             // ve_server_submit_batch's primary launches
+    ve_batch_config batch_cfg{};
 };
 
 namespace {
@@ -178,6 +181,7 @@ int acquire(ve_server *s, Session &ss)
     for (size_t i = 0; i < s->pool.size(); i++) {
         if (!fits(s->pool[i].ctx, ss.desc, hw)) continue;
         ss.ctx = s->pool[i].ctx;
+        ss.ctx->graph = s->graph;
         s->pool.erase(s->pool.begin() + (long)i);
         if (ve::ctx_reset(ss.ctx) != VE_OK) return srv_error(s, VE_EDEVICE, "context reset: %s", ss.ctx->err.c_str());
         return VE_OK;
@@ -197,6 +201,7 @@ int acquire(ve_server *s, Session &ss)
     const int rc = ve::ctx_create(&ss.ctx, &cfg, q, false, err);
     if (rc != VE_OK) return srv_error(s, rc, "session %u: %s", ss.id, err.c_str());
     s->used += ve::ctx_bytes(ss.ctx);
+    ss.ctx->graph = s->graph;
     return VE_OK;
 }
 
@@ -330,6 +335,7 @@ void ve_server_destroy(ve_server *s)
         }
     }
     for (auto &p : s->pool) ve_destroy(p.ctx);
+    if (s->batch) ve_batch_destroy(s->batch);
     for (ve_scene *sc : s->scenes) ve::scene_free(sc);
     for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
     for (RaySlot *c : s->slot_chunks) (void)hipHostFree(c);
@@ -412,6 +418,110 @@ int ve_server_submit(ve_server *s, uint32_t session, const ve_frame_config *fc, 
     const int rc = launch(s, ss, f);
     if (rc != VE_OK) return rc;
     ss.in_flight.push_back(f);
+    return VE_OK;
+}
+
+int ve_server_set_graph(ve_server *s, int enable)
+{
+    if (!s) return ve_global_error(VE_EINVAL, "null server");
+    s->graph = enable != 0;
+    for (auto &kv : s->sessions)
+        if (kv.second.ctx) kv.second.ctx->graph = s->graph;
+    for (auto &p : s->pool) p.ctx->graph = s->graph;
+    return VE_OK;
+}
+
+int ve_server_set_batch(ve_server *s, const ve_batch_config *cfg)
+{
+    if (!s || !cfg) return ve_global_error(VE_EINVAL, "null argument");
+    ve_batch_config c = *cfg;
+    c.device = s->device;
+    c.max_views = std::max(c.max_views ? c.max_views : 256u, s->max_sessions);   // a batch holds every admitted session
+    ve_batch *b = nullptr;
+    const int rc = ve_batch_create(&b, &c);
+    if (rc != VE_OK) return srv_error(s, rc, "batch: %s", ve_last_error(nullptr));
+    if (s->batch) ve_batch_destroy(s->batch);
+    s->batch = b;
+    s->batch_cfg = c;
+    return VE_OK;
+}
+
+int ve_server_submit_batch(ve_server *s, uint32_t n, const uint32_t *sessions, const ve_frame_config *fcs,
+                           const ve_frame_inputs *ins, const float *ubos, const uint64_t *tags, const double *arrival_ms)
+{
+    if (!s || (n && (!sessions || !fcs || !ins || !ubos))) return ve_global_error(VE_EINVAL, "null argument");
+    SRV_HIP(s, hipSetDevice(s->device));
+    if (!s->batch) {
+        ve_batch_config c{};
+        c.device = s->device;
+        if (int rc = ve_server_set_batch(s, &c)) return rc;
+    }
+    const double now = now_ms(s);
+    std::vector<Session *> act;
+    std::vector<Frame> fr;
+    for (uint32_t i = 0; i < n; i++) {
+        auto it = s->sessions.find(sessions[i]);
+        if (it == s->sessions.end() || it->second.close_requested)
+            return srv_error(s, VE_ENOTFOUND, "unknown or closing session %u", sessions[i]);
+        Session &ss = it->second;
+        if (ins[i].width != ss.desc.width || ins[i].height != ss.desc.height)
+            return srv_error(s, VE_EINVAL, "frame %ux%u on a %ux%u session", ins[i].width, ins[i].height,
+                             ss.desc.width, ss.desc.height);
+        for (Session *o : act)
+            if (o == &ss) return srv_error(s, VE_EINVAL, "session %u twice in one batch", ss.id);
+        Frame f;
+        f.tag = tags ? tags[i] : 0;
+        f.index = ss.frames++;
+        f.arrival_ms = arrival_ms && arrival_ms[i] >= 0 ? arrival_ms[i] : now;
+        f.fc = fcs[i];
+        f.in = ins[i];
+        std::memcpy(f.ubo, ubos + 16 * (size_t)i, sizeof f.ubo);
+        if (ss.state != State::Active) {
+            ss.pending.push_back(f);
+            continue;
+        }
+        // a session still busy with an earlier frame goes alone on its stream:
+        // the batch's launch would otherwise wait for that frame
+        if (!ss.in_flight.empty() && hipEventQuery(ss.in_flight.back().end) != hipSuccess) {
+            if (int rc = launch(s, ss, f)) return rc;
+            ss.in_flight.push_back(f);
+            continue;
+        }
+        act.push_back(&ss);
+        fr.push_back(f);
+    }
+    const uint32_t m = (uint32_t)act.size();
+    if (m == 0) return VE_OK;
+    std::vector<ve_ctx *> ctxs(m);
+    std::vector<ve_scene *> scs(m);
+    std::vector<ve::FrameParams> ps(m);
+    std::vector<hipEvent_t> starts(m);
+    int rc;
+    for (uint32_t i = 0; i < m; i++) {
+        Session &ss = *act[i];
+        Frame &f = fr[i];
+        if ((rc = get_event(s, &f.start)) || (rc = get_event(s, &f.end)) || (rc = get_slot(s, &f.slot))) return rc;
+        ctxs[i] = ss.ctx;
+        scs[i] = s->scenes[ss.desc.scene];
+        std::memcpy(ss.ctx->cam, f.ubo, sizeof ss.ctx->cam);
+        if ((rc = ve::frame_begin(ss.ctx, scs[i], &f.fc, &f.in, ps[i])))
+            return srv_error(s, rc, "session %u frame %u: %s", ss.id, f.index, ss.ctx->err.c_str());
+        starts[i] = f.start;
+    }
+    if ((rc = ve::batch_primary(s->batch, m, ctxs.data(), scs.data(), ps.data(), starts.data())))
+        return srv_error(s, rc, "batch: %s", ve::batch_error_text(s->batch));
+    for (uint32_t i = 0; i < m; i++) {
+        Session &ss = *act[i];
+        Frame &f = fr[i];
+        ve_ctx *c = ss.ctx;
+        if ((rc = ve::frame_rest(c, scs[i], ps[i])))
+            return srv_error(s, rc, "session %u frame %u: %s", ss.id, f.index, c->err.c_str());
+        SRV_HIP(s, hipMemcpyAsync(f.slot->v, ve::ctx_ray_totals(c), 3 * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, c->stream));
+        SRV_HIP(s, hipEventRecord(f.end, c->stream));
+        f.seq = s->seq++;
+        ss.in_flight.push_back(f);
+    }
     return VE_OK;
 }
 

@@ -11,7 +11,16 @@ resolution uniform in 256x256..2560x1440, 1-8 samples per visible voxel,
 steady 30 frames/s from its arrival (a frame interval of 33 ms).  Sessions
 arrive as a Poisson process whose rate makes the offered load `lam`
 frames/s.
+
+Modes: 0 = the concurrent server (one stream per session, each frame
+submitted as it falls due); 1 = the plain loop (every frame on one stream, in
+arrival order); "batched" = the concurrent server with continuous batching:
+the frames falling due together go in one ve_server_submit_batch (their
+primary passes in one persistent launch over a global tile queue) and every
+session replays its captured frame graph.
 """
+import json
+import os
 import time
 
 import numpy as np
@@ -86,6 +95,31 @@ def make_workload(lam, duration_s, seed=0, n_scenes=3):
     return sessions
 
 
+TUNING_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+DEFAULT_BATCH = dict(tile_w=8, grab=4, waves_per_cu=16, groups=4)
+
+
+def batch_tuning():
+    """The batched primary's launch shape chosen by autotune.py (tuning.json),
+    else the library defaults."""
+    try:
+        with open(TUNING_PATH) as fh:
+            best = json.load(fh)["batched_primary"]["best"]
+        return {k: int(best[k]) for k in DEFAULT_BATCH}
+    except (OSError, KeyError, ValueError):
+        return dict(DEFAULT_BATCH)
+
+
+def make_server(ve, mode, max_sessions=64):
+    """A session server for a mode (0, 1 or "batched")."""
+    if mode == "batched":
+        srv = ve.Server(queues=0, max_sessions=max_sessions)
+        srv.set_graph(True)
+        srv.set_batch(**batch_tuning())
+        return srv
+    return ve.Server(queues=int(mode), max_sessions=max_sessions)
+
+
 def _ubo(ve, s, depth, k):
     n = 1 << depth
     eye, d, fov = S.orbit_camera(n, k, 240, s["seed"])
@@ -128,20 +162,30 @@ def _replay(ve, srv, scene_depths, scene_ids, sessions, events, fcs, sid, record
     run_workload.t_base = t_base
     n_events, e = len(events), 0
     outstanding = 0
+    batched = getattr(srv, "batched", False)
     while e < n_events or outstanding:
         now = srv.now_ms()
+        due, closing = [], []
         while e < n_events and t_base + events[e][0] * 1e3 <= now:
             ts, i, k = events[e]
             s = sessions[i]
             if sid[i] is None:
                 sid[i] = srv.open(scene_ids[s["scene"]], s["W"], s["H"], s["bounces"] + 1, materials, sky)
             fi = ve.FrameInputs(s["W"], s["H"], k, (s["seed"] + 7919 * k) & 0xFFFFFFFF, k + 1)
-            srv.submit(sid[i], fcs[i], fi, _ubo(ve, s, scene_depths[s["scene"]], k), tag=i,
-                       arrival_ms=t_base + ts * 1e3)
+            req = (sid[i], fcs[i], fi, _ubo(ve, s, scene_depths[s["scene"]], k), i, t_base + ts * 1e3)
+            if batched:
+                if any(d[0] == sid[i] for d in due):   # one frame per session per batch
+                    _submit_due(srv, due, closing)
+                due.append(req)
+                if k == len(s["frames_s"]) - 1:
+                    closing.append(sid[i])
+            else:
+                srv.submit(req[0], req[1], req[2], req[3], tag=req[4], arrival_ms=req[5])
+                if k == len(s["frames_s"]) - 1:
+                    srv.close_session(sid[i])
             outstanding += 1
-            if k == len(s["frames_s"]) - 1:
-                srv.close_session(sid[i])
             e += 1
+        _submit_due(srv, due, closing)
         # never block on the device while arrivals are still to come: a wait
         # for the oldest frame in flight would hold back the frames falling
         # due meanwhile
@@ -152,6 +196,17 @@ def _replay(ve, srv, scene_depths, scene_ids, sessions, events, fcs, sid, record
         if not got and e < n_events:
             time.sleep(max(0.0, min(0.0002, (t_base + events[e][0] * 1e3 - srv.now_ms()) * 1e-3)))
     return records
+
+
+def _submit_due(srv, due, closing):
+    """Submit the collected frames as one batch, then close the sessions whose
+    last frame was among them."""
+    if due:
+        srv.submit_batch([d[0] for d in due], [d[1] for d in due], [d[2] for d in due], [d[3] for d in due],
+                         tags=[d[4] for d in due], arrival_ms=[d[5] for d in due])
+    for c in closing:
+        srv.close_session(c)
+    del due[:], closing[:]
 
 
 def summarize(records, after_ms=None):
@@ -190,7 +245,8 @@ def sweep(ve, lams, duration_s=3.0, seed=0, queues_list=(0, 1), log=None, res_sc
     out = {}
 
     def run(q, lam):
-        srv = ve.Server(queues=q, max_sessions=max_sessions)
+        srv = make_server(ve, q, max_sessions)
+        srv.batched = q == "batched"
         try:
             ids = [srv.add_scene(w, d, info.size) for d, w, info in res_scenes]
             sessions = make_workload(lam, warmup_s + duration_s, seed, len(res_scenes))
@@ -202,7 +258,7 @@ def sweep(ve, lams, duration_s=3.0, seed=0, queues_list=(0, 1), log=None, res_sc
         row = dict(lam=lam, offered_frames_per_s=round(offered, 1), sessions=len(sessions),
                    **summarize(rec, t_measure))
         if log:
-            log("served queues=%d %s" % (q, row))
+            log("served mode=%s %s" % (q, row))
         return row
 
     def ok(row):
@@ -247,10 +303,15 @@ def run_batched(ve, srv, scene_ids, scene_depths, views, frames, materials, sky=
     sids = [srv.open(scene_ids[v["scene"]], v["W"], v["H"], v["bounces"] + 1, materials, sky) for v in views]
     fcs = [ve.FrameConfig(**session_knobs(v["spp"])) for v in views]
     records = []
+    batched = getattr(srv, "batched", False)
     for k in range(frames):
-        for i, v in enumerate(views):
-            fi = ve.FrameInputs(v["W"], v["H"], k, (v["seed"] + 7919 * k) & 0xFFFFFFFF, k + 1)
-            srv.submit(sids[i], fcs[i], fi, _ubo(ve, v, scene_depths[v["scene"]], k), tag=i)
+        ins = [ve.FrameInputs(v["W"], v["H"], k, (v["seed"] + 7919 * k) & 0xFFFFFFFF, k + 1) for v in views]
+        ubos = [_ubo(ve, v, scene_depths[v["scene"]], k) for v in views]
+        if batched:
+            srv.submit_batch(sids, fcs, ins, ubos, tags=list(range(len(views))))
+        else:
+            for i in range(len(views)):
+                srv.submit(sids[i], fcs[i], ins[i], ubos[i], tag=i)
         got = 0
         while got < len(views):
             batch = srv.poll(wait=True)
@@ -262,13 +323,15 @@ def run_batched(ve, srv, scene_ids, scene_depths, views, frames, materials, sky=
     return summarize(records)
 
 
-def batched(ve, res_scenes, count=64, frames=6, seed=0, queues_list=(0, 1), log=None):
-    """The batched configuration on the concurrent server and on the plain loop."""
+def batched(ve, res_scenes, count=64, frames=6, seed=0, queues_list=(0, 1, "batched"), log=None):
+    """The batched configuration on the concurrent server, the plain loop and
+    the batched server (one persistent primary launch per tick, graphs)."""
     mats = S.palette(1)
     views = make_viewports(count, len(res_scenes), seed)
     out = {}
     for q in queues_list:
-        srv = ve.Server(queues=q)
+        srv = make_server(ve, q, max(64, count))
+        srv.batched = q == "batched"
         try:
             ids = [srv.add_scene(w, d, info.size) for d, w, info in res_scenes]
             run_batched(ve, srv, ids, [d for d, _, _ in res_scenes], views, 1, mats)   # warm: contexts created
@@ -276,7 +339,7 @@ def batched(ve, res_scenes, count=64, frames=6, seed=0, queues_list=(0, 1), log=
         finally:
             srv.close()
         if log:
-            log("batched queues=%d %s" % (q, out[q]))
+            log("batched mode=%s %s" % (q, out[q]))
     return out
 
 
@@ -287,4 +350,4 @@ def max_sustained(rows, p99_ms=P99_MS):
 
 
 __all__ = ["served_scenes", "session_knobs", "make_workload", "run_workload", "summarize", "sweep", "max_sustained",
-           "make_viewports", "run_batched", "batched", "CLIENT_FPS"]
+           "make_viewports", "run_batched", "batched", "make_server", "batch_tuning", "CLIENT_FPS"]

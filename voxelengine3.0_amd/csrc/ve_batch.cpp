@@ -23,21 +23,22 @@ struct ve_batch {
     ve_batch_config cfg{};
     uint32_t blocks = 0;                     // persistent grid
     hipStream_t stream = nullptr;
-    ve::BatchView *views_dev = nullptr;
-    uint32_t *bases_dev = nullptr;           // max_views + MAX_GROUPS
-    uint32_t *queue_dev = nullptr;           // MAX_GROUPS counters
+    // one block on the device and in each staging set, filled by one copy:
+    // queue counters (MAX_GROUPS, zero), bases (max_views + MAX_GROUPS), views
+    size_t off_views = 0, block_bytes = 0;
+    unsigned char *dev = nullptr;
     // pinned staging sets: a set is rewritten only once its copy has run (a
     // query, never a host wait: a serving loop must not block); a new set is
     // added when none is free
     struct Staging {
-        ve::BatchView *views = nullptr;
-        uint32_t *bases = nullptr;
+        unsigned char *host = nullptr;
         hipEvent_t copied = nullptr;
     };
     std::vector<Staging> staging;
     std::vector<hipEvent_t> ctx_ev;          // per view: its stream's work before the batch
     hipEvent_t done[64] = {};                // per group: its primary launch finished
     hipEvent_t t0 = nullptr, t1 = nullptr;   // around the primary launch
+    bool launched = false;                   // t1 recorded at least once
     std::vector<ve::FrameParams> params;
     std::string err;
 };
@@ -114,16 +115,18 @@ int batch_primary(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const 
     if (!st) {
         ve_batch::Staging x;
         void *h = nullptr;
-        B_HIP(b, hipHostMalloc(&h, (size_t)b->max_views * sizeof(BatchView), hipHostMallocDefault));
-        x.views = static_cast<BatchView *>(h);
-        B_HIP(b, hipHostMalloc(&h, ((size_t)b->max_views + MAX_GROUPS) * sizeof(uint32_t), hipHostMallocDefault));
-        x.bases = static_cast<uint32_t *>(h);
+        B_HIP(b, hipHostMalloc(&h, b->block_bytes, hipHostMallocDefault));
+        x.host = static_cast<unsigned char *>(h);
         B_HIP(b, hipEventCreateWithFlags(&x.copied, hipEventDisableTiming));
         b->staging.push_back(x);
         st = &b->staging.back();
     }
-    BatchView *vh = st->views;
-    uint32_t *bh = st->bases;   // group g's bases at first[g] + g, first[g + 1] - first[g] + 1 of them
+    uint32_t *qh = reinterpret_cast<uint32_t *>(st->host);
+    uint32_t *bh = qh + MAX_GROUPS;   // group g's bases at first[g] + g, first[g + 1] - first[g] + 1 of them
+    BatchView *vh = reinterpret_cast<BatchView *>(st->host + b->off_views);
+    uint32_t *queue_dev = reinterpret_cast<uint32_t *>(b->dev), *bases_dev = queue_dev + MAX_GROUPS;
+    BatchView *views_dev = reinterpret_cast<BatchView *>(b->dev + b->off_views);
+    std::memset(qh, 0, MAX_GROUPS * sizeof(uint32_t));
     std::vector<uint32_t> totals(groups, 0);
     for (uint32_t gi = 0; gi < groups; gi++) {
         uint32_t total = 0;
@@ -151,14 +154,17 @@ int batch_primary(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const 
         bh[first[gi + 1] + gi] = total;
         totals[gi] = total;
     }
+    // wait only for streams with work still queued (an idle one needs no wait)
     for (uint32_t i = 0; i < n; i++) {
+        const hipError_t q = hipStreamQuery(ctxs[i]->stream);
+        if (q == hipSuccess) continue;
+        if (q != hipErrorNotReady) return batch_error(b, VE_EDEVICE, "hipStreamQuery: %s", hipGetErrorString(q));
         B_HIP(b, hipEventRecord(b->ctx_ev[i], ctxs[i]->stream));
         B_HIP(b, hipStreamWaitEvent(b->stream, b->ctx_ev[i], 0));
     }
-    B_HIP(b, hipMemcpyAsync(b->views_dev, vh, n * sizeof(BatchView), hipMemcpyHostToDevice, b->stream));
-    B_HIP(b, hipMemcpyAsync(b->bases_dev, bh, (n + groups) * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+    B_HIP(b, hipMemcpyAsync(b->dev, st->host, b->off_views + (size_t)n * sizeof(BatchView), hipMemcpyHostToDevice,
+                            b->stream));
     B_HIP(b, hipEventRecord(st->copied, b->stream));
-    B_HIP(b, hipMemsetAsync(b->queue_dev, 0, groups * sizeof(uint32_t), b->stream));
     if (starts)
         for (uint32_t i = 0; i < n; i++) B_HIP(b, hipEventRecord(starts[i], b->stream));
     B_HIP(b, hipEventRecord(b->t0, b->stream));
@@ -168,16 +174,19 @@ int batch_primary(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const 
         // no more blocks than the group's runs of tiles need (one run per wave)
         const uint64_t runs = ((uint64_t)totals[gi] + b->cfg.grab - 1) / b->cfg.grab;
         const uint32_t blocks = (uint32_t)std::min<uint64_t>(b->blocks, (runs + 3) / 4);
-        launch_primary_batch(b->stream, b->views_dev + first[gi], b->bases_dev + first[gi] + gi, m, totals[gi],
-                             b->queue_dev + gi, tw, b->cfg.grab, blocks);
+        launch_primary_batch(b->stream, views_dev + first[gi], bases_dev + first[gi] + gi, m, totals[gi],
+                             queue_dev + gi, tw, b->cfg.grab, blocks);
         B_HIP(b, hipGetLastError());
         B_HIP(b, hipEventRecord(b->done[gi], b->stream));
         for (uint32_t j = first[gi]; j < first[gi + 1]; j++)
             B_HIP(b, hipStreamWaitEvent(ctxs[order[j]]->stream, b->done[gi], 0));
     }
     B_HIP(b, hipEventRecord(b->t1, b->stream));
+    b->launched = true;
     return VE_OK;
 }
+
+bool batch_idle(ve_batch *b) { return !b->launched || hipEventQuery(b->t1) != hipErrorNotReady; }
 
 const char *batch_error_text(const ve_batch *b) { return b->err.c_str(); }
 
@@ -210,9 +219,9 @@ int ve_batch_create(ve_batch **out, const ve_batch_config *cfg)
         b->blocks = (cus * c.waves_per_cu + 3u) / 4u;   // 4 waves per block
     }
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);   // default priority
-    if (e == hipSuccess) e = hipMalloc(&b->views_dev, (size_t)c.max_views * sizeof(ve::BatchView));
-    if (e == hipSuccess) e = hipMalloc(&b->bases_dev, ((size_t)c.max_views + MAX_GROUPS) * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&b->queue_dev, MAX_GROUPS * sizeof(uint32_t));
+    b->off_views = ((2 * (size_t)MAX_GROUPS + c.max_views) * sizeof(uint32_t) + 255) & ~(size_t)255;
+    b->block_bytes = b->off_views + (size_t)c.max_views * sizeof(ve::BatchView);
+    if (e == hipSuccess) e = hipMalloc(&b->dev, b->block_bytes);
     b->ctx_ev.assign(c.max_views, nullptr);
     for (uint32_t i = 0; i < c.max_views && e == hipSuccess; i++)
         e = hipEventCreateWithFlags(&b->ctx_ev[i], hipEventDisableTiming);
@@ -242,12 +251,9 @@ void ve_batch_destroy(ve_batch *b)
         if (e) (void)hipEventDestroy(e);
     for (auto &x : b->staging) {
         if (x.copied) (void)hipEventDestroy(x.copied);
-        if (x.views) (void)hipHostFree(x.views);
-        if (x.bases) (void)hipHostFree(x.bases);
+        if (x.host) (void)hipHostFree(x.host);
     }
-    if (b->views_dev) (void)hipFree(b->views_dev);
-    if (b->bases_dev) (void)hipFree(b->bases_dev);
-    if (b->queue_dev) (void)hipFree(b->queue_dev);
+    if (b->dev) (void)hipFree(b->dev);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }

@@ -121,6 +121,8 @@ int frame_rest(ve_ctx *c, ve_scene *s, const FrameParams &p);
 int batch_primary(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const *scenes, const FrameParams *ps,
                   hipEvent_t *starts);
 const char *batch_error_text(const ve_batch *b);
+// No primary launch of the batch still running (a query, no host wait).
+bool batch_idle(ve_batch *b);
 int scene_upload(int device, hipStream_t stream, const uint32_t *words, uint32_t n_words, uint32_t depth,
                  ve_scene **scene, std::string &err, uint32_t jump_level = JUMP_LEVEL);
 void scene_free(ve_scene *s);

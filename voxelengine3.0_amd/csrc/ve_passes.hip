@@ -2160,57 +2160,73 @@ __device__ __forceinline__ void lb_add(uint32_t *pl, uint4 d, uint4 e)
 // halving stores, all firefly reads before any add) and leave; the others
 // fold the loaded time into their eviction candidate.  Lanes that never
 // match reset their candidate after the loop, in lane order.  One thread.
-__device__ void deposit_lockstep(const FrameParams &p, uint32_t *bk, uint32_t k, const uint32_t *vid, uint4 *d,
-                                 uint4 *e)
+// The lanes' records sit in the bucket's LDS copy after its slots: voxel
+// ids, deposits (direct, indirect; firefly-clamped in place) and eviction
+// candidates, so that this one-thread loop keeps no arrays of its own (they
+// would live in scratch memory and hold the kernel's registers).
+constexpr uint32_t LK_VID = LB_WORDS, LK_D = LB_WORDS + 8u, LK_E = LB_WORDS + 40u, LK_CT = LB_WORDS + 72u,
+                   LK_CI = LB_WORDS + 80u, LK_WORDS = LB_WORDS + 88u;
+
+__device__ __forceinline__ uint4 lk_load(const uint32_t *bk, uint32_t at)
 {
-    bool active[8];
-    uint32_t cand_t[8], cand_i[8];
+    return make_uint4(bk[at], bk[at + 1u], bk[at + 2u], bk[at + 3u]);
+}
+
+__device__ void deposit_lockstep(const FrameParams &p, uint32_t *bk, uint32_t k)
+{
+    uint32_t active = (1u << k) - 1u;
     for (uint32_t l = 0; l < k; l++) {
-        active[l] = true;
-        cand_t[l] = NONE;
-        cand_i[l] = 0u;
+        bk[LK_CT + l] = NONE;
+        bk[LK_CI + l] = 0u;
     }
-    for (uint32_t i = 0; i < LBUFFER_SLOTS; i++) {
-        bool any = false;
-        for (uint32_t l = 0; l < k; l++) any |= active[l];
-        if (!any) break;
+    for (uint32_t i = 0; i < LBUFFER_SLOTS && active; i++) {
         for (uint32_t l = 0; l < k; l++)
-            if (active[l] && bk[i] == 0u) bk[i] = vid[l];
+            if (((active >> l) & 1u) && bk[i] == 0u) bk[i] = bk[LK_VID + l];
         const uint32_t id = bk[i], st = bk[LB_TIME + i];   // every active lane loads the same slot
-        bool match[8];
+        uint32_t match = 0u;
         for (uint32_t l = 0; l < k; l++) {
-            match[l] = active[l] && id == vid[l];
-            if (active[l] && !match[l] && st < cand_t[l]) {
-                cand_t[l] = st;
-                cand_i[l] = i;
+            if (!((active >> l) & 1u)) continue;
+            if (id == bk[LK_VID + l]) match |= 1u << l;
+            else if (st < bk[LK_CT + l]) {
+                bk[LK_CT + l] = st;
+                bk[LK_CI + l] = i;
             }
         }
         uint32_t *pl = bk + LB_PAY + i * LB_PAYW;
-        for (uint32_t l = 0; l < k; l++)   // stale reset (shade.comp:183-186)
-            if (match[l] && !p.disable_stale_reset && st != 0u && st + p.update_time < fp_time(p)) {
-                lb_reset(bk, i, vid[l], d[l], e[l], fp_time(p));
-                match[l] = active[l] = false;
-            }
+        if (match && !p.disable_stale_reset && st != 0u && st + p.update_time < fp_time(p)) {   // shade.comp:183-186
+            for (uint32_t l = 0; l < k; l++)
+                if ((match >> l) & 1u)
+                    lb_reset(bk, i, bk[LK_VID + l], lk_load(bk, LK_D + 4u * l), lk_load(bk, LK_E + 4u * l), fp_time(p));
+            active &= ~match;
+            match = 0u;
+        }
         for (uint32_t ch = 0; ch < 2; ch++) {   // halving at the caps (shade.comp:188-194)
             const uint32_t f0 = ch ? 5u : 0u, cap = ch ? p.cap_indirect : p.cap_direct;
-            bool halve = false;
-            for (uint32_t l = 0; l < k; l++) halve |= match[l] && pl[f0] >= cap;
-            if (halve)   // every halving lane loads the same values before any stores
+            if (match && pl[f0] >= cap)   // every halving lane loads the same values before any stores
                 for (uint32_t f = 0; f < 5; f++) pl[f0 + f] >>= 1;
         }
+        for (uint32_t ch = 0; ch < 2; ch++)   // all firefly reads before any add
+            for (uint32_t l = 0; l < k; l++)
+                if ((match >> l) & 1u) {
+                    const uint32_t at = (ch ? LK_E : LK_D) + 4u * l;
+                    uint4 v = lk_load(bk, at);
+                    lb_firefly(pl, ch ? 5u : 0u, v, p.firefly_k, p.firefly_floor);
+                    bk[at] = v.x;
+                    bk[at + 1u] = v.y;
+                    bk[at + 2u] = v.z;
+                    bk[at + 3u] = v.w;
+                }
         for (uint32_t l = 0; l < k; l++)
-            if (match[l]) lb_firefly(pl, 0, d[l], p.firefly_k, p.firefly_floor);
-        for (uint32_t l = 0; l < k; l++)
-            if (match[l]) lb_firefly(pl, 5, e[l], p.firefly_k, p.firefly_floor);
-        for (uint32_t l = 0; l < k; l++)
-            if (match[l]) {
-                lb_add(pl, d[l], e[l]);
+            if ((match >> l) & 1u) {
+                lb_add(pl, lk_load(bk, LK_D + 4u * l), lk_load(bk, LK_E + 4u * l));
                 bk[LB_TIME + i] = fp_time(p);
-                active[l] = false;
             }
+        active &= ~match;
     }
     for (uint32_t l = 0; l < k; l++)   // bucket full: reset the candidate (shade.comp:254-256)
-        if (active[l] && cand_t[l] != NONE) lb_reset(bk, cand_i[l], vid[l], d[l], e[l], fp_time(p));
+        if (((active >> l) & 1u) && bk[LK_CT + l] != NONE)
+            lb_reset(bk, bk[LK_CI + l], bk[LK_VID + l], lk_load(bk, LK_D + 4u * l), lk_load(bk, LK_E + 4u * l),
+                     fp_time(p));
 }
 
 // Lane-held slot of an lBuffer bucket: owner, time stamp, payload.
@@ -2248,13 +2264,17 @@ __device__ __forceinline__ void lslot_deposit(const FrameParams &p, LSlot &s, ui
 
 // The order key of record `rec` (without its list index): chunk, bounce
 // iteration, SIMD-8 vector, call site, lane.
-__device__ __forceinline__ uint32_t dep_key(const Deferred *__restrict__ deferred, const uint32_t *__restrict__ wpos,
-                                            uint32_t rec)
+__device__ __forceinline__ uint32_t dep_key_meta(uint32_t meta, const uint32_t *__restrict__ wpos)
 {
-    const uint32_t meta = deferred[rec].meta;
     const uint32_t li = meta & 0x3FFFFu, it = (meta >> 18) & 0xFu, site = (meta >> 22) & 3u;
     const uint32_t w = wpos[li];
     return ((li / RAY_CHUNK) << 22) | (it << 18) | ((w >> 3) << 5) | (site << 3) | (w & 7u);
+}
+
+__device__ __forceinline__ uint32_t dep_key(const Deferred *__restrict__ deferred, const uint32_t *__restrict__ wpos,
+                                            uint32_t rec)
+{
+    return dep_key_meta(deferred[rec].meta, wpos);
 }
 
 __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_dep_replay(FrameParams p, uint32_t *__restrict__ lbuf,
@@ -2262,8 +2282,10 @@ __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_dep_replay(FrameParams p,
                                                                   const uint32_t *__restrict__ wpos,
                                                                   uint32_t *__restrict__ counters)
 {
-    __shared__ uint64_t lds_keys_all[REPLAY_WAVES][ORDER_LDS_EVENTS];
-    __shared__ __align__(16) uint32_t bk_all[REPLAY_WAVES][LB_WORDS];
+    // a bucket's list sorts in LDS only while it fits in place (n <= DEP_CAP):
+    // beyond, its keys go to the global spill
+    __shared__ uint64_t lds_keys_all[REPLAY_WAVES][DEP_CAP];
+    __shared__ __align__(16) uint32_t bk_all[REPLAY_WAVES][LK_WORDS];
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t h = blockIdx.x * REPLAY_WAVES + wv;
     if (h >= p.hash_width) return;
@@ -2272,43 +2294,17 @@ __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_dep_replay(FrameParams p,
     if (lane == 0u) dl.cnt[h] = 0u;   // the next frame's count starts from zero
     uint32_t *bk = bk_all[wv];
     const uint32_t *row = dl.slots + (size_t)h * dl.cap;
-    // records: row[i] in place, then this bucket's overflow entries (low word bit 31)
-    SortedKeys sk{n, ~0ull, nullptr, false};
-    if (n <= dl.cap) {
-        sk = sort_bucket(n, [&](uint32_t i) { return dep_key(deferred, wpos, row[i]); }, lds_keys_all[wv], nullptr);
-    } else {
-        uint32_t base = 0;
-        if (lane == 0u) base = atomicAdd(&dl.over_n[1], n);
-        base = __shfl(base, 0, 64);
-        sk.keys = dl.spill + base;   // spill words for all overflowing buckets sum to at most the record count
-        sk.glob = true;
-        for (uint32_t i = lane; i < dl.cap; i += 64u)
-            __hip_atomic_store(sk.keys + i, ((uint64_t)dep_key(deferred, wpos, row[i]) << 32) | i, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t L = dl.over_n[0];
-        uint32_t put = dl.cap;
-        for (uint32_t j0 = 0; j0 < L; j0 += 64u) {
-            const uint32_t j = j0 + lane;
-            const bool m = j < L && dl.over[j].x == h;
-            const uint64_t mb = __ballot(m);
-            const uint32_t at = put + (uint32_t)__popcll(mb & ((1ull << lane) - 1ull));
-            if (m && at < n)
-                __hip_atomic_store(sk.keys + at, ((uint64_t)dep_key(deferred, wpos, dl.over[j].y) << 32) | 0x80000000u | j,
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            put += (uint32_t)__popcll(mb);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
-        wave_sort_keys<true>(sk.keys, n);
-    }
     const bool sl = lane < LBUFFER_SLOTS;
     uint32_t *gb = lbuf + (size_t)h * LB_WORDS;
     LSlot st{NONE, NONE, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
-    if (sl) {
+    if (sl) {   // issued first: the loads overlap the records' gather and the sort
         st.owner = gb[lane];
         st.tstamp = gb[LB_TIME + lane];
 #pragma unroll
         for (uint32_t f = 0; f < LB_PAYW; f++) st.pl[f] = gb[LB_PAY + lane * LB_PAYW + f];
     }
+    // records: row[i] in place, then this bucket's overflow entries (low word bit 31)
+    SortedKeys sk{n, ~0ull, nullptr, false};
     // this lane's record of the window [w0, w0 + 64)
     uint32_t w0 = 0, r_grp = NONE, r_vid = 0u;
     uint4 r_d = make_uint4(0u, 0u, 0u, 0u), r_e = r_d;
@@ -2327,7 +2323,55 @@ __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_dep_replay(FrameParams p,
             r_e = pack_rgb_l2(f3{dd.il[0], dd.il[1], dd.il[2]});
         }
     };
-    load_window(0);
+    if (n <= 64u) {
+        // lane i gathers record row[i] whole; after the sort lane q takes the
+        // q-th record's fields from the lane that gathered it (no second gather)
+        Deferred dd{0u, 0u, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+        uint32_t kh = 0u;
+        if (lane < n) {
+            dd = deferred[row[lane]];
+            kh = dep_key_meta(dd.meta, wpos);
+        }
+        sk.reg = wave_sort64(lane < n ? ((uint64_t)kh << 32) | lane : ~0ull);
+        const int src = (int)((uint32_t)sk.reg & 63u);
+        const uint32_t vid = __shfl(dd.vid, src, 64);
+        const f3 dv{__shfl(dd.dl[0], src, 64), __shfl(dd.dl[1], src, 64), __shfl(dd.dl[2], src, 64)};
+        const f3 ev{__shfl(dd.il[0], src, 64), __shfl(dd.il[1], src, 64), __shfl(dd.il[2], src, 64)};
+        if (lane < n) {
+            r_grp = (uint32_t)(sk.reg >> 35);   // the order key without its lane bits
+            r_vid = vid;
+            r_d = pack_rgb_l2(dv);
+            r_e = pack_rgb_l2(ev);
+        }
+    } else {
+        if (n <= dl.cap) {
+            sk = sort_bucket(n, [&](uint32_t i) { return dep_key(deferred, wpos, row[i]); }, lds_keys_all[wv], nullptr);
+        } else {
+            uint32_t base = 0;
+            if (lane == 0u) base = atomicAdd(&dl.over_n[1], n);
+            base = __shfl(base, 0, 64);
+            sk.keys = dl.spill + base;   // spill words for all overflowing buckets sum to at most the record count
+            sk.glob = true;
+            for (uint32_t i = lane; i < dl.cap; i += 64u)
+                __hip_atomic_store(sk.keys + i, ((uint64_t)dep_key(deferred, wpos, row[i]) << 32) | i, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t L = dl.over_n[0];
+            uint32_t put = dl.cap;
+            for (uint32_t j0 = 0; j0 < L; j0 += 64u) {
+                const uint32_t j = j0 + lane;
+                const bool m = j < L && dl.over[j].x == h;
+                const uint64_t mb = __ballot(m);
+                const uint32_t at = put + (uint32_t)__popcll(mb & ((1ull << lane) - 1ull));
+                if (m && at < n)
+                    __hip_atomic_store(sk.keys + at, ((uint64_t)dep_key(deferred, wpos, dl.over[j].y) << 32) | 0x80000000u | j,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                put += (uint32_t)__popcll(mb);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+            wave_sort_keys<true>(sk.keys, n);
+        }
+        load_window(0);
+    }
     for (uint32_t k = 0; k < n;) {
         if (k + 8u > w0 + 64u && w0 + 64u < n) load_window(k);   // a whole group inside the window
         const uint32_t j = k - w0;
@@ -2359,22 +2403,20 @@ __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_dep_replay(FrameParams p,
                 bk[LB_TIME + lane] = st.tstamp;
                 for (uint32_t f = 0; f < LB_PAYW; f++) bk[LB_PAY + lane * LB_PAYW + f] = st.pl[f];
             }
-            uint32_t vids[8];
-            uint4 ds[8], es[8];
-            for (uint32_t l = 0; l < m; l++) {
-                const int src = (int)(j + l);
-                vids[l] = (uint32_t)__builtin_amdgcn_readlane((int)r_vid, src);
-                ds[l] = make_uint4((uint32_t)__builtin_amdgcn_readlane((int)r_d.x, src),
-                                   (uint32_t)__builtin_amdgcn_readlane((int)r_d.y, src),
-                                   (uint32_t)__builtin_amdgcn_readlane((int)r_d.z, src),
-                                   (uint32_t)__builtin_amdgcn_readlane((int)r_d.w, src));
-                es[l] = make_uint4((uint32_t)__builtin_amdgcn_readlane((int)r_e.x, src),
-                                   (uint32_t)__builtin_amdgcn_readlane((int)r_e.y, src),
-                                   (uint32_t)__builtin_amdgcn_readlane((int)r_e.z, src),
-                                   (uint32_t)__builtin_amdgcn_readlane((int)r_e.w, src));
+            if (lane >= j && lane < j + m) {   // the group's records, lane j + l into place l
+                const uint32_t l = lane - j;
+                bk[LK_VID + l] = r_vid;
+                bk[LK_D + 4u * l] = r_d.x;
+                bk[LK_D + 4u * l + 1u] = r_d.y;
+                bk[LK_D + 4u * l + 2u] = r_d.z;
+                bk[LK_D + 4u * l + 3u] = r_d.w;
+                bk[LK_E + 4u * l] = r_e.x;
+                bk[LK_E + 4u * l + 1u] = r_e.y;
+                bk[LK_E + 4u * l + 2u] = r_e.z;
+                bk[LK_E + 4u * l + 3u] = r_e.w;
             }
             wave_lds_sync();
-            if (lane == 0u) deposit_lockstep(p, bk, m, vids, ds, es);
+            if (lane == 0u) deposit_lockstep(p, bk, m);
             wave_lds_sync();
             if (sl) {
                 st.owner = bk[lane];

@@ -52,6 +52,7 @@ struct Session {
     uint32_t frames = 0;                  // submitted so far
     std::deque<Frame> pending;            // submitted before admission
     std::deque<Frame> in_flight;          // launched, not yet collected
+    bool queued = false;                  // a frame waits in the server's next batch
     unsigned long long last_totals[3] = {0, 0, 0};
 };
 
@@ -87,6 +88,8 @@ struct ve_server {
     bool graph = false;                    // sessions replay their frame graphs (ve_set_graph)
     ve_batch *batch = nullptr;             // ve_server_submit_batch's primary launches
     ve_batch_config batch_cfg{};
+    // frames of ve_server_submit_batch waiting for the next batch: {session, frame}
+    std::vector<std::pair<uint32_t, Frame>> ready;
 };
 
 namespace {
@@ -236,7 +239,8 @@ int admit(ve_server *s)
 // A closed session with nothing left in flight returns its context to the pool.
 void retire(ve_server *s, Session &ss)
 {
-    if (ss.state != State::Active || !ss.close_requested || !ss.in_flight.empty() || !ss.pending.empty()) return;
+    if (ss.state != State::Active || !ss.close_requested || !ss.in_flight.empty() || !ss.pending.empty() || ss.queued)
+        return;
     s->pool.push_back({ss.ctx, s->use_tick++});
     ss.ctx = nullptr;
     ss.state = State::Closed;
@@ -284,6 +288,61 @@ size_t collect(ve_server *s)
         it = (it->second.state == State::Closed) ? s->sessions.erase(it) : std::next(it);
     return n;
 }
+
+// Launch the frames waiting in the next batch: every frame's checks and
+// parameters, the batch's primary launches, then each frame's other passes
+// on its session's stream.  Unless forced (or the batch is full) this waits
+// while the previous batch's primary launch still runs: frames arriving
+// meanwhile join the next batch, so the batches grow with the load and the
+// batch stream never queues more than one of them.
+int flush_ready(ve_server *s, bool force)
+{
+    if (s->ready.empty()) return VE_OK;
+    if (!force && s->ready.size() < s->batch_cfg.max_views && !ve::batch_idle(s->batch)) return VE_OK;
+    std::vector<std::pair<uint32_t, Frame>> rd;
+    rd.swap(s->ready);
+    const uint32_t m = (uint32_t)rd.size();
+    std::vector<Session *> act(m);
+    std::vector<ve_ctx *> ctxs(m);
+    std::vector<ve_scene *> scs(m);
+    std::vector<ve::FrameParams> ps(m);
+    std::vector<hipEvent_t> starts(m);
+    int rc;
+    for (uint32_t i = 0; i < m; i++) {
+        act[i] = &s->sessions.at(rd[i].first);
+        act[i]->queued = false;
+    }
+    for (uint32_t i = 0; i < m; i++) {
+        Session &ss = *act[i];
+        Frame &f = rd[i].second;
+        if ((rc = get_event(s, &f.start)) || (rc = get_event(s, &f.end)) || (rc = get_slot(s, &f.slot))) return rc;
+        ctxs[i] = ss.ctx;
+        scs[i] = s->scenes[ss.desc.scene];
+        std::memcpy(ss.ctx->cam, f.ubo, sizeof ss.ctx->cam);
+        if ((rc = ve::frame_begin(ss.ctx, scs[i], &f.fc, &f.in, ps[i])))
+            return srv_error(s, rc, "session %u frame %u: %s", ss.id, f.index, ss.ctx->err.c_str());
+        starts[i] = f.start;
+    }
+    if ((rc = ve::batch_primary(s->batch, m, ctxs.data(), scs.data(), ps.data(), starts.data())))
+        return srv_error(s, rc, "batch: %s", ve::batch_error_text(s->batch));
+    for (uint32_t i = 0; i < m; i++) {
+        Session &ss = *act[i];
+        Frame &f = rd[i].second;
+        ve_ctx *c = ss.ctx;
+        if ((rc = ve::frame_rest(c, scs[i], ps[i])))
+            return srv_error(s, rc, "session %u frame %u: %s", ss.id, f.index, c->err.c_str());
+        SRV_HIP(s, hipMemcpyAsync(f.slot->v, ve::ctx_ray_totals(c), 3 * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, c->stream));
+        SRV_HIP(s, hipEventRecord(f.end, c->stream));
+        f.seq = s->seq++;
+        ss.in_flight.push_back(f);
+    }
+    return VE_OK;
+}
+
+// Frames of a session must run in order: one waiting in the next batch goes
+// first.
+int flush_for(ve_server *s, const Session &ss) { return ss.queued ? flush_ready(s, true) : VE_OK; }
 
 }  // namespace
 
@@ -415,7 +474,9 @@ int ve_server_submit(ve_server *s, uint32_t session, const ve_frame_config *fc, 
         ss.pending.push_back(f);
         return VE_OK;
     }
-    const int rc = launch(s, ss, f);
+    int rc = flush_for(s, ss);
+    if (rc != VE_OK) return rc;
+    rc = launch(s, ss, f);
     if (rc != VE_OK) return rc;
     ss.in_flight.push_back(f);
     return VE_OK;
@@ -438,8 +499,12 @@ int ve_server_set_batch(ve_server *s, const ve_batch_config *cfg)
     c.device = s->device;
     c.max_views = std::max(c.max_views ? c.max_views : 256u, s->max_sessions);   // a batch holds every admitted session
     ve_batch *b = nullptr;
-    const int rc = ve_batch_create(&b, &c);
+    int rc = ve_batch_create(&b, &c);
     if (rc != VE_OK) return srv_error(s, rc, "batch: %s", ve_last_error(nullptr));
+    if (s->batch && (rc = flush_ready(s, true)) != VE_OK) {
+        ve_batch_destroy(b);
+        return rc;
+    }
     if (s->batch) ve_batch_destroy(s->batch);
     s->batch = b;
     s->batch_cfg = c;
@@ -457,18 +522,17 @@ int ve_server_submit_batch(ve_server *s, uint32_t n, const uint32_t *sessions, c
         if (int rc = ve_server_set_batch(s, &c)) return rc;
     }
     const double now = now_ms(s);
-    std::vector<Session *> act;
-    std::vector<Frame> fr;
     for (uint32_t i = 0; i < n; i++) {
         auto it = s->sessions.find(sessions[i]);
         if (it == s->sessions.end() || it->second.close_requested)
             return srv_error(s, VE_ENOTFOUND, "unknown or closing session %u", sessions[i]);
-        Session &ss = it->second;
-        if (ins[i].width != ss.desc.width || ins[i].height != ss.desc.height)
+        if (ins[i].width != it->second.desc.width || ins[i].height != it->second.desc.height)
             return srv_error(s, VE_EINVAL, "frame %ux%u on a %ux%u session", ins[i].width, ins[i].height,
-                             ss.desc.width, ss.desc.height);
-        for (Session *o : act)
-            if (o == &ss) return srv_error(s, VE_EINVAL, "session %u twice in one batch", ss.id);
+                             it->second.desc.width, it->second.desc.height);
+    }
+    int rc;
+    for (uint32_t i = 0; i < n; i++) {
+        Session &ss = s->sessions.find(sessions[i])->second;
         Frame f;
         f.tag = tags ? tags[i] : 0;
         f.index = ss.frames++;
@@ -480,49 +544,19 @@ int ve_server_submit_batch(ve_server *s, uint32_t n, const uint32_t *sessions, c
             ss.pending.push_back(f);
             continue;
         }
+        if ((rc = flush_for(s, ss))) return rc;   // its earlier frame first
         // a session still busy with an earlier frame goes alone on its stream:
         // the batch's launch would otherwise wait for that frame
         if (!ss.in_flight.empty() && hipEventQuery(ss.in_flight.back().end) != hipSuccess) {
-            if (int rc = launch(s, ss, f)) return rc;
+            if ((rc = launch(s, ss, f))) return rc;
             ss.in_flight.push_back(f);
             continue;
         }
-        act.push_back(&ss);
-        fr.push_back(f);
+        s->ready.emplace_back(ss.id, f);
+        ss.queued = true;
+        if (s->ready.size() >= s->batch_cfg.max_views && (rc = flush_ready(s, true))) return rc;
     }
-    const uint32_t m = (uint32_t)act.size();
-    if (m == 0) return VE_OK;
-    std::vector<ve_ctx *> ctxs(m);
-    std::vector<ve_scene *> scs(m);
-    std::vector<ve::FrameParams> ps(m);
-    std::vector<hipEvent_t> starts(m);
-    int rc;
-    for (uint32_t i = 0; i < m; i++) {
-        Session &ss = *act[i];
-        Frame &f = fr[i];
-        if ((rc = get_event(s, &f.start)) || (rc = get_event(s, &f.end)) || (rc = get_slot(s, &f.slot))) return rc;
-        ctxs[i] = ss.ctx;
-        scs[i] = s->scenes[ss.desc.scene];
-        std::memcpy(ss.ctx->cam, f.ubo, sizeof ss.ctx->cam);
-        if ((rc = ve::frame_begin(ss.ctx, scs[i], &f.fc, &f.in, ps[i])))
-            return srv_error(s, rc, "session %u frame %u: %s", ss.id, f.index, ss.ctx->err.c_str());
-        starts[i] = f.start;
-    }
-    if ((rc = ve::batch_primary(s->batch, m, ctxs.data(), scs.data(), ps.data(), starts.data())))
-        return srv_error(s, rc, "batch: %s", ve::batch_error_text(s->batch));
-    for (uint32_t i = 0; i < m; i++) {
-        Session &ss = *act[i];
-        Frame &f = fr[i];
-        ve_ctx *c = ss.ctx;
-        if ((rc = ve::frame_rest(c, scs[i], ps[i])))
-            return srv_error(s, rc, "session %u frame %u: %s", ss.id, f.index, c->err.c_str());
-        SRV_HIP(s, hipMemcpyAsync(f.slot->v, ve::ctx_ray_totals(c), 3 * sizeof(unsigned long long),
-                                  hipMemcpyDeviceToHost, c->stream));
-        SRV_HIP(s, hipEventRecord(f.end, c->stream));
-        f.seq = s->seq++;
-        ss.in_flight.push_back(f);
-    }
-    return VE_OK;
+    return flush_ready(s, false);
 }
 
 int ve_server_close(ve_server *s, uint32_t session)
@@ -551,6 +585,7 @@ int ve_server_poll(ve_server *s, ve_frame_done *out, uint32_t max, uint32_t *n_o
     collect(s);
     int rc = admit(s);
     if (rc != VE_OK) return rc;
+    if ((rc = flush_ready(s, wait && s->done.empty())) != VE_OK) return rc;
     if (wait && s->done.empty()) {
         // block on the oldest frame in flight
         const Frame *oldest = nullptr;
@@ -582,6 +617,7 @@ int ve_server_read_output(ve_server *s, uint32_t session, float *rgba, size_t by
     const size_t need = (size_t)ss.desc.width * ss.desc.height * 16;
     if (bytes > need) return srv_error(s, VE_EINVAL, "read of %zu bytes from a %zu-byte output", bytes, need);
     SRV_HIP(s, hipSetDevice(s->device));
+    if (int rc = flush_for(s, ss)) return rc;
     SRV_HIP(s, hipMemcpyAsync(rgba, ss.ctx->b.output, bytes, hipMemcpyDeviceToHost, ss.ctx->stream));
     SRV_HIP(s, hipStreamSynchronize(ss.ctx->stream));
     return VE_OK;
@@ -593,7 +629,10 @@ int ve_server_read(ve_server *s, uint32_t session, ve_buffer_id id, void *dst, s
     auto it = s->sessions.find(session);
     if (it == s->sessions.end() || !it->second.ctx)
         return srv_error(s, VE_ENOTFOUND, "session %u holds no device state", session);
-    const int rc = ve_read(it->second.ctx, id, dst, bytes);   // ordered after the session's frames
+    SRV_HIP(s, hipSetDevice(s->device));
+    int rc = flush_for(s, it->second);
+    if (rc != VE_OK) return rc;
+    rc = ve_read(it->second.ctx, id, dst, bytes);   // ordered after the session's frames
     if (rc != VE_OK) return srv_error(s, rc, "session %u: %s", session, it->second.ctx->err.c_str());
     return VE_OK;
 }
@@ -605,6 +644,7 @@ int ve_server_get_stats(ve_server *s, ve_server_stats *o)
     o->admitted = s->admitted;
     o->waiting = (uint32_t)s->waiting.size();
     for (auto &kv : s->sessions) o->frames_in_flight += (uint32_t)kv.second.in_flight.size();
+    o->frames_in_flight += (uint32_t)s->ready.size();
     o->pooled = (uint32_t)s->pool.size();
     o->bytes_in_use = s->used;
     o->budget = s->budget;

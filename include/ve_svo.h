@@ -121,10 +121,14 @@ int ve_scene_release(ve_ctx *ctx, ve_scene *scene);
  * band of rows; n > 0 (at most 4096; default 16) deals runs of n groups of
  * 4 tiles to the XCDs in turn.  jump_level: depth of the traversal jump
  * table of scenes uploaded on this context from now on (0 = default 8;
- * 1..8: 8^level words). */
+ * 1..8: 8^level words).  bounce_blocks: grid of the fused bounce pass in
+ * blocks of 256 threads (0 = one thread per shade-list entry; at most 65536);
+ * with fewer threads than rays, a thread whose ray ends takes the next one
+ * from a queue, so lanes idle less while the longest paths run. */
 typedef struct ve_tuning {
     uint32_t tile_chunk;
     uint32_t jump_level;
+    uint32_t bounce_blocks;
 } ve_tuning;
 int ve_set_tuning(ve_ctx *ctx, const ve_tuning *t);
 int ve_get_tuning(ve_ctx *ctx, ve_tuning *t);
@@ -514,8 +518,14 @@ int  ve_server_get_stats(ve_server *srv, ve_server_stats *out);
  * frames are done run in one launch (a session still busy goes alone on its
  * stream), the rest of each frame on its session's stream;
  * frames of sessions still waiting for admission queue as with
- * ve_server_submit.  Each frame's result equals ve_server_submit's. */
+ * ve_server_submit.  Each frame's result equals ve_server_submit's.  A frame
+ * of ve_server_submit_batch waits while the previous batch's primary launch
+ * runs (ve_server_poll launches it once that launch has ended, and at once
+ * when asked to wait): the frames falling due meanwhile share the next launch.
+ * ve_server_set_tuning: ve_set_tuning for every session's context, present
+ * and future (jump_level is the server's scenes' and is ignored here). */
 int  ve_server_set_graph(ve_server *srv, int enable);
+int  ve_server_set_tuning(ve_server *srv, const ve_tuning *t);
 int  ve_server_set_batch(ve_server *srv, const ve_batch_config *cfg);
 int  ve_server_submit_batch(ve_server *srv, uint32_t n, const uint32_t *sessions, const ve_frame_config *fcs,
                             const ve_frame_inputs *ins, const float *ubos, const uint64_t *tags,

@@ -167,6 +167,7 @@ def lib():
         "ve_batch_primary_ms": (i32, [p, C.POINTER(C.c_float)]),
         "ve_graph_stats": (i32, [p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "ve_server_set_graph": (i32, [p, i32]),
+        "ve_server_set_tuning": (i32, [p, p]),
         "ve_server_set_batch": (i32, [p, C.POINTER(BatchConfig)]),
         "ve_server_submit_batch": (i32, [p, u32, p, p, p, p, p, p]),
         "ve_get_tuning": (i32, [p, p]),
@@ -493,15 +494,15 @@ class Renderer:
         self._ck(lib().ve_graph_stats(self.h, C.byref(a), C.byref(b)))
         return dict(captures=a.value, replays=b.value)
 
-    def set_tuning(self, tile_chunk=0, jump_level=0):
+    def set_tuning(self, tile_chunk=16, jump_level=0, bounce_blocks=0):
         """ve_set_tuning: launch-shape knobs (speed only, never results)."""
-        t = (C.c_uint32 * 2)(tile_chunk, jump_level)
+        t = (C.c_uint32 * 3)(tile_chunk, jump_level, bounce_blocks)
         self._ck(lib().ve_set_tuning(self.h, t))
 
     def tuning(self):
-        t = (C.c_uint32 * 2)()
+        t = (C.c_uint32 * 3)()
         self._ck(lib().ve_get_tuning(self.h, t))
-        return dict(tile_chunk=t[0], jump_level=t[1])
+        return dict(tile_chunk=t[0], jump_level=t[1], bounce_blocks=t[2])
 
     def set_camera(self, ubo):
         ubo = np.ascontiguousarray(ubo, np.float32)
@@ -669,6 +670,11 @@ class Server:
 
     def set_graph(self, on=True):
         self._ck(lib().ve_server_set_graph(self.h, 1 if on else 0))
+
+    def set_tuning(self, tile_chunk=16, bounce_blocks=0):
+        """ve_server_set_tuning: Renderer.set_tuning for every session's context."""
+        t = (C.c_uint32 * 3)(tile_chunk, 0, bounce_blocks)
+        self._ck(lib().ve_server_set_tuning(self.h, t))
 
     def set_batch(self, tile_w=8, grab=4, waves_per_cu=16, groups=4):
         cfg = BatchConfig(0, 0, tile_w, grab, waves_per_cu, groups)

@@ -93,7 +93,7 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.counters, NUM_COUNTERS);
     a(&b.rank_stats, RANK_STATS_WORDS);
     a(&b.rank_part, (size_t)RANK_COPIES * RANK_COPY_WORDS);
-    a(&b.bounce_part, (size_t)BOUNCE_COPIES * 2);
+    a(&b.bounce_part, (size_t)BOUNCE_COPIES * 2 + 1);
     a(&b.emissives, EMISSIVE_CAPACITY);
     a(&b.emissive_claim, (size_t)EMISSIVE_CLAIM_W * EMISSIVE_CLAIM_PROBES);
     a(&b.em_bits, SHADE_BUDGET_MAX / 32);
@@ -176,6 +176,7 @@ void fill_params(ve_ctx *c, const ve_scene *s, const ve_frame_config *fc, const 
     p.jump = s ? s->jump : nullptr;
     p.jump_level = s ? s->jump_level : 0;
     p.tile_chunk = c->tuning.tile_chunk;
+    p.bounce_blocks = c->tuning.bounce_blocks;
     p.hash_width = c->hash_width;
     p.max_bounces = c->max_bounces;
     p.frame_idx = in->frame_idx;
@@ -459,6 +460,7 @@ int ve_set_tuning(ve_ctx *c, const ve_tuning *t)
     if (!t) return ctx_error(c, VE_EINVAL, "null tuning");
     if (t->tile_chunk > 4096u) return ctx_error(c, VE_EINVAL, "tile_chunk %u above 4096", t->tile_chunk);
     if (t->jump_level > JUMP_LEVEL) return ctx_error(c, VE_EINVAL, "jump_level %u above %u", t->jump_level, JUMP_LEVEL);
+    if (t->bounce_blocks > 65536u) return ctx_error(c, VE_EINVAL, "bounce_blocks %u above 65536", t->bounce_blocks);
     c->tuning = *t;
     return VE_OK;
 }

@@ -48,7 +48,7 @@ struct ve_ctx {
     uint32_t *sky_tex = nullptr;         // 6 x sky_edge^2 texels, r | g << 8 | b << 16
     size_t sky_tex_words = 0;            // allocated
     bool node_stats = false;
-    ve_tuning tuning{ve::DEFAULT_TILE_CHUNK, 0u};
+    ve_tuning tuning{ve::DEFAULT_TILE_CHUNK, 0u, 0u};
     bool timing = false;
     uint32_t bounce_iter = 0;           // next trace/shade iteration of per-pass dispatch
     // Deposits recorded by pass-by-pass shade dispatches and not yet applied

@@ -184,6 +184,9 @@ struct FrameParams {
     uint32_t jump_level;
     // screen passes: tile groups per XCD run (0 = one band per XCD)
     uint32_t tile_chunk;
+    // fused bounce pass: grid in blocks of 256 (0 = one thread per shade-list
+    // entry); lanes past the grid's rays take further rays from a queue
+    uint32_t bounce_blocks;
     // non-null: frame_idx, frame_seed, time and cam come from this device block
     const FrameDyn *dyn;
     float cam[16];

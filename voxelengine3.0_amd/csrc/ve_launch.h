@@ -52,7 +52,7 @@ struct DeviceBuffers {
     uint32_t *counters = nullptr;   // NUM_COUNTERS
     uint32_t *rank_stats = nullptr; // RANK_STATS_WORDS
     uint32_t *rank_part = nullptr;  // RANK_COPIES x RANK_COPY_WORDS (zero between rank passes)
-    uint32_t *bounce_part = nullptr;  // BOUNCE_COPIES x 2 (zero between fused bounce passes)
+    uint32_t *bounce_part = nullptr;  // BOUNCE_COPIES x 2 + the ray queue (zero between fused bounce passes)
     EmissiveVoxel *emissives = nullptr;  // EMISSIVE_CAPACITY
     uint32_t *emissive_claim = nullptr;  // EMISSIVE_CLAIM_W x EMISSIVE_CLAIM_PROBES
     uint32_t *em_bits = nullptr;         // SHADE_BUDGET_MAX / 32: first-shade rays that may register a light

@@ -1915,6 +1915,9 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, const uint32_t *__
 // fused per ray: each lane takes one ray entering iteration `first` through
 // trace and shade until it ends, so the phase costs about the longest path
 // rather than the sum over iterations of each iteration's longest trace.
+// With a grid smaller than the ray list (bounce_blocks), a lane whose ray
+// has ended takes the next ray from a queue at the iteration boundary, so a
+// wave keeps its lanes busy while its longest path runs.
 // Shading in these iterations reads nothing a deposit writes (normals,
 // materials, the emissive registry; reservoirs are written at bounce 0
 // only), so deposits can wait: each is appended to `deferred` with its
@@ -1928,10 +1931,10 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, const uint32_t *__
 // waves per SIMD (96 VGPRs with 5 spilled; 109 VGPRs and 4 waves otherwise).
 // Ray counts: every ray entering an iteration counts as traced, as in the
 // per-iteration trace dispatch; each lane counts its own, and each wave adds
-// its sums to one of BOUNCE_COPIES copies at the end (k_bin_place folds
+// its sums to one of BOUNCE_COPIES copies at the end (k_dep_prep folds
 // them), since per-iteration atomics on one counter from every wave cost the
 // traversals a quarter of their time.
-template <bool STATS>
+template <bool STATS, bool REFILL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_bounce(const uint32_t *__restrict__ words, FrameParams p,
                                                 const Ray *__restrict__ rays_in, const uint32_t *__restrict__ live_ptr,
                                                 uint32_t first, uint32_t iters, uint32_t *__restrict__ counters,
@@ -1945,12 +1948,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                                                 Deferred *__restrict__ deferred, uint32_t *__restrict__ bounce_part)
 {
     const uint32_t live = *live_ptr;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
     if (blockIdx.x * blockDim.x >= live) return;   // whole block past the live rays
     __shared__ uint32_t path_cache[VE_MAX_DEPTH * 256];
     NeeCtx nee{words, emissives, rbuf, path_cache + threadIdx.x, 0u};
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t checks_sum = 0, reads = 0, entered = 0, reached = 0;
+    // rays [0, threads) start on their threads; with REFILL the rest are
+    // handed out from the queue to lanes whose ray has ended (no ray's result
+    // depends on the lane or the order: deposits are ordered afterwards by
+    // their keys)
+    uint32_t *queue = bounce_part + 2u * BOUNCE_COPIES;
+    bool drained = !REFILL || threads >= live;
+    uint32_t t = tid, iter_l = first;
     bool active = t < live;
     Ray r;
     uint32_t li = 0u;
@@ -1958,30 +1968,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         r = rays_in[t];
         li = li_in[t];
     }
-    for (uint32_t iter = first; iter < iters; iter++) {
+    for (uint32_t step = 0;; step++) {
+        const uint32_t iter = REFILL ? iter_l : first + step;   // without REFILL every lane is at one iteration
+        if (!REFILL && iter >= iters) break;
+        if (REFILL && !drained) {
+            const uint64_t idle = __ballot(!active);
+            if (idle) {
+                const int leader = __ffsll((unsigned long long)idle) - 1;
+                uint32_t base = 0;
+                if ((int)lane == leader) base = atomicAdd(queue, (uint32_t)__popcll(idle));
+                base = threads + (uint32_t)__shfl((int)base, leader, 64);
+                if (!active) {
+                    const uint32_t q = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                    if (q < live) {
+                        t = q;
+                        iter_l = first;
+                        r = rays_in[q];
+                        li = li_in[q];
+                        active = true;
+                    }
+                }
+                drained = base + (uint32_t)__popcll(idle) >= live;
+            }
+        }
         if (__ballot(active) == 0ull) break;
         if (!active) continue;
+        const uint32_t it = REFILL ? iter_l : iter;
         entered++;
-        reached = iter + 1u;
+        reached = max(reached, it + 1u);
         uint32_t checks = 0;
-        trace_ray<STATS>(words, p, iter, t, r, mats, em_claim, em_bits, path_cache + threadIdx.x, checks, reads);
+        trace_ray<STATS>(words, p, it, t, r, mats, em_claim, em_bits, path_cache + threadIdx.x, checks, reads);
         checks_sum += checks;
-        const int outcome = shade_ray<false>(p, iter, t, r, nbuf, mats, em_reg, counters, nee);
+        const int outcome = shade_ray<false>(p, it, t, r, nbuf, mats, em_reg, counters, nee);
         if (outcome >= RAY_DEPOSITS) {   // at most one deposit per ray: the slot is always in range
             const uint32_t slot = atomicAdd(&counters[VE_CNT_DEFERRED], 1u);
             if (slot < SHADE_BUDGET_MAX)
                 deferred[slot] = Deferred{r.primary_vid,
-                                          li | (iter << 18) | ((uint32_t)(outcome - RAY_DEPOSITS) << 22),
+                                          li | (it << 18) | ((uint32_t)(outcome - RAY_DEPOSITS) << 22),
                                           {r.dlx, r.dly, r.dlz}, {r.ilx, r.ily, r.ilz}};
         }
         active = outcome == RAY_SURVIVES;
+        if (REFILL) active = active && ++iter_l < iters;
     }
     for (int off = 32; off > 0; off >>= 1) {
         entered += __shfl_xor(entered, off, 64);
         reached = max(reached, (uint32_t)__shfl_xor((int)reached, off, 64));
     }
     if (lane == 0u && entered) {
-        uint32_t *part = bounce_part + 2u * ((t >> 6) % BOUNCE_COPIES);
+        uint32_t *part = bounce_part + 2u * ((tid >> 6) % BOUNCE_COPIES);
         atomicAdd(&part[0], entered);
         atomicMax(&part[1], reached);
     }
@@ -2049,6 +2083,7 @@ __global__ __launch_bounds__(256) void k_dep_prep(FrameParams p, const Deferred 
             reached = max(reached, bounce_part[2u * c + 1u]);
             bounce_part[2u * c] = bounce_part[2u * c + 1u] = 0u;
         }
+        bounce_part[2u * BOUNCE_COPIES] = 0u;   // the fused pass's ray queue
         counters[VE_CNT_RAYS_TRACED] += rays;
         totals[1] += rays;
         counters[VE_CNT_BOUNCES_RUN] = max(counters[VE_CNT_BOUNCES_RUN], reached);
@@ -2967,15 +3002,16 @@ void launch_shade(hipStream_t s, const uint32_t *words, const FrameParams &p, De
 void launch_bounce(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t first,
                    uint32_t iters, bool stats)
 {
-    const uint32_t nb = cdiv(p.shade_budget, 256);
-    if (stats)
-        launch(k_bounce<true>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, first), live_in_ptr(b, first), first,
-               iters, b.counters, b.node_stats, b.ray_totals, b.materials, b.nbuf, b.rbuf, b.emissives, b.em_reg,
+    const uint32_t full = cdiv(p.shade_budget, 256);
+    const uint32_t nb = p.bounce_blocks ? std::min(full, p.bounce_blocks) : full;
+    auto go = [&](auto kernel) {
+        launch(kernel, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, first), live_in_ptr(b, first), first, iters,
+               b.counters, b.node_stats, b.ray_totals, b.materials, b.nbuf, b.rbuf, b.emissives, b.em_reg,
                b.emissive_claim, b.em_bits, li_in(b, first), b.deferred, b.bounce_part);
-    else
-        launch(k_bounce<false>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, first), live_in_ptr(b, first), first,
-               iters, b.counters, b.node_stats, b.ray_totals, b.materials, b.nbuf, b.rbuf, b.emissives, b.em_reg,
-               b.emissive_claim, b.em_bits, li_in(b, first), b.deferred, b.bounce_part);
+    };
+    const bool refill = nb < full;
+    if (stats) refill ? go(k_bounce<true, true>) : go(k_bounce<true, false>);
+    else refill ? go(k_bounce<false, true>) : go(k_bounce<false, false>);
 }
 
 int launch_deposits(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_t iters)

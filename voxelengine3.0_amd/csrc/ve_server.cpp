@@ -86,6 +86,8 @@ struct ve_server {
     std::vector<RaySlot *> slot_chunks;
     std::deque<ve_frame_done> done;        // finished, not yet returned
     bool graph = false;                    // sessions replay their frame graphs (ve_set_graph)
+    bool tuned = false;                    // ve_server_set_tuning called: `tuning` goes to every context
+    ve_tuning tuning{};
     ve_batch *batch = nullptr;             // ve_server_submit_batch's primary launches
     ve_batch_config batch_cfg{};
     // frames of ve_server_submit_batch waiting for the next batch: {session, frame}
@@ -185,6 +187,7 @@ int acquire(ve_server *s, Session &ss)
         if (!fits(s->pool[i].ctx, ss.desc, hw)) continue;
         ss.ctx = s->pool[i].ctx;
         ss.ctx->graph = s->graph;
+        if (s->tuned) ss.ctx->tuning = s->tuning;
         s->pool.erase(s->pool.begin() + (long)i);
         if (ve::ctx_reset(ss.ctx) != VE_OK) return srv_error(s, VE_EDEVICE, "context reset: %s", ss.ctx->err.c_str());
         return VE_OK;
@@ -205,6 +208,7 @@ int acquire(ve_server *s, Session &ss)
     if (rc != VE_OK) return srv_error(s, rc, "session %u: %s", ss.id, err.c_str());
     s->used += ve::ctx_bytes(ss.ctx);
     ss.ctx->graph = s->graph;
+    if (s->tuned) ss.ctx->tuning = s->tuning;
     return VE_OK;
 }
 
@@ -489,6 +493,20 @@ int ve_server_set_graph(ve_server *s, int enable)
     for (auto &kv : s->sessions)
         if (kv.second.ctx) kv.second.ctx->graph = s->graph;
     for (auto &p : s->pool) p.ctx->graph = s->graph;
+    return VE_OK;
+}
+
+int ve_server_set_tuning(ve_server *s, const ve_tuning *t)
+{
+    if (!s || !t) return ve_global_error(VE_EINVAL, "null argument");
+    if (t->tile_chunk > 4096u || t->bounce_blocks > 65536u)
+        return srv_error(s, VE_EINVAL, "tile_chunk above 4096 or bounce_blocks above 65536");
+    s->tuning = *t;
+    s->tuning.jump_level = 0u;
+    s->tuned = true;
+    for (auto &kv : s->sessions)
+        if (kv.second.ctx) kv.second.ctx->tuning = s->tuning;
+    for (auto &p : s->pool) p.ctx->tuning = s->tuning;
     return VE_OK;
 }
 

@@ -2264,36 +2264,37 @@ __device__ void deposit_lockstep(const FrameParams &p, uint32_t *bk, uint32_t k)
                      fp_time(p));
 }
 
-// Lane-held slot of an lBuffer bucket: owner, time stamp, payload.
-struct LSlot {
-    uint32_t owner, tstamp, pl[LB_PAYW];
+// Lane-held slot of an lBuffer bucket: lane s (< 32) holds slot s's direct
+// channel (payload words 0-4), lane s + 32 its indirect channel (words 5-9);
+// both keep the slot's owner and time stamp.  depositSampleDual's steps on
+// one channel (halving at its cap, the firefly clamp against its running
+// mean, the add) read nothing of the other, so the two lanes run a deposit
+// side by side.
+struct LChan {
+    uint32_t owner, tstamp, c[5];
 };
 
-__device__ __forceinline__ void lslot_reset(LSlot &s, uint32_t vid, uint4 d, uint4 e, uint32_t time)
-{   // resetLSlotDual (shade.comp:108-126)
+__device__ __forceinline__ void lchan_reset(LChan &s, uint32_t vid, uint4 v, uint32_t time)
+{   // resetLSlotDual (shade.comp:108-126), one channel
     s.owner = vid;
-    s.pl[0] = 1u; s.pl[1] = d.x; s.pl[2] = d.y; s.pl[3] = d.z; s.pl[4] = d.w;
-    s.pl[5] = 1u; s.pl[6] = e.x; s.pl[7] = e.y; s.pl[8] = e.z; s.pl[9] = e.w;
+    s.c[0] = 1u; s.c[1] = v.x; s.c[2] = v.y; s.c[3] = v.z; s.c[4] = v.w;
     s.tstamp = time;
 }
 
 // One invocation's depositSampleDual into slot s (found or claimed), on the
-// lane that holds it.
-__device__ __forceinline__ void lslot_deposit(const FrameParams &p, LSlot &s, uint32_t vid, uint4 d, uint4 e)
+// lane that holds the channel: v is the channel's sample, cap its cap.
+__device__ __forceinline__ void lchan_deposit(const FrameParams &p, LChan &s, uint32_t vid, uint4 v, uint32_t cap)
 {
     const uint32_t st = s.tstamp;
     if (s.owner == 0u) s.owner = vid;
     if (!p.disable_stale_reset && st != 0u && st + p.update_time < fp_time(p)) {   // shade.comp:183-186
-        lslot_reset(s, vid, d, e, fp_time(p));
+        lchan_reset(s, vid, v, fp_time(p));
         return;
     }
-    if (s.pl[0] >= p.cap_direct)   // shade.comp:188-194
-        for (uint32_t f = 0; f < 5; f++) s.pl[f] >>= 1;
-    if (s.pl[5] >= p.cap_indirect)
-        for (uint32_t f = 5; f < 10; f++) s.pl[f] >>= 1;
-    lb_firefly(s.pl, 0, d, p.firefly_k, p.firefly_floor);
-    lb_firefly(s.pl, 5, e, p.firefly_k, p.firefly_floor);
-    lb_add(s.pl, d, e);
+    if (s.c[0] >= cap)   // shade.comp:188-194
+        for (uint32_t f = 0; f < 5; f++) s.c[f] >>= 1;
+    lb_firefly(s.c, 0, v, p.firefly_k, p.firefly_floor);
+    s.c[0] += 1u; s.c[1] += v.x; s.c[2] += v.y; s.c[3] += v.z; s.c[4] += v.w;
     s.tstamp = fp_time(p);
 }
 
@@ -2329,15 +2330,16 @@ __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_dep_replay(FrameParams p,
     if (lane == 0u) dl.cnt[h] = 0u;   // the next frame's count starts from zero
     uint32_t *bk = bk_all[wv];
     const uint32_t *row = dl.slots + (size_t)h * dl.cap;
-    const bool sl = lane < LBUFFER_SLOTS;
+    static_assert(LBUFFER_SLOTS == 32 && LB_PAYW == 10, "one slot per lane pair, five payload words a channel");
+    const bool sl = lane < LBUFFER_SLOTS;   // the lanes that hold the direct channels
+    const uint32_t slot = lane & 31u, ch = lane >> 5;
     uint32_t *gb = lbuf + (size_t)h * LB_WORDS;
-    LSlot st{NONE, NONE, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
-    if (sl) {   // issued first: the loads overlap the records' gather and the sort
-        st.owner = gb[lane];
-        st.tstamp = gb[LB_TIME + lane];
+    LChan st;   // issued first: the loads overlap the records' gather and the sort
+    st.owner = gb[slot];
+    st.tstamp = gb[LB_TIME + slot];
 #pragma unroll
-        for (uint32_t f = 0; f < LB_PAYW; f++) st.pl[f] = gb[LB_PAY + lane * LB_PAYW + f];
-    }
+    for (uint32_t f = 0; f < 5u; f++) st.c[f] = gb[LB_PAY + slot * LB_PAYW + 5u * ch + f];
+    const uint32_t cap = ch ? p.cap_indirect : p.cap_direct;
     // records: row[i] in place, then this bucket's overflow entries (low word bit 31)
     SortedKeys sk{n, ~0ull, nullptr, false};
     // this lane's record of the window [w0, w0 + 64)
@@ -2423,21 +2425,22 @@ __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_dep_replay(FrameParams p,
                                        (uint32_t)__builtin_amdgcn_readlane((int)r_e.y, (int)j),
                                        (uint32_t)__builtin_amdgcn_readlane((int)r_e.z, (int)j),
                                        (uint32_t)__builtin_amdgcn_readlane((int)r_e.w, (int)j));
+            const uint4 v = ch ? e : d;
             const uint64_t hit = __ballot(sl && (st.owner == vid || st.owner == 0u));
             if (hit) {
-                if (lane == (uint32_t)__ffsll((unsigned long long)hit) - 1u) lslot_deposit(p, st, vid, d, e);
+                if (slot == (uint32_t)__ffsll((unsigned long long)hit) - 1u) lchan_deposit(p, st, vid, v, cap);
             } else {   // every slot holds another voxel: reset the least recently stamped one
                 const uint32_t best = wave_min_u32(sl ? st.tstamp : NONE);
                 const uint64_t at = __ballot(sl && st.tstamp == best);
-                if (best != NONE && lane == (uint32_t)__ffsll((unsigned long long)at) - 1u)
-                    lslot_reset(st, vid, d, e, fp_time(p));
+                if (best != NONE && slot == (uint32_t)__ffsll((unsigned long long)at) - 1u)
+                    lchan_reset(st, vid, v, fp_time(p));
             }
         } else {   // several lanes of one vector here: lockstep, on an LDS copy
             if (sl) {
-                bk[lane] = st.owner;
-                bk[LB_TIME + lane] = st.tstamp;
-                for (uint32_t f = 0; f < LB_PAYW; f++) bk[LB_PAY + lane * LB_PAYW + f] = st.pl[f];
+                bk[slot] = st.owner;
+                bk[LB_TIME + slot] = st.tstamp;
             }
+            for (uint32_t f = 0; f < 5u; f++) bk[LB_PAY + slot * LB_PAYW + 5u * ch + f] = st.c[f];
             if (lane >= j && lane < j + m) {   // the group's records, lane j + l into place l
                 const uint32_t l = lane - j;
                 bk[LK_VID + l] = r_vid;
@@ -2453,21 +2456,19 @@ __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_dep_replay(FrameParams p,
             wave_lds_sync();
             if (lane == 0u) deposit_lockstep(p, bk, m);
             wave_lds_sync();
-            if (sl) {
-                st.owner = bk[lane];
-                st.tstamp = bk[LB_TIME + lane];
-                for (uint32_t f = 0; f < LB_PAYW; f++) st.pl[f] = bk[LB_PAY + lane * LB_PAYW + f];
-            }
+            st.owner = bk[slot];
+            st.tstamp = bk[LB_TIME + slot];
+            for (uint32_t f = 0; f < 5u; f++) st.c[f] = bk[LB_PAY + slot * LB_PAYW + 5u * ch + f];
             wave_lds_sync();
         }
         k += m;
     }
     if (sl) {
-        gb[lane] = st.owner;
-        gb[LB_TIME + lane] = st.tstamp;
-#pragma unroll
-        for (uint32_t f = 0; f < LB_PAYW; f++) gb[LB_PAY + lane * LB_PAYW + f] = st.pl[f];
+        gb[slot] = st.owner;
+        gb[LB_TIME + slot] = st.tstamp;
     }
+#pragma unroll
+    for (uint32_t f = 0; f < 5u; f++) gb[LB_PAY + slot * LB_PAYW + 5u * ch + f] = st.c[f];
 }
 
 __global__ void k_dep_done(uint32_t *__restrict__ counters, uint32_t *__restrict__ over_n)

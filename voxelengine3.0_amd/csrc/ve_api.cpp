@@ -666,9 +666,11 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &p)
     } else if ((rc = timed_pass(c, s, VE_PASS_NORMAL, p, 0, ev))) {
         return rc;
     }
+    // the threshold (threshold.comp) runs on the emit kernel's first wave
+    // when the frame folds its small passes
     const ve_pass_id head[] = {VE_PASS_SCHEDULE, VE_PASS_RANK, VE_PASS_THRESHOLD};
     for (ve_pass_id id : head)
-        if ((rc = timed_pass(c, s, id, p, 0, ev))) return rc;
+        if (!(id == VE_PASS_THRESHOLD && p.fold_small_passes) && (rc = timed_pass(c, s, id, p, 0, ev))) return rc;
     if (c->overlap) {
         // the emit kernel's own dispatch marks its completion: the timing
         // event when the pass is timed, ev_emit otherwise
@@ -815,6 +817,7 @@ int ve_frame(ve_ctx *c, ve_scene *s, const ve_frame_config *fc, const ve_frame_i
     FrameParams p;
     fill_params(c, s, fc, in, p);
     p.count_primary_in_reset = 1;
+    p.fold_small_passes = 1;
     p.reuse_primary_hits = c->node_stats ? 0 : 1;   // node counting traces every ray as the reference does
     if (c->graph && !c->overlap && !c->timing) return frame_graph(c, s, p);
     rc = frame_passes(c, s, p);
@@ -1125,6 +1128,7 @@ int frame_begin(ve_ctx *c, ve_scene *s, const ve_frame_config *fc, const ve_fram
     HIP_OK(c, hipSetDevice(c->device));
     fill_params(c, s, fc, in, p);
     p.count_primary_in_reset = 1;
+    p.fold_small_passes = 1;
     p.reuse_primary_hits = 1;
     return VE_OK;
 }

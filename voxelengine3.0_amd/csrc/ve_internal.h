@@ -171,6 +171,11 @@ struct FrameParams {
     // (the primary pass may run ahead on another stream); 0 for passes run
     // one by one, where the primary pass counts them itself
     int32_t count_primary_in_reset;
+    // 1 inside a whole frame: two one-wave passes ride on the next kernel of
+    // the stream instead of a launch of their own -- the threshold on the emit
+    // kernel's first wave, the deposit records' clear on the resolve's first
+    // thread (passes run one by one keep their own launches)
+    int32_t fold_small_passes;
     // 1 inside a whole frame (without node counting): the primary pass keeps
     // each hit's lattice position and the emit pass fills in the hit record
     // of a bounce-0 ray whose camera ray is the pixel's primary ray

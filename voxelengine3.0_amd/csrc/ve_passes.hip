@@ -1118,12 +1118,12 @@ __global__ __launch_bounds__(256) void k_rank_write(FrameParams p, const uint4 *
 // topMidCount); that bin is the largest b whose suffix count (bins b..255)
 // reaches it, i.e. (number of bins with suffix >= count) - 1, as suffix
 // counts fall with b. One wave: four bins per lane, a suffix scan across lanes.
-static_assert(HISTOGRAM_BINS == 256, "k_threshold holds four bins per lane");
-__global__ __launch_bounds__(64) void k_threshold(FrameParams p, uint32_t *__restrict__ rank_stats,
-                                                  const uint32_t *__restrict__ counters)
+static_assert(HISTOGRAM_BINS == 256, "threshold_wave holds four bins per lane");
+__device__ void threshold_wave(const FrameParams &p, uint32_t *__restrict__ rank_stats,
+                               const uint32_t *__restrict__ counters)
 {
     if (counters[VE_CNT_CANDIDATES] == 0u) return;
-    const uint32_t lane = threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
     const uint4 h = reinterpret_cast<const uint4 *>(rank_stats + 8)[lane];
     const uint32_t mine = h.x + h.y + h.z + h.w;
     uint32_t suf = mine;   // bins 4*lane .. 255
@@ -1157,6 +1157,12 @@ __global__ __launch_bounds__(64) void k_threshold(FrameParams p, uint32_t *__res
     }
 }
 
+__global__ __launch_bounds__(64) void k_threshold(FrameParams p, uint32_t *__restrict__ rank_stats,
+                                                  const uint32_t *__restrict__ counters)
+{
+    threshold_wave(p, rank_stats, counters);
+}
+
 // ---------------------------------------------------------------------------
 // emit.comp: one ray per shade-list entry, ray index = list index
 // ---------------------------------------------------------------------------
@@ -1187,8 +1193,12 @@ __global__ __launch_bounds__(256) void k_emit(FrameParams p, const uint2 *__rest
                                               const Material *__restrict__ mats, const uint32_t *__restrict__ em_claim,
                                               uint32_t *__restrict__ em_bits, uint32_t *__restrict__ hit_bits,
                                               unsigned long long *__restrict__ totals, uint32_t *__restrict__ ray_li,
-                                              uint32_t *__restrict__ death)
+                                              uint32_t *__restrict__ death, uint32_t *__restrict__ rank_stats)
 {
+    // the threshold pass, folded: it reads the rank pass's histogram, done
+    // before this launch, and writes thresholds only the next frame's rank
+    // pass reads
+    if (p.fold_small_passes && blockIdx.x == 0 && threadIdx.x < 64u) threshold_wave(p, rank_stats, counters);
     __shared__ uint32_t block_reused;
     if (threadIdx.x == 0) block_reused = 0u;
     __syncthreads();
@@ -2523,11 +2533,15 @@ __device__ Avg voxel_avg(const uint32_t *__restrict__ lbuf, uint32_t hw, uint32_
 __global__ __launch_bounds__(256) void k_resolve_avg(FrameParams p, const uint4 *__restrict__ gbuf,
                                                      const uint32_t *__restrict__ lbuf, float4 *__restrict__ pix_avg,
                                                      uint32_t *__restrict__ pix_cnt, uint32_t *__restrict__ counters,
-                                                     uint32_t *__restrict__ tcount)
+                                                     uint32_t *__restrict__ tcount, uint32_t *__restrict__ dep_over_n)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) {   // k_resolve appends after this pass
         counters[VE_CNT_TIMEOUTS] = 0u;
         counters[VE_CNT_TIMEOUT_TILES] = 0u;
+        if (p.fold_small_passes) {   // k_dep_done, folded: the frame's deposit records are applied
+            counters[VE_CNT_DEFERRED] = 0u;
+            dep_over_n[0] = dep_over_n[1] = 0u;
+        }
     }
     {   // and counts timed-out pixels per 16x16 tile from zero
         const uint32_t n_t = ((p.W + RECON_TILE - 1u) / RECON_TILE) * ((p.H + RECON_TILE - 1u) / RECON_TILE);
@@ -2958,7 +2972,7 @@ void launch_threshold(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 void launch_emit(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 {
     launch(k_emit, dim3(cdiv(p.shade_budget, 256)), dim3(256), 0, s, p, b.shade_list, b.rays, b.counters, b.gbuf, b.ghit,
-           b.materials, b.emissive_claim, b.em_bits, b.hit_bits, b.ray_totals, b.ray_li[0], b.dep_death);
+           b.materials, b.emissive_claim, b.em_bits, b.hit_bits, b.ray_totals, b.ray_li[0], b.dep_death, b.rank_stats);
 }
 
 // Bounce iteration `iter` of the trace/shade loop (renderer.cpp:515-602); the
@@ -3024,7 +3038,7 @@ int launch_deposits(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint3
     launch(k_dep_wpos, dim3(chunks * WPOS_BLOCKS), dim3(1024), 0, s, b.counters, b.dep_death, b.dep_wpos, iters);
     launch(k_dep_replay, dim3(cdiv(p.hash_width, REPLAY_WAVES)), dim3(64 * REPLAY_WAVES), 0, s, p, b.lbuf, dl,
            b.deferred, b.dep_wpos, b.counters);
-    launch(k_dep_done, dim3(1), dim3(1), 0, s, b.counters, b.dep_over_n);
+    if (!p.fold_small_passes) launch(k_dep_done, dim3(1), dim3(1), 0, s, b.counters, b.dep_over_n);   // else the resolve's
     return VE_OK;
 }
 
@@ -3049,7 +3063,7 @@ void launch_display(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 {
     launch(k_resolve_avg, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.pix_avg, b.pix_cnt,
-                       b.counters, b.timeout_tiles);
+                       b.counters, b.timeout_tiles, b.dep_over_n);
     launch(k_resolve, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.nbuf, b.materials,
                        b.pix_avg, b.pix_cnt, b.output, b.timeouts, b.counters, b.timeout_tiles, b.timeout_list);
     // timed-out pixels one wave each; in timeout-heavy frames, the tiles

@@ -149,7 +149,8 @@ def cpu_baseline(words, depth, ubos, fc_kw, budget_s=20.0):
 # offered loads (frames/s) of the served sweep, per mode (each row also
 # reports the load its Poisson realisation actually offered); the sweep then
 # bisects between the highest load within the p99 bound and the next one
-SERVED_LAMS = {0: [1200, 1600, 2000, 2200, 2400], 1: [600, 900, 1100, 1200, 1300, 1400]}
+SERVED_LAMS = {"placed": [50, 200, 600, 1200, 1600, 2000, 2200, 2400], 0: [50, 600, 1200, 1600, 2000, 2200],
+               1: [50, 200, 600, 900, 1100, 1200, 1300], "batched": [50, 600, 1200, 1600, 2000, 2200]}
 
 
 def divergence(ve, caves, frames=32, warmup=4):
@@ -220,34 +221,53 @@ def divergence(ve, caves, frames=32, warmup=4):
 def served_and_batched(ve, extreme, caves):
     """Served: Poisson sessions over three resident fbm octrees (depth 9/10/11),
     3 s of measured arrivals per offered load (after 0.5 s of warm-up), on the
-    concurrent server and on the plain
-    loop (one stream); batched: 64 uneven viewports over the depth-10 scene,
-    the depth-12 cave scene and fbm depth 9 / 11 octrees, 6 frames each, both
-    ways."""
+    concurrent server with each frame placed on the least-loaded of 16 shared
+    streams ("placed"), the concurrent server with one stream per session (0),
+    the plain loop (one stream, 1) and the batching server (frames arriving
+    while a batch's primary launch runs join the next batch); batched: 64
+    uneven viewports over the depth-10 scene, the depth-12 cave scene and fbm
+    depth 9 / 11 octrees, 6 frames each, the same four ways."""
     from voxelengine_amd import serving
 
     def log(m):
         print(m, file=sys.stderr, flush=True)
 
-    sweep = serving.sweep(ve, SERVED_LAMS, 3.0, 0, (0, 1), log=log)
-    conc, plain = sweep[0], sweep[1]
-    best_c, best_p = serving.max_sustained(conc), serving.max_sustained(plain)
+    modes = ("placed", 0, 1, "batched")
+    sweep = serving.sweep(ve, SERVED_LAMS, 3.0, 0, modes, log=log)
+    best = {q: serving.max_sustained(sweep[q]) for q in modes}
+    plain = best[1]
+
+    def ratio(q):
+        return round(best[q] / plain, 3) if plain else None
+
     served = {
         "workload": "poisson sessions: fbm shells depth 9/10/11 (seeds 11/12/13), res U[256..2560]x[256..1440], "
                     "spp 1-8, bounces 1-4, 8-256 frames at 30 fps each; 0.5 s warm-up + 3 s measured per load",
-        "concurrent": conc,
-        "plain": plain,
-        "max_frames_per_s_at_p99_33ms": {"concurrent": best_c, "plain": best_p,
-                                         "ratio": round(best_c / best_p, 3) if best_p else None},
+        "concurrent": sweep["placed"],
+        "per_session_streams": sweep[0],
+        "plain": sweep[1],
+        "batched": sweep["batched"],
+        "max_frames_per_s_at_p99_33ms": {"concurrent": best["placed"], "per_session_streams": best[0], "plain": plain,
+                                         "batched": best["batched"], "ratio": ratio("placed"),
+                                         "ratio_per_session_streams": ratio(0), "ratio_batched": ratio("batched")},
     }
     d9, d11 = serving.served_scenes(((9, 7), (11, 8)))
-    b = serving.batched(ve, [extreme, caves, d9, d11], 64, 6, 0, (0, 1), log=log)
+    b = serving.batched(ve, [extreme, caves, d9, d11], 64, 6, 0, modes, log=log)
+    pf = b[1]["frames_per_s"] if b[1].get("frames") else None
+
+    def bratio(q):
+        return round(b[q]["frames_per_s"] / pf, 3) if pf else None
+
     batched = {
         "workload": "64 viewports on {depth-10 extreme, depth-12 caves, fbm depth 9 seed 7, fbm depth 11 seed 8}, res "
                     "U[128..1920]x[128..1080], spp {1,2,4}, bounces {1,2,4}, 6 frames per viewport",
-        "concurrent": b[0],
+        "concurrent": b["placed"],
+        "per_session_streams": b[0],
         "plain": b[1],
-        "ratio_frames_per_s": round(b[0]["frames_per_s"] / b[1]["frames_per_s"], 3) if b[1].get("frames") else None,
+        "batched": b["batched"],
+        "ratio_frames_per_s": bratio("placed"),
+        "ratio_per_session_streams": bratio(0),
+        "ratio_batched_frames_per_s": bratio("batched"),
     }
     return served, batched
 

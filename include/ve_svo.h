@@ -526,6 +526,14 @@ int  ve_server_get_stats(ve_server *srv, ve_server_stats *out);
  * and future (jump_level is the server's scenes' and is ignored here). */
 int  ve_server_set_graph(ve_server *srv, int enable);
 int  ve_server_set_tuning(ve_server *srv, const ve_tuning *t);
+/* No counterpart in the reference: placement of ve_server_submit's frames on a
+ * server with N >= 2 shared streams.  0 (default): a session keeps the stream
+ * it was given.  1: every frame joins the shared stream whose uncollected
+ * frames hold the least estimated work (pixels x (2 + samples x bounces)); a
+ * session's frames still run in order (a frame placed elsewhere waits for its
+ * session's previous one on the device).  Chosen before the first session
+ * opens (VE_ESTATE after, VE_EINVAL with fewer than two streams). */
+int  ve_server_set_placement(ve_server *srv, int by_load);
 int  ve_server_set_batch(ve_server *srv, const ve_batch_config *cfg);
 int  ve_server_submit_batch(ve_server *srv, uint32_t n, const uint32_t *sessions, const ve_frame_config *fcs,
                             const ve_frame_inputs *ins, const float *ubos, const uint64_t *tags,

@@ -54,10 +54,12 @@ def _alone(ve, scenes, small_scene, v, frames, fc):
         r.close()
 
 
-def _served(ve, scenes, small_scene, views, frames, fc, **server_kw):
+def _served(ve, scenes, small_scene, views, frames, fc, place=False, **server_kw):
     """All views as concurrent sessions, frames submitted round-robin; the
     output of each session is read after each of its frames completes."""
     srv = ve.Server(**server_kw)
+    if place:
+        srv.set_placement(True)
     try:
         scene = srv.add_scene(small_scene["words"], small_scene["depth"], small_scene["info"].size)
         sids = [srv.open(scene, v[0], v[1], v[2], scenes.palette(1), SKY) for v in views]
@@ -108,6 +110,52 @@ def test_sessions_alone_equal_in_mix(ve, scenes, small_scene, queues):
         for a, b in zip(ivs, ivs[1:]):
             assert b[0] >= a[1] - 1e-3
     assert st_after["admitted"] == 0 and st_after["pooled"] == len(views)
+
+
+def test_placement_by_load_keeps_each_session_in_order(ve, scenes, small_scene):
+    """Frames placed on the least-loaded of two shared streams: a session's
+    frames move between streams (each waiting for its predecessor on the
+    device), yet every frame equals the frame rendered alone -- per round,
+    and with all rounds submitted at once (the outputs then compared after
+    the last frame, the ray counts frame by frame)."""
+    print("case server placement", flush=True)
+    fc = ve.FrameConfig(**DET_KNOBS)
+    views = _views(scenes)
+    frames = 3
+    ref = [_alone(ve, scenes, small_scene, v, frames, fc) for v in views]
+    sids, outs, rays, done, st, st_after = _served(ve, scenes, small_scene, views, frames, fc, place=True, queues=2)
+    for k, s in enumerate(sids):
+        for f in range(frames):
+            assert np.array_equal(outs[s][f], ref[k][0][f]), (k, f)
+        assert rays[s] == ref[k][1]
+    srv = ve.Server(queues=2)
+    srv.set_placement(True)
+    try:
+        scene = srv.add_scene(small_scene["words"], small_scene["depth"], small_scene["info"].size)
+        sids = [srv.open(scene, v[0], v[1], v[2], scenes.palette(1), SKY) for v in views]
+        ubos = [_ubos(ve, scenes, v, frames) for v in views]
+        for f in range(frames):
+            for k, (s, v) in enumerate(zip(sids, views)):
+                srv.submit(s, fc, ve.FrameInputs(v[0], v[1], f, frame_seed(f), f + 1), ubos[k][f], tag=f)
+        done = []
+        while len(done) < frames * len(sids):
+            done += srv.poll(wait=True)
+        for k, (s, v) in enumerate(zip(sids, views)):
+            assert np.array_equal(srv.read_output(s, v[0], v[1]), ref[k][0][-1]), k
+            got = sorted((d["frame"], d["rays"]) for d in done if d["session"] == s)
+            assert [r for _, r in got] == ref[k][1], k
+        with pytest.raises(ve.VeError) as e:
+            srv.set_placement(False)   # sessions are open
+        assert e.value.code == ve.VE_ESTATE
+    finally:
+        srv.close()
+    one = ve.Server(queues=0)
+    try:
+        with pytest.raises(ve.VeError) as e:
+            one.set_placement(True)
+        assert e.value.code == ve.VE_EINVAL
+    finally:
+        one.close()
 
 
 def test_admission_waits_for_budget(ve, scenes, small_scene):

@@ -168,6 +168,7 @@ def lib():
         "ve_graph_stats": (i32, [p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "ve_server_set_graph": (i32, [p, i32]),
         "ve_server_set_tuning": (i32, [p, p]),
+        "ve_server_set_placement": (i32, [p, i32]),
         "ve_server_set_batch": (i32, [p, C.POINTER(BatchConfig)]),
         "ve_server_submit_batch": (i32, [p, u32, p, p, p, p, p, p]),
         "ve_get_tuning": (i32, [p, p]),
@@ -670,6 +671,10 @@ class Server:
 
     def set_graph(self, on=True):
         self._ck(lib().ve_server_set_graph(self.h, 1 if on else 0))
+
+    def set_placement(self, by_load=True):
+        """ve_server_set_placement: each frame on the least-loaded shared stream."""
+        self._ck(lib().ve_server_set_placement(self.h, 1 if by_load else 0))
 
     def set_tuning(self, tile_chunk=16, bounce_blocks=0):
         """ve_server_set_tuning: Renderer.set_tuning for every session's context."""

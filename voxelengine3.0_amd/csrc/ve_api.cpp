@@ -915,6 +915,17 @@ int ve_enable_timing(ve_ctx *c, int enable)
     return VE_OK;
 }
 
+// Elapsed time of a pass's event pair.  A pass that launched nothing in the
+// frame (the threshold of a whole frame, which rides on the emit kernel)
+// left its events unrecorded: the query fails, and the error it leaves with
+// the runtime is cleared so that the next launch check does not report it.
+static bool pair_ms(const hipEvent_t *pair, float &ms)
+{
+    if (hipEventElapsedTime(&ms, pair[0], pair[1]) == hipSuccess) return true;
+    (void)hipGetLastError();
+    return false;
+}
+
 int ve_timings_iter(ve_ctx *c, uint32_t iter, float ms_out[VE_PASS_COUNT])
 {
     if (!c) return ve_global_error(VE_EINVAL, "null context");
@@ -931,7 +942,7 @@ int ve_timings_iter(ve_ctx *c, uint32_t iter, float ms_out[VE_PASS_COUNT])
             for (size_t f = 0; f < c->frames_recorded; f++) {
                 const hipEvent_t *pair = &c->frame_events[f][2 * (i * MAX_ITERS + iter)];
                 float ms = 0.0f;
-                if (hipEventElapsedTime(&ms, pair[0], pair[1]) == hipSuccess) sum += ms;
+                if (pair_ms(pair, ms)) sum += ms;
             }
         ms_out[i] = c->frames_recorded ? (float)(sum / (double)c->frames_recorded) : 0.0f;
     }
@@ -954,7 +965,7 @@ int ve_timings(ve_ctx *c, float ms_out[VE_PASS_COUNT])
             for (uint32_t it = 0; it < n_it; it++) {
                 const hipEvent_t *pair = &c->frame_events[f][2 * (i * MAX_ITERS + it)];
                 float ms = 0.0f;
-                if (hipEventElapsedTime(&ms, pair[0], pair[1]) == hipSuccess) sum[i] += ms;
+                if (pair_ms(pair, ms)) sum[i] += ms;
             }
         }
     for (int i = 0; i < VE_PASS_COUNT; i++)

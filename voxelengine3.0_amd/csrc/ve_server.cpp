@@ -38,6 +38,8 @@ struct Frame {
     hipEvent_t start = nullptr, end = nullptr;
     RaySlot *slot = nullptr;
     uint64_t seq = 0;   // launch order across the server
+    int queue = -1;     // shared stream it was placed on (placement by load)
+    uint64_t cost = 0;  // its estimated work there
 };
 
 enum class State { Waiting, Active, Closed };
@@ -53,6 +55,7 @@ struct Session {
     std::deque<Frame> pending;            // submitted before admission
     std::deque<Frame> in_flight;          // launched, not yet collected
     bool queued = false;                  // a frame waits in the server's next batch
+    hipEvent_t reset_ev = nullptr;        // placement by load: after the context's reset on its stream
     unsigned long long last_totals[3] = {0, 0, 0};
 };
 
@@ -69,6 +72,8 @@ struct ve_server {
     uint64_t budget = 0;
     uint64_t used = 0;
     std::vector<hipStream_t> queues;       // shared streams (empty: one per session)
+    bool place = false;                    // each frame on the shared stream with the least work queued
+    std::vector<uint64_t> qload;           // per shared stream: estimated work of its uncollected frames
     uint32_t next_queue = 0;
     std::string err;
     Clock::time_point t0;
@@ -150,6 +155,25 @@ int launch(ve_server *s, Session &ss, Frame &f)
     ve_ctx *c = ss.ctx;
     int rc;
     if ((rc = get_event(s, &f.start)) || (rc = get_event(s, &f.end)) || (rc = get_slot(s, &f.slot))) return rc;
+    if (s->place && s->queues.size() > 1) {
+        // join the shortest queue: the shared stream whose uncollected frames
+        // hold the least estimated work (pixels x (2 + samples x bounces));
+        // the session's previous work, if still running elsewhere, is waited
+        // for on the device
+        size_t best = 0;
+        for (size_t i = 1; i < s->queues.size(); i++)
+            if (s->qload[i] < s->qload[best]) best = i;
+        hipStream_t q = s->queues[best];
+        if (c->stream != q) {
+            hipEvent_t dep = !ss.in_flight.empty() ? ss.in_flight.back().end : ss.reset_ev;
+            if (dep && hipEventQuery(dep) == hipErrorNotReady) SRV_HIP(s, hipStreamWaitEvent(q, dep, 0));
+            c->stream = q;
+        }
+        const uint32_t mb = ss.desc.max_bounces ? ss.desc.max_bounces : 8u;
+        f.queue = (int)best;
+        f.cost = (uint64_t)f.in.width * f.in.height * (2u + (uint64_t)std::max(f.fc.min_samples, 1) * mb);
+        s->qload[best] += f.cost;
+    }
     std::memcpy(c->cam, f.ubo, sizeof c->cam);
     SRV_HIP(s, hipEventRecord(f.start, c->stream));
     rc = ve_frame(c, s->scenes[ss.desc.scene], &f.fc, &f.in);
@@ -177,6 +201,19 @@ void destroy_pooled(ve_server *s, size_t i)
     s->pool.erase(s->pool.begin() + (long)i);
 }
 
+// Placement by load: an event after the context's set-up on its stream,
+// which the session's first frame waits for wherever it is placed.
+int mark_reset(ve_server *s, Session &ss)
+{
+    if (!s->place) return VE_OK;
+    if (!ss.reset_ev) {
+        int rc = get_event(s, &ss.reset_ev);
+        if (rc) return rc;
+    }
+    SRV_HIP(s, hipEventRecord(ss.reset_ev, ss.ctx->stream));
+    return VE_OK;
+}
+
 // Give the session a context: a pooled one that fits (cleared), else a new
 // one if the budget allows after dropping idle pooled contexts.  Returns
 // VE_EBUDGET when it has to wait.
@@ -190,7 +227,7 @@ int acquire(ve_server *s, Session &ss)
         if (s->tuned) ss.ctx->tuning = s->tuning;
         s->pool.erase(s->pool.begin() + (long)i);
         if (ve::ctx_reset(ss.ctx) != VE_OK) return srv_error(s, VE_EDEVICE, "context reset: %s", ss.ctx->err.c_str());
-        return VE_OK;
+        return mark_reset(s, ss);
     }
     const uint64_t need = ve::ctx_device_bytes(ss.desc.width, ss.desc.height, hw);
     while (s->used + need > s->budget && !s->pool.empty()) {
@@ -209,7 +246,7 @@ int acquire(ve_server *s, Session &ss)
     s->used += ve::ctx_bytes(ss.ctx);
     ss.ctx->graph = s->graph;
     if (s->tuned) ss.ctx->tuning = s->tuning;
-    return VE_OK;
+    return mark_reset(s, ss);
 }
 
 // Admit waiting sessions in arrival order while slots and memory allow, and
@@ -247,6 +284,10 @@ void retire(ve_server *s, Session &ss)
         return;
     s->pool.push_back({ss.ctx, s->use_tick++});
     ss.ctx = nullptr;
+    if (ss.reset_ev) {
+        s->free_events.push_back(ss.reset_ev);
+        ss.reset_ev = nullptr;
+    }
     ss.state = State::Closed;
     s->admitted--;
 }
@@ -270,6 +311,7 @@ void finish(ve_server *s, Session &ss, Frame &f)
     }
     d.rays = r;
     s->done.push_back(d);
+    if (f.queue >= 0) s->qload[(size_t)f.queue] -= f.cost;
     s->free_events.push_back(f.start);
     s->free_events.push_back(f.end);
     s->free_slots.push_back(f.slot);
@@ -367,6 +409,7 @@ int ve_server_create(ve_server **out, const ve_server_config *cfg)
         hipStream_t q = nullptr;
         e = hipStreamCreateWithFlags(&q, hipStreamNonBlocking);   // default priority
         if (e == hipSuccess) s->queues.push_back(q);
+        s->qload.push_back(0);
     }
     // device-clock origin: a marker on an idle queue, observed from the host
     hipStream_t q0 = nullptr;
@@ -396,6 +439,7 @@ void ve_server_destroy(ve_server *s)
             (void)hipEventDestroy(f.start);
             (void)hipEventDestroy(f.end);
         }
+        if (kv.second.reset_ev) (void)hipEventDestroy(kv.second.reset_ev);
     }
     for (auto &p : s->pool) ve_destroy(p.ctx);
     if (s->batch) ve_batch_destroy(s->batch);
@@ -493,6 +537,15 @@ int ve_server_set_graph(ve_server *s, int enable)
     for (auto &kv : s->sessions)
         if (kv.second.ctx) kv.second.ctx->graph = s->graph;
     for (auto &p : s->pool) p.ctx->graph = s->graph;
+    return VE_OK;
+}
+
+int ve_server_set_placement(ve_server *s, int by_load)
+{
+    if (!s) return ve_global_error(VE_EINVAL, "null server");
+    if (by_load && s->queues.size() < 2) return srv_error(s, VE_EINVAL, "placement by load needs two or more shared streams");
+    if (!s->sessions.empty()) return srv_error(s, VE_ESTATE, "placement is chosen before the first session opens");
+    s->place = by_load != 0;
     return VE_OK;
 }
 

@@ -1,8 +1,9 @@
 """Served workload: Poisson session arrivals driven against the session
 server (ve_server_*), reporting frame latency percentiles, frames/s and
 Mrays/s.  The same request schedule runs on the concurrent server (one
-stream per session) and on the plain loop (every frame on one stream, in
-arrival order), so the two are compared on identical work.
+stream per session, or frames placed on shared streams by load) and on the
+plain loop (every frame on one stream, in arrival order), so they are
+compared on identical work.
 
 Workload (the served configuration): three resident fbm terrain-shell
 octrees (depth 9, 10, 11; seeds 11, 12, 13); each session picks one, a
@@ -13,8 +14,10 @@ arrive as a Poisson process whose rate makes the offered load `lam`
 frames/s.
 
 Modes: 0 = the concurrent server (one stream per session, each frame
-submitted as it falls due); 1 = the plain loop (every frame on one stream, in
-arrival order); "batched" = the concurrent server with continuous batching:
+submitted as it falls due); "placed" = the concurrent server over 16 shared
+streams, each frame joining the stream whose queued frames hold the least
+estimated work (ve_server_set_placement); 1 = the plain loop (every frame on
+one stream, in arrival order); "batched" = the concurrent server with continuous batching:
 frames go in through ve_server_submit_batch and wait while the previous
 batch's primary launch runs, so the frames falling due meanwhile share one
 persistent primary launch over a global tile queue; every other pass runs
@@ -111,14 +114,21 @@ def batch_tuning():
         return dict(DEFAULT_BATCH)
 
 
+PLACED_STREAMS = 16   # one per hardware queue (bench.py sets GPU_MAX_HW_QUEUES=16)
+
+
 def make_server(ve, mode, max_sessions=64):
-    """A session server for a mode (0, 1 or "batched").  The batching server
+    """A session server for a mode (0, 1, "placed" or "batched").  The batching server
     launches each session's other passes directly: replaying them from frame
     graphs measured slower here (2498 against 2561 frames/s on the batched
     configuration, and more host time per frame in the served sweep)."""
     if mode == "batched":
         srv = ve.Server(queues=0, max_sessions=max_sessions)
         srv.set_batch(**batch_tuning())
+        return srv
+    if mode == "placed":
+        srv = ve.Server(queues=PLACED_STREAMS, max_sessions=max_sessions)
+        srv.set_placement(True)
         return srv
     return ve.Server(queues=int(mode), max_sessions=max_sessions)
 
@@ -327,8 +337,9 @@ def run_batched(ve, srv, scene_ids, scene_depths, views, frames, materials, sky=
 
 
 def batched(ve, res_scenes, count=64, frames=6, seed=0, queues_list=(0, 1, "batched"), log=None):
-    """The batched configuration on the concurrent server, the plain loop and
-    the batched server (one persistent primary launch per tick, graphs)."""
+    """The batched configuration on the servers of `queues_list` (make_server's
+    modes): the batching one runs a tick's primary passes in one persistent
+    launch."""
     mats = S.palette(1)
     views = make_viewports(count, len(res_scenes), seed)
     out = {}

@@ -149,7 +149,7 @@ def cpu_baseline(words, depth, ubos, fc_kw, budget_s=20.0):
 # offered loads (frames/s) of the served sweep, per mode (each row also
 # reports the load its Poisson realisation actually offered); the sweep then
 # bisects between the highest load within the p99 bound and the next one
-SERVED_LAMS = {"placed": [50, 200, 600, 1200, 1600, 2000, 2200, 2400], 0: [50, 600, 1200, 1600, 2000, 2200],
+SERVED_LAMS = {"placed": [50, 200, 600, 1200, 1600, 2000, 2200, 2300, 2400], 0: [50, 600, 1200, 1600, 2000, 2200],
                1: [50, 200, 600, 900, 1100, 1200, 1300], "batched": [50, 600, 1200, 1600, 2000, 2200]}
 
 

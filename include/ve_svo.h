@@ -529,7 +529,9 @@ int  ve_server_set_tuning(ve_server *srv, const ve_tuning *t);
 /* No counterpart in the reference: placement of ve_server_submit's frames on a
  * server with N >= 2 shared streams.  0 (default): a session keeps the stream
  * it was given.  1: every frame joins the shared stream whose uncollected
- * frames hold the least estimated work (pixels x (2 + samples x bounces)); a
+ * frames hold the least estimated device time (a moving average of its
+ * session's frames, else pixels x (2 + samples x bounces) at the server's
+ * recent time per unit); a
  * session's frames still run in order (a frame placed elsewhere waits for its
  * session's previous one on the device).  Chosen before the first session
  * opens (VE_ESTATE after, VE_EINVAL with fewer than two streams). */

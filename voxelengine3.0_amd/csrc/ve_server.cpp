@@ -39,7 +39,8 @@ struct Frame {
     RaySlot *slot = nullptr;
     uint64_t seq = 0;   // launch order across the server
     int queue = -1;     // shared stream it was placed on (placement by load)
-    uint64_t cost = 0;  // its estimated work there
+    double cost = 0;    // its estimated device time there, ms
+    double units = 0;   // its size in work units (pixels x (2 + samples x bounces))
 };
 
 enum class State { Waiting, Active, Closed };
@@ -56,6 +57,7 @@ struct Session {
     std::deque<Frame> in_flight;          // launched, not yet collected
     bool queued = false;                  // a frame waits in the server's next batch
     hipEvent_t reset_ev = nullptr;        // placement by load: after the context's reset on its stream
+    double est_ms = 0;                    // placement by load: moving average of its frames' device time
     unsigned long long last_totals[3] = {0, 0, 0};
 };
 
@@ -73,7 +75,8 @@ struct ve_server {
     uint64_t used = 0;
     std::vector<hipStream_t> queues;       // shared streams (empty: one per session)
     bool place = false;                    // each frame on the shared stream with the least work queued
-    std::vector<uint64_t> qload;           // per shared stream: estimated work of its uncollected frames
+    std::vector<double> qload;             // per shared stream: estimated device time of its uncollected frames, ms
+    double ms_per_unit = 0;                // moving average over collected frames: device time per work unit
     uint32_t next_queue = 0;
     std::string err;
     Clock::time_point t0;
@@ -157,7 +160,7 @@ int launch(ve_server *s, Session &ss, Frame &f)
     if ((rc = get_event(s, &f.start)) || (rc = get_event(s, &f.end)) || (rc = get_slot(s, &f.slot))) return rc;
     if (s->place && s->queues.size() > 1) {
         // join the shortest queue: the shared stream whose uncollected frames
-        // hold the least estimated work (pixels x (2 + samples x bounces));
+        // hold the least estimated device time;
         // the session's previous work, if still running elsewhere, is waited
         // for on the device
         size_t best = 0;
@@ -169,9 +172,12 @@ int launch(ve_server *s, Session &ss, Frame &f)
             if (dep && hipEventQuery(dep) == hipErrorNotReady) SRV_HIP(s, hipStreamWaitEvent(q, dep, 0));
             c->stream = q;
         }
+        // a frame's device time: its session's recent frames', else its size
+        // in work units at the server's recent time per unit
         const uint32_t mb = ss.desc.max_bounces ? ss.desc.max_bounces : 8u;
         f.queue = (int)best;
-        f.cost = (uint64_t)f.in.width * f.in.height * (2u + (uint64_t)std::max(f.fc.min_samples, 1) * mb);
+        f.units = (double)f.in.width * f.in.height * (2.0 + (double)std::max(f.fc.min_samples, 1) * mb);
+        f.cost = ss.est_ms > 0 ? ss.est_ms : f.units * (s->ms_per_unit > 0 ? s->ms_per_unit : 1e-6);
         s->qload[best] += f.cost;
     }
     std::memcpy(c->cam, f.ubo, sizeof c->cam);
@@ -311,7 +317,13 @@ void finish(ve_server *s, Session &ss, Frame &f)
     }
     d.rays = r;
     s->done.push_back(d);
-    if (f.queue >= 0) s->qload[(size_t)f.queue] -= f.cost;
+    if (f.queue >= 0) {
+        s->qload[(size_t)f.queue] = std::max(0.0, s->qload[(size_t)f.queue] - f.cost);
+        const double el = std::max(0.0, (double)(b - a));
+        ss.est_ms = ss.est_ms > 0 ? 0.75 * ss.est_ms + 0.25 * el : el;
+        if (f.units > 0)
+            s->ms_per_unit = s->ms_per_unit > 0 ? 0.95 * s->ms_per_unit + 0.05 * (el / f.units) : el / f.units;
+    }
     s->free_events.push_back(f.start);
     s->free_events.push_back(f.end);
     s->free_slots.push_back(f.slot);
@@ -409,7 +421,7 @@ int ve_server_create(ve_server **out, const ve_server_config *cfg)
         hipStream_t q = nullptr;
         e = hipStreamCreateWithFlags(&q, hipStreamNonBlocking);   // default priority
         if (e == hipSuccess) s->queues.push_back(q);
-        s->qload.push_back(0);
+        s->qload.push_back(0.0);
     }
     // device-clock origin: a marker on an idle queue, observed from the host
     hipStream_t q0 = nullptr;

@@ -1358,20 +1358,76 @@ __device__ f3 lookup_normal(const uint32_t *__restrict__ nbuf, uint32_t hw, uint
     return f3{0.0f, 0.0f, -gsign(rd.z)};
 }
 
+// Exclusive prefix of one value per thread over a block of up to 1024
+// threads (wave scans, then the wave totals from LDS); `total` gets the sum.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *wsum, uint32_t &total)
+{
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, nwv = (blockDim.x + 63u) >> 6;
+    uint32_t incl = v;
+    for (uint32_t o = 1; o < 64u; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63u) wsum[wv] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+    for (uint32_t k = 0; k < nwv; k++) {
+        const uint32_t x = wsum[k];
+        if (k < wv) before += x;
+        total += x;
+    }
+    __syncthreads();   // wsum may be rewritten by the next scan
+    return before + incl - v;
+}
+
+// The emissive record of ray r's hit (shade.comp:604-614).
+__device__ __forceinline__ EmissiveVoxel emissive_of(const Ray &r, const Material *__restrict__ mats, uint32_t vid)
+{
+    const Material &m8 = mats[r.hit_material < MATERIAL_SLOTS ? r.hit_material : MATERIAL_SLOTS - 1u];
+    const f3 ee = muls(f3{m8.color[0], m8.color[1], m8.color[2]}, m8.emissive_intensity);
+    const uint32_t hs = (uint32_t)r.pad0;
+    return EmissiveVoxel{ee.x, ee.y, ee.z, vid, (uint32_t)r.ox, (uint32_t)r.oy, (uint32_t)r.oz, hs > 1u ? hs : 1u};
+}
+
 // Emissive registry of the first shade (shade.comp:592-619) in the
-// reference's order: rays in index order register the ids k_trace flagged
-// (claim probe, append while below capacity).  One block; the bitmap is
-// scanned 1024 words at a time and cleared as it goes; a chunk with set bits
-// is walked by one thread.  Records the count before this frame's
-// registrations and the ray index of each registration, so that shade can
-// give every SIMD-8 vector of rays the registry size it would have seen.
-__global__ __launch_bounds__(1024) void k_em_register(const Ray *__restrict__ rays, const uint32_t *__restrict__ live_ptr,
+// reference's order: rays in index order register the ids k_trace flagged.
+// A ray's claim probes its row in order: its id found -> nothing; an empty
+// probe -> claimed, and registered while the registry is below capacity;
+// every probe another id's -> nothing.  One block; the bitmap is scanned 1024
+// words at a time and cleared as it goes, and the flagged rays of a chunk are
+// taken in order, EM_BATCH at a time:
+//   - a few (at most EM_SERIAL): one thread runs the probes on global memory;
+//   - more: the outcome is computed in parallel.  Only the first ray of an id
+//     can claim (a later one finds the id, or the same full row); claims of
+//     different rows are independent, so one thread per distinct row runs its
+//     first-of-id rays in ray order over the row's probes held in registers;
+//     registration indices are the prefix count of the claims in ray order,
+//     cut at the capacity.
+// Records the count before this frame's registrations and the ray index of
+// each registration, so that shade can give every SIMD-8 vector of rays the
+// registry size it would have seen.  The block is kept to one wave per SIMD
+// (256 threads, four bitmap words each): the pass sits between the trace and
+// the shade on the frame's critical path, while the next frame's primary pass
+// may hold most wave slots of the chip -- a 1024-thread block waited up to
+// 0.5 ms there for a compute unit with 16 free slots.
+constexpr uint32_t EM_THREADS = 256;
+constexpr uint32_t EM_CHUNK = 4u * EM_THREADS;   // bitmap words per scan step
+constexpr uint32_t EM_BATCH = EM_THREADS;        // flagged rays taken at once, one per thread
+constexpr uint32_t EM_KEYS = 2u * EM_BATCH;      // LDS open-addressed tables of the batch's ids and rows
+constexpr uint32_t EM_SERIAL = 24;               // up to this many: one thread
+static_assert(SHADE_BUDGET_MAX % (32u * 4u) == 0u, "the bitmap is read four words at a time");
+__global__ __launch_bounds__(EM_THREADS) void k_em_register(const Ray *__restrict__ rays, const uint32_t *__restrict__ live_ptr,
                                                       const Material *__restrict__ mats, uint32_t *__restrict__ em_claim,
                                                       EmissiveVoxel *__restrict__ emissives, uint32_t *__restrict__ em_bits,
                                                       uint32_t *__restrict__ em_reg, uint32_t *__restrict__ counters)
 {
-    __shared__ uint32_t chunk[1024];
-    __shared__ uint32_t count, nreg;
+    static_assert(EM_KEYS >= 2u * EM_BATCH, "the key tables stay at most half full");
+    __shared__ uint32_t wsum[EM_THREADS / 64u];
+    __shared__ uint32_t list_ri[EM_BATCH], e_vid[EM_BATCH], e_slot[EM_BATCH], e_claim[EM_BATCH], seg[EM_BATCH];
+    __shared__ uint32_t vkey[EM_KEYS], vfirst[EM_KEYS], rkey[EM_KEYS], rslot[EM_KEYS];
+    __shared__ uint32_t slot_row[EM_BATCH], row_cnt[EM_BATCH], row_off[EM_BATCH], row_fill[EM_BATCH];
+    __shared__ uint32_t count, nreg, n_slots;
     const uint32_t tid = threadIdx.x;
     const uint32_t live = *live_ptr;
     const uint32_t nw = (live + 31u) / 32u;
@@ -1380,39 +1436,166 @@ __global__ __launch_bounds__(1024) void k_em_register(const Ray *__restrict__ ra
         nreg = 0u;
         counters[VE_CNT_EM_BASE] = count;
     }
+    for (uint32_t k = tid; k < EM_KEYS; k += blockDim.x) {
+        vkey[k] = NONE;
+        rkey[k] = NONE;
+    }
     __syncthreads();
-    for (uint32_t base = 0; base < nw; base += 1024u) {
-        const uint32_t i = base + tid;
-        const uint32_t w = i < nw ? em_bits[i] : 0u;
-        if (w) em_bits[i] = 0u;
-        chunk[tid] = w;
-        if (__syncthreads_or(w != 0u) && tid == 0) {
-            for (uint32_t k = 0; k < 1024u && base + k < nw; k++) {
-                for (uint32_t m = chunk[k]; m; m &= m - 1u) {
-                    const uint32_t ri = (base + k) * 32u + (uint32_t)(__ffs(m) - 1);
-                    const Ray &r = rays[ri];
-                    const uint32_t vid = r.hit_vid;
-                    const uint32_t ch = claim_hash(vid) % EMISSIVE_CLAIM_W;
-                    for (uint32_t q = 0; q < EMISSIVE_CLAIM_PROBES; q++) {
-                        uint32_t &c = em_claim[(size_t)ch * EMISSIVE_CLAIM_PROBES + q];
-                        if (c == vid) break;
-                        if (c != 0u) continue;
-                        c = vid;
-                        if (count < EMISSIVE_CAPACITY) {
-                            const Material &m8 = mats[r.hit_material < MATERIAL_SLOTS ? r.hit_material : MATERIAL_SLOTS - 1u];
-                            const f3 ee = muls(f3{m8.color[0], m8.color[1], m8.color[2]}, m8.emissive_intensity);
-                            const uint32_t hs = (uint32_t)r.pad0;
-                            emissives[count] = EmissiveVoxel{ee.x, ee.y, ee.z, vid, (uint32_t)r.ox, (uint32_t)r.oy,
-                                                             (uint32_t)r.oz, hs > 1u ? hs : 1u};
-                            count++;
-                            em_reg[nreg++] = ri;
+    for (uint32_t base = 0; base < nw; base += EM_CHUNK) {
+        const uint32_t i = base + 4u * tid;   // this thread's four words: i .. i + 3
+        uint4 w4 = make_uint4(0u, 0u, 0u, 0u);
+        if (i < nw) {
+            w4 = *reinterpret_cast<const uint4 *>(em_bits + i);   // within the SHADE_BUDGET_MAX / 32 words
+            if (i + 1u >= nw) w4.y = 0u;   // words past the live rays hold no flags; keep them out
+            if (i + 2u >= nw) w4.z = 0u;
+            if (i + 3u >= nw) w4.w = 0u;
+            if (w4.x | w4.y | w4.z | w4.w) *reinterpret_cast<uint4 *>(em_bits + i) = make_uint4(0u, 0u, 0u, 0u);
+        }
+        const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
+        if (!__syncthreads_or((w4.x | w4.y | w4.z | w4.w) != 0u)) continue;
+        uint32_t total;
+        const uint32_t first = block_excl_scan((uint32_t)(__popc(w4.x) + __popc(w4.y) + __popc(w4.z) + __popc(w4.w)),
+                                               wsum, total);   // this thread's first list place
+        for (uint32_t r0 = 0; r0 < total; r0 += EM_BATCH) {
+            const uint32_t nb = min(EM_BATCH, total - r0);
+            {   // the batch's rays in order
+                uint32_t at = first;
+                for (uint32_t k = 0; k < 4u && at < r0 + EM_BATCH; k++)
+                    for (uint32_t m = ws[k]; m && at < r0 + EM_BATCH; m &= m - 1u, at++)
+                        if (at >= r0) list_ri[at - r0] = (i + k) * 32u + (uint32_t)(__ffs(m) - 1);
+            }
+            if (tid == 0) n_slots = 0u;
+            __syncthreads();
+            if (nb <= EM_SERIAL) {
+                if (tid == 0) {
+                    for (uint32_t j = 0; j < nb; j++) {
+                        const uint32_t ri = list_ri[j];
+                        const Ray &r = rays[ri];
+                        const uint32_t vid = r.hit_vid;
+                        const uint32_t ch = claim_hash(vid) % EMISSIVE_CLAIM_W;
+                        for (uint32_t q = 0; q < EMISSIVE_CLAIM_PROBES; q++) {
+                            uint32_t &c = em_claim[(size_t)ch * EMISSIVE_CLAIM_PROBES + q];
+                            if (c == vid) break;
+                            if (c != 0u) continue;
+                            c = vid;
+                            if (count < EMISSIVE_CAPACITY) {
+                                emissives[count] = emissive_of(r, mats, vid);
+                                count++;
+                                em_reg[nreg++] = ri;
+                            }
+                            break;
                         }
-                        break;
                     }
                 }
+                __syncthreads();
+                continue;
             }
+            // ids and rows of the batch: the first ray of each id, a slot per row
+            uint32_t vk = 0, rk = 0;
+            if (tid < nb) {
+                const uint32_t vid = rays[list_ri[tid]].hit_vid;
+                const uint32_t row = claim_hash(vid) % EMISSIVE_CLAIM_W;
+                e_vid[tid] = vid;
+                e_claim[tid] = 0u;
+                vk = vid % EM_KEYS;
+                for (;;) {   // at most EM_BATCH keys in EM_KEYS: a free one is always found
+                    const uint32_t prev = atomicCAS(&vkey[vk], NONE, vid);
+                    if (prev == NONE) {
+                        vfirst[vk] = NONE;
+                        break;
+                    }
+                    if (prev == vid) break;
+                    vk = (vk + 1u) % EM_KEYS;
+                }
+                rk = row % EM_KEYS;
+                for (;;) {
+                    const uint32_t prev = atomicCAS(&rkey[rk], NONE, row);
+                    if (prev == NONE) {
+                        const uint32_t sl = atomicAdd(&n_slots, 1u);
+                        slot_row[sl] = row;
+                        row_cnt[sl] = 0u;
+                        row_fill[sl] = 0u;
+                        rslot[rk] = sl;
+                        break;
+                    }
+                    if (prev == row) break;
+                    rk = (rk + 1u) % EM_KEYS;
+                }
+            }
+            __syncthreads();
+            if (tid < nb) atomicMin(&vfirst[vk], tid);
+            __syncthreads();
+            const bool keeper = tid < nb && vfirst[vk] == tid;
+            uint32_t sl = 0;
+            if (tid < nb) {
+                sl = rslot[rk];
+                e_slot[tid] = sl;
+                if (keeper) atomicAdd(&row_cnt[sl], 1u);
+            }
+            __syncthreads();
+            const uint32_t ns = n_slots;
+            uint32_t unused;
+            const uint32_t off = block_excl_scan(tid < ns ? row_cnt[tid] : 0u, wsum, unused);
+            if (tid < ns) row_off[tid] = off;
+            __syncthreads();
+            if (keeper) seg[row_off[sl] + atomicAdd(&row_fill[sl], 1u)] = tid;
+            __syncthreads();
+            if (tid < ns) {   // this row's first-of-id rays in ray order over its probes
+                const uint32_t o = row_off[tid], n = row_cnt[tid];
+                for (uint32_t a = 1; a < n; a++) {   // a row holds few: insertion sort
+                    const uint32_t x = seg[o + a];
+                    uint32_t b = a;
+                    for (; b > 0u && seg[o + b - 1u] > x; b--) seg[o + b] = seg[o + b - 1u];
+                    seg[o + b] = x;
+                }
+                uint32_t *rowp = em_claim + (size_t)slot_row[tid] * EMISSIVE_CLAIM_PROBES;
+                uint32_t pr[EMISSIVE_CLAIM_PROBES];
+#pragma unroll
+                for (uint32_t q = 0; q < EMISSIVE_CLAIM_PROBES; q++) pr[q] = rowp[q];
+                bool changed = false;
+                for (uint32_t a = 0; a < n; a++) {
+                    const uint32_t j = seg[o + a], vid = e_vid[j];
+                    bool done = false;
+#pragma unroll
+                    for (uint32_t q = 0; q < EMISSIVE_CLAIM_PROBES; q++) {
+                        if (done) continue;
+                        if (pr[q] == vid) {
+                            done = true;
+                        } else if (pr[q] == 0u) {
+                            pr[q] = vid;
+                            e_claim[j] = 1u;
+                            changed = true;
+                            done = true;
+                        }
+                    }
+                }
+                if (changed) {
+#pragma unroll
+                    for (uint32_t q = 0; q < EMISSIVE_CLAIM_PROBES; q++) rowp[q] = pr[q];
+                }
+            }
+            __syncthreads();
+            uint32_t claims;
+            const uint32_t cl = tid < nb ? e_claim[tid] : 0u;
+            const uint32_t before = block_excl_scan(cl, wsum, claims);
+            const uint32_t c0 = count, room = EMISSIVE_CAPACITY - min(c0, EMISSIVE_CAPACITY);
+            if (cl && before < room) {   // registered: the claims in ray order, up to the capacity
+                const uint32_t ri = list_ri[tid];
+                emissives[c0 + before] = emissive_of(rays[ri], mats, e_vid[tid]);
+                em_reg[nreg + before] = ri;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                const uint32_t add = min(claims, room);
+                count = c0 + add;
+                nreg += add;
+            }
+            for (uint32_t k = tid; k < EM_KEYS; k += blockDim.x) {
+                vkey[k] = NONE;
+                rkey[k] = NONE;
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (tid == 0) {
         counters[VE_CNT_EMISSIVES] = count;
@@ -3002,7 +3185,7 @@ void launch_trace(hipStream_t s, const uint32_t *words, const FrameParams &p, De
                            live_in_ptr(b, iter), iter, b.counters, b.node_stats, b.ray_totals, b.materials,
                            b.emissive_claim, b.em_bits, live_out_ptr(b, iter), b.hit_bits);
     if (iter == 0)
-        launch(k_em_register, dim3(1), dim3(1024), 0, s, bounce_in(b, iter), live_in_ptr(b, iter),
+        launch(k_em_register, dim3(1), dim3(EM_THREADS), 0, s, bounce_in(b, iter), live_in_ptr(b, iter),
                            b.materials, b.emissive_claim, b.emissives, b.em_bits, b.em_reg, b.counters);
 }
 

@@ -320,7 +320,13 @@ int ve_graph_stats(ve_ctx *ctx, uint64_t *captures, uint64_t *replays);
  * views per call (0 = 256).  The views are split into `groups` launches (0 =
  * 4) of about equal work, largest views first: a group's views start their
  * other passes as soon as its launch ends, beside the next group's launch.
- * voxelengine3.0_amd/autotune.py picks them. */
+ * voxelengine3.0_amd/autotune.py picks them.  whole_frame = 1: the batch runs
+ * whole frames -- after the one primary launch (groups is then 1), every
+ * other pass of all the views runs as one launch per kernel on the batch's
+ * stream, each block finding its view in a table of the views' records, so a
+ * frame of n views takes the same ~22 launches as a frame of one; the views'
+ * streams wait for the last of them.  Results are again those of ve_frame,
+ * bit for bit. */
 typedef struct ve_batch ve_batch;
 typedef struct ve_batch_config {
     int32_t  device;
@@ -329,6 +335,7 @@ typedef struct ve_batch_config {
     uint32_t grab;
     uint32_t waves_per_cu;
     uint32_t groups;
+    uint32_t whole_frame;
 } ve_batch_config;
 int  ve_batch_create(ve_batch **out, const ve_batch_config *cfg);
 void ve_batch_destroy(ve_batch *b);

@@ -1,8 +1,10 @@
 """The batched form (ve_batch_frame) against each view rendered alone
 (ve_frame): uneven views on different scenes, sizes, bounce counts and knobs,
 their primary passes in one persistent launch pulling tiles from a global
-queue; every structure of every view bit-exact frame by frame, for several
-launch shapes and groupings, with and without graph replay."""
+queue -- the other passes on each view's stream, or (whole frames) as one
+launch per kernel over all the views; every structure of every view
+bit-exact frame by frame, for several launch shapes and groupings, with and
+without graph replay."""
 import numpy as np
 import pytest
 
@@ -28,6 +30,7 @@ def views(ve, scenes, small_scene):
         ((fd, fw, finfo), scenes.palette(1), 256, 200, 2, dict(DET, min_samples=2), 7),
         (small, scenes.palette_spec(1), 64, 64, 3, dict(DET, sample_cap_direct=8, sample_cap_indirect=16), 9),
         ((fd, fw, finfo), scenes.palette(1), 200, 150, 3, {}, 11),
+        (small, scenes.palette(1), 48, 40, 1, dict(DET), 13),   # one bounce: no fused bounce pass
     ]
 
 
@@ -44,10 +47,12 @@ def _make(ve, vs, graph):
     return out
 
 
-@pytest.mark.parametrize("tile_w,grab,waves,groups,graph", [(8, 4, 16, 1, False), (32, 1, 4, 4, True),
-                                                            (64, 16, 8, 64, False), (16, 2, 32, 2, True)])
-def test_batch_matches_views_alone(ve, scenes, views, tile_w, grab, waves, groups, graph):
-    batch = ve.Batch(tile_w=tile_w, grab=grab, waves_per_cu=waves, groups=groups)
+@pytest.mark.parametrize("tile_w,grab,waves,groups,graph,whole", [(8, 4, 16, 1, False, False), (32, 1, 4, 4, True, False),
+                                                                  (64, 16, 8, 64, False, False),
+                                                                  (16, 2, 32, 2, True, False), (8, 4, 32, 1, False, True),
+                                                                  (32, 2, 8, 4, False, True)])
+def test_batch_matches_views_alone(ve, scenes, views, tile_w, grab, waves, groups, graph, whole):
+    batch = ve.Batch(tile_w=tile_w, grab=grab, waves_per_cu=waves, groups=groups, whole_frame=whole)
     a, b = _make(ve, views, graph), _make(ve, views, False)
     try:
         for f in range(3):
@@ -62,7 +67,7 @@ def test_batch_matches_views_alone(ve, scenes, views, tile_w, grab, waves, group
                 fcs.append(fc)
                 ins.append(fi)
             batch.frame([r for r, _ in a], [s for _, s in a], fcs, ins)
-            print("case batch", tile_w, grab, waves, groups, graph, "frame", f, flush=True)
+            print("case batch", tile_w, grab, waves, groups, graph, whole, "frame", f, flush=True)
             for i in range(len(views)):
                 for name in BUFS:
                     assert np.array_equal(a[i][0].read(name), b[i][0].read(name)), "view %d frame %d %s" % (i, f, name)

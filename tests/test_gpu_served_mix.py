@@ -76,12 +76,13 @@ def _alone(ve, scenes, scene, sess, seed):
         r.close()
 
 
-@pytest.mark.parametrize("batched", [False, True])
+@pytest.mark.parametrize("batched", [False, True, "whole"])
 def test_sessions_alone_equal_in_mix(ve, scenes, resident, batched):
     """Five sessions overlap on their own streams, frame k of each submitted
     together -- one by one, or as one batch (their primary passes in one
-    persistent launch, the rest replayed from each session's frame graph);
-    every frame equals the session rendered alone."""
+    persistent launch, the rest replayed from each session's frame graph; or,
+    whole frames, every pass one launch over the batch's sessions); every
+    frame equals the session rendered alone."""
     seeds = [101 + 17 * i for i in range(len(SESSIONS))]
     alone = []
     for i, s in enumerate(SESSIONS):
@@ -90,12 +91,14 @@ def test_sessions_alone_equal_in_mix(ve, scenes, resident, batched):
     srv = ve.Server(queues=0)
     try:
         ids = {key: srv.add_scene(w, d, info.size) for key, (d, w, info) in resident.items()}
-        if batched:
+        if batched == "whole":
+            srv.set_batch(tile_w=8, grab=4, waves_per_cu=32, groups=1, whole_frame=True)
+        elif batched:
             srv.set_graph(True)
             srv.set_batch(tile_w=16, grab=4, waves_per_cu=16)
         sids = [srv.open(ids[s[0]], s[1], s[2], s[4] + 1, scenes.palette(1), SKY) for s in SESSIONS]
         for k in range(FRAMES):
-            print("case served mix frame", k, "batched" if batched else "", flush=True)
+            print("case served mix frame", k, batched, flush=True)
             args = [(sids[i], ve.FrameConfig(**_knobs(s[3])), _inputs(ve, s[1], s[2], seeds[i], k),
                      _ubo(ve, scenes, resident[s[0]][0], s[1], s[2], seeds[i], k)) for i, s in enumerate(SESSIONS)]
             if batched:

@@ -116,7 +116,7 @@ class ServerStats(C.Structure):
 
 class BatchConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("max_views", C.c_uint32), ("tile_w", C.c_uint32), ("grab", C.c_uint32),
-                ("waves_per_cu", C.c_uint32), ("groups", C.c_uint32)]
+                ("waves_per_cu", C.c_uint32), ("groups", C.c_uint32), ("whole_frame", C.c_uint32)]
 
 
 _lib = None
@@ -401,14 +401,16 @@ def primary_noalloc(words_dev, n_words, depth, ubo, width, height, controlchecks
 
 class Batch:
     """ve_batch_*: frame k of many Renderers at once (one persistent primary
-    launch over all views, the other passes on each view's stream)."""
+    launch over all views, the other passes on each view's stream; with
+    whole_frame, every pass as one launch over all views)."""
 
-    def __init__(self, device=0, max_views=256, tile_w=8, grab=4, waves_per_cu=16, groups=4):
-        cfg = BatchConfig(device, max_views, tile_w, grab, waves_per_cu, groups)
+    def __init__(self, device=0, max_views=256, tile_w=8, grab=4, waves_per_cu=16, groups=4, whole_frame=False):
+        cfg = BatchConfig(device, max_views, tile_w, grab, waves_per_cu, groups, 1 if whole_frame else 0)
         h = C.c_void_p()
         _check(lib().ve_batch_create(C.byref(h), C.byref(cfg)))
         self.h = h
-        self.config = dict(tile_w=tile_w, grab=grab, waves_per_cu=waves_per_cu, groups=groups)
+        self.config = dict(tile_w=tile_w, grab=grab, waves_per_cu=waves_per_cu, groups=groups,
+                           whole_frame=bool(whole_frame))
 
     def close(self):
         if self.h:
@@ -681,8 +683,8 @@ class Server:
         t = (C.c_uint32 * 3)(tile_chunk, 0, bounce_blocks)
         self._ck(lib().ve_server_set_tuning(self.h, t))
 
-    def set_batch(self, tile_w=8, grab=4, waves_per_cu=16, groups=4):
-        cfg = BatchConfig(0, 0, tile_w, grab, waves_per_cu, groups)
+    def set_batch(self, tile_w=8, grab=4, waves_per_cu=16, groups=4, whole_frame=False):
+        cfg = BatchConfig(0, 0, tile_w, grab, waves_per_cu, groups, 1 if whole_frame else 0)
         self._ck(lib().ve_server_set_batch(self.h, C.byref(cfg)))
 
     def close_session(self, session):

@@ -23,9 +23,10 @@ struct ve_batch {
     ve_batch_config cfg{};
     uint32_t blocks = 0;                     // persistent grid
     hipStream_t stream = nullptr;
-    // one block on the device and in each staging set, filled by one copy:
-    // queue counters (MAX_GROUPS, zero), bases (max_views + MAX_GROUPS), views
-    size_t off_views = 0, block_bytes = 0;
+    // one block on the device and in each staging set, filled by one copy
+    // (Layout): queue counters, tile bases, primary views and, for whole
+    // frames, the view records and block offsets of the other passes
+    size_t block_bytes = 0;
     unsigned char *dev = nullptr;
     // pinned staging sets: a set is rewritten only once its copy has run (a
     // query, never a host wait: a serving loop must not block); a new set is
@@ -71,6 +72,22 @@ bool tile_width_ok(uint32_t w)
     return false;
 }
 
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Byte offsets in the staging / device block of a batch of n views: queue
+// counters (MAX_GROUPS, zero), tile bases (n + MAX_GROUPS), n primary views,
+// then (whole frames) n view records and MK_COUNT rows of n + 1 block offsets.
+struct Layout {
+    size_t views, recs, pre, end;
+    explicit Layout(uint32_t n)
+    {
+        views = align256((2 * (size_t)MAX_GROUPS + n) * sizeof(uint32_t));
+        recs = align256(views + (size_t)n * sizeof(ve::BatchView));
+        pre = align256(recs + (size_t)n * sizeof(ve::ViewRec));
+        end = pre + (size_t)ve::MK_COUNT * (n + 1u) * sizeof(uint32_t);
+    }
+};
+
 }  // namespace
 
 namespace ve {
@@ -81,9 +98,12 @@ namespace ve {
 // each on the batch stream; a view's stream waits for its group's launch only,
 // so the views of early groups run their other passes beside the later
 // launches.  `starts` (optional, one per view) are recorded on the batch
-// stream before the first launch.
+// stream before the first launch.  With `whole`, the launch is one group and
+// every other pass of every view follows it on the batch stream, one launch
+// per kernel over all the views (launch_frames_multi); the views' streams
+// then wait for the last of them.
 int batch_primary(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const *scenes, const FrameParams *ps,
-                  hipEvent_t *starts)
+                  hipEvent_t *starts, bool whole)
 {
     if (n > b->max_views) return batch_error(b, VE_EINVAL, "%u views above the batch's %u", n, b->max_views);
     const uint32_t tw = b->cfg.tile_w, th = 64u / tw;
@@ -95,7 +115,7 @@ int batch_primary(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const 
         order[i] = i;
     }
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return tiles[a] > tiles[c]; });
-    const uint32_t groups = std::max(1u, std::min(std::min(b->cfg.groups, n), MAX_GROUPS));
+    const uint32_t groups = whole ? 1u : std::max(1u, std::min(std::min(b->cfg.groups, n), MAX_GROUPS));
     // group g: sorted views [first[g], first[g + 1]), cut where the running work passes (g + 1) / groups
     std::vector<uint32_t> first(groups + 1, n);
     first[0] = 0;
@@ -121,11 +141,37 @@ int batch_primary(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const 
         b->staging.push_back(x);
         st = &b->staging.back();
     }
+    const Layout L(n);
     uint32_t *qh = reinterpret_cast<uint32_t *>(st->host);
     uint32_t *bh = qh + MAX_GROUPS;   // group g's bases at first[g] + g, first[g + 1] - first[g] + 1 of them
-    BatchView *vh = reinterpret_cast<BatchView *>(st->host + b->off_views);
+    BatchView *vh = reinterpret_cast<BatchView *>(st->host + L.views);
     uint32_t *queue_dev = reinterpret_cast<uint32_t *>(b->dev), *bases_dev = queue_dev + MAX_GROUPS;
-    BatchView *views_dev = reinterpret_cast<BatchView *>(b->dev + b->off_views);
+    BatchView *views_dev = reinterpret_cast<BatchView *>(b->dev + L.views);
+    uint32_t grids[MK_COUNT] = {0};
+    uint32_t lds = 0;
+    if (whole) {   // every view's record and its place in each pass's grid
+        ViewRec *rh = reinterpret_cast<ViewRec *>(st->host + L.recs);
+        uint32_t *ph = reinterpret_cast<uint32_t *>(st->host + L.pre);
+        for (uint32_t i = 0; i < n; i++) {
+            ViewRec &r = rh[i];
+            r.p = ps[i];
+            r.b = ctxs[i]->b;
+            r.words = scenes[i]->words;
+            r.iters = std::min<uint32_t>(ctxs[i]->max_bounces, (uint32_t)MAX_ITERS);   // renderer.cpp:515
+            lds = std::max(lds, timeouts_lds(ps[i]));
+        }
+        for (uint32_t k = 0; k < MK_COUNT; k++) {
+            uint32_t *row = ph + (size_t)k * (n + 1u), at = 0;
+            for (uint32_t i = 0; i < n; i++) {
+                row[i] = at;
+                const uint64_t next = (uint64_t)at + multi_blocks(k, ps[i], rh[i].iters);
+                if (next > 0x7FFFFFFFull) return batch_error(b, VE_ECAPACITY, "whole-frame batch grid above 2^31 blocks");
+                at = (uint32_t)next;
+            }
+            row[n] = at;
+            grids[k] = at;
+        }
+    }
     std::memset(qh, 0, MAX_GROUPS * sizeof(uint32_t));
     std::vector<uint32_t> totals(groups, 0);
     for (uint32_t gi = 0; gi < groups; gi++) {
@@ -162,8 +208,7 @@ int batch_primary(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const 
         B_HIP(b, hipEventRecord(b->ctx_ev[i], ctxs[i]->stream));
         B_HIP(b, hipStreamWaitEvent(b->stream, b->ctx_ev[i], 0));
     }
-    B_HIP(b, hipMemcpyAsync(b->dev, st->host, b->off_views + (size_t)n * sizeof(BatchView), hipMemcpyHostToDevice,
-                            b->stream));
+    B_HIP(b, hipMemcpyAsync(b->dev, st->host, whole ? L.end : L.recs, hipMemcpyHostToDevice, b->stream));
     B_HIP(b, hipEventRecord(st->copied, b->stream));
     if (starts)
         for (uint32_t i = 0; i < n; i++) B_HIP(b, hipEventRecord(starts[i], b->stream));
@@ -177,16 +222,26 @@ int batch_primary(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const 
         launch_primary_batch(b->stream, views_dev + first[gi], bases_dev + first[gi] + gi, m, totals[gi],
                              queue_dev + gi, tw, b->cfg.grab, blocks);
         B_HIP(b, hipGetLastError());
+        if (whole) continue;
         B_HIP(b, hipEventRecord(b->done[gi], b->stream));
         for (uint32_t j = first[gi]; j < first[gi + 1]; j++)
             B_HIP(b, hipStreamWaitEvent(ctxs[order[j]]->stream, b->done[gi], 0));
     }
     B_HIP(b, hipEventRecord(b->t1, b->stream));
+    if (whole) {
+        launch_frames_multi(b->stream, reinterpret_cast<const ViewRec *>(b->dev + L.recs),
+                            reinterpret_cast<const uint32_t *>(b->dev + L.pre), n, grids, lds);
+        B_HIP(b, hipGetLastError());
+        B_HIP(b, hipEventRecord(b->done[0], b->stream));
+        for (uint32_t i = 0; i < n; i++) B_HIP(b, hipStreamWaitEvent(ctxs[i]->stream, b->done[0], 0));
+    }
     b->launched = true;
     return VE_OK;
 }
 
 bool batch_idle(ve_batch *b) { return !b->launched || hipEventQuery(b->t1) != hipErrorNotReady; }
+
+bool batch_whole(const ve_batch *b) { return b->cfg.whole_frame != 0; }
 
 const char *batch_error_text(const ve_batch *b) { return b->err.c_str(); }
 
@@ -219,8 +274,7 @@ int ve_batch_create(ve_batch **out, const ve_batch_config *cfg)
         b->blocks = (cus * c.waves_per_cu + 3u) / 4u;   // 4 waves per block
     }
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);   // default priority
-    b->off_views = ((2 * (size_t)MAX_GROUPS + c.max_views) * sizeof(uint32_t) + 255) & ~(size_t)255;
-    b->block_bytes = b->off_views + (size_t)c.max_views * sizeof(ve::BatchView);
+    b->block_bytes = Layout(c.max_views).end;
     if (e == hipSuccess) e = hipMalloc(&b->dev, b->block_bytes);
     b->ctx_ev.assign(c.max_views, nullptr);
     for (uint32_t i = 0; i < c.max_views && e == hipSuccess; i++)
@@ -277,8 +331,9 @@ int ve_batch_frame(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const
         if (rc) return batch_error(b, rc, "view %u: %s", i, ctxs[i]->err.c_str());
     }
     B_HIP(b, hipSetDevice(b->device));
-    int rc = ve::batch_primary(b, n, ctxs, scenes, b->params.data(), nullptr);
-    if (rc) return rc;
+    const bool whole = b->cfg.whole_frame != 0;
+    int rc = ve::batch_primary(b, n, ctxs, scenes, b->params.data(), nullptr, whole);
+    if (rc || whole) return rc;
     for (uint32_t i = 0; i < n; i++)
         if ((rc = ve::frame_rest(ctxs[i], scenes[i], b->params[i])))
             return batch_error(b, rc, "view %u: %s", i, ctxs[i]->err.c_str());

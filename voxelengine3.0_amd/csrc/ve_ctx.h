@@ -118,8 +118,12 @@ int frame_begin(ve_ctx *c, ve_scene *s, const ve_frame_config *fc, const ve_fram
 int frame_rest(ve_ctx *c, ve_scene *s, const FrameParams &p);
 // The batched primary launch of ve_batch_frame for views prepared by
 // frame_begin (ve_batch.cpp); `starts` (optional) are recorded just before it.
+// With `whole`, every other pass of the views follows on the batch stream,
+// one launch per kernel over all of them, and the views' streams wait for it.
 int batch_primary(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const *scenes, const FrameParams *ps,
-                  hipEvent_t *starts);
+                  hipEvent_t *starts, bool whole = false);
+// The batch runs whole frames (ve_batch_config::whole_frame).
+bool batch_whole(const ve_batch *b);
 const char *batch_error_text(const ve_batch *b);
 // No primary launch of the batch still running (a query, no host wait).
 bool batch_idle(ve_batch *b);

@@ -57,6 +57,18 @@ __device__ __forceinline__ uint32_t fp_frame_seed(const FrameParams &p) { return
 __device__ __forceinline__ uint32_t fp_time(const FrameParams &p) { return p.dyn ? p.dyn->time : p.time; }
 __device__ __forceinline__ const float *fp_cam(const FrameParams &p) { return p.dyn ? p.dyn->cam : p.cam; }
 
+// A pointer read from device memory (a view record of a launch over many
+// views) is generic to the compiler, where the pointer arguments of a kernel
+// are known to address device memory: flat instructions then replace the
+// global ones, and every wait on a load also waits for LDS traffic.  The
+// buffers such records name are device memory, so their pointer slots are
+// read as global-address-space pointers.
+template <typename T>
+__device__ __forceinline__ T *gptr(T *const &slot)
+{
+    typedef __attribute__((address_space(1))) T *GlobalPtr;
+    return (T *)*static_cast<const GlobalPtr *>(static_cast<const void *>(&slot));
+}
 // Block b of a screen pass (4 tiles of 8x8 pixels per block) -> its tile
 // group.  The hardware deals blocks to the 8 XCDs round-robin (XCD b % 8).
 // chunk 0: XCD x takes the x-th contiguous eighth of the groups (one band of

@@ -130,17 +130,19 @@ __global__ __launch_bounds__(256) void k_primary_batch(const BatchView *__restri
             uint32_t checks = 0, reads = 0;
             const f3 dir = primary_dir(bv.cam, x, y, bv.W, bv.H);
             const f3 org = mk(bv.cam[0], bv.cam[1], bv.cam[2]);
-            const Hit h = raycast<false>(bv.words, bv.jump, bv.jump_level, bv.n_words, bv.depth, org, dir, rcp3(dir),
-                                         bv.cc, checks, reads, path_cache + threadIdx.x, 256u);
+            const Hit h = raycast<false>(gptr(bv.words), gptr(bv.jump), bv.jump_level, bv.n_words, bv.depth, org, dir,
+                                         rcp3(dir), bv.cc, checks, reads, path_cache + threadIdx.x, 256u);
             const size_t pix = (size_t)y * bv.W + x;
+            uint4 *gbuf = gptr(bv.gbuf);
+            uint32_t *gpos = gptr(bv.gpos);
             if (h.hit) {
                 const uint32_t bits = bv.depth - 1u;
-                bv.gbuf[pix] = make_uint4(checks, h.material, 0u, voxel_key(h.px, h.py, h.pz) + 1u);
-                bv.gpos[pix] = (h.px >> 1) | ((h.py >> 1) << bits) | ((h.pz >> 1) << (2u * bits));
-                if (bv.reuse) bv.ghit[pix] = make_uint2(h.px | (h.py << 16), h.pz);
+                gbuf[pix] = make_uint4(checks, h.material, 0u, voxel_key(h.px, h.py, h.pz) + 1u);
+                gpos[pix] = (h.px >> 1) | ((h.py >> 1) << bits) | ((h.pz >> 1) << (2u * bits));
+                if (bv.reuse) gptr(bv.ghit)[pix] = make_uint2(h.px | (h.py << 16), h.pz);
             } else {
-                bv.gbuf[pix] = make_uint4(checks, 0u, 0u, 0u);
-                bv.gpos[pix] = 0u;
+                gbuf[pix] = make_uint4(checks, 0u, 0u, 0u);
+                gpos[pix] = 0u;
             }
         }
     }

@@ -97,6 +97,42 @@ struct DeviceBuffers {
 
 void launch_primary(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, bool stats);
 
+// ---- one launch per pass over many views (the whole-frame batched form) ----
+// A view of such a launch: its frame parameters, its context's buffers, its
+// scene and the frame's bounce iterations (renderer.cpp:515).  The table and
+// the block offsets live in device memory.
+struct ViewRec {
+    FrameParams p;
+    DeviceBuffers b;
+    const uint32_t *words;
+    uint32_t iters;
+};
+struct MultiTab {
+    const ViewRec *views;
+    const uint32_t *pre;       // pre[v]: first block of view v; pre[n]: the grid
+    uint32_t n;
+};
+// The kernels after the primary pass, in the order a frame runs them
+// (renderer.cpp:282-649 with the threshold and the deposit-record clear folded
+// into the emit and resolve kernels).
+enum MultiKernel : uint32_t {
+    MK_RESET, MK_NRM_EVENTS, MK_NRM_REPLAY, MK_NRM_COMPUTE, MK_SCHED_CLAIM, MK_SCHED_RANK, MK_SCHED_RANK_OVER,
+    MK_SCHED_VEC_COUNT, MK_SCHED_WRITE, MK_RANK, MK_RANK_WRITE, MK_EMIT, MK_TRACE, MK_EM_REGISTER, MK_SHADE,
+    MK_BOUNCE, MK_DEP_PREP, MK_DEP_WPOS, MK_DEP_REPLAY, MK_RESOLVE_AVG, MK_RESOLVE, MK_RESOLVE_TIMEOUTS, MK_COUNT
+};
+// Blocks of kernel k in one view's whole frame (the grid ve_frame launches it
+// with; 0 when that frame skips it).  Frames of such launches run with the
+// small passes folded and the full bounce grid.
+uint32_t multi_blocks(uint32_t k, const FrameParams &p, uint32_t iters);
+// Dynamic LDS bytes of one view's timeout reconstruction launch.
+uint32_t timeouts_lds(const FrameParams &p);
+// Every pass after the primary pass of n views, one launch per kernel on `s`:
+// `views` (n records) and `pre` (MK_COUNT rows of n + 1 block offsets, row k
+// for kernel k) in device memory; grids[k] = row k's last entry (launches
+// with an empty grid are skipped); `lds` = the largest timeouts_lds.
+void launch_frames_multi(hipStream_t s, const ViewRec *views, const uint32_t *pre, uint32_t n, const uint32_t *grids,
+                         uint32_t lds);
+
 // One view of a batched primary pass: what generate_primary.comp reads for
 // it, and where its G-buffer goes.
 struct BatchView {

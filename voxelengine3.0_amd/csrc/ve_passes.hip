@@ -33,13 +33,70 @@ constexpr int TILE = 8;
 constexpr int NUM_XCD = 8;
 
 // ---------------------------------------------------------------------------
+// Launches over many views
+// ---------------------------------------------------------------------------
+// The frame kernels take a MULTI template parameter.  MULTI = false: one view,
+// arguments as given, blocks = blockIdx.x of gridDim.x.  MULTI = true: one
+// launch covers the same pass of many views (ViewRec, MultiTab); its grid is
+// the concatenation of each view's own grid, and every block finds its view,
+// takes that view's arguments from its record and runs as block x of the
+// view's n.
+
+__device__ __forceinline__ FrameParams global_params(const FrameParams &v)
+{
+    FrameParams p = v;
+    p.sky_tex = gptr(v.sky_tex);
+    p.jump = gptr(v.jump);
+    p.dyn = gptr(v.dyn);
+    return p;
+}
+
+// The block's view: the last v with pre[v] <= blockIdx.x (views with an empty
+// grid share their start with the next view).  Called by whole waves before
+// any branch; every wave of a block computes the same answer.
+__device__ __forceinline__ const ViewRec &multi_view(const MultiTab &mt, uint32_t &x, uint32_t &n)
+{
+    const uint32_t b = blockIdx.x, lane = threadIdx.x & 63u;
+    uint32_t cnt = 0;
+    for (uint32_t base = 0; base < mt.n; base += 64u) {
+        const uint32_t i = base + lane;
+        const uint64_t m = __ballot(i < mt.n && mt.pre[i] <= b);
+        cnt += (uint32_t)__popcll(m);
+        if (m != ~0ull) break;
+    }
+    const uint32_t v = (uint32_t)__builtin_amdgcn_readfirstlane(cnt - 1u);
+    const uint32_t first = mt.pre[v];
+    x = b - first;
+    n = mt.pre[v + 1u] - first;
+    return mt.views[v];
+}
+
+// Bounce iteration `iter` of the trace/shade loop (renderer.cpp:515-602); the
+// ping-pong direction alternates with `iter`.
+__host__ __device__ inline Ray *bounce_in(const DeviceBuffers &b, uint32_t iter) { return (iter & 1u) ? b.rays_next : b.rays; }
+__host__ __device__ inline Ray *bounce_out(const DeviceBuffers &b, uint32_t iter) { return (iter & 1u) ? b.rays : b.rays_next; }
+__host__ __device__ inline uint32_t *live_in_ptr(const DeviceBuffers &b, uint32_t iter)
+{
+    return b.counters + ((iter & 1u) ? VE_CNT_LIVE_B : VE_CNT_LIVE);
+}
+__host__ __device__ inline uint32_t *live_out_ptr(const DeviceBuffers &b, uint32_t iter)
+{
+    return b.counters + ((iter & 1u) ? VE_CNT_LIVE : VE_CNT_LIVE_B);
+}
+__host__ __device__ inline uint32_t *li_in_ptr(const DeviceBuffers &b, uint32_t iter) { return b.ray_li[iter & 1u]; }
+__host__ __device__ inline uint32_t *li_out_ptr(const DeviceBuffers &b, uint32_t iter) { return b.ray_li[(iter & 1u) ^ 1u]; }
+// SIMD-8 vectors of the schedule pass (one 8x8 work group = 8 vectors)
+__host__ __device__ inline uint32_t sched_vectors(const FrameParams &p) { return ((p.W + 7u) / 8u) * ((p.H + 7u) / 8u) * 8u; }
+
+// ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool tile_pixel(const FrameParams &p, uint32_t &x, uint32_t &y)
+__device__ __forceinline__ bool tile_pixel(uint32_t vbx, uint32_t vbn, const FrameParams &p, uint32_t &x,
+                                           uint32_t &y)
 {
     const uint32_t tiles_x = (p.W + TILE - 1) / TILE;
     const uint32_t tiles_y = (p.H + TILE - 1) / TILE;
-    const uint32_t tile = xcd_group(blockIdx.x, gridDim.x, p.tile_chunk) * 4u + (threadIdx.x >> 6);
+    const uint32_t tile = xcd_group(vbx, vbn, p.tile_chunk) * 4u + (threadIdx.x >> 6);
     if (tile >= tiles_x * tiles_y) return false;
     const uint32_t lane = threadIdx.x & 63u;
     x = (tile % tiles_x) * TILE + (lane & 7u);
@@ -402,12 +459,34 @@ struct OrderLists {
     uint32_t spill_cap;
 };
 
+__host__ __device__ inline OrderLists nrm_lists(const DeviceBuffers &b)
+{
+    return OrderLists{b.ord_cap, b.ord_over_cap, b.ord_cnt, b.ord_ev, b.ord_over, b.ord_over_n, b.ord_spill,
+                      b.ord_spill_cap};
+}
+__device__ __forceinline__ OrderLists nrm_lists_global(const DeviceBuffers &b)
+{
+    return OrderLists{b.ord_cap, b.ord_over_cap, gptr(b.ord_cnt), gptr(b.ord_ev), gptr(b.ord_over),
+                      gptr(b.ord_over_n), gptr(b.ord_spill), b.ord_spill_cap};
+}
+
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_nrm_events(FrameParams p, const uint4 *__restrict__ gbuf,
                                                     const uint32_t *__restrict__ gpos, OrderLists ol,
-                                                    uint32_t *__restrict__ counters)
+                                                    uint32_t *__restrict__ counters,
+                                                    MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        p = global_params(V.p);
+        gbuf = gptr(V.b.gbuf);
+        gpos = gptr(V.b.gpos);
+        ol = nrm_lists_global(V.b);
+        counters = gptr(V.b.counters);
+    }
     uint32_t x = 0, y = 0;
-    const bool valid = tile_pixel(p, x, y);
+    const bool valid = tile_pixel(vbx, vbn, p, x, y);
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (1ull << lane) - 1ull;
     const size_t pix = valid ? (size_t)y * p.W + x : 0;
@@ -495,13 +574,24 @@ __device__ __forceinline__ uint4 listed_event(const OrderLists &ol, uint32_t h, 
 
 constexpr uint32_t REPLAY_WAVES = 4;   // buckets per block of the replays (one per wave)
 
+template <bool MULTI>
 __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_nrm_replay(FrameParams p, uint32_t *__restrict__ nbuf,
                                                                   OrderLists ol, uint32_t *__restrict__ work,
-                                                                  uint32_t *__restrict__ counters)
+                                                                  uint32_t *__restrict__ counters,
+                                                                  MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        p = global_params(V.p);
+        nbuf = gptr(V.b.nbuf);
+        ol = nrm_lists_global(V.b);
+        work = gptr(V.b.ord_work);
+        counters = gptr(V.b.counters);
+    }
     __shared__ uint64_t lds_keys_all[REPLAY_WAVES][ORDER_LDS_EVENTS];
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    const uint32_t h = blockIdx.x * REPLAY_WAVES + wv;
+    const uint32_t h = vbx * REPLAY_WAVES + wv;
     if (h >= p.hash_width) return;
     const bool slot_lane = lane < NBUFFER_SLOTS;
     const uint32_t n = ol.cnt[h];
@@ -573,12 +663,23 @@ __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_nrm_replay(FrameParams p,
 // Work word NONE: the slot keeps its normal.  (A packed position of NONE
 // would need every coordinate bit set past depth 11, where the reference's
 // packing has already overflowed; such a slot keeps its old normal.)
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_nrm_compute(const uint32_t *__restrict__ words, FrameParams p,
                                                      uint32_t *__restrict__ nbuf, const uint32_t *__restrict__ work,
-                                                     uint32_t *__restrict__ over_n)
+                                                     uint32_t *__restrict__ over_n,
+                                                     MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        words = gptr(V.words);
+        p = global_params(V.p);
+        nbuf = gptr(V.b.nbuf);
+        work = gptr(V.b.ord_work);
+        over_n = gptr(V.b.ord_over_n);
+    }
     __shared__ uint32_t path_cache[VE_MAX_DEPTH * 256];
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = vbx * blockDim.x + threadIdx.x;
     if (t == 0) over_n[0] = over_n[1] = 0u;   // the replay, the lists' last reader, is done
     if (t >= p.hash_width * NBUFFER_SLOTS) return;
     const uint32_t src = work[t];
@@ -629,13 +730,28 @@ __device__ __forceinline__ uint32_t sched_home(uint32_t v)
 // slot. A slot is {id, ~lowest execution rank, pixels, 0}: the complement
 // makes a zeroed slot "free" and lets one 64-bit CAS both claim the slot and
 // keep the lowest rank; the pixel count is added on the same cache line.
+template <bool MULTI>
 __global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *__restrict__ gbuf, uint4 *__restrict__ tab,
                                                     uint32_t *__restrict__ cand_at, uint32_t *__restrict__ counters,
                                                     uint4 *__restrict__ over, uint32_t *__restrict__ over_n,
-                                                    uint32_t *__restrict__ row_over, uint32_t *__restrict__ rows)
+                                                    uint32_t *__restrict__ row_over, uint32_t *__restrict__ rows,
+                                                    MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        p = global_params(V.p);
+        gbuf = gptr(V.b.gbuf);
+        tab = gptr(V.b.sched_tab);
+        cand_at = gptr(V.b.cand_at);
+        counters = gptr(V.b.counters);
+        over = gptr(V.b.sched_over);
+        over_n = gptr(V.b.sched_over_n);
+        row_over = gptr(V.b.sched_row_over);
+        rows = gptr(V.b.sched_rows);
+    }
     const uint32_t lane = threadIdx.x;
-    const uint32_t e = blockIdx.x * 64u + lane;
+    const uint32_t e = vbx * 64u + lane;
     cand_at[e] = NONE;
     uint32_t pix;
     uint32_t vid = 0u;
@@ -690,13 +806,24 @@ __global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *
 // Writes the reference-layout claim and pixel-count maps (all 64 probes, so
 // no separate clear is needed), marks each kept voxel's first pixel, and
 // clears the row region for the next frame.
+template <bool MULTI>
 __global__ __launch_bounds__(64) void k_sched_rank(uint4 *__restrict__ tab, uint32_t *__restrict__ claim,
                                                    uint32_t *__restrict__ samples, uint32_t *__restrict__ cand_at,
-                                                   const uint32_t *__restrict__ row_over)
+                                                   const uint32_t *__restrict__ row_over,
+                                                   MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        tab = gptr(V.b.sched_tab);
+        claim = gptr(V.b.claim);
+        samples = gptr(V.b.samples);
+        cand_at = gptr(V.b.cand_at);
+        row_over = gptr(V.b.sched_row_over);
+    }
     constexpr uint32_t PER_LANE = SCHED_SLOTS / 64u;
     __shared__ uint32_t firsts[SCHED_SLOTS];
-    const uint32_t h = blockIdx.x, lane = threadIdx.x;
+    const uint32_t h = vbx, lane = threadIdx.x;
     if (row_over[h] != 0u) return;   // k_sched_rank_over ranks it
     uint4 *row = tab + (size_t)h * SCHED_SLOTS;
     uint4 m[PER_LANE];
@@ -742,18 +869,33 @@ constexpr uint32_t SCHED_OVER_BLOCKS = 256;  // blocks striding over the flagged
 // execution rank is its records' lowest and its pixel count their sum. Then
 // the same ranking as k_sched_rank. One block per flagged row; the table row
 // and the flag are cleared for the next frame.
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_sched_rank_over(uint4 *__restrict__ tab, uint32_t *__restrict__ claim,
                                                          uint32_t *__restrict__ samples, uint32_t *__restrict__ cand_at,
                                                          const uint4 *__restrict__ over, const uint32_t *__restrict__ over_n,
                                                          uint32_t *__restrict__ row_over, const uint32_t *__restrict__ rows,
-                                                         uint32_t *__restrict__ counters)
+                                                         uint32_t *__restrict__ counters,
+                                                         MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        tab = gptr(V.b.sched_tab);
+        claim = gptr(V.b.claim);
+        samples = gptr(V.b.samples);
+        cand_at = gptr(V.b.cand_at);
+        over = gptr(V.b.sched_over);
+        over_n = gptr(V.b.sched_over_n);
+        row_over = gptr(V.b.sched_row_over);
+        rows = gptr(V.b.sched_rows);
+        counters = gptr(V.b.counters);
+    }
     __shared__ uint32_t tv[SCHED_SLOTS], tf[SCHED_SLOTS], tc[SCHED_SLOTS];
     __shared__ uint32_t rv[SCHED_ROW_OVER], rf[SCHED_ROW_OVER], rc[SCHED_ROW_OVER];
     __shared__ uint32_t nt, nr, nrep;
     const uint32_t t = threadIdx.x;
     const uint32_t no = min(over_n[0], SCHED_OVER_CAP), nrows = over_n[1];
-    for (uint32_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+    for (uint32_t r = vbx; r < nrows; r += vbn) {
         const uint32_t h = rows[r];
         uint4 *row = tab + (size_t)h * SCHED_SLOTS;
         if (t == 0u) {
@@ -866,13 +1008,24 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t x, uint32_t *lds
 
 // Candidates first-claimed in each SIMD-8 vector, and their sum per block of
 // 256 vectors (k_sched_write turns these into the vectors' offsets).
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_sched_vec_count(const uint32_t *__restrict__ cand_at, uint32_t n_vec,
                                                          uint32_t *__restrict__ vec_cnt,
                                                          uint32_t *__restrict__ vec_chunk,
-                                                         uint32_t *__restrict__ over_n)
+                                                         uint32_t *__restrict__ over_n,
+                                                         MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        cand_at = gptr(V.b.cand_at);
+        n_vec = sched_vectors(V.p);
+        vec_cnt = gptr(V.b.vec_cnt);
+        vec_chunk = gptr(V.b.vec_chunk);
+        over_n = gptr(V.b.sched_over_n);
+    }
     __shared__ uint32_t lds[4];
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t v = vbx * blockDim.x + threadIdx.x;
     if (v == 0u) {   // the overflow records of this frame are consumed
         over_n[0] = 0u;
         over_n[1] = 0u;
@@ -883,23 +1036,37 @@ __global__ __launch_bounds__(256) void k_sched_vec_count(const uint32_t *__restr
         vec_cnt[v] = c;
     }
     const uint32_t t = block_sum256(c, lds);
-    if (threadIdx.x == 0) vec_chunk[blockIdx.x] = t;
+    if (threadIdx.x == 0) vec_chunk[vbx] = t;
 }
 
 // Inside a vector, wins happen probe by probe (lockstep loop) and, within a
 // probe, in lane order: sort the vector's <= 8 winners by (probe, lane).
 // A vector's first candidate index: the block's offset (the sum of the
 // earlier blocks' totals) plus the prefix inside the block.
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_sched_write(FrameParams p, const uint32_t *__restrict__ cand_at, uint32_t n_vec,
                                                      const uint32_t *__restrict__ vec_cnt,
                                                      const uint32_t *__restrict__ vec_chunk,
                                                      const uint32_t *__restrict__ claim, uint4 *__restrict__ candidates,
-                                                     uint32_t *__restrict__ counters)
+                                                     uint32_t *__restrict__ counters,
+                                                     MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        p = global_params(V.p);
+        cand_at = gptr(V.b.cand_at);
+        n_vec = sched_vectors(V.p);
+        vec_cnt = gptr(V.b.vec_cnt);
+        vec_chunk = gptr(V.b.vec_chunk);
+        claim = gptr(V.b.claim);
+        candidates = gptr(V.b.candidates);
+        counters = gptr(V.b.counters);
+    }
     __shared__ uint32_t lds[4];
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t v = vbx * blockDim.x + threadIdx.x;
     uint32_t before = 0;
-    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += blockDim.x) before += vec_chunk[i];
+    for (uint32_t i = threadIdx.x; i < vbx; i += blockDim.x) before += vec_chunk[i];
     const uint32_t off = block_sum256(before, lds);
     const uint32_t cnt = v < n_vec ? vec_cnt[v] : 0u;
     const uint32_t vbase = off + block_excl_scan256(cnt, lds);
@@ -963,13 +1130,29 @@ __device__ float indirect_variance(const uint32_t *__restrict__ lbuf, uint32_t h
 // Candidates are strided over a fixed grid (the count is known only on the
 // device), so each block merges many candidates into its histogram and
 // min/max before the few global atomics of its flush.
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_rank(FrameParams p, const uint4 *__restrict__ candidates,
                                               const uint32_t *__restrict__ samples_map, const uint32_t *__restrict__ lbuf,
                                               const uint32_t *__restrict__ rank_stats,
                                               uint32_t *__restrict__ rank_part, const uint32_t *__restrict__ counters,
                                               int32_t *__restrict__ nsamp, uint32_t *__restrict__ vec_tot,
-                                              uint32_t *__restrict__ chunk_tot)
+                                              uint32_t *__restrict__ chunk_tot,
+                                              MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        p = global_params(V.p);
+        candidates = gptr(V.b.candidates);
+        samples_map = gptr(V.b.samples);
+        lbuf = gptr(V.b.lbuf);
+        rank_stats = gptr(V.b.rank_stats);
+        rank_part = gptr(V.b.rank_part);
+        counters = gptr(V.b.counters);
+        nsamp = gptr(V.b.nsamp);
+        vec_tot = gptr(V.b.vec_cnt);
+        chunk_tot = gptr(V.b.rank_chunk);
+    }
     __shared__ uint32_t hist[HISTOGRAM_BINS];
     __shared__ uint32_t sum_lds[4];
     __shared__ uint32_t bmin, bmax;
@@ -983,7 +1166,7 @@ __global__ __launch_bounds__(256) void k_rank(FrameParams p, const uint4 *__rest
     const uint32_t n = counters[VE_CNT_CANDIDATES];
     const float tHigh = __uint_as_float(rank_stats[0]);
     const float tMid = __uint_as_float(rank_stats[1]);
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    for (uint32_t base = vbx * blockDim.x; base < n; base += vbn * blockDim.x) {
         const uint32_t t = base + threadIdx.x;
         active += min(n - base, (uint32_t)blockDim.x);
         int32_t s = 0;
@@ -1025,7 +1208,7 @@ __global__ __launch_bounds__(256) void k_rank(FrameParams p, const uint4 *__rest
         atomicMax(&bmax, my_max);
     }
     __syncthreads();
-    uint32_t *part = rank_part + (size_t)(blockIdx.x % RANK_COPIES) * RANK_COPY_WORDS;
+    uint32_t *part = rank_part + (size_t)(vbx % RANK_COPIES) * RANK_COPY_WORDS;
     if (threadIdx.x == 0 && active) {
         atomicAdd(&part[0], active);
         atomicMax(&part[1], ~bmin);   // the complement: a zeroed copy holds "no minimum"
@@ -1065,24 +1248,39 @@ __device__ void rank_fold_copies(uint32_t *__restrict__ rank_stats, uint32_t *__
 
 // Lockstep order of rank.comp's sample loop inside a vector: iteration i
 // appends, in lane order, every lane with more than i samples.
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_rank_write(FrameParams p, const uint4 *__restrict__ candidates,
                                                     const int32_t *__restrict__ nsamp,
                                                     const uint32_t *__restrict__ vec_tot,
                                                     const uint32_t *__restrict__ chunk_tot, uint2 *__restrict__ shade_list,
                                                     uint32_t *__restrict__ counters, uint32_t *__restrict__ rank_stats,
-                                                    uint32_t *__restrict__ rank_part)
+                                                    uint32_t *__restrict__ rank_part,
+                                                    MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        p = global_params(V.p);
+        candidates = gptr(V.b.candidates);
+        nsamp = gptr(V.b.nsamp);
+        vec_tot = gptr(V.b.vec_cnt);
+        chunk_tot = gptr(V.b.rank_chunk);
+        shade_list = gptr(V.b.shade_list);
+        counters = gptr(V.b.counters);
+        rank_stats = gptr(V.b.rank_stats);
+        rank_part = gptr(V.b.rank_part);
+    }
     static_assert(HISTOGRAM_BINS == 256, "the fold takes one bin per thread of a 256-thread block");
     __shared__ uint32_t lds[4];
-    if (blockIdx.x == 0) rank_fold_copies(rank_stats, rank_part);
+    if (vbx == 0) rank_fold_copies(rank_stats, rank_part);
     const uint32_t n = counters[VE_CNT_CANDIDATES];
-    if (blockIdx.x * blockDim.x >= n) return;   // the whole block
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (vbx * blockDim.x >= n) return;   // the whole block
+    const uint32_t t = vbx * blockDim.x + threadIdx.x;
     const uint32_t v = t >> 3, l = t & 7u, v0 = v * 8u;
     // the vector's first shade-list index: the totals of the earlier 256-
     // candidate chunks plus the vectors before it in this one
     uint32_t before = 0;
-    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += blockDim.x) before += chunk_tot[i];
+    for (uint32_t i = threadIdx.x; i < vbx; i += blockDim.x) before += chunk_tot[i];
     const uint32_t off = block_sum256(before, lds);
     const uint32_t mine = (l == 0u && t < n) ? vec_tot[v] : 0u;
     const uint32_t excl = block_excl_scan256(mine, lds);
@@ -1187,22 +1385,42 @@ __device__ __forceinline__ bool emissive_unknown(const uint32_t *__restrict__ em
 // position, size 1, id, material, emissive flag) and marks it in `hit_bits`;
 // the trace pass then leaves it alone.  The ray still counts as traced (the
 // reference traverses it again); `VE_CNT_REUSED` and ray total 3 count it.
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_emit(FrameParams p, const uint2 *__restrict__ shade_list,
                                               Ray *__restrict__ rays, uint32_t *__restrict__ counters,
                                               const uint4 *__restrict__ gbuf, const uint2 *__restrict__ ghit,
                                               const Material *__restrict__ mats, const uint32_t *__restrict__ em_claim,
                                               uint32_t *__restrict__ em_bits, uint32_t *__restrict__ hit_bits,
                                               unsigned long long *__restrict__ totals, uint32_t *__restrict__ ray_li,
-                                              uint32_t *__restrict__ death, uint32_t *__restrict__ rank_stats)
+                                              uint32_t *__restrict__ death, uint32_t *__restrict__ rank_stats,
+                                              MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        p = global_params(V.p);
+        shade_list = gptr(V.b.shade_list);
+        rays = gptr(V.b.rays);
+        counters = gptr(V.b.counters);
+        gbuf = gptr(V.b.gbuf);
+        ghit = gptr(V.b.ghit);
+        mats = gptr(V.b.materials);
+        em_claim = gptr(V.b.emissive_claim);
+        em_bits = gptr(V.b.em_bits);
+        hit_bits = gptr(V.b.hit_bits);
+        totals = gptr(V.b.ray_totals);
+        ray_li = gptr(V.b.ray_li[0]);
+        death = gptr(V.b.dep_death);
+        rank_stats = gptr(V.b.rank_stats);
+    }
     // the threshold pass, folded: it reads the rank pass's histogram, done
     // before this launch, and writes thresholds only the next frame's rank
     // pass reads
-    if (p.fold_small_passes && blockIdx.x == 0 && threadIdx.x < 64u) threshold_wave(p, rank_stats, counters);
+    if (p.fold_small_passes && vbx == 0 && threadIdx.x < 64u) threshold_wave(p, rank_stats, counters);
     __shared__ uint32_t block_reused;
     if (threadIdx.x == 0) block_reused = 0u;
     __syncthreads();
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = vbx * blockDim.x + threadIdx.x;
     const uint32_t n = counters[VE_CNT_SHADE];
     if (t == 0) counters[VE_CNT_LIVE] = n;
     bool reused = false;
@@ -1285,17 +1503,35 @@ __device__ __forceinline__ void trace_ray(const uint32_t *__restrict__ words, co
     }
 }
 
-template <bool STATS>
+template <bool STATS, bool MULTI>
 __global__ __launch_bounds__(256) void k_trace(const uint32_t *__restrict__ words, FrameParams p, Ray *__restrict__ rays,
                                                const uint32_t *__restrict__ live_ptr, uint32_t iter,
                                                uint32_t *__restrict__ counters, unsigned long long *__restrict__ stats,
                                                unsigned long long *__restrict__ totals,
                                                const Material *__restrict__ mats,
                                                const uint32_t *__restrict__ em_claim, uint32_t *__restrict__ em_bits,
-                                               uint32_t *__restrict__ live_out, const uint32_t *__restrict__ hit_bits)
+                                               uint32_t *__restrict__ live_out, const uint32_t *__restrict__ hit_bits,
+                                               MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        words = gptr(V.words);
+        p = global_params(V.p);
+        rays = gptr(V.b.rays);
+        live_ptr = gptr(V.b.counters) + VE_CNT_LIVE;
+        iter = 0u;
+        counters = gptr(V.b.counters);
+        stats = gptr(V.b.node_stats);
+        totals = gptr(V.b.ray_totals);
+        mats = gptr(V.b.materials);
+        em_claim = gptr(V.b.emissive_claim);
+        em_bits = gptr(V.b.em_bits);
+        live_out = gptr(V.b.counters) + VE_CNT_LIVE_B;
+        hit_bits = gptr(V.b.hit_bits);
+    }
     const uint32_t live = *live_ptr;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = vbx * blockDim.x + threadIdx.x;
     // the shade pass of this iteration appends its survivors from zero; the
     // counter was last read by the previous iteration
     if (t == 0) *live_out = 0u;
@@ -1417,11 +1653,25 @@ constexpr uint32_t EM_BATCH = EM_THREADS;        // flagged rays taken at once, 
 constexpr uint32_t EM_KEYS = 2u * EM_BATCH;      // LDS open-addressed tables of the batch's ids and rows
 constexpr uint32_t EM_SERIAL = 24;               // up to this many: one thread
 static_assert(SHADE_BUDGET_MAX % (32u * 4u) == 0u, "the bitmap is read four words at a time");
+template <bool MULTI>
 __global__ __launch_bounds__(EM_THREADS) void k_em_register(const Ray *__restrict__ rays, const uint32_t *__restrict__ live_ptr,
                                                       const Material *__restrict__ mats, uint32_t *__restrict__ em_claim,
                                                       EmissiveVoxel *__restrict__ emissives, uint32_t *__restrict__ em_bits,
-                                                      uint32_t *__restrict__ em_reg, uint32_t *__restrict__ counters)
+                                                      uint32_t *__restrict__ em_reg, uint32_t *__restrict__ counters,
+                                                            MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        rays = gptr(V.b.rays);
+        live_ptr = gptr(V.b.counters) + VE_CNT_LIVE;
+        mats = gptr(V.b.materials);
+        em_claim = gptr(V.b.emissive_claim);
+        emissives = gptr(V.b.emissives);
+        em_bits = gptr(V.b.em_bits);
+        em_reg = gptr(V.b.em_reg);
+        counters = gptr(V.b.counters);
+    }
     static_assert(EM_KEYS >= 2u * EM_BATCH, "the key tables stay at most half full");
     __shared__ uint32_t wsum[EM_THREADS / 64u];
     __shared__ uint32_t list_ri[EM_BATCH], e_vid[EM_BATCH], e_slot[EM_BATCH], e_claim[EM_BATCH], seg[EM_BATCH];
@@ -2055,6 +2305,7 @@ __device__ __forceinline__ void record_deposit(bool dep, const Ray &r, uint32_t 
                                   {r.dlx, r.dly, r.dlz}, {r.ilx, r.ily, r.ilz}};
 }
 
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_shade(FrameParams p, const uint32_t *__restrict__ words, uint32_t iter,
                                                const Ray *__restrict__ rays_in, const uint32_t *__restrict__ live_in,
                                                const uint32_t *__restrict__ li_in, Ray *__restrict__ rays_out,
@@ -2063,12 +2314,34 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, const uint32_t *__
                                                uint32_t *__restrict__ rbuf, const Material *__restrict__ mats,
                                                const EmissiveVoxel *__restrict__ emissives,
                                                const uint32_t *__restrict__ em_reg, uint32_t *__restrict__ counters,
-                                               unsigned long long *__restrict__ totals)
+                                               unsigned long long *__restrict__ totals,
+                                               MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        p = global_params(V.p);
+        words = gptr(V.words);
+        iter = 0u;
+        rays_in = gptr(V.b.rays);
+        live_in = gptr(V.b.counters) + VE_CNT_LIVE;
+        li_in = gptr(V.b.ray_li[0]);
+        rays_out = gptr(V.b.rays_next);
+        live_out = gptr(V.b.counters) + VE_CNT_LIVE_B;
+        li_out = gptr(V.b.ray_li[1]);
+        deferred = gptr(V.b.deferred);
+        nbuf = gptr(V.b.nbuf);
+        rbuf = gptr(V.b.rbuf);
+        mats = gptr(V.b.materials);
+        emissives = gptr(V.b.emissives);
+        em_reg = gptr(V.b.em_reg);
+        counters = gptr(V.b.counters);
+        totals = gptr(V.b.ray_totals);
+    }
     __shared__ uint32_t path_cache[VE_MAX_DEPTH * 256];
     NeeCtx nee{words, emissives, rbuf, path_cache + threadIdx.x, 0u};
     const uint32_t live = *live_in;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = vbx * blockDim.x + threadIdx.x;
     bool survive = false;
     int outcome = RAY_DONE;
     uint32_t li = 0u;
@@ -2127,7 +2400,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameParams p, const uint32_t *__
 // its sums to one of BOUNCE_COPIES copies at the end (k_dep_prep folds
 // them), since per-iteration atomics on one counter from every wave cost the
 // traversals a quarter of their time.
-template <bool STATS, bool REFILL>
+template <bool STATS, bool REFILL, bool MULTI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_bounce(const uint32_t *__restrict__ words, FrameParams p,
                                                 const Ray *__restrict__ rays_in, const uint32_t *__restrict__ live_ptr,
                                                 uint32_t first, uint32_t iters, uint32_t *__restrict__ counters,
@@ -2138,11 +2411,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                                                 const uint32_t *__restrict__ em_reg,
                                                 const uint32_t *__restrict__ em_claim, uint32_t *__restrict__ em_bits,
                                                 const uint32_t *__restrict__ li_in,
-                                                Deferred *__restrict__ deferred, uint32_t *__restrict__ bounce_part)
+                                                Deferred *__restrict__ deferred, uint32_t *__restrict__ bounce_part,
+                                                                                        MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        words = gptr(V.words);
+        p = global_params(V.p);
+        rays_in = gptr(V.b.rays_next);
+        live_ptr = gptr(V.b.counters) + VE_CNT_LIVE_B;
+        first = 1u;
+        iters = V.iters;
+        counters = gptr(V.b.counters);
+        stats = gptr(V.b.node_stats);
+        totals = gptr(V.b.ray_totals);
+        mats = gptr(V.b.materials);
+        nbuf = gptr(V.b.nbuf);
+        rbuf = gptr(V.b.rbuf);
+        emissives = gptr(V.b.emissives);
+        em_reg = gptr(V.b.em_reg);
+        em_claim = gptr(V.b.emissive_claim);
+        em_bits = gptr(V.b.em_bits);
+        li_in = gptr(V.b.ray_li[1]);
+        deferred = gptr(V.b.deferred);
+        bounce_part = gptr(V.b.bounce_part);
+    }
     const uint32_t live = *live_ptr;
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, threads = gridDim.x * blockDim.x;
-    if (blockIdx.x * blockDim.x >= live) return;   // whole block past the live rays
+    const uint32_t tid = vbx * blockDim.x + threadIdx.x, threads = vbn * blockDim.x;
+    if (vbx * blockDim.x >= live) return;   // whole block past the live rays
     __shared__ uint32_t path_cache[VE_MAX_DEPTH * 256];
     NeeCtx nee{words, emissives, rbuf, path_cache + threadIdx.x, 0u};
     const uint32_t lane = threadIdx.x & 63u;
@@ -2264,12 +2561,35 @@ struct DepLists {
     uint32_t cap_total;    // over / spill capacity (SHADE_BUDGET_MAX)
 };
 
+__host__ __device__ inline DepLists dep_lists(const DeviceBuffers &b)
+{
+    return DepLists{DEP_CAP, b.dep_cnt, b.dep_slots, b.dep_over, b.dep_over_n, b.dep_spill, SHADE_BUDGET_MAX};
+}
+__device__ __forceinline__ DepLists dep_lists_global(const DeviceBuffers &b)
+{
+    return DepLists{DEP_CAP, gptr(b.dep_cnt), gptr(b.dep_slots), gptr(b.dep_over), gptr(b.dep_over_n),
+                    gptr(b.dep_spill), SHADE_BUDGET_MAX};
+}
+
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_dep_prep(FrameParams p, const Deferred *__restrict__ deferred,
                                                   uint32_t *__restrict__ counters, uint32_t *__restrict__ bounce_part,
                                                   unsigned long long *__restrict__ totals, uint32_t *__restrict__ death,
-                                                  DepLists dl)
+                                                  DepLists dl,
+                                                  MultiTab mt)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {   // the fused bounce pass's per-wave ray counts
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        p = global_params(V.p);
+        deferred = gptr(V.b.deferred);
+        counters = gptr(V.b.counters);
+        bounce_part = gptr(V.b.bounce_part);
+        totals = gptr(V.b.ray_totals);
+        death = gptr(V.b.dep_death);
+        dl = dep_lists_global(V.b);
+    }
+    if (vbx == 0 && threadIdx.x == 0) {   // the fused bounce pass's per-wave ray counts
         uint32_t rays = 0, reached = 0;
         for (uint32_t c = 0; c < BOUNCE_COPIES; c++) {
             rays += bounce_part[2u * c];
@@ -2282,7 +2602,7 @@ __global__ __launch_bounds__(256) void k_dep_prep(FrameParams p, const Deferred 
         counters[VE_CNT_BOUNCES_RUN] = max(counters[VE_CNT_BOUNCES_RUN], reached);
     }
     const uint32_t n = min(counters[VE_CNT_DEFERRED], SHADE_BUDGET_MAX);
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    for (uint32_t e = vbx * blockDim.x + threadIdx.x; e < n; e += vbn * blockDim.x) {
         const Deferred d = deferred[e];
         death[d.meta & 0x3FFFFu] = (d.meta >> 18) & 0xFu;
         if (d.vid == 0u) continue;
@@ -2299,14 +2619,24 @@ __global__ __launch_bounds__(256) void k_dep_prep(FrameParams p, const Deferred 
 // chunk's earlier entries are counted by every block (at most 47 x 1024
 // loads a thread-block, all in flight), the block's own with ballots.
 constexpr uint32_t WPOS_BLOCKS = RAY_CHUNK / 1024u;
+template <bool MULTI>
 __global__ __launch_bounds__(1024) void k_dep_wpos(const uint32_t *__restrict__ counters,
                                                    const uint32_t *__restrict__ death, uint32_t *__restrict__ wpos,
-                                                   uint32_t iters)
+                                                   uint32_t iters,
+                                                   MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        counters = gptr(V.b.counters);
+        death = gptr(V.b.dep_death);
+        wpos = gptr(V.b.dep_wpos);
+        iters = V.iters;
+    }
     __shared__ uint32_t wsum[16][MAX_BOUNCE_ITERS];
     __shared__ uint32_t before_blk[MAX_BOUNCE_ITERS];
     const uint32_t ns = counters[VE_CNT_SHADE], t = threadIdx.x, lane = t & 63u, w = t >> 6;
-    const uint32_t c = blockIdx.x / WPOS_BLOCKS, k = blockIdx.x % WPOS_BLOCKS;
+    const uint32_t c = vbx / WPOS_BLOCKS, k = vbx % WPOS_BLOCKS;
     const uint32_t lo = c * RAY_CHUNK, i = lo + k * 1024u + t;
     if (lo + k * 1024u >= ns) return;
     const uint64_t below = (1ull << lane) - 1ull;
@@ -2506,17 +2836,29 @@ __device__ __forceinline__ uint32_t dep_key(const Deferred *__restrict__ deferre
     return dep_key_meta(deferred[rec].meta, wpos);
 }
 
+template <bool MULTI>
 __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_dep_replay(FrameParams p, uint32_t *__restrict__ lbuf,
                                                                   DepLists dl, const Deferred *__restrict__ deferred,
                                                                   const uint32_t *__restrict__ wpos,
-                                                                  uint32_t *__restrict__ counters)
+                                                                  uint32_t *__restrict__ counters,
+                                                                  MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        p = global_params(V.p);
+        lbuf = gptr(V.b.lbuf);
+        dl = dep_lists_global(V.b);
+        deferred = gptr(V.b.deferred);
+        wpos = gptr(V.b.dep_wpos);
+        counters = gptr(V.b.counters);
+    }
     // a bucket's list sorts in LDS only while it fits in place (n <= DEP_CAP):
     // beyond, its keys go to the global spill
     __shared__ uint64_t lds_keys_all[REPLAY_WAVES][DEP_CAP];
     __shared__ __align__(16) uint32_t bk_all[REPLAY_WAVES][LK_WORDS];
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    const uint32_t h = blockIdx.x * REPLAY_WAVES + wv;
+    const uint32_t h = vbx * REPLAY_WAVES + wv;
     if (h >= p.hash_width) return;
     const uint32_t n = dl.cnt[h];
     if (n == 0u) return;
@@ -2713,12 +3055,26 @@ __device__ Avg voxel_avg(const uint32_t *__restrict__ lbuf, uint32_t hw, uint32_
 // timeout gather of part 2 read these per-pixel values instead of probing the
 // lBuffer hash again for each tap (up to 440 taps for a timed-out pixel).
 // Lanes of a tile that see the same voxel share one lookup.
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_resolve_avg(FrameParams p, const uint4 *__restrict__ gbuf,
                                                      const uint32_t *__restrict__ lbuf, float4 *__restrict__ pix_avg,
                                                      uint32_t *__restrict__ pix_cnt, uint32_t *__restrict__ counters,
-                                                     uint32_t *__restrict__ tcount, uint32_t *__restrict__ dep_over_n)
+                                                     uint32_t *__restrict__ tcount, uint32_t *__restrict__ dep_over_n,
+                                                     MultiTab mt)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {   // k_resolve appends after this pass
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        p = global_params(V.p);
+        gbuf = gptr(V.b.gbuf);
+        lbuf = gptr(V.b.lbuf);
+        pix_avg = gptr(V.b.pix_avg);
+        pix_cnt = gptr(V.b.pix_cnt);
+        counters = gptr(V.b.counters);
+        tcount = gptr(V.b.timeout_tiles);
+        dep_over_n = gptr(V.b.dep_over_n);
+    }
+    if (vbx == 0 && threadIdx.x == 0) {   // k_resolve appends after this pass
         counters[VE_CNT_TIMEOUTS] = 0u;
         counters[VE_CNT_TIMEOUT_TILES] = 0u;
         if (p.fold_small_passes) {   // k_dep_done, folded: the frame's deposit records are applied
@@ -2728,10 +3084,10 @@ __global__ __launch_bounds__(256) void k_resolve_avg(FrameParams p, const uint4 
     }
     {   // and counts timed-out pixels per 16x16 tile from zero
         const uint32_t n_t = ((p.W + RECON_TILE - 1u) / RECON_TILE) * ((p.H + RECON_TILE - 1u) / RECON_TILE);
-        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_t; i += gridDim.x * blockDim.x) tcount[i] = 0u;
+        for (uint32_t i = vbx * blockDim.x + threadIdx.x; i < n_t; i += vbn * blockDim.x) tcount[i] = 0u;
     }
     uint32_t x = 0, y = 0;
-    const bool valid = tile_pixel(p, x, y);
+    const bool valid = tile_pixel(vbx, vbn, p, x, y);
     const size_t pix = valid ? (size_t)y * p.W + x : 0;
     const uint32_t vid = valid ? gbuf[pix].w : 0u;
     // leader = lowest lane holding the same id; leaders look up in parallel
@@ -2762,15 +3118,33 @@ __global__ __launch_bounds__(256) void k_resolve_avg(FrameParams p, const uint4 
 // AvgPass part 2 (resolve.comp:362-382): per-pixel colour of the selected view.
 // Timed-out pixels (reconstructTimeout) are appended to `timeouts` and
 // coloured by k_resolve_timeouts.
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_resolve(FrameParams p, const uint4 *__restrict__ gbuf,
                                                  const uint32_t *__restrict__ lbuf, const uint32_t *__restrict__ nbuf,
                                                  const Material *__restrict__ mats, const float4 *__restrict__ pix_avg,
                                                  const uint32_t *__restrict__ pix_cnt, float4 *__restrict__ out,
                                                  uint32_t *__restrict__ timeouts, uint32_t *__restrict__ counters,
-                                                 uint32_t *__restrict__ tflag, uint32_t *__restrict__ tlist)
+                                                 uint32_t *__restrict__ tflag, uint32_t *__restrict__ tlist,
+                                                 MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        p = global_params(V.p);
+        gbuf = gptr(V.b.gbuf);
+        lbuf = gptr(V.b.lbuf);
+        nbuf = gptr(V.b.nbuf);
+        mats = gptr(V.b.materials);
+        pix_avg = gptr(V.b.pix_avg);
+        pix_cnt = gptr(V.b.pix_cnt);
+        out = gptr(V.b.output);
+        timeouts = gptr(V.b.timeouts);
+        counters = gptr(V.b.counters);
+        tflag = gptr(V.b.timeout_tiles);
+        tlist = gptr(V.b.timeout_list);
+    }
     uint32_t x = 0, y = 0;
-    const bool valid = tile_pixel(p, x, y);
+    const bool valid = tile_pixel(vbx, vbn, p, x, y);
     const size_t pix = valid ? (size_t)y * p.W + x : 0;
     const uint4 g = valid ? gbuf[pix] : make_uint4(0u, 0u, 0u, 0u);
     const uint32_t checks = g.x, mi = g.y, vid = g.w;
@@ -3031,6 +3405,7 @@ __device__ __forceinline__ void resolve_timeout_list(FrameParams p, const uint4 
 // `list_blocks` take the listed pixels one wave each, the rest the dense
 // 16x16 tiles from LDS (with the tiled path on; the two roles write disjoint
 // pixels and only read the per-tile counts).
+template <bool MULTI>
 __global__ __launch_bounds__(256) void k_resolve_timeouts(FrameParams p, const uint4 *__restrict__ gbuf,
                                                           const float4 *__restrict__ pix_avg,
                                                           const uint32_t *__restrict__ pix_cnt,
@@ -3038,13 +3413,28 @@ __global__ __launch_bounds__(256) void k_resolve_timeouts(FrameParams p, const u
                                                           uint32_t *__restrict__ tflag,
                                                           const uint32_t *__restrict__ tlist,
                                                           const uint32_t *__restrict__ counters,
-                                                          float4 *__restrict__ out, uint32_t list_blocks)
+                                                          float4 *__restrict__ out, uint32_t list_blocks,
+                                                          MultiTab mt)
 {
-    if (blockIdx.x < list_blocks)
-        resolve_timeout_list(p, gbuf, pix_avg, pix_cnt, timeouts, counters, tflag, out, blockIdx.x, list_blocks);
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        p = global_params(V.p);
+        gbuf = gptr(V.b.gbuf);
+        pix_avg = gptr(V.b.pix_avg);
+        pix_cnt = gptr(V.b.pix_cnt);
+        timeouts = gptr(V.b.timeouts);
+        tflag = gptr(V.b.timeout_tiles);
+        tlist = gptr(V.b.timeout_list);
+        counters = gptr(V.b.counters);
+        out = gptr(V.b.output);
+        list_blocks = RESOLVE_TIMEOUT_BLOCKS;
+    }
+    if (vbx < list_blocks)
+        resolve_timeout_list(p, gbuf, pix_avg, pix_cnt, timeouts, counters, tflag, out, vbx, list_blocks);
     else
-        resolve_timeout_tiles(p, gbuf, pix_avg, pix_cnt, tflag, tlist, counters, out, blockIdx.x - list_blocks,
-                              gridDim.x - list_blocks);
+        resolve_timeout_tiles(p, gbuf, pix_avg, pix_cnt, tflag, tlist, counters, out, vbx - list_blocks,
+                              vbn - list_blocks);
 }
 
 // RenderPass (final.frag:11-19): ACES filmic curve per channel, clamped to
@@ -3083,28 +3473,32 @@ void launch_fill(hipStream_t s, uint32_t *p, uint32_t v, size_t n)
     if (n) launch(k_fill_u32, dim3(cdiv(n, 256)), dim3(256), 0, s, p, v, n);
 }
 
-static OrderLists nrm_lists(const DeviceBuffers &b)
-{
-    return OrderLists{b.ord_cap, b.ord_over_cap, b.ord_cnt, b.ord_ev, b.ord_over, b.ord_over_n, b.ord_spill,
-                      b.ord_spill_cap};
-}
-
 void launch_normal(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b)
 {
     const OrderLists ol = nrm_lists(b);
-    launch(k_nrm_events, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.gpos, ol, b.counters);
-    launch(k_nrm_replay, dim3(cdiv(p.hash_width, REPLAY_WAVES)), dim3(64 * REPLAY_WAVES), 0, s, p, b.nbuf, ol,
-           b.ord_work, b.counters);
-    launch(k_nrm_compute, dim3(cdiv((size_t)p.hash_width * NBUFFER_SLOTS, 256)), dim3(256), 0, s, words, p, b.nbuf,
-           b.ord_work, b.ord_over_n);
+    launch(k_nrm_events<false>, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.gpos, ol, b.counters, MultiTab{});
+    launch(k_nrm_replay<false>, dim3(cdiv(p.hash_width, REPLAY_WAVES)), dim3(64 * REPLAY_WAVES), 0, s, p, b.nbuf, ol,
+           b.ord_work, b.counters, MultiTab{});
+    launch(k_nrm_compute<false>, dim3(cdiv((size_t)p.hash_width * NBUFFER_SLOTS, 256)), dim3(256), 0, s, words, p, b.nbuf,
+           b.ord_work, b.ord_over_n, MultiTab{});
 }
 
 // Per-frame clear of RankStats' per-frame half and the counters the frame
 // accumulates into (renderer.cpp:343-375); fill kernels, not memsets, so the
 // sequence stays capturable.
+template <bool MULTI>
 __global__ void k_frame_reset(FrameParams p, uint32_t *__restrict__ rank_stats, uint32_t *__restrict__ counters,
-                              unsigned long long *__restrict__ totals)
+                              unsigned long long *__restrict__ totals,
+                              MultiTab mt)
 {
+    uint32_t vbx = blockIdx.x, vbn = gridDim.x;
+    if constexpr (MULTI) {
+        const ViewRec &V = multi_view(mt, vbx, vbn);
+        p = global_params(V.p);
+        rank_stats = gptr(V.b.rank_stats);
+        counters = gptr(V.b.counters);
+        totals = gptr(V.b.ray_totals);
+    }
     const uint32_t i = threadIdx.x;
     const uint32_t primary = p.count_primary_in_reset ? p.W * p.H : 0u;   // one primary ray per pixel
     if (i < 4) rank_stats[4 + i] = i == 0 ? 0x7F7FFFFFu : 0u;
@@ -3115,7 +3509,7 @@ __global__ void k_frame_reset(FrameParams p, uint32_t *__restrict__ rank_stats, 
 
 void launch_reset(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 {
-    launch(k_frame_reset, dim3(1), dim3(256), 0, s, p, b.rank_stats, b.counters, b.ray_totals);
+    launch(k_frame_reset<false>, dim3(1), dim3(256), 0, s, p, b.rank_stats, b.counters, b.ray_totals, MultiTab{});
 }
 
 int launch_schedule(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
@@ -3123,16 +3517,16 @@ int launch_schedule(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
     const uint32_t gw = (p.W + 7u) / 8u, gh = (p.H + 7u) / 8u;
     const uint32_t groups = gw * gh;
     const uint32_t n_vec = groups * 8u;
-    launch(k_sched_claim, dim3(groups), dim3(64), 0, s, p, b.gbuf, b.sched_tab, b.cand_at, b.counters, b.sched_over,
-           b.sched_over_n, b.sched_row_over, b.sched_rows);
-    launch(k_sched_rank, dim3(CLAIM_WIDTH), dim3(64), 0, s, b.sched_tab, b.claim, b.samples, b.cand_at,
-           b.sched_row_over);
-    launch(k_sched_rank_over, dim3(SCHED_OVER_BLOCKS), dim3(256), 0, s, b.sched_tab, b.claim, b.samples, b.cand_at,
-           b.sched_over, b.sched_over_n, b.sched_row_over, b.sched_rows, b.counters);
-    launch(k_sched_vec_count, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, b.cand_at, n_vec, b.vec_cnt, b.vec_chunk,
-           b.sched_over_n);
-    launch(k_sched_write, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, p, b.cand_at, n_vec, b.vec_cnt, b.vec_chunk,
-           b.claim, b.candidates, b.counters);
+    launch(k_sched_claim<false>, dim3(groups), dim3(64), 0, s, p, b.gbuf, b.sched_tab, b.cand_at, b.counters, b.sched_over,
+           b.sched_over_n, b.sched_row_over, b.sched_rows, MultiTab{});
+    launch(k_sched_rank<false>, dim3(CLAIM_WIDTH), dim3(64), 0, s, b.sched_tab, b.claim, b.samples, b.cand_at,
+           b.sched_row_over, MultiTab{});
+    launch(k_sched_rank_over<false>, dim3(SCHED_OVER_BLOCKS), dim3(256), 0, s, b.sched_tab, b.claim, b.samples, b.cand_at,
+           b.sched_over, b.sched_over_n, b.sched_row_over, b.sched_rows, b.counters, MultiTab{});
+    launch(k_sched_vec_count<false>, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, b.cand_at, n_vec, b.vec_cnt, b.vec_chunk,
+           b.sched_over_n, MultiTab{});
+    launch(k_sched_write<false>, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, p, b.cand_at, n_vec, b.vec_cnt, b.vec_chunk,
+           b.claim, b.candidates, b.counters, MultiTab{});
     return VE_OK;
 }
 
@@ -3140,10 +3534,10 @@ int launch_rank(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 {
     // candidates <= distinct visible voxels <= pixels
     const uint32_t bound = (uint32_t)std::min<size_t>((size_t)p.W * p.H, CANDIDATE_CAPACITY);
-    launch(k_rank, dim3(std::min<uint32_t>(cdiv(bound, 256), RANK_BLOCKS)), dim3(256), 0, s, p, b.candidates,
-                       b.samples, b.lbuf, b.rank_stats, b.rank_part, b.counters, b.nsamp, b.vec_cnt, b.rank_chunk);
-    launch(k_rank_write, dim3(cdiv(bound, 256)), dim3(256), 0, s, p, b.candidates, b.nsamp, b.vec_cnt,
-                       b.rank_chunk, b.shade_list, b.counters, b.rank_stats, b.rank_part);
+    launch(k_rank<false>, dim3(std::min<uint32_t>(cdiv(bound, 256), RANK_BLOCKS)), dim3(256), 0, s, p, b.candidates,
+                       b.samples, b.lbuf, b.rank_stats, b.rank_part, b.counters, b.nsamp, b.vec_cnt, b.rank_chunk, MultiTab{});
+    launch(k_rank_write<false>, dim3(cdiv(bound, 256)), dim3(256), 0, s, p, b.candidates, b.nsamp, b.vec_cnt,
+                       b.rank_chunk, b.shade_list, b.counters, b.rank_stats, b.rank_part, MultiTab{});
     return VE_OK;
 }
 
@@ -3154,47 +3548,33 @@ void launch_threshold(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 
 void launch_emit(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
 {
-    launch(k_emit, dim3(cdiv(p.shade_budget, 256)), dim3(256), 0, s, p, b.shade_list, b.rays, b.counters, b.gbuf, b.ghit,
-           b.materials, b.emissive_claim, b.em_bits, b.hit_bits, b.ray_totals, b.ray_li[0], b.dep_death, b.rank_stats);
+    launch(k_emit<false>, dim3(cdiv(p.shade_budget, 256)), dim3(256), 0, s, p, b.shade_list, b.rays, b.counters, b.gbuf, b.ghit,
+           b.materials, b.emissive_claim, b.em_bits, b.hit_bits, b.ray_totals, b.ray_li[0], b.dep_death, b.rank_stats, MultiTab{});
 }
 
-// Bounce iteration `iter` of the trace/shade loop (renderer.cpp:515-602); the
-// ping-pong direction alternates with `iter`.
-static inline Ray *bounce_in(DeviceBuffers &b, uint32_t iter) { return (iter & 1u) ? b.rays_next : b.rays; }
-static inline Ray *bounce_out(DeviceBuffers &b, uint32_t iter) { return (iter & 1u) ? b.rays : b.rays_next; }
-static inline uint32_t *live_in_ptr(DeviceBuffers &b, uint32_t iter)
-{
-    return b.counters + ((iter & 1u) ? VE_CNT_LIVE_B : VE_CNT_LIVE);
-}
-static inline uint32_t *live_out_ptr(DeviceBuffers &b, uint32_t iter)
-{
-    return b.counters + ((iter & 1u) ? VE_CNT_LIVE : VE_CNT_LIVE_B);
-}
-static inline uint32_t *li_in(DeviceBuffers &b, uint32_t iter) { return b.ray_li[iter & 1u]; }
-static inline uint32_t *li_out(DeviceBuffers &b, uint32_t iter) { return b.ray_li[(iter & 1u) ^ 1u]; }
 
 void launch_trace(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t iter, bool stats)
 {
     const uint32_t nb = cdiv(p.shade_budget, 256);
     if (stats)
-        launch(k_trace<true>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, iter), live_in_ptr(b, iter),
+        launch(k_trace<true, false>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, iter), live_in_ptr(b, iter),
                            iter, b.counters, b.node_stats, b.ray_totals, b.materials, b.emissive_claim, b.em_bits,
-                           live_out_ptr(b, iter), b.hit_bits);
+                           live_out_ptr(b, iter), b.hit_bits, MultiTab{});
     else
-        launch(k_trace<false>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, iter),
+        launch(k_trace<false, false>, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, iter),
                            live_in_ptr(b, iter), iter, b.counters, b.node_stats, b.ray_totals, b.materials,
-                           b.emissive_claim, b.em_bits, live_out_ptr(b, iter), b.hit_bits);
+                           b.emissive_claim, b.em_bits, live_out_ptr(b, iter), b.hit_bits, MultiTab{});
     if (iter == 0)
-        launch(k_em_register, dim3(1), dim3(EM_THREADS), 0, s, bounce_in(b, iter), live_in_ptr(b, iter),
-                           b.materials, b.emissive_claim, b.emissives, b.em_bits, b.em_reg, b.counters);
+        launch(k_em_register<false>, dim3(1), dim3(EM_THREADS), 0, s, bounce_in(b, iter), live_in_ptr(b, iter),
+                           b.materials, b.emissive_claim, b.emissives, b.em_bits, b.em_reg, b.counters, MultiTab{});
 }
 
 void launch_shade(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t iter)
 {
     const uint32_t nb = cdiv(p.shade_budget, 256);
-    launch(k_shade, dim3(nb), dim3(256), 0, s, p, words, iter, bounce_in(b, iter), live_in_ptr(b, iter),
-           li_in(b, iter), bounce_out(b, iter), live_out_ptr(b, iter), li_out(b, iter), b.deferred, b.nbuf, b.rbuf,
-           b.materials, b.emissives, b.em_reg, b.counters, b.ray_totals);
+    launch(k_shade<false>, dim3(nb), dim3(256), 0, s, p, words, iter, bounce_in(b, iter), live_in_ptr(b, iter),
+           li_in_ptr(b, iter), bounce_out(b, iter), live_out_ptr(b, iter), li_out_ptr(b, iter), b.deferred, b.nbuf, b.rbuf,
+           b.materials, b.emissives, b.em_reg, b.counters, b.ray_totals, MultiTab{});
 }
 
 void launch_bounce(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, uint32_t first,
@@ -3205,22 +3585,22 @@ void launch_bounce(hipStream_t s, const uint32_t *words, const FrameParams &p, D
     auto go = [&](auto kernel) {
         launch(kernel, dim3(nb), dim3(256), 0, s, words, p, bounce_in(b, first), live_in_ptr(b, first), first, iters,
                b.counters, b.node_stats, b.ray_totals, b.materials, b.nbuf, b.rbuf, b.emissives, b.em_reg,
-               b.emissive_claim, b.em_bits, li_in(b, first), b.deferred, b.bounce_part);
+               b.emissive_claim, b.em_bits, li_in_ptr(b, first), b.deferred, b.bounce_part, MultiTab{});
     };
     const bool refill = nb < full;
-    if (stats) refill ? go(k_bounce<true, true>) : go(k_bounce<true, false>);
-    else refill ? go(k_bounce<false, true>) : go(k_bounce<false, false>);
+    if (stats) refill ? go(k_bounce<true, true, false>) : go(k_bounce<true, false, false>);
+    else refill ? go(k_bounce<false, true, false>) : go(k_bounce<false, false, false>);
 }
 
 int launch_deposits(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_t iters)
 {
     const uint32_t chunks = (p.shade_budget + RAY_CHUNK - 1u) / RAY_CHUNK;
-    const DepLists dl{DEP_CAP, b.dep_cnt, b.dep_slots, b.dep_over, b.dep_over_n, b.dep_spill, SHADE_BUDGET_MAX};
-    launch(k_dep_prep, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, p, b.deferred, b.counters, b.bounce_part, b.ray_totals,
-           b.dep_death, dl);
-    launch(k_dep_wpos, dim3(chunks * WPOS_BLOCKS), dim3(1024), 0, s, b.counters, b.dep_death, b.dep_wpos, iters);
-    launch(k_dep_replay, dim3(cdiv(p.hash_width, REPLAY_WAVES)), dim3(64 * REPLAY_WAVES), 0, s, p, b.lbuf, dl,
-           b.deferred, b.dep_wpos, b.counters);
+    const DepLists dl = dep_lists(b);
+    launch(k_dep_prep<false>, dim3(DEFERRED_BLOCKS), dim3(256), 0, s, p, b.deferred, b.counters, b.bounce_part, b.ray_totals,
+           b.dep_death, dl, MultiTab{});
+    launch(k_dep_wpos<false>, dim3(chunks * WPOS_BLOCKS), dim3(1024), 0, s, b.counters, b.dep_death, b.dep_wpos, iters, MultiTab{});
+    launch(k_dep_replay<false>, dim3(cdiv(p.hash_width, REPLAY_WAVES)), dim3(64 * REPLAY_WAVES), 0, s, p, b.lbuf, dl,
+           b.deferred, b.dep_wpos, b.counters, MultiTab{});
     if (!p.fold_small_passes) launch(k_dep_done, dim3(1), dim3(1), 0, s, b.counters, b.dep_over_n);   // else the resolve's
     return VE_OK;
 }
@@ -3243,23 +3623,144 @@ void launch_display(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
     launch(k_display, dim3(cdiv(n, 256)), dim3(256), 0, s, n, b.output, b.display);
 }
 
-void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
+// Blocks of the tiled timeout reconstruction (dense 16x16 tiles staged in LDS;
+// none for radii past RECON_TILED_MAX) and their dynamic LDS bytes.
+static uint32_t timeout_tile_blocks(const FrameParams &p)
 {
-    launch(k_resolve_avg, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.pix_avg, b.pix_cnt,
-                       b.counters, b.timeout_tiles, b.dep_over_n);
-    launch(k_resolve, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.nbuf, b.materials,
-                       b.pix_avg, b.pix_cnt, b.output, b.timeouts, b.counters, b.timeout_tiles, b.timeout_list);
-    // timed-out pixels one wave each; in timeout-heavy frames, the tiles
-    // holding RECON_DENSE or more go to the tiled blocks instead
-    uint32_t tile_blocks = 0, lds = 0;
-    if (p.reconstruction_radius <= (int)RECON_TILED_MAX) {
-        const uint32_t D = RECON_TILE + 2u * (uint32_t)std::max(p.reconstruction_radius, 0);
-        tile_blocks = std::min(cdiv(p.W, RECON_TILE) * cdiv(p.H, RECON_TILE), RECON_TILE_BLOCKS);
-        lds = (uint32_t)(D * D * sizeof(float4));
-    }
-    launch(k_resolve_timeouts, dim3(RESOLVE_TIMEOUT_BLOCKS + tile_blocks), dim3(256), lds, s, p, b.gbuf, b.pix_avg,
-           b.pix_cnt, b.timeouts, b.timeout_tiles, b.timeout_list, b.counters, b.output, RESOLVE_TIMEOUT_BLOCKS);
+    if (p.reconstruction_radius > (int)RECON_TILED_MAX) return 0u;
+    return std::min(cdiv(p.W, RECON_TILE) * cdiv(p.H, RECON_TILE), RECON_TILE_BLOCKS);
 }
 
+uint32_t timeouts_lds(const FrameParams &p)
+{
+    if (p.reconstruction_radius > (int)RECON_TILED_MAX) return 0u;
+    const uint32_t D = RECON_TILE + 2u * (uint32_t)std::max(p.reconstruction_radius, 0);
+    return (uint32_t)(D * D * sizeof(float4));
+}
+
+void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
+{
+    launch(k_resolve_avg<false>, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.pix_avg, b.pix_cnt,
+                       b.counters, b.timeout_tiles, b.dep_over_n, MultiTab{});
+    launch(k_resolve<false>, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.lbuf, b.nbuf, b.materials,
+                       b.pix_avg, b.pix_cnt, b.output, b.timeouts, b.counters, b.timeout_tiles, b.timeout_list, MultiTab{});
+    // timed-out pixels one wave each; in timeout-heavy frames, the tiles
+    // holding RECON_DENSE or more go to the tiled blocks instead
+    launch(k_resolve_timeouts<false>, dim3(RESOLVE_TIMEOUT_BLOCKS + timeout_tile_blocks(p)), dim3(256), timeouts_lds(p),
+           s, p, b.gbuf, b.pix_avg, b.pix_cnt, b.timeouts, b.timeout_tiles, b.timeout_list, b.counters, b.output,
+           RESOLVE_TIMEOUT_BLOCKS, MultiTab{});
+}
+
+
+// ---------------------------------------------------------------------------
+// one launch per pass over many views
+// ---------------------------------------------------------------------------
+uint32_t multi_blocks(uint32_t k, const FrameParams &p, uint32_t iters)
+{
+    const uint32_t rays = cdiv(p.shade_budget, 256);
+    const uint32_t bound = (uint32_t)std::min<size_t>((size_t)p.W * p.H, CANDIDATE_CAPACITY);
+    const uint32_t groups = ((p.W + 7u) / 8u) * ((p.H + 7u) / 8u);
+    switch (k) {
+    case MK_RESET: return 1u;
+    case MK_NRM_EVENTS: return screen_blocks(p);
+    case MK_NRM_REPLAY: return cdiv(p.hash_width, REPLAY_WAVES);
+    case MK_NRM_COMPUTE: return cdiv((size_t)p.hash_width * NBUFFER_SLOTS, 256);
+    case MK_SCHED_CLAIM: return groups;
+    case MK_SCHED_RANK: return CLAIM_WIDTH;
+    case MK_SCHED_RANK_OVER: return SCHED_OVER_BLOCKS;
+    case MK_SCHED_VEC_COUNT:
+    case MK_SCHED_WRITE: return cdiv(sched_vectors(p), 256);
+    case MK_RANK: return std::min<uint32_t>(cdiv(bound, 256), RANK_BLOCKS);
+    case MK_RANK_WRITE: return cdiv(bound, 256);
+    case MK_EMIT: return rays;
+    case MK_TRACE:
+    case MK_SHADE: return iters > 0 ? rays : 0u;
+    case MK_EM_REGISTER: return iters > 0 ? 1u : 0u;
+    case MK_BOUNCE: return iters > 1 ? rays : 0u;
+    case MK_DEP_PREP: return iters > 0 ? DEFERRED_BLOCKS : 0u;
+    case MK_DEP_WPOS: return iters > 0 ? (p.shade_budget + RAY_CHUNK - 1u) / RAY_CHUNK * WPOS_BLOCKS : 0u;
+    case MK_DEP_REPLAY: return iters > 0 ? cdiv(p.hash_width, REPLAY_WAVES) : 0u;
+    case MK_RESOLVE_AVG:
+    case MK_RESOLVE: return screen_blocks(p);
+    case MK_RESOLVE_TIMEOUTS: return RESOLVE_TIMEOUT_BLOCKS + timeout_tile_blocks(p);
+    default: return 0u;
+    }
+}
+
+// The multi launches pass null / zero for every argument their kernels take
+// from the view records instead.
+void launch_frames_multi(hipStream_t s, const ViewRec *views, const uint32_t *pre, uint32_t n, const uint32_t *grids,
+                         uint32_t lds)
+{
+    auto tab = [&](uint32_t k) { return MultiTab{views, pre + (size_t)k * (n + 1u), n}; };
+    const FrameParams z{};
+    const OrderLists ol{};
+    const DepLists dl{};
+    if (grids[MK_RESET]) launch(k_frame_reset<true>, dim3(grids[MK_RESET]), dim3(256), 0, s, z, nullptr, nullptr, nullptr, tab(MK_RESET));
+    if (grids[MK_NRM_EVENTS])
+        launch(k_nrm_events<true>, dim3(grids[MK_NRM_EVENTS]), dim3(256), 0, s, z, nullptr, nullptr, ol, nullptr,
+               tab(MK_NRM_EVENTS));
+    if (grids[MK_NRM_REPLAY])
+        launch(k_nrm_replay<true>, dim3(grids[MK_NRM_REPLAY]), dim3(64 * REPLAY_WAVES), 0, s, z, nullptr, ol, nullptr,
+               nullptr, tab(MK_NRM_REPLAY));
+    if (grids[MK_NRM_COMPUTE])
+        launch(k_nrm_compute<true>, dim3(grids[MK_NRM_COMPUTE]), dim3(256), 0, s, nullptr, z, nullptr, nullptr, nullptr,
+               tab(MK_NRM_COMPUTE));
+    if (grids[MK_SCHED_CLAIM])
+        launch(k_sched_claim<true>, dim3(grids[MK_SCHED_CLAIM]), dim3(64), 0, s, z, nullptr, nullptr, nullptr, nullptr,
+               nullptr, nullptr, nullptr, nullptr, tab(MK_SCHED_CLAIM));
+    if (grids[MK_SCHED_RANK])
+        launch(k_sched_rank<true>, dim3(grids[MK_SCHED_RANK]), dim3(64), 0, s, nullptr, nullptr, nullptr, nullptr, nullptr,
+               tab(MK_SCHED_RANK));
+    if (grids[MK_SCHED_RANK_OVER])
+        launch(k_sched_rank_over<true>, dim3(grids[MK_SCHED_RANK_OVER]), dim3(256), 0, s, nullptr, nullptr, nullptr,
+               nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, tab(MK_SCHED_RANK_OVER));
+    if (grids[MK_SCHED_VEC_COUNT])
+        launch(k_sched_vec_count<true>, dim3(grids[MK_SCHED_VEC_COUNT]), dim3(256), 0, s, nullptr, 0u, nullptr, nullptr,
+               nullptr, tab(MK_SCHED_VEC_COUNT));
+    if (grids[MK_SCHED_WRITE])
+        launch(k_sched_write<true>, dim3(grids[MK_SCHED_WRITE]), dim3(256), 0, s, z, nullptr, 0u, nullptr, nullptr,
+               nullptr, nullptr, nullptr, tab(MK_SCHED_WRITE));
+    if (grids[MK_RANK])
+        launch(k_rank<true>, dim3(grids[MK_RANK]), dim3(256), 0, s, z, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+               nullptr, nullptr, nullptr, tab(MK_RANK));
+    if (grids[MK_RANK_WRITE])
+        launch(k_rank_write<true>, dim3(grids[MK_RANK_WRITE]), dim3(256), 0, s, z, nullptr, nullptr, nullptr, nullptr,
+               nullptr, nullptr, nullptr, nullptr, tab(MK_RANK_WRITE));
+    if (grids[MK_EMIT])
+        launch(k_emit<true>, dim3(grids[MK_EMIT]), dim3(256), 0, s, z, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+               nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, tab(MK_EMIT));
+    if (grids[MK_TRACE])
+        launch(k_trace<false, true>, dim3(grids[MK_TRACE]), dim3(256), 0, s, nullptr, z, nullptr, nullptr, 0u, nullptr,
+               nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, tab(MK_TRACE));
+    if (grids[MK_EM_REGISTER])
+        launch(k_em_register<true>, dim3(grids[MK_EM_REGISTER]), dim3(EM_THREADS), 0, s, nullptr, nullptr, nullptr,
+               nullptr, nullptr, nullptr, nullptr, nullptr, tab(MK_EM_REGISTER));
+    if (grids[MK_SHADE])
+        launch(k_shade<true>, dim3(grids[MK_SHADE]), dim3(256), 0, s, z, nullptr, 0u, nullptr, nullptr, nullptr, nullptr,
+               nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, tab(MK_SHADE));
+    if (grids[MK_BOUNCE])
+        launch(k_bounce<false, false, true>, dim3(grids[MK_BOUNCE]), dim3(256), 0, s, nullptr, z, nullptr, nullptr, 0u,
+               0u, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+               nullptr, nullptr, tab(MK_BOUNCE));
+    if (grids[MK_DEP_PREP])
+        launch(k_dep_prep<true>, dim3(grids[MK_DEP_PREP]), dim3(256), 0, s, z, nullptr, nullptr, nullptr, nullptr,
+               nullptr, dl, tab(MK_DEP_PREP));
+    if (grids[MK_DEP_WPOS])
+        launch(k_dep_wpos<true>, dim3(grids[MK_DEP_WPOS]), dim3(1024), 0, s, nullptr, nullptr, nullptr, 0u,
+               tab(MK_DEP_WPOS));
+    if (grids[MK_DEP_REPLAY])
+        launch(k_dep_replay<true>, dim3(grids[MK_DEP_REPLAY]), dim3(64 * REPLAY_WAVES), 0, s, z, nullptr, dl, nullptr,
+               nullptr, nullptr, tab(MK_DEP_REPLAY));
+    if (grids[MK_RESOLVE_AVG])
+        launch(k_resolve_avg<true>, dim3(grids[MK_RESOLVE_AVG]), dim3(256), 0, s, z, nullptr, nullptr, nullptr, nullptr,
+               nullptr, nullptr, nullptr, tab(MK_RESOLVE_AVG));
+    if (grids[MK_RESOLVE])
+        launch(k_resolve<true>, dim3(grids[MK_RESOLVE]), dim3(256), 0, s, z, nullptr, nullptr, nullptr, nullptr, nullptr,
+               nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, tab(MK_RESOLVE));
+    if (grids[MK_RESOLVE_TIMEOUTS])
+        launch(k_resolve_timeouts<true>, dim3(grids[MK_RESOLVE_TIMEOUTS]), dim3(256), lds, s, z, nullptr, nullptr,
+               nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0u, tab(MK_RESOLVE_TIMEOUTS));
+}
 
 }  // namespace ve

@@ -349,7 +349,8 @@ size_t collect(ve_server *s)
 
 // Launch the frames waiting in the next batch: every frame's checks and
 // parameters, the batch's primary launches, then each frame's other passes
-// on its session's stream.  Unless forced (or the batch is full) this waits
+// on its session's stream (or, whole-frame batches, as one launch per kernel
+// over all the batch's frames on the batch stream).  Unless forced (or the batch is full) this waits
 // while the previous batch's primary launch still runs: frames arriving
 // meanwhile join the next batch, so the batches grow with the load and the
 // batch stream never queues more than one of them.
@@ -381,13 +382,15 @@ int flush_ready(ve_server *s, bool force)
             return srv_error(s, rc, "session %u frame %u: %s", ss.id, f.index, ss.ctx->err.c_str());
         starts[i] = f.start;
     }
-    if ((rc = ve::batch_primary(s->batch, m, ctxs.data(), scs.data(), ps.data(), starts.data())))
+    const bool whole = ve::batch_whole(s->batch);
+    if ((rc = ve::batch_primary(s->batch, m, ctxs.data(), scs.data(), ps.data(), starts.data(), whole)))
         return srv_error(s, rc, "batch: %s", ve::batch_error_text(s->batch));
     for (uint32_t i = 0; i < m; i++) {
         Session &ss = *act[i];
         Frame &f = rd[i].second;
         ve_ctx *c = ss.ctx;
-        if ((rc = ve::frame_rest(c, scs[i], ps[i])))
+        // whole frames: the session's stream already waits for the batch's last launch
+        if (!whole && (rc = ve::frame_rest(c, scs[i], ps[i])))
             return srv_error(s, rc, "session %u frame %u: %s", ss.id, f.index, c->err.c_str());
         SRV_HIP(s, hipMemcpyAsync(f.slot->v, ve::ctx_ray_totals(c), 3 * sizeof(unsigned long long),
                                   hipMemcpyDeviceToHost, c->stream));

@@ -126,6 +126,10 @@ def make_server(ve, mode, max_sessions=64):
         srv = ve.Server(queues=0, max_sessions=max_sessions)
         srv.set_batch(**batch_tuning())
         return srv
+    if mode == "whole":
+        srv = ve.Server(queues=0, max_sessions=max_sessions)
+        srv.set_batch(**dict(batch_tuning(), groups=1, whole_frame=True))
+        return srv
     if mode == "placed":
         srv = ve.Server(queues=PLACED_STREAMS, max_sessions=max_sessions)
         srv.set_placement(True)
@@ -259,7 +263,7 @@ def sweep(ve, lams, duration_s=3.0, seed=0, queues_list=(0, 1), log=None, res_sc
 
     def run(q, lam):
         srv = make_server(ve, q, max_sessions)
-        srv.batched = q == "batched"
+        srv.batched = q in ("batched", "whole")
         try:
             ids = [srv.add_scene(w, d, info.size) for d, w, info in res_scenes]
             sessions = make_workload(lam, warmup_s + duration_s, seed, len(res_scenes))
@@ -345,7 +349,7 @@ def batched(ve, res_scenes, count=64, frames=6, seed=0, queues_list=(0, 1, "batc
     out = {}
     for q in queues_list:
         srv = make_server(ve, q, max(64, count))
-        srv.batched = q == "batched"
+        srv.batched = q in ("batched", "whole")
         try:
             ids = [srv.add_scene(w, d, info.size) for d, w, info in res_scenes]
             run_batched(ve, srv, ids, [d for d, _, _ in res_scenes], views, 1, mats)   # warm: contexts created

@@ -34,26 +34,33 @@ def views(ve, scenes, small_scene):
     ]
 
 
-def _make(ve, vs, graph):
+def _make(ve, vs, graph, bounce_blocks=0):
     out = []
     for (sc, mats, W, H, mb, _, _) in vs:
         r = ve.Renderer(W, H, hash_width=16384, max_bounces=mb)
         r.set_overlap(False)
         if graph:
             r.set_graph(True)
+        if bounce_blocks:
+            r.set_tuning(bounce_blocks=bounce_blocks)
         r.set_materials(mats)
         r.set_sky_constant((120, 160, 210))
         out.append((r, r.upload(sc[1], sc[0], sc[2].size)))
     return out
 
 
-@pytest.mark.parametrize("tile_w,grab,waves,groups,graph,whole", [(8, 4, 16, 1, False, False), (32, 1, 4, 4, True, False),
-                                                                  (64, 16, 8, 64, False, False),
-                                                                  (16, 2, 32, 2, True, False), (8, 4, 32, 1, False, True),
-                                                                  (32, 2, 8, 4, False, True)])
-def test_batch_matches_views_alone(ve, scenes, views, tile_w, grab, waves, groups, graph, whole):
+@pytest.mark.parametrize("tile_w,grab,waves,groups,graph,whole,bounce_blocks", [(8, 4, 16, 1, False, False, 0),
+                                                                                (32, 1, 4, 4, True, False, 0),
+                                                                                (64, 16, 8, 64, False, False, 0),
+                                                                                (16, 2, 32, 2, True, False, 0),
+                                                                                (8, 4, 32, 1, False, True, 0),
+                                                                                (32, 2, 8, 4, False, True, 0),
+                                                                                (8, 4, 32, 1, False, True, 3)])
+def test_batch_matches_views_alone(ve, scenes, views, tile_w, grab, waves, groups, graph, whole, bounce_blocks):
+    """bounce_blocks: the batched views carry that bounce-grid tuning (the
+    views alone keep the full grid); results may not depend on it."""
     batch = ve.Batch(tile_w=tile_w, grab=grab, waves_per_cu=waves, groups=groups, whole_frame=whole)
-    a, b = _make(ve, views, graph), _make(ve, views, False)
+    a, b = _make(ve, views, graph, bounce_blocks), _make(ve, views, False)
     try:
         for f in range(3):
             fcs, ins = [], []
@@ -67,7 +74,7 @@ def test_batch_matches_views_alone(ve, scenes, views, tile_w, grab, waves, group
                 fcs.append(fc)
                 ins.append(fi)
             batch.frame([r for r, _ in a], [s for _, s in a], fcs, ins)
-            print("case batch", tile_w, grab, waves, groups, graph, whole, "frame", f, flush=True)
+            print("case batch", tile_w, grab, waves, groups, graph, whole, bounce_blocks, "frame", f, flush=True)
             for i in range(len(views)):
                 for name in BUFS:
                     assert np.array_equal(a[i][0].read(name), b[i][0].read(name)), "view %d frame %d %s" % (i, f, name)

@@ -150,7 +150,8 @@ def cpu_baseline(words, depth, ubos, fc_kw, budget_s=20.0):
 # reports the load its Poisson realisation actually offered); the sweep then
 # bisects between the highest load within the p99 bound and the next one
 SERVED_LAMS = {"placed": [50, 200, 600, 1200, 1600, 2000, 2200, 2300, 2400], 0: [50, 600, 1200, 1600, 2000, 2200],
-               1: [50, 200, 600, 900, 1100, 1200, 1300], "batched": [50, 600, 1200, 1600, 2000, 2200]}
+               1: [50, 200, 600, 900, 1100, 1200, 1300], "batched": [50, 600, 1200, 1600, 2000, 2200],
+               "whole": [50, 600, 1200, 1600, 2000, 2200]}
 
 
 def divergence(ve, caves, frames=32, warmup=4):
@@ -223,16 +224,18 @@ def served_and_batched(ve, extreme, caves):
     3 s of measured arrivals per offered load (after 0.5 s of warm-up), on the
     concurrent server with each frame placed on the least-loaded of 16 shared
     streams ("placed"), the concurrent server with one stream per session (0),
-    the plain loop (one stream, 1) and the batching server (frames arriving
-    while a batch's primary launch runs join the next batch); batched: 64
-    uneven viewports over the depth-10 scene, the depth-12 cave scene and fbm
-    depth 9 / 11 octrees, 6 frames each, the same four ways."""
+    the plain loop (one stream, 1), the batching server (frames arriving
+    while a batch's primary launch runs join the next batch) and the
+    whole-frame batching server (every pass of a batch's frames as one launch
+    per kernel); batched: 64 uneven viewports over the depth-10 scene, the
+    depth-12 cave scene and fbm depth 9 / 11 octrees, 6 frames each, the same
+    five ways."""
     from voxelengine_amd import serving
 
     def log(m):
         print(m, file=sys.stderr, flush=True)
 
-    modes = ("placed", 0, 1, "batched")
+    modes = ("placed", 0, 1, "batched", "whole")
     sweep = serving.sweep(ve, SERVED_LAMS, 3.0, 0, modes, log=log)
     best = {q: serving.max_sustained(sweep[q]) for q in modes}
     plain = best[1]
@@ -247,9 +250,11 @@ def served_and_batched(ve, extreme, caves):
         "per_session_streams": sweep[0],
         "plain": sweep[1],
         "batched": sweep["batched"],
+        "whole_frame_batches": sweep["whole"],
         "max_frames_per_s_at_p99_33ms": {"concurrent": best["placed"], "per_session_streams": best[0], "plain": plain,
-                                         "batched": best["batched"], "ratio": ratio("placed"),
-                                         "ratio_per_session_streams": ratio(0), "ratio_batched": ratio("batched")},
+                                         "batched": best["batched"], "whole_frame_batches": best["whole"],
+                                         "ratio": ratio("placed"), "ratio_per_session_streams": ratio(0),
+                                         "ratio_batched": ratio("batched"), "ratio_whole_frame_batches": ratio("whole")},
     }
     d9, d11 = serving.served_scenes(((9, 7), (11, 8)))
     b = serving.batched(ve, [extreme, caves, d9, d11], 64, 6, 0, modes, log=log)
@@ -265,9 +270,11 @@ def served_and_batched(ve, extreme, caves):
         "per_session_streams": b[0],
         "plain": b[1],
         "batched": b["batched"],
+        "whole_frame_batches": b["whole"],
         "ratio_frames_per_s": bratio("placed"),
         "ratio_per_session_streams": bratio(0),
         "ratio_batched_frames_per_s": bratio("batched"),
+        "ratio_whole_frame_batches": bratio("whole"),
     }
     return served, batched
 

@@ -21,7 +21,9 @@ one stream, in arrival order); "batched" = the concurrent server with continuous
 frames go in through ve_server_submit_batch and wait while the previous
 batch's primary launch runs, so the frames falling due meanwhile share one
 persistent primary launch over a global tile queue; every other pass runs
-on the session's own stream.
+on the session's own stream; "whole" = the same batching with whole-frame
+batches: after the shared primary launch, every other pass of the batch's
+frames runs as one launch per kernel over all of them.
 """
 import json
 import os
@@ -118,7 +120,7 @@ PLACED_STREAMS = 16   # one per hardware queue (bench.py sets GPU_MAX_HW_QUEUES=
 
 
 def make_server(ve, mode, max_sessions=64):
-    """A session server for a mode (0, 1, "placed" or "batched").  The batching server
+    """A session server for a mode (0, 1, "placed", "batched" or "whole").  The batching server
     launches each session's other passes directly: replaying them from frame
     graphs measured slower here (2498 against 2561 frames/s on the batched
     configuration, and more host time per frame in the served sweep)."""

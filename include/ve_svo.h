@@ -344,8 +344,16 @@ const char *ve_batch_last_error(const ve_batch *b);
  * synchronisation. */
 int  ve_batch_frame(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const *scenes,
                     const ve_frame_config *fcs, const ve_frame_inputs *ins);
-/* Milliseconds of the last batched primary launch (waits for it). */
+/* Milliseconds of the last batched primary launch (waits for it; for
+ * whole-frame batches, of the whole batch). */
 int  ve_batch_primary_ms(ve_batch *b, float *ms);
+/* Whole-frame batches whose every view's context replays graphs
+ * (ve_set_graph): the batch's launches (the primary launch and one launch
+ * per later kernel) are captured into one HIP graph per launch shape (view
+ * count, grids, dynamic LDS) and replayed; the views' records are data in the
+ * batch's device block, copied before each replay.  No counterpart in the
+ * reference.  Counts of captures and replays. */
+int  ve_batch_graph_stats(ve_batch *b, uint64_t *captures, uint64_t *replays);
 
 /* Optional per-pass timing: while enabled, every ve_frame binds HIP events to
  * the first and last kernel dispatch of each pass (no host synchronisation,

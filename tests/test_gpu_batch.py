@@ -4,7 +4,7 @@ their primary passes in one persistent launch pulling tiles from a global
 queue -- the other passes on each view's stream, or (whole frames) as one
 launch per kernel over all the views; every structure of every view
 bit-exact frame by frame, for several launch shapes and groupings, with and
-without graph replay."""
+without graph replay (per view, or of the whole-frame batch's launches)."""
 import numpy as np
 import pytest
 
@@ -55,7 +55,8 @@ def _make(ve, vs, graph, bounce_blocks=0):
                                                                                 (16, 2, 32, 2, True, False, 0),
                                                                                 (8, 4, 32, 1, False, True, 0),
                                                                                 (32, 2, 8, 4, False, True, 0),
-                                                                                (8, 4, 32, 1, False, True, 3)])
+                                                                                (8, 4, 32, 1, False, True, 3),
+                                                                                (16, 4, 16, 1, True, True, 0)])
 def test_batch_matches_views_alone(ve, scenes, views, tile_w, grab, waves, groups, graph, whole, bounce_blocks):
     """bounce_blocks: the batched views carry that bounce-grid tuning (the
     views alone keep the full grid); results may not depend on it."""
@@ -81,7 +82,9 @@ def test_batch_matches_views_alone(ve, scenes, views, tile_w, grab, waves, group
                 ra, rb = hash_map(a[i][0].read("RBUFFER"), 8, 16), hash_map(b[i][0].read("RBUFFER"), 8, 16)
                 assert np.array_equal(ra[0], rb[0]) and np.array_equal(ra[1], rb[1]), "view %d frame %d RBUFFER" % (i, f)
         assert batch.primary_ms() > 0.0
-        if graph:
+        if graph and whole:   # one batch graph, captured on the first frame and replayed for each
+            assert batch.graph_stats() == dict(captures=1, replays=3)
+        elif graph:
             assert all(r.graph_stats()["replays"] == 3 for r, _ in a)
     finally:
         for r, _ in a + b:

@@ -165,6 +165,7 @@ def lib():
         "ve_batch_last_error": (C.c_char_p, [p]),
         "ve_batch_frame": (i32, [p, u32, p, p, p, p]),
         "ve_batch_primary_ms": (i32, [p, C.POINTER(C.c_float)]),
+        "ve_batch_graph_stats": (i32, [p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "ve_graph_stats": (i32, [p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "ve_server_set_graph": (i32, [p, i32]),
         "ve_server_set_tuning": (i32, [p, p]),
@@ -438,6 +439,12 @@ class Batch:
         ms = C.c_float()
         _check(lib().ve_batch_primary_ms(self.h, C.byref(ms)))
         return ms.value
+
+    def graph_stats(self):
+        """ve_batch_graph_stats: whole-frame batch graphs captured and replayed."""
+        c, r = C.c_uint64(), C.c_uint64()
+        _check(lib().ve_batch_graph_stats(self.h, C.byref(c), C.byref(r)))
+        return dict(captures=c.value, replays=r.value)
 
 
 class Renderer:

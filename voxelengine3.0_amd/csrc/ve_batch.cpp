@@ -41,6 +41,12 @@ struct ve_batch {
     hipEvent_t t0 = nullptr, t1 = nullptr;   // around the primary launch
     bool launched = false;                   // t1 recorded at least once
     std::vector<ve::FrameParams> params;
+    // whole-frame batches of graph-replaying contexts: the batch's launches
+    // captured once per launch shape (the views' records and block offsets
+    // are data in the device block, copied before each replay)
+    hipGraphExec_t wexec = nullptr;
+    std::vector<uint32_t> wkey;
+    uint64_t w_captures = 0, w_replays = 0;
     std::string err;
 };
 
@@ -213,28 +219,70 @@ int batch_primary(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const 
     if (starts)
         for (uint32_t i = 0; i < n; i++) B_HIP(b, hipEventRecord(starts[i], b->stream));
     B_HIP(b, hipEventRecord(b->t0, b->stream));
+    auto primary_blocks = [&](uint32_t gi) {   // no more blocks than the group's runs of tiles need (one run per wave)
+        const uint64_t runs = ((uint64_t)totals[gi] + b->cfg.grab - 1) / b->cfg.grab;
+        return (uint32_t)std::min<uint64_t>(b->blocks, (runs + 3) / 4);
+    };
+    if (whole) {
+        // one group: the primary launch, then every later pass over all the views
+        auto launches = [&]() -> int {
+            launch_primary_batch(b->stream, views_dev, bases_dev, n, totals[0], queue_dev, tw, b->cfg.grab,
+                                 primary_blocks(0));
+            B_HIP(b, hipGetLastError());
+            launch_frames_multi(b->stream, reinterpret_cast<const ViewRec *>(b->dev + L.recs),
+                                reinterpret_cast<const uint32_t *>(b->dev + L.pre), n, grids, lds);
+            B_HIP(b, hipGetLastError());
+            return VE_OK;
+        };
+        bool graph = true;   // replayed from a captured graph when every view's context replays graphs
+        for (uint32_t i = 0; i < n; i++) graph = graph && ctxs[i]->graph;
+        if (graph) {
+            std::vector<uint32_t> key{n, tw, b->cfg.grab, primary_blocks(0), totals[0], lds};
+            key.insert(key.end(), grids, grids + MK_COUNT);
+            if (!b->wexec || key != b->wkey) {
+                if (b->wexec) {
+                    B_HIP(b, hipGraphExecDestroy(b->wexec));
+                    b->wexec = nullptr;
+                }
+                B_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+                const int rc = launches();
+                hipGraph_t g = nullptr;
+                const hipError_t e = hipStreamEndCapture(b->stream, &g);
+                if (rc || e != hipSuccess) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc ? rc : batch_error(b, VE_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+                }
+                const hipError_t ei = hipGraphInstantiate(&b->wexec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ei != hipSuccess) {
+                    b->wexec = nullptr;
+                    return batch_error(b, VE_EDEVICE, "graph instantiate: %s", hipGetErrorString(ei));
+                }
+                b->wkey = std::move(key);
+                b->w_captures++;
+            }
+            B_HIP(b, hipGraphLaunch(b->wexec, b->stream));
+            b->w_replays++;
+        } else if (int rc = launches()) {
+            return rc;
+        }
+        B_HIP(b, hipEventRecord(b->t1, b->stream));   // the whole batch (its primary launch is inside it)
+        B_HIP(b, hipEventRecord(b->done[0], b->stream));
+        for (uint32_t i = 0; i < n; i++) B_HIP(b, hipStreamWaitEvent(ctxs[i]->stream, b->done[0], 0));
+        b->launched = true;
+        return VE_OK;
+    }
     for (uint32_t gi = 0; gi < groups; gi++) {
         const uint32_t m = first[gi + 1] - first[gi];
         if (m == 0) continue;
-        // no more blocks than the group's runs of tiles need (one run per wave)
-        const uint64_t runs = ((uint64_t)totals[gi] + b->cfg.grab - 1) / b->cfg.grab;
-        const uint32_t blocks = (uint32_t)std::min<uint64_t>(b->blocks, (runs + 3) / 4);
         launch_primary_batch(b->stream, views_dev + first[gi], bases_dev + first[gi] + gi, m, totals[gi],
-                             queue_dev + gi, tw, b->cfg.grab, blocks);
+                             queue_dev + gi, tw, b->cfg.grab, primary_blocks(gi));
         B_HIP(b, hipGetLastError());
-        if (whole) continue;
         B_HIP(b, hipEventRecord(b->done[gi], b->stream));
         for (uint32_t j = first[gi]; j < first[gi + 1]; j++)
             B_HIP(b, hipStreamWaitEvent(ctxs[order[j]]->stream, b->done[gi], 0));
     }
     B_HIP(b, hipEventRecord(b->t1, b->stream));
-    if (whole) {
-        launch_frames_multi(b->stream, reinterpret_cast<const ViewRec *>(b->dev + L.recs),
-                            reinterpret_cast<const uint32_t *>(b->dev + L.pre), n, grids, lds);
-        B_HIP(b, hipGetLastError());
-        B_HIP(b, hipEventRecord(b->done[0], b->stream));
-        for (uint32_t i = 0; i < n; i++) B_HIP(b, hipStreamWaitEvent(ctxs[i]->stream, b->done[0], 0));
-    }
     b->launched = true;
     return VE_OK;
 }
@@ -303,6 +351,7 @@ void ve_batch_destroy(ve_batch *b)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : {b->t0, b->t1})
         if (e) (void)hipEventDestroy(e);
+    if (b->wexec) (void)hipGraphExecDestroy(b->wexec);
     for (auto &x : b->staging) {
         if (x.copied) (void)hipEventDestroy(x.copied);
         if (x.host) (void)hipHostFree(x.host);
@@ -337,6 +386,14 @@ int ve_batch_frame(ve_batch *b, uint32_t n, ve_ctx *const *ctxs, ve_scene *const
     for (uint32_t i = 0; i < n; i++)
         if ((rc = ve::frame_rest(ctxs[i], scenes[i], b->params[i])))
             return batch_error(b, rc, "view %u: %s", i, ctxs[i]->err.c_str());
+    return VE_OK;
+}
+
+int ve_batch_graph_stats(ve_batch *b, uint64_t *captures, uint64_t *replays)
+{
+    if (!b) return ve_global_error(VE_EINVAL, "null batch");
+    if (captures) *captures = b->w_captures;
+    if (replays) *replays = b->w_replays;
     return VE_OK;
 }
 

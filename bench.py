@@ -289,6 +289,9 @@ def main():
                     help="skip the served (Poisson sessions) and batched (64 viewports) measurements")
     ap.add_argument("--no-divergence", action="store_true",
                     help="skip the depth-12 divergence configuration (and, with it, the batched set's cave scene)")
+    ap.add_argument("--batched-only", metavar="MODE",
+                    help="run only the batched configuration (64 viewports, 6 frames each) on one server mode "
+                         "(0, 1, placed, batched or whole) and print its summary (for kernel traces)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device copy (4 launches), the reference kernel for scaling "
                          "the FETCH_SIZE / WRITE_SIZE counters")
@@ -301,6 +304,20 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": x.numel() * 4, "launches": 4}))
+        return
+    if args.batched_only:
+        from ve_loader import load_package
+        ve = load_package()
+        from voxelengine_amd import scenes as S, serving
+        mode = int(args.batched_only) if args.batched_only.isdigit() else args.batched_only
+        depth, ops = S.scene_extreme(1)
+        words, info = ve.octree_build(ops, depth)
+        cdepth, cops = S.scene_caves(1)
+        cwords, cinfo = ve.octree_build(cops, cdepth, max_words=1 << 28)
+        del ops, cops
+        d9, d11 = serving.served_scenes(((9, 7), (11, 8)))
+        b = serving.batched(ve, [(depth, words, info), (cdepth, cwords, cinfo), d9, d11], 64, 6, 0, (mode,))
+        print(json.dumps({"batched_mode": str(mode), **b[mode]}))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X, and the "

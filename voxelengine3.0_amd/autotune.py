@@ -8,8 +8,9 @@ pull size (grab: tiles a wave takes per atomic) and grid size (waves per
 compute unit); then, for the best shape, the traversal jump-table depth of
 the scenes (the octree levels a ray skips through one table lookup, 6..8)
 and the number of launch groups, timed on whole ticks (every view's frame
-done).  Writes every measurement and the chosen shape to tuning.json beside
-this file, which serving.batch_tuning() reads.
+done), and the same ticks as whole-frame batches (every later pass one
+launch over all the views).  Writes every measurement and the chosen shape
+to tuning.json beside this file, which serving.batch_tuning() reads.
 
     python -m voxelengine_amd.autotune            (through tests/ve_loader)
     python voxelengine3.0_amd/autotune.py [--quick] [--out PATH]
@@ -107,15 +108,17 @@ def tune(ve, res_scenes, count=64, seed=0, ticks=3, quick=False, log=print):
     for jl in ((8,) if quick else JUMPS):
         up, scenes, rends, args = _views(ve, S, serving, res_scenes, count, seed, jl)
         try:
-            for gr in ((1, 4) if quick else GROUPS):
+            for gr, whole in [(g, False) for g in ((1, 4) if quick else GROUPS)] + [(1, True)]:
                 batch = ve.Batch(max_views=count, tile_w=best["tile_w"], grab=best["grab"],
-                                 waves_per_cu=best["waves_per_cu"], groups=gr)
+                                 waves_per_cu=best["waves_per_cu"], groups=gr, whole_frame=whole)
                 try:
                     _tick(ve, S, batch, rends, scenes, args, res_scenes, 0)
                     res = [_tick(ve, S, batch, rends, scenes, args, res_scenes, k + 1) for k in range(ticks)]
                 finally:
                     batch.close()
-                row = dict(jump_level=jl, groups=gr, primary_ms=round(float(np.median([a for a, _ in res])), 4),
+                # whole-frame batches: primary_ms is the whole batch's launches
+                row = dict(jump_level=jl, groups=gr, whole_frame=whole,
+                           primary_ms=round(float(np.median([a for a, _ in res])), 4),
                            tick_ms=round(float(np.median([b for _, b in res])), 4))
                 out["stage2"].append(row)
                 log("stage 2 %s" % row)
@@ -123,10 +126,12 @@ def tune(ve, res_scenes, count=64, seed=0, ticks=3, quick=False, log=print):
             for r in rends:
                 r.close()
             up.close()
-    b2 = min(out["stage2"], key=lambda r: r["tick_ms"])
+    b2 = min((r for r in out["stage2"] if not r["whole_frame"]), key=lambda r: r["tick_ms"])
+    bw = min((r for r in out["stage2"] if r["whole_frame"]), key=lambda r: r["tick_ms"])
     out["best"] = dict(tile_w=best["tile_w"], grab=best["grab"], waves_per_cu=best["waves_per_cu"],
                        groups=b2["groups"], jump_level=b2["jump_level"], primary_ms=best["primary_ms"],
                        tick_ms=b2["tick_ms"])
+    out["best_whole_frame"] = dict(jump_level=bw["jump_level"], tick_ms=bw["tick_ms"])
     return out
 
 

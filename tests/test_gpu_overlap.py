@@ -1,5 +1,6 @@
-"""Cross-frame overlap (the next frame's primary pass on a second stream,
-into a second G-buffer set, beside the previous frame's bounce passes)
+"""Cross-frame overlap (the next frame's primary and schedule passes on a
+second stream, into a second G-buffer set, and its normal pass on a third,
+into a second nBuffer set, beside the previous frame's bounce passes)
 renders exactly what the reference's serial pass order renders.
 
 A queue of frames with a moving camera, a scene edit between two frames, a
@@ -182,3 +183,70 @@ def test_frame_equals_pass_by_pass(ve, scenes, small_scene, min_samples):
     for name in HASHED:
         assert np.array_equal(fused[name][0], by_pass[name][0]), name
         assert np.array_equal(fused[name][1], by_pass[name][1]), name
+
+
+def _row_voxels(row, chunks, seed=5):
+    """Random voxels of a 512^3 grid (y in 64..447) whose ids fall in claim
+    row `row`: voxelKey + 1 then claimHash % 16384 (internal.glsl:263-291,
+    schedule.comp:36-43) in 32-bit arithmetic."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(chunks):
+        x, y, z = rng.integers(0, 512, size=(3, 1 << 24), dtype=np.uint32)
+        y = np.uint32(64) + y % np.uint32(384)
+        v = x | (y << np.uint32(10)) | (z << np.uint32(20))
+        for s, m in ((16, 0x7FEB352D), (15, 0x846CA68B)):
+            v ^= v >> np.uint32(s)
+            v *= np.uint32(m)
+        v ^= v >> np.uint32(16)
+        v = (v & np.uint32(0x7FFFFFFF)) + np.uint32(1)
+        v ^= v >> np.uint32(16)
+        v *= np.uint32(0x7FEB352D)
+        v ^= v >> np.uint32(15)
+        sel = v % np.uint32(16384) == row
+        out.append(np.stack([x[sel], y[sel], z[sel]]))
+    return np.unique(np.concatenate(out, axis=1), axis=1).astype(np.int64)
+
+
+def test_overflow_flag_per_frame(ve, scenes):
+    """A claim row crowded past the schedule's exact-order capacity (more
+    than 2048 distinct visible voxels in one row) raises the overflow
+    counter.  With the overlap on, the frame's schedule pass runs ahead of
+    the frame's reset and reports it into its buffer set's side word, which
+    the frame's resolve folds into the counter and clears: the flag reads the
+    same as the serial context's frame by frame, and frames that see no
+    crowded row read it clear in both buffer sets."""
+    from test_gpu_claim_overflow import claim_rows, voxel_ids
+    P = _row_voxels(777, 6)
+    assert np.all(claim_rows(voxel_ids(*(P.astype(np.uint64)))) == 777)
+    ops = scenes._ops(P[0], P[1], P[2], scenes.voxel_materials(P[0], P[1], P[2], 1))
+    words, info = ve.octree_build(ops, 9)
+    W = H = 1024
+    crowded = [True, True, False, True, False]
+    flags = {}
+    for overlap in (False, True):
+        rend = ve.Renderer(W, H, hash_width=4096, max_bounces=1)
+        try:
+            rend.set_overlap(overlap)
+            rend.set_materials(scenes.palette(1))
+            rend.set_sky_constant((140, 180, 230))
+            sc = rend.upload(words, 9, info.size)
+            fc = ve.FrameConfig(**DET_KNOBS)
+            seq = []
+            for f, crowd in enumerate(crowded):
+                if crowd:
+                    e, d = scenes.look_at((256.0 + 4.0 * f, 900.0, -500.0), (256.0, 250.0, 256.0))
+                else:   # up and away from above the grid: sky only
+                    e, d = scenes.look_at((256.0, 600.0, 256.0), (300.0, 1000.0, 330.0))
+                rend.set_camera(ve.camera_ubo(e, d, 50.0, 1.0))
+                rend.frame(sc, fc, ve.FrameInputs(W, H, f, frame_seed(f), f + 1))
+                c = rend.counters()
+                seq.append(int(c[ve.CNT["OVERFLOW"]]))
+                # the one row's first 64 claimants (CLAIM_PROBES), or nothing
+                assert int(c[ve.CNT["CANDIDATES"]]) == (64 if crowd else 0)
+            print("case overflow flag overlap", overlap, seq, flush=True)
+            rend.release(sc)
+            flags[overlap] = seq
+        finally:
+            rend.close()
+    assert flags[False] == flags[True] == [1 if c else 0 for c in crowded]

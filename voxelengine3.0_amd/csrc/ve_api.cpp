@@ -81,7 +81,8 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.output, P); a(&b.display, P); a(&b.pix_avg, P); a(&b.pix_cnt, P); a(&b.timeouts, P);
     a(&b.timeout_tiles, (size_t)((max_w + RECON_TILE - 1) / RECON_TILE) * ((max_h + RECON_TILE - 1) / RECON_TILE));
     a(&b.timeout_list, (size_t)((max_w + RECON_TILE - 1) / RECON_TILE) * ((max_h + RECON_TILE - 1) / RECON_TILE));
-    a(&b.nbuf, (size_t)hash_width * NBUFFER_STRIDE * NBUFFER_SLOTS);
+    for (int k = 0; k < (overlap ? 2 : 1); k++) a(&b.nbuf_set[k], (size_t)hash_width * NBUFFER_STRIDE * NBUFFER_SLOTS);
+    b.nbuf = b.nbuf_set[0];
     a(&b.lbuf, (size_t)hash_width * LBUFFER_STRIDE * LBUFFER_SLOTS);
     a(&b.rbuf, (size_t)hash_width * RBUFFER_STRIDE * RBUFFER_SLOTS);
     a(&b.claim, (size_t)CLAIM_WIDTH * CLAIM_PROBES);
@@ -90,7 +91,7 @@ void plan_buffers(Alloc &a, DeviceBuffers &b, uint32_t max_w, uint32_t max_h, ui
     a(&b.shade_list, SHADE_BUDGET_MAX);
     a(&b.rays, SHADE_BUDGET_MAX);
     a(&b.rays_next, SHADE_BUDGET_MAX);
-    a(&b.counters, NUM_COUNTERS);
+    a(&b.counters, NUM_COUNTERS + COUNTER_SIDE_WORDS);
     a(&b.rank_stats, RANK_STATS_WORDS);
     a(&b.rank_part, (size_t)RANK_COPIES * RANK_COPY_WORDS);
     a(&b.bounce_part, (size_t)BOUNCE_COPIES * 2 + 1);
@@ -360,7 +361,7 @@ void ve_destroy(ve_ctx *c)
     for (auto &fe : c->frame_events)
         for (hipEvent_t e : fe)
             if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {c->ev_primary, c->ev_emit, c->ev_fence, c->ev_head, c->ev_normal})
+    for (hipEvent_t e : {c->ev_primary, c->ev_sched, c->ev_emit, c->ev_fence, c->ev_normal})
         if (e) (void)hipEventDestroy(e);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (hipEvent_t e : c->dyn_ev)
@@ -550,9 +551,9 @@ int flush_deposits(ve_ctx *c)
     return VE_OK;
 }
 
-// Enqueue one pass; every launch goes on the context's stream.  The primary
-// and normal passes run on `side_stream` when given (an overlapped frame's
-// auxiliary streams).
+// Enqueue one pass; every launch goes on the context's stream.  The primary,
+// schedule and normal passes run on `side_stream` when given (an overlapped
+// frame's auxiliary streams).
 int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, hipStream_t side_stream = nullptr)
 {
     int rc = VE_OK;
@@ -567,7 +568,7 @@ int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, hipS
         break;
     case VE_PASS_NORMAL: launch_normal(side_stream ? side_stream : c->stream, s->words, p, c->b); break;
     case VE_PASS_RESET: launch_reset(c->stream, p, c->b); break;
-    case VE_PASS_SCHEDULE: rc = launch_schedule(c->stream, p, c->b); break;
+    case VE_PASS_SCHEDULE: rc = launch_schedule(side_stream ? side_stream : c->stream, p, c->b); break;
     case VE_PASS_RANK: rc = launch_rank(c->stream, p, c->b); break;
     case VE_PASS_THRESHOLD: launch_threshold(c->stream, p, c->b); break;
     case VE_PASS_EMIT:
@@ -627,23 +628,32 @@ int timed_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, ui
 
 // renderer.cpp:282-649 without its host readbacks: counts stay on the device
 // and every kernel sizes its own work from them.  With cross-frame overlap
-// the primary pass goes to the auxiliary stream, into the other G-buffer set,
-// once the previous frame's emit pass is done: it then runs beside that
-// frame's trace/shade/resolve passes, which leave most of the device idle
-// (their time is the tail of the longest rays).  The set's last readers
-// belong to the frame before last, whose passes all precede that emit pass
-// on the main stream.  The main stream waits for it before the normal pass.
-int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &p)
+// the primary and schedule passes go to the auxiliary stream, into the other
+// G-buffer set, once the previous frame's emit pass is done, and the normal
+// pass to the second auxiliary stream, into the other nBuffer set: they then
+// run beside that frame's trace/shade/resolve passes, which leave most of the
+// device idle (their time is the tail of the longest rays).  The sets' last
+// readers belong to the frame before last, whose passes all precede that
+// emit pass on the main stream.  The main stream waits for the schedule pass
+// before the rank pass and for the normal pass before the trace pass.
+int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &frame)
 {
     hipEvent_t *ev = nullptr;
     if (c->timing && !(ev = frame_slot(c))) return ctx_error(c, VE_EDEVICE, "hipEventCreate failed");
     int rc;
+    FrameParams p = frame;
     if (c->overlap) {
+        // deposits that lone shade passes left pending go on the stream
+        // before the fence that the auxiliary streams wait for
+        if ((rc = flush_deposits(c))) return rc;
         const uint32_t k = c->gsel ^ 1u;
         c->gsel = k;
         c->b.gbuf = c->b.gbuf_set[k];
         c->b.gpos = c->b.gpos_set[k];
         c->b.ghit = c->b.ghit_set[k];
+        const uint32_t *nbuf_prev = c->b.nbuf;
+        c->b.nbuf = c->b.nbuf_set[k];
+        p.early_set = (int32_t)(1u + k);
         if (c->emit_done) HIP_OK(c, hipStreamWaitEvent(c->aux, c->emit_done, 0));
         if (c->aux_fence) {
             HIP_OK(c, hipEventRecord(c->ev_fence, c->stream));
@@ -652,25 +662,26 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &p)
         }
         if ((rc = timed_pass(c, s, VE_PASS_PRIMARY, p, 0, ev, c->aux))) return rc;
         HIP_OK(c, hipEventRecord(c->ev_primary, c->aux));
+        if ((rc = timed_pass(c, s, VE_PASS_SCHEDULE, p, 0, ev, c->aux))) return rc;
+        HIP_OK(c, hipEventRecord(c->ev_sched, c->aux));
+        HIP_OK(c, hipStreamWaitEvent(c->aux2, c->ev_primary, 0));
+        if (nbuf_prev != c->b.nbuf)
+            HIP_OK(c, hipMemcpyAsync(c->b.nbuf, nbuf_prev, (size_t)c->hash_width * NBUFFER_STRIDE * NBUFFER_SLOTS * 4,
+                                     hipMemcpyDeviceToDevice, c->aux2));
+        if ((rc = timed_pass(c, s, VE_PASS_NORMAL, p, 0, ev, c->aux2))) return rc;
+        HIP_OK(c, hipEventRecord(c->ev_normal, c->aux2));
         if ((rc = timed_pass(c, s, VE_PASS_RESET, p, 0, ev))) return rc;
-        HIP_OK(c, hipStreamWaitEvent(c->stream, c->ev_primary, 0));
+        HIP_OK(c, hipStreamWaitEvent(c->stream, c->ev_sched, 0));
     } else {
         if ((rc = timed_pass(c, s, VE_PASS_RESET, p, 0, ev))) return rc;
         if (!c->skip_primary && (rc = timed_pass(c, s, VE_PASS_PRIMARY, p, 0, ev))) return rc;
-    }
-    if (c->overlap) {   // normal pass beside schedule .. emit
-        HIP_OK(c, hipEventRecord(c->ev_head, c->stream));
-        HIP_OK(c, hipStreamWaitEvent(c->aux2, c->ev_head, 0));
-        if ((rc = timed_pass(c, s, VE_PASS_NORMAL, p, 0, ev, c->aux2))) return rc;
-        HIP_OK(c, hipEventRecord(c->ev_normal, c->aux2));
-    } else if ((rc = timed_pass(c, s, VE_PASS_NORMAL, p, 0, ev))) {
-        return rc;
+        if ((rc = timed_pass(c, s, VE_PASS_NORMAL, p, 0, ev))) return rc;
+        if ((rc = timed_pass(c, s, VE_PASS_SCHEDULE, p, 0, ev))) return rc;
     }
     // the threshold (threshold.comp) runs on the emit kernel's first wave
     // when the frame folds its small passes
-    const ve_pass_id head[] = {VE_PASS_SCHEDULE, VE_PASS_RANK, VE_PASS_THRESHOLD};
-    for (ve_pass_id id : head)
-        if (!(id == VE_PASS_THRESHOLD && p.fold_small_passes) && (rc = timed_pass(c, s, id, p, 0, ev))) return rc;
+    if ((rc = timed_pass(c, s, VE_PASS_RANK, p, 0, ev))) return rc;
+    if (!p.fold_small_passes && (rc = timed_pass(c, s, VE_PASS_THRESHOLD, p, 0, ev))) return rc;
     if (c->overlap) {
         // the emit kernel's own dispatch marks its completion: the timing
         // event when the pass is timed, ev_emit otherwise
@@ -1092,7 +1103,7 @@ int ctx_create(ve_ctx **out, const ve_renderer_config *cfg, hipStream_t shared, 
                 if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_emit, hipEventDisableSystemFence);
                 if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fence, fl);
                 if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->aux2, hipStreamNonBlocking);   // default priority
-                if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_head, fl);
+                if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_sched, fl);
                 if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_normal, fl);
             }
         }
@@ -1166,6 +1177,7 @@ int ctx_reset(ve_ctx *c)
     c->b.gbuf = c->b.gbuf_set[0];
     c->b.gpos = c->b.gpos_set[0];
     c->b.ghit = c->b.ghit_set[0];
+    c->b.nbuf = c->b.nbuf_set[0];
     c->emit_done = nullptr;
     c->aux_fence = true;
     c->node_stats = false;

@@ -64,21 +64,25 @@ struct ve_ctx {
     // Cross-frame overlap (contexts that own their stream): the primary pass,
     // which reads only the camera and the octree, runs on `aux` into the other
     // G-buffer set while the previous frame's latency-bound bounce passes
-    // still run on `stream`.
+    // still run on `stream`, and the schedule pass follows it there: it reads
+    // only the G-buffer and writes the claim map and candidate list, which no
+    // pass after the previous frame's emit pass reads.
     bool overlap = false;
     hipStream_t aux = nullptr;
-    uint32_t gsel = 0;                      // G-buffer set of the latest frame
+    uint32_t gsel = 0;                      // G-buffer (and nBuffer) set of the latest frame
     hipEvent_t ev_primary = nullptr;        // on aux: primary pass written
+    hipEvent_t ev_sched = nullptr;          // on aux: schedule pass written
     hipEvent_t ev_emit = nullptr;           // on stream: emit pass of the latest frame done
     hipEvent_t emit_done = nullptr;         // the event that marks it (ev_emit, or its timing event)
     hipEvent_t ev_fence = nullptr;          // on stream: everything enqueued so far
-    // The normal pass writes only the nBuffer, which the schedule, rank,
-    // threshold and emit passes never touch: inside an overlapped frame it
-    // runs on aux2 beside them, after the frame's primary pass and the
-    // previous frame's passes (ev_head), and the main stream waits for it
-    // (ev_normal) before the first pass that reads normals.
+    // The normal pass updates the nBuffer, which shading and the resolve
+    // read: inside an overlapped frame it runs on aux2 after the frame's
+    // primary pass, into the nBuffer set of the frame's G-buffer set, first
+    // copied from the previous frame's; that set's last readers belong to
+    // the frame before last, whose passes precede the previous emit pass.
+    // The main stream waits for it (ev_normal) before the first pass that
+    // reads normals.
     hipStream_t aux2 = nullptr;
-    hipEvent_t ev_head = nullptr;           // on stream: the frame's reset pass, after its primary pass
     hipEvent_t ev_normal = nullptr;         // on aux2: normal pass written
     bool aux_fence = true;                  // the next primary must follow all of stream's work
     // Graph replay (ve_set_graph): the frame's launches are captured once into

@@ -32,6 +32,9 @@ constexpr uint32_t RR_START_BOUNCE = 2;        // wavefront.glsl:112
 constexpr uint32_t TRUSTED_THRESHOLD = 128;    // resolve.comp:82
 constexpr uint32_t RANK_STATS_WORDS = 264;     // wavefront.glsl:243-255
 constexpr uint32_t NUM_COUNTERS = 16;
+// past the counters: the overflow side words of early schedule / normal
+// passes, {schedule, normal} per buffer set (FrameParams::early_set)
+constexpr uint32_t COUNTER_SIDE_WORDS = 4;
 constexpr uint32_t SCHED_SLOTS = 128;          // ordering table: voxels per claim row before overflow
 // Voxels past a claim row's table go to a per-frame overflow list, and their
 // rows are ranked exactly by a separate pass: up to SCHED_OVER_CAP records
@@ -180,6 +183,12 @@ struct FrameParams {
     // each hit's lattice position and the emit pass fills in the hit record
     // of a bounce-0 ray whose camera ray is the pixel's primary ray
     int32_t reuse_primary_hits;
+    // 1 + the frame's buffer set when its schedule and normal passes run on
+    // the auxiliary streams ahead of the previous frame's later passes (0
+    // otherwise): they report overflow into that set's side words past the
+    // counters, the resolve's first thread folds those into the overflow
+    // counter, and the reset pass keeps the schedule's candidate count
+    int32_t early_set;
     // sky: a cube map of sky_edge^2 texels per face (faces +X -X +Y -Y +Z -Z,
     // texel r | g << 8 | b << 16), or, with sky_edge 0, the constant `sky`
     uint32_t sky_edge;

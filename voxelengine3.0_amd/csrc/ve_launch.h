@@ -40,7 +40,8 @@ struct DeviceBuffers {
     uint32_t *timeouts = nullptr;   // P: timed-out pixels (resolve)
     uint32_t *timeout_tiles = nullptr;  // per 16x16 tile: holds a timed-out pixel (zero between frames)
     uint32_t *timeout_list = nullptr;   // the flagged 16x16 tiles, in listing order
-    uint32_t *nbuf = nullptr;       // hash_width x 96
+    uint32_t *nbuf = nullptr;       // hash_width x 96: the current set
+    uint32_t *nbuf_set[2] = {nullptr, nullptr};  // second set only with cross-frame overlap
     uint32_t *lbuf = nullptr;       // hash_width x 384
     uint32_t *rbuf = nullptr;       // hash_width x 128
     uint32_t *claim = nullptr;      // CLAIM_WIDTH x CLAIM_PROBES
@@ -49,7 +50,7 @@ struct DeviceBuffers {
     uint2 *shade_list = nullptr;    // SHADE_BUDGET_MAX
     Ray *rays = nullptr;            // SHADE_BUDGET_MAX (current bounce)
     Ray *rays_next = nullptr;       // SHADE_BUDGET_MAX (survivors)
-    uint32_t *counters = nullptr;   // NUM_COUNTERS
+    uint32_t *counters = nullptr;   // NUM_COUNTERS + COUNTER_SIDE_WORDS
     uint32_t *rank_stats = nullptr; // RANK_STATS_WORDS
     uint32_t *rank_part = nullptr;  // RANK_COPIES x RANK_COPY_WORDS (zero between rank passes)
     uint32_t *bounce_part = nullptr;  // BOUNCE_COPIES x 2 + the ray queue (zero between fused bounce passes)

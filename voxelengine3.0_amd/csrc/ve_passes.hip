@@ -473,7 +473,7 @@ __device__ __forceinline__ OrderLists nrm_lists_global(const DeviceBuffers &b)
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_nrm_events(FrameParams p, const uint4 *__restrict__ gbuf,
                                                     const uint32_t *__restrict__ gpos, OrderLists ol,
-                                                    uint32_t *__restrict__ counters,
+                                                    uint32_t *__restrict__ overflow,
                                                     MultiTab mt)
 {
     uint32_t vbx = blockIdx.x, vbn = gridDim.x;
@@ -483,7 +483,7 @@ __global__ __launch_bounds__(256) void k_nrm_events(FrameParams p, const uint4 *
         gbuf = gptr(V.b.gbuf);
         gpos = gptr(V.b.gpos);
         ol = nrm_lists_global(V.b);
-        counters = gptr(V.b.counters);
+        overflow = gptr(V.b.counters) + VE_CNT_OVERFLOW;
     }
     uint32_t x = 0, y = 0;
     const bool valid = tile_pixel(vbx, vbn, p, x, y);
@@ -528,7 +528,7 @@ __global__ __launch_bounds__(256) void k_nrm_events(FrameParams p, const uint4 *
     } else {
         const uint32_t o = atomicAdd(&ol.over_n[0], 1u);
         if (o < ol.over_cap) ol.over[o] = make_uint4(e, h, vid, pk);
-        else atomicOr(&counters[VE_CNT_OVERFLOW], 2u);   // past the list: no longer the reference's order
+        else atomicOr(overflow, 2u);   // past the list: no longer the reference's order
     }
 }
 
@@ -577,7 +577,7 @@ constexpr uint32_t REPLAY_WAVES = 4;   // buckets per block of the replays (one 
 template <bool MULTI>
 __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_nrm_replay(FrameParams p, uint32_t *__restrict__ nbuf,
                                                                   OrderLists ol, uint32_t *__restrict__ work,
-                                                                  uint32_t *__restrict__ counters,
+                                                                  uint32_t *__restrict__ overflow,
                                                                   MultiTab mt)
 {
     uint32_t vbx = blockIdx.x, vbn = gridDim.x;
@@ -587,7 +587,7 @@ __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_nrm_replay(FrameParams p,
         nbuf = gptr(V.b.nbuf);
         ol = nrm_lists_global(V.b);
         work = gptr(V.b.ord_work);
-        counters = gptr(V.b.counters);
+        overflow = gptr(V.b.counters) + VE_CNT_OVERFLOW;
     }
     __shared__ uint64_t lds_keys_all[REPLAY_WAVES][ORDER_LDS_EVENTS];
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
@@ -600,7 +600,7 @@ __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_nrm_replay(FrameParams p,
     if (lane == 0u) ol.cnt[h] = 0u;   // the next pass's count starts from zero
     const SortedKeys sk = sort_listed(ol, h, n, lds_keys_all[wv]);
     if (sk.n < n) {   // events lost past the lists: no longer the reference's order
-        if (lane == 0u) atomicOr(&counters[VE_CNT_OVERFLOW], 2u);
+        if (lane == 0u) atomicOr(overflow, 2u);
         if (sk.n == 0u) return;
     }
     uint32_t *bucket = nbuf + (size_t)h * NB_WORDS;
@@ -732,7 +732,7 @@ __device__ __forceinline__ uint32_t sched_home(uint32_t v)
 // keep the lowest rank; the pixel count is added on the same cache line.
 template <bool MULTI>
 __global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *__restrict__ gbuf, uint4 *__restrict__ tab,
-                                                    uint32_t *__restrict__ cand_at, uint32_t *__restrict__ counters,
+                                                    uint32_t *__restrict__ cand_at, uint32_t *__restrict__ overflow,
                                                     uint4 *__restrict__ over, uint32_t *__restrict__ over_n,
                                                     uint32_t *__restrict__ row_over, uint32_t *__restrict__ rows,
                                                     MultiTab mt)
@@ -744,7 +744,7 @@ __global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *
         gbuf = gptr(V.b.gbuf);
         tab = gptr(V.b.sched_tab);
         cand_at = gptr(V.b.cand_at);
-        counters = gptr(V.b.counters);
+        overflow = gptr(V.b.counters) + VE_CNT_OVERFLOW;
         over = gptr(V.b.sched_over);
         over_n = gptr(V.b.sched_over_n);
         row_over = gptr(V.b.sched_row_over);
@@ -797,7 +797,7 @@ __global__ __launch_bounds__(64) void k_sched_claim(FrameParams p, const uint4 *
     if (i < SCHED_OVER_CAP)
         over[i] = make_uint4(vid, ~e, lanes, h);
     else
-        atomicOr(&counters[VE_CNT_OVERFLOW], 1u);
+        atomicOr(overflow, 1u);
     if (atomicExch(&row_over[h], 1u) == 0u) rows[atomicAdd(&over_n[1], 1u)] = h;
 }
 
@@ -874,7 +874,7 @@ __global__ __launch_bounds__(256) void k_sched_rank_over(uint4 *__restrict__ tab
                                                          uint32_t *__restrict__ samples, uint32_t *__restrict__ cand_at,
                                                          const uint4 *__restrict__ over, const uint32_t *__restrict__ over_n,
                                                          uint32_t *__restrict__ row_over, const uint32_t *__restrict__ rows,
-                                                         uint32_t *__restrict__ counters,
+                                                         uint32_t *__restrict__ overflow,
                                                          MultiTab mt)
 {
     uint32_t vbx = blockIdx.x, vbn = gridDim.x;
@@ -888,7 +888,7 @@ __global__ __launch_bounds__(256) void k_sched_rank_over(uint4 *__restrict__ tab
         over_n = gptr(V.b.sched_over_n);
         row_over = gptr(V.b.sched_row_over);
         rows = gptr(V.b.sched_rows);
-        counters = gptr(V.b.counters);
+        overflow = gptr(V.b.counters) + VE_CNT_OVERFLOW;
     }
     __shared__ uint32_t tv[SCHED_SLOTS], tf[SCHED_SLOTS], tc[SCHED_SLOTS];
     __shared__ uint32_t rv[SCHED_ROW_OVER], rf[SCHED_ROW_OVER], rc[SCHED_ROW_OVER];
@@ -926,7 +926,7 @@ __global__ __launch_bounds__(256) void k_sched_rank_over(uint4 *__restrict__ tab
         }
         __syncthreads();
         const uint32_t n_t = nt, n_r = min(nr, SCHED_ROW_OVER);
-        if (t == 0u && nr > SCHED_ROW_OVER) atomicOr(&counters[VE_CNT_OVERFLOW], 1u);
+        if (t == 0u && nr > SCHED_ROW_OVER) atomicOr(overflow, 1u);
         // representative record of each voxel: its lowest first rank (ranks
         // are distinct, so exactly one record per voxel)
         for (uint32_t i = t; i < n_r; i += blockDim.x) {
@@ -3077,6 +3077,11 @@ __global__ __launch_bounds__(256) void k_resolve_avg(FrameParams p, const uint4 
     if (vbx == 0 && threadIdx.x == 0) {   // k_resolve appends after this pass
         counters[VE_CNT_TIMEOUTS] = 0u;
         counters[VE_CNT_TIMEOUT_TILES] = 0u;
+        if (p.early_set) {   // the overflow that the frame's early schedule and normal passes reported
+            uint32_t *side = counters + NUM_COUNTERS + 2u * (uint32_t)(p.early_set - 1);
+            counters[VE_CNT_OVERFLOW] |= side[0] | side[1];
+            side[0] = side[1] = 0u;
+        }
         if (p.fold_small_passes) {   // k_dep_done, folded: the frame's deposit records are applied
             counters[VE_CNT_DEFERRED] = 0u;
             dep_over_n[0] = dep_over_n[1] = 0u;
@@ -3473,12 +3478,23 @@ void launch_fill(hipStream_t s, uint32_t *p, uint32_t v, size_t n)
     if (n) launch(k_fill_u32, dim3(cdiv(n, 256)), dim3(256), 0, s, p, v, n);
 }
 
+// The word a frame's schedule (which = 0) or normal (1) pass reports overflow
+// into: the overflow counter, or, when those passes run ahead of the reset
+// pass on the auxiliary streams (early_set), the buffer set's side word past
+// the counters, which the frame's resolve folds into the counter.
+static uint32_t *overflow_word(const FrameParams &p, DeviceBuffers &b, uint32_t which)
+{
+    if (!p.early_set) return b.counters + VE_CNT_OVERFLOW;
+    return b.counters + NUM_COUNTERS + 2u * (uint32_t)(p.early_set - 1) + which;
+}
+
 void launch_normal(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b)
 {
     const OrderLists ol = nrm_lists(b);
-    launch(k_nrm_events<false>, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.gpos, ol, b.counters, MultiTab{});
+    uint32_t *ovf = overflow_word(p, b, 1u);
+    launch(k_nrm_events<false>, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.gpos, ol, ovf, MultiTab{});
     launch(k_nrm_replay<false>, dim3(cdiv(p.hash_width, REPLAY_WAVES)), dim3(64 * REPLAY_WAVES), 0, s, p, b.nbuf, ol,
-           b.ord_work, b.counters, MultiTab{});
+           b.ord_work, ovf, MultiTab{});
     launch(k_nrm_compute<false>, dim3(cdiv((size_t)p.hash_width * NBUFFER_SLOTS, 256)), dim3(256), 0, s, words, p, b.nbuf,
            b.ord_work, b.ord_over_n, MultiTab{});
 }
@@ -3503,7 +3519,9 @@ __global__ void k_frame_reset(FrameParams p, uint32_t *__restrict__ rank_stats, 
     const uint32_t primary = p.count_primary_in_reset ? p.W * p.H : 0u;   // one primary ray per pixel
     if (i < 4) rank_stats[4 + i] = i == 0 ? 0x7F7FFFFFu : 0u;
     if (i < HISTOGRAM_BINS) rank_stats[8 + i] = 0u;
-    if (i < NUM_COUNTERS && i != VE_CNT_EMISSIVES) counters[i] = i == VE_CNT_PRIMARY_RAYS ? primary : 0u;
+    // an early schedule pass (early_set) may already have written the count
+    const bool keep = i == VE_CNT_EMISSIVES || (p.early_set && i == VE_CNT_CANDIDATES);
+    if (i < NUM_COUNTERS && !keep) counters[i] = i == VE_CNT_PRIMARY_RAYS ? primary : 0u;
     if (i == 0) totals[0] += primary;
 }
 
@@ -3517,12 +3535,13 @@ int launch_schedule(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
     const uint32_t gw = (p.W + 7u) / 8u, gh = (p.H + 7u) / 8u;
     const uint32_t groups = gw * gh;
     const uint32_t n_vec = groups * 8u;
-    launch(k_sched_claim<false>, dim3(groups), dim3(64), 0, s, p, b.gbuf, b.sched_tab, b.cand_at, b.counters, b.sched_over,
+    uint32_t *ovf = overflow_word(p, b, 0u);
+    launch(k_sched_claim<false>, dim3(groups), dim3(64), 0, s, p, b.gbuf, b.sched_tab, b.cand_at, ovf, b.sched_over,
            b.sched_over_n, b.sched_row_over, b.sched_rows, MultiTab{});
     launch(k_sched_rank<false>, dim3(CLAIM_WIDTH), dim3(64), 0, s, b.sched_tab, b.claim, b.samples, b.cand_at,
            b.sched_row_over, MultiTab{});
     launch(k_sched_rank_over<false>, dim3(SCHED_OVER_BLOCKS), dim3(256), 0, s, b.sched_tab, b.claim, b.samples, b.cand_at,
-           b.sched_over, b.sched_over_n, b.sched_row_over, b.sched_rows, b.counters, MultiTab{});
+           b.sched_over, b.sched_over_n, b.sched_row_over, b.sched_rows, ovf, MultiTab{});
     launch(k_sched_vec_count<false>, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, b.cand_at, n_vec, b.vec_cnt, b.vec_chunk,
            b.sched_over_n, MultiTab{});
     launch(k_sched_write<false>, dim3(cdiv(n_vec, 256)), dim3(256), 0, s, p, b.cand_at, n_vec, b.vec_cnt, b.vec_chunk,

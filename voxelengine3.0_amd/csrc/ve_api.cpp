@@ -566,7 +566,7 @@ int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, hipS
     case VE_PASS_PRIMARY:
         launch_primary(side_stream ? side_stream : c->stream, s->words, p, c->b, c->node_stats);
         break;
-    case VE_PASS_NORMAL: launch_normal(side_stream ? side_stream : c->stream, s->words, p, c->b); break;
+    case VE_PASS_NORMAL: launch_normal(side_stream ? side_stream : c->stream, s->words, p, c->b, c->nbuf_from); break;
     case VE_PASS_RESET: launch_reset(c->stream, p, c->b); break;
     case VE_PASS_SCHEDULE: rc = launch_schedule(side_stream ? side_stream : c->stream, p, c->b); break;
     case VE_PASS_RANK: rc = launch_rank(c->stream, p, c->b); break;
@@ -651,7 +651,7 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &frame)
         c->b.gbuf = c->b.gbuf_set[k];
         c->b.gpos = c->b.gpos_set[k];
         c->b.ghit = c->b.ghit_set[k];
-        const uint32_t *nbuf_prev = c->b.nbuf;
+        uint32_t *nbuf_prev = c->b.nbuf;
         c->b.nbuf = c->b.nbuf_set[k];
         p.early_set = (int32_t)(1u + k);
         if (c->emit_done) HIP_OK(c, hipStreamWaitEvent(c->aux, c->emit_done, 0));
@@ -665,13 +665,14 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &frame)
         if ((rc = timed_pass(c, s, VE_PASS_SCHEDULE, p, 0, ev, c->aux))) return rc;
         HIP_OK(c, hipEventRecord(c->ev_sched, c->aux));
         HIP_OK(c, hipStreamWaitEvent(c->aux2, c->ev_primary, 0));
-        if (nbuf_prev != c->b.nbuf)
-            HIP_OK(c, hipMemcpyAsync(c->b.nbuf, nbuf_prev, (size_t)c->hash_width * NBUFFER_STRIDE * NBUFFER_SLOTS * 4,
-                                     hipMemcpyDeviceToDevice, c->aux2));
-        if ((rc = timed_pass(c, s, VE_PASS_NORMAL, p, 0, ev, c->aux2))) return rc;
+        c->nbuf_from = nbuf_prev;   // the replay copies the buckets it leaves unchanged
+        rc = timed_pass(c, s, VE_PASS_NORMAL, p, 0, ev, c->aux2);
+        c->nbuf_from = nullptr;
+        if (rc) return rc;
+        HIP_OK(c, hipStreamWaitEvent(c->aux2, c->ev_sched, 0));
         HIP_OK(c, hipEventRecord(c->ev_normal, c->aux2));
         if ((rc = timed_pass(c, s, VE_PASS_RESET, p, 0, ev))) return rc;
-        HIP_OK(c, hipStreamWaitEvent(c->stream, c->ev_sched, 0));
+        HIP_OK(c, hipStreamWaitEvent(c->stream, c->ev_normal, 0));
     } else {
         if ((rc = timed_pass(c, s, VE_PASS_RESET, p, 0, ev))) return rc;
         if (!c->skip_primary && (rc = timed_pass(c, s, VE_PASS_PRIMARY, p, 0, ev))) return rc;
@@ -698,7 +699,6 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &frame)
     } else if ((rc = timed_pass(c, s, VE_PASS_EMIT, p, 0, ev))) {
         return rc;
     }
-    if (c->overlap) HIP_OK(c, hipStreamWaitEvent(c->stream, c->ev_normal, 0));   // shading and resolve read normals
     const uint32_t iters = std::min<uint32_t>(c->max_bounces, (uint32_t)MAX_ITERS);   // renderer.cpp:515
     // iteration 0 (emissive registration, reservoirs) as its own passes; the
     // later ones fused per ray, timed as trace / shade of iteration 1

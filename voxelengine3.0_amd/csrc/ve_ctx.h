@@ -84,6 +84,7 @@ struct ve_ctx {
     // reads normals.
     hipStream_t aux2 = nullptr;
     hipEvent_t ev_normal = nullptr;         // on aux2: normal pass written
+    const uint32_t *nbuf_from = nullptr;    // while an overlapped frame enqueues its normal pass: the previous set
     bool aux_fence = true;                  // the next primary must follow all of stream's work
     // Graph replay (ve_set_graph): the frame's launches are captured once into
     // a HIP graph and replayed; the per-frame inputs travel in the device

@@ -154,7 +154,10 @@ constexpr uint32_t BATCH_TILE_WIDTHS[4] = {8u, 16u, 32u, 64u};
 void launch_primary_batch(hipStream_t s, const BatchView *views, const uint32_t *bases, uint32_t n, uint32_t total,
                           uint32_t *queue, uint32_t tile_w, uint32_t grab, uint32_t blocks);
 void launch_fill(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
-void launch_normal(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b);
+// prev: the nBuffer set the pass starts from when it writes into another
+// one (b.nbuf), else null
+void launch_normal(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b,
+                   const uint32_t *prev = nullptr);
 void launch_reset(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 int launch_schedule(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 int launch_rank(hipStream_t s, const FrameParams &p, DeviceBuffers &b);

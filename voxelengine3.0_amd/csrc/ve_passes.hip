@@ -574,8 +574,16 @@ __device__ __forceinline__ uint4 listed_event(const OrderLists &ol, uint32_t h, 
 
 constexpr uint32_t REPLAY_WAVES = 4;   // buckets per block of the replays (one per wave)
 
+// One slot's words of an nBuffer bucket, from another set.
+__device__ __forceinline__ void nrm_copy_slot(uint32_t *bucket, const uint32_t *from, uint32_t sl)
+{
+    bucket[sl] = from[sl];
+    bucket[NB_TIME + sl] = from[NB_TIME + sl];
+    bucket[NB_NRM + sl] = from[NB_NRM + sl];
+}
+
 template <bool MULTI>
-__global__ __launch_bounds__(64 * REPLAY_WAVES) void k_nrm_replay(FrameParams p, uint32_t *__restrict__ nbuf,
+__global__ __launch_bounds__(64 * REPLAY_WAVES) void k_nrm_replay(FrameParams p, uint32_t *nbuf, const uint32_t *prev,
                                                                   OrderLists ol, uint32_t *__restrict__ work,
                                                                   uint32_t *__restrict__ overflow,
                                                                   MultiTab mt)
@@ -585,6 +593,7 @@ __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_nrm_replay(FrameParams p,
         const ViewRec &V = multi_view(mt, vbx, vbn);
         p = global_params(V.p);
         nbuf = gptr(V.b.nbuf);
+        prev = nbuf;
         ol = nrm_lists_global(V.b);
         work = gptr(V.b.ord_work);
         overflow = gptr(V.b.counters) + VE_CNT_OVERFLOW;
@@ -596,17 +605,28 @@ __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_nrm_replay(FrameParams p,
     const bool slot_lane = lane < NBUFFER_SLOTS;
     const uint32_t n = ol.cnt[h];
     if (slot_lane) work[h * NBUFFER_SLOTS + lane] = NONE;   // no new normal unless the replay gives one
-    if (n == 0u) return;
+    // a pass into another nBuffer set (prev != nbuf) starts from the previous
+    // set's bucket and writes all of it
+    uint32_t *bucket = nbuf + (size_t)h * NB_WORDS;
+    const uint32_t *from = prev + (size_t)h * NB_WORDS;
+    const bool copy = prev != nbuf;
+    if (n == 0u) {
+        if (copy && slot_lane) nrm_copy_slot(bucket, from, lane);
+        return;
+    }
     if (lane == 0u) ol.cnt[h] = 0u;   // the next pass's count starts from zero
     const SortedKeys sk = sort_listed(ol, h, n, lds_keys_all[wv]);
     if (sk.n < n) {   // events lost past the lists: no longer the reference's order
         if (lane == 0u) atomicOr(overflow, 2u);
-        if (sk.n == 0u) return;
+        if (sk.n == 0u) {
+            if (copy && slot_lane) nrm_copy_slot(bucket, from, lane);
+            return;
+        }
     }
-    uint32_t *bucket = nbuf + (size_t)h * NB_WORDS;
-    uint32_t owner = slot_lane ? bucket[lane] : NONE;
-    uint32_t tstamp = slot_lane ? bucket[NB_TIME + lane] : NONE;
-    const uint32_t nrm = slot_lane ? bucket[NB_NRM + lane] : 0u;
+    uint32_t owner = slot_lane ? from[lane] : NONE;
+    uint32_t tstamp = slot_lane ? from[NB_TIME + lane] : NONE;
+    const uint32_t nrm = slot_lane ? from[NB_NRM + lane] : 0u;
+    if (copy && slot_lane) bucket[NB_NRM + lane] = nrm;
     bool has_src = false;   // the slot takes the normal at packed position `src`
     uint32_t src = 0u;
     uint32_t cur_vec = NONE, ev_vid = 0u, ev_pos = 0u, last_vid = 0u, last_pos = 0u;
@@ -3488,13 +3508,13 @@ static uint32_t *overflow_word(const FrameParams &p, DeviceBuffers &b, uint32_t 
     return b.counters + NUM_COUNTERS + 2u * (uint32_t)(p.early_set - 1) + which;
 }
 
-void launch_normal(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b)
+void launch_normal(hipStream_t s, const uint32_t *words, const FrameParams &p, DeviceBuffers &b, const uint32_t *prev)
 {
     const OrderLists ol = nrm_lists(b);
     uint32_t *ovf = overflow_word(p, b, 1u);
     launch(k_nrm_events<false>, dim3(screen_blocks(p)), dim3(256), 0, s, p, b.gbuf, b.gpos, ol, ovf, MultiTab{});
-    launch(k_nrm_replay<false>, dim3(cdiv(p.hash_width, REPLAY_WAVES)), dim3(64 * REPLAY_WAVES), 0, s, p, b.nbuf, ol,
-           b.ord_work, ovf, MultiTab{});
+    launch(k_nrm_replay<false>, dim3(cdiv(p.hash_width, REPLAY_WAVES)), dim3(64 * REPLAY_WAVES), 0, s, p, b.nbuf,
+           prev ? prev : b.nbuf, ol, b.ord_work, ovf, MultiTab{});
     launch(k_nrm_compute<false>, dim3(cdiv((size_t)p.hash_width * NBUFFER_SLOTS, 256)), dim3(256), 0, s, words, p, b.nbuf,
            b.ord_work, b.ord_over_n, MultiTab{});
 }
@@ -3720,8 +3740,8 @@ void launch_frames_multi(hipStream_t s, const ViewRec *views, const uint32_t *pr
         launch(k_nrm_events<true>, dim3(grids[MK_NRM_EVENTS]), dim3(256), 0, s, z, nullptr, nullptr, ol, nullptr,
                tab(MK_NRM_EVENTS));
     if (grids[MK_NRM_REPLAY])
-        launch(k_nrm_replay<true>, dim3(grids[MK_NRM_REPLAY]), dim3(64 * REPLAY_WAVES), 0, s, z, nullptr, ol, nullptr,
-               nullptr, tab(MK_NRM_REPLAY));
+        launch(k_nrm_replay<true>, dim3(grids[MK_NRM_REPLAY]), dim3(64 * REPLAY_WAVES), 0, s, z, nullptr, nullptr, ol,
+               nullptr, nullptr, tab(MK_NRM_REPLAY));
     if (grids[MK_NRM_COMPUTE])
         launch(k_nrm_compute<true>, dim3(grids[MK_NRM_COMPUTE]), dim3(256), 0, s, nullptr, z, nullptr, nullptr, nullptr,
                tab(MK_NRM_COMPUTE));

@@ -50,6 +50,24 @@ int ctx_error(ve_ctx *c, int code, const char *fmt, ...)
         if (e_ != hipSuccess) return ctx_error(ctx, VE_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
+// An overlapped frame's resolve pass runs on aux3 beside the next frame's
+// first passes; every call other than the next frame makes the main stream
+// wait for it first, so that the main stream again follows everything the
+// context has enqueued.
+int join_resolve(ve_ctx *c)
+{
+    if (!c->resolve_pending) return VE_OK;
+    HIP_OK(c, hipSetDevice(c->device));
+    HIP_OK(c, hipStreamWaitEvent(c->stream, c->resolve_pending, 0));
+    c->resolve_pending = nullptr;
+    return VE_OK;
+}
+
+#define JOIN_RESOLVE(ctx)                         \
+    do {                                          \
+        if (int rc_ = join_resolve(ctx)) return rc_; \
+    } while (0)
+
 // Every device buffer of a context, in one list, carved out of one
 // allocation at 256-byte offsets: with `base` null the list only adds up the
 // bytes (admission control prices a context before creating it), with `base`
@@ -355,13 +373,15 @@ void ve_destroy(ve_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->aux) (void)hipStreamSynchronize(c->aux);
     if (c->aux2) (void)hipStreamSynchronize(c->aux2);
+    if (c->aux3) (void)hipStreamSynchronize(c->aux3);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto &a : c->allocations) (void)hipFree(a.first);
     if (c->sky_tex) (void)hipFree(c->sky_tex);
     for (auto &fe : c->frame_events)
         for (hipEvent_t e : fe)
             if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {c->ev_primary, c->ev_sched, c->ev_emit, c->ev_fence, c->ev_normal})
+    for (hipEvent_t e : {c->ev_primary, c->ev_sched, c->ev_emit, c->ev_fence, c->ev_normal, c->ev_dep, c->ev_resolve[0],
+                         c->ev_resolve[1]})
         if (e) (void)hipEventDestroy(e);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (hipEvent_t e : c->dyn_ev)
@@ -369,6 +389,7 @@ void ve_destroy(ve_ctx *c)
     if (c->dyn_host) (void)hipHostFree(c->dyn_host);
     if (c->aux) (void)hipStreamDestroy(c->aux);
     if (c->aux2) (void)hipStreamDestroy(c->aux2);
+    if (c->aux3) (void)hipStreamDestroy(c->aux3);
     if (c->stream && c->owns_stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -378,6 +399,7 @@ int ve_scene_upload(ve_ctx *c, const uint32_t *words, uint32_t n_words, uint32_t
     if (!c) return ve_global_error(VE_EINVAL, "null context");
     std::string err;
     c->aux_fence = true;   // the primary pass reads scenes; it follows the upload
+    JOIN_RESOLVE(c);
     const int rc = ve::scene_upload(c->device, c->stream, words, n_words, depth, scene, err, c->tuning.jump_level);
     return rc == VE_OK ? VE_OK : ctx_error(c, rc, "%s", err.c_str());
 }
@@ -389,6 +411,7 @@ int ve_scene_update(ve_ctx *c, ve_scene *s, const uint32_t *words, uint32_t n_wo
     if (!s || !words || n_words == 0 || (n_slots && !slots)) return ctx_error(c, VE_EINVAL, "null argument");
     if (n_words < s->n_words) return ctx_error(c, VE_EINVAL, "node array shrank from %u to %u words", s->n_words, n_words);
     HIP_OK(c, hipSetDevice(c->device));
+    JOIN_RESOLVE(c);
     c->aux_fence = true;   // the next overlapped primary pass reads the edited words
     if (n_words > s->alloc_words) {   // grow: a new array with headroom, whole image copied
         for (uint32_t i = 0; i < n_words; i++)
@@ -450,6 +473,7 @@ int ve_scene_release(ve_ctx *c, ve_scene *s)
     if (!c) return ve_global_error(VE_EINVAL, "null context");
     if (!s) return ctx_error(c, VE_ENOTFOUND, "null scene");
     (void)hipSetDevice(c->device);
+    JOIN_RESOLVE(c);
     (void)hipStreamSynchronize(c->stream);
     ve::scene_free(s);
     return VE_OK;
@@ -487,6 +511,7 @@ int ve_set_materials(ve_ctx *c, const void *entries, uint32_t n)
     if (!c) return ve_global_error(VE_EINVAL, "null context");
     if (!entries || n == 0 || n > MATERIAL_SLOTS) return ctx_error(c, VE_EINVAL, "material count %u outside [1,127]", n);
     HIP_OK(c, hipSetDevice(c->device));
+    JOIN_RESOLVE(c);
     HIP_OK(c, hipMemcpyAsync(c->b.materials, entries, (size_t)n * sizeof(Material), hipMemcpyHostToDevice, c->stream));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     return VE_OK;
@@ -519,6 +544,7 @@ int ve_set_sky(ve_ctx *c, const uint8_t *const faces[6], int edge)
             tex[f * texels + i] = (uint32_t)t[0] | (uint32_t)t[1] << 8 | (uint32_t)t[2] << 16;
         }
     HIP_OK(c, hipSetDevice(c->device));
+    JOIN_RESOLVE(c);
     // frames already enqueued may still sample the current texture
     HIP_OK(c, hipStreamSynchronize(c->stream));
     if (tex.size() > c->sky_tex_words) {
@@ -552,8 +578,8 @@ int flush_deposits(ve_ctx *c)
 }
 
 // Enqueue one pass; every launch goes on the context's stream.  The primary,
-// schedule and normal passes run on `side_stream` when given (an overlapped
-// frame's auxiliary streams).
+// schedule, normal and resolve passes run on `side_stream` when given (an
+// overlapped frame's auxiliary streams).
 int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, hipStream_t side_stream = nullptr)
 {
     int rc = VE_OK;
@@ -588,7 +614,7 @@ int run_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, hipS
             c->dep_params = p;
         }
         break;
-    case VE_PASS_RESOLVE: launch_resolve(c->stream, p, c->b); break;
+    case VE_PASS_RESOLVE: launch_resolve(side_stream ? side_stream : c->stream, p, c->b); break;
     case VE_PASS_DISPLAY: launch_display(c->stream, p, c->b); break;
     default: return ctx_error(c, VE_EINVAL, "unknown pass %d", (int)pass);
     }
@@ -631,11 +657,12 @@ int timed_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, ui
 // the primary and schedule passes go to the auxiliary stream, into the other
 // G-buffer set, once the previous frame's emit pass is done, and the normal
 // pass to the second auxiliary stream, into the other nBuffer set: they then
-// run beside that frame's trace/shade/resolve passes, which leave most of the
-// device idle (their time is the tail of the longest rays).  The sets' last
-// readers belong to the frame before last, whose passes all precede that
-// emit pass on the main stream.  The main stream waits for the schedule pass
-// before the rank pass and for the normal pass before the trace pass.
+// run beside that frame's trace/shade passes, which leave most of the device
+// idle (their time is the tail of the longest rays).  The main stream waits
+// for them before the rank pass.  The resolve pass goes to the third
+// auxiliary stream after the frame's deposits and runs beside the next
+// frame's first passes; the next frame's deposits wait for it, and so does
+// the primary pass two frames on, the next writer of the sets it reads.
 int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &frame)
 {
     hipEvent_t *ev = nullptr;
@@ -660,6 +687,7 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &frame)
             HIP_OK(c, hipStreamWaitEvent(c->aux, c->ev_fence, 0));
             c->aux_fence = false;
         }
+        if (c->resolve_rec[k]) HIP_OK(c, hipStreamWaitEvent(c->aux, c->ev_resolve[k], 0));   // the sets' last reader
         if ((rc = timed_pass(c, s, VE_PASS_PRIMARY, p, 0, ev, c->aux))) return rc;
         HIP_OK(c, hipEventRecord(c->ev_primary, c->aux));
         if ((rc = timed_pass(c, s, VE_PASS_SCHEDULE, p, 0, ev, c->aux))) return rc;
@@ -710,6 +738,10 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &frame)
         if (ev) pass_timer = PassTimer{ev[2 * (VE_PASS_TRACE * MAX_ITERS + 1)], ev[2 * (VE_PASS_TRACE * MAX_ITERS + 1) + 1]};
         launch_bounce(c->stream, s->words, p, c->b, 1u, iters, c->node_stats);
     }
+    if (c->resolve_pending) {   // the previous frame's resolve reads the lBuffer
+        HIP_OK(c, hipStreamWaitEvent(c->stream, c->resolve_pending, 0));
+        c->resolve_pending = nullptr;
+    }
     if (iters > 0) {   // the frame's deposits, timed with the last shade pass
         const uint32_t slot = 2 * (VE_PASS_SHADE * MAX_ITERS + (iters > 1 ? 1 : 0));
         if (ev) pass_timer = PassTimer{iters > 1 ? ev[slot] : nullptr, ev[slot + 1]};
@@ -718,7 +750,19 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &frame)
         if (rc) return ctx_error(c, rc, "deposits: device library call failed");
         HIP_OK(c, hipGetLastError());
     }
-    if ((rc = timed_pass(c, s, VE_PASS_RESOLVE, p, 0, ev))) return rc;
+    if (c->overlap) {   // the resolve beside the next frame's first passes
+        launch_dep_done(c->stream, p, c->b);
+        HIP_OK(c, hipGetLastError());
+        HIP_OK(c, hipEventRecord(c->ev_dep, c->stream));
+        HIP_OK(c, hipStreamWaitEvent(c->aux3, c->ev_dep, 0));
+        if ((rc = timed_pass(c, s, VE_PASS_RESOLVE, p, 0, ev, c->aux3))) return rc;
+        const uint32_t k = c->gsel;
+        HIP_OK(c, hipEventRecord(c->ev_resolve[k], c->aux3));
+        c->resolve_rec[k] = true;
+        c->resolve_pending = c->ev_resolve[k];
+    } else if ((rc = timed_pass(c, s, VE_PASS_RESOLVE, p, 0, ev))) {
+        return rc;
+    }
     if (ev) {
         c->iters_recorded = std::min<uint32_t>(iters, 2u);
         c->frames_recorded++;
@@ -817,6 +861,7 @@ int ve_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const ve_frame_config *fc, 
     // an overlapped primary pass must not run ahead of a pass that may touch
     // the G-buffers; the display pass reads only the resolved image
     if (pass != VE_PASS_DISPLAY) c->aux_fence = true;
+    JOIN_RESOLVE(c);
     return run_pass(c, s, pass, p);
 }
 
@@ -854,6 +899,7 @@ int ve_read(ve_ctx *c, ve_buffer_id id, void *dst, size_t bytes)
     if (!v.ptr) return ctx_error(c, VE_ENOTFOUND, "unknown buffer id %d", (int)id);
     if (!dst || bytes > v.bytes) return ctx_error(c, VE_EINVAL, "read of %zu bytes from a %zu-byte buffer", bytes, v.bytes);
     HIP_OK(c, hipSetDevice(c->device));
+    JOIN_RESOLVE(c);
     if (int rc = flush_deposits(c)) return rc;
     if (is_hash_buffer(id)) {   // returned in the reference's slot-interleaved layout
         std::vector<uint32_t> tmp(v.bytes / 4), ref(v.bytes / 4);
@@ -876,6 +922,7 @@ int ve_write(ve_ctx *c, ve_buffer_id id, const void *src, size_t bytes)
     if (!src || bytes > v.bytes) return ctx_error(c, VE_EINVAL, "write of %zu bytes to a %zu-byte buffer", bytes, v.bytes);
     c->aux_fence = true;
     HIP_OK(c, hipSetDevice(c->device));
+    JOIN_RESOLVE(c);
     if (int rc = flush_deposits(c)) return rc;
     if (is_hash_buffer(id)) {   // accepted in the reference layout; whole image only
         if (bytes != v.bytes) return ctx_error(c, VE_EINVAL, "hash images are written whole (%zu bytes)", v.bytes);
@@ -897,6 +944,7 @@ int ve_sync(ve_ctx *c)
     HIP_OK(c, hipStreamSynchronize(c->stream));
     if (c->aux) HIP_OK(c, hipStreamSynchronize(c->aux));
     if (c->aux2) HIP_OK(c, hipStreamSynchronize(c->aux2));
+    if (c->aux3) HIP_OK(c, hipStreamSynchronize(c->aux3));
     return VE_OK;
 }
 
@@ -904,6 +952,7 @@ int ve_set_overlap(ve_ctx *c, int enable)
 {
     if (!c) return ve_global_error(VE_EINVAL, "null context");
     if (enable && !c->aux) return ctx_error(c, VE_EINVAL, "context has no auxiliary stream for overlap");
+    JOIN_RESOLVE(c);
     c->overlap = enable != 0;
     c->emit_done = nullptr;
     c->aux_fence = true;
@@ -945,6 +994,7 @@ int ve_timings_iter(ve_ctx *c, uint32_t iter, float ms_out[VE_PASS_COUNT])
     HIP_OK(c, hipSetDevice(c->device));
     if (c->aux) HIP_OK(c, hipStreamSynchronize(c->aux));
     if (c->aux2) HIP_OK(c, hipStreamSynchronize(c->aux2));
+    if (c->aux3) HIP_OK(c, hipStreamSynchronize(c->aux3));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     for (int i = 0; i < VE_PASS_COUNT; i++) {
         double sum = 0.0;
@@ -967,6 +1017,7 @@ int ve_timings(ve_ctx *c, float ms_out[VE_PASS_COUNT])
     HIP_OK(c, hipSetDevice(c->device));
     if (c->aux) HIP_OK(c, hipStreamSynchronize(c->aux));
     if (c->aux2) HIP_OK(c, hipStreamSynchronize(c->aux2));
+    if (c->aux3) HIP_OK(c, hipStreamSynchronize(c->aux3));
     HIP_OK(c, hipStreamSynchronize(c->stream));
     double sum[VE_PASS_COUNT] = {0};
     for (size_t f = 0; f < c->frames_recorded; f++)
@@ -1105,6 +1156,9 @@ int ctx_create(ve_ctx **out, const ve_renderer_config *cfg, hipStream_t shared, 
                 if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->aux2, hipStreamNonBlocking);   // default priority
                 if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_sched, fl);
                 if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_normal, fl);
+                if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->aux3, hipStreamNonBlocking);   // default priority
+                if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_dep, fl);
+                for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipEventCreateWithFlags(&c->ev_resolve[k], fl);
             }
         }
     }
@@ -1178,6 +1232,8 @@ int ctx_reset(ve_ctx *c)
     c->b.gpos = c->b.gpos_set[0];
     c->b.ghit = c->b.ghit_set[0];
     c->b.nbuf = c->b.nbuf_set[0];
+    c->resolve_rec[0] = c->resolve_rec[1] = false;
+    c->resolve_pending = nullptr;
     c->emit_done = nullptr;
     c->aux_fence = true;
     c->node_stats = false;

@@ -77,14 +77,25 @@ struct ve_ctx {
     hipEvent_t ev_fence = nullptr;          // on stream: everything enqueued so far
     // The normal pass updates the nBuffer, which shading and the resolve
     // read: inside an overlapped frame it runs on aux2 after the frame's
-    // primary pass, into the nBuffer set of the frame's G-buffer set, first
-    // copied from the previous frame's; that set's last readers belong to
-    // the frame before last, whose passes precede the previous emit pass.
-    // The main stream waits for it (ev_normal) before the first pass that
-    // reads normals.
+    // primary pass, into the nBuffer set of the frame's G-buffer set, which
+    // its replay fills from the previous frame's set.  The main stream waits
+    // for it, and through it for the schedule pass (ev_normal), before the
+    // rank pass.
     hipStream_t aux2 = nullptr;
-    hipEvent_t ev_normal = nullptr;         // on aux2: normal pass written
+    hipEvent_t ev_normal = nullptr;         // on aux2: normal and schedule passes written
     const uint32_t *nbuf_from = nullptr;    // while an overlapped frame enqueues its normal pass: the previous set
+    // The resolve pass reads the lBuffer, the frame's G-buffer and nBuffer
+    // sets and the materials, and writes only the image and its own scratch:
+    // inside an overlapped frame it runs on aux3 after the frame's deposits
+    // (ev_dep), beside the next frame's first passes.  The next deposits wait
+    // for it on the main stream, the primary pass two frames on (the next
+    // writer of its sets) on aux, and any other call on the context joins
+    // it to the main stream first (resolve_pending).
+    hipStream_t aux3 = nullptr;
+    hipEvent_t ev_dep = nullptr;            // on stream: the frame's deposits applied
+    hipEvent_t ev_resolve[2] = {};          // on aux3: the latest resolve of each buffer set done
+    bool resolve_rec[2] = {false, false};   // ev_resolve[k] recorded
+    hipEvent_t resolve_pending = nullptr;   // the resolve the main stream has not waited for yet
     bool aux_fence = true;                  // the next primary must follow all of stream's work
     // Graph replay (ve_set_graph): the frame's launches are captured once into
     // a HIP graph and replayed; the per-frame inputs travel in the device

@@ -184,10 +184,12 @@ struct FrameParams {
     // of a bounce-0 ray whose camera ray is the pixel's primary ray
     int32_t reuse_primary_hits;
     // 1 + the frame's buffer set when its schedule and normal passes run on
-    // the auxiliary streams ahead of the previous frame's later passes (0
-    // otherwise): they report overflow into that set's side words past the
-    // counters, the resolve's first thread folds those into the overflow
-    // counter, and the reset pass keeps the schedule's candidate count
+    // the auxiliary streams ahead of the previous frame's later passes, and
+    // its resolve on a third one beside the next frame's (0 otherwise): the
+    // early passes report overflow into that set's side words past the
+    // counters, which the kernel clearing the deposit records folds into the
+    // overflow counter; the reset pass keeps the schedule's candidate count
+    // and the resolve's timeout counts
     int32_t early_set;
     // sky: a cube map of sky_edge^2 texels per face (faces +X -X +Y -Y +Z -Z,
     // texel r | g << 8 | b << 16), or, with sky_edge 0, the constant `sky`

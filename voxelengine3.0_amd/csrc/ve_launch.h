@@ -175,6 +175,7 @@ void launch_bounce(hipStream_t s, const uint32_t *words, const FrameParams &p, D
 // reference's order.
 int launch_deposits(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint32_t iters);
 void launch_resolve(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
+void launch_dep_done(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 void launch_display(hipStream_t s, const FrameParams &p, DeviceBuffers &b);
 void launch_scatter_words(hipStream_t s, uint32_t *words, const uint2 *pairs, uint32_t n);
 // The jump table of a node array (8^level entries).

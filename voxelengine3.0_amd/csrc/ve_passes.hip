@@ -3026,10 +3026,15 @@ __global__ __launch_bounds__(64 * REPLAY_WAVES) void k_dep_replay(FrameParams p,
     for (uint32_t f = 0; f < 5u; f++) gb[LB_PAY + slot * LB_PAYW + 5u * ch + f] = st.c[f];
 }
 
-__global__ void k_dep_done(uint32_t *__restrict__ counters, uint32_t *__restrict__ over_n)
+__global__ void k_dep_done(uint32_t *__restrict__ counters, uint32_t *__restrict__ over_n, int32_t early_set)
 {
     counters[VE_CNT_DEFERRED] = 0u;   // the frame's records are applied
     over_n[0] = over_n[1] = 0u;
+    if (early_set) {   // the overflow that the frame's early schedule and normal passes reported
+        uint32_t *side = counters + NUM_COUNTERS + 2u * (uint32_t)(early_set - 1);
+        counters[VE_CNT_OVERFLOW] |= side[0] | side[1];
+        side[0] = side[1] = 0u;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -3097,12 +3102,7 @@ __global__ __launch_bounds__(256) void k_resolve_avg(FrameParams p, const uint4 
     if (vbx == 0 && threadIdx.x == 0) {   // k_resolve appends after this pass
         counters[VE_CNT_TIMEOUTS] = 0u;
         counters[VE_CNT_TIMEOUT_TILES] = 0u;
-        if (p.early_set) {   // the overflow that the frame's early schedule and normal passes reported
-            uint32_t *side = counters + NUM_COUNTERS + 2u * (uint32_t)(p.early_set - 1);
-            counters[VE_CNT_OVERFLOW] |= side[0] | side[1];
-            side[0] = side[1] = 0u;
-        }
-        if (p.fold_small_passes) {   // k_dep_done, folded: the frame's deposit records are applied
+        if (p.fold_small_passes && !p.early_set) {   // k_dep_done, folded: the frame's deposit records are applied
             counters[VE_CNT_DEFERRED] = 0u;
             dep_over_n[0] = dep_over_n[1] = 0u;
         }
@@ -3539,8 +3539,10 @@ __global__ void k_frame_reset(FrameParams p, uint32_t *__restrict__ rank_stats, 
     const uint32_t primary = p.count_primary_in_reset ? p.W * p.H : 0u;   // one primary ray per pixel
     if (i < 4) rank_stats[4 + i] = i == 0 ? 0x7F7FFFFFu : 0u;
     if (i < HISTOGRAM_BINS) rank_stats[8 + i] = 0u;
-    // an early schedule pass (early_set) may already have written the count
-    const bool keep = i == VE_CNT_EMISSIVES || (p.early_set && i == VE_CNT_CANDIDATES);
+    // with early_set, the schedule pass may already have written the count and
+    // the previous frame's resolve may still be counting its timed-out pixels
+    const bool early = i == VE_CNT_CANDIDATES || i == VE_CNT_TIMEOUTS || i == VE_CNT_TIMEOUT_TILES;
+    const bool keep = i == VE_CNT_EMISSIVES || (p.early_set && early);
     if (i < NUM_COUNTERS && !keep) counters[i] = i == VE_CNT_PRIMARY_RAYS ? primary : 0u;
     if (i == 0) totals[0] += primary;
 }
@@ -3640,8 +3642,15 @@ int launch_deposits(hipStream_t s, const FrameParams &p, DeviceBuffers &b, uint3
     launch(k_dep_wpos<false>, dim3(chunks * WPOS_BLOCKS), dim3(1024), 0, s, b.counters, b.dep_death, b.dep_wpos, iters, MultiTab{});
     launch(k_dep_replay<false>, dim3(cdiv(p.hash_width, REPLAY_WAVES)), dim3(64 * REPLAY_WAVES), 0, s, p, b.lbuf, dl,
            b.deferred, b.dep_wpos, b.counters, MultiTab{});
-    if (!p.fold_small_passes) launch(k_dep_done, dim3(1), dim3(1), 0, s, b.counters, b.dep_over_n);   // else the resolve's
+    if (!p.fold_small_passes) launch(k_dep_done, dim3(1), dim3(1), 0, s, b.counters, b.dep_over_n, 0);   // else the resolve's
     return VE_OK;
+}
+
+// An overlapped frame's resolve runs on another stream: the deposit records'
+// clear and the fold of the early passes' overflow take a launch of their own.
+void launch_dep_done(hipStream_t s, const FrameParams &p, DeviceBuffers &b)
+{
+    launch(k_dep_done, dim3(1), dim3(1), 0, s, b.counters, b.dep_over_n, (int32_t)p.early_set);
 }
 
 // Partial node-array update (Octree::UpdateNode's glBufferSubData per word).

@@ -14,8 +14,9 @@ trace pass as the reference does (the bounce-0 camera rays, whose hit records
 come from the primary pass's traversal, are listed as reused_rays_per_frame,
 and mrays_per_s_traversed_only leaves them out) (inputs resident in HBM; the host only
 enqueues; frames as an application renders them: the context's default
-cross-frame overlap, where a frame's primary pass runs on a second stream
-beside the previous frame's bounce passes, and no instrumentation).  The same
+cross-frame overlap, where a frame's primary, schedule and normal passes run
+on other streams beside the previous frame's bounce passes, and its resolve
+beside the next frame's first passes, and no instrumentation).  The same
 number of frames then runs again with per-pass HIP events bound to the
 passes' kernel dispatches (pass_ms), and once more with the overlap off (the
 reference's strict pass order) for pass_ms_serial.  The roofline entry is the
@@ -487,7 +488,8 @@ def main():
             # emit pass took from the primary pass (the same traversal)
             "reused_rays_per_frame": round(totals["reused"] / args.steps, 1),
             "mrays_per_s_traversed_only": round((rays - totals["reused"]) / elapsed / 1e6, 3),
-            "overlap": "primary pass of frame k+1 beside the bounce passes of frame k",
+            "overlap": "primary, schedule and normal passes of frame k+1 beside the bounce passes of frame k; "
+                       "resolve of frame k beside the first passes of frame k+1",
             "pass_ms": {k: round(v, 4) for k, v in pass_ms.items()},
             "pass_ms_serial": {k: round(v, 4) for k, v in pass_ms_serial.items()},
             "scene_build_s": round(t_build, 2),

@@ -284,15 +284,16 @@ int ve_buffer_size(ve_ctx *ctx, ve_buffer_id id, size_t *bytes);
 int ve_sync(ve_ctx *ctx);
 
 /* Cross-frame overlap (on by default for contexts made by ve_create): each
- * ve_frame renders its primary pass into a second G-buffer on an auxiliary
- * stream, beside the previous frame's trace/shade/resolve passes, and the
- * context's stream waits for it before the normal pass; the normal pass runs
- * on a second auxiliary stream beside the schedule, rank, threshold and emit
- * passes (it writes only the nBuffer, which they do not use), joined before
- * the trace pass.  Every frame's results are those of the serial order.  Off: all passes in order on the
- * context's stream (Renderer::run's sequence).  No counterpart in the
- * reference, whose GL passes run in submission order; VE_EINVAL for a
- * context without the second stream (server sessions). */
+ * ve_frame renders its primary and schedule passes into a second G-buffer
+ * set on an auxiliary stream, and its normal pass into a second nBuffer set
+ * on another, beside the previous frame's trace/shade passes; the context's
+ * stream waits for them before the rank pass.  The frame's resolve runs on a
+ * third auxiliary stream beside the next frame's first passes; any call on
+ * the context other than the next ve_frame first waits for it.  Every
+ * frame's results are those of the serial order.  Off: all passes in order
+ * on the context's stream (Renderer::run's sequence).  No counterpart in
+ * the reference, whose GL passes run in submission order; VE_EINVAL for a
+ * context without the auxiliary streams (server sessions). */
 int ve_set_overlap(ve_ctx *ctx, int enable);
 
 /* No counterpart in the reference: graph replay.  While enabled (and the

@@ -3,9 +3,9 @@ second stream, into a second G-buffer set, and its normal pass on a third,
 into a second nBuffer set, beside the previous frame's bounce passes)
 renders exactly what the reference's serial pass order renders.
 
-A queue of frames with a moving camera, a scene edit between two frames, a
-lone pass call and a display pass is enqueued with no host synchronisation
-in between, so the primary passes really run ahead; every image, counter,
+A queue of frames with a moving camera, a material change and a scene edit
+between two frames, a lone pass call and a display pass is enqueued with no
+other host synchronisation in between, so the primary passes really run ahead; every image, counter,
 list and ray total is then compared bit for bit, and the nBuffer / lBuffer
 contents voxel by voxel, against a context making the same calls with the
 overlap switched off.  The knobs are the order-independent ones, so two
@@ -46,6 +46,8 @@ def _render(ve, scenes, overlap):
                 cpu.insertSphere(np.array([64.0, 70.0, 64.0], np.float32), 12.0, 5)
                 octree.applyEdits(cpu)
                 rend.update(scene, octree)
+            if f == 4:   # the previous frame's resolve must still see the old materials
+                rend.set_materials(scenes.palette_spec(1))
             if f == 8:   # a lone pass between frames
                 rend.run_pass(scene, "PRIMARY", fc, inputs)
             rend.frame(scene, fc, inputs)

@@ -711,28 +711,36 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &frame)
     // when the frame folds its small passes
     if ((rc = timed_pass(c, s, VE_PASS_RANK, p, 0, ev))) return rc;
     if (!p.fold_small_passes && (rc = timed_pass(c, s, VE_PASS_THRESHOLD, p, 0, ev))) return rc;
-    if (c->overlap) {
-        // the emit kernel's own dispatch marks its completion: the timing
-        // event when the pass is timed, ev_emit otherwise
+    const uint32_t iters = std::min<uint32_t>(c->max_bounces, (uint32_t)MAX_ITERS);   // renderer.cpp:515
+    // the next frame's early passes follow this frame's first shade pass (or,
+    // without bounces, its emit pass): the kernel's own dispatch marks its
+    // completion, the timing event when the pass is timed, ev_emit otherwise
+    auto anchored = [&](ve_pass_id pass) -> int {
         if (ev) {
-            if ((rc = timed_pass(c, s, VE_PASS_EMIT, p, 0, ev))) return rc;
-            c->emit_done = ev[2 * (VE_PASS_EMIT * MAX_ITERS) + 1];
-        } else {
-            pass_timer = PassTimer{nullptr, c->ev_emit};
-            rc = run_pass(c, s, VE_PASS_EMIT, p);
-            pass_timer = PassTimer{};
-            if (rc) return rc;
-            c->emit_done = c->ev_emit;
+            if (int r = timed_pass(c, s, pass, p, 0, ev)) return r;
+            c->emit_done = ev[2 * (pass * MAX_ITERS) + 1];
+            return VE_OK;
         }
+        pass_timer = PassTimer{nullptr, c->ev_emit};
+        const int r = run_pass(c, s, pass, p);
+        pass_timer = PassTimer{};
+        if (r == VE_OK) c->emit_done = c->ev_emit;
+        return r;
+    };
+    if (c->overlap && iters == 0) {
+        if ((rc = anchored(VE_PASS_EMIT))) return rc;
     } else if ((rc = timed_pass(c, s, VE_PASS_EMIT, p, 0, ev))) {
         return rc;
     }
-    const uint32_t iters = std::min<uint32_t>(c->max_bounces, (uint32_t)MAX_ITERS);   // renderer.cpp:515
     // iteration 0 (emissive registration, reservoirs) as its own passes; the
     // later ones fused per ray, timed as trace / shade of iteration 1
     if (iters > 0) {
         if ((rc = timed_pass(c, s, VE_PASS_TRACE, p, 0, ev))) return rc;
-        if ((rc = timed_pass(c, s, VE_PASS_SHADE, p, 0, ev))) return rc;
+        if (c->overlap) {
+            if ((rc = anchored(VE_PASS_SHADE))) return rc;
+        } else if ((rc = timed_pass(c, s, VE_PASS_SHADE, p, 0, ev))) {
+            return rc;
+        }
     }
     if (iters > 1) {
         if (ev) pass_timer = PassTimer{ev[2 * (VE_PASS_TRACE * MAX_ITERS + 1)], ev[2 * (VE_PASS_TRACE * MAX_ITERS + 1) + 1]};

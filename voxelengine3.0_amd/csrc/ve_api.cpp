@@ -655,14 +655,15 @@ int timed_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, ui
 // renderer.cpp:282-649 without its host readbacks: counts stay on the device
 // and every kernel sizes its own work from them.  With cross-frame overlap
 // the primary and schedule passes go to the auxiliary stream, into the other
-// G-buffer set, once the previous frame's emit pass is done, and the normal
-// pass to the second auxiliary stream, into the other nBuffer set: they then
-// run beside that frame's trace/shade passes, which leave most of the device
-// idle (their time is the tail of the longest rays).  The main stream waits
-// for them before the rank pass.  The resolve pass goes to the third
-// auxiliary stream after the frame's deposits and runs beside the next
-// frame's first passes; the next frame's deposits wait for it, and so does
-// the primary pass two frames on, the next writer of the sets it reads.
+// G-buffer set, once the previous frame's first shade pass is done, and the
+// normal pass to the second auxiliary stream, into the other nBuffer set:
+// they then run beside that frame's fused bounce and deposit passes, which
+// leave most of the device idle (their time is the tail of the longest
+// rays).  The main stream waits for them before the rank pass.  The resolve
+// pass goes to the third auxiliary stream after the frame's deposits and
+// runs beside the next frame's first passes; the next frame's deposits wait
+// for it, and so does the primary pass two frames on, the next writer of the
+// sets it reads.
 int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &frame)
 {
     hipEvent_t *ev = nullptr;

@@ -66,13 +66,15 @@ struct ve_ctx {
     // G-buffer set while the previous frame's latency-bound bounce passes
     // still run on `stream`, and the schedule pass follows it there: it reads
     // only the G-buffer and writes the claim map and candidate list, which no
-    // pass after the previous frame's emit pass reads.
+    // pass after the previous frame's emit pass reads.  They start once the
+    // previous frame's first shade pass is done (its emit pass, without
+    // bounces).
     bool overlap = false;
     hipStream_t aux = nullptr;
     uint32_t gsel = 0;                      // G-buffer (and nBuffer) set of the latest frame
     hipEvent_t ev_primary = nullptr;        // on aux: primary pass written
     hipEvent_t ev_sched = nullptr;          // on aux: schedule pass written
-    hipEvent_t ev_emit = nullptr;           // on stream: emit pass of the latest frame done
+    hipEvent_t ev_emit = nullptr;           // on stream: first shade (or emit) pass of the latest frame done
     hipEvent_t emit_done = nullptr;         // the event that marks it (ev_emit, or its timing event)
     hipEvent_t ev_fence = nullptr;          // on stream: everything enqueued so far
     // The normal pass updates the nBuffer, which shading and the resolve

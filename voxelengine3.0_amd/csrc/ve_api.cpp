@@ -659,7 +659,8 @@ int timed_pass(ve_ctx *c, ve_scene *s, ve_pass_id pass, const FrameParams &p, ui
 // normal pass to the second auxiliary stream, into the other nBuffer set:
 // they then run beside that frame's fused bounce and deposit passes, which
 // leave most of the device idle (their time is the tail of the longest
-// rays).  The main stream waits for them before the rank pass.  The resolve
+// rays).  The main stream waits for the schedule pass before the rank pass
+// and for the normal pass before the trace pass.  The resolve
 // pass goes to the third auxiliary stream after the frame's deposits and
 // runs beside the next frame's first passes; the next frame's deposits wait
 // for it, and so does the primary pass two frames on, the next writer of the
@@ -698,10 +699,9 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &frame)
         rc = timed_pass(c, s, VE_PASS_NORMAL, p, 0, ev, c->aux2);
         c->nbuf_from = nullptr;
         if (rc) return rc;
-        HIP_OK(c, hipStreamWaitEvent(c->aux2, c->ev_sched, 0));
         HIP_OK(c, hipEventRecord(c->ev_normal, c->aux2));
         if ((rc = timed_pass(c, s, VE_PASS_RESET, p, 0, ev))) return rc;
-        HIP_OK(c, hipStreamWaitEvent(c->stream, c->ev_normal, 0));
+        HIP_OK(c, hipStreamWaitEvent(c->stream, c->ev_sched, 0));
     } else {
         if ((rc = timed_pass(c, s, VE_PASS_RESET, p, 0, ev))) return rc;
         if (!c->skip_primary && (rc = timed_pass(c, s, VE_PASS_PRIMARY, p, 0, ev))) return rc;
@@ -733,6 +733,7 @@ int frame_passes(ve_ctx *c, ve_scene *s, const FrameParams &frame)
     } else if ((rc = timed_pass(c, s, VE_PASS_EMIT, p, 0, ev))) {
         return rc;
     }
+    if (c->overlap) HIP_OK(c, hipStreamWaitEvent(c->stream, c->ev_normal, 0));   // shading and resolve read normals
     // iteration 0 (emissive registration, reservoirs) as its own passes; the
     // later ones fused per ray, timed as trace / shade of iteration 1
     if (iters > 0) {

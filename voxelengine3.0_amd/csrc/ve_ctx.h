@@ -81,10 +81,10 @@ struct ve_ctx {
     // read: inside an overlapped frame it runs on aux2 after the frame's
     // primary pass, into the nBuffer set of the frame's G-buffer set, which
     // its replay fills from the previous frame's set.  The main stream waits
-    // for it, and through it for the schedule pass (ev_normal), before the
-    // rank pass.
+    // for it (ev_normal) before the trace pass, and for the schedule pass
+    // (ev_sched) before the rank pass.
     hipStream_t aux2 = nullptr;
-    hipEvent_t ev_normal = nullptr;         // on aux2: normal and schedule passes written
+    hipEvent_t ev_normal = nullptr;         // on aux2: normal pass written
     const uint32_t *nbuf_from = nullptr;    // while an overlapped frame enqueues its normal pass: the previous set
     // The resolve pass reads the lBuffer, the frame's G-buffer and nBuffer
     // sets and the materials, and writes only the image and its own scratch:

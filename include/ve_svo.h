@@ -287,7 +287,8 @@ int ve_sync(ve_ctx *ctx);
  * ve_frame renders its primary and schedule passes into a second G-buffer
  * set on an auxiliary stream, and its normal pass into a second nBuffer set
  * on another, beside the previous frame's bounce passes; the context's
- * stream waits for them before the rank pass.  The frame's resolve runs on a
+ * stream waits for the schedule pass before the rank pass and for the
+ * normal pass before the trace pass.  The frame's resolve runs on a
  * third auxiliary stream beside the next frame's first passes; any call on
  * the context other than the next ve_frame first waits for it.  Every
  * frame's results are those of the serial order.  Off: all passes in order
